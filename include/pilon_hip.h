@@ -1,0 +1,221 @@
+/*
+ * pilon_hip.h — C ABI of the MI355X pileup engine (libpilonhip.so).
+ *
+ * This is the drop-in boundary for Pilon's pileup-construction path.  Every
+ * entry point below replaces one piece of the Scala surface of Pilon 1.24:
+ *
+ *   PileUpRegion(name, start, stop)        pilon/PileUpRegion.scala:26
+ *   PileUpRegion.addRead(r, refBases, lr)  pilon/PileUpRegion.scala:102-220
+ *   BamFile.process / validateRead         pilon/BamFile.scala:101-148
+ *   PileUpRegion.postProcess               pilon/PileUpRegion.scala:226-229
+ *   GenomeRegion.postProcess (Pass 1)      pilon/GenomeRegion.scala:214-272
+ *   PileUp fields + PileUp.baseCall        pilon/PileUp.scala:25-257
+ *   GenomeRegion.identifyAndFixIssues      pilon/GenomeRegion.scala:307-380,413
+ *   GenomeRegion.fixIssues/writeChanges    pilon/GenomeRegion.scala:557-657
+ *
+ * Conventions: plain C types only; 0 = success; the library owns device
+ * memory, the caller owns host memory; one pl_ctx per host thread; objects
+ * are not thread-safe.  pl_last_error() returns a thread-local message for
+ * the last failing call.
+ */
+#ifndef PILON_HIP_H
+#define PILON_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum pl_status {
+  PL_OK = 0,
+  PL_E_INVAL = 1,     /* bad argument or malformed read (e.g. CIGAR/SEQ length mismatch) */
+  PL_E_OOM = 2,       /* device or host allocation failed */
+  PL_E_DEVICE = 3,    /* HIP runtime error or no usable GPU */
+  PL_E_EDGE = 4,      /* input on which Pilon 1.24 itself throws (see below) */
+  PL_E_OVERFLOW = 5,  /* a 32-bit device counter could leave the range Pilon's Long keeps */
+  PL_E_BUSY = 6,      /* serving queue full */
+  PL_E_REJECTED = 7   /* serving request larger than the device-memory budget */
+} pl_status;
+
+/* PL_E_EDGE mirrors the inputs on which the reference raises an exception
+ * instead of producing output: an insertion or deletion left-normalised past
+ * region.start (negative pileup index, PileUpRegion.scala:161,182) and a
+ * soft clip on a read without SEQ (bases(rOff) out of range,
+ * PileUpRegion.scala:201). */
+
+/* ---------------------------------------------------------------- config */
+/* The knobs the path reads from Pilon's global object (Pilon.scala:52-73). */
+typedef struct pl_config {
+  int32_t flank;          /* Pilon.flank = 10                  */
+  int32_t min_mq;         /* Pilon.minMq = 0                   */
+  int32_t min_qual;       /* Pilon.minQual = 0                 */
+  int32_t default_qual;   /* Pilon.defaultQual = 10 (a Byte)   */
+  double  min_depth;      /* Pilon.minDepth = 0.1              */
+  int32_t min_min_depth;  /* Pilon.minMinDepth = 5             */
+  int32_t old_indel;      /* Pilon.oldIndel = false            */
+  int32_t fix_snps;       /* "snps" in --fix                   */
+  int32_t fix_indels;     /* "indels" in --fix                 */
+  int32_t fix_amb;        /* "amb" in --fix                    */
+  int32_t iupac;          /* --iupac                           */
+  int32_t longread;       /* Pilon.longread (any --nanopore/--pacbio BAM) */
+  int32_t nonpf;          /* --nonpf                           */
+  int32_t duplicates;     /* --duplicates                      */
+} pl_config;
+
+/* Defaults identical to Pilon 1.24 with "--fix bases". */
+void pl_config_default(pl_config* cfg);
+
+/* ------------------------------------------------------------ read batch */
+/* Extra bits the host packer sets in the 16-bit flag word (above the SAM
+ * flags, which stop at 0x800). */
+#define PL_RF_NO_QUAL        0x2000u  /* BAM QUAL absent: use default_qual (PileUpRegion.scala:114-115) */
+#define PL_RF_MATE_SAME_REF  0x4000u  /* refID == next refID (PileUpRegion.scala:107) */
+#define PL_RF_EXOTIC         0x8000u  /* SEQ holds a non-ACGT code: bases come from nib */
+
+/* Packed structure-of-arrays of reads of ONE contig, sorted by pos.  It is
+ * what a JVM/C++ host produces from htsjdk SAMRecords (or BAM records):
+ *   pos      = getAlignmentStart            tlen = getInferredInsertSize
+ *   flag     = getFlags | PL_RF_*           mapq = getMappingQuality
+ *   cigar    = BAM cigar words (len<<4|op)  bases/quals = getReadBases/getBaseQualities
+ * Bases are 2-bit codes A=0 C=1 G=2 T=3; a read with any other base sets
+ * PL_RF_EXOTIC and lists its 4-bit BAM codes ("=ACMGRSVTWYHKDBN") in nib. */
+typedef struct pl_read_soa {
+  int64_t         n_reads;
+  const int32_t*  pos;        /* [n] 1-based alignment start, non-decreasing          */
+  const int32_t*  tlen;       /* [n]                                                   */
+  const uint16_t* flag;       /* [n]                                                   */
+  const uint8_t*  mapq;       /* [n]                                                   */
+  const int64_t*  cigar_off;  /* [n+1] prefix offsets into cigar                       */
+  const uint32_t* cigar;      /* [cigar_off[n]]                                        */
+  const int64_t*  seq_off;    /* [n+1] prefix offsets into the per-base streams        */
+  const uint8_t*  bases2;     /* base k at byte k>>2, bits 2*(k&3)                     */
+  const int8_t*   quals;      /* [seq_off[n]] Phred bytes as Java reads them (signed)  */
+  int64_t         n_exotic;
+  const int64_t*  exo_read;   /* [n_exotic] ascending read indices with PL_RF_EXOTIC   */
+  const int64_t*  exo_off;    /* [n_exotic] first nibble of that read in nib           */
+  const uint8_t*  nib;        /* nibble k at byte k>>1, high nibble when k is even     */
+} pl_read_soa;
+
+/* ------------------------------------------------------------- outputs */
+/* PileUp counters, field-major: out[f*G + i] for position i of the region.
+ * Order follows the field list of PileUp (PileUp.scala:26-39). */
+enum pl_field {
+  PL_F_COUNT_A = 0, PL_F_COUNT_C, PL_F_COUNT_G, PL_F_COUNT_T,   /* baseCount.sums */
+  PL_F_QUAL_A, PL_F_QUAL_C, PL_F_QUAL_G, PL_F_QUAL_T,           /* qualSum.sums   */
+  PL_F_MQSUM, PL_F_QSUM, PL_F_PHYSCOV, PL_F_INSERTSIZE, PL_F_BADPAIR,
+  PL_F_DELETIONS, PL_F_DELQUAL, PL_F_INSERTIONS, PL_F_INSQUAL, PL_F_CLIPS,
+  PL_NFIELDS
+};
+
+/* One PileUp.BaseCall plus the Pass-1 disposition (PileUp.scala:132-255,
+ * GenomeRegion.scala:237-272), 8 bytes. */
+typedef struct pl_call {
+  uint8_t  base;   /* 0..3 = A,C,G,T; 4 = N (no bases)     BaseCall.base    */
+  uint8_t  alt;    /* 0..3                                  BaseCall.altBase */
+  uint16_t flags;  /* PL_CF_* below                                          */
+  int32_t  q;      /* BaseCall.q = score / n                                 */
+} pl_call;
+
+#define PL_CF_HOMO        0x0001u  /* BaseCall.homo              */
+#define PL_CF_HIGHCONF    0x0002u  /* q >= 10                     */
+#define PL_CF_CALLED      0x0004u  /* base != 'N' || indel        */
+#define PL_CF_INDEL       0x0008u  /* indel call present          */
+#define PL_CF_HOMOINDEL   0x0010u  /* BaseCall.homoIndel          */
+#define PL_CF_INSERTION   0x0020u  /* isInsertion                 */
+#define PL_CF_DELETION    0x0040u  /* isDeletion                  */
+#define PL_CF_SCOREPOS    0x0080u  /* score > 0                   */
+#define PL_CF_DISP_SHIFT  8        /* 3-bit disposition           */
+#define PL_CF_DISP_MASK   0x0700u
+#define PL_CF_DELETED     0x0800u  /* GenomeRegion.deleted(i)     */
+
+enum pl_disposition { PL_D_NONE = 0, PL_D_CONFIRMED = 1, PL_D_SNP = 2,
+                      PL_D_INS = 3, PL_D_DEL = 4, PL_D_AMB = 5 };
+
+typedef struct pl_region_stats {
+  int64_t read_count;   /* PileUpRegion.readCount                         */
+  int64_t base_count;   /* PileUpRegion.baseCount                         */
+  int64_t coverage;     /* roundDiv(baseCount, size)                      */
+  int32_t min_depth;    /* GenomeRegion.minDepth                          */
+  int32_t n_confirmed;  /* confirmed count ("Confirmed N of M bases")     */
+  int32_t snps, amb, ins, ins_bases, dels, del_bases;
+  int32_t n_unknown_ops;/* 'P' ops that print "unknown cigar op"          */
+  int64_t aligned_bases;/* M/=/X bases with locus in the region (bench unit) */
+} pl_region_stats;
+
+/* ------------------------------------------------------------- context */
+typedef struct pl_ctx pl_ctx;
+typedef struct pl_region pl_region;
+
+pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out);
+void      pl_ctx_destroy(pl_ctx* ctx);
+const char* pl_last_error(void);
+
+/* Upload a whole contig (raw FASTA bytes, case kept) and keep it resident.
+ * Replaces the refBases argument of addRead (PileUpRegion.scala:102) and
+ * GenomeRegion.contigBases (GenomeRegion.scala:35). */
+pl_status pl_ref_upload(pl_ctx* ctx, int32_t contig_id, const char* name,
+                        const uint8_t* ascii, int64_t len);
+
+/* Upload a sorted batch of one contig's reads and keep it resident; returns
+ * an id usable by pl_region_use_readset and by the batch/serving forms. */
+pl_status pl_readset_upload(pl_ctx* ctx, int32_t contig_id,
+                            const pl_read_soa* reads, int32_t* readset_id);
+
+/* --------------------------------------------------- plain region form */
+/* new PileUpRegion(name, start, stop): 1-based inclusive, stop <= len.   */
+pl_status pl_region_open(pl_ctx* ctx, int32_t contig_id, int32_t start,
+                         int32_t stop, pl_region** out);
+/* BamFile.process for one BAM: the region keeps the reads that overlap
+ * [max(start-10000,0), min(stop+10000,len)] and pass validateRead, and adds
+ * each with addRead.  Callable repeatedly (one call per BAM or batch). */
+pl_status pl_region_add_reads(pl_region* reg, const pl_read_soa* reads);
+/* Same, for a resident read set. */
+pl_status pl_region_use_readset(pl_region* reg, int32_t readset_id);
+/* PileUpRegion.postProcess + GenomeRegion.postProcess Pass 1.  pileup
+ * (PL_NFIELDS*G int32) and calls (G) are optional caller buffers. */
+pl_status pl_region_finish(pl_region* reg, int32_t* pileup, pl_call* calls,
+                           pl_region_stats* stats);
+/* identifyAndFixIssues (snps/indels) + writeChanges + fixed bases.  The
+ * library allocates *changes_text (the .changes lines) and *bases; free
+ * them with pl_free.  new_name/offset are writeChanges' arguments. */
+pl_status pl_region_fix(pl_region* reg, const char* new_name, int64_t offset,
+                        char** changes_text, int64_t* changes_len,
+                        uint8_t** bases, int64_t* bases_len);
+void      pl_region_close(pl_region* reg);
+void      pl_free(void* p);
+
+/* ------------------------------------------------- batched region form */
+/* One region of a batch: the GenomeFile.processRegions loop body
+ * (GenomeFile.scala:101-120) for thousands of regions in one launch.     */
+typedef struct pl_region_desc {
+  int32_t contig_id;
+  int32_t start, stop;     /* 1-based inclusive                              */
+  int32_t readset_id;      /* resident reads of that contig                  */
+} pl_region_desc;
+
+typedef struct pl_batch_opts {
+  int32_t window;          /* positions per workgroup window; 0 = autotuned  */
+  int32_t block;           /* threads per workgroup; 0 = autotuned           */
+  int32_t persistent;      /* 1 = persistent work-queue launch               */
+  int32_t use_graph;       /* 1 = replay a captured hipGraph                 */
+  int32_t copy_out;        /* 1 = copy pileups/calls to the host buffers     */
+} pl_batch_opts;
+
+/* Runs every region of the batch on the device.  pileup/calls may be NULL
+ * (results stay resident); otherwise they hold sum(G) entries in region
+ * order.  stats has n entries. */
+pl_status pl_batch_run(pl_ctx* ctx, const pl_region_desc* regions, int32_t n,
+                       const pl_batch_opts* opts, int32_t* pileup,
+                       pl_call* calls, pl_region_stats* stats);
+
+/* Device time of the last pl_batch_run's pipeline in milliseconds, measured
+ * with HIP events on the library's stream, plus the share of the window
+ * kernel alone. */
+pl_status pl_last_timing(pl_ctx* ctx, double* pipeline_ms, double* window_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PILON_HIP_H */

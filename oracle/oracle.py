@@ -1,0 +1,132 @@
+"""Python handle on the C oracle (pilon_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product (pilon_amd/).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from pilon_amd.abi import PlConfig, PlReadSoa, PlCall, PlRegionStats, NFIELDS, call_dtype
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libpilon_oracle.so")
+SRC = os.path.join(HERE, "pilon_oracle.c")
+
+
+def build():
+    """Compile the oracle with gcc (oracle/Makefile does the same)."""
+    subprocess.check_call(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
+                           "-o", LIB, SRC, "-lm"])
+
+
+class OrcResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("edge_read", C.c_int64), ("edge_why", C.c_char * 64),
+                ("size", C.c_int32), ("pile", C.POINTER(C.c_int32)), ("bcq", C.POINTER(C.c_int64)),
+                ("calls", C.POINTER(PlCall)), ("coverage", C.POINTER(C.c_int32)),
+                ("badCoverage", C.POINTER(C.c_int32)), ("physCoverage", C.POINTER(C.c_int32)),
+                ("insertSize", C.POINTER(C.c_int32)), ("weightedQual", C.POINTER(C.c_int8)),
+                ("weightedMq", C.POINTER(C.c_int8)), ("clips", C.POINTER(C.c_int16)),
+                ("stats", PlRegionStats), ("changes_text", C.c_char_p), ("changes_len", C.c_int64),
+                ("bases", C.POINTER(C.c_uint8)), ("bases_len", C.c_int64),
+                ("n_indel", C.c_int32), ("indel_pos", C.POINTER(C.c_int32)),
+                ("indel_off", C.POINTER(C.c_int64)), ("indel_str", C.POINTER(C.c_uint8))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_region_run.restype = C.POINTER(OrcResult)
+        L.orc_region_run.argtypes = [C.POINTER(PlConfig), C.c_char_p, C.c_int64, C.c_char_p,
+                                     C.c_int32, C.c_int32, C.POINTER(C.POINTER(PlReadSoa)),
+                                     C.c_int32, C.c_char_p, C.c_int64]
+        L.orc_result_free.argtypes = [C.POINTER(OrcResult)]
+        L.orc_java_round.restype = C.c_int64
+        L.orc_java_round.argtypes = [C.c_double]
+        L.orc_roundDivI.restype = C.c_int32
+        L.orc_roundDivI.argtypes = [C.c_int32, C.c_int32]
+        L.orc_roundDivL.restype = C.c_int64
+        L.orc_roundDivL.argtypes = [C.c_int64, C.c_int64]
+        L.orc_basecall.restype = None
+        L.orc_basecall.argtypes = [C.POINTER(PlConfig), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   C.c_int32, C.POINTER(C.c_int32), C.c_char_p,
+                                   C.c_int32, C.POINTER(C.c_int32), C.c_char_p,
+                                   C.POINTER(PlCall), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0):
+    """One GenomeRegion through the oracle; returns a dict of numpy arrays."""
+    L = lib()
+    if isinstance(contig, str):
+        contig = contig.encode()
+    contig = bytes(contig)
+    if new_name is None:
+        new_name = name + "_pilon"
+    soas = [rs.soa() for rs in readsets]
+    arr = (C.POINTER(PlReadSoa) * max(1, len(soas)))(*[C.pointer(s) for s in soas])
+    rp = L.orc_region_run(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
+                          arr, len(soas), new_name.encode(), offset)
+    r = rp.contents
+    out = {"status": r.status, "edge_read": r.edge_read, "edge_why": r.edge_why.decode()}
+    if r.status == 0:
+        size = r.size
+        out["size"] = size
+        out["pile"] = _arr(r.pile, NFIELDS * size, np.int32).reshape(NFIELDS, size)
+        out["bcq"] = _arr(r.bcq, 8 * size, np.int64).reshape(8, size)
+        raw = np.ctypeslib.as_array(C.cast(r.calls, C.POINTER(C.c_uint8)), shape=(8 * size,)).copy()
+        out["calls"] = raw.view(call_dtype())
+        out["coverage"] = _arr(r.coverage, size, np.int32)
+        out["badCoverage"] = _arr(r.badCoverage, size, np.int32)
+        out["physCoverage"] = _arr(r.physCoverage, size, np.int32)
+        out["insertSize"] = _arr(r.insertSize, size, np.int32)
+        out["weightedQual"] = _arr(r.weightedQual, size, np.int8)
+        out["weightedMq"] = _arr(r.weightedMq, size, np.int8)
+        out["clips"] = _arr(r.clips, size, np.int16)
+        out["stats"] = r.stats.as_dict()
+        out["changes"] = C.string_at(r.changes_text, r.changes_len).decode() if r.changes_len else ""
+        out["bases"] = bytes(_arr(r.bases, r.bases_len, np.uint8))
+        n = r.n_indel
+        pos = _arr(r.indel_pos, n, np.int32)
+        off = _arr(r.indel_off, n + 1, np.int64)
+        sb = bytes(_arr(r.indel_str, int(off[-1]) if n else 0, np.uint8))
+        out["indels"] = {int(pos[k]): sb[off[k]:off[k + 1]].decode() for k in range(n)}
+    L.orc_result_free(rp)
+    return out
+
+
+def java_round(x):
+    return lib().orc_java_round(float(x))
+
+
+def basecall(cfg, bc, qs, mqSum=0, qSum=0, deletions=0, insertions=0, insQual=0, delQual=0,
+             ins_list=(), del_list=()):
+    """BaseCall on a hand-made PileUp; returns (pl_call dict, indel_len, score)."""
+    L = lib()
+    b = (C.c_int64 * 4)(*bc)
+    q = (C.c_int64 * 4)(*qs)
+    il = (C.c_int32 * max(1, len(ins_list)))(*[len(s) for s in ins_list])
+    dl = (C.c_int32 * max(1, len(del_list)))(*[len(s) for s in del_list])
+    out = PlCall()
+    ilen = C.c_int32()
+    score = C.c_int64()
+    L.orc_basecall(C.byref(cfg), b, q, mqSum, qSum, deletions, insertions, insQual, delQual,
+                   len(ins_list), il, "".join(ins_list).encode(), len(del_list), dl,
+                   "".join(del_list).encode(), C.byref(out), C.byref(ilen), C.byref(score))
+    return {"base": out.base, "alt": out.alt, "flags": out.flags, "q": out.q}, ilen.value, score.value

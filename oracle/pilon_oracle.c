@@ -1,0 +1,887 @@
+/*
+ * pilon_oracle.c — CPU restatement of Pilon 1.24's pileup-construction path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU engine is
+ * compared against; it is never linked into libpilonhip.so and nothing in the
+ * product calls it.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py load it.
+ *
+ * It follows the Scala sources line by line, one read and one position at a
+ * time, with Java's integer semantics (Int wraps at 32 bits, Long at 64,
+ * division truncates).  Every function names the reference lines it restates.
+ * Reference: broadinstitute/pilon v1.24, src/main/scala/org/broadinstitute/pilon/
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include <math.h>
+#include "../include/pilon_hip.h"
+
+/* ---------------------------------------------------------- Java ints */
+static inline int32_t jadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+static inline int32_t jsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+static inline int32_t jmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+static inline int32_t jdiv(int32_t a, int32_t b) {            /* b != 0 */
+  if (a == INT32_MIN && b == -1) return INT32_MIN;
+  return a / b;
+}
+static inline int64_t ldiv64(int64_t a, int64_t b) {
+  if (a == INT64_MIN && b == -1) return INT64_MIN;
+  return a / b;
+}
+
+/* Utils.roundDiv / Utils.pct, Int and Long overloads (Utils.scala:23-26). */
+static int32_t roundDivI(int32_t n, int32_t d) { return d > 0 ? jdiv(jadd(n, d / 2), d) : 0; }
+static int64_t roundDivL(int64_t n, int64_t d) { return d > 0 ? ldiv64(n + d / 2, d) : 0; }
+static int32_t pctI(int32_t n, int32_t d) { return roundDivI(jmul(100, n), d); }
+
+/* java.lang.Math.round(double): the nearest long, ties toward +infinity
+ * (its javadoc).  floor(x) + [x - floor(x) >= 1/2]; the subtraction is exact
+ * for every finite double. */
+int64_t orc_java_round(double x) {
+  if (x != x) return 0;
+  if (x >= 9.2233720368547758e18) return INT64_MAX;
+  if (x <= -9.2233720368547758e18) return INT64_MIN;
+  double f = floor(x);
+  return (int64_t)f + ((x - f) >= 0.5 ? 1 : 0);
+}
+
+/* ------------------------------------------------------- strings/lists */
+typedef struct { const uint8_t* s; int32_t len; } ostr;
+typedef struct { ostr* v; int32_t n, cap; } olist;
+
+static void olist_push(olist* l, const uint8_t* s, int32_t len) {
+  if (l->n == l->cap) {
+    l->cap = l->cap ? 2 * l->cap : 4;
+    l->v = (ostr*)realloc(l->v, sizeof(ostr) * (size_t)l->cap);
+  }
+  l->v[l->n].s = s;
+  l->v[l->n].len = len;
+  l->n++;
+}
+
+/* byte arena for indel strings (they live until the region is freed) */
+typedef struct arena_blk { struct arena_blk* next; size_t used, cap; uint8_t data[]; } arena_blk;
+typedef struct { arena_blk* head; } arena;
+static uint8_t* arena_alloc(arena* a, size_t n) {
+  if (!a->head || a->head->used + n > a->head->cap) {
+    size_t cap = n > (1u << 20) ? n : (1u << 20);
+    arena_blk* b = (arena_blk*)malloc(sizeof(arena_blk) + cap);
+    b->next = a->head; b->used = 0; b->cap = cap; a->head = b;
+  }
+  uint8_t* p = a->head->data + a->head->used;
+  a->head->used += n;
+  return p;
+}
+static void arena_free(arena* a) {
+  while (a->head) { arena_blk* n = a->head->next; free(a->head); a->head = n; }
+}
+
+/* ------------------------------------------------------------- PileUp */
+/* PileUp fields (PileUp.scala:26-41); BaseSum sums are Long (BaseSum.scala:24). */
+typedef struct {
+  int64_t bc[4], qs[4];
+  int32_t mqSum, qSum, physCov, insertSize, badPair, deletions, delQual,
+          insertions, insQual, clips;
+  olist ins, del;
+} PileUp;
+
+static int baseIndex(uint8_t c) {                     /* PileUp.scala:46-52 */
+  switch (c) { case 'A': return 0; case 'C': return 1; case 'G': return 2; case 'T': return 3; }
+  return -1;
+}
+
+static int64_t pu_count(const PileUp* p) { return p->bc[0] + p->bc[1] + p->bc[2] + p->bc[3]; }   /* :43 */
+static int64_t pu_depth(const PileUp* p) { return pu_count(p) + p->deletions; }                  /* :44 */
+
+/* PileUp.add (PileUp.scala:75-84) */
+static void pu_add(PileUp* p, uint8_t base, int32_t qual, int32_t mq, const pl_config* cfg) {
+  int bi = baseIndex(base);
+  if (bi >= 0 && qual >= cfg->min_qual) {
+    int32_t mq1 = jadd(mq, 1);
+    p->bc[bi] += 1;
+    p->qs[bi] += (int64_t)jmul(qual, mq1);
+    p->mqSum = jadd(p->mqSum, mq1);
+    p->qSum = jadd(p->qSum, qual);
+  }
+}
+/* PileUp.addInsertion (PileUp.scala:98-105): mqSum untouched. */
+static void pu_addInsertion(PileUp* p, const uint8_t* s, int32_t len, int32_t qual, int32_t mq) {
+  int32_t mq1 = jadd(mq, 1);
+  p->insQual = jadd(p->insQual, mq1);
+  p->qSum = jadd(p->qSum, qual);
+  olist_push(&p->ins, s, len);
+  p->insertions = jadd(p->insertions, 1);
+}
+/* PileUp.addDeletion (PileUp.scala:107-114) */
+static void pu_addDeletion(PileUp* p, const uint8_t* s, int32_t len, int32_t qual, int32_t mq) {
+  int32_t mq1 = jadd(mq, 1);
+  p->mqSum = jadd(p->mqSum, mq1);
+  p->delQual = jadd(p->delQual, mq1);
+  p->qSum = jadd(p->qSum, qual);
+  olist_push(&p->del, s, len);
+  p->deletions = jadd(p->deletions, 1);
+}
+
+/* BaseSum.order: stable descending sort of 0..3 (BaseSum.scala:57-60). */
+static void basesum_order(const int64_t s[4], int ord[4]) {
+  for (int i = 0; i < 4; i++) ord[i] = i;
+  for (int i = 1; i < 4; i++) {            /* insertion sort, strict > keeps ties in index order */
+    int x = ord[i], j = i - 1;
+    while (j >= 0 && s[x] > s[ord[j]]) { ord[j + 1] = ord[j]; j--; }
+    ord[j + 1] = x;
+  }
+}
+
+/* --------------------------------------------------------- BaseCall */
+typedef struct {
+  int base;              /* 0..3, 4 = 'N' */
+  int alt;
+  int homo;
+  int64_t score, q;
+  int highConf;
+  const uint8_t* indelStr; int32_t indelLen;
+  int isIns, isDel, indel, homoIndel, called;
+} BaseCall;
+
+static int cmp_ostr(const void* a, const void* b) {
+  const ostr* x = (const ostr*)a; const ostr* y = (const ostr*)b;
+  if (x->len != y->len) return x->len < y->len ? -1 : 1;
+  return memcmp(x->s, y->s, (size_t)x->len);
+}
+
+/* BaseCall.hetIndelCall (PileUp.scala:209-247) incl. the oldIndel branch. */
+static void hetIndelCall(const PileUp* p, const olist* l, int32_t pct, const pl_config* cfg,
+                         const uint8_t** win, int32_t* winLen, int* homo) {
+  *win = NULL; *winLen = 0; *homo = 1;
+  if (pu_depth(p) < cfg->min_min_depth || pct < 5 || l->n == 0) return;
+  /* map(indelStr) += 1 ; winner = map.toSeq.sortBy(_._2).last */
+  ostr* v = (ostr*)malloc(sizeof(ostr) * (size_t)l->n);
+  memcpy(v, l->v, sizeof(ostr) * (size_t)l->n);
+  qsort(v, (size_t)l->n, sizeof(ostr), cmp_ostr);
+  int32_t best = 0; ostr bestS = {0, 0};
+  for (int32_t i = 0; i < l->n;) {
+    int32_t j = i + 1;
+    while (j < l->n && cmp_ostr(&v[i], &v[j]) == 0) j++;
+    if (j - i >= best) { best = j - i; bestS = v[i]; }
+    i = j;
+  }
+  free(v);
+  if (best < 2 || best <= l->n / 2) return;
+  for (int32_t i = 0; i < bestS.len; i++) if (bestS.s[i] == 'N') return;
+  int32_t L = bestS.len;
+  if (cfg->old_indel) {
+    if (pct >= 33 && pct >= 50 - L) { *win = bestS.s; *winLen = L; *homo = 1; }
+    return;
+  }
+  int32_t middle = (45 - L) > 10 ? (45 - L) : 10;
+  int32_t low = middle / 2;
+  int32_t high = middle + middle - low;
+  if (pct > high) { *win = bestS.s; *winLen = L; *homo = 1; }
+  else if (pct >= low) { *win = bestS.s; *winLen = L; *homo = 0; }
+}
+
+/* PileUp.insPct / delPct (PileUp.scala:122-123) */
+static int32_t insPct(const PileUp* p) {
+  int32_t a = pctI(p->insQual, p->mqSum), b = pctI(p->insertions, (int32_t)pu_count(p));
+  return a > b ? a : b;
+}
+static int32_t delPct(const PileUp* p) {
+  int32_t a = pctI(p->delQual, p->mqSum);
+  int32_t b = pctI(p->deletions, jadd((int32_t)pu_count(p), p->deletions));
+  return a > b ? a : b;
+}
+
+/* new BaseCall (PileUp.scala:132-191) */
+static void baseCall(const PileUp* p, const pl_config* cfg, BaseCall* bc) {
+  memset(bc, 0, sizeof(*bc));
+  int64_t n = pu_count(p);
+  int ord[4];
+  basesum_order(p->qSum > 0 ? p->qs : p->bc, ord);
+  bc->base = n > 0 ? ord[0] : 4;
+  int bI = ord[0], aI = ord[1];
+  bc->alt = aI;
+  int64_t baseSum = p->qs[bI], altBaseSum = p->qs[aI];
+  int64_t total = p->qs[0] + p->qs[1] + p->qs[2] + p->qs[3];
+  int64_t homoScore = baseSum - (total - baseSum);
+  int64_t halfTotal = ldiv64(total, 2);
+  int64_t d1 = halfTotal - baseSum, d2 = halfTotal - altBaseSum;
+  int64_t heteroScore = total - (d1 < 0 ? -d1 : d1) - (d2 < 0 ? -d2 : d2);
+  bc->homo = homoScore >= heteroScore;
+  int64_t dd = homoScore - heteroScore; if (dd < 0) dd = -dd;
+  bc->score = p->mqSum > 0 ? ldiv64(dd * n, (int64_t)p->mqSum) : 0;
+  bc->q = n > 0 ? ldiv64(bc->score, n) : 0;
+  bc->highConf = bc->q >= 10;
+  /* insertCall / deletionCall (PileUp.scala:151-162,183-191) */
+  const uint8_t* s = NULL; int32_t L = 0; int h = 1;
+  if (p->insertions > 2 && p->insertions > p->deletions)
+    hetIndelCall(p, &p->ins, insPct(p), cfg, &s, &L, &h);
+  if (s) { bc->isIns = 1; bc->indel = 1; bc->homoIndel = h; bc->indelStr = s; bc->indelLen = L; }
+  else {
+    h = 1;
+    if (p->deletions > 2 && p->deletions > p->insertions)
+      hetIndelCall(p, &p->del, delPct(p), cfg, &s, &L, &h);
+    if (s) { bc->isDel = 1; bc->indel = 1; bc->homoIndel = h; bc->indelStr = s; bc->indelLen = L; }
+    else { bc->homoIndel = 1; }
+  }
+  bc->called = (bc->base != 4) || bc->indel;
+}
+
+/* ------------------------------------------------------- PileUpRegion */
+typedef struct {
+  int32_t start, stop, size;
+  PileUp* pu;
+  int64_t baseCount;
+  int32_t readCount;
+  int32_t physCovStart, insertSizeStart;
+  int64_t unknownOps;
+  int64_t alignedBases;
+  arena ar;
+  int edge;  int64_t edgeRead; const char* edgeWhy;
+} PRegion;
+
+static inline int inRegion(const PRegion* R, int64_t l) { return l >= R->start && l <= R->stop; }   /* Region.scala:23 */
+static inline int beforeRegion(const PRegion* R, int64_t l) { return l < R->start; }             /* :24 */
+
+/* PileUpRegion.add / remove (PileUpRegion.scala:38-58) */
+static void pr_add(PRegion* R, int64_t locus, uint8_t base, int32_t qual, int32_t mq, int pair,
+                   const pl_config* cfg) {
+  if (inRegion(R, locus)) {
+    PileUp* p = &R->pu[locus - R->start];
+    if (pair) { pu_add(p, base, qual, mq, cfg); R->baseCount += 1; }
+    else p->badPair = jadd(p->badPair, 1);
+  }
+}
+static void pr_remove(PRegion* R, int64_t locus, int pair) {
+  if (inRegion(R, locus)) {
+    if (!pair) R->pu[locus - R->start].badPair = jsub(R->pu[locus - R->start].badPair, 1);
+  }
+}
+
+/* PileUpRegion.physCovIncr (PileUpRegion.scala:62-88) */
+static int32_t pr_physCovIncr(PRegion* R, int32_t aStart, int32_t aEnd, int32_t iSize, int paired, int valid) {
+  if (!valid || (paired && iSize <= 0)) return 0;
+  int32_t start, end;
+  if (!paired) { start = aStart < aEnd ? aStart : aEnd; end = aStart > aEnd ? aStart : aEnd; }
+  else if (iSize > 0) { start = aStart; end = jadd(aStart, iSize); }
+  else { start = jadd(jadd(aEnd, 1), iSize); end = jadd(aEnd, 1); }
+  int32_t insertSize = jsub(end, start);
+  if (inRegion(R, start)) {
+    PileUp* p = &R->pu[start - R->start];
+    p->physCov = jadd(p->physCov, 1);
+    p->insertSize = jadd(p->insertSize, insertSize);
+  } else if (beforeRegion(R, start) && !beforeRegion(R, end)) {
+    R->physCovStart = jadd(R->physCovStart, 1);
+    R->insertSizeStart = jadd(R->insertSizeStart, insertSize);
+  }
+  if (inRegion(R, end)) {
+    PileUp* p = &R->pu[end - R->start];
+    p->physCov = jsub(p->physCov, 1);
+    p->insertSize = jsub(p->insertSize, insertSize);
+  }
+  return insertSize;
+}
+
+/* PileUpRegion.computePhysCov (PileUpRegion.scala:90-100) */
+static void pr_computePhysCov(PRegion* R) {
+  R->pu[0].physCov = jadd(R->pu[0].physCov, R->physCovStart);
+  R->pu[0].insertSize = jadd(R->pu[0].insertSize, R->insertSizeStart);
+  for (int32_t i = 1; i < R->size; i++) {
+    R->pu[i].physCov = jadd(R->pu[i].physCov, R->pu[i - 1].physCov);
+    R->pu[i].insertSize = jadd(R->pu[i].insertSize, R->pu[i - 1].insertSize);
+  }
+  for (int32_t i = 0; i < R->size; i++)
+    if (R->pu[i].physCov > 0) R->pu[i].insertSize = jdiv(R->pu[i].insertSize, R->pu[i].physCov);
+}
+
+/* A decoded SAMRecord view. */
+typedef struct {
+  int32_t length;           /* getReadLength */
+  const uint8_t* bases;     /* getReadBases (ASCII) */
+  const int8_t* quals;      /* getBaseQualities or the defaultQual fill */
+  int32_t mq, flag, tlen, aStart, aEnd;
+  const uint32_t* cigar; int64_t ncigar;
+  int64_t index;
+} ReadView;
+
+static const char NIB[] = "=ACMGRSVTWYHKDBN";
+static int consumesRead(int op) { return op == 0 || op == 1 || op == 4 || op == 7 || op == 8; }
+static int consumesRef(int op) { return op == 0 || op == 2 || op == 3 || op == 7 || op == 8; }
+
+#define EDGE(R, r, why) do { if (!(R)->edge) { (R)->edge = 1; (R)->edgeRead = (r)->index; (R)->edgeWhy = (why); } return; } while (0)
+
+/* PileUpRegion.addRead with longRead = 0 (PileUpRegion.scala:102-220). */
+static void pr_addRead(PRegion* R, const ReadView* r, const uint8_t* refBases, const pl_config* cfg) {
+  const int32_t length = r->length;
+  const uint8_t* bases = r->bases;
+  const int32_t mq = r->mq;
+  const int paired = (r->flag & 0x1) != 0;
+  const int valid = (mq >= cfg->min_mq) &&
+      (!paired || (((r->flag & 0x2) != 0) && ((r->flag & PL_RF_MATE_SAME_REF) != 0)));
+  const int32_t insert = r->tlen, aStart = r->aStart, aEnd = r->aEnd;
+  const int8_t* quals = r->quals;
+  const int32_t flank = cfg->flank;
+#define TRUSTED(o) ((o) >= flank && jsub(length, flank) > (o))
+
+  int32_t clipped = 0;
+  for (int64_t k = 0; k < r->ncigar; k++)
+    if ((r->cigar[k] & 0xf) == 4) clipped = jadd(clipped, (int32_t)(r->cigar[k] >> 4));
+  int32_t adjMq = roundDivI(jmul(mq, jsub(length, clipped)), length);      /* :139-141 */
+  int32_t indelMq = adjMq;                                                  /* :142 (longRead = 0) */
+
+  int32_t readOffset = 0, refOffset = 0;
+  for (int64_t k = 0; k < r->ncigar; k++) {
+    int32_t len = (int32_t)(r->cigar[k] >> 4);
+    int op = (int)(r->cigar[k] & 0xf);
+    int32_t locus = jadd(aStart, refOffset);
+    if (op == 1) {                                                          /* I :150-162 */
+      int32_t iloc = locus;
+      if (valid && TRUSTED(readOffset) && inRegion(R, iloc)) {
+        uint8_t* ins = arena_alloc(&R->ar, (size_t)len + 1);
+        memcpy(ins, bases + readOffset, (size_t)len);
+        while (iloc > 1 && refBases[iloc - 2] == ins[len - 1]) {
+          iloc -= 1;
+          uint8_t last = ins[len - 1];                 /* slice(len-1,len) ++ slice(0,len-1) */
+          memmove(ins + 1, ins, (size_t)len - 1);
+          ins[0] = last;
+        }
+        if (iloc < R->start) EDGE(R, r, "insertion normalised before region start");
+        pu_addInsertion(&R->pu[iloc - R->start], ins, len, quals[readOffset], indelMq);
+      }
+    } else if (op == 2) {                                                   /* D :163-183 */
+      int32_t dloc = locus, rloc = readOffset;
+      if (valid && TRUSTED(readOffset) && inRegion(R, dloc) && inRegion(R, (int64_t)dloc + len - 1)) {
+        while (dloc > 1 && rloc > 0 && refBases[dloc - 2] == refBases[dloc + len - 2]) {
+          dloc -= 1;
+          rloc -= 1;
+          uint8_t base = bases[rloc];
+          int32_t qual = quals[rloc];
+          if (TRUSTED(rloc) && inRegion(R, dloc)) {
+            pr_remove(R, dloc, valid);
+            if (inRegion(R, (int64_t)dloc + len)) pr_add(R, (int64_t)dloc + len, base, qual, adjMq, valid, cfg);
+          }
+        }
+        if (dloc < R->start) EDGE(R, r, "deletion normalised before region start");
+        pu_addDeletion(&R->pu[dloc - R->start], refBases + (dloc - 1), len, quals[readOffset], indelMq);
+      }
+    } else if (op == 0 || op == 7 || op == 8) {                             /* M/=/X :184-193 */
+      for (int32_t i = 0; i < len; i++) {
+        int32_t rOff = readOffset + i;
+        if (TRUSTED(rOff)) pr_add(R, (int64_t)locus + i, bases[rOff], quals[rOff], adjMq, valid, cfg);
+      }
+    } else if (op == 4) {                                                   /* S :194-206 */
+      int32_t clipStart = readOffset == 0 ? jsub(locus, len) : locus;
+      int32_t clipEnd = jsub(jadd(clipStart, len), 1);
+      if (inRegion(R, clipStart)) R->pu[clipStart - R->start].clips = jadd(R->pu[clipStart - R->start].clips, 1);
+      if (inRegion(R, clipEnd)) R->pu[clipEnd - R->start].clips = jadd(R->pu[clipEnd - R->start].clips, 1);
+      for (int32_t i = 0; i < len; i++) {
+        int32_t rOff = readOffset + i;
+        if (rOff >= length) EDGE(R, r, "soft clip beyond read bases");
+        int64_t locusPlus = (int64_t)clipStart + i;
+        if (inRegion(R, locusPlus)) pr_add(R, locusPlus, bases[rOff], quals[rOff], adjMq, 0, cfg);
+      }
+    } else if (op == 5 || op == 3) {                                        /* H, N :207-210 */
+    } else {
+      R->unknownOps++;                                                      /* :211-212 */
+    }
+    if (consumesRead(op)) readOffset = jadd(readOffset, len);
+    if (consumesRef(op)) refOffset = jadd(refOffset, len);
+  }
+  R->readCount = jadd(R->readCount, 1);
+  pr_physCovIncr(R, aStart, aEnd, insert, paired, valid);
+#undef TRUSTED
+}
+
+/* ------------------------------------------------------- read access */
+typedef struct {
+  const pl_read_soa* s;
+  uint8_t* bases; int8_t* quals; int32_t cap;
+} Decoder;
+
+/* htsjdk getAlignmentEnd: 0 for an unmapped read, else start + ref length - 1. */
+static int32_t alignment_end(const pl_read_soa* s, int64_t i) {
+  if (s->flag[i] & 0x4) return 0;
+  int32_t rl = 0;
+  for (int64_t k = s->cigar_off[i]; k < s->cigar_off[i + 1]; k++)
+    if (consumesRef((int)(s->cigar[k] & 0xf))) rl += (int32_t)(s->cigar[k] >> 4);
+  return s->pos[i] + rl - 1;
+}
+
+static int64_t exotic_slot(const pl_read_soa* s, int64_t i) {
+  int64_t lo = 0, hi = s->n_exotic;
+  while (lo < hi) { int64_t m = (lo + hi) / 2; if (s->exo_read[m] < i) lo = m + 1; else hi = m; }
+  return (lo < s->n_exotic && s->exo_read[lo] == i) ? lo : -1;
+}
+
+static int decode_read(Decoder* d, int64_t i, const pl_config* cfg, ReadView* r) {
+  const pl_read_soa* s = d->s;
+  int64_t o = s->seq_off[i];
+  int32_t L = (int32_t)(s->seq_off[i + 1] - o);
+  if (L + 1 > d->cap) {
+    d->cap = 2 * (L + 1);
+    d->bases = (uint8_t*)realloc(d->bases, (size_t)d->cap);
+    d->quals = (int8_t*)realloc(d->quals, (size_t)d->cap);
+  }
+  if (s->flag[i] & PL_RF_EXOTIC) {
+    int64_t slot = exotic_slot(s, i);
+    if (slot < 0) return -1;
+    int64_t nb = s->exo_off[slot];
+    for (int32_t k = 0; k < L; k++) {
+      int64_t q = nb + k;
+      uint8_t byte = s->nib[q >> 1];
+      d->bases[k] = (uint8_t)NIB[(q & 1) ? (byte & 0xf) : (byte >> 4)];
+    }
+  } else {
+    for (int32_t k = 0; k < L; k++) {
+      int64_t q = o + k;
+      d->bases[k] = (uint8_t)"ACGT"[(s->bases2[q >> 2] >> (2 * (q & 3))) & 3];
+    }
+  }
+  for (int32_t k = 0; k < L; k++)
+    d->quals[k] = (s->flag[i] & PL_RF_NO_QUAL) ? (int8_t)cfg->default_qual : s->quals[o + k];
+  r->length = L; r->bases = d->bases; r->quals = d->quals;
+  r->mq = s->mapq[i]; r->flag = s->flag[i]; r->tlen = s->tlen[i];
+  r->aStart = s->pos[i]; r->aEnd = alignment_end(s, i);
+  r->cigar = s->cigar + s->cigar_off[i]; r->ncigar = s->cigar_off[i + 1] - s->cigar_off[i];
+  r->index = i;
+  /* CIGAR and SEQ must agree (htsjdk accepts such records silently, Pilon
+   * then indexes past the arrays); a read without SEQ is allowed. */
+  if (L > 0) {
+    int64_t rl = 0;
+    for (int64_t k = 0; k < r->ncigar; k++)
+      if (consumesRead((int)(r->cigar[k] & 0xf))) rl += (int64_t)(r->cigar[k] >> 4);
+    if (rl != L) return -2;
+  }
+  for (int64_t k = 0; k < r->ncigar; k++) if ((r->cigar[k] & 0xf) > 8) return -2;
+  return 0;
+}
+
+/* BamFile.validateRead (BamFile.scala:101-105) */
+static int validateRead(uint16_t flag, const pl_config* cfg) {
+  return (cfg->nonpf || !(flag & 0x200)) && (cfg->duplicates || !(flag & 0x400)) && !(flag & 0x100);
+}
+
+/* ------------------------------------------------------------ fixes */
+typedef struct { int32_t loc; const uint8_t* was; int32_t wasLen; const uint8_t* patch; int32_t patchLen; int64_t seq; } Fix;
+typedef struct { Fix* v; int32_t n, cap; } FixList;
+
+static void fix_prepend(FixList* l, Fix f) {               /* list ::= fix (kept reversed, fixed later) */
+  if (l->n == l->cap) { l->cap = l->cap ? 2 * l->cap : 16; l->v = (Fix*)realloc(l->v, sizeof(Fix) * (size_t)l->cap); }
+  l->v[l->n++] = f;
+}
+/* materialise a prepend-built list in List order (last prepended first) */
+static void fix_as_list(const FixList* l, Fix* out) { for (int32_t i = 0; i < l->n; i++) out[i] = l->v[l->n - 1 - i]; }
+
+static int cmp_fix_stable(const void* a, const void* b) {
+  const Fix* x = (const Fix*)a; const Fix* y = (const Fix*)b;
+  if (x->loc != y->loc) return x->loc < y->loc ? -1 : 1;
+  return x->seq < y->seq ? -1 : (x->seq > y->seq);
+}
+
+/* GenomeRegion.fixFixList (GenomeRegion.scala:557-593).  in[] is in List order;
+ * returns the number of kept fixes written to out[] in coordinate order. */
+static int32_t fixFixList(const Fix* in, int32_t n, Fix* out) {
+  Fix* f = (Fix*)malloc(sizeof(Fix) * (size_t)(n ? n : 1));
+  for (int32_t i = 0; i < n; i++) { f[i] = in[i]; f[i].seq = i; }
+  qsort(f, (size_t)n, sizeof(Fix), cmp_fix_stable);       /* sortWith is a stable sort */
+  int32_t m = 0, i = 0;
+  if (n == 0) { free(f); return 0; }
+  Fix cur = f[0]; i = 1;
+  while (1) {
+    if (i >= n) { out[m++] = cur; break; }
+    Fix nx = f[i];
+    int32_t s1 = cur.loc, e1 = cur.loc + ((cur.wasLen - 1) > 0 ? (cur.wasLen - 1) : 0);
+    int32_t s2 = nx.loc, e2 = nx.loc + ((nx.wasLen - 1) > 0 ? (nx.wasLen - 1) : 0);
+    if (s2 <= e1 && e2 >= s1) {                                       /* Region.overlaps */
+      int32_t l1 = cur.wasLen + cur.patchLen, l2 = nx.wasLen + nx.patchLen;
+      if (l1 < l2) cur = nx;
+      i++;
+    } else {
+      out[m++] = cur; cur = nx; i++;
+    }
+  }
+  free(f);
+  return m;
+}
+
+/* GenomeRegion.fixIssues (GenomeRegion.scala:595-621), on a growable buffer. */
+static void fixIssues(uint8_t** bases, int64_t* blen, const PRegion* R, const Fix* in, int32_t n) {
+  Fix* kept = (Fix*)malloc(sizeof(Fix) * (size_t)(n ? n : 1));
+  int32_t m = fixFixList(in, n, kept);
+  for (int32_t k = m - 1; k >= 0; k--) {
+    const Fix* f = &kept[k];
+    int64_t start = f->loc - R->start;
+    if (f->wasLen == f->patchLen) {
+      for (int32_t i = 0; i < f->wasLen; i++) (*bases)[start + i] = f->patch[i];
+    } else {
+      int64_t nl = *blen - f->wasLen + f->patchLen;
+      uint8_t* nb = (uint8_t*)malloc((size_t)(nl ? nl : 1));
+      memcpy(nb, *bases, (size_t)start);
+      memcpy(nb + start, f->patch, (size_t)f->patchLen);
+      memcpy(nb + start + f->patchLen, *bases + start + f->wasLen, (size_t)(*blen - start - f->wasLen));
+      free(*bases); *bases = nb; *blen = nl;
+    }
+  }
+  free(kept);
+}
+
+/* ----------------------------------------------------------- results */
+typedef struct orc_result {
+  int32_t status;            /* pl_status */
+  int64_t edge_read;
+  char    edge_why[64];
+  int32_t size;
+  int32_t* pile;             /* [PL_NFIELDS*size] Int view (BaseSum Long truncated) */
+  int64_t* bcq;              /* [8*size] exact BaseSum Longs                       */
+  pl_call* calls;            /* [size] BaseCall on the final PileUp + dispositions */
+  int32_t* coverage;         /* GenomeRegion.coverage etc. (GenomeRegion.scala:247-253) */
+  int32_t* badCoverage;
+  int32_t* physCoverage;
+  int32_t* insertSize;
+  int8_t*  weightedQual;
+  int8_t*  weightedMq;
+  int16_t* clips;
+  pl_region_stats stats;
+  char*    changes_text; int64_t changes_len;
+  uint8_t* bases; int64_t bases_len;
+  /* the called indel string at each INS/DEL change, packed: for change c,
+   * indel_pos[c] (region index) and indel_str[indel_off[c] .. indel_off[c+1]) */
+  int32_t  n_indel; int32_t* indel_pos; int64_t* indel_off; uint8_t* indel_str;
+} orc_result;
+
+void orc_result_free(orc_result* r) {
+  if (!r) return;
+  free(r->pile); free(r->bcq); free(r->calls); free(r->coverage); free(r->badCoverage);
+  free(r->physCoverage); free(r->insertSize); free(r->weightedQual); free(r->weightedMq);
+  free(r->clips); free(r->changes_text); free(r->bases);
+  free(r->indel_pos); free(r->indel_off); free(r->indel_str);
+  free(r);
+}
+
+static void sb_append(char** buf, int64_t* len, int64_t* cap, const char* s, int64_t n) {
+  if (*len + n + 1 > *cap) { *cap = 2 * (*len + n + 1) + 256; *buf = (char*)realloc(*buf, (size_t)*cap); }
+  memcpy(*buf + *len, s, (size_t)n); *len += n; (*buf)[*len] = 0;
+}
+/* Region.regionString (Region.scala:42) */
+static void region_string(char** buf, int64_t* len, int64_t* cap, const char* name, int64_t s, int64_t e) {
+  char tmp[64];
+  sb_append(buf, len, cap, name, (int64_t)strlen(name));
+  int n = (e + 1 - s < 2) ? snprintf(tmp, sizeof tmp, ":%lld", (long long)s)
+                          : snprintf(tmp, sizeof tmp, ":%lld-%lld", (long long)s, (long long)e);
+  sb_append(buf, len, cap, tmp, n);
+}
+
+enum { K_SNP = 1, K_INS, K_DEL, K_AMB };
+
+static uint8_t upper(uint8_t c) { return (c >= 'a' && c <= 'z') ? (uint8_t)(c - 32) : c; }
+
+/* Bases.toIUPAC (Bases.scala:67-88) */
+static uint8_t toIUPAC(int b1, int b2) {
+  static const char T[16] = {0, 'A', 'C', 'M', 'G', 'R', 'S', 'V', 'T', 'W', 'Y', 'H', 'K', 'D', 'B', 'N'};
+  int bits = (1 << b1) | (1 << b2);
+  return (uint8_t)T[bits];
+}
+
+/*
+ * Run one GenomeRegion end to end: BamFile.process over each read set,
+ * postProcess Pass 1, identifyAndFixIssues (snps/indels), writeChanges.
+ * contig: raw bytes, name: contig name, new_name/offset: writeChanges args.
+ */
+orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
+                           const char* name, int32_t start, int32_t stop,
+                           const pl_read_soa* const* sets, int32_t n_sets,
+                           const char* new_name, int64_t offset) {
+  orc_result* res = (orc_result*)calloc(1, sizeof(orc_result));
+  if (start < 1 || stop < start || stop > contig_len) { res->status = PL_E_INVAL; return res; }  /* GenomeRegion.scala:34 */
+  PRegion R; memset(&R, 0, sizeof R);
+  R.start = start; R.stop = stop; R.size = stop + 1 - start;
+  R.pu = (PileUp*)calloc((size_t)R.size, sizeof(PileUp));
+  const int32_t size = R.size;
+
+  /* BamFile.process (BamFile.scala:108-148): query ±10 kb, validateRead, addRead */
+  int64_t qs = (int64_t)start - 10000 > 0 ? (int64_t)start - 10000 : 0;
+  int64_t qe = (int64_t)stop + 10000 < contig_len ? (int64_t)stop + 10000 : contig_len;
+  for (int32_t si = 0; si < n_sets && !R.edge; si++) {
+    const pl_read_soa* s = sets[si];
+    Decoder d = {s, NULL, NULL, 0};
+    for (int64_t i = 0; i < s->n_reads; i++) {
+      int32_t aEnd = alignment_end(s, i);
+      int64_t spanEnd = aEnd != 0 ? aEnd : s->pos[i];
+      if (!(s->pos[i] <= qe && spanEnd >= qs)) continue;           /* queryOverlapping */
+      if (!validateRead(s->flag[i], cfg)) continue;
+      ReadView r;
+      int rc = decode_read(&d, i, cfg, &r);
+      if (rc != 0) { res->status = PL_E_INVAL; res->edge_read = i; snprintf(res->edge_why, sizeof res->edge_why, "malformed read"); break; }
+      /* aligned-base unit: M/=/X bases whose locus is in the region */
+      int32_t ro = 0, fo = 0;
+      for (int64_t k = 0; k < r.ncigar; k++) {
+        int op = (int)(r.cigar[k] & 0xf); int32_t len = (int32_t)(r.cigar[k] >> 4);
+        if (op == 0 || op == 7 || op == 8) {
+          int64_t a = (int64_t)r.aStart + fo, b = a + len - 1;
+          if (a < start) a = start;
+          if (b > stop) b = stop;
+          if (b >= a) R.alignedBases += b - a + 1;
+        }
+        if (consumesRead(op)) ro += len;
+        if (consumesRef(op)) fo += len;
+      }
+      pr_addRead(&R, &r, contig, cfg);
+      if (R.edge) break;
+    }
+    free(d.bases); free(d.quals);
+    if (res->status) break;
+  }
+  if (R.edge) {
+    res->status = PL_E_EDGE; res->edge_read = R.edgeRead;
+    snprintf(res->edge_why, sizeof res->edge_why, "%s", R.edgeWhy);
+  }
+  if (res->status) {
+    for (int32_t i = 0; i < size; i++) { free(R.pu[i].ins.v); free(R.pu[i].del.v); }
+    free(R.pu); arena_free(&R.ar);
+    return res;
+  }
+
+  /* GenomeRegion.postProcess (GenomeRegion.scala:214-272) */
+  pr_computePhysCov(&R);
+  int64_t meanCoverage = roundDivL(R.baseCount, (int64_t)size);          /* PileUpRegion.coverage */
+  int32_t minDepth;
+  if (cfg->min_depth >= 1) minDepth = (int32_t)cfg->min_depth;
+  else {
+    int32_t md = (int32_t)orc_java_round(cfg->min_depth * (double)meanCoverage);
+    minDepth = md > cfg->min_min_depth ? md : cfg->min_min_depth;
+  }
+  uint8_t* confirmed = (uint8_t*)calloc((size_t)size, 1);
+  uint8_t* deleted = (uint8_t*)calloc((size_t)size, 1);
+  uint8_t* kindAt = (uint8_t*)calloc((size_t)size, 1);                  /* changeMap */
+  res->coverage = (int32_t*)calloc((size_t)size, 4);
+  res->badCoverage = (int32_t*)calloc((size_t)size, 4);
+  res->physCoverage = (int32_t*)calloc((size_t)size, 4);
+  res->insertSize = (int32_t*)calloc((size_t)size, 4);
+  res->weightedQual = (int8_t*)calloc((size_t)size, 1);
+  res->weightedMq = (int8_t*)calloc((size_t)size, 1);
+  res->clips = (int16_t*)calloc((size_t)size, 2);
+  const int fixamb = cfg->iupac || cfg->fix_amb;
+  if (R.readCount != 0) {
+    for (int32_t i = 0; i < size; i++) {
+      PileUp* pu = &R.pu[i];
+      int64_t n = pu_depth(pu);
+      BaseCall bc; baseCall(pu, cfg, &bc);
+      uint8_t b = bc.base == 4 ? 'N' : (uint8_t)"ACGT"[bc.base];
+      uint8_t rr = upper(contig[(int64_t)start + i - 1]);              /* refBase (GenomeRegion.scala:783-787) */
+      res->coverage[i] = (int32_t)n;
+      res->badCoverage[i] = pu->badPair;
+      res->physCoverage[i] = pu->physCov;
+      res->insertSize[i] = pu->insertSize;
+      int64_t qsum = pu->qs[0] + pu->qs[1] + pu->qs[2] + pu->qs[3];
+      res->weightedQual[i] = (int8_t)roundDivL(qsum, (int64_t)pu->mqSum);   /* PileUp.scala:60-62 */
+      res->weightedMq[i] = (int8_t)roundDivL(qsum, (int64_t)pu->qSum);     /* PileUp.scala:56-58 */
+      res->clips[i] = (int16_t)pu->clips;
+      if (n >= minDepth && rr != 'N' && !deleted[i] && bc.called) {
+        if (bc.homo && b == rr && bc.highConf && !bc.indel) confirmed[i] = 1;
+        else if (bc.isIns && bc.homoIndel) kindAt[i] = K_INS;
+        else if (bc.isDel && bc.homoIndel) {
+          kindAt[i] = K_DEL;
+          for (int32_t j = 1; j < bc.indelLen; j++) {
+            deleted[i + j] = 1;
+            R.pu[i + j].deletions = jadd(R.pu[i + j].deletions, pu->deletions);
+          }
+        } else if (b != rr && bc.score > 0) {
+          if (bc.homo) kindAt[i] = K_SNP;
+          else if (fixamb || (uint8_t)"ACGT"[bc.alt] != rr) kindAt[i] = K_AMB;
+        }
+      }
+    }
+  }
+
+  /* final PileUp state and BaseCall per position */
+  res->size = size;
+  res->pile = (int32_t*)malloc(sizeof(int32_t) * (size_t)size * PL_NFIELDS);
+  res->bcq = (int64_t*)malloc(sizeof(int64_t) * (size_t)size * 8);
+  res->calls = (pl_call*)calloc((size_t)size, sizeof(pl_call));
+  int nConfirmed = 0;
+  for (int32_t i = 0; i < size; i++) {
+    PileUp* p = &R.pu[i];
+    for (int b = 0; b < 4; b++) {
+      res->bcq[b * (size_t)size + i] = p->bc[b];
+      res->bcq[(4 + b) * (size_t)size + i] = p->qs[b];
+      res->pile[(PL_F_COUNT_A + b) * (size_t)size + i] = (int32_t)p->bc[b];
+      res->pile[(PL_F_QUAL_A + b) * (size_t)size + i] = (int32_t)p->qs[b];
+    }
+    res->pile[PL_F_MQSUM * (size_t)size + i] = p->mqSum;
+    res->pile[PL_F_QSUM * (size_t)size + i] = p->qSum;
+    res->pile[PL_F_PHYSCOV * (size_t)size + i] = p->physCov;
+    res->pile[PL_F_INSERTSIZE * (size_t)size + i] = p->insertSize;
+    res->pile[PL_F_BADPAIR * (size_t)size + i] = p->badPair;
+    res->pile[PL_F_DELETIONS * (size_t)size + i] = p->deletions;
+    res->pile[PL_F_DELQUAL * (size_t)size + i] = p->delQual;
+    res->pile[PL_F_INSERTIONS * (size_t)size + i] = p->insertions;
+    res->pile[PL_F_INSQUAL * (size_t)size + i] = p->insQual;
+    res->pile[PL_F_CLIPS * (size_t)size + i] = p->clips;
+    BaseCall bc; baseCall(p, cfg, &bc);
+    pl_call* c = &res->calls[i];
+    c->base = (uint8_t)bc.base; c->alt = (uint8_t)bc.alt; c->q = (int32_t)bc.q;
+    uint16_t f = 0;
+    if (bc.homo) f |= PL_CF_HOMO;
+    if (bc.highConf) f |= PL_CF_HIGHCONF;
+    if (bc.called) f |= PL_CF_CALLED;
+    if (bc.indel) f |= PL_CF_INDEL;
+    if (bc.homoIndel) f |= PL_CF_HOMOINDEL;
+    if (bc.isIns) f |= PL_CF_INSERTION;
+    if (bc.isDel) f |= PL_CF_DELETION;
+    if (bc.score > 0) f |= PL_CF_SCOREPOS;
+    int disp = PL_D_NONE;
+    if (confirmed[i]) disp = PL_D_CONFIRMED;
+    else if (kindAt[i] == K_SNP) disp = PL_D_SNP;
+    else if (kindAt[i] == K_INS) disp = PL_D_INS;
+    else if (kindAt[i] == K_DEL) disp = PL_D_DEL;
+    else if (kindAt[i] == K_AMB) disp = PL_D_AMB;
+    f |= (uint16_t)(disp << PL_CF_DISP_SHIFT);
+    if (deleted[i]) f |= PL_CF_DELETED;
+    c->flags = f;
+    nConfirmed += confirmed[i];
+  }
+
+  /* identifyAndFixIssues, snps/indels path (GenomeRegion.scala:307-380,413) */
+  FixList snpL = {0}, smallL = {0};
+  int snps = 0, ins = 0, dels = 0, insBases = 0, delBases = 0, amb = 0;
+  res->indel_pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)(size + 1));
+  res->indel_off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(size + 2));
+  int64_t strCap = 1024, strLen = 0; res->indel_str = (uint8_t*)malloc((size_t)strCap);
+  res->indel_off[0] = 0;
+  static const uint8_t dot = '.';
+  (void)dot;
+  uint8_t* oneChar = (uint8_t*)malloc((size_t)size * 2 + 2);             /* storage for 1-char strings */
+  for (int32_t i = 0; i < size; i++) {                                   /* changeList is sorted */
+    if (!kindAt[i]) continue;
+    int32_t loc = start + i;
+    uint8_t rBase = upper(contig[(int64_t)loc - 1]);
+    BaseCall bc; baseCall(&R.pu[i], cfg, &bc);
+    uint8_t cBase = bc.base == 4 ? 'N' : (uint8_t)"ACGT"[bc.base];
+    oneChar[2 * i] = rBase; oneChar[2 * i + 1] = cBase;
+    Fix f; memset(&f, 0, sizeof f); f.loc = loc;
+    switch (kindAt[i]) {
+      case K_SNP:
+        if (cfg->fix_snps) { f.was = &oneChar[2 * i]; f.wasLen = 1; f.patch = &oneChar[2 * i + 1]; f.patchLen = 1; fix_prepend(&snpL, f); }
+        snps++;
+        break;
+      case K_AMB:
+        if (cfg->fix_snps && !cfg->longread) {
+          if (cfg->iupac) {
+            oneChar[2 * i + 1] = toIUPAC(bc.base == 4 ? 0 : bc.base, bc.alt);
+            f.was = &oneChar[2 * i]; f.wasLen = 1; f.patch = &oneChar[2 * i + 1]; f.patchLen = 1; fix_prepend(&smallL, f);
+          } else {
+            f.was = &oneChar[2 * i]; f.wasLen = 1; f.patch = &oneChar[2 * i + 1]; f.patchLen = 1; fix_prepend(&snpL, f);
+          }
+          amb++;
+        }
+        break;
+      case K_INS:
+        if (cfg->fix_indels) { f.was = NULL; f.wasLen = 0; f.patch = bc.indelStr; f.patchLen = bc.indelLen; fix_prepend(&smallL, f); }
+        ins++; insBases += bc.indelLen;
+        break;
+      case K_DEL:
+        if (cfg->fix_indels) { f.was = bc.indelStr; f.wasLen = bc.indelLen; f.patch = NULL; f.patchLen = 0; fix_prepend(&smallL, f); }
+        dels++; delBases += bc.indelLen;
+        break;
+    }
+    if (kindAt[i] == K_INS || kindAt[i] == K_DEL) {
+      if (strLen + bc.indelLen > strCap) { strCap = 2 * (strLen + bc.indelLen); res->indel_str = (uint8_t*)realloc(res->indel_str, (size_t)strCap); }
+      memcpy(res->indel_str + strLen, bc.indelStr, (size_t)bc.indelLen);
+      strLen += bc.indelLen;
+      res->indel_pos[res->n_indel] = i;
+      res->indel_off[++res->n_indel] = strLen;
+    }
+  }
+
+  /* region bases, then fixIssues(snpFixList); fixIssues(smallFixList ++ bigFixList) */
+  int64_t blen = size;
+  uint8_t* bases = (uint8_t*)malloc((size_t)size);
+  memcpy(bases, contig + start - 1, (size_t)size);                       /* GenomeRegion.refBases */
+  Fix* tmp = (Fix*)malloc(sizeof(Fix) * (size_t)(snpL.n + smallL.n + 1));
+  fix_as_list(&snpL, tmp);
+  fixIssues(&bases, &blen, &R, tmp, snpL.n);
+  fix_as_list(&smallL, tmp);
+  fixIssues(&bases, &blen, &R, tmp, smallL.n);
+
+  /* writeChanges (GenomeRegion.scala:645-657) */
+  fix_as_list(&snpL, tmp);
+  fix_as_list(&smallL, tmp + snpL.n);
+  Fix* kept = (Fix*)malloc(sizeof(Fix) * (size_t)(snpL.n + smallL.n + 1));
+  int32_t nk = fixFixList(tmp, snpL.n + smallL.n, kept);
+  char* txt = NULL; int64_t tl = 0, tc = 0;
+  sb_append(&txt, &tl, &tc, "", 0);
+  int64_t delta = 0;
+  for (int32_t k = 0; k < nk; k++) {
+    const Fix* f = &kept[k];
+    int64_t newLoc = f->loc + delta;
+    region_string(&txt, &tl, &tc, name, f->loc, (int64_t)f->loc + f->wasLen - 1);
+    sb_append(&txt, &tl, &tc, " ", 1);
+    region_string(&txt, &tl, &tc, new_name, newLoc + offset, newLoc + offset + f->patchLen - 1);
+    sb_append(&txt, &tl, &tc, " ", 1);
+    if (f->wasLen == 0) sb_append(&txt, &tl, &tc, ".", 1); else sb_append(&txt, &tl, &tc, (const char*)f->was, f->wasLen);
+    sb_append(&txt, &tl, &tc, " ", 1);
+    if (f->patchLen == 0) sb_append(&txt, &tl, &tc, ".", 1); else sb_append(&txt, &tl, &tc, (const char*)f->patch, f->patchLen);
+    sb_append(&txt, &tl, &tc, "\n", 1);
+    delta += f->patchLen - f->wasLen;
+  }
+  res->changes_text = txt; res->changes_len = tl;
+  res->bases = bases; res->bases_len = blen;
+
+  res->stats.read_count = R.readCount;
+  res->stats.base_count = R.baseCount;
+  res->stats.coverage = meanCoverage;
+  res->stats.min_depth = minDepth;
+  res->stats.n_confirmed = nConfirmed;
+  res->stats.snps = snps; res->stats.amb = amb; res->stats.ins = ins; res->stats.ins_bases = insBases;
+  res->stats.dels = dels; res->stats.del_bases = delBases;
+  res->stats.n_unknown_ops = (int32_t)R.unknownOps;
+  res->stats.aligned_bases = R.alignedBases;
+
+  free(tmp); free(kept); free(snpL.v); free(smallL.v); free(oneChar);
+  free(confirmed); free(deleted); free(kindAt);
+  for (int32_t i = 0; i < size; i++) { free(R.pu[i].ins.v); free(R.pu[i].del.v); }
+  free(R.pu);
+  /* indel strings point into the arena: copied above, so it can go */
+  arena_free(&R.ar);
+  return res;
+}
+
+/* Exposed for known-answer tests of the small helpers. */
+int32_t orc_roundDivI(int32_t n, int32_t d) { return roundDivI(n, d); }
+int64_t orc_roundDivL(int64_t n, int64_t d) { return roundDivL(n, d); }
+
+/* BaseCall on a hand-made PileUp: counts bc[4], qs[4], and the Int fields in
+ * PileUp order (mqSum,qSum,deletions,insertions,insQual,delQual); indel lists
+ * are given as packed strings (ins then del).  Used to pin BaseCall and
+ * hetIndelCall on exhaustive small cases. */
+void orc_basecall(const pl_config* cfg, const int64_t* bc4, const int64_t* qs4,
+                  int32_t mqSum, int32_t qSum, int32_t deletions, int32_t insertions,
+                  int32_t insQual, int32_t delQual,
+                  int32_t n_ins, const int32_t* ins_len, const uint8_t* ins_bytes,
+                  int32_t n_del, const int32_t* del_len, const uint8_t* del_bytes,
+                  pl_call* out, int32_t* indel_len, int64_t* score) {
+  PileUp p; memset(&p, 0, sizeof p);
+  for (int b = 0; b < 4; b++) { p.bc[b] = bc4[b]; p.qs[b] = qs4[b]; }
+  p.mqSum = mqSum; p.qSum = qSum; p.deletions = deletions; p.insertions = insertions;
+  p.insQual = insQual; p.delQual = delQual;
+  const uint8_t* s = ins_bytes;
+  for (int32_t k = 0; k < n_ins; k++) { olist_push(&p.ins, s, ins_len[k]); s += ins_len[k]; }
+  s = del_bytes;
+  for (int32_t k = 0; k < n_del; k++) { olist_push(&p.del, s, del_len[k]); s += del_len[k]; }
+  BaseCall bc; baseCall(&p, cfg, &bc);
+  out->base = (uint8_t)bc.base; out->alt = (uint8_t)bc.alt; out->q = (int32_t)bc.q;
+  uint16_t f = 0;
+  if (bc.homo) f |= PL_CF_HOMO;
+  if (bc.highConf) f |= PL_CF_HIGHCONF;
+  if (bc.called) f |= PL_CF_CALLED;
+  if (bc.indel) f |= PL_CF_INDEL;
+  if (bc.homoIndel) f |= PL_CF_HOMOINDEL;
+  if (bc.isIns) f |= PL_CF_INSERTION;
+  if (bc.isDel) f |= PL_CF_DELETION;
+  if (bc.score > 0) f |= PL_CF_SCOREPOS;
+  out->flags = f;
+  *indel_len = bc.indelLen;
+  *score = bc.score;
+  free(p.ins.v); free(p.del.v);
+}

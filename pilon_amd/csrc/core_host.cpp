@@ -1,0 +1,293 @@
+// core_host.cpp — host build of the kernel core (pl_core.h), for CPU tests only.
+//
+// It runs the same per-read walk, window flush, indel pass and Pass-1
+// finalize the gfx950 kernels run, one window and one read at a time, so the
+// window decomposition (scan margins, outliers, deferral of indel columns,
+// deletion pass) can be compared with the oracle without a GPU.  It is not
+// part of libpilonhip.so and the product never loads it.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pl_core.h"
+#include "pl_host.h"
+
+namespace {
+
+struct HostSink {
+  int W;
+  uint64_t *bs, *mqq, *ins_, *del_;
+  uint32_t *bad_, *clp_;
+  int32_t *pc, *is;
+  std::vector<IndelRec>* recs;
+  int64_t bcount = 0, aligned_n = 0, edge_read = -1;
+  void frag(int p, int d, int32_t size) { pc[p] = j_add(pc[p], d); is[p] = j_add(is[p], size); }
+  void aligned(int n) { aligned_n += n; }
+  void base(int p, int b, int32_t q, int32_t mq1) { bs[b * W + p] += pack_base(q, mq1); mqq[p] += pack_mq_q(mq1, q); }
+  void count_base() { bcount++; }
+  void bad(int p, int n) { bad_[p] += (uint32_t)n; }
+  void clip(int p) { clp_[p] += 1; }
+  void ins(int p, int32_t q, int32_t mq1, const IndelRec& e) {
+    ins_[p] += (1ull << 32) + (uint64_t)mq1;
+    mqq[p] += (uint64_t)(int64_t)q << 32;
+    recs->push_back(e);
+  }
+  void del(int p, int32_t q, int32_t mq1, const IndelRec& e) {
+    del_[p] += (1ull << 32) + (uint64_t)mq1;
+    mqq[p] += pack_mq_q(mq1, q);
+    recs->push_back(e);
+  }
+  void edge(int64_t r) { if (edge_read < 0 || r < edge_read) edge_read = r; }
+};
+
+struct Handle {
+  std::vector<std::string> changes, bases;
+};
+
+int64_t lower_pos(const int32_t* pos, int64_t n, int64_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) { int64_t m = (lo + hi) / 2; if (pos[m] < x) lo = m + 1; else hi = m; }
+  return lo;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Runs n_jobs regions of one contig with window size W.  pile (PL_NFIELDS x
+// G) and calls (G) are written in job order; stats/errs have n_jobs entries.
+void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const char* name,
+             const pl_read_soa* s, int32_t n_jobs, const int32_t* starts, const int32_t* stops,
+             int32_t W, int32_t* pile_out, pl_call* calls_out, pl_region_stats* stats_out,
+             int32_t* errs) {
+  KConf cf;
+  cf.flank = pc->flank; cf.min_mq = pc->min_mq; cf.min_qual = pc->min_qual;
+  cf.default_qual = pc->default_qual; cf.min_min_depth = pc->min_min_depth;
+  cf.old_indel = pc->old_indel; cf.fixamb = pc->iupac || pc->fix_amb;
+  cf.nonpf = pc->nonpf; cf.duplicates = pc->duplicates; cf.min_depth = pc->min_depth;
+  const int64_t n = s->n_reads;
+  std::vector<int4> span((size_t)n + 1);
+  std::vector<int2> ai((size_t)n + 1);
+  RSDev R;
+  std::memset(&R, 0, sizeof R);
+  R.n = n; R.pos = s->pos; R.tlen = s->tlen; R.flag = s->flag; R.mapq = s->mapq;
+  R.cigar_off = s->cigar_off; R.cigar = s->cigar; R.seq_off = s->seq_off;
+  R.bases2 = s->bases2; R.quals = s->quals; R.n_exotic = s->n_exotic;
+  R.exo_read = s->exo_read; R.exo_off = s->exo_off; R.nib = s->nib;
+  R.span = span.data(); R.ai = ai.data(); R.ref = ref; R.ref_len = ref_len;
+  // ---- read-set prep (k_readset_prep)
+  int32_t max_back = 0, max_lead = 0;
+  std::vector<int64_t> outliers;
+  for (int64_t r = 0; r < n; r++) {
+    PrepOut o = prep_read(R, r, cf);
+    span[r] = o.span;
+    ai[r] = o.ai;
+  }
+  for (int64_t r = 0; r < n; r++) {
+    uint32_t info = (uint32_t)ai[r].y;
+    if (!(info & RI_KEEP)) continue;
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    if (span[r].x <= span[r].y) { lo = span[r].x; hi = span[r].y; }
+    if (span[r].z < span[r].w) { lo = std::min<int64_t>(lo, span[r].z); hi = std::max<int64_t>(hi, (int64_t)span[r].w - 1); }
+    if (lo > hi) continue;
+    int64_t lead = R.pos[r] - lo, back = hi - R.pos[r];
+    if (lead > PL_MAX_LEAD || back > PL_MAX_BACK) { outliers.push_back(r); ai[r].y = (int)(info | RI_OUTLIER); continue; }
+    max_back = std::max<int32_t>(max_back, (int32_t)std::max<int64_t>(back, 0));
+    max_lead = std::max<int32_t>(max_lead, (int32_t)std::max<int64_t>(lead, 0));
+  }
+  // ---- jobs and windows
+  int64_t G = 0;
+  std::vector<JobDev> jobs((size_t)n_jobs);
+  std::vector<JobStat> st((size_t)n_jobs);
+  for (int32_t j = 0; j < n_jobs; j++) {
+    JobDev& J = jobs[j];
+    std::memset(&J, 0, sizeof J);
+    J.rs = 0; J.start = starts[j]; J.stop = stops[j];
+    J.qs = (int32_t)std::max<int64_t>((int64_t)J.start - 10000, 0);
+    J.qe = (int32_t)std::min<int64_t>((int64_t)J.stop + 10000, ref_len);
+    J.size = J.stop - J.start + 1;
+    J.out = G;
+    G += J.size;
+    std::memset(&st[j], 0, sizeof(JobStat));
+  }
+  int32_t* pile = pile_out;
+  pl_call* calls = calls_out;
+  std::vector<IndelRec> recs;
+  std::vector<uint32_t> defer;
+  std::vector<Cand> cands;
+  std::vector<uint64_t> bs(4 * (size_t)W), mqq(W), insA(W), delA(W);
+  std::vector<uint32_t> badA(W), clpA(W);
+  std::vector<int32_t> pcA(W), isA(W);
+  for (int32_t j = 0; j < n_jobs; j++) {
+    const JobDev& J = jobs[j];
+    JobStat& S = st[j];
+    // count (k_jobprep): kept reads overlapping the query
+    for (int64_t r = 0; r < n; r++) {
+      uint32_t info = (uint32_t)ai[r].y;
+      if (!(info & RI_KEEP)) continue;
+      int32_t spanEnd = ai[r].x != 0 ? ai[r].x : R.pos[r];
+      if (!(R.pos[r] <= J.qe && spanEnd >= J.qs)) continue;
+      S.read_count++;
+      S.unknown_ops += (info & RI_UNK_MASK) >> RI_UNK_SHIFT;
+      if ((info & (RI_BAD | RI_EDGE_CLIP)) && (S.err == 0 || r < S.err_read)) {
+        S.err = (info & RI_BAD) ? PL_E_INVAL : PL_E_EDGE;
+        S.err_read = r;
+      }
+    }
+    for (int32_t p0 = 0; p0 < J.size; p0 += W) {
+      const int32_t p1 = std::min<int32_t>(J.size, p0 + W);
+      const int32_t L0 = J.start + p0, L1 = J.start + p1;
+      std::fill(bs.begin(), bs.end(), 0); std::fill(mqq.begin(), mqq.end(), 0);
+      std::fill(insA.begin(), insA.end(), 0); std::fill(delA.begin(), delA.end(), 0);
+      std::fill(badA.begin(), badA.end(), 0); std::fill(clpA.begin(), clpA.end(), 0);
+      std::fill(pcA.begin(), pcA.end(), 0); std::fill(isA.begin(), isA.end(), 0);
+      HostSink sk;
+      sk.W = W; sk.bs = bs.data(); sk.mqq = mqq.data(); sk.ins_ = insA.data(); sk.del_ = delA.data();
+      sk.bad_ = badA.data(); sk.clp_ = clpA.data(); sk.pc = pcA.data(); sk.is = isA.data(); sk.recs = &recs;
+      const int64_t lo = lower_pos(R.pos, n, (int64_t)L0 - max_back);
+      const int64_t hi = lower_pos(R.pos, n, (int64_t)L1 + max_lead);
+      for (int64_t r = lo; r < hi; r++) {
+        if ((uint32_t)ai[r].y & RI_OUTLIER) continue;
+        walk_read(R, J, cf, r, L0, L1, 0, 1, sk);
+      }
+      for (int64_t r : outliers) walk_read(R, J, cf, r, L0, L1, 0, 1, sk);
+      if (sk.edge_read >= 0 && (S.err == 0 || sk.edge_read < S.err_read)) { S.err = PL_E_EDGE; S.err_read = sk.edge_read; }
+      S.base_count += sk.bcount;
+      S.aligned += sk.aligned_n;
+      int32_t apc = 0, ais = 0;
+      for (int32_t p = 0; p < p1 - p0; p++) {
+        apc = j_add(apc, pcA[p]);
+        ais = j_add(ais, isA[p]);
+        uint64_t b4[4] = {bs[0 * W + p], bs[1 * W + p], bs[2 * W + p], bs[3 * W + p]};
+        const int64_t g = J.out + p0 + p;
+        PosOut o;
+        finalize_position(b4, mqq[p], insA[p], delA[p], badA[p], clpA[p], apc, ais, ref[L0 + p - 1], cf, o);
+        if (o.overflow && S.err == 0) { S.err = PL_E_OVERFLOW; S.err_read = -1; }
+        for (int f = 0; f < PL_NFIELDS; f++) pile[(int64_t)f * G + g] = o.f[f];
+        calls[g] = o.call;
+        if (o.defer) defer.push_back((uint32_t)g);
+        else {
+          const int d = call_disp(o.call);
+          if (d == PL_D_CONFIRMED) {
+            int64_t dep = 0;
+            for (int b = 0; b < 4; b++) dep += (uint32_t)o.f[PL_F_COUNT_A + b];
+            dep += o.f[PL_F_DELETIONS];
+            S.hist[dep > 255 ? 255 : dep]++;
+          } else if (d == PL_D_SNP || d == PL_D_AMB) {
+            int64_t dep = 0;
+            for (int b = 0; b < 4; b++) dep += (uint32_t)o.f[PL_F_COUNT_A + b];
+            dep += o.f[PL_F_DELETIONS];
+            cands.push_back({(uint32_t)g, (int32_t)dep, d, 0, -1});
+          }
+        }
+      }
+    }
+  }
+  // ---- sort records by key (stable), indel pass on deferred columns
+  std::vector<uint32_t> rkey(recs.size()), rval(recs.size());
+  for (size_t i = 0; i < recs.size(); i++) rval[i] = (uint32_t)i;
+  std::stable_sort(rval.begin(), rval.end(), [&](uint32_t a, uint32_t b) { return recs[a].key < recs[b].key; });
+  for (size_t i = 0; i < recs.size(); i++) rkey[i] = recs[rval[i]].key;
+  auto job_of = [&](int64_t g) {
+    int32_t j = 0;
+    while (j + 1 < n_jobs && jobs[j + 1].out <= g) j++;
+    return j;
+  };
+  for (uint32_t g : defer) {
+    const int32_t j = job_of(g);
+    const JobDev& J = jobs[j];
+    Col c = load_col(pile, G, g);
+    int32_t ilen;
+    int64_t wrec;
+    pl_call cl = full_call(&R, J, c, ref[J.start + (g - J.out) - 1], recs.data(), rkey.data(), rval.data(),
+                           (int64_t)recs.size(), g, cf, &ilen, &wrec);
+    calls[g] = cl;
+    const int d = call_disp(cl);
+    const int64_t dep = c.depth();
+    if (d == PL_D_CONFIRMED) st[j].hist[dep > 255 ? 255 : dep]++;
+    else if (d != PL_D_NONE) cands.push_back({g, (int32_t)dep, d, ilen, wrec});
+  }
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.gpos < b.gpos; });
+  std::vector<uint32_t> mutated((size_t)G + 1);
+  uint32_t mut_n = 0;
+  int32_t ovf = 0;
+  Handle* H = new Handle();
+  for (int32_t j = 0; j < n_jobs; j++) {
+    const JobDev& J = jobs[j];
+    int64_t a = 0, b = 0;
+    while (a < (int64_t)cands.size() && cands[a].gpos < (uint64_t)J.out) a++;
+    b = a;
+    while (b < (int64_t)cands.size() && cands[b].gpos < (uint64_t)(J.out + J.size)) b++;
+    uint32_t m0 = mut_n;
+    finalize_job(cf, J, st[j], pile, G, calls, cands.data(), a, b, mutated.data(), &mut_n, (uint32_t)mutated.size(), &ovf);
+    for (uint32_t k = m0; k < mut_n; k++) {
+      const uint32_t g = mutated[k];
+      Col c = load_col(pile, G, g);
+      int32_t ilen;
+      int64_t wrec;
+      pl_call cl = full_call(&R, J, c, ref[J.start + (g - J.out) - 1], recs.data(), rkey.data(), rval.data(),
+                             (int64_t)recs.size(), g, cf, &ilen, &wrec);
+      calls[g].base = cl.base; calls[g].alt = cl.alt; calls[g].q = cl.q;
+      calls[g].flags = (uint16_t)((cl.flags & ~PL_CF_DISP_MASK) | (calls[g].flags & PL_CF_DISP_MASK) | PL_CF_DELETED);
+    }
+    // final dispositions and the change list
+    std::vector<plh::Change> ch;
+    for (int32_t p = 0; p < J.size; p++) {
+      const int64_t g = J.out + p;
+      int64_t dep = 0;
+      for (int b2 = 0; b2 < 4; b2++) dep += (uint32_t)pile[(int64_t)(PL_F_COUNT_A + b2) * G + g];
+      dep += pile[(int64_t)PL_F_DELETIONS * G + g];
+      calls[g].flags = plh::final_flags(calls[g].flags, dep, st[j].min_depth, st[j].read_count);
+    }
+    for (int64_t k = a; k < b; k++) {
+      const Cand& c = cands[k];
+      if (!c.kind) continue;
+      plh::Change x;
+      const int32_t p = (int32_t)(c.gpos - J.out);
+      x.loc = J.start + p;
+      x.kind = c.kind;
+      x.ref = to_upper(ref[x.loc - 1]);
+      x.base = calls[c.gpos].base;
+      x.alt = calls[c.gpos].alt;
+      if (c.kind == PL_D_INS) {
+        const IndelRec& e = recs[c.rec];
+        for (int32_t q = 0; q < e.len; q++) x.indel.push_back((char)rec_char(R, e, q));
+      } else if (c.kind == PL_D_DEL) {
+        x.indel.assign((const char*)ref + x.loc - 1, (size_t)c.len);
+      }
+      ch.push_back(x);
+    }
+    std::string nm(name);
+    std::string rb((const char*)ref + J.start - 1, (size_t)J.size);
+    plh::FixResult fr = plh::fix_region(*pc, nm, J.start, rb, ch, nm + "_pilon", 0);
+    H->changes.push_back(fr.changes);
+    H->bases.push_back(fr.bases);
+    pl_region_stats& o = stats_out[j];
+    std::memset(&o, 0, sizeof o);
+    o.read_count = st[j].read_count;
+    o.base_count = st[j].base_count;
+    o.coverage = round_div_l(st[j].base_count, J.size);
+    o.min_depth = st[j].min_depth;
+    o.n_confirmed = st[j].n_confirmed;
+    o.snps = fr.snps; o.amb = fr.amb; o.ins = fr.ins; o.ins_bases = fr.ins_bases;
+    o.dels = fr.dels; o.del_bases = fr.del_bases;
+    o.n_unknown_ops = (int32_t)st[j].unknown_ops;
+    o.aligned_bases = st[j].aligned;
+    errs[j] = st[j].err;
+  }
+  (void)ovf;
+  return H;
+}
+
+int64_t ch_text(void* h, int32_t j, int32_t which, const char** p) {
+  Handle* H = (Handle*)h;
+  const std::string& s = which == 0 ? H->changes[j] : H->bases[j];
+  *p = s.data();
+  return (int64_t)s.size();
+}
+
+void ch_free(void* h) { delete (Handle*)h; }
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// pl_host.h — host-side pieces of the path that act on the sparse change
+// list: final dispositions, identifyAndFixIssues (snps/indels), fixFixList,
+// fixIssues and writeChanges.  Header-only C++17.
+//
+// Reference: broadinstitute/pilon v1.24, src/main/scala/org/broadinstitute/pilon/
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include "pl_common.h"
+
+namespace plh {
+
+// A position's disposition after Pass 1: the window kernel stores the
+// candidate assuming depth >= minDepth and not deleted; these are the two
+// remaining tests of GenomeRegion.scala:255.  No Pass 1 runs when the region
+// has no reads (GenomeRegion.scala:229-231).
+inline uint16_t final_flags(uint16_t flags, int64_t depth, int32_t min_depth, int64_t read_count) {
+  const int cand = (flags & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT;
+  int disp = cand;
+  if (read_count == 0 || (flags & PL_CF_DELETED) || depth < min_depth) disp = PL_D_NONE;
+  return (uint16_t)((flags & ~PL_CF_DISP_MASK) | (disp << PL_CF_DISP_SHIFT));
+}
+
+struct Change {
+  int32_t loc;        // 1-based locus
+  int kind;           // PL_D_SNP / INS / DEL / AMB
+  uint8_t ref;        // refBase (upper case)
+  uint8_t base, alt;  // BaseCall.base / altBase codes (0..3, 4 = N)
+  std::string indel;  // BaseCall.insertion / deletion
+};
+
+struct Fix { int32_t loc; std::string was, patch; };
+
+inline char iupac(int b1, int b2) {               // Bases.toIUPAC (Bases.scala:85-88)
+  static const char T[16] = {0, 'A', 'C', 'M', 'G', 'R', 'S', 'V', 'T', 'W', 'Y', 'H', 'K', 'D', 'B', 'N'};
+  return T[(1 << b1) | (1 << b2)];
+}
+
+// GenomeRegion.fixFixList (GenomeRegion.scala:557-593): stable sort by locus,
+// of overlapping neighbours keep the one with the larger was+patch, ties to
+// the earlier.
+inline std::vector<Fix> fix_fix_list(std::vector<Fix> in) {
+  std::stable_sort(in.begin(), in.end(), [](const Fix& a, const Fix& b) { return a.loc < b.loc; });
+  std::vector<Fix> out;
+  if (in.empty()) return out;
+  Fix cur = in[0];
+  for (size_t i = 1; i < in.size(); i++) {
+    const Fix& nx = in[i];
+    const int64_t e1 = (int64_t)cur.loc + std::max<int64_t>((int64_t)cur.was.size() - 1, 0);
+    const int64_t e2 = (int64_t)nx.loc + std::max<int64_t>((int64_t)nx.was.size() - 1, 0);
+    if ((int64_t)nx.loc <= e1 && e2 >= (int64_t)cur.loc) {
+      if (cur.was.size() + cur.patch.size() < nx.was.size() + nx.patch.size()) cur = nx;
+    } else {
+      out.push_back(cur);
+      cur = nx;
+    }
+  }
+  out.push_back(cur);
+  return out;
+}
+
+// GenomeRegion.fixIssues (GenomeRegion.scala:595-621): apply right to left.
+inline void apply_fixes(std::string& bases, int32_t region_start, const std::vector<Fix>& list) {
+  std::vector<Fix> kept = fix_fix_list(list);
+  for (size_t k = kept.size(); k-- > 0;) {
+    const Fix& f = kept[k];
+    const size_t at = (size_t)(f.loc - region_start);
+    if (f.was.size() == f.patch.size()) {
+      for (size_t i = 0; i < f.was.size(); i++) bases[at + i] = f.patch[i];
+    } else {
+      bases = bases.substr(0, at) + f.patch + bases.substr(at + f.was.size());
+    }
+  }
+}
+
+inline std::string region_string(const std::string& name, int64_t s, int64_t e) {  // Region.regionString
+  char buf[64];
+  if (e + 1 - s < 2) snprintf(buf, sizeof buf, ":%lld", (long long)s);
+  else snprintf(buf, sizeof buf, ":%lld-%lld", (long long)s, (long long)e);
+  return name + buf;
+}
+
+struct FixResult { std::string changes, bases; int snps = 0, amb = 0, ins = 0, ins_bases = 0, dels = 0, del_bases = 0; };
+
+// identifyAndFixIssues for the snps/indels fixes (GenomeRegion.scala:307-380,
+// 413) and writeChanges (:645-657) for one region.  changes must be sorted by
+// locus; region_bases is contigBases[start-1 .. stop-1].
+inline FixResult fix_region(const pl_config& cf, const std::string& name, int32_t start,
+                            const std::string& region_bases, const std::vector<Change>& changes,
+                            const std::string& new_name, int64_t offset) {
+  FixResult R;
+  std::vector<Fix> snp, small;   // in the order the Scala code prepends them
+  for (const Change& c : changes) {
+    const std::string r(1, (char)c.ref);
+    const std::string b(1, c.base == 4 ? 'N' : "ACGT"[c.base]);
+    switch (c.kind) {
+      case PL_D_SNP:
+        if (cf.fix_snps) snp.push_back({c.loc, r, b});
+        R.snps++;
+        break;
+      case PL_D_AMB:
+        if (cf.fix_snps && !cf.longread) {
+          if (cf.iupac) small.push_back({c.loc, r, std::string(1, iupac(c.base, c.alt))});
+          else snp.push_back({c.loc, r, b});
+          R.amb++;
+        }
+        break;
+      case PL_D_INS:
+        if (cf.fix_indels) small.push_back({c.loc, "", c.indel});
+        R.ins++;
+        R.ins_bases += (int)c.indel.size();
+        break;
+      case PL_D_DEL:
+        if (cf.fix_indels) small.push_back({c.loc, c.indel, ""});
+        R.dels++;
+        R.del_bases += (int)c.indel.size();
+        break;
+    }
+  }
+  // List order: the last prepended first
+  std::reverse(snp.begin(), snp.end());
+  std::reverse(small.begin(), small.end());
+  R.bases = region_bases;
+  apply_fixes(R.bases, start, snp);
+  apply_fixes(R.bases, start, small);
+  std::vector<Fix> all = snp;
+  all.insert(all.end(), small.begin(), small.end());
+  int64_t delta = 0;
+  for (const Fix& f : fix_fix_list(all)) {
+    const int64_t newLoc = f.loc + delta;
+    R.changes += region_string(name, f.loc, (int64_t)f.loc + (int64_t)f.was.size() - 1);
+    R.changes += ' ';
+    R.changes += region_string(new_name, newLoc + offset, newLoc + offset + (int64_t)f.patch.size() - 1);
+    R.changes += ' ';
+    R.changes += f.was.empty() ? std::string(".") : f.was;
+    R.changes += ' ';
+    R.changes += f.patch.empty() ? std::string(".") : f.patch;
+    R.changes += '\n';
+    delta += (int64_t)f.patch.size() - (int64_t)f.was.size();
+  }
+  return R;
+}
+
+}  // namespace plh

@@ -1,0 +1,194 @@
+"""Seeded synthetic workloads for the pileup path (drafts, truths, aligned reads).
+
+Wraps libpilon_synth.so (pilon_amd/synth/synth.cpp).  The presets follow the
+five benchmark configurations: c1 (100 kb, 40x, CPU-sized), c2 (5 Mb at 65% GC
+with MAPQ-0 repeats, 100x), c3 (3,000 log-normal contigs, ~60 Mb, 60x), c4 (50
+x 4 Mb resident for serving, 50x) and c5 (20 x 25 Mb, 2x250 bp, 80x, diploid,
+breakpoints and 2,000x hotspots).  ``scale`` shrinks lengths for tests.
+"""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+from .abi import PlReadSoa
+from .reads import ReadBatch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libpilon_synth.so")
+SRC = os.path.join(HERE, "synth", "synth.cpp")
+
+
+class SynthParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_contigs", C.c_int32), ("contig_lens", C.POINTER(C.c_int64)),
+                ("gc", C.c_double), ("read_len", C.c_int32), ("coverage", C.c_double),
+                ("insert_mean", C.c_double), ("insert_sd", C.c_double), ("sub_rate", C.c_double),
+                ("err_indel_rate", C.c_double), ("err_indel_max", C.c_int32), ("clip_frac", C.c_double),
+                ("clip_min", C.c_int32), ("clip_max", C.c_int32), ("improper_frac", C.c_double),
+                ("n_snv", C.c_int32), ("n_indel", C.c_int32), ("snv_rate", C.c_double),
+                ("indel_rate", C.c_double), ("indel_max", C.c_int32), ("repeat_copies", C.c_int32),
+                ("repeat_len", C.c_int32), ("qual_lo", C.c_int32), ("qual_hi", C.c_int32),
+                ("qual_decay", C.c_int32), ("mapq", C.c_int32), ("n_rate", C.c_double),
+                ("het_rate", C.c_double), ("het_a", C.c_double), ("het_b", C.c_double),
+                ("n_hotspots", C.c_int32), ("hotspot_len", C.c_int32), ("hotspot_cov", C.c_double),
+                ("n_breakpoints", C.c_int32), ("bp_clip_frac", C.c_double), ("bp_clip_min", C.c_int32),
+                ("bp_clip_max", C.c_int32), ("threads", C.c_int32)]
+
+
+def build():
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall",
+                           "-o", LIB, SRC])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB)
+        L.synth_default_params.argtypes = [C.POINTER(SynthParams)]
+        L.synth_generate.restype = C.c_void_p
+        L.synth_generate.argtypes = [C.POINTER(SynthParams)]
+        L.synth_free.argtypes = [C.c_void_p]
+        L.synth_contig.restype = C.c_int64
+        L.synth_contig.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_int64)]
+        L.synth_variants.restype = C.c_int64
+        L.synth_variants.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.synth_reads.argtypes = [C.c_void_p, C.c_int32, C.POINTER(PlReadSoa)]
+        _lib = L
+    return _lib
+
+
+def lognormal_lengths(n, median, sigma, lo, hi, total, seed):
+    """Contig lengths LogNormal(ln median, sigma) clipped to [lo, hi], scaled to ~total."""
+    rng = np.random.default_rng(seed)
+    x = np.exp(rng.normal(math.log(median), sigma, n))
+    x = np.clip(x, lo, hi)
+    x = np.clip(x * (total / x.sum()), lo, hi)
+    return np.round(x).astype(np.int64)
+
+
+def preset(name, scale=1.0, seed=None):
+    """Parameter dict of a benchmark configuration; scale < 1 shrinks the genome."""
+    if name == "c1":
+        p = dict(seed=1001, lens=[int(100_000 * scale)], gc=0.5, read_len=150, coverage=40,
+                 insert_mean=400, insert_sd=40, sub_rate=0.003, n_snv=max(1, int(200 * scale)),
+                 n_indel=max(1, int(20 * scale)), indel_max=10)
+    elif name == "c2":
+        p = dict(seed=1002, lens=[int(5_000_000 * scale)], gc=0.65, read_len=150, coverage=100,
+                 insert_mean=400, insert_sd=40, sub_rate=0.005, err_indel_rate=0.0002,
+                 err_indel_max=3, clip_frac=1 / 200, clip_min=10, clip_max=60, improper_frac=0.01,
+                 snv_rate=0.0001, indel_rate=0.00002, indel_max=10, repeat_copies=10,
+                 repeat_len=max(200, int(5000 * min(1.0, scale * 10))))
+    elif name == "c3":
+        n = max(3, int(3000 * scale))
+        lens = lognormal_lengths(n, 8000, 1.35, 1000, 2_000_000, 60_000_000 * scale, 1003)
+        p = dict(seed=1003, lens=list(lens), gc=0.5, read_len=150, coverage=60, insert_mean=350,
+                 insert_sd=35, sub_rate=0.005, err_indel_rate=0.0002, err_indel_max=3,
+                 clip_frac=1 / 200, clip_min=10, clip_max=60, improper_frac=0.01,
+                 snv_rate=0.0001, indel_rate=0.00002, indel_max=10)
+    elif name == "c4":
+        n = max(1, int(50 * min(1.0, scale * 4)))
+        p = dict(seed=1004, lens=[int(4_000_000 * scale)] * n, gc=0.42, read_len=150, coverage=50,
+                 insert_mean=400, insert_sd=40, sub_rate=0.005, err_indel_rate=0.0002,
+                 err_indel_max=3, clip_frac=1 / 200, clip_min=10, clip_max=60, improper_frac=0.01,
+                 snv_rate=0.0001, indel_rate=0.00002, indel_max=10)
+    elif name == "c5":
+        n = max(1, int(20 * min(1.0, scale * 4)))
+        p = dict(seed=1005, lens=[int(25_000_000 * scale)] * n, gc=0.5, read_len=250, coverage=80,
+                 insert_mean=500, insert_sd=50, sub_rate=0.005, err_indel_rate=0.0002,
+                 err_indel_max=3, improper_frac=0.01, snv_rate=0.0001, indel_rate=0.00002,
+                 indel_max=10, het_rate=0.001, n_hotspots=max(1, int(round(50 / n))),
+                 hotspot_len=int(10_000 * min(1.0, scale * 10)), hotspot_cov=2000,
+                 n_breakpoints=max(1, int(round(1000 / n))), bp_clip_frac=0.05 * 20,
+                 bp_clip_min=30, bp_clip_max=200)
+    else:
+        raise ValueError(name)
+    if seed is not None:
+        p["seed"] = seed
+    return p
+
+
+class SynthData:
+    """Generated contigs; keeps the C++ buffers alive."""
+
+    def __init__(self, params):
+        L = lib()
+        sp = SynthParams()
+        L.synth_default_params(C.byref(sp))
+        lens = [int(x) for x in params["lens"]]
+        self._lens = (C.c_int64 * len(lens))(*lens)
+        sp.n_contigs = len(lens)
+        sp.contig_lens = self._lens
+        for k, v in params.items():
+            if k == "lens":
+                continue
+            setattr(sp, k, v)
+        self.params = params
+        self.h = L.synth_generate(C.byref(sp))
+        self.n = len(lens)
+        self.names = [f"ctg{i:04d}" for i in range(self.n)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().synth_free(self.h)
+            self.h = None
+
+    def draft(self, i):
+        d, t, tl = C.c_void_p(), C.c_void_p(), C.c_int64()
+        n = lib().synth_contig(self.h, i, C.byref(d), C.byref(t), C.byref(tl))
+        return C.string_at(d, n)
+
+    def truth(self, i):
+        d, t, tl = C.c_void_p(), C.c_void_p(), C.c_int64()
+        lib().synth_contig(self.h, i, C.byref(d), C.byref(t), C.byref(tl))
+        return C.string_at(t, tl.value)
+
+    def variants(self, i):
+        n = lib().synth_variants(self.h, i, None, None, None, 0)
+        pos = np.zeros(n, np.int64)
+        kind = np.zeros(n, np.int32)
+        ln = np.zeros(n, np.int32)
+        lib().synth_variants(self.h, i, pos.ctypes.data, kind.ctypes.data, ln.ctypes.data, n)
+        return pos, kind, ln
+
+    def soa(self, i):
+        """pl_read_soa pointing straight into the generator's buffers (no copy)."""
+        s = PlReadSoa()
+        lib().synth_reads(self.h, i, C.byref(s))
+        return s
+
+    def n_reads(self, i):
+        return self.soa(i).n_reads
+
+    def batch(self, i):
+        """A numpy-owned copy of contig i's reads."""
+        s = self.soa(i)
+        n = s.n_reads
+
+        def arr(ptr, count, ctype, dtype):
+            if count == 0 or not ptr:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).astype(dtype, copy=True)
+
+        co = arr(s.cigar_off, n + 1, C.c_int64, np.int64)
+        so = arr(s.seq_off, n + 1, C.c_int64, np.int64)
+        nb = int(so[-1])
+        nib_len = 1
+        ex_r = arr(s.exo_read, s.n_exotic, C.c_int64, np.int64)
+        ex_o = arr(s.exo_off, s.n_exotic, C.c_int64, np.int64)
+        if s.n_exotic:
+            last = int(ex_r[-1])
+            nib_len = (int(ex_o[-1]) + int(so[last + 1] - so[last]) + 1) // 2 + 1
+        return ReadBatch(arr(s.pos, n, C.c_int32, np.int32), arr(s.tlen, n, C.c_int32, np.int32),
+                         arr(s.flag, n, C.c_uint16, np.uint16), arr(s.mapq, n, C.c_uint8, np.uint8),
+                         co, arr(s.cigar, int(co[-1]), C.c_uint32, np.uint32), so,
+                         arr(s.bases2, (nb + 3) // 4 + 8, C.c_uint8, np.uint8),
+                         arr(s.quals, nb + 8, C.c_int8, np.int8), ex_r, ex_o,
+                         arr(s.nib, nib_len, C.c_uint8, np.uint8))
