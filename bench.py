@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the pileup path on one MI355X.
+
+A step is one pass of the batched form over the whole workload: every region
+of the configuration piled (window kernel), its indel evidence sorted and
+called, Pass 1 finalised — one pl_batch_run with the reads and contigs
+already resident in HBM.  The default workload is the many-contig batching
+configuration (3,000 log-normal contigs, ~60 Mb, 2x150 bp at 60x).
+
+Prints ONE JSON line.  value = aligned bases piled per second (M/=/X read
+bases whose locus lies in the region, counted once per region) over the K
+timed steps.  Beside it: the plain per-region launch on the same workload,
+the window kernel's HBM roofline fraction (algorithmic bytes / kernel time,
+HIP events on the library's stream), and the CPU oracle timed on a bounded
+sample on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "aligned bases piled/sec on 1 MI355X (batched/served vs plain); p99 latency"
+WORKLOADS = {
+    "c1": "one 100 kb contig, 2x150 bp at 40x, 200 SNVs + 20 indels",
+    "c2": "one 5 Mb contig at 65% GC with 10 MAPQ-0 repeat copies, 2x150 bp at 100x",
+    "c3": "3,000 contigs log-normal (median 8 kb, ~60 Mb), 2x150 bp at 60x, one persistent launch",
+    "c4": "50 x 4 Mb contigs resident, 2x150 bp at 50x",
+    "c5": "20 x 25 Mb contigs (60 chunk regions), 2x250 bp at 80x, diploid, hotspots",
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def regions_for(name, data, rids):
+    """The regions Pilon would process: each contig split in chunks of at most
+    10 Mb (GenomeFile.contigRegions, GenomeFile.scala:67-74)."""
+    out = []
+    for ci in range(data.n):
+        L = len(data.draft(ci))
+        n_chunks = (L + 10_000_000 - 1) // 10_000_000
+        chunk = (L + n_chunks - 1) // n_chunks
+        for s in range(1, L + 1, chunk):
+            out.append((ci, s, min(L, s + chunk - 1), rids[ci]))
+    return out
+
+
+def read_bytes(soa):
+    """Per-read algorithmic bytes: 16 B header + 4 B per CIGAR op + 2-bit bases + 1 B quals."""
+    import ctypes as C
+    n = soa.n_reads
+    co = np.ctypeslib.as_array(C.cast(soa.cigar_off, C.POINTER(C.c_int64)), shape=(n + 1,))
+    so = np.ctypeslib.as_array(C.cast(soa.seq_off, C.POINTER(C.c_int64)), shape=(n + 1,))
+    L = np.diff(so)
+    return int(16 * n + 4 * int(co[-1]) + int(((L + 3) // 4).sum()) + int(L.sum()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--window", type=int, default=1024)
+    ap.add_argument("--persistent", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-region comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU oracle sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path would shard "
+              "by region with no communication — see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from pilon_amd.abi import default_config
+    from pilon_amd.engine import Context
+    from pilon_amd.synth import SynthData, preset
+
+    cfg = default_config()
+    t0 = time.time()
+    params = preset(args.config, scale=args.scale)
+    data = SynthData(params)
+    log(f"bench: generated {args.config} x{args.scale}: {data.n} contigs in {time.time() - t0:.1f}s")
+    ctx = Context(cfg, device=0)
+    rids = []
+    alg_read_bytes = 0
+    n_reads = 0
+    t0 = time.time()
+    for ci in range(data.n):
+        ctx.ref_upload(ci, data.names[ci], data.draft(ci))
+        soa = data.soa(ci)
+        rids.append(ctx.readset_upload(ci, soa))
+        alg_read_bytes += read_bytes(soa)
+        n_reads += soa.n_reads
+    log(f"bench: uploaded {n_reads} reads in {time.time() - t0:.1f}s")
+    regions = regions_for(args.config, data, rids)
+    descs = ctx.batch_descs(regions)
+    G = sum(e - s + 1 for _, s, e, _ in regions)
+
+    # one checked pass for the unit (aligned bases) and the sizes
+    _, _, stats = ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False,
+                                want_stats=True, descs=descs)
+    aligned = sum(s["aligned_bases"] for s in stats)
+    info = ctx.batch_info()
+    log(f"bench: {len(regions)} regions, G={G}, aligned={aligned}, records={info['n_records']}, "
+        f"windows={info['n_windows']}, pipeline {info['pipeline_ms']:.3f} ms, window kernel {info['window_ms']:.3f} ms")
+
+    for _ in range(args.warmup):
+        ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False, want_stats=False,
+                      descs=descs)
+    pipe, win = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False, want_stats=False,
+                      descs=descs)
+        inf = ctx.batch_info()
+        pipe.append(inf["pipeline_ms"])
+        win.append(inf["window_ms"])
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    value = aligned / (ms_per_step / 1000.0)
+    pipe_ms = float(np.mean(pipe))
+    win_ms = float(np.mean(win))
+    b_alg = alg_read_bytes + G * 80.25 + 32 * info["n_records"]
+    achieved = b_alg / (win_ms / 1000.0) / 1e9
+    log(f"bench: {ms_per_step:.3f} ms/step wall, device pipeline {pipe_ms:.3f} ms, window kernel {win_ms:.3f} ms, "
+        f"{value / 1e9:.2f} G bases/s, window kernel {achieved:.0f} GB/s algorithmic")
+
+    extra = {}
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        dev = 0.0
+        for r in regions:
+            ctx.batch_run([r], window=args.window, persistent=0, copy_out=False, want_stats=False)
+            dev += ctx.batch_info()["pipeline_ms"]
+        plain_s = time.perf_counter() - t0
+        extra["plain_per_region"] = {"wall_ms": plain_s * 1000.0, "device_ms": dev,
+                                     "bases_per_s": aligned / plain_s,
+                                     "batched_vs_plain": (aligned / (ms_per_step / 1000.0)) / (aligned / plain_s)}
+        log(f"bench: plain per-region launches: {plain_s * 1000:.1f} ms wall ({dev:.1f} ms device) -> "
+            f"batched is {extra['plain_per_region']['batched_vs_plain']:.2f}x")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle import oracle as O
+        done_bases, spent, nreg = 0, 0.0, 0
+        for (ci, s, e, _), st in zip(regions, stats):
+            b = data.batch(ci)
+            tc = time.perf_counter()
+            r = O.run_region(cfg, data.draft(ci), s, e, [b], name=data.names[ci])
+            spent += time.perf_counter() - tc
+            assert r["status"] == 0
+            done_bases += r["stats"]["aligned_bases"]
+            nreg += 1
+            if spent >= args.cpu_seconds:
+                break
+        cpu = {"value": done_bases / spent, "unit": "bases/s", "cores": 1, "kind": "port",
+               "sample": f"oracle (C restatement of Pilon's path, 1 thread) on the first {nreg} of "
+                         f"{len(regions)} regions, {done_bases} aligned bases in {spent:.1f}s"}
+        log(f"bench: CPU oracle {cpu['value'] / 1e6:.1f} M bases/s on {nreg} regions")
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "bases/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "scale": args.scale,
+                   "regions": len(regions), "positions": G, "reads": n_reads, "aligned_bases": aligned,
+                   "window": args.window, "persistent": args.persistent},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_window",
+                     "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms},
+        "device_pipeline_ms": pipe_ms,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -215,6 +215,22 @@ pl_status pl_batch_run(pl_ctx* ctx, const pl_region_desc* regions, int32_t n,
  * kernel alone. */
 pl_status pl_last_timing(pl_ctx* ctx, double* pipeline_ms, double* window_ms);
 
+/* Sizes and HIP-event times of the last batch (pl_batch_run or
+ * pl_region_finish). */
+typedef struct pl_batch_info {
+  int64_t positions;       /* G, positions piled                          */
+  int32_t n_regions;
+  int32_t n_windows;       /* workgroup windows of the window kernel      */
+  int64_t n_records;       /* insertion/deletion evidence records         */
+  int64_t n_candidates;    /* SNP/AMB/INS/DEL candidates before Pass 1    */
+  int64_t n_deferred;      /* columns whose BaseCall needed indel strings */
+  int32_t window;          /* window size used                            */
+  int32_t persistent;
+  double  pipeline_ms;     /* whole device pipeline                       */
+  double  window_ms;       /* window kernel alone                         */
+} pl_batch_info;
+pl_status pl_last_batch_info(pl_ctx* ctx, pl_batch_info* info);
+
 #ifdef __cplusplus
 }
 #endif

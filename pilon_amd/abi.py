@@ -90,3 +90,13 @@ class PlBatchOpts(C.Structure):
 def call_dtype():
     import numpy as np
     return np.dtype([("base", np.uint8), ("alt", np.uint8), ("flags", np.uint16), ("q", np.int32)])
+
+
+class PlBatchInfo(C.Structure):
+    _fields_ = [("positions", C.c_int64), ("n_regions", C.c_int32), ("n_windows", C.c_int32),
+                ("n_records", C.c_int64), ("n_candidates", C.c_int64), ("n_deferred", C.c_int64),
+                ("window", C.c_int32), ("persistent", C.c_int32), ("pipeline_ms", C.c_double),
+                ("window_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}

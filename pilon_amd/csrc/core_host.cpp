@@ -111,6 +111,8 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
     J.out = G;
     G += J.size;
     std::memset(&st[j], 0, sizeof(JobStat));
+    st[j].err_read_edge = INT64_MAX;
+    st[j].err_read_inval = INT64_MAX;
   }
   int32_t* pile = pile_out;
   pl_call* calls = calls_out;
@@ -131,10 +133,8 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
       if (!(R.pos[r] <= J.qe && spanEnd >= J.qs)) continue;
       S.read_count++;
       S.unknown_ops += (info & RI_UNK_MASK) >> RI_UNK_SHIFT;
-      if ((info & (RI_BAD | RI_EDGE_CLIP)) && (S.err == 0 || r < S.err_read)) {
-        S.err = (info & RI_BAD) ? PL_E_INVAL : PL_E_EDGE;
-        S.err_read = r;
-      }
+      if ((info & RI_BAD) && r < S.err_read_inval) S.err_read_inval = r;
+      if ((info & RI_EDGE_CLIP) && r < S.err_read_edge) S.err_read_edge = r;
     }
     for (int32_t p0 = 0; p0 < J.size; p0 += W) {
       const int32_t p1 = std::min<int32_t>(J.size, p0 + W);
@@ -153,7 +153,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
         walk_read(R, J, cf, r, L0, L1, 0, 1, sk);
       }
       for (int64_t r : outliers) walk_read(R, J, cf, r, L0, L1, 0, 1, sk);
-      if (sk.edge_read >= 0 && (S.err == 0 || sk.edge_read < S.err_read)) { S.err = PL_E_EDGE; S.err_read = sk.edge_read; }
+      if (sk.edge_read >= 0 && sk.edge_read < S.err_read_edge) S.err_read_edge = sk.edge_read;
       S.base_count += sk.bcount;
       S.aligned += sk.aligned_n;
       int32_t apc = 0, ais = 0;
@@ -164,7 +164,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
         const int64_t g = J.out + p0 + p;
         PosOut o;
         finalize_position(b4, mqq[p], insA[p], delA[p], badA[p], clpA[p], apc, ais, ref[L0 + p - 1], cf, o);
-        if (o.overflow && S.err == 0) { S.err = PL_E_OVERFLOW; S.err_read = -1; }
+        if (o.overflow) S.overflow = 1;
         for (int f = 0; f < PL_NFIELDS; f++) pile[(int64_t)f * G + g] = o.f[f];
         calls[g] = o.call;
         if (o.defer) defer.push_back((uint32_t)g);
@@ -275,7 +275,8 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
     o.dels = fr.dels; o.del_bases = fr.del_bases;
     o.n_unknown_ops = (int32_t)st[j].unknown_ops;
     o.aligned_bases = st[j].aligned;
-    errs[j] = st[j].err;
+    errs[j] = st[j].err_read_inval < st[j].err_read_edge ? PL_E_INVAL
+              : st[j].err_read_edge != INT64_MAX ? PL_E_EDGE : st[j].overflow ? PL_E_OVERFLOW : PL_OK;
   }
   (void)ovf;
   return H;

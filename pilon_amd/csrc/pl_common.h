@@ -71,14 +71,14 @@ struct JobStat {             // device-written per job
   int64_t base_count;
   int64_t aligned;
   int64_t unknown_ops;
-  int32_t err;               // pl_status
+  int64_t err_read_edge;     // lowest read on which Pilon throws (INT64_MAX: none)
+  int64_t err_read_inval;    // lowest malformed read (INT64_MAX: none)
+  int32_t overflow;          // a column beyond PL_DEPTH_LIMIT
   int32_t min_depth;
-  int64_t err_read;          // lowest offending read index
   int32_t n_confirmed;
-  int32_t snps, amb, ins, ins_bases, dels, del_bases;
-  int32_t n_changes;
   int32_t cand_lo, cand_hi;  // range in the sorted candidate list
-  int32_t hist[256];         // confirmed candidates by depth (255 = 255+)
+  int32_t pad;
+  uint32_t hist[256];        // confirmed candidates by depth (255 = 255+)
 };
 
 // one piece of insertion/deletion evidence (PileUp.addInsertion/addDeletion)

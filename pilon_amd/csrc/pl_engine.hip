@@ -1,0 +1,865 @@
+// pl_engine.hip — host side of libpilonhip.so: device buffers, the batch
+// pipeline, exports, and the C ABI of include/pilon_hip.h.
+//
+// Layout in HBM (per batch of regions, G positions in total):
+//   pile   int32 [PL_NFIELDS][G]   field-major PileUp counters
+//   calls  pl_call [G]             BaseCall + disposition, 8 B
+//   rec    IndelRec [cap]          insertion/deletion evidence, 32 B each
+//   cand   Cand [cap]              change candidates (sparse)
+// Resident per contig: the raw reference bytes; per read set: the packed
+// reads plus 24 B/read of derived facts (span, alignment end, info word) and
+// a 4 B/read prefix count of kept reads.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pl_kernels.hip"
+#include "pl_host.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+pl_status fail(pl_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return s;
+}
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) return fail(PL_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+struct DBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (bytes <= cap) return hipSuccess;
+    if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; }
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+struct Contig {
+  std::string name;
+  std::vector<uint8_t> host;
+  uint8_t* dev = nullptr;
+  int64_t len = 0;
+};
+
+struct ReadSet {
+  int32_t contig = -1;
+  int64_t n = 0;
+  RSDev d;
+  std::vector<void*> mem;
+  uint32_t* kp = nullptr;
+  int32_t scal[8] = {0};
+  int64_t bytes = 0;
+  ~ReadSet() { for (void* p : mem) (void)hipFree(p); }
+};
+
+// host-owned copy of a pl_read_soa (plain region form)
+struct HostReads {
+  std::vector<int32_t> pos, tlen;
+  std::vector<uint16_t> flag;
+  std::vector<uint8_t> mapq;
+  std::vector<int64_t> cigar_off{0};
+  std::vector<uint32_t> cigar;
+  std::vector<int64_t> seq_off{0};
+  std::vector<uint8_t> bases2;  // 2-bit codes, one per base here (packed on upload)
+  std::vector<int8_t> quals;
+  std::vector<int64_t> exo_read, exo_off;
+  std::vector<uint8_t> nibc;    // one nibble code per base of exotic reads
+};
+
+}  // namespace
+
+struct pl_ctx {
+  pl_config pc;
+  KConf cf;
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4];
+  std::map<int32_t, Contig> contigs;
+  std::vector<std::unique_ptr<ReadSet>> rsets;
+  bool rs_dirty = true;
+  // batch state
+  DBuf rsarr, kparr, jobs, wins, worder, costk, costv, costtk, costtv, pile, calls, st, rec, rkey, rval, rtk, rtv,
+      cand, cands, ckey, cval, ctk, ctv, defer, mutated, counters, hist, tiles, strpool, stridx, stroff;
+  uint32_t rec_cap = 0, cand_cap = 0, defer_cap = 0, mut_cap = 0;
+  std::vector<JobDev> hjobs;
+  std::vector<WinDev> hwins;
+  std::vector<JobStat> hst;
+  std::vector<Cand> hcand;
+  int64_t G = 0;
+  int32_t W = 1024;
+  int32_t n_wins = 0;
+  bool exported_flags = false;
+  double last_ms = 0, last_win_ms = 0;
+  uint32_t last_counts[8] = {0};
+  int32_t last_persistent = 0;
+  EngineArgs args;
+  uint32_t* rkey_sorted = nullptr;
+  uint32_t* rval_sorted = nullptr;
+};
+
+struct pl_region {
+  pl_ctx* ctx;
+  int32_t contig;
+  int32_t start, stop;
+  int32_t readset = -1;          // resident set in use
+  std::vector<HostReads> batches; // plain add_reads copies
+  bool finished = false;
+  pl_region_stats stats;
+  std::vector<plh::Change> changes;
+};
+
+namespace {
+
+pl_status upload(void** dst, const void* src, size_t bytes, size_t pad, std::vector<void*>& mem, hipStream_t s) {
+  HIPCHK(hipMalloc(dst, bytes + pad + 16));
+  mem.push_back(*dst);
+  if (bytes) HIPCHK(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s));
+  if (pad + 16) HIPCHK(hipMemsetAsync((uint8_t*)*dst + bytes, 0, pad + 16, s));
+  return PL_OK;
+}
+
+pl_status scan_u32(pl_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
+  const int64_t tiles = std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+  HIPCHK(c->tiles.ensure(sizeof(uint32_t) * (size_t)tiles));
+  k_scan_tiles<<<dim3((unsigned)tiles), dim3(256), 0, c->stream>>>(in, n, c->tiles.as<uint32_t>());
+  k_scan_top<<<dim3(1), dim3(1024), 0, c->stream>>>(c->tiles.as<uint32_t>(), tiles);
+  k_scan_apply<<<dim3((unsigned)tiles), dim3(256), 0, c->stream>>>(in, n, c->tiles.as<uint32_t>(), out);
+  HIPCHK(hipGetLastError());
+  return PL_OK;
+}
+
+// LSD radix sort of (key, val) pairs whose count is in device memory;
+// returns in *rk/*rv the buffers holding the sorted result.
+pl_status radix_sort(pl_ctx* c, uint32_t* k, uint32_t* v, uint32_t* tk, uint32_t* tv, const uint32_t* n_dev,
+                     uint32_t cap, int bits, uint32_t** rk, uint32_t** rv) {
+  const int32_t nb = (int32_t)std::max<int64_t>(1, ((int64_t)cap + RS_TILE - 1) / RS_TILE);
+  HIPCHK(c->hist.ensure(sizeof(uint32_t) * ((size_t)256 * nb + 1)));
+  uint32_t *ik = k, *iv = v, *ok = tk, *ov = tv;
+  const int passes = std::max(1, (bits + 7) / 8);
+  for (int p = 0; p < passes; p++) {
+    k_rs_hist<<<dim3(nb), dim3(256), 0, c->stream>>>(ik, n_dev, cap, 8 * p, c->hist.as<uint32_t>(), nb);
+    pl_status s = scan_u32(c, c->hist.as<uint32_t>(), c->hist.as<uint32_t>(), (int64_t)256 * nb);
+    if (s) return s;
+    k_rs_scatter<<<dim3(nb), dim3(256), 0, c->stream>>>(ik, iv, ok, ov, n_dev, cap, 8 * p, c->hist.as<uint32_t>(), nb);
+    HIPCHK(hipGetLastError());
+    std::swap(ik, ok);
+    std::swap(iv, ov);
+  }
+  *rk = ik;
+  *rv = iv;
+  return PL_OK;
+}
+
+int bits_for(uint64_t maxkey) {
+  int b = 0;
+  while (b < 32 && (maxkey >> b)) b++;
+  return std::max(b, 1);
+}
+
+void to_kconf(const pl_config& p, KConf& k) {
+  k.flank = p.flank; k.min_mq = p.min_mq; k.min_qual = p.min_qual; k.default_qual = p.default_qual;
+  k.min_min_depth = p.min_min_depth; k.old_indel = p.old_indel; k.fixamb = p.iupac || p.fix_amb;
+  k.nonpf = p.nonpf; k.duplicates = p.duplicates; k.min_depth = p.min_depth;
+}
+
+// nibble stream length of a pl_read_soa
+int64_t nib_bytes(const pl_read_soa* s) {
+  int64_t mx = 0;
+  for (int64_t k = 0; k < s->n_exotic; k++) {
+    const int64_t r = s->exo_read[k];
+    mx = std::max<int64_t>(mx, s->exo_off[k] + (s->seq_off[r + 1] - s->seq_off[r]));
+  }
+  return (mx + 1) / 2 + 1;
+}
+
+pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int32_t* out_id) {
+  auto it = c->contigs.find(contig_id);
+  if (it == c->contigs.end()) return fail(PL_E_INVAL, "contig %d not uploaded", contig_id);
+  const int64_t n = s->n_reads;
+  if (n < 0) return fail(PL_E_INVAL, "negative read count");
+  for (int64_t i = 1; i < n; i++)
+    if (s->pos[i] < s->pos[i - 1]) return fail(PL_E_INVAL, "reads are not coordinate-sorted at read %lld", (long long)i);
+  for (int64_t k = 1; k < s->n_exotic; k++)
+    if (s->exo_read[k] <= s->exo_read[k - 1]) return fail(PL_E_INVAL, "exo_read must be ascending");
+  std::unique_ptr<ReadSet> rs(new ReadSet());
+  rs->contig = contig_id;
+  rs->n = n;
+  RSDev& d = rs->d;
+  std::memset(&d, 0, sizeof d);
+  d.n = n;
+  const int64_t ncig = s->cigar_off[n];
+  const int64_t nb = s->seq_off[n];
+  hipStream_t st = c->stream;
+  pl_status e;
+  void* p;
+#define UP(field, src, bytes, pad)                                  \
+  if ((e = upload(&p, (src), (size_t)(bytes), (pad), rs->mem, st))) return e; \
+  field = (decltype(field))p;
+  UP(d.pos, s->pos, 4 * n, 0);
+  UP(d.tlen, s->tlen, 4 * n, 0);
+  UP(d.flag, s->flag, 2 * n, 0);
+  UP(d.mapq, s->mapq, n, 0);
+  UP(d.cigar_off, s->cigar_off, 8 * (n + 1), 0);
+  UP(d.cigar, s->cigar, 4 * ncig, 0);
+  UP(d.seq_off, s->seq_off, 8 * (n + 1), 0);
+  UP(d.bases2, s->bases2, (nb + 3) / 4, 64);
+  UP(d.quals, s->quals, nb, 64);
+  UP(d.exo_read, s->exo_read, 8 * s->n_exotic, 0);
+  UP(d.exo_off, s->exo_off, 8 * s->n_exotic, 0);
+  UP(d.nib, s->nib, s->n_exotic ? nib_bytes(s) : 0, 64);
+#undef UP
+  d.n_exotic = s->n_exotic;
+  d.ref = it->second.dev;
+  d.ref_len = it->second.len;
+  HIPCHK(hipMalloc(&p, 16 * (size_t)(n + 1))); rs->mem.push_back(p); d.span = (int4*)p;
+  HIPCHK(hipMalloc(&p, 8 * (size_t)(n + 1))); rs->mem.push_back(p); d.ai = (int2*)p;
+  HIPCHK(hipMalloc(&p, 64)); rs->mem.push_back(p); d.scal = (int32_t*)p;
+  uint32_t* keepflag;
+  HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); rs->mem.push_back(p); keepflag = (uint32_t*)p;
+  HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); rs->mem.push_back(p); rs->kp = (uint32_t*)p;
+  int64_t ocap = 4096 + n / 256;
+  int64_t* outl;
+  HIPCHK(hipMalloc(&p, 8 * (size_t)ocap)); rs->mem.push_back(p); outl = (int64_t*)p;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    HIPCHK(hipMemsetAsync(d.scal, 0, 64, st));
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (n + 255) / 256), 8192);
+    k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(d, c->cf, keepflag, d.scal, outl, ocap);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rs->scal, d.scal, 32, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (rs->scal[3] <= ocap) break;
+    ocap = rs->scal[3];
+    HIPCHK(hipMalloc(&p, 8 * (size_t)ocap)); rs->mem.push_back(p); outl = (int64_t*)p;
+  }
+  pl_status ss = scan_u32(c, keepflag, rs->kp, n);
+  if (ss) return ss;
+  const int64_t no = rs->scal[3];
+  if (no > 0) {
+    std::vector<int64_t> h((size_t)no);
+    HIPCHK(hipMemcpyAsync(h.data(), outl, 8 * (size_t)no, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::sort(h.begin(), h.end());
+    HIPCHK(hipMemcpyAsync(outl, h.data(), 8 * (size_t)no, hipMemcpyHostToDevice, st));
+  }
+  d.outliers = outl;
+  d.n_outlier = no;
+  HIPCHK(hipStreamSynchronize(st));
+  rs->bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 24 * n + 8 * n;
+  c->rsets.push_back(std::move(rs));
+  c->rs_dirty = true;
+  *out_id = (int32_t)c->rsets.size() - 1;
+  return PL_OK;
+}
+
+struct JobReq { int32_t rs, start, stop; };
+
+pl_status run_jobs(pl_ctx* c, const std::vector<JobReq>& reqs, int32_t W, int persistent) {
+  if (reqs.empty()) return fail(PL_E_INVAL, "empty batch");
+  if (W < PL_WIN_BLOCK || W % PL_WIN_BLOCK || W > 2048) return fail(PL_E_INVAL, "window %d must be a multiple of %d in [%d, 2048]", W, PL_WIN_BLOCK, PL_WIN_BLOCK);
+  c->W = W;
+  c->exported_flags = false;
+  // ---- jobs and windows
+  c->hjobs.resize(reqs.size());
+  c->hwins.clear();
+  int64_t G = 0;
+  for (size_t j = 0; j < reqs.size(); j++) {
+    const JobReq& q = reqs[j];
+    if (q.rs < 0 || q.rs >= (int32_t)c->rsets.size() || !c->rsets[q.rs]) return fail(PL_E_INVAL, "bad read set %d", q.rs);
+    const Contig& ct = c->contigs[c->rsets[q.rs]->contig];
+    if (q.start < 1 || q.stop < q.start || q.stop > ct.len)
+      return fail(PL_E_INVAL, "region %d-%d outside contig of length %lld", q.start, q.stop, (long long)ct.len);
+    JobDev& J = c->hjobs[j];
+    std::memset(&J, 0, sizeof J);
+    J.rs = q.rs; J.start = q.start; J.stop = q.stop;
+    J.qs = (int32_t)std::max<int64_t>((int64_t)q.start - 10000, 0);
+    J.qe = (int32_t)std::min<int64_t>((int64_t)q.stop + 10000, ct.len);
+    J.size = q.stop - q.start + 1;
+    J.out = G;
+    J.win0 = (int32_t)c->hwins.size();
+    for (int32_t p0 = 0; p0 < J.size; p0 += W) {
+      WinDev w;
+      std::memset(&w, 0, sizeof w);
+      w.job = (int32_t)j; w.p0 = p0; w.p1 = std::min<int32_t>(J.size, p0 + W);
+      c->hwins.push_back(w);
+    }
+    J.nwin = (int32_t)c->hwins.size() - J.win0;
+    G += J.size;
+  }
+  if (G >= (int64_t)1 << 31) return fail(PL_E_INVAL, "batch of %lld positions exceeds 2^31", (long long)G);
+  c->G = G;
+  const int32_t n_jobs = (int32_t)reqs.size();
+  const int32_t n_wins = (int32_t)c->hwins.size();
+  c->n_wins = n_wins;
+  hipStream_t st = c->stream;
+  // ---- resident read-set table
+  if (c->rs_dirty) {
+    std::vector<RSDev> rsd(c->rsets.size());
+    std::vector<const uint32_t*> kps(c->rsets.size(), nullptr);
+    for (size_t i = 0; i < c->rsets.size(); i++) if (c->rsets[i]) { rsd[i] = c->rsets[i]->d; kps[i] = c->rsets[i]->kp; }
+    HIPCHK(c->rsarr.ensure(sizeof(RSDev) * rsd.size()));
+    HIPCHK(c->kparr.ensure(sizeof(void*) * kps.size()));
+    HIPCHK(hipMemcpyAsync(c->rsarr.p, rsd.data(), sizeof(RSDev) * rsd.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->kparr.p, kps.data(), sizeof(void*) * kps.size(), hipMemcpyHostToDevice, st));
+    c->rs_dirty = false;
+  }
+  // ---- capacities (grown and re-run on overflow)
+  if (!c->rec_cap) { c->rec_cap = 1u << 18; c->cand_cap = 1u << 16; c->defer_cap = 1u << 16; c->mut_cap = 1u << 16; }
+  c->rec_cap = std::max<uint32_t>(c->rec_cap, (uint32_t)std::min<int64_t>(G / 16 + 65536, 1ll << 30));
+  c->cand_cap = std::max<uint32_t>(c->cand_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
+  c->defer_cap = std::max<uint32_t>(c->defer_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
+  c->mut_cap = std::max<uint32_t>(c->mut_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
+  HIPCHK(c->jobs.ensure(sizeof(JobDev) * n_jobs));
+  HIPCHK(c->wins.ensure(sizeof(WinDev) * n_wins));
+  HIPCHK(c->worder.ensure(4 * (size_t)n_wins));
+  HIPCHK(c->costk.ensure(4 * (size_t)n_wins));
+  HIPCHK(c->costv.ensure(4 * (size_t)n_wins));
+  HIPCHK(c->costtk.ensure(4 * (size_t)n_wins));
+  HIPCHK(c->costtv.ensure(4 * (size_t)n_wins));
+  HIPCHK(c->pile.ensure(4 * (size_t)PL_NFIELDS * G));
+  HIPCHK(c->calls.ensure(sizeof(pl_call) * (size_t)G));
+  HIPCHK(c->st.ensure(sizeof(JobStat) * n_jobs));
+  HIPCHK(c->counters.ensure(64));
+  HIPCHK(hipMemcpyAsync(c->jobs.p, c->hjobs.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->wins.p, c->hwins.data(), sizeof(WinDev) * n_wins, hipMemcpyHostToDevice, st));
+  for (int attempt = 0; attempt < 6; attempt++) {
+    HIPCHK(c->rec.ensure(sizeof(IndelRec) * (size_t)c->rec_cap));
+    HIPCHK(c->rkey.ensure(4 * (size_t)c->rec_cap)); HIPCHK(c->rval.ensure(4 * (size_t)c->rec_cap));
+    HIPCHK(c->rtk.ensure(4 * (size_t)c->rec_cap)); HIPCHK(c->rtv.ensure(4 * (size_t)c->rec_cap));
+    HIPCHK(c->cand.ensure(sizeof(Cand) * (size_t)c->cand_cap));
+    HIPCHK(c->cands.ensure(sizeof(Cand) * (size_t)c->cand_cap));
+    HIPCHK(c->ckey.ensure(4 * (size_t)c->cand_cap)); HIPCHK(c->cval.ensure(4 * (size_t)c->cand_cap));
+    HIPCHK(c->ctk.ensure(4 * (size_t)c->cand_cap)); HIPCHK(c->ctv.ensure(4 * (size_t)c->cand_cap));
+    HIPCHK(c->defer.ensure(4 * (size_t)c->defer_cap));
+    HIPCHK(c->mutated.ensure(4 * (size_t)c->mut_cap));
+    uint32_t* cnt = c->counters.as<uint32_t>();
+    EngineArgs& A = c->args;
+    std::memset(&A, 0, sizeof A);
+    A.cf = c->cf;
+    A.rs = c->rsarr.as<RSDev>();
+    A.jobs = c->jobs.as<JobDev>();
+    A.n_jobs = n_jobs;
+    A.wins = c->wins.as<WinDev>();
+    A.n_wins = n_wins;
+    A.W = W;
+    A.queue = (int32_t*)(cnt + 5);
+    A.worder = c->worder.as<int32_t>();
+    A.pile = c->pile.as<int32_t>();
+    A.calls = c->calls.as<pl_call>();
+    A.G = G;
+    A.st = c->st.as<JobStat>();
+    A.rec = c->rec.as<IndelRec>(); A.rec_n = cnt + 0; A.rec_cap = c->rec_cap;
+    A.cand = c->cand.as<Cand>(); A.cand_n = cnt + 1; A.cand_cap = c->cand_cap;
+    A.defer = c->defer.as<uint32_t>(); A.defer_n = cnt + 2; A.defer_cap = c->defer_cap;
+    A.mutated = c->mutated.as<uint32_t>(); A.mutated_n = cnt + 3; A.mutated_cap = c->mut_cap;
+    A.overflow = (int32_t*)(cnt + 4);
+
+    HIPCHK(hipEventRecord(c->ev[0], st));
+    const int64_t words = (int64_t)n_jobs * (sizeof(JobStat) / 4);
+    k_init_stats<<<dim3((unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (words + 255) / 256))), dim3(256), 0, st>>>(A.st, n_jobs, cnt, 16);
+    k_init_stats2<<<dim3((n_jobs + 255) / 256), dim3(256), 0, st>>>(A.st, n_jobs);
+    k_plan<<<dim3((n_wins + 255) / 256), dim3(256), 0, st>>>(A.rs, A.jobs, c->wins.as<WinDev>(), n_wins, c->costk.as<uint32_t>());
+    HIPCHK(hipGetLastError());
+    if (persistent) {
+      std::vector<uint32_t> iota((size_t)n_wins);
+      for (int32_t i = 0; i < n_wins; i++) iota[i] = (uint32_t)i;
+      HIPCHK(hipMemcpyAsync(c->costv.p, iota.data(), 4 * (size_t)n_wins, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(cnt + 8, &n_wins, 4, hipMemcpyHostToDevice, st));
+      uint32_t *sk, *sv;
+      pl_status s = radix_sort(c, c->costk.as<uint32_t>(), c->costv.as<uint32_t>(), c->costtk.as<uint32_t>(),
+                               c->costtv.as<uint32_t>(), cnt + 8, (uint32_t)n_wins, 32, &sk, &sv);
+      if (s) return s;
+      A.worder = (const int32_t*)sv;
+    }
+    k_jobprep<<<dim3(n_jobs), dim3(64), 0, st>>>(A.rs, (const uint32_t* const*)c->kparr.p, A.jobs, A.st);
+    HIPCHK(hipGetLastError());
+    const size_t lds = 72 * (size_t)W + 1024;
+    HIPCHK(hipFuncSetAttribute((const void*)k_window, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int grid = n_wins;
+    if (persistent) {
+      const int per_cu = std::max(1, std::min(8, (int)(160 * 1024 / (lds + 512))));
+      grid = std::min(n_wins, c->n_cu * per_cu);
+    }
+    HIPCHK(hipEventRecord(c->ev[2], st));
+    k_window<<<dim3(grid), dim3(PL_WIN_BLOCK), lds, st>>>(A, persistent);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[3], st));
+    // evidence records: sort by (position, kind), then the deferred columns
+    const uint32_t gb = (uint32_t)std::min<int64_t>(4096, std::max<int64_t>(1, ((int64_t)c->rec_cap + 255) / 256));
+    k_rec_keys<<<dim3(gb), dim3(256), 0, st>>>(A.rec, A.rec_n, A.rec_cap, c->rkey.as<uint32_t>(), c->rval.as<uint32_t>());
+    uint32_t *rk, *rv;
+    pl_status s = radix_sort(c, c->rkey.as<uint32_t>(), c->rval.as<uint32_t>(), c->rtk.as<uint32_t>(), c->rtv.as<uint32_t>(),
+                             A.rec_n, A.rec_cap, bits_for(2 * (uint64_t)G), &rk, &rv);
+    if (s) return s;
+    A.rkey = rk; A.rval = rv;
+    c->rkey_sorted = rk; c->rval_sorted = rv;
+    const uint32_t gd = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)c->defer_cap + 255) / 256));
+    k_indel_pass<<<dim3(gd), dim3(256), 0, st>>>(A, A.defer, A.defer_n, A.defer_cap, 0);
+    HIPCHK(hipGetLastError());
+    // candidates by position, then Pass 1 per region
+    const uint32_t gc = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)c->cand_cap + 255) / 256));
+    k_cand_keys<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, c->ckey.as<uint32_t>(), c->cval.as<uint32_t>());
+    uint32_t *ck, *cv;
+    s = radix_sort(c, c->ckey.as<uint32_t>(), c->cval.as<uint32_t>(), c->ctk.as<uint32_t>(), c->ctv.as<uint32_t>(),
+                   A.cand_n, A.cand_cap, bits_for((uint64_t)G), &ck, &cv);
+    if (s) return s;
+    k_cand_gather<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, cv, c->cands.as<Cand>());
+    k_finalize<<<dim3((n_jobs + 63) / 64), dim3(64), 0, st>>>(A, c->cands.as<Cand>(), A.cand_n, A.cand_cap, c->cands.as<Cand>());
+    const uint32_t gm = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)c->mut_cap + 255) / 256));
+    k_indel_pass<<<dim3(gm), dim3(256), 0, st>>>(A, A.mutated, A.mutated_n, A.mutated_cap, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], st));
+    uint32_t hc[8];
+    HIPCHK(hipMemcpyAsync(hc, cnt, 32, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0, wms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    HIPCHK(hipEventElapsedTime(&wms, c->ev[2], c->ev[3]));
+    c->last_ms = ms;
+    c->last_win_ms = wms;
+    std::memcpy(c->last_counts, hc, sizeof hc);
+    c->last_persistent = persistent;
+    bool grow = false;
+    if (hc[0] > c->rec_cap) { c->rec_cap = (uint32_t)std::min<uint64_t>(2ull * hc[0], 1u << 30); grow = true; }
+    if (hc[1] > c->cand_cap) { c->cand_cap = (uint32_t)std::min<uint64_t>(2ull * hc[1], 1u << 30); grow = true; }
+    if (hc[2] > c->defer_cap) { c->defer_cap = (uint32_t)std::min<uint64_t>(2ull * hc[2], 1u << 30); grow = true; }
+    if (hc[3] > c->mut_cap) { c->mut_cap = (uint32_t)std::min<uint64_t>(2ull * hc[3], 1u << 30); grow = true; }
+    if (!grow) {
+      if (hc[4]) return fail(PL_E_DEVICE, "capacity overflow flag set without an overfull counter");
+      return PL_OK;
+    }
+  }
+  return fail(PL_E_OOM, "evidence buffers kept overflowing");
+}
+
+// copy stats (and candidates) of the last batch to the host
+pl_status fetch_batch_meta(pl_ctx* c) {
+  const size_t nj = c->hjobs.size();
+  c->hst.resize(nj);
+  HIPCHK(hipMemcpyAsync(c->hst.data(), c->st.p, sizeof(JobStat) * nj, hipMemcpyDeviceToHost, c->stream));
+  uint32_t cn = 0;
+  HIPCHK(hipMemcpyAsync(&cn, c->counters.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  cn = std::min(cn, c->cand_cap);
+  c->hcand.resize(cn);
+  if (cn) HIPCHK(hipMemcpyAsync(c->hcand.data(), c->cands.p, sizeof(Cand) * cn, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PL_OK;
+}
+
+pl_status job_status(pl_ctx* c, size_t j) {
+  const JobStat& S = c->hst[j];
+  if (S.err_read_inval != INT64_MAX && S.err_read_inval <= S.err_read_edge)
+    return fail(PL_E_INVAL, "malformed read %lld (CIGAR and SEQ disagree or bad op)", (long long)S.err_read_inval);
+  if (S.err_read_edge != INT64_MAX)
+    return fail(PL_E_EDGE, "read %lld: Pilon 1.24 throws here (indel normalised before region start, or clip without SEQ)",
+                (long long)S.err_read_edge);
+  if (S.overflow) return fail(PL_E_OVERFLOW, "a column exceeds %d bases", PL_DEPTH_LIMIT);
+  return PL_OK;
+}
+
+// stats + change list of job j of the last batch (needs fetch_batch_meta)
+pl_status job_changes(pl_ctx* c, size_t j, pl_region_stats* out, std::vector<plh::Change>* changes) {
+  const JobDev& J = c->hjobs[j];
+  const JobStat& S = c->hst[j];
+  const Contig& ct = c->contigs[c->rsets[J.rs]->contig];
+  std::vector<plh::Change> ch;
+  std::vector<int64_t> ins_idx;
+  for (int32_t k = S.cand_lo; k < S.cand_hi && k < (int32_t)c->hcand.size(); k++) {
+    const Cand& cd = c->hcand[k];
+    if (!cd.kind) continue;
+    plh::Change x;
+    x.loc = J.start + (int32_t)(cd.gpos - J.out);
+    x.kind = cd.kind;
+    x.ref = to_upper(ct.host[x.loc - 1]);
+    x.base = 4; x.alt = 0;
+    if (cd.kind == PL_D_DEL) x.indel.assign((const char*)ct.host.data() + x.loc - 1, (size_t)cd.len);
+    if (cd.kind == PL_D_INS) ins_idx.push_back(k);
+    ch.push_back(x);
+  }
+  // base/alt of the changed columns
+  std::vector<pl_call> cl(ch.size());
+  for (size_t i = 0; i < ch.size(); i++)
+    HIPCHK(hipMemcpyAsync(&cl[i], c->calls.as<pl_call>() + J.out + (ch[i].loc - J.start), sizeof(pl_call),
+                          hipMemcpyDeviceToHost, c->stream));
+  // insertion strings decoded on the device
+  std::string pool;
+  std::vector<int64_t> off(ins_idx.size() + 1, 0);
+  if (!ins_idx.empty()) {
+    for (size_t i = 0; i < ins_idx.size(); i++) off[i + 1] = off[i] + c->hcand[ins_idx[i]].len;
+    HIPCHK(c->stridx.ensure(8 * ins_idx.size()));
+    HIPCHK(c->stroff.ensure(8 * off.size()));
+    HIPCHK(c->strpool.ensure((size_t)off.back() + 16));
+    HIPCHK(hipMemcpyAsync(c->stridx.p, ins_idx.data(), 8 * ins_idx.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->stroff.p, off.data(), 8 * off.size(), hipMemcpyHostToDevice, c->stream));
+    k_ins_strings<<<dim3((unsigned)((ins_idx.size() + 63) / 64)), dim3(64), 0, c->stream>>>(
+        c->args, c->cands.as<Cand>(), c->stridx.as<int64_t>(), c->stroff.as<int64_t>(), (int32_t)ins_idx.size(),
+        c->strpool.as<uint8_t>());
+    HIPCHK(hipGetLastError());
+    pool.resize((size_t)off.back());
+    HIPCHK(hipMemcpyAsync(&pool[0], c->strpool.p, (size_t)off.back(), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  size_t ii = 0;
+  for (size_t i = 0; i < ch.size(); i++) {
+    ch[i].base = cl[i].base;
+    ch[i].alt = cl[i].alt;
+    if (ch[i].kind == PL_D_INS) { ch[i].indel = pool.substr((size_t)off[ii], (size_t)(off[ii + 1] - off[ii])); ii++; }
+  }
+  if (out) {
+    std::memset(out, 0, sizeof *out);
+    out->read_count = S.read_count;
+    out->base_count = S.base_count;
+    out->coverage = round_div_l(S.base_count, J.size);
+    out->min_depth = S.min_depth;
+    out->n_confirmed = S.n_confirmed;
+    out->n_unknown_ops = (int32_t)S.unknown_ops;
+    out->aligned_bases = S.aligned;
+    for (const plh::Change& x : ch) {
+      if (x.kind == PL_D_SNP) out->snps++;
+      else if (x.kind == PL_D_AMB) { if (c->pc.fix_snps && !c->pc.longread) out->amb++; }
+      else if (x.kind == PL_D_INS) { out->ins++; out->ins_bases += (int32_t)x.indel.size(); }
+      else if (x.kind == PL_D_DEL) { out->dels++; out->del_bases += (int32_t)x.indel.size(); }
+    }
+  }
+  if (changes) *changes = std::move(ch);
+  return PL_OK;
+}
+
+// pileup / final calls of job j of the last batch into host buffers
+pl_status job_export(pl_ctx* c, size_t j, int32_t* pile, pl_call* calls) {
+  const JobDev& J = c->hjobs[j];
+  if (calls && !c->exported_flags) {
+    k_final_flags<<<dim3(c->n_wins), dim3(256), 0, c->stream>>>(c->args);
+    HIPCHK(hipGetLastError());
+    c->exported_flags = true;
+  }
+  if (pile)
+    for (int f = 0; f < PL_NFIELDS; f++)
+      HIPCHK(hipMemcpyAsync(pile + (size_t)f * J.size, c->pile.as<int32_t>() + (size_t)f * c->G + J.out,
+                            4 * (size_t)J.size, hipMemcpyDeviceToHost, c->stream));
+  if (calls)
+    HIPCHK(hipMemcpyAsync(calls, c->calls.as<pl_call>() + J.out, sizeof(pl_call) * (size_t)J.size,
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PL_OK;
+}
+
+void append_reads(HostReads& h, const pl_read_soa* s) {
+  const int64_t n = s->n_reads;
+  const int64_t r0 = (int64_t)h.pos.size();
+  h.pos.insert(h.pos.end(), s->pos, s->pos + n);
+  h.tlen.insert(h.tlen.end(), s->tlen, s->tlen + n);
+  h.flag.insert(h.flag.end(), s->flag, s->flag + n);
+  h.mapq.insert(h.mapq.end(), s->mapq, s->mapq + n);
+  const int64_t c0 = h.cigar_off.back(), s0 = h.seq_off.back();
+  for (int64_t i = 0; i < n; i++) {
+    h.cigar_off.push_back(c0 + s->cigar_off[i + 1]);
+    h.seq_off.push_back(s0 + s->seq_off[i + 1]);
+  }
+  h.cigar.insert(h.cigar.end(), s->cigar, s->cigar + s->cigar_off[n]);
+  const int64_t nb = s->seq_off[n];
+  h.quals.insert(h.quals.end(), s->quals, s->quals + nb);
+  for (int64_t q = 0; q < nb; q++) h.bases2.push_back((s->bases2[q >> 2] >> (2 * (q & 3))) & 3);
+  for (int64_t k = 0; k < s->n_exotic; k++) {
+    const int64_t r = s->exo_read[k];
+    const int64_t L = s->seq_off[r + 1] - s->seq_off[r];
+    h.exo_read.push_back(r0 + r);
+    h.exo_off.push_back((int64_t)h.nibc.size());
+    for (int64_t q = 0; q < L; q++) {
+      const int64_t x = s->exo_off[k] + q;
+      const uint8_t b = s->nib[x >> 1];
+      h.nibc.push_back((x & 1) ? (b & 15) : (b >> 4));
+    }
+  }
+}
+
+// merge the plain-form batches into one coordinate-sorted set and upload it
+pl_status upload_host_batches(pl_ctx* c, int32_t contig, std::vector<HostReads>& bs, int32_t* id) {
+  struct Ref { int32_t b; int64_t r; };
+  std::vector<Ref> order;
+  for (int32_t b = 0; b < (int32_t)bs.size(); b++)
+    for (int64_t r = 0; r < (int64_t)bs[b].pos.size(); r++) order.push_back({b, r});
+  std::stable_sort(order.begin(), order.end(), [&](const Ref& x, const Ref& y) { return bs[x.b].pos[x.r] < bs[y.b].pos[y.r]; });
+  const int64_t n = (int64_t)order.size();
+  std::vector<int32_t> pos(n), tlen(n);
+  std::vector<uint16_t> flag(n);
+  std::vector<uint8_t> mapq(n);
+  std::vector<int64_t> cigar_off(n + 1, 0), seq_off(n + 1, 0), exo_read, exo_off;
+  std::vector<uint32_t> cigar;
+  std::vector<int8_t> quals;
+  std::vector<uint8_t> codes, nibc;
+  for (int64_t i = 0; i < n; i++) {
+    const HostReads& h = bs[order[i].b];
+    const int64_t r = order[i].r;
+    pos[i] = h.pos[r]; tlen[i] = h.tlen[r]; flag[i] = h.flag[r]; mapq[i] = h.mapq[r];
+    cigar.insert(cigar.end(), h.cigar.begin() + h.cigar_off[r], h.cigar.begin() + h.cigar_off[r + 1]);
+    quals.insert(quals.end(), h.quals.begin() + h.seq_off[r], h.quals.begin() + h.seq_off[r + 1]);
+    codes.insert(codes.end(), h.bases2.begin() + h.seq_off[r], h.bases2.begin() + h.seq_off[r + 1]);
+    cigar_off[i + 1] = (int64_t)cigar.size();
+    seq_off[i + 1] = (int64_t)quals.size();
+    auto it = std::lower_bound(h.exo_read.begin(), h.exo_read.end(), r);
+    if (it != h.exo_read.end() && *it == r) {
+      const int64_t k = it - h.exo_read.begin();
+      exo_read.push_back(i);
+      exo_off.push_back((int64_t)nibc.size());
+      nibc.insert(nibc.end(), h.nibc.begin() + h.exo_off[k], h.nibc.begin() + h.exo_off[k] + (h.seq_off[r + 1] - h.seq_off[r]));
+    }
+  }
+  std::vector<uint8_t> bases2((codes.size() + 3) / 4 + 8, 0), nib(nibc.size() / 2 + 8, 0);
+  for (size_t q = 0; q < codes.size(); q++) bases2[q >> 2] |= (uint8_t)(codes[q] << (2 * (q & 3)));
+  for (size_t q = 0; q < nibc.size(); q++) nib[q >> 1] |= (uint8_t)((q & 1) ? nibc[q] : (nibc[q] << 4));
+  pl_read_soa s;
+  std::memset(&s, 0, sizeof s);
+  s.n_reads = n;
+  s.pos = pos.data(); s.tlen = tlen.data(); s.flag = flag.data(); s.mapq = mapq.data();
+  s.cigar_off = cigar_off.data(); s.cigar = cigar.data(); s.seq_off = seq_off.data();
+  s.bases2 = bases2.data(); s.quals = quals.data();
+  s.n_exotic = (int64_t)exo_read.size(); s.exo_read = exo_read.data(); s.exo_off = exo_off.data(); s.nib = nib.data();
+  return readset_upload(c, contig, &s, id);
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+void pl_config_default(pl_config* c) {
+  std::memset(c, 0, sizeof *c);
+  c->flank = 10; c->min_mq = 0; c->min_qual = 0; c->default_qual = 10; c->min_depth = 0.1;
+  c->min_min_depth = 5; c->fix_snps = 1; c->fix_indels = 1;
+}
+
+const char* pl_last_error(void) { return g_err.c_str(); }
+
+pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out) {
+  if (!cfg || !out) return fail(PL_E_INVAL, "null argument");
+  if (cfg->longread) return fail(PL_E_INVAL, "long-read BAMs (addRead longRead > 0) are not supported");
+  if (cfg->flank < 0) return fail(PL_E_INVAL, "flank must be >= 0");
+  if (cfg->default_qual < -128 || cfg->default_qual > 127) return fail(PL_E_INVAL, "defaultQual is a Byte");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(PL_E_DEVICE, "no HIP device %d (have %d)", device, n);
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  std::unique_ptr<pl_ctx> c(new pl_ctx());
+  c->pc = *cfg;
+  to_kconf(*cfg, c->cf);
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+  *out = c.release();
+  return PL_OK;
+}
+
+void pl_ctx_destroy(pl_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->rsets.clear();
+  for (auto& kv : c->contigs) if (kv.second.dev) (void)hipFree(kv.second.dev);
+  DBuf* bufs[] = {&c->rsarr, &c->kparr, &c->jobs, &c->wins, &c->worder, &c->costk, &c->costv, &c->costtk, &c->costtv,
+                  &c->pile, &c->calls, &c->st, &c->rec, &c->rkey, &c->rval, &c->rtk, &c->rtv, &c->cand, &c->cands,
+                  &c->ckey, &c->cval, &c->ctk, &c->ctv, &c->defer, &c->mutated, &c->counters, &c->hist, &c->tiles,
+                  &c->strpool, &c->stridx, &c->stroff};
+  for (DBuf* b : bufs) b->release();
+  for (int i = 0; i < 4; i++) (void)hipEventDestroy(c->ev[i]);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+pl_status pl_ref_upload(pl_ctx* c, int32_t contig_id, const char* name, const uint8_t* ascii, int64_t len) {
+  if (!c || !ascii || len <= 0) return fail(PL_E_INVAL, "bad reference");
+  if (len >= ((int64_t)1 << 31) - 1) return fail(PL_E_INVAL, "contig longer than 2^31");
+  HIPCHK(hipSetDevice(c->device));
+  Contig& ct = c->contigs[contig_id];
+  if (ct.dev) { HIPCHK(hipStreamSynchronize(c->stream)); HIPCHK(hipFree(ct.dev)); ct.dev = nullptr; }
+  ct.name = name ? name : "";
+  ct.host.assign(ascii, ascii + len);
+  ct.len = len;
+  HIPCHK(hipMalloc((void**)&ct.dev, (size_t)len + 64));
+  HIPCHK(hipMemcpyAsync(ct.dev, ascii, (size_t)len, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(ct.dev + len, 0, 64, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PL_OK;
+}
+
+pl_status pl_readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* reads, int32_t* id) {
+  if (!c || !reads || !id) return fail(PL_E_INVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  return readset_upload(c, contig_id, reads, id);
+}
+
+pl_status pl_region_open(pl_ctx* c, int32_t contig_id, int32_t start, int32_t stop, pl_region** out) {
+  if (!c || !out) return fail(PL_E_INVAL, "null argument");
+  auto it = c->contigs.find(contig_id);
+  if (it == c->contigs.end()) return fail(PL_E_INVAL, "contig %d not uploaded", contig_id);
+  if (start < 1 || stop < start || stop > it->second.len)
+    return fail(PL_E_INVAL, "GenomeRegion stop point must be within contig (%d-%d of %lld)", start, stop, (long long)it->second.len);
+  pl_region* r = new pl_region();
+  r->ctx = c; r->contig = contig_id; r->start = start; r->stop = stop;
+  std::memset(&r->stats, 0, sizeof r->stats);
+  *out = r;
+  return PL_OK;
+}
+
+pl_status pl_region_add_reads(pl_region* r, const pl_read_soa* reads) {
+  if (!r || !reads) return fail(PL_E_INVAL, "null argument");
+  if (r->finished) return fail(PL_E_INVAL, "region already finished");
+  if (r->readset >= 0) return fail(PL_E_INVAL, "region already uses a resident read set");
+  if (reads->n_reads == 0) return PL_OK;
+  for (int64_t i = 1; i < reads->n_reads; i++)
+    if (reads->pos[i] < reads->pos[i - 1]) return fail(PL_E_INVAL, "reads must be coordinate-sorted within a call");
+  r->batches.emplace_back();   // one batch per BAM; merged by position at finish
+  append_reads(r->batches.back(), reads);
+  return PL_OK;
+}
+
+pl_status pl_region_use_readset(pl_region* r, int32_t id) {
+  if (!r) return fail(PL_E_INVAL, "null argument");
+  if (!r->batches.empty() || r->finished) return fail(PL_E_INVAL, "region already has reads");
+  if (id < 0 || id >= (int32_t)r->ctx->rsets.size() || !r->ctx->rsets[id] || r->ctx->rsets[id]->contig != r->contig)
+    return fail(PL_E_INVAL, "read set %d is not a set of this contig", id);
+  r->readset = id;
+  return PL_OK;
+}
+
+pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_region_stats* stats) {
+  if (!r) return fail(PL_E_INVAL, "null argument");
+  pl_ctx* c = r->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  int32_t id = r->readset;
+  bool temp = false;
+  if (id < 0) {
+    if (r->batches.empty()) r->batches.emplace_back();
+    pl_status s = upload_host_batches(c, r->contig, r->batches, &id);
+    if (s) return s;
+    temp = true;
+    r->batches.clear();
+  }
+  std::vector<JobReq> q{{id, r->start, r->stop}};
+  const int32_t W = 1024;
+  pl_status s = run_jobs(c, q, W, 0);
+  if (!s) s = fetch_batch_meta(c);
+  if (!s) s = job_status(c, 0);
+  if (!s) s = job_changes(c, 0, &r->stats, &r->changes);
+  if (!s) s = job_export(c, 0, pile, calls);
+  if (temp) { c->rsets[id].reset(); c->rs_dirty = true; }
+  if (s) return s;
+  if (stats) *stats = r->stats;
+  r->finished = true;
+  return PL_OK;
+}
+
+pl_status pl_region_fix(pl_region* r, const char* new_name, int64_t offset, char** changes_text, int64_t* changes_len,
+                        uint8_t** bases, int64_t* bases_len) {
+  if (!r || !r->finished) return fail(PL_E_INVAL, "region not finished");
+  const Contig& ct = r->ctx->contigs[r->contig];
+  std::string rb((const char*)ct.host.data() + r->start - 1, (size_t)(r->stop - r->start + 1));
+  std::string nn = new_name ? new_name : ct.name + "_pilon";
+  plh::FixResult fr = plh::fix_region(r->ctx->pc, ct.name, r->start, rb, r->changes, nn, offset);
+  if (changes_text) {
+    *changes_text = (char*)malloc(fr.changes.size() + 1);
+    std::memcpy(*changes_text, fr.changes.c_str(), fr.changes.size() + 1);
+  }
+  if (changes_len) *changes_len = (int64_t)fr.changes.size();
+  if (bases) {
+    *bases = (uint8_t*)malloc(fr.bases.size() + 1);
+    std::memcpy(*bases, fr.bases.data(), fr.bases.size());
+  }
+  if (bases_len) *bases_len = (int64_t)fr.bases.size();
+  return PL_OK;
+}
+
+void pl_region_close(pl_region* r) { delete r; }
+void pl_free(void* p) { free(p); }
+
+pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, const pl_batch_opts* opts, int32_t* pile,
+                       pl_call* calls, pl_region_stats* stats) {
+  if (!c || !regions || n <= 0) return fail(PL_E_INVAL, "empty batch");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<JobReq> q((size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    const pl_region_desc& d = regions[i];
+    if (d.readset_id < 0 || d.readset_id >= (int32_t)c->rsets.size() || !c->rsets[d.readset_id] ||
+        c->rsets[d.readset_id]->contig != d.contig_id)
+      return fail(PL_E_INVAL, "region %d: read set %d is not a set of contig %d", i, d.readset_id, d.contig_id);
+    q[i] = {d.readset_id, d.start, d.stop};
+  }
+  const int32_t W = (opts && opts->window) ? opts->window : 1024;
+  const int persistent = opts ? opts->persistent : 0;
+  pl_status s = run_jobs(c, q, W, persistent);
+  if (s) return s;
+  s = fetch_batch_meta(c);
+  if (s) return s;
+  for (int32_t i = 0; i < n; i++) if ((s = job_status(c, i))) return s;
+  if (stats)
+    for (int32_t i = 0; i < n; i++) if ((s = job_changes(c, i, &stats[i], nullptr))) return s;
+  if (pile || calls) {
+    int64_t off = 0;
+    for (int32_t i = 0; i < n; i++) {
+      const int64_t sz = c->hjobs[i].size;
+      std::vector<int32_t> tmp;
+      if (pile) tmp.resize((size_t)PL_NFIELDS * sz);
+      if ((s = job_export(c, i, pile ? tmp.data() : nullptr, calls ? calls + off : nullptr))) return s;
+      if (pile)
+        for (int f = 0; f < PL_NFIELDS; f++)
+          std::memcpy(pile + (size_t)f * c->G + off, tmp.data() + (size_t)f * sz, 4 * (size_t)sz);
+      off += sz;
+    }
+  }
+  return PL_OK;
+}
+
+pl_status pl_last_batch_info(pl_ctx* c, pl_batch_info* o) {
+  if (!c || !o) return fail(PL_E_INVAL, "null argument");
+  std::memset(o, 0, sizeof *o);
+  o->positions = c->G;
+  o->n_regions = (int32_t)c->hjobs.size();
+  o->n_windows = c->n_wins;
+  o->n_records = c->last_counts[0];
+  o->n_candidates = c->last_counts[1];
+  o->n_deferred = c->last_counts[2];
+  o->window = c->W;
+  o->persistent = c->last_persistent;
+  o->pipeline_ms = c->last_ms;
+  o->window_ms = c->last_win_ms;
+  return PL_OK;
+}
+
+pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
+  if (!c) return fail(PL_E_INVAL, "null argument");
+  if (pipeline_ms) *pipeline_ms = c->last_ms;
+  if (window_ms) *window_ms = c->last_win_ms;
+  return PL_OK;
+}
+
+}  // extern "C"

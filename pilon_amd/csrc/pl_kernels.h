@@ -1,0 +1,8 @@
+// pl_kernels.h — launch constants shared by the kernels and the host engine.
+#pragma once
+#define SCAN_TILE 4096            // items per scan tile (256 threads x 16)
+#define RS_ITEMS 16               // radix sort: keys per thread per block
+#define RS_TILE (256 * RS_ITEMS)
+#ifndef PL_WIN_BLOCK
+#define PL_WIN_BLOCK 256          // threads per window workgroup
+#endif

@@ -1,0 +1,579 @@
+// pl_kernels.hip — gfx950 kernels of the pileup path.
+//
+//   k_prep           per read, once per resident read set: alignment end,
+//                    validity/adjMq, touched span, fragment, outliers
+//   k_scan_*         exclusive scans (keep-read prefix, radix histograms)
+//   k_plan           per window: the sorted read range it must scan
+//   k_jobprep        per region: readCount (BamFile.process), P ops, errors
+//   k_window         the hot kernel: one workgroup owns a window of W
+//                    positions, accumulates every counter in LDS, scans
+//                    physCov, and writes counters + BaseCall once
+//   k_rs_*           LSD radix sort (evidence records, change candidates)
+//   k_indel_pass     BaseCall of columns whose indel strings matter
+//   k_finalize       Pass 1 per region: minDepth, deletion pass
+//   k_recompute      BaseCall of columns changed by the deletion pass
+//   k_final_flags    final dispositions for export
+//
+// Reference: broadinstitute/pilon v1.24, src/main/scala/org/broadinstitute/pilon/
+#include <hip/hip_runtime.h>
+#include "pl_core.h"
+#include "pl_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ int64_t lower_pos_dev(const int32_t* pos, int64_t lo, int64_t hi, int64_t x) {
+  while (lo < hi) {
+    int64_t m = (lo + hi) >> 1;
+    if ((int64_t)pos[m] < x) lo = m + 1; else hi = m;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return lane ? ((~0ull) >> (64 - lane)) : 0ull;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ k_prep
+__global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, int32_t* scal, int64_t* outliers, int64_t out_cap) {
+  const int64_t n = R.n;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    PrepOut o = prep_read(R, r, cf);
+    uint32_t info = (uint32_t)o.ai.y;
+    const bool keep = (info & RI_KEEP) != 0;
+    if (keep) {
+      int64_t lo = INT64_MAX, hi = INT64_MIN;
+      if (o.span.x <= o.span.y) { lo = o.span.x; hi = o.span.y; }
+      if (o.span.z < o.span.w) {
+        lo = lo < o.span.z ? lo : o.span.z;
+        hi = hi > (int64_t)o.span.w - 1 ? hi : (int64_t)o.span.w - 1;
+      }
+      const int32_t pos = R.pos[r];
+      if (lo <= hi) {
+        const int64_t lead = pos - lo, back = hi - pos;
+        if (lead > PL_MAX_LEAD || back > PL_MAX_BACK) {
+          info |= RI_OUTLIER;
+          int k = atomicAdd(&scal[3], 1);
+          if (k < out_cap) outliers[k] = r;
+        } else {
+          atomicMax(&scal[0], (int32_t)(back > 0 ? back : 0));
+          atomicMax(&scal[1], (int32_t)(lead > 0 ? lead : 0));
+        }
+      }
+      const int32_t spanEnd = o.ai.x != 0 ? o.ai.x : pos;
+      atomicMax(&scal[2], spanEnd - pos > 0 ? spanEnd - pos : 0);
+      if (info & RI_UNK_MASK) atomicOr(&scal[4], 1);
+      if (info & (RI_BAD | RI_EDGE_CLIP)) atomicOr(&scal[5], 1);
+    }
+    R.span[r] = o.span;
+    R.ai[r] = make_int2(o.ai.x, (int)info);
+    keepflag[r] = keep ? 1u : 0u;
+  }
+}
+
+// ------------------------------------------------------------------ scans
+// Exclusive scan of n uint32 in tiles of SCAN_TILE; out has n+1 entries.
+__global__ void __launch_bounds__(256) k_scan_tiles(const uint32_t* in, int64_t n, uint32_t* tile_sums) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+  for (int i = threadIdx.x; i < SCAN_TILE; i += 256) {
+    const int64_t k = base + i;
+    if (k < n) s += in[k];
+  }
+  s = wave_sum(s);
+  __shared__ uint32_t ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* tile_sums, int64_t nt) {
+  __shared__ uint32_t ws[16];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t b0 = 0; b0 < nt; b0 += 1024) {
+    const int64_t k = b0 + threadIdx.x;
+    uint32_t v = k < nt ? tile_sums[k] : 0;
+    uint32_t inc = wave_incl_scan(v);
+    if ((threadIdx.x & 63) == 63) ws[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint32_t add = carry;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) add += ws[w];
+    if (k < nt) tile_sums[k] = add + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = add + inc;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scan_apply(const uint32_t* in, int64_t n, const uint32_t* tile_sums, uint32_t* out) {
+  // each thread owns SCAN_TILE/256 consecutive items
+  constexpr int PER = SCAN_TILE / 256;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * PER;
+  uint32_t v[PER];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    v[i] = (base + i < n) ? in[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t inc = wave_incl_scan(s);
+  __shared__ uint32_t ws[4];
+  if ((threadIdx.x & 63) == 63) ws[threadIdx.x >> 6] = inc;
+  __syncthreads();
+  uint32_t acc = tile_sums[blockIdx.x] + inc - s;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) acc += ws[w];
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    if (base + i < n) out[base + i] = acc;
+    if (base + i == n - 1) out[n] = acc + v[i];
+    acc += v[i];
+  }
+  if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+}
+
+// ------------------------------------------------------------------ plan
+__global__ void k_plan(const RSDev* rsets, const JobDev* jobs, WinDev* wins, int32_t n_wins, uint32_t* cost) {
+  const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n_wins) return;
+  WinDev& win = wins[w];
+  const JobDev& J = jobs[win.job];
+  const RSDev& R = rsets[J.rs];
+  const int64_t L0 = (int64_t)J.start + win.p0, L1 = (int64_t)J.start + win.p1;
+  const int32_t back = R.scal[0], lead = R.scal[1];
+  win.lo = lower_pos_dev(R.pos, 0, R.n, L0 - back);
+  win.hi = lower_pos_dev(R.pos, win.lo, R.n, L1 + lead);
+  // descending-cost order key for the work queue (largest windows first)
+  const int64_t c = (win.hi - win.lo) + (int64_t)(win.p1 - win.p0) / 16;
+  cost[w] = 0xffffffffu - (uint32_t)(c > 0xfffffffll ? 0xfffffffll : c);
+}
+
+// ------------------------------------------------------------------ jobprep
+// readCount of BamFile.process (BamFile.scala:118-139): kept reads whose span
+// overlaps [qs, qe], from the keep-prefix plus the reads straddling qs.
+__global__ void __launch_bounds__(64) k_jobprep(const RSDev* rsets, const uint32_t* const* keep_prefix,
+                                                const JobDev* jobs, JobStat* st) {
+  const JobDev& J = jobs[blockIdx.x];
+  const RSDev& R = rsets[J.rs];
+  const uint32_t* kp = keep_prefix[J.rs];
+  JobStat& S = st[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int64_t a = lower_pos_dev(R.pos, 0, R.n, J.qs);
+  const int64_t b = lower_pos_dev(R.pos, a, R.n, (int64_t)J.qe + 1);
+  const int64_t a1 = lower_pos_dev(R.pos, a, b, (int64_t)J.qs + 1);
+  const int64_t a2 = lower_pos_dev(R.pos, 0, a, (int64_t)J.qs - R.scal[2]);
+  int64_t adj = 0;
+  // reads at pos == qs whose span ends before it (no reference-consuming op)
+  for (int64_t r = a + lane; r < a1; r += 64) {
+    const int2 ai = R.ai[r];
+    if ((ai.y & RI_KEEP) && ai.x != 0 && ai.x < J.qs) adj--;
+  }
+  // reads that start before qs and reach it
+  for (int64_t r = a2 + lane; r < a; r += 64) {
+    const int2 ai = R.ai[r];
+    const int32_t se = ai.x != 0 ? ai.x : R.pos[r];
+    if ((ai.y & RI_KEEP) && se >= J.qs) adj++;
+  }
+  adj = wave_sum(adj);
+  // P ops and malformed/edge reads: only scanned when the set has any
+  int64_t unk = 0, bad_r = INT64_MAX, edge_r = INT64_MAX;
+  if (R.scal[4] || R.scal[5]) {
+    for (int64_t r = a2 + lane; r < b; r += 64) {
+      const int2 ai = R.ai[r];
+      if (!(ai.y & RI_KEEP)) continue;
+      const int32_t se = ai.x != 0 ? ai.x : R.pos[r];
+      if (!(R.pos[r] <= J.qe && se >= J.qs)) continue;
+      unk += ((uint32_t)ai.y & RI_UNK_MASK) >> RI_UNK_SHIFT;
+      if ((ai.y & RI_BAD) && r < bad_r) bad_r = r;
+      if ((ai.y & RI_EDGE_CLIP) && r < edge_r) edge_r = r;
+    }
+    unk = wave_sum(unk);
+    for (int d = 32; d >= 1; d >>= 1) {
+      int64_t o1 = __shfl_xor(bad_r, d, 64), o2 = __shfl_xor(edge_r, d, 64);
+      bad_r = o1 < bad_r ? o1 : bad_r;
+      edge_r = o2 < edge_r ? o2 : edge_r;
+    }
+  }
+  if (lane == 0) {
+    S.read_count = (int64_t)kp[b] - (int64_t)kp[a] + adj;
+    S.unknown_ops = unk;
+    if (bad_r != INT64_MAX) atomicMin((unsigned long long*)&S.err_read_inval, (unsigned long long)bad_r);
+    if (edge_r != INT64_MAX) atomicMin((unsigned long long*)&S.err_read_edge, (unsigned long long)edge_r);
+  }
+}
+
+__global__ void k_init_stats(JobStat* st, int32_t n_jobs, uint32_t* counters, int n_counters) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_counters) counters[i] = 0;
+  const int64_t words = (int64_t)n_jobs * (sizeof(JobStat) / 4);
+  for (int64_t k = i; k < words; k += (int64_t)gridDim.x * blockDim.x) ((uint32_t*)st)[k] = 0;
+}
+__global__ void k_init_stats2(JobStat* st, int32_t n_jobs) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n_jobs) { st[j].err_read_edge = INT64_MAX; st[j].err_read_inval = INT64_MAX; }
+}
+
+// ------------------------------------------------------------------ window
+namespace {
+
+struct LdsSink {
+  int W;
+  unsigned long long *bs, *mqq, *ins_, *del_;
+  uint32_t *bad_, *clp_;
+  int32_t *pc, *is;
+  IndelRec* rec;
+  uint32_t* rec_n;
+  uint32_t rec_cap;
+  int32_t* overflow;
+  JobStat* st;
+  int64_t bcount, aligned_n;
+  __device__ void frag(int p, int d, int32_t size) { atomicAdd(&pc[p], d); atomicAdd(&is[p], size); }
+  __device__ void aligned(int n) { aligned_n += n; }
+  __device__ void base(int p, int b, int32_t q, int32_t mq1) {
+    atomicAdd(&bs[b * W + p], (unsigned long long)pack_base(q, mq1));
+    atomicAdd(&mqq[p], (unsigned long long)pack_mq_q(mq1, q));
+  }
+  __device__ void count_base() { bcount++; }
+  __device__ void bad(int p, int n) { atomicAdd(&bad_[p], (uint32_t)n); }
+  __device__ void clip(int p) { atomicAdd(&clp_[p], 1u); }
+  __device__ void emit(const IndelRec& e) {
+    uint32_t i = atomicAdd(rec_n, 1u);
+    if (i < rec_cap) rec[i] = e;
+    else *overflow = 1;
+  }
+  __device__ void ins(int p, int32_t q, int32_t mq1, const IndelRec& e) {
+    atomicAdd(&ins_[p], (unsigned long long)((1ull << 32) + (uint64_t)mq1));
+    atomicAdd(&mqq[p], (unsigned long long)((uint64_t)(int64_t)q << 32));
+    emit(e);
+  }
+  __device__ void del(int p, int32_t q, int32_t mq1, const IndelRec& e) {
+    atomicAdd(&del_[p], (unsigned long long)((1ull << 32) + (uint64_t)mq1));
+    atomicAdd(&mqq[p], (unsigned long long)pack_mq_q(mq1, q));
+    emit(e);
+  }
+  __device__ void edge(int64_t r) { atomicMin((unsigned long long*)&st->err_read_edge, (unsigned long long)r); }
+};
+
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(PL_WIN_BLOCK) k_window(EngineArgs A, int persistent) {
+  extern __shared__ unsigned long long lds[];
+  const int W = A.W;
+  unsigned long long* bs = lds;
+  unsigned long long* mqq = bs + 4 * W;
+  unsigned long long* insA = mqq + W;
+  unsigned long long* delA = insA + W;
+  uint32_t* badA = (uint32_t*)(delA + W);
+  uint32_t* clpA = badA + W;
+  int32_t* pcA = (int32_t*)(clpA + W);
+  int32_t* isA = pcA + W;
+  uint32_t* hist = (uint32_t*)(isA + W);
+  __shared__ int32_t s_w;
+  __shared__ int32_t ws_pc[PL_WIN_BLOCK / 64], ws_is[PL_WIN_BLOCK / 64];
+  __shared__ unsigned long long red[2][PL_WIN_BLOCK / 64];
+  __shared__ int32_t s_ovf;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = PL_WIN_BLOCK / 64;
+  const int lds_words = (72 * W) / 8 + 256 / 2;   // u64 words incl. hist
+
+  for (int iter = 0;; iter++) {
+    int32_t wi;
+    if (persistent) {
+      if (tid == 0) s_w = atomicAdd(A.queue, 1);
+      __syncthreads();
+      const int32_t w = s_w;
+      __syncthreads();
+      if (w >= A.n_wins) break;
+      wi = A.worder[w];
+    } else {
+      if (iter > 0) break;
+      wi = blockIdx.x;
+      if (wi >= A.n_wins) break;
+    }
+    const WinDev win = A.wins[wi];
+    const JobDev J = A.jobs[win.job];
+    const RSDev R = A.rs[J.rs];
+    const int32_t L0 = J.start + win.p0, L1 = J.start + win.p1, nW = win.p1 - win.p0;
+    for (int k = tid; k < lds_words; k += PL_WIN_BLOCK) lds[k] = 0ull;
+    if (tid == 0) s_ovf = 0;
+    __syncthreads();
+
+    LdsSink sk;
+    sk.W = W; sk.bs = bs; sk.mqq = mqq; sk.ins_ = insA; sk.del_ = delA; sk.bad_ = badA; sk.clp_ = clpA;
+    sk.pc = pcA; sk.is = isA; sk.rec = A.rec; sk.rec_n = A.rec_n; sk.rec_cap = A.rec_cap;
+    sk.overflow = A.overflow; sk.st = &A.st[win.job]; sk.bcount = 0; sk.aligned_n = 0;
+
+    for (int64_t r0 = win.lo + wave; r0 < win.hi; r0 += nwaves) {
+      const int64_t r = uniform64(r0);
+      if ((uint32_t)R.ai[r].y & RI_OUTLIER) continue;
+      walk_read(R, J, A.cf, r, L0, L1, lane, 64, sk);
+    }
+    for (int64_t k = wave; k < R.n_outlier; k += nwaves) {
+      const int64_t r = uniform64(R.outliers[k]);
+      walk_read(R, J, A.cf, r, L0, L1, lane, 64, sk);
+    }
+    __syncthreads();
+
+    // inclusive scan of the physCov / insertSize differences over the window
+    {
+      const int per = W / PL_WIN_BLOCK;
+      const int b0 = tid * per;
+      int32_t spc = 0, sis = 0;
+      for (int i = 0; i < per; i++) { spc = j_add(spc, pcA[b0 + i]); sis = j_add(sis, isA[b0 + i]); }
+      int32_t ipc = spc, iis = sis;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        int32_t o1 = __shfl_up(ipc, d, 64), o2 = __shfl_up(iis, d, 64);
+        if (lane >= d) { ipc = j_add(ipc, o1); iis = j_add(iis, o2); }
+      }
+      if (lane == 63) { ws_pc[wave] = ipc; ws_is[wave] = iis; }
+      __syncthreads();
+      int32_t cpc = j_sub(ipc, spc), cis = j_sub(iis, sis);
+      for (int w = 0; w < wave; w++) { cpc = j_add(cpc, ws_pc[w]); cis = j_add(cis, ws_is[w]); }
+      for (int i = 0; i < per; i++) {
+        cpc = j_add(cpc, pcA[b0 + i]);
+        cis = j_add(cis, isA[b0 + i]);
+        pcA[b0 + i] = cpc;
+        isA[b0 + i] = cis;
+      }
+    }
+    __syncthreads();
+
+    const int64_t G = A.G;
+    for (int p = tid; p < nW; p += PL_WIN_BLOCK) {
+      const uint64_t b4[4] = {bs[p], bs[W + p], bs[2 * W + p], bs[3 * W + p]};
+      PosOut o;
+      finalize_position(b4, mqq[p], insA[p], delA[p], badA[p], clpA[p], pcA[p], isA[p], R.ref[L0 + p - 1], A.cf, o);
+      const int64_t g = J.out + win.p0 + p;
+#pragma unroll
+      for (int f = 0; f < PL_NFIELDS; f++) A.pile[(int64_t)f * G + g] = o.f[f];
+      A.calls[g] = o.call;
+      if (o.overflow) s_ovf = 1;
+      if (o.defer) {
+        uint32_t k = atomicAdd(A.defer_n, 1u);
+        if (k < A.defer_cap) A.defer[k] = (uint32_t)g;
+        else *A.overflow = 1;
+      } else {
+        const int d = call_disp(o.call);
+        int64_t dep = (int64_t)(uint32_t)o.f[0] + (uint32_t)o.f[1] + (uint32_t)o.f[2] + (uint32_t)o.f[3] + o.f[PL_F_DELETIONS];
+        if (d == PL_D_CONFIRMED) atomicAdd(&hist[dep > 255 ? 255 : (dep < 0 ? 0 : dep)], 1u);
+        else if (d == PL_D_SNP || d == PL_D_AMB) {
+          uint32_t k = atomicAdd(A.cand_n, 1u);
+          if (k < A.cand_cap) { Cand c; c.gpos = (uint32_t)g; c.depth = (int32_t)dep; c.kind = d; c.len = 0; c.rec = -1; A.cand[k] = c; }
+          else *A.overflow = 1;
+        }
+      }
+    }
+    // region baseCount and the aligned-base unit
+    unsigned long long bc = wave_sum((unsigned long long)sk.bcount);
+    unsigned long long al = wave_sum((unsigned long long)sk.aligned_n);
+    if (lane == 0) { red[0][wave] = bc; red[1][wave] = al; }
+    __syncthreads();
+    JobStat& S = A.st[win.job];
+    if (tid == 0) {
+      unsigned long long t0 = 0, t1 = 0;
+      for (int w = 0; w < nwaves; w++) { t0 += red[0][w]; t1 += red[1][w]; }
+      if (t0) atomicAdd((unsigned long long*)&S.base_count, t0);
+      if (t1) atomicAdd((unsigned long long*)&S.aligned, t1);
+      if (s_ovf) atomicOr(&S.overflow, 1);
+    }
+    for (int d = tid; d < 256; d += PL_WIN_BLOCK)
+      if (hist[d]) atomicAdd((uint32_t*)&S.hist[d], hist[d]);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ records
+__global__ void k_rec_keys(const IndelRec* rec, const uint32_t* rec_n, uint32_t cap, uint32_t* key, uint32_t* val) {
+  const uint32_t n = *rec_n < cap ? *rec_n : cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    key[i] = rec[i].key;
+    val[i] = i;
+  }
+}
+__global__ void k_cand_keys(const Cand* c, const uint32_t* cand_n, uint32_t cap, uint32_t* key, uint32_t* val) {
+  const uint32_t n = *cand_n < cap ? *cand_n : cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    key[i] = c[i].gpos;
+    val[i] = i;
+  }
+}
+__global__ void k_cand_gather(const Cand* in, const uint32_t* cand_n, uint32_t cap, const uint32_t* val, Cand* out) {
+  const uint32_t n = *cand_n < cap ? *cand_n : cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[val[i]];
+}
+
+// ------------------------------------------------------------------ radix sort
+__global__ void __launch_bounds__(256) k_rs_hist(const uint32_t* keys, const uint32_t* n_ptr, uint32_t cap, int shift,
+                                                 uint32_t* hist, int32_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t n = *n_ptr < cap ? *n_ptr : cap;
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int i = 0; i < RS_ITEMS; i++) {
+    const int64_t idx = base + (int64_t)i * 256 + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                                                    const uint32_t* n_ptr, uint32_t cap, int shift,
+                                                    const uint32_t* off, int32_t nblocks) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[4][256];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  run[tid] = off[(int64_t)tid * nblocks + blockIdx.x];
+  for (int w = 0; w < 4; w++) wc[w][tid] = 0;
+  __syncthreads();
+  const uint32_t n = *n_ptr < cap ? *n_ptr : cap;
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int round = 0; round < RS_ITEMS; round++) {
+    const int64_t idx = base + (int64_t)round * 256 + tid;
+    const bool valid = idx < n;
+    const uint32_t k = valid ? kin[idx] : 0u;
+    const uint32_t v = valid ? vin[idx] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = ((d >> b) & 1u) != 0;
+      const uint64_t bb = __ballot(valid && bit);
+      m &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
+    if (valid && rank == 0) wc[wave][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = run[d];
+      for (int w = 0; w < wave; w++) pre += wc[w][d];
+      kout[pre + rank] = k;
+      vout[pre + rank] = v;
+    }
+    __syncthreads();
+    run[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+    for (int w = 0; w < 4; w++) wc[w][tid] = 0;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ indel pass
+namespace {
+__device__ __forceinline__ int32_t job_of(const JobDev* jobs, int32_t n_jobs, int64_t g) {
+  int32_t lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    int32_t m = (lo + hi + 1) >> 1;
+    if (jobs[m].out <= g) lo = m; else hi = m - 1;
+  }
+  return lo;
+}
+}  // namespace
+
+// mode 0: deferred columns (writes call, appends candidates, counts confirmed)
+// mode 1: columns changed by the deletion pass (keeps disposition bits)
+__global__ void k_indel_pass(EngineArgs A, const uint32_t* list, const uint32_t* list_n, uint32_t cap, int mode) {
+  const uint32_t n = *list_n < cap ? *list_n : cap;
+  const uint32_t nrec = *A.rec_n < A.rec_cap ? *A.rec_n : A.rec_cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t g = list[i];
+    const int32_t j = job_of(A.jobs, A.n_jobs, g);
+    const JobDev& J = A.jobs[j];
+    const RSDev& R = A.rs[J.rs];
+    Col c = load_col(A.pile, A.G, g);
+    int32_t ilen;
+    int64_t wrec;
+    const uint8_t refb = R.ref[J.start + (int64_t)(g - J.out) - 1];
+    pl_call cl = full_call(A.rs, J, c, refb, A.rec, A.rkey, A.rval, nrec, g, A.cf, &ilen, &wrec);
+    if (mode == 0) {
+      A.calls[g] = cl;
+      const int d = call_disp(cl);
+      const int64_t dep = c.depth();
+      if (d == PL_D_CONFIRMED) atomicAdd((uint32_t*)&A.st[j].hist[dep > 255 ? 255 : (dep < 0 ? 0 : dep)], 1u);
+      else if (d != PL_D_NONE) {
+        uint32_t k = atomicAdd(A.cand_n, 1u);
+        if (k < A.cand_cap) { Cand cd; cd.gpos = g; cd.depth = (int32_t)dep; cd.kind = d; cd.len = ilen; cd.rec = wrec; A.cand[k] = cd; }
+        else *A.overflow = 1;
+      }
+    } else {
+      pl_call& o = A.calls[g];
+      o.base = cl.base; o.alt = cl.alt; o.q = cl.q;
+      o.flags = (uint16_t)((cl.flags & ~PL_CF_DISP_MASK) | (o.flags & PL_CF_DISP_MASK) | PL_CF_DELETED);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ Pass 1
+__global__ void k_finalize(EngineArgs A, const Cand* sorted, const uint32_t* cand_n, uint32_t cap, Cand* sorted_mut) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= A.n_jobs) return;
+  const uint32_t n = *cand_n < cap ? *cand_n : cap;
+  const JobDev& J = A.jobs[j];
+  // candidates of this job: gpos in [out, out+size)
+  int64_t lo = 0, hi = n;
+  while (lo < hi) { int64_t m = (lo + hi) >> 1; if ((int64_t)sorted[m].gpos < J.out) lo = m + 1; else hi = m; }
+  int64_t a = lo;
+  hi = n;
+  while (lo < hi) { int64_t m = (lo + hi) >> 1; if ((int64_t)sorted[m].gpos < J.out + J.size) lo = m + 1; else hi = m; }
+  const int64_t b = lo;
+  A.st[j].cand_lo = (int32_t)a;
+  A.st[j].cand_hi = (int32_t)b;
+  finalize_job(A.cf, J, A.st[j], A.pile, A.G, A.calls, sorted_mut, a, b, A.mutated, A.mutated_n, A.mutated_cap, A.overflow);
+}
+
+// ------------------------------------------------------------------ export
+// final dispositions of one window's positions (GenomeRegion.scala:255)
+__global__ void k_final_flags(EngineArgs A) {
+  const WinDev win = A.wins[blockIdx.x];
+  const JobDev& J = A.jobs[win.job];
+  const JobStat& S = A.st[win.job];
+  for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
+    const int64_t g = J.out + p;
+    int64_t dep = 0;
+    for (int b = 0; b < 4; b++) dep += (uint32_t)A.pile[(int64_t)(PL_F_COUNT_A + b) * A.G + g];
+    dep += A.pile[(int64_t)PL_F_DELETIONS * A.G + g];
+    uint16_t f = A.calls[g].flags;
+    const int cand = (f & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT;
+    int disp = cand;
+    if (S.read_count == 0 || (f & PL_CF_DELETED) || dep < S.min_depth) disp = PL_D_NONE;
+    A.calls[g].flags = (uint16_t)((f & ~PL_CF_DISP_MASK) | (disp << PL_CF_DISP_SHIFT));
+  }
+}
+
+// insertion strings of INS changes into a byte pool
+__global__ void k_ins_strings(EngineArgs A, const Cand* sorted, const int64_t* idx, const int64_t* off, int32_t n, uint8_t* pool) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Cand& c = sorted[idx[i]];
+  const IndelRec& e = A.rec[c.rec];
+  const int32_t j = job_of(A.jobs, A.n_jobs, c.gpos);
+  const RSDev& R = A.rs[A.jobs[j].rs];
+  for (int32_t q = 0; q < e.len; q++) pool[off[i] + q] = rec_char(R, e, q);
+}

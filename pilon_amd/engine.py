@@ -1,0 +1,246 @@
+"""Python binding of libpilonhip.so with the shape of Pilon's Scala surface.
+
+    ctx = Context(default_config())
+    ctx.ref_upload(0, "ctg", draft_bytes)
+    pr = PileUpRegion(ctx, 0, start, stop)      # new PileUpRegion(name, start, stop)
+    pr.addReads(batch)                           # BamFile.process -> addRead per read
+    pr.postProcess()                             # computePhysCov + GenomeRegion Pass 1
+    pr(i).baseCount, pr(i).baseCall ...          # PileUp fields / BaseCall
+    changes, bases = pr.fix()                    # identifyAndFixIssues + writeChanges
+
+Every call goes through the C ABI; there is no Python or CPU fallback: if
+the HIP library is missing or no GPU is usable, construction raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (PlConfig, PlReadSoa, PlCall, PlRegionStats, PlRegionDesc, PlBatchOpts, PlBatchInfo, NFIELDS, F,
+                  STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK, PL_CF_DISP_SHIFT)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libpilonhip.so")
+
+_lib = None
+
+
+class PilonError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built (run __graft_entry__.build()); no CPU fallback exists")
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        sig = {
+            "pl_config_default": (None, [P(PlConfig)]),
+            "pl_last_error": (C.c_char_p, []),
+            "pl_ctx_create": (C.c_int, [P(PlConfig), C.c_int, P(C.c_void_p)]),
+            "pl_ctx_destroy": (None, [C.c_void_p]),
+            "pl_ref_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_char_p, C.c_int64]),
+            "pl_readset_upload": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa), P(C.c_int32)]),
+            "pl_region_open": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_void_p)]),
+            "pl_region_add_reads": (C.c_int, [C.c_void_p, P(PlReadSoa)]),
+            "pl_region_use_readset": (C.c_int, [C.c_void_p, C.c_int32]),
+            "pl_region_finish": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, P(PlRegionStats)]),
+            "pl_region_fix": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, P(C.c_void_p), P(C.c_int64),
+                                        P(C.c_void_p), P(C.c_int64)]),
+            "pl_region_close": (None, [C.c_void_p]),
+            "pl_free": (None, [C.c_void_p]),
+            "pl_batch_run": (C.c_int, [C.c_void_p, P(PlRegionDesc), C.c_int32, P(PlBatchOpts), C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
+            "pl_last_timing": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double)]),
+            "pl_last_batch_info": (C.c_int, [C.c_void_p, P(PlBatchInfo)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise PilonError(st, lib().pl_last_error().decode(errors="replace"))
+
+
+class Context:
+    """pl_ctx: one GPU, the resident contigs and read sets."""
+
+    def __init__(self, config=None, device=0):
+        self.config = config if config is not None else default_config()
+        h = C.c_void_p()
+        _check(lib().pl_ctx_create(C.byref(self.config), device, C.byref(h)))
+        self.h = h
+        self.contigs = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pl_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def ref_upload(self, contig_id, name, bases):
+        bases = bytes(bases)
+        _check(lib().pl_ref_upload(self.h, contig_id, name.encode(), bases, len(bases)))
+        self.contigs[contig_id] = (name, bases)
+
+    def readset_upload(self, contig_id, reads):
+        """reads: a ReadBatch or a PlReadSoa; returns the resident read-set id."""
+        soa = reads.soa() if hasattr(reads, "soa") else reads
+        rid = C.c_int32()
+        _check(lib().pl_readset_upload(self.h, contig_id, C.byref(soa), C.byref(rid)))
+        return rid.value
+
+    def batch_descs(self, regions):
+        n = len(regions)
+        return (PlRegionDesc * n)(*[PlRegionDesc(c, s, e, r) for c, s, e, r in regions])
+
+    def batch_run(self, regions, window=0, persistent=0, copy_out=True, want_stats=True, descs=None):
+        """regions: list of (contig_id, start, stop, readset_id).  Returns
+        (pile [NFIELDS, G], calls [G], stats list) — arrays are None when
+        copy_out is False (results stay on the device), stats is None when
+        want_stats is False."""
+        n = len(regions)
+        if descs is None:
+            descs = self.batch_descs(regions)
+        opts = PlBatchOpts(window=window, block=0, persistent=persistent, use_graph=0, copy_out=int(copy_out))
+        G = sum(e - s + 1 for _, s, e, _ in regions)
+        stats = (PlRegionStats * n)()
+        pile = calls = None
+        if copy_out:
+            pile = np.zeros((NFIELDS, G), np.int32)
+            calls = np.zeros(G, call_dtype())
+        _check(lib().pl_batch_run(self.h, descs, n, C.byref(opts),
+                                  pile.ctypes.data if copy_out else None,
+                                  calls.ctypes.data if copy_out else None, stats if want_stats else None))
+        return pile, calls, ([stats[i].as_dict() for i in range(n)] if want_stats else None)
+
+    def batch_info(self):
+        info = PlBatchInfo()
+        _check(lib().pl_last_batch_info(self.h, C.byref(info)))
+        return info.as_dict()
+
+    def last_timing(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().pl_last_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+class BaseSum:
+    """A view of one BaseSum (BaseSum.scala:23-74)."""
+
+    def __init__(self, sums):
+        self.sums = [int(x) for x in sums]
+
+    def sum(self):
+        return sum(self.sums)
+
+    def order(self):
+        return sorted(range(4), key=lambda i: -self.sums[i])  # stable: ties keep A<C<G<T
+
+    def maxBase(self):
+        o = self.order()
+        return o[0], self.sums[o[0]], self.sum()
+
+
+class PileUp:
+    """Read-only view of one position (PileUp.scala:25-41) plus its BaseCall."""
+
+    def __init__(self, pile_col, call):
+        f = pile_col
+        self.baseCount = BaseSum(f[F["count_A"]:F["count_T"] + 1])
+        self.qualSum = BaseSum(f[F["qual_A"]:F["qual_T"] + 1])
+        for name in ("mqSum", "qSum", "physCov", "insertSize", "badPair", "deletions", "delQual",
+                     "insertions", "insQual", "clips"):
+            setattr(self, name, int(f[F[name]]))
+        self.call = call
+
+    def count(self):
+        return self.baseCount.sum()
+
+    def depth(self):
+        return self.count() + self.deletions
+
+    @property
+    def baseCall(self):
+        c = self.call
+        return {"base": "ACGTN"[c["base"]], "altBase": "ACGT"[c["alt"]], "q": int(c["q"]),
+                "flags": int(c["flags"]),
+                "disposition": (int(c["flags"]) & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT}
+
+
+class PileUpRegion:
+    """PileUpRegion (PileUpRegion.scala:26) + the GenomeRegion Pass-1 output."""
+
+    def __init__(self, ctx, contig_id, start, stop):
+        self.ctx = ctx
+        self.contig_id = contig_id
+        self.start, self.stop = start, stop
+        self.size = stop + 1 - start
+        h = C.c_void_p()
+        _check(lib().pl_region_open(ctx.h, contig_id, start, stop, C.byref(h)))
+        self.h = h
+        self.pile = None
+        self.calls = None
+        self.stats = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pl_region_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def addReads(self, reads):
+        """BamFile.process for one BAM's worth of reads (query, validateRead, addRead)."""
+        soa = reads.soa() if hasattr(reads, "soa") else reads
+        _check(lib().pl_region_add_reads(self.h, C.byref(soa)))
+
+    def useReadSet(self, readset_id):
+        _check(lib().pl_region_use_readset(self.h, readset_id))
+
+    def postProcess(self):
+        pile = np.zeros((NFIELDS, self.size), np.int32)
+        calls = np.zeros(self.size, call_dtype())
+        st = PlRegionStats()
+        _check(lib().pl_region_finish(self.h, pile.ctypes.data, calls.ctypes.data, C.byref(st)))
+        self.pile, self.calls, self.stats = pile, calls, st.as_dict()
+        return self
+
+    @property
+    def readCount(self):
+        return self.stats["read_count"]
+
+    @property
+    def baseCount(self):
+        return self.stats["base_count"]
+
+    @property
+    def coverage(self):
+        return self.stats["coverage"]
+
+    def __call__(self, i):
+        """apply(i): the PileUp at region index i (PileUpRegion.scala:233)."""
+        return PileUp(self.pile[:, i], self.calls[i])
+
+    def fix(self, new_name=None, offset=0):
+        """identifyAndFixIssues + writeChanges: returns (changes text, fixed bases)."""
+        t, tl, b, bl = C.c_void_p(), C.c_int64(), C.c_void_p(), C.c_int64()
+        _check(lib().pl_region_fix(self.h, new_name.encode() if new_name else None, offset,
+                                   C.byref(t), C.byref(tl), C.byref(b), C.byref(bl)))
+        text = C.string_at(t, tl.value).decode()
+        bases = C.string_at(b, bl.value)
+        lib().pl_free(t)
+        lib().pl_free(b)
+        return text, bases
