@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--window", type=int, default=1024)
     ap.add_argument("--persistent", type=int, default=1)
+    ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-region comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU oracle sample")
     args = ap.parse_args()
@@ -107,7 +108,7 @@ def main():
 
     # one checked pass for the unit (aligned bases) and the sizes
     _, _, stats = ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False,
-                                want_stats=True, descs=descs)
+                                want_stats=True, descs=descs, block=args.block)
     aligned = sum(s["aligned_bases"] for s in stats)
     info = ctx.batch_info()
     log(f"bench: {len(regions)} regions, G={G}, aligned={aligned}, records={info['n_records']}, "
@@ -115,12 +116,12 @@ def main():
 
     for _ in range(args.warmup):
         ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False, want_stats=False,
-                      descs=descs)
+                      descs=descs, block=args.block)
     pipe, win = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False, want_stats=False,
-                      descs=descs)
+                      descs=descs, block=args.block)
         inf = ctx.batch_info()
         pipe.append(inf["pipeline_ms"])
         win.append(inf["window_ms"])
@@ -139,7 +140,7 @@ def main():
         t0 = time.perf_counter()
         dev = 0.0
         for r in regions:
-            ctx.batch_run([r], window=args.window, persistent=0, copy_out=False, want_stats=False)
+            ctx.batch_run([r], window=args.window, persistent=0, copy_out=False, want_stats=False, block=args.block)
             dev += ctx.batch_info()["pipeline_ms"]
         plain_s = time.perf_counter() - t0
         extra["plain_per_region"] = {"wall_ms": plain_s * 1000.0, "device_ms": dev,
@@ -173,7 +174,7 @@ def main():
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "scale": args.scale,
                    "regions": len(regions), "positions": G, "reads": n_reads, "aligned_bases": aligned,
-                   "window": args.window, "persistent": args.persistent},
+                   "window": args.window, "block": args.block, "persistent": args.persistent},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_window",
                      "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms},

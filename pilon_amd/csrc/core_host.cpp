@@ -68,32 +68,27 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
   cf.old_indel = pc->old_indel; cf.fixamb = pc->iupac || pc->fix_amb;
   cf.nonpf = pc->nonpf; cf.duplicates = pc->duplicates; cf.min_depth = pc->min_depth;
   const int64_t n = s->n_reads;
-  std::vector<int4> span((size_t)n + 1);
-  std::vector<int2> ai((size_t)n + 1);
+  std::vector<RMeta> meta((size_t)n + 1);
   RSDev R;
   std::memset(&R, 0, sizeof R);
   R.n = n; R.pos = s->pos; R.tlen = s->tlen; R.flag = s->flag; R.mapq = s->mapq;
   R.cigar_off = s->cigar_off; R.cigar = s->cigar; R.seq_off = s->seq_off;
   R.bases2 = s->bases2; R.quals = s->quals; R.n_exotic = s->n_exotic;
   R.exo_read = s->exo_read; R.exo_off = s->exo_off; R.nib = s->nib;
-  R.span = span.data(); R.ai = ai.data(); R.ref = ref; R.ref_len = ref_len;
+  R.meta = meta.data(); R.ref = ref; R.ref_len = ref_len;
   // ---- read-set prep (k_readset_prep)
   int32_t max_back = 0, max_lead = 0;
   std::vector<int64_t> outliers;
+  for (int64_t r = 0; r < n; r++) meta[r] = prep_read(R, r, cf).m;
   for (int64_t r = 0; r < n; r++) {
-    PrepOut o = prep_read(R, r, cf);
-    span[r] = o.span;
-    ai[r] = o.ai;
-  }
-  for (int64_t r = 0; r < n; r++) {
-    uint32_t info = (uint32_t)ai[r].y;
-    if (!(info & RI_KEEP)) continue;
+    const RMeta& m = meta[r];
+    if (!(m.info & RI_KEEP)) continue;
     int64_t lo = INT64_MAX, hi = INT64_MIN;
-    if (span[r].x <= span[r].y) { lo = span[r].x; hi = span[r].y; }
-    if (span[r].z < span[r].w) { lo = std::min<int64_t>(lo, span[r].z); hi = std::max<int64_t>(hi, (int64_t)span[r].w - 1); }
+    if (m.fp_lo <= m.fp_hi) { lo = m.fp_lo; hi = m.fp_hi; }
+    if (m.fr_lo < m.fr_hi) { lo = std::min<int64_t>(lo, m.fr_lo); hi = std::max<int64_t>(hi, (int64_t)m.fr_hi - 1); }
     if (lo > hi) continue;
     int64_t lead = R.pos[r] - lo, back = hi - R.pos[r];
-    if (lead > PL_MAX_LEAD || back > PL_MAX_BACK) { outliers.push_back(r); ai[r].y = (int)(info | RI_OUTLIER); continue; }
+    if (lead > PL_MAX_LEAD || back > PL_MAX_BACK) { outliers.push_back(r); meta[r].info |= RI_OUTLIER; continue; }
     max_back = std::max<int32_t>(max_back, (int32_t)std::max<int64_t>(back, 0));
     max_lead = std::max<int32_t>(max_lead, (int32_t)std::max<int64_t>(lead, 0));
   }
@@ -127,9 +122,9 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
     JobStat& S = st[j];
     // count (k_jobprep): kept reads overlapping the query
     for (int64_t r = 0; r < n; r++) {
-      uint32_t info = (uint32_t)ai[r].y;
+      uint32_t info = meta[r].info;
       if (!(info & RI_KEEP)) continue;
-      int32_t spanEnd = ai[r].x != 0 ? ai[r].x : R.pos[r];
+      int32_t spanEnd = meta[r].aend != 0 ? meta[r].aend : R.pos[r];
       if (!(R.pos[r] <= J.qe && spanEnd >= J.qs)) continue;
       S.read_count++;
       S.unknown_ops += (info & RI_UNK_MASK) >> RI_UNK_SHIFT;
@@ -148,11 +143,36 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
       sk.bad_ = badA.data(); sk.clp_ = clpA.data(); sk.pc = pcA.data(); sk.is = isA.data(); sk.recs = &recs;
       const int64_t lo = lower_pos(R.pos, n, (int64_t)L0 - max_back);
       const int64_t hi = lower_pos(R.pos, n, (int64_t)L1 + max_lead);
+      // same staging as the window kernel: runs for the flattened loop,
+      // one-wave walks for the rest
+      std::vector<Run> runs;
+      std::vector<int64_t> cplx;
       for (int64_t r = lo; r < hi; r++) {
-        if ((uint32_t)ai[r].y & RI_OUTLIER) continue;
-        walk_read(R, J, cf, r, L0, L1, 0, 1, sk);
+        const RMeta m = meta[r];
+        if ((m.info & RI_OUTLIER) || !read_in_query(m, J)) continue;
+        frag_event(m, J, L0, L1, sk);
+        if (m.fp_hi < L0 || m.fp_lo >= L1) continue;
+        Run rr[2];
+        int got = -1;
+        if (m.info & RI_SIMPLE) got = simple_run(m, R.seq_off[r], cf, L0, L1, rr, &sk.aligned_n, &sk.bcount);
+        else if (!(m.info & RI_EXOTIC)) got = stage_read(R, J, cf, r, m, L0, L1, rr, &sk.aligned_n, &sk.bcount, sk);
+        if (got < 0) cplx.push_back(r);
+        for (int k = 0; k < got; k++) runs.push_back(rr[k]);
       }
-      for (int64_t r : outliers) walk_read(R, J, cf, r, L0, L1, 0, 1, sk);
+      for (const Run& ru : runs)
+        for (int32_t i = 0; i < ru.n; i++) {
+          const int p = ru.p0 + i;
+          if (ru.fl & EF_VALID) {
+            const int64_t q = ru.q + i;
+            const int32_t qual = (ru.fl & EF_NOQUAL) ? cf.default_qual : (int32_t)R.quals[q];
+            const int b = (R.bases2[q >> 2] >> (2 * (q & 3))) & 3;
+            if (qual >= cf.min_qual) sk.base(p, b, qual, (int32_t)((ru.fl >> 16) & 0x1ffu));
+          } else {
+            sk.bad(p, 1);
+          }
+        }
+      for (int64_t r : cplx) walk_read(R, J, cf, r, L0, L1, 0, 1, sk, false);
+      for (int64_t r : outliers) walk_read(R, J, cf, r, L0, L1, 0, 1, sk, true);
       if (sk.edge_read >= 0 && sk.edge_read < S.err_read_edge) S.err_read_edge = sk.edge_read;
       S.base_count += sk.bcount;
       S.aligned += sk.aligned_n;

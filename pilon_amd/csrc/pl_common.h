@@ -19,11 +19,22 @@
 #define RI_BAD          0x00008000u  // CIGAR/SEQ disagree, or op code > 8
 #define RI_UNK_SHIFT    16           // number of 'P' ops, saturated at 255
 #define RI_UNK_MASK     0x00ff0000u
+#define RI_SIMPLE       0x01000000u  // one M/=/X op covering all of SEQ, 2-bit bases
 
 // Margins beyond which a read is an outlier and is handled from the per-set
 // outlier list instead of the sorted scan.
 #define PL_MAX_LEAD 4096
 #define PL_MAX_BACK 8192
+
+// Per-read facts derived once per read set (k_prep), 32 bytes.
+struct __attribute__((aligned(16))) RMeta {
+  int32_t pos;               // getAlignmentStart
+  uint32_t info;             // RI_* bits and adjMq
+  int32_t fp_lo, fp_hi;      // positions the read can touch (M, clips, slid indels); empty if lo > hi
+  int32_t fr_lo, fr_hi;      // physCovIncr fragment [lo, hi); none if lo >= hi
+  int32_t aend;              // getAlignmentEnd (0 for unmapped)
+  uint32_t op0;              // first CIGAR word (0 if none)
+};
 
 struct RSDev {               // one resident read set (one contig, sorted by pos)
   int64_t n;
@@ -40,8 +51,7 @@ struct RSDev {               // one resident read set (one contig, sorted by pos
   const int64_t* exo_read;
   const int64_t* exo_off;
   const uint8_t* nib;
-  int4* span;                // fp_lo, fp_hi (touched positions), fr_lo, fr_hi (fragment [lo,hi))
-  int2* ai;                  // aend (getAlignmentEnd), info word
+  RMeta* meta;               // [n] derived per-read facts
   const uint8_t* ref;        // contig bytes
   int64_t ref_len;
   int64_t n_outlier;

@@ -145,7 +145,7 @@ PL_HD uint32_t del_hash(const uint8_t* ref, int64_t off, int32_t len, uint32_t* 
 // (PileUpRegion.scala:107,139-141), the span of positions the read can touch
 // (M bases, soft clips, slid indels) and its physical-coverage fragment
 // (physCovIncr, PileUpRegion.scala:62-73).
-struct PrepOut { int4 span; int2 ai; int32_t events; };
+struct PrepOut { RMeta m; int32_t events; };
 
 PL_HD PrepOut prep_read(const RSDev& R, int64_t r, const KConf& cf) {
   PrepOut o;
@@ -182,6 +182,11 @@ PL_HD PrepOut prep_read(const RSDev& R, int64_t r, const KConf& cf) {
   if (bad) info |= RI_BAD;
   if (L == 0 && hasS) info |= RI_EDGE_CLIP;
   info |= (uint32_t)(nP > 255 ? 255 : nP) << RI_UNK_SHIFT;
+  const uint32_t op0 = c1 > c0 ? R.cigar[c0] : 0u;
+  if (c1 - c0 == 1 && L > 0 && !bad && !(flag & PL_RF_EXOTIC)) {
+    const int op = (int)(op0 & 15);
+    if (op == 0 || op == 7 || op == 8) info |= RI_SIMPLE;
+  }
 
   int32_t lo = INT_MAX, hi = INT_MIN, events = 0;
   if (!bad) {
@@ -225,40 +230,102 @@ PL_HD PrepOut prep_read(const RSDev& R, int64_t r, const KConf& cf) {
     if (!paired) { flo = pos < aend ? pos : aend; fhi = pos > aend ? pos : aend; }
     else { flo = pos; fhi = j_add(pos, tlen); }
   }
-  o.span = make_int4(lo, hi, flo, fhi);
-  o.ai = make_int2(aend, (int)info);
+  o.m.pos = pos; o.m.info = info; o.m.fp_lo = lo; o.m.fp_hi = hi;
+  o.m.fr_lo = flo; o.m.fr_hi = fhi; o.m.aend = aend; o.m.op0 = op0;
   o.events = events;
   return o;
 }
 
 // ------------------------------------------------------------ the read walk
+// Insertion event of one I op (PileUpRegion.scala:150-162): gate, left
+// normalisation, then addInsertion at the final locus if it lies in the window.
+template <class Sink>
+PL_HD void ins_event(const RSDev& R, const JobDev& J, const ReadAcc& a, int64_t r, int32_t locus, int32_t readOffset,
+                     int32_t len, int32_t mq1, int32_t L0, int32_t L1, Sink& s) {
+  int32_t rot;
+  const int32_t il = slide_insertion(R.ref, a, readOffset, len, locus, &rot);
+  if (il < J.start) { s.edge(r); return; }
+  if (il >= L0 && il < L1) {
+    uint32_t hasN;
+    const uint32_t h = ins_hash(a, readOffset, len, rot, &hasN);
+    IndelRec e;
+    e.key = (uint32_t)(((uint64_t)(J.out + (il - J.start)) << 1) | 0u);
+    e.len = len; e.qual = (int16_t)a.qual(readOffset); e.mq1 = (uint16_t)mq1;
+    e.hash = h; e.rot = rot; e.flags = (a.exotic ? 1u : 0u) | (hasN ? 2u : 0u);
+    e.src = (a.exotic ? a.nib0 : a.seq0) + readOffset;
+    s.ins(il - L0, a.qual(readOffset), mq1, e);
+  }
+}
+
+// Deletion event of one D op (PileUpRegion.scala:163-183): left normalisation
+// with the double-counting add of each slid-over base (remove() is a no-op for
+// valid reads), then addDeletion at the final locus.
+template <class Sink>
+PL_HD void del_event(const RSDev& R, const JobDev& J, const KConf& cf, const ReadAcc& a, int64_t r, int32_t locus,
+                     int32_t readOffset, int32_t len, int32_t L, int32_t mq1, int32_t L0, int32_t L1, Sink& s) {
+  int32_t dl = locus, rl = readOffset;
+  const int32_t flank = cf.flank;
+  while (dl > 1 && rl > 0 && R.ref[dl - 2] == R.ref[dl + len - 2]) {
+    dl--; rl--;
+    const bool trl = rl >= flank && j_sub(L, flank) > rl;
+    if (trl && dl >= J.start && dl <= J.stop && dl + len >= J.start && dl + len <= J.stop) {
+      const int32_t t = dl + len;
+      if (t >= L0 && t < L1) {
+        const int b = a.bi(rl);
+        const int32_t q = a.qual(rl);
+        if (b >= 0 && q >= cf.min_qual) s.base(t - L0, b, q, mq1);
+        s.count_base();
+      }
+    }
+  }
+  if (dl < J.start) { s.edge(r); return; }
+  if (dl >= L0 && dl < L1) {
+    uint32_t hasN;
+    const uint32_t h = del_hash(R.ref, (int64_t)dl - 1, len, &hasN);
+    IndelRec e;
+    e.key = (uint32_t)(((uint64_t)(J.out + (dl - J.start)) << 1) | 1u);
+    e.len = len; e.qual = (int16_t)a.qual(readOffset); e.mq1 = (uint16_t)mq1;
+    e.hash = h; e.rot = 0; e.flags = hasN ? 2u : 0u;
+    e.src = (int64_t)dl - 1;
+    s.del(dl - L0, a.qual(readOffset), mq1, e);
+  }
+}
+
+// Is the read one BamFile.process hands to addRead for this region?
+PL_HD bool read_in_query(const RMeta& m, const JobDev& J) {
+  if (!(m.info & RI_KEEP) || (m.info & RI_BAD)) return false;
+  const int32_t spanEnd = m.aend != 0 ? m.aend : m.pos;
+  return m.pos <= J.qe && spanEnd >= J.qs;                 // queryOverlapping (BamFile.scala:118-119)
+}
+
+// physCovIncr restricted to window [L0, L1): the region's prefix sum of these
+// differences is #{fragments: max(start, region.start) <= i < end}.
+template <class Sink>
+PL_HD void frag_event(const RMeta& m, const JobDev& J, int32_t L0, int32_t L1, Sink& s) {
+  if (m.fr_lo < m.fr_hi) {
+    int32_t a = m.fr_lo > J.start ? m.fr_lo : J.start;
+    a = a > L0 ? a : L0;
+    if (a < m.fr_hi && a < L1) {
+      const int32_t size = j_sub(m.fr_hi, m.fr_lo);
+      s.frag(a - L0, 1, size);
+      if (m.fr_hi < L1) s.frag(m.fr_hi - L0, -1, j_sub(0, size));
+    }
+  }
+}
+
 // PileUpRegion.addRead restricted to the positions of one window [L0, L1)
 // (a slice of region [J.start, J.stop]).  Lanes stride over the bases of an
 // op; lane 0 handles the per-op events.  The Sink receives integer
 // increments only, so the result does not depend on read order.
 template <class Sink>
 PL_HD void walk_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r, int32_t L0, int32_t L1,
-                     int lane, int nlanes, Sink& s) {
-  const int2 ai = R.ai[r];
-  const uint32_t info = (uint32_t)ai.y;
-  if (!(info & RI_KEEP) || (info & RI_BAD)) return;
-  const int32_t pos = R.pos[r];
-  const int32_t aend = ai.x;
-  const int32_t spanEnd = aend != 0 ? aend : pos;
-  if (!(pos <= J.qe && spanEnd >= J.qs)) return;           // queryOverlapping (BamFile.scala:118-119)
-  const int4 sp = R.span[r];
-  // physical coverage (PileUpRegion.scala:62-100): the region's prefix sum of
-  // these differences is #{fragments: max(start, region.start) <= i < end}.
-  if (sp.z < sp.w) {
-    int32_t a = sp.z > J.start ? sp.z : J.start;
-    a = a > L0 ? a : L0;
-    if (a < sp.w && a < L1 && lane == 0) {
-      int32_t size = j_sub(sp.w, sp.z);
-      s.frag(a - L0, 1, size);
-      if (sp.w < L1) s.frag(sp.w - L0, -1, j_sub(0, size));
-    }
-  }
-  if (sp.y < L0 || sp.x >= L1) return;
+                     int lane, int nlanes, Sink& s, bool do_frag = true) {
+  const RMeta m = R.meta[r];
+  if (!read_in_query(m, J)) return;
+  if (do_frag && lane == 0) frag_event(m, J, L0, L1, s);
+  if (m.fp_hi < L0 || m.fp_lo >= L1) return;
+  const uint32_t info = m.info;
+  const int32_t pos = m.pos;
   const bool valid = (info & RI_VALID) != 0;
   const int32_t adjMq = (int32_t)(info & RI_ADJMQ_MASK);
   const int32_t mq1 = adjMq + 1;
@@ -306,56 +373,148 @@ PL_HD void walk_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r
       for (int32_t i = i0 + lane; i < i1; i += nlanes) s.bad(cs + i - L0, 1);
     } else if (op == 1) {                                      // I (:150-162)
       const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
-      if (lane == 0 && valid && tr && locus >= J.start && locus <= J.stop) {
-        int32_t rot;
-        const int32_t il = slide_insertion(R.ref, a, readOffset, len, locus, &rot);
-        if (il < J.start) s.edge(r);
-        else if (il >= L0 && il < L1) {
-          uint32_t hasN;
-          uint32_t h = ins_hash(a, readOffset, len, rot, &hasN);
-          IndelRec e;
-          e.key = (uint32_t)(((uint64_t)(J.out + (il - J.start)) << 1) | 0u);
-          e.len = len; e.qual = (int16_t)a.qual(readOffset); e.mq1 = (uint16_t)mq1;
-          e.hash = h; e.rot = rot; e.flags = (a.exotic ? 1u : 0u) | (hasN ? 2u : 0u);
-          e.src = (a.exotic ? a.nib0 : a.seq0) + readOffset;
-          s.ins(il - L0, a.qual(readOffset), mq1, e);
-        }
-      }
+      if (lane == 0 && valid && tr && locus >= J.start && locus <= J.stop)
+        ins_event(R, J, a, r, locus, readOffset, len, mq1, L0, L1, s);
     } else if (op == 2) {                                      // D (:163-183)
       const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
       if (lane == 0 && valid && tr && locus >= J.start && locus <= J.stop &&
-          locus + len - 1 >= J.start && locus + len - 1 <= J.stop) {
-        int32_t dl = locus, rl = readOffset;
-        while (dl > 1 && rl > 0 && R.ref[dl - 2] == R.ref[dl + len - 2]) {
-          dl--; rl--;
-          const bool trl = rl >= flank && j_sub(L, flank) > rl;
-          // remove() is a no-op for valid reads; add() lands len bases later
-          if (trl && dl >= J.start && dl <= J.stop && dl + len >= J.start && dl + len <= J.stop) {
-            const int32_t t = dl + len;
-            if (t >= L0 && t < L1) {
-              const int b = a.bi(rl);
-              const int32_t q = a.qual(rl);
-              if (b >= 0 && q >= cf.min_qual) s.base(t - L0, b, q, mq1);
-              s.count_base();
-            }
-          }
-        }
-        if (dl < J.start) s.edge(r);
-        else if (dl >= L0 && dl < L1) {
-          uint32_t hasN;
-          uint32_t h = del_hash(R.ref, (int64_t)dl - 1, len, &hasN);
-          IndelRec e;
-          e.key = (uint32_t)(((uint64_t)(J.out + (dl - J.start)) << 1) | 1u);
-          e.len = len; e.qual = (int16_t)a.qual(readOffset); e.mq1 = (uint16_t)mq1;
-          e.hash = h; e.rot = 0; e.flags = hasN ? 2u : 0u;
-          e.src = (int64_t)dl - 1;
-          s.del(dl - L0, a.qual(readOffset), mq1, e);
-        }
-      }
+          locus + len - 1 >= J.start && locus + len - 1 <= J.stop)
+        del_event(R, J, cf, a, r, locus, readOffset, len, L, mq1, L0, L1, s);
     }
     if (consumes_read(op)) readOffset += len;
     if (consumes_ref(op)) refOffset += len;
   }
+}
+
+// ------------------------------------------------------------ flattened runs
+// A run of consecutive bases of one read inside the window for the flattened
+// base loop: M/=/X bases (trusted, valid reads) or positions that only take a
+// badPair increment (clipped bases, bases of invalid reads).
+#define EF_VALID  (1u << 25)
+#define EF_NOQUAL (1u << 26)
+#define EF_BAD    (1u << 27)
+struct Run {
+  int32_t p0;        // window index of the first base
+  int32_t n;
+  int64_t q;         // base-stream index of the first base
+  uint32_t fl;       // mq1 << 16 | EF_* bits
+};
+
+// The run of a single-op read (RI_SIMPLE): its trusted bases in the window.
+PL_HD int simple_run(const RMeta& m, int64_t seq0, const KConf& cf, int32_t L0, int32_t L1, Run* out,
+                     int64_t* aligned, int64_t* bcount) {
+  const int32_t pos = m.pos;
+  const int32_t len = (int32_t)(m.op0 >> 4);   // == L for a simple read
+  const int32_t ab = pos > L0 ? pos : L0, ae = pos + len < L1 ? pos + len : L1;
+  if (ae > ab) *aligned += ae - ab;
+  int32_t o0 = L0 - pos, o1 = L1 - pos;
+  if (o0 < cf.flank) o0 = cf.flank;
+  if (o1 > len - cf.flank) o1 = len - cf.flank;
+  if (o1 <= o0) return 0;
+  const bool valid = (m.info & RI_VALID) != 0;
+  out->p0 = pos + o0 - L0;
+  out->n = o1 - o0;
+  out->q = seq0 + o0;
+  out->fl = valid ? ((((m.info & RI_ADJMQ_MASK) + 1u) << 16) | EF_VALID | ((m.info & RI_NOQUAL) ? EF_NOQUAL : 0u))
+                  : EF_BAD;
+  if (valid) *bcount += o1 - o0;
+  return 1;
+}
+
+// A multi-op read in one thread: at most two runs (M segments or clips) go to
+// the flattened loop, clips and indel events are applied here.  Returns the
+// number of runs, or -1 (nothing done) when the read needs more than two.
+template <class Sink>
+PL_HD int stage_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r, const RMeta& m, int32_t L0,
+                     int32_t L1, Run* runs, int64_t* aligned, int64_t* bcount, Sink& s) {
+  const int64_t c0 = R.cigar_off[r], c1 = R.cigar_off[r + 1];
+  const int64_t seq0 = R.seq_off[r];
+  const int32_t L = (int32_t)(R.seq_off[r + 1] - seq0);
+  const int32_t flank = cf.flank;
+  const int32_t pos = m.pos;
+  int cnt = 0;
+  {
+    int32_t readOffset = 0, refOffset = 0;
+    for (int64_t k = c0; k < c1; k++) {
+      const uint32_t w = R.cigar[k];
+      const int op = (int)(w & 15);
+      const int32_t len = (int32_t)(w >> 4);
+      const int32_t locus = pos + refOffset;
+      if (op == 0 || op == 7 || op == 8) {
+        int32_t i0 = L0 - locus, i1 = len;
+        if (i0 < 0) i0 = 0;
+        if (flank - readOffset > i0) i0 = flank - readOffset;
+        if (L1 - locus < i1) i1 = L1 - locus;
+        if (L - flank - readOffset < i1) i1 = L - flank - readOffset;
+        if (i1 > i0) cnt++;
+      } else if (op == 4) {
+        const int32_t cs = readOffset == 0 ? locus - len : locus;
+        int32_t i0 = L0 - cs, i1 = L1 - cs;
+        if (i0 < 0) i0 = 0;
+        if (i1 > len) i1 = len;
+        if (i1 > i0) cnt++;
+      }
+      if (consumes_read(op)) readOffset += len;
+      if (consumes_ref(op)) refOffset += len;
+    }
+  }
+  if (cnt > 2) return -1;
+  const uint32_t info = m.info;
+  const bool valid = (info & RI_VALID) != 0;
+  const int32_t mq1 = (int32_t)(info & RI_ADJMQ_MASK) + 1;
+  const ReadAcc a = read_acc(R, r, info, cf.default_qual);
+  const uint32_t fl = valid ? (((uint32_t)mq1 << 16) | EF_VALID | ((info & RI_NOQUAL) ? EF_NOQUAL : 0u)) : EF_BAD;
+  int nr = 0;
+  int32_t readOffset = 0, refOffset = 0;
+  for (int64_t k = c0; k < c1; k++) {
+    const uint32_t w = R.cigar[k];
+    const int op = (int)(w & 15);
+    const int32_t len = (int32_t)(w >> 4);
+    const int32_t locus = pos + refOffset;
+    if (op == 0 || op == 7 || op == 8) {
+      const int32_t ab = locus > L0 ? locus : L0, ae = locus + len < L1 ? locus + len : L1;
+      if (ae > ab) *aligned += ae - ab;
+      int32_t i0 = L0 - locus, i1 = len;
+      if (i0 < 0) i0 = 0;
+      if (flank - readOffset > i0) i0 = flank - readOffset;
+      if (L1 - locus < i1) i1 = L1 - locus;
+      if (L - flank - readOffset < i1) i1 = L - flank - readOffset;
+      if (i1 > i0) {
+        runs[nr].p0 = locus + i0 - L0;
+        runs[nr].n = i1 - i0;
+        runs[nr].q = seq0 + readOffset + i0;
+        runs[nr].fl = fl;
+        nr++;
+        if (valid) *bcount += i1 - i0;
+      }
+    } else if (op == 4) {
+      const int32_t cs = readOffset == 0 ? locus - len : locus;
+      const int32_t ce = cs + len - 1;
+      if (cs >= L0 && cs < L1) s.clip(cs - L0);
+      if (ce >= L0 && ce < L1) s.clip(ce - L0);
+      int32_t i0 = L0 - cs, i1 = L1 - cs;
+      if (i0 < 0) i0 = 0;
+      if (i1 > len) i1 = len;
+      if (i1 > i0) {
+        runs[nr].p0 = cs + i0 - L0;
+        runs[nr].n = i1 - i0;
+        runs[nr].q = seq0;
+        runs[nr].fl = EF_BAD;
+        nr++;
+      }
+    } else if (op == 1) {
+      const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
+      if (valid && tr && locus >= J.start && locus <= J.stop)
+        ins_event(R, J, a, r, locus, readOffset, len, mq1, L0, L1, s);
+    } else if (op == 2) {
+      const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
+      if (valid && tr && locus >= J.start && locus <= J.stop && locus + len - 1 >= J.start && locus + len - 1 <= J.stop)
+        del_event(R, J, cf, a, r, locus, readOffset, len, L, mq1, L0, L1, s);
+    }
+    if (consumes_read(op)) readOffset += len;
+    if (consumes_ref(op)) refOffset += len;
+  }
+  return nr;
 }
 
 // ------------------------------------------------------------ column call

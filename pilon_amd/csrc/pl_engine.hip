@@ -238,8 +238,7 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   d.n_exotic = s->n_exotic;
   d.ref = it->second.dev;
   d.ref_len = it->second.len;
-  HIPCHK(hipMalloc(&p, 16 * (size_t)(n + 1))); rs->mem.push_back(p); d.span = (int4*)p;
-  HIPCHK(hipMalloc(&p, 8 * (size_t)(n + 1))); rs->mem.push_back(p); d.ai = (int2*)p;
+  HIPCHK(hipMalloc(&p, sizeof(RMeta) * (size_t)(n + 1))); rs->mem.push_back(p); d.meta = (RMeta*)p;
   HIPCHK(hipMalloc(&p, 64)); rs->mem.push_back(p); d.scal = (int32_t*)p;
   uint32_t* keepflag;
   HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); rs->mem.push_back(p); keepflag = (uint32_t*)p;
@@ -271,7 +270,7 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   d.outliers = outl;
   d.n_outlier = no;
   HIPCHK(hipStreamSynchronize(st));
-  rs->bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 24 * n + 8 * n;
+  rs->bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 32 * n + 8 * n;
   c->rsets.push_back(std::move(rs));
   c->rs_dirty = true;
   *out_id = (int32_t)c->rsets.size() - 1;
@@ -280,9 +279,10 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
 
 struct JobReq { int32_t rs, start, stop; };
 
-pl_status run_jobs(pl_ctx* c, const std::vector<JobReq>& reqs, int32_t W, int persistent) {
+pl_status run_jobs(pl_ctx* c, const std::vector<JobReq>& reqs, int32_t W, int persistent, int block) {
   if (reqs.empty()) return fail(PL_E_INVAL, "empty batch");
-  if (W < PL_WIN_BLOCK || W % PL_WIN_BLOCK || W > 2048) return fail(PL_E_INVAL, "window %d must be a multiple of %d in [%d, 2048]", W, PL_WIN_BLOCK, PL_WIN_BLOCK);
+  if (W < 64 || W % 64 || W > 2048) return fail(PL_E_INVAL, "window %d must be a multiple of 64 in [64, 2048]", W);
+  if (block != 256 && block != 512 && block != 1024) return fail(PL_E_INVAL, "block %d must be 256, 512 or 1024", block);
   c->W = W;
   c->exported_flags = false;
   // ---- jobs and windows
@@ -400,14 +400,24 @@ pl_status run_jobs(pl_ctx* c, const std::vector<JobReq>& reqs, int32_t W, int pe
     k_jobprep<<<dim3(n_jobs), dim3(64), 0, st>>>(A.rs, (const uint32_t* const*)c->kparr.p, A.jobs, A.st);
     HIPCHK(hipGetLastError());
     const size_t lds = 72 * (size_t)W + 1024;
-    HIPCHK(hipFuncSetAttribute((const void*)k_window, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (lds + 32 * (size_t)block + 1024 > 160 * 1024)
+      return fail(PL_E_INVAL, "window %d with block %d needs more than the 160 KB of LDS of a CU", W, block);
+    const void* kfn = block == 256 ? (const void*)k_window<256> : block == 512 ? (const void*)k_window<512>
+                                                                               : (const void*)k_window<1024>;
+    HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int grid = n_wins;
     if (persistent) {
-      const int per_cu = std::max(1, std::min(8, (int)(160 * 1024 / (lds + 512))));
+      // workgroups per CU: LDS (dynamic + 16 B/thread of tile arrays) and 32 waves
+      const size_t per_wg = lds + 32 * (size_t)block + 512;
+      const int by_lds = (int)(160 * 1024 / per_wg);
+      const int by_waves = 32 / (block / 64);
+      const int per_cu = std::max(1, std::min(by_lds, by_waves));
       grid = std::min(n_wins, c->n_cu * per_cu);
     }
     HIPCHK(hipEventRecord(c->ev[2], st));
-    k_window<<<dim3(grid), dim3(PL_WIN_BLOCK), lds, st>>>(A, persistent);
+    if (block == 256) k_window<256><<<dim3(grid), dim3(256), lds, st>>>(A, persistent);
+    else if (block == 512) k_window<512><<<dim3(grid), dim3(512), lds, st>>>(A, persistent);
+    else k_window<1024><<<dim3(grid), dim3(1024), lds, st>>>(A, persistent);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[3], st));
     // evidence records: sort by (position, kind), then the deferred columns
@@ -767,7 +777,7 @@ pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_regio
   }
   std::vector<JobReq> q{{id, r->start, r->stop}};
   const int32_t W = 1024;
-  pl_status s = run_jobs(c, q, W, 0);
+  pl_status s = run_jobs(c, q, W, 0, 512);
   if (!s) s = fetch_batch_meta(c);
   if (!s) s = job_status(c, 0);
   if (!s) s = job_changes(c, 0, &r->stats, &r->changes);
@@ -816,7 +826,8 @@ pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, cons
   }
   const int32_t W = (opts && opts->window) ? opts->window : 1024;
   const int persistent = opts ? opts->persistent : 0;
-  pl_status s = run_jobs(c, q, W, persistent);
+  const int block = (opts && opts->block) ? opts->block : 512;
+  pl_status s = run_jobs(c, q, W, persistent, block);
   if (s) return s;
   s = fetch_batch_meta(c);
   if (s) return s;

@@ -6,3 +6,6 @@
 #ifndef PL_WIN_BLOCK
 #define PL_WIN_BLOCK 256          // threads per window workgroup
 #endif
+#ifndef PL_UNROLL
+#define PL_UNROLL 4               // bases per lane per iteration of the flattened loop
+#endif

@@ -59,20 +59,19 @@ __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, int32_t* scal, int
   const int64_t n = R.n;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     PrepOut o = prep_read(R, r, cf);
-    uint32_t info = (uint32_t)o.ai.y;
-    const bool keep = (info & RI_KEEP) != 0;
+    RMeta m = o.m;
+    const bool keep = (m.info & RI_KEEP) != 0;
     if (keep) {
       int64_t lo = INT64_MAX, hi = INT64_MIN;
-      if (o.span.x <= o.span.y) { lo = o.span.x; hi = o.span.y; }
-      if (o.span.z < o.span.w) {
-        lo = lo < o.span.z ? lo : o.span.z;
-        hi = hi > (int64_t)o.span.w - 1 ? hi : (int64_t)o.span.w - 1;
+      if (m.fp_lo <= m.fp_hi) { lo = m.fp_lo; hi = m.fp_hi; }
+      if (m.fr_lo < m.fr_hi) {
+        lo = lo < m.fr_lo ? lo : m.fr_lo;
+        hi = hi > (int64_t)m.fr_hi - 1 ? hi : (int64_t)m.fr_hi - 1;
       }
-      const int32_t pos = R.pos[r];
       if (lo <= hi) {
-        const int64_t lead = pos - lo, back = hi - pos;
+        const int64_t lead = m.pos - lo, back = hi - m.pos;
         if (lead > PL_MAX_LEAD || back > PL_MAX_BACK) {
-          info |= RI_OUTLIER;
+          m.info |= RI_OUTLIER;
           int k = atomicAdd(&scal[3], 1);
           if (k < out_cap) outliers[k] = r;
         } else {
@@ -80,13 +79,12 @@ __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, int32_t* scal, int
           atomicMax(&scal[1], (int32_t)(lead > 0 ? lead : 0));
         }
       }
-      const int32_t spanEnd = o.ai.x != 0 ? o.ai.x : pos;
-      atomicMax(&scal[2], spanEnd - pos > 0 ? spanEnd - pos : 0);
-      if (info & RI_UNK_MASK) atomicOr(&scal[4], 1);
-      if (info & (RI_BAD | RI_EDGE_CLIP)) atomicOr(&scal[5], 1);
+      const int32_t spanEnd = m.aend != 0 ? m.aend : m.pos;
+      atomicMax(&scal[2], spanEnd - m.pos > 0 ? spanEnd - m.pos : 0);
+      if (m.info & RI_UNK_MASK) atomicOr(&scal[4], 1);
+      if (m.info & (RI_BAD | RI_EDGE_CLIP)) atomicOr(&scal[5], 1);
     }
-    R.span[r] = o.span;
-    R.ai[r] = make_int2(o.ai.x, (int)info);
+    R.meta[r] = m;
     keepflag[r] = keep ? 1u : 0u;
   }
 }
@@ -186,27 +184,27 @@ __global__ void __launch_bounds__(64) k_jobprep(const RSDev* rsets, const uint32
   int64_t adj = 0;
   // reads at pos == qs whose span ends before it (no reference-consuming op)
   for (int64_t r = a + lane; r < a1; r += 64) {
-    const int2 ai = R.ai[r];
-    if ((ai.y & RI_KEEP) && ai.x != 0 && ai.x < J.qs) adj--;
+    const RMeta m = R.meta[r];
+    if ((m.info & RI_KEEP) && m.aend != 0 && m.aend < J.qs) adj--;
   }
   // reads that start before qs and reach it
   for (int64_t r = a2 + lane; r < a; r += 64) {
-    const int2 ai = R.ai[r];
-    const int32_t se = ai.x != 0 ? ai.x : R.pos[r];
-    if ((ai.y & RI_KEEP) && se >= J.qs) adj++;
+    const RMeta m = R.meta[r];
+    const int32_t se = m.aend != 0 ? m.aend : m.pos;
+    if ((m.info & RI_KEEP) && se >= J.qs) adj++;
   }
   adj = wave_sum(adj);
   // P ops and malformed/edge reads: only scanned when the set has any
   int64_t unk = 0, bad_r = INT64_MAX, edge_r = INT64_MAX;
   if (R.scal[4] || R.scal[5]) {
     for (int64_t r = a2 + lane; r < b; r += 64) {
-      const int2 ai = R.ai[r];
-      if (!(ai.y & RI_KEEP)) continue;
-      const int32_t se = ai.x != 0 ? ai.x : R.pos[r];
-      if (!(R.pos[r] <= J.qe && se >= J.qs)) continue;
-      unk += ((uint32_t)ai.y & RI_UNK_MASK) >> RI_UNK_SHIFT;
-      if ((ai.y & RI_BAD) && r < bad_r) bad_r = r;
-      if ((ai.y & RI_EDGE_CLIP) && r < edge_r) edge_r = r;
+      const RMeta m = R.meta[r];
+      if (!(m.info & RI_KEEP)) continue;
+      const int32_t se = m.aend != 0 ? m.aend : m.pos;
+      if (!(m.pos <= J.qe && se >= J.qs)) continue;
+      unk += (m.info & RI_UNK_MASK) >> RI_UNK_SHIFT;
+      if ((m.info & RI_BAD) && r < bad_r) bad_r = r;
+      if ((m.info & RI_EDGE_CLIP) && r < edge_r) edge_r = r;
     }
     unk = wave_sum(unk);
     for (int d = 32; d >= 1; d >>= 1) {
@@ -283,7 +281,18 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(PL_WIN_BLOCK) k_window(EngineArgs A, int persistent) {
+// One workgroup owns a window of W positions; every counter of those
+// positions lives in LDS until the window is done.  Reads are taken in tiles
+// of BLOCK (one per thread): the headers of a tile are loaded in parallel,
+// each read is classified, and the trusted in-window bases of all "simple"
+// reads (one M/=/X op, 2-bit bases) are processed as one flattened stream in
+// which consecutive lanes take consecutive bases — coalesced loads and
+// conflict-free LDS atomics.  Reads with clips, indels or exotic bases go
+// through walk_read, one wave per read.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) {
+  constexpr int NW = BLOCK / 64;
+  constexpr int T = BLOCK;
   extern __shared__ unsigned long long lds[];
   const int W = A.W;
   unsigned long long* bs = lds;
@@ -295,12 +304,19 @@ __global__ void __launch_bounds__(PL_WIN_BLOCK) k_window(EngineArgs A, int persi
   int32_t* pcA = (int32_t*)(clpA + W);
   int32_t* isA = pcA + W;
   uint32_t* hist = (uint32_t*)(isA + W);
+  __shared__ int32_t s_off[2 * T + 1];
+  __shared__ uint32_t s_pi[2 * T];
+  __shared__ int32_t s_q0[2 * T];
+  __shared__ int32_t s_cplx[T];
+  __shared__ int32_t s_ncplx, s_nrun;
+  __shared__ uint64_t ws_r[NW];
   __shared__ int32_t s_w;
-  __shared__ int32_t ws_pc[PL_WIN_BLOCK / 64], ws_is[PL_WIN_BLOCK / 64];
-  __shared__ unsigned long long red[2][PL_WIN_BLOCK / 64];
+  __shared__ int32_t ws_a[NW], ws_b[NW];
+  __shared__ unsigned long long red[2][NW];
   __shared__ int32_t s_ovf;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = PL_WIN_BLOCK / 64;
-  const int lds_words = (72 * W) / 8 + 256 / 2;   // u64 words incl. hist
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lds_words = (72 * W) / 8 + 256 / 2;
+  const KConf cf = A.cf;
 
   for (int iter = 0;; iter++) {
     int32_t wi;
@@ -320,8 +336,8 @@ __global__ void __launch_bounds__(PL_WIN_BLOCK) k_window(EngineArgs A, int persi
     const JobDev J = A.jobs[win.job];
     const RSDev R = A.rs[J.rs];
     const int32_t L0 = J.start + win.p0, L1 = J.start + win.p1, nW = win.p1 - win.p0;
-    for (int k = tid; k < lds_words; k += PL_WIN_BLOCK) lds[k] = 0ull;
-    if (tid == 0) s_ovf = 0;
+    for (int k = tid; k < lds_words; k += BLOCK) lds[k] = 0ull;
+    if (tid == 0) { s_ovf = 0; s_ncplx = 0; }
     __syncthreads();
 
     LdsSink sk;
@@ -329,34 +345,150 @@ __global__ void __launch_bounds__(PL_WIN_BLOCK) k_window(EngineArgs A, int persi
     sk.pc = pcA; sk.is = isA; sk.rec = A.rec; sk.rec_n = A.rec_n; sk.rec_cap = A.rec_cap;
     sk.overflow = A.overflow; sk.st = &A.st[win.job]; sk.bcount = 0; sk.aligned_n = 0;
 
-    for (int64_t r0 = win.lo + wave; r0 < win.hi; r0 += nwaves) {
-      const int64_t r = uniform64(r0);
-      if ((uint32_t)R.ai[r].y & RI_OUTLIER) continue;
-      walk_read(R, J, A.cf, r, L0, L1, lane, 64, sk);
+    for (int64_t t0 = win.lo; t0 < win.hi; t0 += T) {
+      const int64_t qbase = R.seq_off[t0];
+      // ---- stage: one read per thread -> up to two runs
+      const int64_t r = t0 + tid;
+      Run runs[2];
+      int nrun = 0;
+      if (r < win.hi) {
+        const RMeta m = R.meta[r];
+        const int64_t seq0 = R.seq_off[r];
+        if (!(m.info & RI_OUTLIER) && read_in_query(m, J)) {
+          frag_event(m, J, L0, L1, sk);
+          if (m.fp_hi >= L0 && m.fp_lo < L1) {
+            if (m.info & RI_SIMPLE) {
+              nrun = simple_run(m, seq0, cf, L0, L1, runs, &sk.aligned_n, &sk.bcount);
+            } else {
+              int got = -1;
+              if (!(m.info & RI_EXOTIC))
+                got = stage_read(R, J, cf, r, m, L0, L1, runs, &sk.aligned_n, &sk.bcount, sk);
+              if (got < 0) {
+                const int k = atomicAdd(&s_ncplx, 1);
+                s_cplx[k] = tid;
+              } else {
+                nrun = got;
+              }
+            }
+          }
+        }
+      }
+      // ---- exclusive scan of (runs, bases) over the tile
+      {
+        uint64_t v = 0;
+        for (int j = 0; j < nrun; j++) v += (1ull << 32) + (uint64_t)runs[j].n;
+        const uint64_t inc = wave_incl_scan(v);
+        if (lane == 63) ws_r[wave] = inc;
+        __syncthreads();
+        uint64_t ex = inc - v;
+        for (int w2 = 0; w2 < wave; w2++) ex += ws_r[w2];
+        int32_t ei = (int32_t)(ex >> 32), off = (int32_t)(uint32_t)ex;
+        for (int j = 0; j < nrun; j++) {
+          s_off[ei] = off;
+          s_pi[ei] = (uint32_t)runs[j].p0 | runs[j].fl;
+          s_q0[ei] = (int32_t)(runs[j].q - qbase);
+          off += runs[j].n;
+          ei++;
+        }
+        if (tid == T - 1) { s_off[ei] = off; s_nrun = ei; }
+      }
+      __syncthreads();
+      // ---- flattened bases of all runs
+      const int32_t E = s_nrun;
+      const int32_t B = s_off[E];
+      const int32_t per = ((B + NW * 64 - 1) / (NW * 64)) * 64;
+      const int32_t k0 = wave * per;
+      const int32_t k1 = B < k0 + per ? B : k0 + per;
+      if (k0 < k1) {
+        int lo = 0, hi = E;
+        while (lo < hi) {
+          const int m = (lo + hi + 1) >> 1;
+          if (s_off[m] <= k0) lo = m; else hi = m - 1;
+        }
+        int cur = lo;
+        const int32_t dq = cf.default_qual, mq = cf.min_qual;
+        for (int32_t base = k0; base < k1; base += 64 * PL_UNROLL) {
+          int rr = cur;
+          int32_t pj[PL_UNROLL];
+          int64_t qj[PL_UNROLL];
+          uint32_t prj[PL_UNROLL];
+          bool act[PL_UNROLL];
+#pragma unroll
+          for (int j = 0; j < PL_UNROLL; j++) {
+            const int32_t k = base + j * 64 + lane;
+            act[j] = k < k1;
+            pj[j] = 0; qj[j] = qbase; prj[j] = 0;
+            if (act[j]) {
+              while (k >= s_off[rr + 1]) rr++;
+              prj[j] = s_pi[rr];
+              const int32_t d = k - s_off[rr];
+              pj[j] = (int)(prj[j] & 0xffffu) + d;
+              qj[j] = qbase + s_q0[rr] + d;
+            }
+          }
+          int32_t qv[PL_UNROLL];
+          uint32_t bv[PL_UNROLL];
+#pragma unroll
+          for (int j = 0; j < PL_UNROLL; j++) {
+            const bool ld = act[j] && (prj[j] & EF_VALID);
+            qv[j] = ld ? (int32_t)R.quals[qj[j]] : 0;
+            bv[j] = ld ? (uint32_t)R.bases2[qj[j] >> 2] : 0u;
+          }
+#pragma unroll
+          for (int j = 0; j < PL_UNROLL; j++) {
+            if (!act[j]) continue;
+            const uint32_t pr = prj[j];
+            if (pr & EF_VALID) {
+              const int32_t qual = (pr & EF_NOQUAL) ? dq : qv[j];
+              const int b = (int)((bv[j] >> (2 * (qj[j] & 3))) & 3u);
+              if (qual >= mq) {
+                const int32_t mq1 = (int32_t)((pr >> 16) & 0x1ffu);
+                atomicAdd(&bs[b * W + pj[j]], (unsigned long long)pack_base(qual, mq1));
+                atomicAdd(&mqq[pj[j]], (unsigned long long)pack_mq_q(mq1, qual));
+              }
+            } else {
+              atomicAdd(&badA[pj[j]], 1u);
+            }
+          }
+          const int32_t kl = (k1 - 1 < base + 64 * PL_UNROLL - 1) ? k1 - 1 : base + 64 * PL_UNROLL - 1;
+          cur = __shfl(rr, (kl - base) & 63, 64);
+        }
+      }
+      // ---- everything else about these reads, one wave per read
+      const int nc = s_ncplx;
+      for (int i = wave; i < nc; i += NW) {
+        const int64_t rc = uniform64(t0 + s_cplx[i]);
+        walk_read(R, J, cf, rc, L0, L1, lane, 64, sk, false);
+      }
+      __syncthreads();
+      if (tid == 0) s_ncplx = 0;
+      __syncthreads();
     }
-    for (int64_t k = wave; k < R.n_outlier; k += nwaves) {
+    for (int64_t k = wave; k < R.n_outlier; k += NW) {
       const int64_t r = uniform64(R.outliers[k]);
-      walk_read(R, J, A.cf, r, L0, L1, lane, 64, sk);
+      walk_read(R, J, cf, r, L0, L1, lane, 64, sk, true);
     }
     __syncthreads();
 
     // inclusive scan of the physCov / insertSize differences over the window
     {
-      const int per = W / PL_WIN_BLOCK;
+      const int per = (W + BLOCK - 1) / BLOCK;
       const int b0 = tid * per;
       int32_t spc = 0, sis = 0;
-      for (int i = 0; i < per; i++) { spc = j_add(spc, pcA[b0 + i]); sis = j_add(sis, isA[b0 + i]); }
+      for (int i = 0; i < per; i++)
+        if (b0 + i < W) { spc = j_add(spc, pcA[b0 + i]); sis = j_add(sis, isA[b0 + i]); }
       int32_t ipc = spc, iis = sis;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         int32_t o1 = __shfl_up(ipc, d, 64), o2 = __shfl_up(iis, d, 64);
         if (lane >= d) { ipc = j_add(ipc, o1); iis = j_add(iis, o2); }
       }
-      if (lane == 63) { ws_pc[wave] = ipc; ws_is[wave] = iis; }
+      if (lane == 63) { ws_a[wave] = ipc; ws_b[wave] = iis; }
       __syncthreads();
       int32_t cpc = j_sub(ipc, spc), cis = j_sub(iis, sis);
-      for (int w = 0; w < wave; w++) { cpc = j_add(cpc, ws_pc[w]); cis = j_add(cis, ws_is[w]); }
+      for (int w2 = 0; w2 < wave; w2++) { cpc = j_add(cpc, ws_a[w2]); cis = j_add(cis, ws_b[w2]); }
       for (int i = 0; i < per; i++) {
+        if (b0 + i >= W) break;
         cpc = j_add(cpc, pcA[b0 + i]);
         cis = j_add(cis, isA[b0 + i]);
         pcA[b0 + i] = cpc;
@@ -366,10 +498,10 @@ __global__ void __launch_bounds__(PL_WIN_BLOCK) k_window(EngineArgs A, int persi
     __syncthreads();
 
     const int64_t G = A.G;
-    for (int p = tid; p < nW; p += PL_WIN_BLOCK) {
+    for (int p = tid; p < nW; p += BLOCK) {
       const uint64_t b4[4] = {bs[p], bs[W + p], bs[2 * W + p], bs[3 * W + p]};
       PosOut o;
-      finalize_position(b4, mqq[p], insA[p], delA[p], badA[p], clpA[p], pcA[p], isA[p], R.ref[L0 + p - 1], A.cf, o);
+      finalize_position(b4, mqq[p], insA[p], delA[p], badA[p], clpA[p], pcA[p], isA[p], R.ref[L0 + p - 1], cf, o);
       const int64_t g = J.out + win.p0 + p;
 #pragma unroll
       for (int f = 0; f < PL_NFIELDS; f++) A.pile[(int64_t)f * G + g] = o.f[f];
@@ -397,17 +529,21 @@ __global__ void __launch_bounds__(PL_WIN_BLOCK) k_window(EngineArgs A, int persi
     __syncthreads();
     JobStat& S = A.st[win.job];
     if (tid == 0) {
-      unsigned long long t0 = 0, t1 = 0;
-      for (int w = 0; w < nwaves; w++) { t0 += red[0][w]; t1 += red[1][w]; }
-      if (t0) atomicAdd((unsigned long long*)&S.base_count, t0);
-      if (t1) atomicAdd((unsigned long long*)&S.aligned, t1);
+      unsigned long long t0s = 0, t1s = 0;
+      for (int w2 = 0; w2 < NW; w2++) { t0s += red[0][w2]; t1s += red[1][w2]; }
+      if (t0s) atomicAdd((unsigned long long*)&S.base_count, t0s);
+      if (t1s) atomicAdd((unsigned long long*)&S.aligned, t1s);
       if (s_ovf) atomicOr(&S.overflow, 1);
     }
-    for (int d = tid; d < 256; d += PL_WIN_BLOCK)
+    for (int d = tid; d < 256; d += BLOCK)
       if (hist[d]) atomicAdd((uint32_t*)&S.hist[d], hist[d]);
     __syncthreads();
   }
 }
+
+template __global__ void k_window<256>(EngineArgs, int);
+template __global__ void k_window<512>(EngineArgs, int);
+template __global__ void k_window<1024>(EngineArgs, int);
 
 // ------------------------------------------------------------------ records
 __global__ void k_rec_keys(const IndelRec* rec, const uint32_t* rec_n, uint32_t cap, uint32_t* key, uint32_t* val) {

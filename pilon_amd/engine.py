@@ -105,7 +105,7 @@ class Context:
         n = len(regions)
         return (PlRegionDesc * n)(*[PlRegionDesc(c, s, e, r) for c, s, e, r in regions])
 
-    def batch_run(self, regions, window=0, persistent=0, copy_out=True, want_stats=True, descs=None):
+    def batch_run(self, regions, window=0, persistent=0, copy_out=True, want_stats=True, descs=None, block=0):
         """regions: list of (contig_id, start, stop, readset_id).  Returns
         (pile [NFIELDS, G], calls [G], stats list) — arrays are None when
         copy_out is False (results stay on the device), stats is None when
@@ -113,7 +113,7 @@ class Context:
         n = len(regions)
         if descs is None:
             descs = self.batch_descs(regions)
-        opts = PlBatchOpts(window=window, block=0, persistent=persistent, use_graph=0, copy_out=int(copy_out))
+        opts = PlBatchOpts(window=window, block=block, persistent=persistent, use_graph=0, copy_out=int(copy_out))
         G = sum(e - s + 1 for _, s, e, _ in regions)
         stats = (PlRegionStats * n)()
         pile = calls = None

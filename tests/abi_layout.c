@@ -1,0 +1,13 @@
+/* Prints sizeof/offsetof of the C ABI structs so tests can check the ctypes mirror. */
+#include <stddef.h>
+#include <stdio.h>
+#include "../include/pilon_hip.h"
+#define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f))
+int main(void) {
+  printf("pl_config %zu\npl_read_soa %zu\npl_call %zu\npl_region_stats %zu\npl_region_desc %zu\npl_batch_opts %zu\npl_batch_info %zu\n",
+         sizeof(pl_config), sizeof(pl_read_soa), sizeof(pl_call), sizeof(pl_region_stats),
+         sizeof(pl_region_desc), sizeof(pl_batch_opts), sizeof(pl_batch_info));
+  P(pl_config, min_depth); P(pl_config, duplicates); P(pl_read_soa, nib); P(pl_read_soa, n_exotic);
+  P(pl_region_stats, aligned_bases); P(pl_batch_info, window_ms); P(pl_call, q);
+  return 0;
+}

@@ -1,0 +1,170 @@
+"""Hand-derived known-answer cases for Pilon's pileup path.
+
+Each case is a tiny contig + reads whose expected counters were worked out by
+hand from the Scala semantics (file:line cited per case).  The expected values
+are literals, not recomputed by any model, so a dropped term, a wrong offset
+or a swapped operand in either the oracle or the GPU engine fails them.
+Default config: flank 10, minMq 0, minQual 0, defaultQual 10, minMinDepth 5.
+"""
+from pilon_amd.reads import pack_reads
+
+# 36 bp, no homopolymers beyond 'CCC' at 16-18 (used by the deletion case)
+REF_A = "ACGTAGCTAGCTTGACCCATGCATGCAAGTTCAGCA"
+# 40 bp with 'ACAC' at 13-16 for the insertion rotation case
+REF_B = "GCTAGCTAGCTGACACTTGCATGGATCCTGACGTTGCATG"
+# 40 bp for the two-base deletion call; positions 14/16 differ (no slide)
+REF_C = "GATCGATTGCAGTCAGCTTACGGATCCAGTAGCATGCATC"
+
+
+def q_ramp(n, base=20):
+    return [base + i for i in range(n)]
+
+
+def read(pos, cigar, seq, qual=None, flag=0, mapq=60, tlen=0, mate_same_ref=True):
+    return dict(pos=pos, cigar=cigar, seq=seq, qual=qual, flag=flag, mapq=mapq, tlen=tlen,
+                mate_same_ref=mate_same_ref)
+
+
+def case_trusted_flank():
+    """PileUpRegion.scala:118,184-193; PileUp.scala:75-84; physCovIncr :62-88.
+    One unpaired 30M read at 1, quals 30, MAPQ 60: only offsets 10..19 are
+    trusted -> loci 11..20 get count 1, qualSum 30*61, mqSum 61, qSum 30.
+    Unpaired fragment [1, 30): physCov 1 at 1..29, insertSize 29."""
+    reads = [read(1, "30M", REF_A[0:30], [30] * 30)]
+    exp = {}
+    for locus in range(11, 21):
+        b = "ACGT".index(REF_A[locus - 1])
+        exp[locus] = {f"count_{'ACGT'[b]}": 1, f"qual_{'ACGT'[b]}": 1830, "mqSum": 61, "qSum": 30,
+                      "physCov": 1, "insertSize": 29}
+    exp[1] = {"physCov": 1, "insertSize": 29, "mqSum": 0}
+    exp[29] = {"physCov": 1, "insertSize": 29, "mqSum": 0}
+    exp[30] = {"physCov": 0, "insertSize": 0}
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp, stats={"base_count": 10, "read_count": 1})
+
+
+def case_soft_clip():
+    """PileUpRegion.scala:139-141,194-206.  5S25M at 11: clipStart 6, clipEnd
+    10 (clips +1 each), badPair +1 at 6..10; adjMq = roundDiv(60*25, 30) = 50,
+    so mq1 51 and qualSum 30*51 = 1530 at the trusted loci 16..25."""
+    seq = "TTTTT" + REF_A[10:35]
+    reads = [read(11, "5S25M", seq, [30] * 30)]
+    exp = {6: {"clips": 1, "badPair": 1}, 7: {"clips": 0, "badPair": 1}, 10: {"clips": 1, "badPair": 1},
+           11: {"badPair": 0, "mqSum": 0}, 15: {"mqSum": 0}}
+    for locus in range(16, 26):
+        b = REF_A[locus - 1]
+        exp[locus] = {f"count_{b}": 1, f"qual_{b}": 1530, "mqSum": 51, "qSum": 30}
+    exp[26] = {"mqSum": 0}
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp, stats={"base_count": 10})
+
+
+def case_invalid_pair():
+    """PileUpRegion.scala:40-46,107: a paired read without the proper flag is
+    invalid: its trusted bases go to badPair only, no counts, no baseCount,
+    no physical coverage."""
+    reads = [read(1, "30M", REF_A[0:30], [30] * 30, flag=0x1 | 0x40, tlen=200)]
+    exp = {locus: {"badPair": 1, "mqSum": 0, "qSum": 0, "physCov": 0} for locus in range(11, 21)}
+    exp[10] = {"badPair": 0}
+    exp[21] = {"badPair": 0}
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp, stats={"base_count": 0, "read_count": 1})
+
+
+def case_deletion_slide():
+    """PileUpRegion.scala:163-183.  17M1D13M at 1 deletes the third C of CCC
+    (locus 18).  It slides to 16; on each step the slid-over read base is
+    added len bases later (remove() is a no-op for valid reads): the C of
+    offset 16 (q36) lands on 18 and the C of offset 15 (q35) on 17, so 17 and
+    18 count that base twice.  The deletion is recorded at 16 with the
+    pre-normalisation quality quals(17) = 37; it adds mq1 to mqSum."""
+    seq = REF_A[0:17] + REF_A[18:31]
+    reads = [read(1, "17M1D13M", seq, q_ramp(30))]
+    exp = {
+        15: {"count_A": 1, "qual_A": 34 * 61, "mqSum": 61, "qSum": 34},
+        16: {"count_C": 1, "qual_C": 35 * 61, "mqSum": 122, "qSum": 35 + 37, "deletions": 1, "delQual": 61},
+        17: {"count_C": 2, "qual_C": (36 + 35) * 61, "mqSum": 122, "qSum": 71},
+        18: {"count_C": 1, "qual_C": 36 * 61, "mqSum": 61, "qSum": 36, "deletions": 0},
+        19: {"count_A": 1, "qual_A": 37 * 61, "qSum": 37},
+        21: {"count_G": 1, "qual_G": 39 * 61, "qSum": 39},
+        22: {"mqSum": 0},
+    }
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp, stats={"base_count": 12})
+
+
+def case_insertion_rotation(n_reads=6, start=1):
+    """PileUpRegion.scala:150-162; PileUp.scala:98-105,209-246.
+    16M2I12M with inserted AC after locus 16 of ...G ACAC T...: the insertion
+    slides left while the base before it equals its last base, rotating
+    AC->CA->AC->CA->AC, and stops at 13 (G before it).  Six reads: insertions
+    6, insQual 366, qSum at 13 = 6*(32+36) (base q32 + insertion q36); mqSum
+    only counts the bases (insertion adds no mqSum).  insPct 100 > high 65
+    (L=2: middle 43, low 21) -> homozygous insertion AC at 13, fixed."""
+    seq = REF_B[0:16] + "AC" + REF_B[16:28]
+    reads = [read(1, "16M2I12M", seq, q_ramp(30)) for _ in range(n_reads)]
+    exp = {13: {"count_A": n_reads, "qual_A": n_reads * 32 * 61, "mqSum": n_reads * 61,
+                "qSum": n_reads * (32 + 36), "insertions": n_reads, "insQual": n_reads * 61},
+           17: {"insertions": 0, "count_T": n_reads, "qSum": n_reads * 38}}
+    fixed = REF_B[0:12] + "AC" + REF_B[12:]
+    changes = "ctg:13 ctg_pilon:13-14 . AC\n"
+    return dict(ref=REF_B, start=start, stop=40, reads=reads, expect=exp,
+                changes=changes if start == 1 else None, bases=fixed if start == 1 else None,
+                stats={"base_count": n_reads * 8, "ins": 1, "ins_bases": 2, "min_depth": 5})
+
+
+def case_deletion_call():
+    """GenomeRegion.scala:255-264; PileUp.scala:188-191,209-246.  Six reads
+    14M2D16M delete loci 15-16 (no slide: locus 14 'C' != locus 16 'T').  At
+    15: count 0, deletions 6, delPct 100 > 65 -> homozygous DEL 'AG'; Pass 1
+    marks 16 deleted and adds pu.deletions (6) into its pileup."""
+    seq = REF_C[0:14] + REF_C[16:32]
+    reads = [read(1, "14M2D16M", seq, [30] * 30) for _ in range(6)]
+    exp = {15: {"deletions": 6, "delQual": 366, "mqSum": 366, "qSum": 180, "count_A": 0},
+           16: {"deletions": 6, "delQual": 0, "mqSum": 0}}
+    fixed = REF_C[0:14] + REF_C[16:]
+    changes = "ctg:15-16 ctg_pilon:15 %s .\n" % REF_C[14:16]
+    return dict(ref=REF_C, start=1, stop=40, reads=reads, expect=exp, changes=changes, bases=fixed,
+                stats={"base_count": 60, "dels": 1, "del_bases": 2},
+                deleted=[16], disp={15: 4, 16: 0})
+
+
+def case_physcov_pairs():
+    """PileUpRegion.scala:62-100.  Region 101-200 of a 300 bp contig.  A proper
+    pair starting at 50 with TLEN 120 covers [50,170): it enters through
+    physCovStart.  A second starting at 150 with TLEN 100 covers [150,250).
+    Their mates (TLEN < 0) add nothing.  physCov 1/2/1 and insertSize
+    120, (120+100)/2 = 110, 100 on 101-149 / 150-169 / 170-200."""
+    ref = (REF_A * 9)[:300]
+    pf = 0x1 | 0x2
+    reads = [read(50, "30M", ref[49:79], [30] * 30, flag=pf | 0x40 | 0x20, tlen=120),
+             read(140, "30M", ref[139:169], [30] * 30, flag=pf | 0x80 | 0x10, tlen=-120),
+             read(150, "30M", ref[149:179], [30] * 30, flag=pf | 0x40 | 0x20, tlen=100),
+             read(220, "30M", ref[219:249], [30] * 30, flag=pf | 0x80 | 0x10, tlen=-100)]
+    exp = {101: {"physCov": 1, "insertSize": 120}, 149: {"physCov": 1, "insertSize": 120},
+           150: {"physCov": 2, "insertSize": 110}, 169: {"physCov": 2, "insertSize": 110},
+           170: {"physCov": 1, "insertSize": 100}, 200: {"physCov": 1, "insertSize": 100}}
+    return dict(ref=ref, start=101, stop=200, reads=reads, expect=exp, stats={"read_count": 4})
+
+
+def case_n_base():
+    """PileUp.scala:46-52,77 and PileUpRegion.scala:40-43: an N at a trusted
+    offset is counted in the region baseCount but not in the pileup."""
+    seq = REF_A[0:12] + "N" + REF_A[13:30]
+    reads = [read(1, "30M", seq, [30] * 30)]
+    exp = {13: {"mqSum": 0, "qSum": 0, "count_A": 0, "count_C": 0, "count_G": 0, "count_T": 0},
+           12: {"mqSum": 61}}
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp, stats={"base_count": 10})
+
+
+def case_no_qual():
+    """PileUpRegion.scala:114-115: a read without QUAL uses defaultQual 10."""
+    reads = [read(1, "30M", REF_A[0:30], None)]
+    exp = {11: {"qSum": 10, "mqSum": 61}}
+    b = REF_A[10]
+    exp[11][f"qual_{b}"] = 610
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp)
+
+
+ALL = [case_trusted_flank, case_soft_clip, case_invalid_pair, case_deletion_slide, case_insertion_rotation,
+       case_deletion_call, case_physcov_pairs, case_n_base, case_no_qual]
+
+
+def batch_of(case):
+    return pack_reads(case["reads"])

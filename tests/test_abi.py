@@ -1,0 +1,70 @@
+"""The C ABI: libpilonhip.so loads without a GPU, exports every function the
+header declares, and the ctypes mirror has the C layouts."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from pilon_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pilon_hip.h")
+LIB = os.path.join(ROOT, "pilon_amd", "libpilonhip.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pl_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for must in ["pl_ctx_create", "pl_ref_upload", "pl_readset_upload", "pl_region_open", "pl_region_add_reads",
+                 "pl_region_finish", "pl_region_fix", "pl_batch_run", "pl_last_error", "pl_ctx_destroy"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "build first (__graft_entry__.build())"
+    lib = C.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_config_default_matches_pilon():
+    lib = C.CDLL(LIB)
+    c = abi.PlConfig()
+    lib.pl_config_default(C.byref(c))
+    d = abi.default_config()
+    for name, _ in abi.PlConfig._fields_:
+        assert getattr(c, name) == getattr(d, name), name
+    assert (c.flank, c.default_qual, c.min_min_depth, c.min_depth) == (10, 10, 5, 0.1)
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    """Without a usable GPU the context refuses to start (no CPU fallback)."""
+    import torch  # noqa: F401  (only to know whether this host has a GPU)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from pilon_amd.engine import Context, PilonError
+    with pytest.raises(PilonError):
+        Context()
+
+
+def test_struct_layouts_match_c(tmp_path):
+    exe = tmp_path / "abi_layout"
+    subprocess.check_call(["gcc", "-O0", "-o", str(exe), os.path.join(ROOT, "tests", "abi_layout.c")])
+    out = dict(l.rsplit(" ", 1) for l in subprocess.check_output([str(exe)]).decode().strip().split("\n"))
+    sizes = {"pl_config": abi.PlConfig, "pl_read_soa": abi.PlReadSoa, "pl_call": abi.PlCall,
+             "pl_region_stats": abi.PlRegionStats, "pl_region_desc": abi.PlRegionDesc,
+             "pl_batch_opts": abi.PlBatchOpts, "pl_batch_info": abi.PlBatchInfo}
+    for name, cls in sizes.items():
+        assert int(out[name]) == C.sizeof(cls), name
+    for key, val in out.items():
+        if "." in key:
+            sname, field = key.split(".")
+            cls = sizes[sname]
+            assert getattr(cls, field).offset == int(val), key

@@ -1,0 +1,47 @@
+"""The hand-derived known-answer cases through libpilonhip.so on the GPU."""
+import pytest
+
+from pilon_amd.abi import default_config
+from tests import kat_cases as K
+from tests.helpers import assert_same_region, assert_same_stats
+from tests.test_oracle_kat import check_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine():
+    from pilon_amd.engine import Context
+    ctx = Context(default_config(), device=0)
+    yield ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("make", K.ALL, ids=[f.__name__ for f in K.ALL])
+def test_kat_gpu(engine, make):
+    from pilon_amd.engine import PileUpRegion
+    from oracle import oracle as O
+    case = make()
+    print(f"case: {make.__name__}", flush=True)
+    engine.ref_upload(100, "ctg", case["ref"].encode())
+    pr = PileUpRegion(engine, 100, case["start"], case["stop"])
+    pr.addReads(K.batch_of(case))
+    pr.postProcess()
+    changes, bases = pr.fix()
+    res = {"status": 0, "pile": pr.pile, "calls": pr.calls, "stats": pr.stats, "changes": changes, "bases": bases}
+    check_case(res, case)
+    ref = O.run_region(default_config(), case["ref"], case["start"], case["stop"], [K.batch_of(case)], name="ctg")
+    assert_same_region(pr.pile, pr.calls, ref, make.__name__)
+    assert_same_stats(pr.stats, ref["stats"])
+
+
+def test_edge_gpu(engine):
+    from pilon_amd.engine import PileUpRegion, PilonError
+    print("case: insertion slides before region start", flush=True)
+    case = K.case_insertion_rotation(start=14)
+    engine.ref_upload(101, "ctg", case["ref"].encode())
+    pr = PileUpRegion(engine, 101, 14, 40)
+    pr.addReads(K.batch_of(case))
+    with pytest.raises(PilonError) as e:
+        pr.postProcess()
+    assert e.value.status == 4

@@ -51,13 +51,13 @@ def test_c1_full_region(engine):
     assert ref["stats"]["snps"] == 200
 
 
-@pytest.mark.parametrize("window", [256, 512, 1024, 2048])
+@pytest.mark.parametrize("window,block", [(256, 256), (512, 512), (896, 512), (1024, 1024), (2048, 256), (64, 512)])
 @pytest.mark.parametrize("persistent", [0, 1])
-def test_c2_small_batch_windows(engine, window, persistent):
+def test_c2_small_batch_windows(engine, window, block, persistent):
     """Scaled c2 (clips, improper pairs, MAPQ-0 repeats, error indels, N bases)
     as three chunk regions plus the whole contig, in one batch."""
     from pilon_amd.synth import SynthData, preset
-    print(f"case: c2 x0.04 window={window} persistent={persistent}", flush=True)
+    print(f"case: c2 x0.04 window={window} block={block} persistent={persistent}", flush=True)
     p = preset("c2", scale=0.04)
     p["n_rate"] = 0.001
     p["err_indel_rate"] = 0.002
@@ -68,7 +68,7 @@ def test_c2_small_batch_windows(engine, window, persistent):
     L = len(draft)
     regions = [(1, L // 3), (L // 3 + 1, 2 * L // 3), (2 * L // 3 + 1, L), (5000, 5200), (1, L)]
     pile, calls, stats = engine.batch_run([(1, s, e, rid) for s, e in regions], window=window,
-                                          persistent=persistent)
+                                          persistent=persistent, block=block)
     off = 0
     for (s, e), st in zip(regions, stats):
         n = e - s + 1
@@ -128,7 +128,18 @@ def test_deterministic(engine):
     engine.ref_upload(5, d.names[0], draft)
     rid = engine.readset_upload(5, b)
     L = len(draft)
-    out1 = engine.batch_run([(5, 1, L, rid)], window=512, persistent=1)
-    out2 = engine.batch_run([(5, 1, L, rid)], window=2048, persistent=0)
+    out1 = engine.batch_run([(5, 1, L, rid)], window=512, persistent=1, block=256)
+    out2 = engine.batch_run([(5, 1, L, rid)], window=2048, persistent=0, block=256)
     assert np.array_equal(out1[0], out2[0])
     assert np.array_equal(out1[1].view(np.uint64), out2[1].view(np.uint64))
+
+
+def test_window_too_big_for_lds(engine):
+    """A window/block pair that cannot fit a CU's LDS is refused, not launched."""
+    from pilon_amd.engine import PilonError
+    draft = b"ACGT" * 600
+    engine.ref_upload(6, "lds", draft)
+    rid = engine.readset_upload(6, pack_reads([]))
+    with pytest.raises(PilonError) as e:
+        engine.batch_run([(6, 1, len(draft), rid)], window=2048, block=1024)
+    assert e.value.status == 1

@@ -1,0 +1,184 @@
+"""Pins for the oracle: hand-derived known answers, closed forms and
+invariants.  The reference ships no tests or fixtures (it is Scala + htsjdk
+and cannot run here), so these are what fixes the oracle's semantics."""
+import numpy as np
+import pytest
+
+from pilon_amd.abi import default_config, F, D_INS, D_DEL, PL_CF_DELETED, PL_CF_DISP_MASK, PL_CF_DISP_SHIFT
+from pilon_amd.abi import PL_CF_HOMO, PL_CF_INDEL, PL_CF_HOMOINDEL, PL_CF_DELETION, PL_CF_INSERTION
+from tests import kat_cases as K
+
+
+def oracle():
+    from oracle import oracle as O
+    return O
+
+
+def check_case(res, case):
+    assert res["status"] == 0, res
+    start = case["start"]
+    for locus, fields in case["expect"].items():
+        for name, want in fields.items():
+            got = int(res["pile"][F[name], locus - start])
+            assert got == want, f"locus {locus} {name}: got {got} want {want}"
+    for k, v in case.get("stats", {}).items():
+        assert res["stats"][k] == v, (k, res["stats"][k], v)
+    if case.get("changes") is not None:
+        assert res["changes"] == case["changes"]
+    if case.get("bases") is not None:
+        assert res["bases"] == case["bases"].encode()
+    for locus in case.get("deleted", []):
+        assert res["calls"][locus - start]["flags"] & PL_CF_DELETED
+    for locus, d in case.get("disp", {}).items():
+        assert (int(res["calls"][locus - start]["flags"]) & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT == d
+
+
+@pytest.mark.parametrize("make", K.ALL, ids=[f.__name__ for f in K.ALL])
+def test_kat_oracle(make):
+    case = make()
+    res = oracle().run_region(default_config(), case["ref"], case["start"], case["stop"], [K.batch_of(case)])
+    check_case(res, case)
+
+
+def test_insertion_slides_past_region_start_is_edge():
+    """An insertion normalised left of region.start makes Pilon index a
+    negative pileup (PileUpRegion.scala:161): reported as PL_E_EDGE."""
+    case = K.case_insertion_rotation(start=14)
+    res = oracle().run_region(default_config(), case["ref"], 14, 40, [K.batch_of(case)])
+    assert res["status"] == 4
+
+
+def test_clip_without_seq_is_edge():
+    """A soft clip on a read without SEQ indexes bases(rOff) out of range
+    (PileUpRegion.scala:201)."""
+    b = K.pack_reads([K.read(5, "5S20M", "*", None)])
+    res = oracle().run_region(default_config(), K.REF_A, 1, 36, [b])
+    assert res["status"] == 4
+
+
+def test_java_round():
+    """Math.round: ties toward +infinity; 0.49999999999999994 rounds to 0."""
+    O = oracle()
+    assert [O.java_round(x) for x in (0.49999999999999994, 0.5, 1.5, 2.5, -0.5, -1.5, 4.5, 3.5000000000000004)] == \
+        [0, 1, 2, 3, 0, -1, 5, 4]
+
+
+def _bc(**kw):
+    return oracle().basecall(default_config(), **kw)
+
+
+def test_basecall_order_ties():
+    """BaseSum.order is a stable descending sort (BaseSum.scala:57-60): ties
+    keep A<C<G<T; qualSum decides when qSum > 0, else baseCount."""
+    c, _, _ = _bc(bc=[5, 5, 0, 0], qs=[100, 100, 0, 0], mqSum=10, qSum=10)
+    assert (c["base"], c["alt"]) == (0, 1)
+    c, _, _ = _bc(bc=[0, 0, 0, 2], qs=[0, 0, 0, 50], mqSum=10, qSum=10)
+    assert (c["base"], c["alt"]) == (3, 0)
+    c, _, _ = _bc(bc=[1, 3, 3, 0], qs=[0, 0, 0, 0], mqSum=10, qSum=0)
+    assert (c["base"], c["alt"]) == (1, 2)
+    c, _, _ = _bc(bc=[0, 0, 0, 0], qs=[0, 0, 0, 0])
+    assert c["base"] == 4          # 'N' when no bases
+
+
+def test_basecall_scores():
+    """PileUp.scala:142-150,166: two equal alleles -> hetero; score and q by hand.
+    total 200, baseSum 100, altBaseSum 100: homoScore 0, heteroScore 200,
+    score = 200*n/mqSum = 200*10/10 = 200, q = 20."""
+    c, _, score = _bc(bc=[5, 5, 0, 0], qs=[100, 100, 0, 0], mqSum=10, qSum=10)
+    assert score == 200 and c["q"] == 20 and not (c["flags"] & PL_CF_HOMO)
+    # pure column: total 600, homoScore 600, heteroScore 600-|300-600|-|300-0| = 0
+    # -> homo; score = 600*10/610 = 9 (Long division), q = 0
+    c, _, score = _bc(bc=[10, 0, 0, 0], qs=[600, 0, 0, 0], mqSum=610, qSum=60)
+    assert c["flags"] & PL_CF_HOMO and score == 9 and c["q"] == 0
+
+
+@pytest.mark.parametrize("count,flags_want", [
+    (5, "homo"),     # delPct = round(100*10/15) = 67 > high 63 (L=3: middle 42, low 21)
+    (20, "het"),     # 33: 21 <= 33 <= 63
+    (40, "none"),    # 20 < 21
+])
+def test_het_indel_thresholds(count, flags_want):
+    """hetIndelCall (PileUp.scala:229-246) on 10 deletions of length 3."""
+    c, L, _ = _bc(bc=[count, 0, 0, 0], qs=[count * 30, 0, 0, 0], mqSum=100000, qSum=1, deletions=10,
+                  del_list=["AAA"] * 10)
+    if flags_want == "none":
+        assert not (c["flags"] & PL_CF_INDEL)
+    else:
+        assert c["flags"] & PL_CF_DELETION and L == 3
+        assert bool(c["flags"] & PL_CF_HOMOINDEL) == (flags_want == "homo")
+
+
+def test_het_indel_majority_and_n():
+    """A winner needs count > n/2 (5 of 10 is not enough); an N rejects it."""
+    kw = dict(bc=[5, 0, 0, 0], qs=[150, 0, 0, 0], mqSum=100000, qSum=1, deletions=10)
+    c, _, _ = _bc(del_list=["AAA"] * 5 + ["AAAA"] * 5, **kw)
+    assert not (c["flags"] & PL_CF_INDEL)
+    c, L, _ = _bc(del_list=["AAA"] * 6 + ["AAAA"] * 4, **kw)
+    assert c["flags"] & PL_CF_DELETION and L == 3
+    c, _, _ = _bc(bc=[5, 0, 0, 0], qs=[150, 0, 0, 0], mqSum=100000, qSum=1, insertions=10, insQual=0,
+                  ins_list=["ANA"] * 10)
+    assert not (c["flags"] & PL_CF_INDEL)
+    c, L, _ = _bc(bc=[5, 0, 0, 0], qs=[150, 0, 0, 0], mqSum=100000, qSum=1, insertions=10, insQual=0,
+                  ins_list=["ACA"] * 10)
+    assert c["flags"] & PL_CF_INSERTION and L == 3
+
+
+def test_physcov_closed_form():
+    """On synthetic pairs physCov[i] = #{fragments: max(start, region.start)
+    <= i < end} and insertSize = sum(sizes)/physCov, counted directly."""
+    from pilon_amd.synth import SynthData, preset
+    d = SynthData(preset("c2", scale=0.01))
+    draft, b = d.draft(0), d.batch(0)
+    s, e = 5001, 30000
+    res = oracle().run_region(default_config(), draft, s, e, [b])
+    cov = np.zeros(e - s + 1, np.int64)
+    isz = np.zeros(e - s + 1, np.int64)
+    for i in range(b.n):
+        f = int(b.flag[i])
+        if not (f & 1) or not (f & 2) or b.tlen[i] <= 0 or (f & 0x900):
+            continue
+        a, z = int(b.pos[i]), int(b.pos[i]) + int(b.tlen[i])
+        lo, hi = max(a, s), min(z, e + 1)
+        if lo < hi:
+            cov[lo - s:hi - s] += 1
+            isz[lo - s:hi - s] += z - a
+    want_is = np.where(cov > 0, isz // np.maximum(cov, 1), 0)
+    assert np.array_equal(res["pile"][F["physCov"]], cov)
+    assert np.array_equal(res["pile"][F["insertSize"]], want_is)
+
+
+def test_count_equals_aligned_bases_without_flank():
+    """With flank 0, minQual 0, all-valid M-only reads of ACGT bases:
+    sum(count) = region baseCount = M bases in the region."""
+    from pilon_amd.reads import pack_reads
+    rng = np.random.default_rng(5)
+    ref = "".join(rng.choice(list("ACGT"), 2000))
+    reads = []
+    for _ in range(300):
+        p = int(rng.integers(1, 1900))
+        reads.append(K.read(p, "80M", ref[p - 1:p + 79], [30] * 80))
+    b = pack_reads(reads)
+    cfg = default_config(flank=0)
+    res = oracle().run_region(cfg, ref, 501, 1500, [b])
+    total = int(res["pile"][F["count_A"]:F["count_T"] + 1].sum())
+    assert total == res["stats"]["base_count"] == res["stats"]["aligned_bases"]
+
+
+def test_planted_variants_called():
+    """End to end on c1: every planted SNV is a SNP change to the truth base;
+    every planted indel is fixed, except deletions Pilon's slide double count
+    (PileUpRegion.scala:174-178) pushes below its homozygous threshold."""
+    from pilon_amd.synth import SynthData, preset
+    d = SynthData(preset("c1"))
+    draft, truth, b = d.draft(0), d.truth(0), d.batch(0)
+    res = oracle().run_region(default_config(), draft, 1, len(draft), [b], name="ctg")
+    lines = [l.split() for l in res["changes"].strip().split("\n")]
+    pos, kind, ln = d.variants(0)
+    snv = {int(p) + 1 for p, k in zip(pos, kind) if k == 0}
+    snp_changes = {int(l[0].split(":")[1]) for l in lines if l[2] != "." and l[3] != "." and len(l[2]) == 1 and len(l[3]) == 1}
+    assert snv <= snp_changes
+    assert res["stats"]["snps"] == len(snv)
+    ins = [(int(p) + 2, int(n)) for p, k, n in zip(pos, kind, ln) if k == 1]
+    dels = [(int(p) + 2, int(n)) for p, k, n in zip(pos, kind, ln) if k == 2]
+    assert res["stats"]["ins"] == len(ins)
+    assert res["stats"]["dels"] >= len(dels) - 2
