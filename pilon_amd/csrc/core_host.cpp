@@ -80,6 +80,20 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
   int32_t max_back = 0, max_lead = 0;
   std::vector<int64_t> outliers;
   for (int64_t r = 0; r < n; r++) meta[r] = prep_read(R, r, cf).m;
+  // segment table (k_prep counts, k_segfill fills)
+  std::vector<Seg> segs;
+  for (int64_t r = 0; r < n; r++) {
+    RMeta& m = meta[r];
+    if (!((m.info & RI_KEEP) && !(m.info & (RI_SIMPLE | RI_EXOTIC | RI_BAD)))) continue;
+    const int c = build_segs(R, r, m.info, cf, nullptr);
+    const size_t o = segs.size();
+    segs.resize(o + (size_t)c);
+    if (c) build_segs(R, r, m.info, cf, segs.data() + o);
+    m.op0 = (uint32_t)o;
+    m.info = (m.info & ~RI_SEG_MASK) | ((uint32_t)(c < RI_SEG_MAX ? c : RI_SEG_MAX) << RI_SEG_SHIFT);
+  }
+  segs.emplace_back();
+  R.segs = segs.data();
   for (int64_t r = 0; r < n; r++) {
     const RMeta& m = meta[r];
     if (!(m.info & RI_KEEP)) continue;
@@ -155,7 +169,9 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
         Run rr[2];
         int got = -1;
         if (m.info & RI_SIMPLE) got = simple_run(m, R.seq_off[r], cf, L0, L1, rr, &sk.aligned_n, &sk.bcount);
-        else if (!(m.info & RI_EXOTIC)) got = stage_read(R, J, cf, r, m, L0, L1, rr, &sk.aligned_n, &sk.bcount, sk);
+        else if (!(m.info & RI_EXOTIC) && ((m.info & RI_SEG_MASK) >> RI_SEG_SHIFT) < RI_SEG_MAX)
+          got = stage_segs(R, J, cf, r, m, R.seq_off[r], (int32_t)(R.seq_off[r + 1] - R.seq_off[r]), L0, L1, rr[0], rr[1],
+                           &sk.aligned_n, &sk.bcount, sk);
         if (got < 0) cplx.push_back(r);
         for (int k = 0; k < got; k++) runs.push_back(rr[k]);
       }

@@ -20,6 +20,9 @@
 #define RI_UNK_SHIFT    16           // number of 'P' ops, saturated at 255
 #define RI_UNK_MASK     0x00ff0000u
 #define RI_SIMPLE       0x01000000u  // one M/=/X op covering all of SEQ, 2-bit bases
+#define RI_SEG_SHIFT    25           // complex reads: number of segments (RMeta.op0 = first)
+#define RI_SEG_MASK     0xfe000000u
+#define RI_SEG_MAX      127          // more segments: the read is walked instead
 
 // Margins beyond which a read is an outlier and is handled from the per-set
 // outlier list instead of the sorted scan.
@@ -52,6 +55,7 @@ struct RSDev {               // one resident read set (one contig, sorted by pos
   const int64_t* exo_off;
   const uint8_t* nib;
   RMeta* meta;               // [n] derived per-read facts
+  const struct Seg* segs;    // segment table of the complex reads
   const uint8_t* ref;        // contig bytes
   int64_t ref_len;
   int64_t n_outlier;

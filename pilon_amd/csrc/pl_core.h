@@ -426,7 +426,7 @@ PL_HD int simple_run(const RMeta& m, int64_t seq0, const KConf& cf, int32_t L0, 
 // number of runs, or -1 (nothing done) when the read needs more than two.
 template <class Sink>
 PL_HD int stage_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r, const RMeta& m, int32_t L0,
-                     int32_t L1, Run* runs, int64_t* aligned, int64_t* bcount, Sink& s) {
+                     int32_t L1, Run& run0, Run& run1, int64_t* aligned, int64_t* bcount, Sink& s) {
   const int64_t c0 = R.cigar_off[r], c1 = R.cigar_off[r + 1];
   const int64_t seq0 = R.seq_off[r];
   const int32_t L = (int32_t)(R.seq_off[r + 1] - seq0);
@@ -480,10 +480,11 @@ PL_HD int stage_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r
       if (L1 - locus < i1) i1 = L1 - locus;
       if (L - flank - readOffset < i1) i1 = L - flank - readOffset;
       if (i1 > i0) {
-        runs[nr].p0 = locus + i0 - L0;
-        runs[nr].n = i1 - i0;
-        runs[nr].q = seq0 + readOffset + i0;
-        runs[nr].fl = fl;
+        Run& ru = nr == 0 ? run0 : run1;
+        ru.p0 = locus + i0 - L0;
+        ru.n = i1 - i0;
+        ru.q = seq0 + readOffset + i0;
+        ru.fl = fl;
         nr++;
         if (valid) *bcount += i1 - i0;
       }
@@ -496,10 +497,11 @@ PL_HD int stage_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r
       if (i0 < 0) i0 = 0;
       if (i1 > len) i1 = len;
       if (i1 > i0) {
-        runs[nr].p0 = cs + i0 - L0;
-        runs[nr].n = i1 - i0;
-        runs[nr].q = seq0;
-        runs[nr].fl = EF_BAD;
+        Run& ru = nr == 0 ? run0 : run1;
+        ru.p0 = cs + i0 - L0;
+        ru.n = i1 - i0;
+        ru.q = seq0;
+        ru.fl = EF_BAD;
         nr++;
       }
     } else if (op == 1) {
@@ -517,6 +519,196 @@ PL_HD int stage_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r
   return nr;
 }
 
+// ------------------------------------------------------------ segment table
+// The region-independent part of a multi-op read, computed once per read set:
+// its M/S ops, and its insertion/deletion events with Pilon's left
+// normalisation already applied (final locus, rotation, string hash) and,
+// for deletions, each slid-over base that addRead adds len bases later.
+// Region gates (inRegion) are applied at window time.
+enum { SEG_M = 0, SEG_S = 1, SEG_I = 2, SEG_D = 3, SEG_A = 4 };
+struct __attribute__((aligned(16))) Seg {
+  int32_t kq;        // kind | quals(readOffset or rl) << 8
+  int32_t locus;     // M: first locus; S: clipStart; I/D: locus before normalisation; A: slid dloc
+  int32_t roff;      // read offset (A: rl)
+  int32_t len;       // op length (A: deletion length)
+  int32_t fin;       // I: final iloc; D: final dloc; A: base index of read base rl (-1 if not ACGT)
+  uint32_t hash;     // I/D: FNV-1a of the final string
+  int32_t rotn;      // I: rotations | hasN << 30; D: hasN << 30
+  int32_t pad;
+};
+PL_HD int seg_kind(const Seg& g) { return g.kq & 0xff; }
+PL_HD int32_t seg_qual(const Seg& g) { return g.kq >> 8; }
+
+// Writes the segments of read r into out (if non-null); returns their count.
+PL_HD int build_segs(const RSDev& R, int64_t r, uint32_t info, const KConf& cf, Seg* out) {
+  const int64_t c0 = R.cigar_off[r], c1 = R.cigar_off[r + 1];
+  const int32_t L = (int32_t)(R.seq_off[r + 1] - R.seq_off[r]);
+  const int32_t pos = R.pos[r];
+  const bool valid = (info & RI_VALID) != 0;
+  const int32_t flank = cf.flank;
+  const ReadAcc a = read_acc(R, r, info, cf.default_qual);
+  int n = 0;
+  int32_t readOffset = 0, refOffset = 0;
+  for (int64_t k = c0; k < c1; k++) {
+    const uint32_t w = R.cigar[k];
+    const int op = (int)(w & 15);
+    const int32_t len = (int32_t)(w >> 4);
+    const int32_t locus = pos + refOffset;
+    const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
+    if ((op == 0 || op == 7 || op == 8) && len > 0) {
+      if (out) { Seg& g = out[n]; g.kq = SEG_M; g.locus = locus; g.roff = readOffset; g.len = len; g.fin = 0; g.hash = 0; g.rotn = 0; g.pad = 0; }
+      n++;
+    } else if (op == 4 && len > 0) {
+      if (out) {
+        Seg& g = out[n];
+        g.kq = SEG_S; g.locus = readOffset == 0 ? locus - len : locus; g.roff = readOffset; g.len = len;
+        g.fin = 0; g.hash = 0; g.rotn = 0; g.pad = 0;
+      }
+      n++;
+    } else if (op == 1 && len > 0 && valid && tr && locus <= R.ref_len) {
+      if (out) {
+        int32_t rot;
+        const int32_t il = slide_insertion(R.ref, a, readOffset, len, locus, &rot);
+        uint32_t hasN;
+        const uint32_t h = ins_hash(a, readOffset, len, rot, &hasN);
+        Seg& g = out[n];
+        g.kq = SEG_I | (a.qual(readOffset) << 8); g.locus = locus; g.roff = readOffset; g.len = len;
+        g.fin = il; g.hash = h; g.rotn = rot | (int32_t)(hasN << 30); g.pad = 0;
+      }
+      n++;
+    } else if (op == 2 && len > 0 && valid && tr && (int64_t)locus + len - 1 <= R.ref_len) {
+      int32_t dl = locus, rl = readOffset;
+      const int d_at = n++;
+      while (dl > 1 && rl > 0 && R.ref[dl - 2] == R.ref[dl + len - 2]) {
+        dl--; rl--;
+        if (rl >= flank && j_sub(L, flank) > rl) {
+          if (out) {
+            Seg& g = out[n];
+            g.kq = SEG_A | (a.qual(rl) << 8); g.locus = dl; g.roff = rl; g.len = len;
+            g.fin = a.bi(rl); g.hash = 0; g.rotn = 0; g.pad = 0;
+          }
+          n++;
+        }
+      }
+      if (out) {
+        uint32_t hasN;
+        const uint32_t h = del_hash(R.ref, (int64_t)dl - 1, len, &hasN);
+        Seg& g = out[d_at];
+        g.kq = SEG_D | (a.qual(readOffset) << 8); g.locus = locus; g.roff = readOffset; g.len = len;
+        g.fin = dl; g.hash = h; g.rotn = (int32_t)(hasN << 30); g.pad = 0;
+      }
+    }
+    if (consumes_read(op)) readOffset += len;
+    if (consumes_ref(op)) refOffset += len;
+  }
+  return n;
+}
+
+// A multi-op read at window time from its segments: at most two runs go to
+// the flattened loop; clips and indel events are applied here with the
+// region gates of PileUpRegion.addRead.  Returns the number of runs, or -1
+// (nothing done) when the read needs more than two.
+template <class Sink>
+PL_HD int stage_segs(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r, const RMeta& m, int64_t seq0,
+                     int32_t L, int32_t L0, int32_t L1, Run& run0, Run& run1, int64_t* aligned, int64_t* bcount,
+                     Sink& s) {
+  const int ns = (int)((m.info & RI_SEG_MASK) >> RI_SEG_SHIFT);
+  const Seg* sg = R.segs + m.op0;
+  const int32_t flank = cf.flank;
+  int cnt = 0;
+  for (int i = 0; i < ns; i++) {
+    const Seg g = sg[i];
+    const int kd = seg_kind(g);
+    if (kd == SEG_M) {
+      int32_t i0 = L0 - g.locus, i1 = g.len;
+      if (i0 < 0) i0 = 0;
+      if (flank - g.roff > i0) i0 = flank - g.roff;
+      if (L1 - g.locus < i1) i1 = L1 - g.locus;
+      if (L - flank - g.roff < i1) i1 = L - flank - g.roff;
+      if (i1 > i0) cnt++;
+    } else if (kd == SEG_S) {
+      int32_t i0 = L0 - g.locus, i1 = L1 - g.locus;
+      if (i0 < 0) i0 = 0;
+      if (i1 > g.len) i1 = g.len;
+      if (i1 > i0) cnt++;
+    }
+  }
+  if (cnt > 2) return -1;
+  const uint32_t info = m.info;
+  const bool valid = (info & RI_VALID) != 0;
+  const int32_t mq1 = (int32_t)(info & RI_ADJMQ_MASK) + 1;
+  const uint32_t fl = valid ? (((uint32_t)mq1 << 16) | EF_VALID | ((info & RI_NOQUAL) ? EF_NOQUAL : 0u)) : EF_BAD;
+  int nr = 0;
+  bool dgate = false;
+  for (int i = 0; i < ns; i++) {
+    const Seg g = sg[i];
+    const int kd = seg_kind(g);
+    if (kd == SEG_M) {
+      const int32_t ab = g.locus > L0 ? g.locus : L0, ae = g.locus + g.len < L1 ? g.locus + g.len : L1;
+      if (ae > ab) *aligned += ae - ab;
+      int32_t i0 = L0 - g.locus, i1 = g.len;
+      if (i0 < 0) i0 = 0;
+      if (flank - g.roff > i0) i0 = flank - g.roff;
+      if (L1 - g.locus < i1) i1 = L1 - g.locus;
+      if (L - flank - g.roff < i1) i1 = L - flank - g.roff;
+      if (i1 > i0) {
+        Run& ru = nr == 0 ? run0 : run1;
+        ru.p0 = g.locus + i0 - L0; ru.n = i1 - i0; ru.q = seq0 + g.roff + i0; ru.fl = fl;
+        nr++;
+        if (valid) *bcount += i1 - i0;
+      }
+    } else if (kd == SEG_S) {
+      const int32_t cs = g.locus, ce = cs + g.len - 1;
+      if (cs >= L0 && cs < L1) s.clip(cs - L0);
+      if (ce >= L0 && ce < L1) s.clip(ce - L0);
+      int32_t i0 = L0 - cs, i1 = L1 - cs;
+      if (i0 < 0) i0 = 0;
+      if (i1 > g.len) i1 = g.len;
+      if (i1 > i0) {
+        Run& ru = nr == 0 ? run0 : run1;
+        ru.p0 = cs + i0 - L0; ru.n = i1 - i0; ru.q = seq0; ru.fl = EF_BAD;
+        nr++;
+      }
+    } else if (kd == SEG_I) {                                // (:150-162)
+      if (g.locus >= J.start && g.locus <= J.stop) {
+        const int32_t il = g.fin;
+        if (il < J.start) s.edge(r);
+        else if (il >= L0 && il < L1) {
+          IndelRec e;
+          e.key = (uint32_t)(((uint64_t)(J.out + (il - J.start)) << 1) | 0u);
+          e.len = g.len; e.qual = (int16_t)seg_qual(g); e.mq1 = (uint16_t)mq1;
+          e.hash = g.hash; e.rot = g.rotn & 0x3fffffff; e.flags = (g.rotn >> 30) & 1 ? 2u : 0u;
+          e.src = seq0 + g.roff;
+          s.ins(il - L0, seg_qual(g), mq1, e);
+        }
+      }
+    } else if (kd == SEG_D) {                                // (:163-183)
+      const int32_t e1 = g.locus + g.len - 1;
+      dgate = g.locus >= J.start && g.locus <= J.stop && e1 >= J.start && e1 <= J.stop;
+      if (dgate) {
+        const int32_t dl = g.fin;
+        if (dl < J.start) s.edge(r);
+        else if (dl >= L0 && dl < L1) {
+          IndelRec e;
+          e.key = (uint32_t)(((uint64_t)(J.out + (dl - J.start)) << 1) | 1u);
+          e.len = g.len; e.qual = (int16_t)seg_qual(g); e.mq1 = (uint16_t)mq1;
+          e.hash = g.hash; e.rot = 0; e.flags = (g.rotn >> 30) & 1 ? 2u : 0u;
+          e.src = (int64_t)dl - 1;
+          s.del(dl - L0, seg_qual(g), mq1, e);
+        }
+      }
+    } else if (kd == SEG_A) {                                // slide add (:174-178)
+      const int32_t dl = g.locus, t = g.locus + g.len;
+      if (dgate && dl >= J.start && dl <= J.stop && t >= J.start && t <= J.stop && t >= L0 && t < L1) {
+        const int32_t q = seg_qual(g);
+        if (g.fin >= 0 && q >= cf.min_qual) s.base(t - L0, g.fin, q, mq1);
+        s.count_base();
+      }
+    }
+  }
+  return nr;
+}
+
 // ------------------------------------------------------------ column call
 struct Col {                 // one PileUp, Long BaseSums widened
   int64_t bc[4], qs[4];
@@ -525,16 +717,19 @@ struct Col {                 // one PileUp, Long BaseSums widened
   PL_HD int64_t depth() const { return count() + deletions; }
 };
 
-// stable descending order of 0..3 (BaseSum.order, BaseSum.scala:57-60)
+// stable descending order of 0..3 (BaseSum.order, BaseSum.scala:57-60):
+// index i's rank is #{j: s[j] > s[i]} + #{j < i: s[j] == s[i]}; written with
+// scalar selects so no local array needs addressable memory.
 PL_HD void order4(const int64_t s[4], int& o0, int& o1) {
-  int ord[4] = {0, 1, 2, 3};
-  for (int i = 1; i < 4; i++) {
-    int x = ord[i], j = i - 1;
-    while (j >= 0 && s[x] > s[ord[j]]) { ord[j + 1] = ord[j]; j--; }
-    ord[j + 1] = x;
-  }
-  o0 = ord[0]; o1 = ord[1];
+  const int64_t a = s[0], b = s[1], c = s[2], d = s[3];
+  const int r0 = (b > a) + (c > a) + (d > a);
+  const int r1 = (a > b) + (c > b) + (d > b) + (a == b);
+  const int r2 = (a > c) + (b > c) + (d > c) + (a == c) + (b == c);
+  o0 = r0 == 0 ? 0 : r1 == 0 ? 1 : r2 == 0 ? 2 : 3;
+  o1 = r0 == 1 ? 0 : r1 == 1 ? 1 : r2 == 1 ? 2 : 3;
 }
+
+PL_HD int64_t pick4(const int64_t s[4], int i) { return i == 0 ? s[0] : i == 1 ? s[1] : i == 2 ? s[2] : s[3]; }
 
 struct BaseFacts { int base, alt; bool homo; int64_t score, q; };
 
@@ -546,7 +741,7 @@ PL_HD BaseFacts base_facts(const Col& c) {
   order4(c.qSum > 0 ? c.qs : c.bc, o0, o1);
   f.base = n > 0 ? o0 : 4;
   f.alt = o1;
-  const int64_t baseSum = c.qs[o0], altBaseSum = c.qs[o1];
+  const int64_t baseSum = pick4(c.qs, o0), altBaseSum = pick4(c.qs, o1);
   const int64_t total = c.qs[0] + c.qs[1] + c.qs[2] + c.qs[3];
   const int64_t homoScore = baseSum - (total - baseSum);
   const int64_t half = total / 2;
@@ -557,8 +752,14 @@ PL_HD BaseFacts base_facts(const Col& c) {
   f.homo = homoScore >= heteroScore;
   int64_t dd = homoScore - heteroScore;
   dd = dd < 0 ? -dd : dd;
-  f.score = c.mqSum > 0 ? (dd * n) / (int64_t)c.mqSum : 0;
-  f.q = n > 0 ? f.score / n : 0;
+  // (|homo-hetero| * n) / mqSum and score / n as Long; 32-bit when they fit
+  const int64_t num = dd * n;
+  if (c.mqSum <= 0) f.score = 0;
+  else if ((uint64_t)num <= 0xffffffffull) f.score = (int64_t)((uint32_t)num / (uint32_t)c.mqSum);
+  else f.score = num / (int64_t)c.mqSum;
+  if (n <= 0) f.q = 0;
+  else if ((uint64_t)f.score <= 0xffffffffull && (uint64_t)n <= 0xffffffffull) f.q = (int64_t)((uint32_t)f.score / (uint32_t)n);
+  else f.q = f.score / n;
   return f;
 }
 

@@ -75,6 +75,7 @@ struct ReadSet {
   uint32_t* kp = nullptr;
   int32_t scal[8] = {0};
   int64_t bytes = 0;
+  uint32_t nseg = 0;
   ~ReadSet() { for (void* p : mem) (void)hipFree(p); }
 };
 
@@ -240,8 +241,10 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   d.ref_len = it->second.len;
   HIPCHK(hipMalloc(&p, sizeof(RMeta) * (size_t)(n + 1))); rs->mem.push_back(p); d.meta = (RMeta*)p;
   HIPCHK(hipMalloc(&p, 64)); rs->mem.push_back(p); d.scal = (int32_t*)p;
-  uint32_t* keepflag;
-  HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); rs->mem.push_back(p); keepflag = (uint32_t*)p;
+  uint32_t *keepflag, *segcount, *segoff;
+  HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); keepflag = (uint32_t*)p;
+  HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); segcount = (uint32_t*)p;
+  HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); segoff = (uint32_t*)p;
   HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); rs->mem.push_back(p); rs->kp = (uint32_t*)p;
   int64_t ocap = 4096 + n / 256;
   int64_t* outl;
@@ -249,7 +252,7 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   for (int attempt = 0; attempt < 2; attempt++) {
     HIPCHK(hipMemsetAsync(d.scal, 0, 64, st));
     const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (n + 255) / 256), 8192);
-    k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(d, c->cf, keepflag, d.scal, outl, ocap);
+    k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(d, c->cf, keepflag, segcount, d.scal, outl, ocap);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(rs->scal, d.scal, 32, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -259,6 +262,20 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   }
   pl_status ss = scan_u32(c, keepflag, rs->kp, n);
   if (ss) return ss;
+  // segment table of the multi-op reads
+  ss = scan_u32(c, segcount, segoff, n);
+  if (ss) return ss;
+  uint32_t nseg = 0;
+  HIPCHK(hipMemcpyAsync(&nseg, segoff + n, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMalloc(&p, sizeof(Seg) * ((size_t)nseg + 1))); rs->mem.push_back(p);
+  d.segs = (const Seg*)p;
+  {
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (n + 255) / 256), 8192);
+    k_segfill<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(d, c->cf, segoff, (Seg*)p);
+    HIPCHK(hipGetLastError());
+  }
+  rs->nseg = nseg;
   const int64_t no = rs->scal[3];
   if (no > 0) {
     std::vector<int64_t> h((size_t)no);
@@ -270,6 +287,9 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   d.outliers = outl;
   d.n_outlier = no;
   HIPCHK(hipStreamSynchronize(st));
+  (void)hipFree(keepflag);
+  (void)hipFree(segcount);
+  (void)hipFree(segoff);
   rs->bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 32 * n + 8 * n;
   c->rsets.push_back(std::move(rs));
   c->rs_dirty = true;

@@ -7,5 +7,5 @@
 #define PL_WIN_BLOCK 256          // threads per window workgroup
 #endif
 #ifndef PL_UNROLL
-#define PL_UNROLL 4               // bases per lane per iteration of the flattened loop
+#define PL_UNROLL 8               // bases per lane per iteration of the flattened loop
 #endif

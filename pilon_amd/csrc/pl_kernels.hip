@@ -55,11 +55,18 @@ __device__ __forceinline__ T wave_sum(T v) {
 }  // namespace
 
 // ------------------------------------------------------------------ k_prep
-__global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, int32_t* scal, int64_t* outliers, int64_t out_cap) {
+// segments are built for multi-op reads (not RI_SIMPLE) that addRead processes
+PL_HD bool wants_segs(uint32_t info) {
+  return (info & RI_KEEP) && !(info & (RI_SIMPLE | RI_EXOTIC | RI_BAD));
+}
+
+__global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount, int32_t* scal, int64_t* outliers,
+                       int64_t out_cap) {
   const int64_t n = R.n;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     PrepOut o = prep_read(R, r, cf);
     RMeta m = o.m;
+    segcount[r] = wants_segs(m.info) ? (uint32_t)build_segs(R, r, m.info, cf, nullptr) : 0u;
     const bool keep = (m.info & RI_KEEP) != 0;
     if (keep) {
       int64_t lo = INT64_MAX, hi = INT64_MIN;
@@ -86,6 +93,21 @@ __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, int32_t* scal, int
     }
     R.meta[r] = m;
     keepflag[r] = keep ? 1u : 0u;
+  }
+}
+
+// fills the segment table; RMeta.op0 of a multi-op read becomes its first
+// segment and the info word carries the count (RI_SEG_MAX = too many: walked)
+__global__ void k_segfill(RSDev R, KConf cf, const uint32_t* segoff, Seg* segs) {
+  const int64_t n = R.n;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    RMeta m = R.meta[r];
+    if (!wants_segs(m.info)) continue;
+    const uint32_t o = segoff[r], c = segoff[r + 1] - o;
+    if (c) build_segs(R, r, m.info, cf, segs + o);
+    m.op0 = o;
+    m.info = (m.info & ~RI_SEG_MASK) | ((c < RI_SEG_MAX ? c : (uint32_t)RI_SEG_MAX) << RI_SEG_SHIFT);
+    R.meta[r] = m;
   }
 }
 
@@ -316,7 +338,7 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
   __shared__ int32_t s_ovf;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lds_words = (72 * W) / 8 + 256 / 2;
-  const KConf cf = A.cf;
+  const KConf& cf = A.cf;
 
   for (int iter = 0;; iter++) {
     int32_t wi;
@@ -332,9 +354,9 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
       wi = blockIdx.x;
       if (wi >= A.n_wins) break;
     }
-    const WinDev win = A.wins[wi];
-    const JobDev J = A.jobs[win.job];
-    const RSDev R = A.rs[J.rs];
+    const WinDev& win = A.wins[wi];
+    const JobDev& J = A.jobs[win.job];
+    const RSDev& R = A.rs[J.rs];
     const int32_t L0 = J.start + win.p0, L1 = J.start + win.p1, nW = win.p1 - win.p0;
     for (int k = tid; k < lds_words; k += BLOCK) lds[k] = 0ull;
     if (tid == 0) { s_ovf = 0; s_ncplx = 0; }
@@ -349,7 +371,8 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
       const int64_t qbase = R.seq_off[t0];
       // ---- stage: one read per thread -> up to two runs
       const int64_t r = t0 + tid;
-      Run runs[2];
+      Run ru0, ru1;
+      ru0.n = 0; ru1.n = 0;
       int nrun = 0;
       if (r < win.hi) {
         const RMeta m = R.meta[r];
@@ -358,11 +381,14 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
           frag_event(m, J, L0, L1, sk);
           if (m.fp_hi >= L0 && m.fp_lo < L1) {
             if (m.info & RI_SIMPLE) {
-              nrun = simple_run(m, seq0, cf, L0, L1, runs, &sk.aligned_n, &sk.bcount);
+              nrun = simple_run(m, seq0, cf, L0, L1, &ru0, &sk.aligned_n, &sk.bcount);
             } else {
               int got = -1;
-              if (!(m.info & RI_EXOTIC))
-                got = stage_read(R, J, cf, r, m, L0, L1, runs, &sk.aligned_n, &sk.bcount, sk);
+              const uint32_t nseg = (m.info & RI_SEG_MASK) >> RI_SEG_SHIFT;
+              if (!(m.info & RI_EXOTIC) && nseg < RI_SEG_MAX) {
+                const int32_t L = (int32_t)(R.seq_off[r + 1] - seq0);
+                got = stage_segs(R, J, cf, r, m, seq0, L, L0, L1, ru0, ru1, &sk.aligned_n, &sk.bcount, sk);
+              }
               if (got < 0) {
                 const int k = atomicAdd(&s_ncplx, 1);
                 s_cplx[k] = tid;
@@ -375,20 +401,21 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
       }
       // ---- exclusive scan of (runs, bases) over the tile
       {
-        uint64_t v = 0;
-        for (int j = 0; j < nrun; j++) v += (1ull << 32) + (uint64_t)runs[j].n;
+        const uint64_t v = (nrun > 0 ? (1ull << 32) + (uint64_t)ru0.n : 0ull) +
+                           (nrun > 1 ? (1ull << 32) + (uint64_t)ru1.n : 0ull);
         const uint64_t inc = wave_incl_scan(v);
         if (lane == 63) ws_r[wave] = inc;
         __syncthreads();
         uint64_t ex = inc - v;
         for (int w2 = 0; w2 < wave; w2++) ex += ws_r[w2];
         int32_t ei = (int32_t)(ex >> 32), off = (int32_t)(uint32_t)ex;
-        for (int j = 0; j < nrun; j++) {
-          s_off[ei] = off;
-          s_pi[ei] = (uint32_t)runs[j].p0 | runs[j].fl;
-          s_q0[ei] = (int32_t)(runs[j].q - qbase);
-          off += runs[j].n;
-          ei++;
+        if (nrun > 0) {
+          s_off[ei] = off; s_pi[ei] = (uint32_t)ru0.p0 | ru0.fl; s_q0[ei] = (int32_t)(ru0.q - qbase);
+          off += ru0.n; ei++;
+        }
+        if (nrun > 1) {
+          s_off[ei] = off; s_pi[ei] = (uint32_t)ru1.p0 | ru1.fl; s_q0[ei] = (int32_t)(ru1.q - qbase);
+          off += ru1.n; ei++;
         }
         if (tid == T - 1) { s_off[ei] = off; s_nrun = ei; }
       }
