@@ -40,6 +40,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def dev_sync():
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+
+
 def regions_for(name, data, rids):
     """The regions Pilon would process: each contig split in chunks of at most
     10 Mb (GenomeFile.contigRegions, GenomeFile.scala:67-74)."""
@@ -70,9 +76,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--window", type=int, default=1024)
+    ap.add_argument("--window", type=int, default=0, help="positions per window; 0 = autotuned")
     ap.add_argument("--persistent", type=int, default=1)
-    ap.add_argument("--block", type=int, default=512)
+    ap.add_argument("--block", type=int, default=0, help="threads per workgroup; 0 = autotuned")
+    ap.add_argument("--graph", type=int, default=1, help="time the steps as hipGraph replays")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-region comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU oracle sample")
     args = ap.parse_args()
@@ -106,41 +113,53 @@ def main():
     descs = ctx.batch_descs(regions)
     G = sum(e - s + 1 for _, s, e, _ in regions)
 
-    # one checked pass for the unit (aligned bases) and the sizes
+    # one checked pass for the unit (aligned bases) and the sizes; with
+    # window/block 0 it also runs the autotuner's search (cached by shape)
     _, _, stats = ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False,
                                 want_stats=True, descs=descs, block=args.block)
     aligned = sum(s["aligned_bases"] for s in stats)
     info = ctx.batch_info()
+    W, B = info["window"], info["block"]
     log(f"bench: {len(regions)} regions, G={G}, aligned={aligned}, records={info['n_records']}, "
-        f"windows={info['n_windows']}, pipeline {info['pipeline_ms']:.3f} ms, window kernel {info['window_ms']:.3f} ms")
+        f"windows={info['n_windows']}, window {W} block {B}{' (autotuned)' if not (args.window and args.block) else ''}")
+
+    def step(graph):
+        ctx.batch_run(regions, window=W, persistent=args.persistent, copy_out=False, want_stats=False,
+                      descs=descs, block=B, use_graph=bool(graph))
 
     for _ in range(args.warmup):
-        ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False, want_stats=False,
-                      descs=descs, block=args.block)
-    pipe, win = [], []
+        step(args.graph)
+    pipe = []
+    # each step returns after the library's stream has drained; the device
+    # syncs around the loop make the bracket explicit
+    dev_sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.batch_run(regions, window=args.window, persistent=args.persistent, copy_out=False, want_stats=False,
-                      descs=descs, block=args.block)
-        inf = ctx.batch_info()
-        pipe.append(inf["pipeline_ms"])
-        win.append(inf["window_ms"])
+        step(args.graph)
+        pipe.append(ctx.batch_info()["pipeline_ms"])
+    dev_sync()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = aligned / (ms_per_step / 1000.0)
     pipe_ms = float(np.mean(pipe))
+    # the window kernel's own time: HIP events around it on the library's
+    # stream, in eager launches of the same batch (a graph replay has none)
+    win = []
+    for _ in range(max(3, args.steps // 2)):
+        step(False)
+        win.append(ctx.batch_info()["window_ms"])
     win_ms = float(np.mean(win))
     b_alg = alg_read_bytes + G * 80.25 + 32 * info["n_records"]
     achieved = b_alg / (win_ms / 1000.0) / 1e9
-    log(f"bench: {ms_per_step:.3f} ms/step wall, device pipeline {pipe_ms:.3f} ms, window kernel {win_ms:.3f} ms, "
-        f"{value / 1e9:.2f} G bases/s, window kernel {achieved:.0f} GB/s algorithmic")
+    log(f"bench: {ms_per_step:.3f} ms/step wall ({'graph' if args.graph else 'eager'}), device pipeline {pipe_ms:.3f} ms, "
+        f"window kernel {win_ms:.3f} ms, {value / 1e9:.2f} G bases/s, window kernel {achieved:.0f} GB/s algorithmic")
 
     extra = {}
     if not args.no_plain:
         t0 = time.perf_counter()
         dev = 0.0
         for r in regions:
-            ctx.batch_run([r], window=args.window, persistent=0, copy_out=False, want_stats=False, block=args.block)
+            ctx.batch_run([r], window=512, persistent=0, copy_out=False, want_stats=False, block=256)
             dev += ctx.batch_info()["pipeline_ms"]
         plain_s = time.perf_counter() - t0
         extra["plain_per_region"] = {"wall_ms": plain_s * 1000.0, "device_ms": dev,
@@ -174,7 +193,8 @@ def main():
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "scale": args.scale,
                    "regions": len(regions), "positions": G, "reads": n_reads, "aligned_bases": aligned,
-                   "window": args.window, "block": args.block, "persistent": args.persistent},
+                   "window": W, "block": B, "autotuned": not (args.window and args.block),
+                   "persistent": args.persistent, "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_window",
                      "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms},

@@ -226,10 +226,61 @@ typedef struct pl_batch_info {
   int64_t n_deferred;      /* columns whose BaseCall needed indel strings */
   int32_t window;          /* window size used                            */
   int32_t persistent;
+  int32_t block;           /* workgroup size used                         */
+  int32_t use_graph;       /* 1 = the pipeline ran as a graph replay      */
   double  pipeline_ms;     /* whole device pipeline                       */
-  double  window_ms;       /* window kernel alone                         */
+  double  window_ms;       /* window kernel alone; -1 after a graph replay */
 } pl_batch_info;
 pl_status pl_last_batch_info(pl_ctx* ctx, pl_batch_info* info);
+
+/* ------------------------------------------------------- serving form */
+/* A request loop over the contigs and read sets resident in the context.
+ * Polish requests (one region each) arrive at any time.  Each lane is an
+ * independent batch pipeline on its own stream, driven by a host thread; an
+ * idle lane takes every waiting request that fits beside the requests in
+ * flight (estimated device bytes against mem_budget, FIFO, up to max_batch)
+ * and runs them as one batch, so requests that arrive while others are on
+ * the GPU join the next free lane and batches overlap on the device
+ * (continuous batching).  A request's result is identical alone or in any
+ * batch.  Each request is GenomeFile.processRegions' loop body for one region
+ * (GenomeFile.scala:101-120); the serving loop has no reference counterpart.
+ * While a server is open the context's contigs and read sets are fixed
+ * (uploads return PL_E_BUSY). */
+typedef struct pl_serve_opts {
+  int32_t lanes;           /* concurrent batches; 0 = 2 (at most 16)                  */
+  int32_t max_batch;       /* requests per batch; 0 = 256                             */
+  int64_t mem_budget;      /* device bytes for in-flight requests; 0 = 1/4 of free     */
+  int32_t max_queue;       /* waiting requests before PL_E_BUSY; 0 = 65536            */
+  int32_t window, block;   /* launch shape; 0 = 512 / 256                             */
+  int32_t copy_out;        /* 1 = keep each result's pileup and calls for pl_serve_fetch */
+} pl_serve_opts;
+
+typedef struct pl_serve_result {
+  int64_t ticket;
+  int64_t tag;             /* the caller's tag from pl_serve_submit                   */
+  int32_t status;          /* pl_status of this request                               */
+  int32_t batch_size;      /* requests in the batch it ran in                         */
+  pl_region_stats stats;
+  int64_t est_bytes;       /* the admission estimate                                  */
+  double  queue_ms;        /* submit -> its batch started                             */
+  double  latency_ms;      /* submit -> result ready                                  */
+  double  batch_device_ms; /* device time of its batch (HIP events)                   */
+} pl_serve_result;
+
+typedef struct pl_server pl_server;
+pl_status pl_serve_open(pl_ctx* ctx, const pl_serve_opts* opts, pl_server** out);
+/* PL_E_REJECTED: the request alone exceeds mem_budget; PL_E_BUSY: queue full. */
+pl_status pl_serve_submit(pl_server* srv, const pl_region_desc* req, int64_t tag, int64_t* ticket);
+/* Up to max finished requests; waits up to wait_ms (-1 = until one finishes
+ * or nothing is outstanding, 0 = no wait). */
+pl_status pl_serve_poll(pl_server* srv, pl_serve_result* out, int32_t max, int32_t wait_ms, int32_t* n_out);
+/* Pileup (PL_NFIELDS*G, field-major) and calls of a polled request of a
+ * copy_out server; the kept copy is freed afterwards. */
+pl_status pl_serve_fetch(pl_server* srv, int64_t ticket, int32_t* pileup, pl_call* calls);
+/* Waits until every submitted request has finished. */
+pl_status pl_serve_drain(pl_server* srv);
+/* Drains the queue, stops the lanes and frees their device memory. */
+void      pl_serve_close(pl_server* srv);
 
 #ifdef __cplusplus
 }

@@ -92,11 +92,27 @@ def call_dtype():
     return np.dtype([("base", np.uint8), ("alt", np.uint8), ("flags", np.uint16), ("q", np.int32)])
 
 
+class PlServeOpts(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("max_batch", C.c_int32), ("mem_budget", C.c_int64),
+                ("max_queue", C.c_int32), ("window", C.c_int32), ("block", C.c_int32), ("copy_out", C.c_int32)]
+
+
+class PlServeResult(C.Structure):
+    _fields_ = [("ticket", C.c_int64), ("tag", C.c_int64), ("status", C.c_int32), ("batch_size", C.c_int32),
+                ("stats", PlRegionStats), ("est_bytes", C.c_int64), ("queue_ms", C.c_double),
+                ("latency_ms", C.c_double), ("batch_device_ms", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["stats"] = self.stats.as_dict()
+        return d
+
+
 class PlBatchInfo(C.Structure):
     _fields_ = [("positions", C.c_int64), ("n_regions", C.c_int32), ("n_windows", C.c_int32),
                 ("n_records", C.c_int64), ("n_candidates", C.c_int64), ("n_deferred", C.c_int64),
-                ("window", C.c_int32), ("persistent", C.c_int32), ("pipeline_ms", C.c_double),
-                ("window_ms", C.c_double)]
+                ("window", C.c_int32), ("persistent", C.c_int32), ("block", C.c_int32), ("use_graph", C.c_int32),
+                ("pipeline_ms", C.c_double), ("window_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -102,7 +102,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
     if (m.fr_lo < m.fr_hi) { lo = std::min<int64_t>(lo, m.fr_lo); hi = std::max<int64_t>(hi, (int64_t)m.fr_hi - 1); }
     if (lo > hi) continue;
     int64_t lead = R.pos[r] - lo, back = hi - R.pos[r];
-    if (lead > PL_MAX_LEAD || back > PL_MAX_BACK) { outliers.push_back(r); meta[r].info |= RI_OUTLIER; continue; }
+    if (lead > PL_MAX_LEAD || back > PL_MAX_BACK || R.seq_off[r + 1] - R.seq_off[r] >= PL_MAX_SEQ) { outliers.push_back(r); meta[r].info |= RI_OUTLIER; continue; }
     max_back = std::max<int32_t>(max_back, (int32_t)std::max<int64_t>(back, 0));
     max_lead = std::max<int32_t>(max_lead, (int32_t)std::max<int64_t>(lead, 0));
   }

@@ -28,6 +28,11 @@
 // outlier list instead of the sorted scan.
 #define PL_MAX_LEAD 4096
 #define PL_MAX_BACK 8192
+// a read with this much sequence or more is walked as an outlier, and a read
+// whose sequence starts this far past its tile's first read takes the
+// complex path: together they keep tile-relative sequence offsets in int32
+#define PL_MAX_SEQ (1 << 20)
+#define PL_TILE_SPAN (1 << 30)
 
 // Per-read facts derived once per read set (k_prep), 32 bytes.
 struct __attribute__((aligned(16))) RMeta {

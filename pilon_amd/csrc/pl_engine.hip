@@ -11,6 +11,7 @@
 // a 4 B/read prefix count of kept reads.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -79,6 +80,12 @@ struct ReadSet {
   ~ReadSet() { for (void* p : mem) (void)hipFree(p); }
 };
 
+// one region of a batch, by resident read set
+struct JobReq {
+  int32_t rs, start, stop;
+  bool operator==(const JobReq& o) const { return rs == o.rs && start == o.start && stop == o.stop; }
+};
+
 // host-owned copy of a pl_read_soa (plain region form)
 struct HostReads {
   std::vector<int32_t> pos, tlen;
@@ -95,34 +102,71 @@ struct HostReads {
 
 }  // namespace
 
-struct pl_ctx {
-  pl_config pc;
-  KConf cf;
-  int device = 0;
-  int n_cu = 256;
+// One independent batch pipeline: its stream, events, device buffers and
+// host-side tables.  A context has one (the plain and batched forms); a
+// server has one per concurrent batch.
+struct Lane {
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4];
-  std::map<int32_t, Contig> contigs;
-  std::vector<std::unique_ptr<ReadSet>> rsets;
-  bool rs_dirty = true;
-  // batch state
+  hipEvent_t ev[4] = {};
   DBuf rsarr, kparr, jobs, wins, worder, costk, costv, costtk, costtv, pile, calls, st, rec, rkey, rval, rtk, rtv,
-      cand, cands, ckey, cval, ctk, ctv, defer, mutated, counters, hist, tiles, strpool, stridx, stroff;
+      cand, cands, ckey, cval, ctk, ctv, defer, mutated, counters, hist, tiles, strpool, stridx, stroff, gath;
   uint32_t rec_cap = 0, cand_cap = 0, defer_cap = 0, mut_cap = 0;
   std::vector<JobDev> hjobs;
   std::vector<WinDev> hwins;
   std::vector<JobStat> hst;
   std::vector<Cand> hcand;
   int64_t G = 0;
-  int32_t W = 1024;
+  int32_t W = 0;
   int32_t n_wins = 0;
   bool exported_flags = false;
   double last_ms = 0, last_win_ms = 0;
   uint32_t last_counts[8] = {0};
   int32_t last_persistent = 0;
+  int32_t last_block = 0;
+  bool last_graph = false;
   EngineArgs args;
   uint32_t* rkey_sorted = nullptr;
   uint32_t* rval_sorted = nullptr;
+  // job/window tables on the device describe last_reqs at window W
+  std::vector<JobReq> last_reqs;
+  bool tables_valid = false;
+  uint64_t rs_seen = 0;            // read-set table version on the device
+  // captured batch pipeline (pl_batch_opts.use_graph)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> gkey;
+  EngineArgs gargs;
+
+  size_t device_bytes() const {
+    const DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st,
+                       &rec, &rkey, &rval, &rtk, &rtv, &cand, &cands, &ckey, &cval, &ctk, &ctv, &defer, &mutated,
+                       &counters, &hist, &tiles, &strpool, &stridx, &stroff, &gath};
+    size_t t = 0;
+    for (const DBuf* x : b) t += x->cap;
+    return t;
+  }
+  void free_buffers() {
+    DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st, &rec,
+                 &rkey, &rval, &rtk, &rtv, &cand, &cands, &ckey, &cval, &ctk, &ctv, &defer, &mutated, &counters,
+                 &hist, &tiles, &strpool, &stridx, &stroff, &gath};
+    for (DBuf* x : b) x->release();
+    tables_valid = false;
+    rs_seen = 0;
+    rec_cap = cand_cap = defer_cap = mut_cap = 0;
+  }
+};
+
+struct pl_ctx {
+  pl_config pc;
+  KConf cf;
+  int device = 0;
+  int n_cu = 256;
+  std::map<int32_t, Contig> contigs;
+  std::vector<std::unique_ptr<ReadSet>> rsets;
+  uint64_t rs_version = 1;         // bumped whenever rsets changes
+  Lane main;                       // plain and batched forms
+  std::map<uint64_t, std::pair<int32_t, int32_t>> tuned;   // autotuned (window, block) per batch shape
+  int serving = 0;                 // servers open on this context
 };
 
 struct pl_region {
@@ -146,29 +190,29 @@ pl_status upload(void** dst, const void* src, size_t bytes, size_t pad, std::vec
   return PL_OK;
 }
 
-pl_status scan_u32(pl_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
+pl_status scan_u32(Lane* L, const uint32_t* in, uint32_t* out, int64_t n) {
   const int64_t tiles = std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
-  HIPCHK(c->tiles.ensure(sizeof(uint32_t) * (size_t)tiles));
-  k_scan_tiles<<<dim3((unsigned)tiles), dim3(256), 0, c->stream>>>(in, n, c->tiles.as<uint32_t>());
-  k_scan_top<<<dim3(1), dim3(1024), 0, c->stream>>>(c->tiles.as<uint32_t>(), tiles);
-  k_scan_apply<<<dim3((unsigned)tiles), dim3(256), 0, c->stream>>>(in, n, c->tiles.as<uint32_t>(), out);
+  HIPCHK(L->tiles.ensure(sizeof(uint32_t) * (size_t)tiles));
+  k_scan_tiles<<<dim3((unsigned)tiles), dim3(256), 0, L->stream>>>(in, n, L->tiles.as<uint32_t>());
+  k_scan_top<<<dim3(1), dim3(1024), 0, L->stream>>>(L->tiles.as<uint32_t>(), tiles);
+  k_scan_apply<<<dim3((unsigned)tiles), dim3(256), 0, L->stream>>>(in, n, L->tiles.as<uint32_t>(), out);
   HIPCHK(hipGetLastError());
   return PL_OK;
 }
 
 // LSD radix sort of (key, val) pairs whose count is in device memory;
 // returns in *rk/*rv the buffers holding the sorted result.
-pl_status radix_sort(pl_ctx* c, uint32_t* k, uint32_t* v, uint32_t* tk, uint32_t* tv, const uint32_t* n_dev,
+pl_status radix_sort(Lane* L, uint32_t* k, uint32_t* v, uint32_t* tk, uint32_t* tv, const uint32_t* n_dev,
                      uint32_t cap, int bits, uint32_t** rk, uint32_t** rv) {
   const int32_t nb = (int32_t)std::max<int64_t>(1, ((int64_t)cap + RS_TILE - 1) / RS_TILE);
-  HIPCHK(c->hist.ensure(sizeof(uint32_t) * ((size_t)256 * nb + 1)));
+  HIPCHK(L->hist.ensure(sizeof(uint32_t) * ((size_t)256 * nb + 1)));
   uint32_t *ik = k, *iv = v, *ok = tk, *ov = tv;
   const int passes = std::max(1, (bits + 7) / 8);
   for (int p = 0; p < passes; p++) {
-    k_rs_hist<<<dim3(nb), dim3(256), 0, c->stream>>>(ik, n_dev, cap, 8 * p, c->hist.as<uint32_t>(), nb);
-    pl_status s = scan_u32(c, c->hist.as<uint32_t>(), c->hist.as<uint32_t>(), (int64_t)256 * nb);
+    k_rs_hist<<<dim3(nb), dim3(256), 0, L->stream>>>(ik, n_dev, cap, 8 * p, L->hist.as<uint32_t>(), nb);
+    pl_status s = scan_u32(L, L->hist.as<uint32_t>(), L->hist.as<uint32_t>(), (int64_t)256 * nb);
     if (s) return s;
-    k_rs_scatter<<<dim3(nb), dim3(256), 0, c->stream>>>(ik, iv, ok, ov, n_dev, cap, 8 * p, c->hist.as<uint32_t>(), nb);
+    k_rs_scatter<<<dim3(nb), dim3(256), 0, L->stream>>>(ik, iv, ok, ov, n_dev, cap, 8 * p, L->hist.as<uint32_t>(), nb);
     HIPCHK(hipGetLastError());
     std::swap(ik, ok);
     std::swap(iv, ov);
@@ -201,6 +245,7 @@ int64_t nib_bytes(const pl_read_soa* s) {
 }
 
 pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int32_t* out_id) {
+  Lane* L = c ? &c->main : nullptr;
   auto it = c->contigs.find(contig_id);
   if (it == c->contigs.end()) return fail(PL_E_INVAL, "contig %d not uploaded", contig_id);
   const int64_t n = s->n_reads;
@@ -217,7 +262,7 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   d.n = n;
   const int64_t ncig = s->cigar_off[n];
   const int64_t nb = s->seq_off[n];
-  hipStream_t st = c->stream;
+  hipStream_t st = L->stream;
   pl_status e;
   void* p;
 #define UP(field, src, bytes, pad)                                  \
@@ -260,10 +305,10 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
     ocap = rs->scal[3];
     HIPCHK(hipMalloc(&p, 8 * (size_t)ocap)); rs->mem.push_back(p); outl = (int64_t*)p;
   }
-  pl_status ss = scan_u32(c, keepflag, rs->kp, n);
+  pl_status ss = scan_u32(L, keepflag, rs->kp, n);
   if (ss) return ss;
   // segment table of the multi-op reads
-  ss = scan_u32(c, segcount, segoff, n);
+  ss = scan_u32(L, segcount, segoff, n);
   if (ss) return ss;
   uint32_t nseg = 0;
   HIPCHK(hipMemcpyAsync(&nseg, segoff + n, 4, hipMemcpyDeviceToHost, st));
@@ -292,194 +337,287 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   (void)hipFree(segoff);
   rs->bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 32 * n + 8 * n;
   c->rsets.push_back(std::move(rs));
-  c->rs_dirty = true;
+  c->rs_version++;
   *out_id = (int32_t)c->rsets.size() - 1;
   return PL_OK;
 }
 
-struct JobReq { int32_t rs, start, stop; };
+// Everything a captured graph bakes in: the kernel arguments (buffer
+// pointers, capacities, sizes) and the launch shapes.  A replay is valid
+// only while all of it is unchanged.
+struct GraphKey {
+  EngineArgs A;
+  int32_t W, block, persistent, grid;
+  uint64_t lds;
+  const void* bufs[8];
+};
 
-pl_status run_jobs(pl_ctx* c, const std::vector<JobReq>& reqs, int32_t W, int persistent, int block) {
+void drop_graph(Lane* L) {
+  if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
+  if (L->graph) (void)hipGraphDestroy(L->graph);
+  L->gexec = nullptr;
+  L->graph = nullptr;
+}
+
+pl_status lane_init(Lane* L) {
+  HIPCHK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+  for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&L->ev[i]));
+  return PL_OK;
+}
+
+void lane_destroy(Lane* L) {
+  if (L->stream) (void)hipStreamSynchronize(L->stream);
+  drop_graph(L);
+  L->free_buffers();
+  for (int i = 0; i < 4; i++) if (L->ev[i]) (void)hipEventDestroy(L->ev[i]);
+  if (L->stream) (void)hipStreamDestroy(L->stream);
+  L->stream = nullptr;
+  for (int i = 0; i < 4; i++) L->ev[i] = nullptr;
+}
+
+// the device pipeline of one batch, enqueued on the lane's stream; with
+// timed set, events bracket the window kernel (not while capturing)
+pl_status enqueue_batch(Lane* L, EngineArgs& A, int persistent, int block, int grid, size_t lds, bool timed) {
+  hipStream_t st = L->stream;
+  uint32_t* cnt = L->counters.as<uint32_t>();
+  const int32_t n_jobs = A.n_jobs, n_wins = A.n_wins;
+  const int64_t G = A.G;
+  const int64_t words = (int64_t)n_jobs * (sizeof(JobStat) / 4);
+  k_init_stats<<<dim3((unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (words + 255) / 256))), dim3(256), 0, st>>>(A.st, n_jobs, cnt, 16);
+  k_init_stats2<<<dim3((n_jobs + 255) / 256), dim3(256), 0, st>>>(A.st, n_jobs);
+  k_plan<<<dim3((n_wins + 255) / 256), dim3(256), 0, st>>>(A.rs, A.jobs, L->wins.as<WinDev>(), n_wins, L->costk.as<uint32_t>());
+  HIPCHK(hipGetLastError());
+  A.worder = L->worder.as<int32_t>();
+  if (persistent) {
+    k_iota<<<dim3((unsigned)std::min<int64_t>(1024, (n_wins + 255) / 256)), dim3(256), 0, st>>>(L->costv.as<uint32_t>(), n_wins, cnt + 8);
+    uint32_t *sk, *sv;
+    pl_status s = radix_sort(L, L->costk.as<uint32_t>(), L->costv.as<uint32_t>(), L->costtk.as<uint32_t>(),
+                             L->costtv.as<uint32_t>(), cnt + 8, (uint32_t)n_wins, 32, &sk, &sv);
+    if (s) return s;
+    A.worder = (const int32_t*)sv;
+  }
+  k_jobprep<<<dim3(n_jobs), dim3(64), 0, st>>>(A.rs, (const uint32_t* const*)L->kparr.p, A.jobs, A.st);
+  HIPCHK(hipGetLastError());
+  if (timed) HIPCHK(hipEventRecord(L->ev[2], st));
+  if (block == 256) k_window<256><<<dim3(grid), dim3(256), lds, st>>>(A, persistent);
+  else if (block == 512) k_window<512><<<dim3(grid), dim3(512), lds, st>>>(A, persistent);
+  else k_window<1024><<<dim3(grid), dim3(1024), lds, st>>>(A, persistent);
+  HIPCHK(hipGetLastError());
+  if (timed) HIPCHK(hipEventRecord(L->ev[3], st));
+  // evidence records: sort by (position, kind), then the deferred columns
+  const uint32_t gb = (uint32_t)std::min<int64_t>(4096, std::max<int64_t>(1, ((int64_t)A.rec_cap + 255) / 256));
+  k_rec_keys<<<dim3(gb), dim3(256), 0, st>>>(A.rec, A.rec_n, A.rec_cap, L->rkey.as<uint32_t>(), L->rval.as<uint32_t>());
+  uint32_t *rk, *rv;
+  pl_status s = radix_sort(L, L->rkey.as<uint32_t>(), L->rval.as<uint32_t>(), L->rtk.as<uint32_t>(), L->rtv.as<uint32_t>(),
+                           A.rec_n, A.rec_cap, bits_for(2 * (uint64_t)G), &rk, &rv);
+  if (s) return s;
+  A.rkey = rk; A.rval = rv;
+  L->rkey_sorted = rk; L->rval_sorted = rv;
+  const uint32_t gd = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)A.defer_cap + 255) / 256));
+  k_indel_pass<<<dim3(gd), dim3(256), 0, st>>>(A, A.defer, A.defer_n, A.defer_cap, 0);
+  HIPCHK(hipGetLastError());
+  // candidates by position, then Pass 1 per region
+  const uint32_t gc = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)A.cand_cap + 255) / 256));
+  k_cand_keys<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, L->ckey.as<uint32_t>(), L->cval.as<uint32_t>());
+  uint32_t *ck, *cv;
+  s = radix_sort(L, L->ckey.as<uint32_t>(), L->cval.as<uint32_t>(), L->ctk.as<uint32_t>(), L->ctv.as<uint32_t>(),
+                 A.cand_n, A.cand_cap, bits_for((uint64_t)G), &ck, &cv);
+  if (s) return s;
+  k_cand_gather<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, cv, L->cands.as<Cand>());
+  k_finalize<<<dim3((n_jobs + 63) / 64), dim3(64), 0, st>>>(A, L->cands.as<Cand>(), A.cand_n, A.cand_cap, L->cands.as<Cand>());
+  const uint32_t gm = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)A.mutated_cap + 255) / 256));
+  k_indel_pass<<<dim3(gm), dim3(256), 0, st>>>(A, A.mutated, A.mutated_n, A.mutated_cap, 1);
+  HIPCHK(hipGetLastError());
+  return PL_OK;
+}
+
+// LDS the window kernel needs for a window of W positions (dynamic part)
+size_t window_lds(int32_t W) { return 72 * (size_t)W + 1024; }
+// ... and per workgroup in total, with the per-wave run tables
+size_t window_lds_total(int32_t W, int block) { return window_lds(W) + 32 * (size_t)block + 1024; }
+
+pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t W, int persistent, int block, bool use_graph) {
   if (reqs.empty()) return fail(PL_E_INVAL, "empty batch");
   if (W < 64 || W % 64 || W > 2048) return fail(PL_E_INVAL, "window %d must be a multiple of 64 in [64, 2048]", W);
   if (block != 256 && block != 512 && block != 1024) return fail(PL_E_INVAL, "block %d must be 256, 512 or 1024", block);
-  c->W = W;
-  c->exported_flags = false;
-  // ---- jobs and windows
-  c->hjobs.resize(reqs.size());
-  c->hwins.clear();
-  int64_t G = 0;
-  for (size_t j = 0; j < reqs.size(); j++) {
-    const JobReq& q = reqs[j];
-    if (q.rs < 0 || q.rs >= (int32_t)c->rsets.size() || !c->rsets[q.rs]) return fail(PL_E_INVAL, "bad read set %d", q.rs);
-    const Contig& ct = c->contigs[c->rsets[q.rs]->contig];
-    if (q.start < 1 || q.stop < q.start || q.stop > ct.len)
-      return fail(PL_E_INVAL, "region %d-%d outside contig of length %lld", q.start, q.stop, (long long)ct.len);
-    JobDev& J = c->hjobs[j];
-    std::memset(&J, 0, sizeof J);
-    J.rs = q.rs; J.start = q.start; J.stop = q.stop;
-    J.qs = (int32_t)std::max<int64_t>((int64_t)q.start - 10000, 0);
-    J.qe = (int32_t)std::min<int64_t>((int64_t)q.stop + 10000, ct.len);
-    J.size = q.stop - q.start + 1;
-    J.out = G;
-    J.win0 = (int32_t)c->hwins.size();
-    for (int32_t p0 = 0; p0 < J.size; p0 += W) {
-      WinDev w;
-      std::memset(&w, 0, sizeof w);
-      w.job = (int32_t)j; w.p0 = p0; w.p1 = std::min<int32_t>(J.size, p0 + W);
-      c->hwins.push_back(w);
+  if (window_lds_total(W, block) > 160 * 1024)
+    return fail(PL_E_INVAL, "window %d with block %d needs more than the 160 KB of LDS of a CU", W, block);
+  L->exported_flags = false;
+  hipStream_t st = L->stream;
+  // ---- jobs and windows (kept on the device while the batch repeats)
+  const bool same = L->tables_valid && L->W == W && reqs == L->last_reqs;
+  if (!same) {
+    L->tables_valid = false;
+    L->W = W;
+    L->hjobs.resize(reqs.size());
+    L->hwins.clear();
+    int64_t G = 0;
+    for (size_t j = 0; j < reqs.size(); j++) {
+      const JobReq& q = reqs[j];
+      if (q.rs < 0 || q.rs >= (int32_t)c->rsets.size() || !c->rsets[q.rs]) return fail(PL_E_INVAL, "bad read set %d", q.rs);
+      const Contig& ct = c->contigs.at(c->rsets[q.rs]->contig);
+      if (q.start < 1 || q.stop < q.start || q.stop > ct.len)
+        return fail(PL_E_INVAL, "region %d-%d outside contig of length %lld", q.start, q.stop, (long long)ct.len);
+      JobDev& J = L->hjobs[j];
+      std::memset(&J, 0, sizeof J);
+      J.rs = q.rs; J.start = q.start; J.stop = q.stop;
+      J.qs = (int32_t)std::max<int64_t>((int64_t)q.start - 10000, 0);
+      J.qe = (int32_t)std::min<int64_t>((int64_t)q.stop + 10000, ct.len);
+      J.size = q.stop - q.start + 1;
+      J.out = G;
+      J.win0 = (int32_t)L->hwins.size();
+      for (int32_t p0 = 0; p0 < J.size; p0 += W) {
+        WinDev w;
+        std::memset(&w, 0, sizeof w);
+        w.job = (int32_t)j; w.p0 = p0; w.p1 = std::min<int32_t>(J.size, p0 + W);
+        L->hwins.push_back(w);
+      }
+      J.nwin = (int32_t)L->hwins.size() - J.win0;
+      G += J.size;
     }
-    J.nwin = (int32_t)c->hwins.size() - J.win0;
-    G += J.size;
+    if (G >= (int64_t)1 << 31) return fail(PL_E_INVAL, "batch of %lld positions exceeds 2^31", (long long)G);
+    L->G = G;
+    L->n_wins = (int32_t)L->hwins.size();
   }
-  if (G >= (int64_t)1 << 31) return fail(PL_E_INVAL, "batch of %lld positions exceeds 2^31", (long long)G);
-  c->G = G;
+  const int64_t G = L->G;
   const int32_t n_jobs = (int32_t)reqs.size();
-  const int32_t n_wins = (int32_t)c->hwins.size();
-  c->n_wins = n_wins;
-  hipStream_t st = c->stream;
+  const int32_t n_wins = L->n_wins;
   // ---- resident read-set table
-  if (c->rs_dirty) {
+  if (L->rs_seen != c->rs_version) {
     std::vector<RSDev> rsd(c->rsets.size());
     std::vector<const uint32_t*> kps(c->rsets.size(), nullptr);
     for (size_t i = 0; i < c->rsets.size(); i++) if (c->rsets[i]) { rsd[i] = c->rsets[i]->d; kps[i] = c->rsets[i]->kp; }
-    HIPCHK(c->rsarr.ensure(sizeof(RSDev) * rsd.size()));
-    HIPCHK(c->kparr.ensure(sizeof(void*) * kps.size()));
-    HIPCHK(hipMemcpyAsync(c->rsarr.p, rsd.data(), sizeof(RSDev) * rsd.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(c->kparr.p, kps.data(), sizeof(void*) * kps.size(), hipMemcpyHostToDevice, st));
-    c->rs_dirty = false;
+    HIPCHK(L->rsarr.ensure(sizeof(RSDev) * rsd.size()));
+    HIPCHK(L->kparr.ensure(sizeof(void*) * kps.size()));
+    HIPCHK(hipMemcpyAsync(L->rsarr.p, rsd.data(), sizeof(RSDev) * rsd.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(L->kparr.p, kps.data(), sizeof(void*) * kps.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    L->rs_seen = c->rs_version;
   }
   // ---- capacities (grown and re-run on overflow)
-  if (!c->rec_cap) { c->rec_cap = 1u << 18; c->cand_cap = 1u << 16; c->defer_cap = 1u << 16; c->mut_cap = 1u << 16; }
-  c->rec_cap = std::max<uint32_t>(c->rec_cap, (uint32_t)std::min<int64_t>(G / 16 + 65536, 1ll << 30));
-  c->cand_cap = std::max<uint32_t>(c->cand_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
-  c->defer_cap = std::max<uint32_t>(c->defer_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
-  c->mut_cap = std::max<uint32_t>(c->mut_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
-  HIPCHK(c->jobs.ensure(sizeof(JobDev) * n_jobs));
-  HIPCHK(c->wins.ensure(sizeof(WinDev) * n_wins));
-  HIPCHK(c->worder.ensure(4 * (size_t)n_wins));
-  HIPCHK(c->costk.ensure(4 * (size_t)n_wins));
-  HIPCHK(c->costv.ensure(4 * (size_t)n_wins));
-  HIPCHK(c->costtk.ensure(4 * (size_t)n_wins));
-  HIPCHK(c->costtv.ensure(4 * (size_t)n_wins));
-  HIPCHK(c->pile.ensure(4 * (size_t)PL_NFIELDS * G));
-  HIPCHK(c->calls.ensure(sizeof(pl_call) * (size_t)G));
-  HIPCHK(c->st.ensure(sizeof(JobStat) * n_jobs));
-  HIPCHK(c->counters.ensure(64));
-  HIPCHK(hipMemcpyAsync(c->jobs.p, c->hjobs.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(c->wins.p, c->hwins.data(), sizeof(WinDev) * n_wins, hipMemcpyHostToDevice, st));
+  if (!L->rec_cap) { L->rec_cap = 1u << 18; L->cand_cap = 1u << 16; L->defer_cap = 1u << 16; L->mut_cap = 1u << 16; }
+  L->rec_cap = std::max<uint32_t>(L->rec_cap, (uint32_t)std::min<int64_t>(G / 16 + 65536, 1ll << 30));
+  L->cand_cap = std::max<uint32_t>(L->cand_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
+  L->defer_cap = std::max<uint32_t>(L->defer_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
+  L->mut_cap = std::max<uint32_t>(L->mut_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
+  HIPCHK(L->jobs.ensure(sizeof(JobDev) * n_jobs));
+  HIPCHK(L->wins.ensure(sizeof(WinDev) * n_wins));
+  HIPCHK(L->worder.ensure(4 * (size_t)n_wins));
+  HIPCHK(L->costk.ensure(4 * (size_t)n_wins));
+  HIPCHK(L->costv.ensure(4 * (size_t)n_wins));
+  HIPCHK(L->costtk.ensure(4 * (size_t)n_wins));
+  HIPCHK(L->costtv.ensure(4 * (size_t)n_wins));
+  HIPCHK(L->pile.ensure(4 * (size_t)PL_NFIELDS * G));
+  HIPCHK(L->calls.ensure(sizeof(pl_call) * (size_t)G));
+  HIPCHK(L->st.ensure(sizeof(JobStat) * n_jobs));
+  HIPCHK(L->counters.ensure(64));
+  if (!same) {
+    HIPCHK(hipMemcpyAsync(L->jobs.p, L->hjobs.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(L->wins.p, L->hwins.data(), sizeof(WinDev) * n_wins, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    L->last_reqs = reqs;
+    L->tables_valid = true;
+  }
+  const size_t lds = window_lds(W);
+  const void* kfn = block == 256 ? (const void*)k_window<256> : block == 512 ? (const void*)k_window<512>
+                                                                             : (const void*)k_window<1024>;
+  HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int grid = n_wins;
+  if (persistent) {
+    // workgroups per CU: LDS (dynamic + static run tables) and 32 waves
+    const int by_lds = (int)(160 * 1024 / window_lds_total(W, block));
+    const int by_waves = 32 / (block / 64);
+    const int per_cu = std::max(1, std::min(by_lds, by_waves));
+    grid = std::min(n_wins, c->n_cu * per_cu);
+  }
   for (int attempt = 0; attempt < 6; attempt++) {
-    HIPCHK(c->rec.ensure(sizeof(IndelRec) * (size_t)c->rec_cap));
-    HIPCHK(c->rkey.ensure(4 * (size_t)c->rec_cap)); HIPCHK(c->rval.ensure(4 * (size_t)c->rec_cap));
-    HIPCHK(c->rtk.ensure(4 * (size_t)c->rec_cap)); HIPCHK(c->rtv.ensure(4 * (size_t)c->rec_cap));
-    HIPCHK(c->cand.ensure(sizeof(Cand) * (size_t)c->cand_cap));
-    HIPCHK(c->cands.ensure(sizeof(Cand) * (size_t)c->cand_cap));
-    HIPCHK(c->ckey.ensure(4 * (size_t)c->cand_cap)); HIPCHK(c->cval.ensure(4 * (size_t)c->cand_cap));
-    HIPCHK(c->ctk.ensure(4 * (size_t)c->cand_cap)); HIPCHK(c->ctv.ensure(4 * (size_t)c->cand_cap));
-    HIPCHK(c->defer.ensure(4 * (size_t)c->defer_cap));
-    HIPCHK(c->mutated.ensure(4 * (size_t)c->mut_cap));
-    uint32_t* cnt = c->counters.as<uint32_t>();
-    EngineArgs& A = c->args;
+    HIPCHK(L->rec.ensure(sizeof(IndelRec) * (size_t)L->rec_cap));
+    HIPCHK(L->rkey.ensure(4 * (size_t)L->rec_cap)); HIPCHK(L->rval.ensure(4 * (size_t)L->rec_cap));
+    HIPCHK(L->rtk.ensure(4 * (size_t)L->rec_cap)); HIPCHK(L->rtv.ensure(4 * (size_t)L->rec_cap));
+    HIPCHK(L->cand.ensure(sizeof(Cand) * (size_t)L->cand_cap));
+    HIPCHK(L->cands.ensure(sizeof(Cand) * (size_t)L->cand_cap));
+    HIPCHK(L->ckey.ensure(4 * (size_t)L->cand_cap)); HIPCHK(L->cval.ensure(4 * (size_t)L->cand_cap));
+    HIPCHK(L->ctk.ensure(4 * (size_t)L->cand_cap)); HIPCHK(L->ctv.ensure(4 * (size_t)L->cand_cap));
+    HIPCHK(L->defer.ensure(4 * (size_t)L->defer_cap));
+    HIPCHK(L->mutated.ensure(4 * (size_t)L->mut_cap));
+    // sort scratch for the largest sort, allocated before any capture
+    {
+      const uint32_t mx = std::max<uint32_t>(std::max<uint32_t>(L->rec_cap, L->cand_cap), (uint32_t)n_wins);
+      const int64_t nb = std::max<int64_t>(1, ((int64_t)mx + RS_TILE - 1) / RS_TILE);
+      HIPCHK(L->hist.ensure(sizeof(uint32_t) * ((size_t)256 * nb + 1)));
+      HIPCHK(L->tiles.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, (256 * nb + SCAN_TILE - 1) / SCAN_TILE)));
+    }
+    uint32_t* cnt = L->counters.as<uint32_t>();
+    EngineArgs& A = L->args;
     std::memset(&A, 0, sizeof A);
     A.cf = c->cf;
-    A.rs = c->rsarr.as<RSDev>();
-    A.jobs = c->jobs.as<JobDev>();
+    A.rs = L->rsarr.as<RSDev>();
+    A.jobs = L->jobs.as<JobDev>();
     A.n_jobs = n_jobs;
-    A.wins = c->wins.as<WinDev>();
+    A.wins = L->wins.as<WinDev>();
     A.n_wins = n_wins;
     A.W = W;
     A.queue = (int32_t*)(cnt + 5);
-    A.worder = c->worder.as<int32_t>();
-    A.pile = c->pile.as<int32_t>();
-    A.calls = c->calls.as<pl_call>();
+    A.worder = L->worder.as<int32_t>();
+    A.pile = L->pile.as<int32_t>();
+    A.calls = L->calls.as<pl_call>();
     A.G = G;
-    A.st = c->st.as<JobStat>();
-    A.rec = c->rec.as<IndelRec>(); A.rec_n = cnt + 0; A.rec_cap = c->rec_cap;
-    A.cand = c->cand.as<Cand>(); A.cand_n = cnt + 1; A.cand_cap = c->cand_cap;
-    A.defer = c->defer.as<uint32_t>(); A.defer_n = cnt + 2; A.defer_cap = c->defer_cap;
-    A.mutated = c->mutated.as<uint32_t>(); A.mutated_n = cnt + 3; A.mutated_cap = c->mut_cap;
+    A.st = L->st.as<JobStat>();
+    A.rec = L->rec.as<IndelRec>(); A.rec_n = cnt + 0; A.rec_cap = L->rec_cap;
+    A.cand = L->cand.as<Cand>(); A.cand_n = cnt + 1; A.cand_cap = L->cand_cap;
+    A.defer = L->defer.as<uint32_t>(); A.defer_n = cnt + 2; A.defer_cap = L->defer_cap;
+    A.mutated = L->mutated.as<uint32_t>(); A.mutated_n = cnt + 3; A.mutated_cap = L->mut_cap;
     A.overflow = (int32_t*)(cnt + 4);
 
-    HIPCHK(hipEventRecord(c->ev[0], st));
-    const int64_t words = (int64_t)n_jobs * (sizeof(JobStat) / 4);
-    k_init_stats<<<dim3((unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (words + 255) / 256))), dim3(256), 0, st>>>(A.st, n_jobs, cnt, 16);
-    k_init_stats2<<<dim3((n_jobs + 255) / 256), dim3(256), 0, st>>>(A.st, n_jobs);
-    k_plan<<<dim3((n_wins + 255) / 256), dim3(256), 0, st>>>(A.rs, A.jobs, c->wins.as<WinDev>(), n_wins, c->costk.as<uint32_t>());
-    HIPCHK(hipGetLastError());
-    if (persistent) {
-      std::vector<uint32_t> iota((size_t)n_wins);
-      for (int32_t i = 0; i < n_wins; i++) iota[i] = (uint32_t)i;
-      HIPCHK(hipMemcpyAsync(c->costv.p, iota.data(), 4 * (size_t)n_wins, hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(cnt + 8, &n_wins, 4, hipMemcpyHostToDevice, st));
-      uint32_t *sk, *sv;
-      pl_status s = radix_sort(c, c->costk.as<uint32_t>(), c->costv.as<uint32_t>(), c->costtk.as<uint32_t>(),
-                               c->costtv.as<uint32_t>(), cnt + 8, (uint32_t)n_wins, 32, &sk, &sv);
+    if (use_graph) {
+      GraphKey key;
+      std::memset(&key, 0, sizeof key);
+      key.A = A;
+      key.W = W; key.block = block; key.persistent = persistent; key.grid = grid; key.lds = lds;
+      key.bufs[0] = L->hist.p; key.bufs[1] = L->tiles.p; key.bufs[2] = L->rsarr.p; key.bufs[3] = L->kparr.p;
+      key.bufs[4] = L->cands.p; key.bufs[5] = L->costtk.p; key.bufs[6] = L->rtk.p; key.bufs[7] = L->ctk.p;
+      if (!L->gexec || L->gkey.size() != sizeof key || std::memcmp(L->gkey.data(), &key, sizeof key) != 0) {
+        drop_graph(L);
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        EngineArgs Acap = A;
+        pl_status s = enqueue_batch(L, Acap, persistent, block, grid, lds, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (s) { if (g) (void)hipGraphDestroy(g); return s; }
+        HIPCHK(e);
+        L->graph = g;
+        HIPCHK(hipGraphInstantiate(&L->gexec, g, nullptr, nullptr, 0));
+        L->gkey.assign((const uint8_t*)&key, (const uint8_t*)&key + sizeof key);
+        L->gargs = Acap;
+      }
+      A = L->gargs;
+      HIPCHK(hipEventRecord(L->ev[0], st));
+      HIPCHK(hipGraphLaunch(L->gexec, st));
+      HIPCHK(hipEventRecord(L->ev[1], st));
+    } else {
+      HIPCHK(hipEventRecord(L->ev[0], st));
+      pl_status s = enqueue_batch(L, A, persistent, block, grid, lds, true);
       if (s) return s;
-      A.worder = (const int32_t*)sv;
+      HIPCHK(hipEventRecord(L->ev[1], st));
     }
-    k_jobprep<<<dim3(n_jobs), dim3(64), 0, st>>>(A.rs, (const uint32_t* const*)c->kparr.p, A.jobs, A.st);
-    HIPCHK(hipGetLastError());
-    const size_t lds = 72 * (size_t)W + 1024;
-    if (lds + 32 * (size_t)block + 1024 > 160 * 1024)
-      return fail(PL_E_INVAL, "window %d with block %d needs more than the 160 KB of LDS of a CU", W, block);
-    const void* kfn = block == 256 ? (const void*)k_window<256> : block == 512 ? (const void*)k_window<512>
-                                                                               : (const void*)k_window<1024>;
-    HIPCHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int grid = n_wins;
-    if (persistent) {
-      // workgroups per CU: LDS (dynamic + 16 B/thread of tile arrays) and 32 waves
-      const size_t per_wg = lds + 32 * (size_t)block + 512;
-      const int by_lds = (int)(160 * 1024 / per_wg);
-      const int by_waves = 32 / (block / 64);
-      const int per_cu = std::max(1, std::min(by_lds, by_waves));
-      grid = std::min(n_wins, c->n_cu * per_cu);
-    }
-    HIPCHK(hipEventRecord(c->ev[2], st));
-    if (block == 256) k_window<256><<<dim3(grid), dim3(256), lds, st>>>(A, persistent);
-    else if (block == 512) k_window<512><<<dim3(grid), dim3(512), lds, st>>>(A, persistent);
-    else k_window<1024><<<dim3(grid), dim3(1024), lds, st>>>(A, persistent);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[3], st));
-    // evidence records: sort by (position, kind), then the deferred columns
-    const uint32_t gb = (uint32_t)std::min<int64_t>(4096, std::max<int64_t>(1, ((int64_t)c->rec_cap + 255) / 256));
-    k_rec_keys<<<dim3(gb), dim3(256), 0, st>>>(A.rec, A.rec_n, A.rec_cap, c->rkey.as<uint32_t>(), c->rval.as<uint32_t>());
-    uint32_t *rk, *rv;
-    pl_status s = radix_sort(c, c->rkey.as<uint32_t>(), c->rval.as<uint32_t>(), c->rtk.as<uint32_t>(), c->rtv.as<uint32_t>(),
-                             A.rec_n, A.rec_cap, bits_for(2 * (uint64_t)G), &rk, &rv);
-    if (s) return s;
-    A.rkey = rk; A.rval = rv;
-    c->rkey_sorted = rk; c->rval_sorted = rv;
-    const uint32_t gd = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)c->defer_cap + 255) / 256));
-    k_indel_pass<<<dim3(gd), dim3(256), 0, st>>>(A, A.defer, A.defer_n, A.defer_cap, 0);
-    HIPCHK(hipGetLastError());
-    // candidates by position, then Pass 1 per region
-    const uint32_t gc = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)c->cand_cap + 255) / 256));
-    k_cand_keys<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, c->ckey.as<uint32_t>(), c->cval.as<uint32_t>());
-    uint32_t *ck, *cv;
-    s = radix_sort(c, c->ckey.as<uint32_t>(), c->cval.as<uint32_t>(), c->ctk.as<uint32_t>(), c->ctv.as<uint32_t>(),
-                   A.cand_n, A.cand_cap, bits_for((uint64_t)G), &ck, &cv);
-    if (s) return s;
-    k_cand_gather<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, cv, c->cands.as<Cand>());
-    k_finalize<<<dim3((n_jobs + 63) / 64), dim3(64), 0, st>>>(A, c->cands.as<Cand>(), A.cand_n, A.cand_cap, c->cands.as<Cand>());
-    const uint32_t gm = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)c->mut_cap + 255) / 256));
-    k_indel_pass<<<dim3(gm), dim3(256), 0, st>>>(A, A.mutated, A.mutated_n, A.mutated_cap, 1);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[1], st));
+    L->rkey_sorted = A.rkey; L->rval_sorted = A.rval;
     uint32_t hc[8];
     HIPCHK(hipMemcpyAsync(hc, cnt, 32, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    float ms = 0, wms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    HIPCHK(hipEventElapsedTime(&wms, c->ev[2], c->ev[3]));
-    c->last_ms = ms;
-    c->last_win_ms = wms;
-    std::memcpy(c->last_counts, hc, sizeof hc);
-    c->last_persistent = persistent;
+    float ms = 0, wms = -1.0f;
+    HIPCHK(hipEventElapsedTime(&ms, L->ev[0], L->ev[1]));
+    if (!use_graph) HIPCHK(hipEventElapsedTime(&wms, L->ev[2], L->ev[3]));
+    L->last_ms = ms;
+    L->last_win_ms = wms;
+    L->last_graph = use_graph;
+    std::memcpy(L->last_counts, hc, sizeof hc);
+    L->last_persistent = persistent;
+    L->last_block = block;
     bool grow = false;
-    if (hc[0] > c->rec_cap) { c->rec_cap = (uint32_t)std::min<uint64_t>(2ull * hc[0], 1u << 30); grow = true; }
-    if (hc[1] > c->cand_cap) { c->cand_cap = (uint32_t)std::min<uint64_t>(2ull * hc[1], 1u << 30); grow = true; }
-    if (hc[2] > c->defer_cap) { c->defer_cap = (uint32_t)std::min<uint64_t>(2ull * hc[2], 1u << 30); grow = true; }
-    if (hc[3] > c->mut_cap) { c->mut_cap = (uint32_t)std::min<uint64_t>(2ull * hc[3], 1u << 30); grow = true; }
+    if (hc[0] > L->rec_cap) { L->rec_cap = (uint32_t)std::min<uint64_t>(2ull * hc[0], 1u << 30); grow = true; }
+    if (hc[1] > L->cand_cap) { L->cand_cap = (uint32_t)std::min<uint64_t>(2ull * hc[1], 1u << 30); grow = true; }
+    if (hc[2] > L->defer_cap) { L->defer_cap = (uint32_t)std::min<uint64_t>(2ull * hc[2], 1u << 30); grow = true; }
+    if (hc[3] > L->mut_cap) { L->mut_cap = (uint32_t)std::min<uint64_t>(2ull * hc[3], 1u << 30); grow = true; }
     if (!grow) {
       if (hc[4]) return fail(PL_E_DEVICE, "capacity overflow flag set without an overfull counter");
       return PL_OK;
@@ -488,23 +626,69 @@ pl_status run_jobs(pl_ctx* c, const std::vector<JobReq>& reqs, int32_t W, int pe
   return fail(PL_E_OOM, "evidence buffers kept overflowing");
 }
 
-// copy stats (and candidates) of the last batch to the host
-pl_status fetch_batch_meta(pl_ctx* c) {
-  const size_t nj = c->hjobs.size();
-  c->hst.resize(nj);
-  HIPCHK(hipMemcpyAsync(c->hst.data(), c->st.p, sizeof(JobStat) * nj, hipMemcpyDeviceToHost, c->stream));
-  uint32_t cn = 0;
-  HIPCHK(hipMemcpyAsync(&cn, c->counters.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  cn = std::min(cn, c->cand_cap);
-  c->hcand.resize(cn);
-  if (cn) HIPCHK(hipMemcpyAsync(c->hcand.data(), c->cands.p, sizeof(Cand) * cn, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+// Batch shape the tuned launch configuration is cached under: log2 buckets
+// of the positions, the region count and the read density.
+uint64_t shape_key(pl_ctx* c, const std::vector<JobReq>& q, int persistent) {
+  double G = 0, reads = 0;
+  for (const JobReq& j : q) {
+    const ReadSet* rs = (j.rs >= 0 && j.rs < (int32_t)c->rsets.size()) ? c->rsets[j.rs].get() : nullptr;
+    const double size = (double)j.stop - j.start + 1;
+    G += size;
+    if (rs) {
+      const double len = (double)std::max<int64_t>(1, c->contigs.at(rs->contig).len);
+      reads += (double)rs->n * size / len;
+    }
+  }
+  auto lg = [](double x) { return (uint64_t)std::floor(std::log2(std::max(1.0, x))); };
+  return lg(G) | (lg((double)q.size()) << 8) | (lg(16.0 * reads / std::max(1.0, G)) << 16) | ((uint64_t)persistent << 24);
+}
+
+// Picks (window, block) for a batch by timing every candidate that fits the
+// LDS of a CU on the batch itself (each trial is a complete, valid run) and
+// caches the fastest per batch shape.  A caller-fixed window or block is
+// kept and only the other one is searched.
+pl_status autotune(pl_ctx* c, Lane* L, const std::vector<JobReq>& q, int persistent, int32_t* W, int32_t* block) {
+  const uint64_t key = shape_key(c, q, persistent);
+  auto it = c->tuned.find(key);
+  if (it == c->tuned.end()) {
+    static const int32_t Ws[] = {256, 384, 512, 640, 768, 1024};
+    static const int32_t Bs[] = {256, 512};
+    double best = 1e30;
+    std::pair<int32_t, int32_t> pick(PL_DEFAULT_WINDOW, PL_DEFAULT_BLOCK);
+    for (int32_t w : Ws) {
+      if (*W && w != *W) continue;
+      for (int32_t b : Bs) {
+        if (*block && b != *block) continue;
+        if (window_lds_total(w, b) > 160 * 1024) continue;
+        pl_status s = run_jobs(c, L, q, w, persistent, b, false);
+        if (s) return s;
+        if (L->last_ms < best) { best = L->last_ms; pick = {w, b}; }
+      }
+    }
+    it = c->tuned.emplace(key, pick).first;
+  }
+  if (!*W) *W = it->second.first;
+  if (!*block) *block = it->second.second;
   return PL_OK;
 }
 
-pl_status job_status(pl_ctx* c, size_t j) {
-  const JobStat& S = c->hst[j];
+// copy stats (and candidates) of the last batch to the host
+pl_status fetch_batch_meta(Lane* L) {
+  const size_t nj = L->hjobs.size();
+  L->hst.resize(nj);
+  HIPCHK(hipMemcpyAsync(L->hst.data(), L->st.p, sizeof(JobStat) * nj, hipMemcpyDeviceToHost, L->stream));
+  uint32_t cn = 0;
+  HIPCHK(hipMemcpyAsync(&cn, L->counters.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  cn = std::min(cn, L->cand_cap);
+  L->hcand.resize(cn);
+  if (cn) HIPCHK(hipMemcpyAsync(L->hcand.data(), L->cands.p, sizeof(Cand) * cn, hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  return PL_OK;
+}
+
+pl_status job_status(Lane* L, size_t j) {
+  const JobStat& S = L->hst[j];
   if (S.err_read_inval != INT64_MAX && S.err_read_inval <= S.err_read_edge)
     return fail(PL_E_INVAL, "malformed read %lld (CIGAR and SEQ disagree or bad op)", (long long)S.err_read_inval);
   if (S.err_read_edge != INT64_MAX)
@@ -515,14 +699,14 @@ pl_status job_status(pl_ctx* c, size_t j) {
 }
 
 // stats + change list of job j of the last batch (needs fetch_batch_meta)
-pl_status job_changes(pl_ctx* c, size_t j, pl_region_stats* out, std::vector<plh::Change>* changes) {
-  const JobDev& J = c->hjobs[j];
-  const JobStat& S = c->hst[j];
-  const Contig& ct = c->contigs[c->rsets[J.rs]->contig];
+pl_status job_changes(pl_ctx* c, Lane* L, size_t j, pl_region_stats* out, std::vector<plh::Change>* changes) {
+  const JobDev& J = L->hjobs[j];
+  const JobStat& S = L->hst[j];
+  const Contig& ct = c->contigs.at(c->rsets[J.rs]->contig);
   std::vector<plh::Change> ch;
   std::vector<int64_t> ins_idx;
-  for (int32_t k = S.cand_lo; k < S.cand_hi && k < (int32_t)c->hcand.size(); k++) {
-    const Cand& cd = c->hcand[k];
+  for (int32_t k = S.cand_lo; k < S.cand_hi && k < (int32_t)L->hcand.size(); k++) {
+    const Cand& cd = L->hcand[k];
     if (!cd.kind) continue;
     plh::Change x;
     x.loc = J.start + (int32_t)(cd.gpos - J.out);
@@ -536,26 +720,26 @@ pl_status job_changes(pl_ctx* c, size_t j, pl_region_stats* out, std::vector<plh
   // base/alt of the changed columns
   std::vector<pl_call> cl(ch.size());
   for (size_t i = 0; i < ch.size(); i++)
-    HIPCHK(hipMemcpyAsync(&cl[i], c->calls.as<pl_call>() + J.out + (ch[i].loc - J.start), sizeof(pl_call),
-                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&cl[i], L->calls.as<pl_call>() + J.out + (ch[i].loc - J.start), sizeof(pl_call),
+                          hipMemcpyDeviceToHost, L->stream));
   // insertion strings decoded on the device
   std::string pool;
   std::vector<int64_t> off(ins_idx.size() + 1, 0);
   if (!ins_idx.empty()) {
-    for (size_t i = 0; i < ins_idx.size(); i++) off[i + 1] = off[i] + c->hcand[ins_idx[i]].len;
-    HIPCHK(c->stridx.ensure(8 * ins_idx.size()));
-    HIPCHK(c->stroff.ensure(8 * off.size()));
-    HIPCHK(c->strpool.ensure((size_t)off.back() + 16));
-    HIPCHK(hipMemcpyAsync(c->stridx.p, ins_idx.data(), 8 * ins_idx.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->stroff.p, off.data(), 8 * off.size(), hipMemcpyHostToDevice, c->stream));
-    k_ins_strings<<<dim3((unsigned)((ins_idx.size() + 63) / 64)), dim3(64), 0, c->stream>>>(
-        c->args, c->cands.as<Cand>(), c->stridx.as<int64_t>(), c->stroff.as<int64_t>(), (int32_t)ins_idx.size(),
-        c->strpool.as<uint8_t>());
+    for (size_t i = 0; i < ins_idx.size(); i++) off[i + 1] = off[i] + L->hcand[ins_idx[i]].len;
+    HIPCHK(L->stridx.ensure(8 * ins_idx.size()));
+    HIPCHK(L->stroff.ensure(8 * off.size()));
+    HIPCHK(L->strpool.ensure((size_t)off.back() + 16));
+    HIPCHK(hipMemcpyAsync(L->stridx.p, ins_idx.data(), 8 * ins_idx.size(), hipMemcpyHostToDevice, L->stream));
+    HIPCHK(hipMemcpyAsync(L->stroff.p, off.data(), 8 * off.size(), hipMemcpyHostToDevice, L->stream));
+    k_ins_strings<<<dim3((unsigned)((ins_idx.size() + 63) / 64)), dim3(64), 0, L->stream>>>(
+        L->args, L->cands.as<Cand>(), L->stridx.as<int64_t>(), L->stroff.as<int64_t>(), (int32_t)ins_idx.size(),
+        L->strpool.as<uint8_t>());
     HIPCHK(hipGetLastError());
     pool.resize((size_t)off.back());
-    HIPCHK(hipMemcpyAsync(&pool[0], c->strpool.p, (size_t)off.back(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&pool[0], L->strpool.p, (size_t)off.back(), hipMemcpyDeviceToHost, L->stream));
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
   size_t ii = 0;
   for (size_t i = 0; i < ch.size(); i++) {
     ch[i].base = cl[i].base;
@@ -583,21 +767,21 @@ pl_status job_changes(pl_ctx* c, size_t j, pl_region_stats* out, std::vector<plh
 }
 
 // pileup / final calls of job j of the last batch into host buffers
-pl_status job_export(pl_ctx* c, size_t j, int32_t* pile, pl_call* calls) {
-  const JobDev& J = c->hjobs[j];
-  if (calls && !c->exported_flags) {
-    k_final_flags<<<dim3(c->n_wins), dim3(256), 0, c->stream>>>(c->args);
+pl_status job_export(Lane* L, size_t j, int32_t* pile, pl_call* calls) {
+  const JobDev& J = L->hjobs[j];
+  if (calls && !L->exported_flags) {
+    k_final_flags<<<dim3(L->n_wins), dim3(256), 0, L->stream>>>(L->args);
     HIPCHK(hipGetLastError());
-    c->exported_flags = true;
+    L->exported_flags = true;
   }
   if (pile)
     for (int f = 0; f < PL_NFIELDS; f++)
-      HIPCHK(hipMemcpyAsync(pile + (size_t)f * J.size, c->pile.as<int32_t>() + (size_t)f * c->G + J.out,
-                            4 * (size_t)J.size, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(pile + (size_t)f * J.size, L->pile.as<int32_t>() + (size_t)f * L->G + J.out,
+                            4 * (size_t)J.size, hipMemcpyDeviceToHost, L->stream));
   if (calls)
-    HIPCHK(hipMemcpyAsync(calls, c->calls.as<pl_call>() + J.out, sizeof(pl_call) * (size_t)J.size,
-                          hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(calls, L->calls.as<pl_call>() + J.out, sizeof(pl_call) * (size_t)J.size,
+                          hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
   return PL_OK;
 }
 
@@ -704,8 +888,8 @@ pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out) {
   to_kconf(*cfg, c->cf);
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
-  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+  pl_status st = lane_init(&c->main);
+  if (st) { lane_destroy(&c->main); return st; }
   *out = c.release();
   return PL_OK;
 }
@@ -713,37 +897,33 @@ pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out) {
 void pl_ctx_destroy(pl_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  lane_destroy(&c->main);
   c->rsets.clear();
   for (auto& kv : c->contigs) if (kv.second.dev) (void)hipFree(kv.second.dev);
-  DBuf* bufs[] = {&c->rsarr, &c->kparr, &c->jobs, &c->wins, &c->worder, &c->costk, &c->costv, &c->costtk, &c->costtv,
-                  &c->pile, &c->calls, &c->st, &c->rec, &c->rkey, &c->rval, &c->rtk, &c->rtv, &c->cand, &c->cands,
-                  &c->ckey, &c->cval, &c->ctk, &c->ctv, &c->defer, &c->mutated, &c->counters, &c->hist, &c->tiles,
-                  &c->strpool, &c->stridx, &c->stroff};
-  for (DBuf* b : bufs) b->release();
-  for (int i = 0; i < 4; i++) (void)hipEventDestroy(c->ev[i]);
-  (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
 pl_status pl_ref_upload(pl_ctx* c, int32_t contig_id, const char* name, const uint8_t* ascii, int64_t len) {
+  Lane* L = c ? &c->main : nullptr;
   if (!c || !ascii || len <= 0) return fail(PL_E_INVAL, "bad reference");
   if (len >= ((int64_t)1 << 31) - 1) return fail(PL_E_INVAL, "contig longer than 2^31");
+  if (c->serving) return fail(PL_E_BUSY, "a server is open on this context: its contigs are fixed");
   HIPCHK(hipSetDevice(c->device));
   Contig& ct = c->contigs[contig_id];
-  if (ct.dev) { HIPCHK(hipStreamSynchronize(c->stream)); HIPCHK(hipFree(ct.dev)); ct.dev = nullptr; }
+  if (ct.dev) { HIPCHK(hipStreamSynchronize(L->stream)); HIPCHK(hipFree(ct.dev)); ct.dev = nullptr; }
   ct.name = name ? name : "";
   ct.host.assign(ascii, ascii + len);
   ct.len = len;
   HIPCHK(hipMalloc((void**)&ct.dev, (size_t)len + 64));
-  HIPCHK(hipMemcpyAsync(ct.dev, ascii, (size_t)len, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemsetAsync(ct.dev + len, 0, 64, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpyAsync(ct.dev, ascii, (size_t)len, hipMemcpyHostToDevice, L->stream));
+  HIPCHK(hipMemsetAsync(ct.dev + len, 0, 64, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
   return PL_OK;
 }
 
 pl_status pl_readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* reads, int32_t* id) {
   if (!c || !reads || !id) return fail(PL_E_INVAL, "null argument");
+  if (c->serving) return fail(PL_E_BUSY, "a server is open on this context: its read sets are fixed");
   HIPCHK(hipSetDevice(c->device));
   return readset_upload(c, contig_id, reads, id);
 }
@@ -785,10 +965,12 @@ pl_status pl_region_use_readset(pl_region* r, int32_t id) {
 pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_region_stats* stats) {
   if (!r) return fail(PL_E_INVAL, "null argument");
   pl_ctx* c = r->ctx;
+  Lane* L = &c->main;
   HIPCHK(hipSetDevice(c->device));
   int32_t id = r->readset;
   bool temp = false;
   if (id < 0) {
+    if (c->serving) return fail(PL_E_BUSY, "a server is open on this context: use a resident read set");
     if (r->batches.empty()) r->batches.emplace_back();
     pl_status s = upload_host_batches(c, r->contig, r->batches, &id);
     if (s) return s;
@@ -796,13 +978,12 @@ pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_regio
     r->batches.clear();
   }
   std::vector<JobReq> q{{id, r->start, r->stop}};
-  const int32_t W = 1024;
-  pl_status s = run_jobs(c, q, W, 0, 512);
-  if (!s) s = fetch_batch_meta(c);
-  if (!s) s = job_status(c, 0);
-  if (!s) s = job_changes(c, 0, &r->stats, &r->changes);
-  if (!s) s = job_export(c, 0, pile, calls);
-  if (temp) { c->rsets[id].reset(); c->rs_dirty = true; }
+  pl_status s = run_jobs(c, L, q, PL_DEFAULT_WINDOW, 0, PL_DEFAULT_BLOCK, false);
+  if (!s) s = fetch_batch_meta(L);
+  if (!s) s = job_status(L, 0);
+  if (!s) s = job_changes(c, L, 0, &r->stats, &r->changes);
+  if (!s) s = job_export(L, 0, pile, calls);
+  if (temp) { c->rsets[id].reset(); c->rs_version++; }
   if (s) return s;
   if (stats) *stats = r->stats;
   r->finished = true;
@@ -835,6 +1016,7 @@ void pl_free(void* p) { free(p); }
 pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, const pl_batch_opts* opts, int32_t* pile,
                        pl_call* calls, pl_region_stats* stats) {
   if (!c || !regions || n <= 0) return fail(PL_E_INVAL, "empty batch");
+  Lane* L = &c->main;
   HIPCHK(hipSetDevice(c->device));
   std::vector<JobReq> q((size_t)n);
   for (int32_t i = 0; i < n; i++) {
@@ -844,26 +1026,30 @@ pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, cons
       return fail(PL_E_INVAL, "region %d: read set %d is not a set of contig %d", i, d.readset_id, d.contig_id);
     q[i] = {d.readset_id, d.start, d.stop};
   }
-  const int32_t W = (opts && opts->window) ? opts->window : 1024;
+  int32_t W = opts ? opts->window : 0;
+  int32_t block = opts ? opts->block : 0;
   const int persistent = opts ? opts->persistent : 0;
-  const int block = (opts && opts->block) ? opts->block : 512;
-  pl_status s = run_jobs(c, q, W, persistent, block);
+  const bool use_graph = opts && opts->use_graph;
+  pl_status s;
+  if (W == 0 || block == 0) {
+    if ((s = autotune(c, L, q, persistent, &W, &block))) return s;
+  }
+  if ((s = run_jobs(c, L, q, W, persistent, block, use_graph))) return s;
+  s = fetch_batch_meta(L);
   if (s) return s;
-  s = fetch_batch_meta(c);
-  if (s) return s;
-  for (int32_t i = 0; i < n; i++) if ((s = job_status(c, i))) return s;
+  for (int32_t i = 0; i < n; i++) if ((s = job_status(L, i))) return s;
   if (stats)
-    for (int32_t i = 0; i < n; i++) if ((s = job_changes(c, i, &stats[i], nullptr))) return s;
+    for (int32_t i = 0; i < n; i++) if ((s = job_changes(c, L, i, &stats[i], nullptr))) return s;
   if (pile || calls) {
     int64_t off = 0;
     for (int32_t i = 0; i < n; i++) {
-      const int64_t sz = c->hjobs[i].size;
+      const int64_t sz = L->hjobs[i].size;
       std::vector<int32_t> tmp;
       if (pile) tmp.resize((size_t)PL_NFIELDS * sz);
-      if ((s = job_export(c, i, pile ? tmp.data() : nullptr, calls ? calls + off : nullptr))) return s;
+      if ((s = job_export(L, i, pile ? tmp.data() : nullptr, calls ? calls + off : nullptr))) return s;
       if (pile)
         for (int f = 0; f < PL_NFIELDS; f++)
-          std::memcpy(pile + (size_t)f * c->G + off, tmp.data() + (size_t)f * sz, 4 * (size_t)sz);
+          std::memcpy(pile + (size_t)f * L->G + off, tmp.data() + (size_t)f * sz, 4 * (size_t)sz);
       off += sz;
     }
   }
@@ -871,26 +1057,32 @@ pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, cons
 }
 
 pl_status pl_last_batch_info(pl_ctx* c, pl_batch_info* o) {
+  Lane* L = c ? &c->main : nullptr;
   if (!c || !o) return fail(PL_E_INVAL, "null argument");
   std::memset(o, 0, sizeof *o);
-  o->positions = c->G;
-  o->n_regions = (int32_t)c->hjobs.size();
-  o->n_windows = c->n_wins;
-  o->n_records = c->last_counts[0];
-  o->n_candidates = c->last_counts[1];
-  o->n_deferred = c->last_counts[2];
-  o->window = c->W;
-  o->persistent = c->last_persistent;
-  o->pipeline_ms = c->last_ms;
-  o->window_ms = c->last_win_ms;
+  o->positions = L->G;
+  o->n_regions = (int32_t)L->hjobs.size();
+  o->n_windows = L->n_wins;
+  o->n_records = L->last_counts[0];
+  o->n_candidates = L->last_counts[1];
+  o->n_deferred = L->last_counts[2];
+  o->window = L->W;
+  o->persistent = L->last_persistent;
+  o->block = L->last_block;
+  o->use_graph = L->last_graph ? 1 : 0;
+  o->pipeline_ms = L->last_ms;
+  o->window_ms = L->last_win_ms;
   return PL_OK;
 }
 
 pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
+  Lane* L = c ? &c->main : nullptr;
   if (!c) return fail(PL_E_INVAL, "null argument");
-  if (pipeline_ms) *pipeline_ms = c->last_ms;
-  if (window_ms) *window_ms = c->last_win_ms;
+  if (pipeline_ms) *pipeline_ms = L->last_ms;
+  if (window_ms) *window_ms = L->last_win_ms;
   return PL_OK;
 }
 
 }  // extern "C"
+
+#include "pl_serve.hip"

@@ -9,3 +9,8 @@
 #ifndef PL_UNROLL
 #define PL_UNROLL 8               // bases per lane per iteration of the flattened loop
 #endif
+#ifndef PL_WIN_WAVES_EU
+#define PL_WIN_WAVES_EU 1         // minimum waves per SIMD the window kernel is compiled for
+#endif
+#define PL_DEFAULT_WINDOW 512     // plain region form and autotuner fallback
+#define PL_DEFAULT_BLOCK 256

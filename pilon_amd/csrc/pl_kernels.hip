@@ -45,6 +45,18 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
   return v;
 }
 
+// slot of this lane in a list grown by the lanes of the wave whose pred is
+// set: one atomic on the list counter per wave
+__device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if (!m) return 0u;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0u;
+  if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  return base + (uint32_t)__popcll(m & lanemask_lt());
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -77,7 +89,7 @@ __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount
       }
       if (lo <= hi) {
         const int64_t lead = m.pos - lo, back = hi - m.pos;
-        if (lead > PL_MAX_LEAD || back > PL_MAX_BACK) {
+        if (lead > PL_MAX_LEAD || back > PL_MAX_BACK || R.seq_off[r + 1] - R.seq_off[r] >= PL_MAX_SEQ) {
           m.info |= RI_OUTLIER;
           int k = atomicAdd(&scal[3], 1);
           if (k < out_cap) outliers[k] = r;
@@ -249,6 +261,13 @@ __global__ void k_init_stats(JobStat* st, int32_t n_jobs, uint32_t* counters, in
   const int64_t words = (int64_t)n_jobs * (sizeof(JobStat) / 4);
   for (int64_t k = i; k < words; k += (int64_t)gridDim.x * blockDim.x) ((uint32_t*)st)[k] = 0;
 }
+// identity permutation + its length, for the window cost sort (a kernel,
+// not a host copy, so the whole batch pipeline can be captured in a graph)
+__global__ void k_iota(uint32_t* v, int32_t n, uint32_t* n_out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = (uint32_t)i;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *n_out = (uint32_t)n;
+}
 __global__ void k_init_stats2(JobStat* st, int32_t n_jobs) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n_jobs) { st[j].err_read_edge = INT64_MAX; st[j].err_read_inval = INT64_MAX; }
@@ -295,6 +314,19 @@ struct LdsSink {
   __device__ void edge(int64_t r) { atomicMin((unsigned long long*)&st->err_read_edge, (unsigned long long)r); }
 };
 
+// global-address-space views: loads through them compile to global_load_*
+// (vector memory counter only) instead of flat loads that also hold the LDS
+// counter the atomics wait on
+typedef const int8_t __attribute__((address_space(1))) gi8;
+typedef const uint8_t __attribute__((address_space(1))) gu8;
+
+struct __attribute__((aligned(16))) RunL {  // one run of a tile, one 16-byte LDS read
+  int32_t off;     // first flattened base of the run
+  uint32_t pi;     // window position | EF_* flags | mq1 << 16
+  int32_t q0;      // sequence offset of the run's first base, from the tile's 4-aligned base
+  int32_t pad;
+};
+
 __device__ __forceinline__ int64_t uniform64(int64_t v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
@@ -312,9 +344,8 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
 // conflict-free LDS atomics.  Reads with clips, indels or exotic bases go
 // through walk_read, one wave per read.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) {
+__global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A, int persistent) {
   constexpr int NW = BLOCK / 64;
-  constexpr int T = BLOCK;
   extern __shared__ unsigned long long lds[];
   const int W = A.W;
   unsigned long long* bs = lds;
@@ -326,12 +357,7 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
   int32_t* pcA = (int32_t*)(clpA + W);
   int32_t* isA = pcA + W;
   uint32_t* hist = (uint32_t*)(isA + W);
-  __shared__ int32_t s_off[2 * T + 1];
-  __shared__ uint32_t s_pi[2 * T];
-  __shared__ int32_t s_q0[2 * T];
-  __shared__ int32_t s_cplx[T];
-  __shared__ int32_t s_ncplx, s_nrun;
-  __shared__ uint64_t ws_r[NW];
+  __shared__ RunL s_run[NW * (2 * 64 + 1)];
   __shared__ int32_t s_w;
   __shared__ int32_t ws_a[NW], ws_b[NW];
   __shared__ unsigned long long red[2][NW];
@@ -348,7 +374,9 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
       const int32_t w = s_w;
       __syncthreads();
       if (w >= A.n_wins) break;
-      wi = A.worder[w];
+      // the index comes through LDS: make it wave-uniform so every
+      // descriptor field below is a scalar load
+      wi = __builtin_amdgcn_readfirstlane(A.worder[__builtin_amdgcn_readfirstlane(w)]);
     } else {
       if (iter > 0) break;
       wi = blockIdx.x;
@@ -358,8 +386,13 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
     const JobDev& J = A.jobs[win.job];
     const RSDev& R = A.rs[J.rs];
     const int32_t L0 = J.start + win.p0, L1 = J.start + win.p1, nW = win.p1 - win.p0;
+    // hot streams of this read set, held in scalar registers for the window
+    const int8_t* __restrict__ gq = R.quals;
+    const uint8_t* __restrict__ gb = R.bases2;
+    const int64_t* __restrict__ gso = R.seq_off;
+    const RMeta* __restrict__ gm = R.meta;
     for (int k = tid; k < lds_words; k += BLOCK) lds[k] = 0ull;
-    if (tid == 0) { s_ovf = 0; s_ncplx = 0; }
+    if (tid == 0) s_ovf = 0;
     __syncthreads();
 
     LdsSink sk;
@@ -367,99 +400,116 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
     sk.pc = pcA; sk.is = isA; sk.rec = A.rec; sk.rec_n = A.rec_n; sk.rec_cap = A.rec_cap;
     sk.overflow = A.overflow; sk.st = &A.st[win.job]; sk.bcount = 0; sk.aligned_n = 0;
 
-    for (int64_t t0 = win.lo; t0 < win.hi; t0 += T) {
-      const int64_t qbase = R.seq_off[t0];
-      // ---- stage: one read per thread -> up to two runs
-      const int64_t r = t0 + tid;
+    // Each wave takes its own chunks of 64 reads (wave-strided over the
+    // window's read range): staging, the run scan and the base loop are
+    // wave-local, so the read loop has no workgroup barrier.
+    RunL* wrun = s_run + wave * (2 * 64 + 1);
+    for (int64_t t0 = win.lo + 64 * wave; t0 < win.hi; t0 += BLOCK) {
+      const int64_t qbase = gso[t0];
+      // ---- stage: one read per lane -> up to two runs
+      const int64_t r = t0 + lane;
       Run ru0, ru1;
       ru0.n = 0; ru1.n = 0;
       int nrun = 0;
+      bool cplx = false;
       if (r < win.hi) {
-        const RMeta m = R.meta[r];
-        const int64_t seq0 = R.seq_off[r];
+        const RMeta m = gm[r];
+        const int64_t seq0 = gso[r];
         if (!(m.info & RI_OUTLIER) && read_in_query(m, J)) {
           frag_event(m, J, L0, L1, sk);
           if (m.fp_hi >= L0 && m.fp_lo < L1) {
-            if (m.info & RI_SIMPLE) {
+            if (seq0 - qbase >= PL_TILE_SPAN) {
+              cplx = true;
+            } else if (m.info & RI_SIMPLE) {
               nrun = simple_run(m, seq0, cf, L0, L1, &ru0, &sk.aligned_n, &sk.bcount);
             } else {
               int got = -1;
               const uint32_t nseg = (m.info & RI_SEG_MASK) >> RI_SEG_SHIFT;
               if (!(m.info & RI_EXOTIC) && nseg < RI_SEG_MAX) {
-                const int32_t L = (int32_t)(R.seq_off[r + 1] - seq0);
+                const int32_t L = (int32_t)(gso[r + 1] - seq0);
                 got = stage_segs(R, J, cf, r, m, seq0, L, L0, L1, ru0, ru1, &sk.aligned_n, &sk.bcount, sk);
               }
-              if (got < 0) {
-                const int k = atomicAdd(&s_ncplx, 1);
-                s_cplx[k] = tid;
-              } else {
-                nrun = got;
-              }
+              if (got < 0) cplx = true;
+              else nrun = got;
             }
           }
         }
       }
-      // ---- exclusive scan of (runs, bases) over the tile
+      // ---- wave-local exclusive scan of (runs, bases)
+      const uint64_t v = (nrun > 0 ? (1ull << 32) + (uint64_t)ru0.n : 0ull) +
+                         (nrun > 1 ? (1ull << 32) + (uint64_t)ru1.n : 0ull);
+      const uint64_t inc = wave_incl_scan(v);
+      const uint64_t tot = __shfl(inc, 63, 64);
       {
-        const uint64_t v = (nrun > 0 ? (1ull << 32) + (uint64_t)ru0.n : 0ull) +
-                           (nrun > 1 ? (1ull << 32) + (uint64_t)ru1.n : 0ull);
-        const uint64_t inc = wave_incl_scan(v);
-        if (lane == 63) ws_r[wave] = inc;
-        __syncthreads();
-        uint64_t ex = inc - v;
-        for (int w2 = 0; w2 < wave; w2++) ex += ws_r[w2];
+        const uint64_t ex = inc - v;
         int32_t ei = (int32_t)(ex >> 32), off = (int32_t)(uint32_t)ex;
+        const int32_t qsh = (int32_t)(qbase & 3);
         if (nrun > 0) {
-          s_off[ei] = off; s_pi[ei] = (uint32_t)ru0.p0 | ru0.fl; s_q0[ei] = (int32_t)(ru0.q - qbase);
+          wrun[ei] = RunL{off, (uint32_t)ru0.p0 | ru0.fl, (int32_t)(ru0.q - qbase) + qsh, 0};
           off += ru0.n; ei++;
         }
         if (nrun > 1) {
-          s_off[ei] = off; s_pi[ei] = (uint32_t)ru1.p0 | ru1.fl; s_q0[ei] = (int32_t)(ru1.q - qbase);
-          off += ru1.n; ei++;
+          wrun[ei] = RunL{off, (uint32_t)ru1.p0 | ru1.fl, (int32_t)(ru1.q - qbase) + qsh, 0};
         }
-        if (tid == T - 1) { s_off[ei] = off; s_nrun = ei; }
       }
-      __syncthreads();
-      // ---- flattened bases of all runs
-      const int32_t E = s_nrun;
-      const int32_t B = s_off[E];
-      const int32_t per = ((B + NW * 64 - 1) / (NW * 64)) * 64;
-      const int32_t k0 = wave * per;
-      const int32_t k1 = B < k0 + per ? B : k0 + per;
-      if (k0 < k1) {
-        int lo = 0, hi = E;
-        while (lo < hi) {
-          const int m = (lo + hi + 1) >> 1;
-          if (s_off[m] <= k0) lo = m; else hi = m - 1;
-        }
-        int cur = lo;
+      const int32_t E = (int32_t)(tot >> 32);
+      const int32_t B = (int32_t)(uint32_t)tot;
+      if (lane == 0) wrun[E] = RunL{B, 0u, 0, 0};
+      // LDS operations of one wave complete in order: the table written
+      // above is what the loop below reads
+      __builtin_amdgcn_wave_barrier();
+      // ---- flattened bases of all runs of the chunk
+      if (B > 0) {
+        const int32_t k1 = B;
+        int cur = 0;
         const int32_t dq = cf.default_qual, mq = cf.min_qual;
-        for (int32_t base = k0; base < k1; base += 64 * PL_UNROLL) {
-          int rr = cur;
+        // chunk-relative global pointers: 32-bit lane offsets from a uniform base
+        const int64_t qb4 = qbase & ~(int64_t)3;
+        const gi8* tq = (const gi8*)(gq + qb4);
+        const gu8* tb = (const gu8*)(gb + (qb4 >> 2));
+        for (int32_t base = 0; base < k1; base += 64 * PL_UNROLL) {
+          // run of each k = the last run whose first flattened base is <= k.
+          // The run starts inside this iteration's span are read once, 64 at
+          // a time (one LDS read per lane), and counted per k with scalar
+          // compares: no dependent LDS round trips per base.
+          const int32_t lim = (k1 < base + 64 * PL_UNROLL) ? k1 : base + 64 * PL_UNROLL;
+          int rr[PL_UNROLL];
+#pragma unroll
+          for (int j = 0; j < PL_UNROLL; j++) rr[j] = cur;
+          for (int c = cur;;) {
+            const int ib = c + 1 + lane;
+            const int32_t bnd = ib <= E ? wrun[ib].off : INT32_MAX;
+            const uint64_t mk = __ballot(bnd < lim);
+            const int nb = __popcll(mk);
+            for (int i = 0; i < nb; i++) {
+              const int32_t b = __builtin_amdgcn_readlane(bnd, i);
+#pragma unroll
+              for (int j = 0; j < PL_UNROLL; j++) rr[j] += (base + j * 64 + lane >= b) ? 1 : 0;
+            }
+            if (nb < 64) break;
+            c += 64;
+          }
           int32_t pj[PL_UNROLL];
-          int64_t qj[PL_UNROLL];
+          int32_t qj[PL_UNROLL];
           uint32_t prj[PL_UNROLL];
           bool act[PL_UNROLL];
 #pragma unroll
           for (int j = 0; j < PL_UNROLL; j++) {
             const int32_t k = base + j * 64 + lane;
             act[j] = k < k1;
-            pj[j] = 0; qj[j] = qbase; prj[j] = 0;
-            if (act[j]) {
-              while (k >= s_off[rr + 1]) rr++;
-              prj[j] = s_pi[rr];
-              const int32_t d = k - s_off[rr];
-              pj[j] = (int)(prj[j] & 0xffffu) + d;
-              qj[j] = qbase + s_q0[rr] + d;
-            }
+            const RunL ru = wrun[act[j] ? rr[j] : cur];
+            prj[j] = act[j] ? ru.pi : 0u;
+            const int32_t d = k - ru.off;
+            pj[j] = (int)(ru.pi & 0xffffu) + d;
+            qj[j] = ru.q0 + d;
           }
           int32_t qv[PL_UNROLL];
           uint32_t bv[PL_UNROLL];
 #pragma unroll
           for (int j = 0; j < PL_UNROLL; j++) {
             const bool ld = act[j] && (prj[j] & EF_VALID);
-            qv[j] = ld ? (int32_t)R.quals[qj[j]] : 0;
-            bv[j] = ld ? (uint32_t)R.bases2[qj[j] >> 2] : 0u;
+            qv[j] = ld ? (int32_t)tq[qj[j]] : 0;
+            bv[j] = ld ? (uint32_t)tb[(uint32_t)qj[j] >> 2] : 0u;
           }
 #pragma unroll
           for (int j = 0; j < PL_UNROLL; j++) {
@@ -477,19 +527,15 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
               atomicAdd(&badA[pj[j]], 1u);
             }
           }
-          const int32_t kl = (k1 - 1 < base + 64 * PL_UNROLL - 1) ? k1 - 1 : base + 64 * PL_UNROLL - 1;
-          cur = __shfl(rr, (kl - base) & 63, 64);
+          cur = __shfl(rr[PL_UNROLL - 1], 63, 64);
         }
       }
-      // ---- everything else about these reads, one wave per read
-      const int nc = s_ncplx;
-      for (int i = wave; i < nc; i += NW) {
-        const int64_t rc = uniform64(t0 + s_cplx[i]);
-        walk_read(R, J, cf, rc, L0, L1, lane, 64, sk, false);
+      // ---- everything else about the chunk's complex reads, one at a time
+      for (uint64_t cm = __ballot(cplx); cm; cm &= cm - 1) {
+        const int l = __ffsll((unsigned long long)cm) - 1;
+        walk_read(R, J, cf, t0 + l, L0, L1, lane, 64, sk, false);
       }
-      __syncthreads();
-      if (tid == 0) s_ncplx = 0;
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
     }
     for (int64_t k = wave; k < R.n_outlier; k += NW) {
       const int64_t r = uniform64(R.outliers[k]);
@@ -534,19 +580,21 @@ __global__ void __launch_bounds__(BLOCK) k_window(EngineArgs A, int persistent) 
       for (int f = 0; f < PL_NFIELDS; f++) A.pile[(int64_t)f * G + g] = o.f[f];
       A.calls[g] = o.call;
       if (o.overflow) s_ovf = 1;
+      const int d = call_disp(o.call);
+      const int64_t dep = (int64_t)(uint32_t)o.f[0] + (uint32_t)o.f[1] + (uint32_t)o.f[2] + (uint32_t)o.f[3] +
+                          o.f[PL_F_DELETIONS];
+      const bool is_cand = !o.defer && (d == PL_D_SNP || d == PL_D_AMB);
+      if (!o.defer && d == PL_D_CONFIRMED) atomicAdd(&hist[dep > 255 ? 255 : (dep < 0 ? 0 : dep)], 1u);
+      // one global atomic per wave for each list
+      const uint32_t kd = wave_append(A.defer_n, o.defer);
+      const uint32_t kc = wave_append(A.cand_n, is_cand);
       if (o.defer) {
-        uint32_t k = atomicAdd(A.defer_n, 1u);
-        if (k < A.defer_cap) A.defer[k] = (uint32_t)g;
+        if (kd < A.defer_cap) A.defer[kd] = (uint32_t)g;
         else *A.overflow = 1;
-      } else {
-        const int d = call_disp(o.call);
-        int64_t dep = (int64_t)(uint32_t)o.f[0] + (uint32_t)o.f[1] + (uint32_t)o.f[2] + (uint32_t)o.f[3] + o.f[PL_F_DELETIONS];
-        if (d == PL_D_CONFIRMED) atomicAdd(&hist[dep > 255 ? 255 : (dep < 0 ? 0 : dep)], 1u);
-        else if (d == PL_D_SNP || d == PL_D_AMB) {
-          uint32_t k = atomicAdd(A.cand_n, 1u);
-          if (k < A.cand_cap) { Cand c; c.gpos = (uint32_t)g; c.depth = (int32_t)dep; c.kind = d; c.len = 0; c.rec = -1; A.cand[k] = c; }
-          else *A.overflow = 1;
-        }
+      }
+      if (is_cand) {
+        if (kc < A.cand_cap) { Cand c; c.gpos = (uint32_t)g; c.depth = (int32_t)dep; c.kind = d; c.len = 0; c.rec = -1; A.cand[kc] = c; }
+        else *A.overflow = 1;
       }
     }
     // region baseCount and the aligned-base unit

@@ -16,8 +16,9 @@ import os
 
 import numpy as np
 
-from .abi import (PlConfig, PlReadSoa, PlCall, PlRegionStats, PlRegionDesc, PlBatchOpts, PlBatchInfo, NFIELDS, F,
-                  STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK, PL_CF_DISP_SHIFT)
+from .abi import (PlConfig, PlReadSoa, PlCall, PlRegionStats, PlRegionDesc, PlBatchOpts, PlBatchInfo, PlServeOpts,
+                  PlServeResult, NFIELDS, F, STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK,
+                  PL_CF_DISP_SHIFT)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libpilonhip.so")
@@ -57,6 +58,12 @@ def lib():
                                        C.c_void_p, C.c_void_p]),
             "pl_last_timing": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double)]),
             "pl_last_batch_info": (C.c_int, [C.c_void_p, P(PlBatchInfo)]),
+            "pl_serve_open": (C.c_int, [C.c_void_p, P(PlServeOpts), P(C.c_void_p)]),
+            "pl_serve_submit": (C.c_int, [C.c_void_p, P(PlRegionDesc), C.c_int64, P(C.c_int64)]),
+            "pl_serve_poll": (C.c_int, [C.c_void_p, P(PlServeResult), C.c_int32, C.c_int32, P(C.c_int32)]),
+            "pl_serve_fetch": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+            "pl_serve_drain": (C.c_int, [C.c_void_p]),
+            "pl_serve_close": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -105,15 +112,19 @@ class Context:
         n = len(regions)
         return (PlRegionDesc * n)(*[PlRegionDesc(c, s, e, r) for c, s, e, r in regions])
 
-    def batch_run(self, regions, window=0, persistent=0, copy_out=True, want_stats=True, descs=None, block=0):
+    def batch_run(self, regions, window=0, persistent=0, copy_out=True, want_stats=True, descs=None, block=0,
+                  use_graph=False):
         """regions: list of (contig_id, start, stop, readset_id).  Returns
         (pile [NFIELDS, G], calls [G], stats list) — arrays are None when
         copy_out is False (results stay on the device), stats is None when
-        want_stats is False."""
+        want_stats is False.  window/block 0 = autotuned (cached per batch
+        shape); use_graph replays the captured device pipeline when the batch
+        repeats."""
         n = len(regions)
         if descs is None:
             descs = self.batch_descs(regions)
-        opts = PlBatchOpts(window=window, block=block, persistent=persistent, use_graph=0, copy_out=int(copy_out))
+        opts = PlBatchOpts(window=window, block=block, persistent=persistent, use_graph=int(bool(use_graph)),
+                           copy_out=int(copy_out))
         G = sum(e - s + 1 for _, s, e, _ in regions)
         stats = (PlRegionStats * n)()
         pile = calls = None
@@ -134,6 +145,55 @@ class Context:
         a, b = C.c_double(), C.c_double()
         _check(lib().pl_last_timing(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class Server:
+    """pl_server: the serving form.  Requests (one region each) are admitted
+    by estimated device memory and run with continuous batching on `lanes`
+    concurrent batch pipelines over the context's resident data."""
+
+    def __init__(self, ctx, lanes=2, max_batch=256, mem_budget=0, max_queue=0, window=0, block=0, copy_out=False):
+        self.ctx = ctx
+        self.copy_out = bool(copy_out)
+        opts = PlServeOpts(lanes=lanes, max_batch=max_batch, mem_budget=int(mem_budget), max_queue=max_queue,
+                           window=window, block=block, copy_out=int(bool(copy_out)))
+        h = C.c_void_p()
+        _check(lib().pl_serve_open(ctx.h, C.byref(opts), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def submit(self, contig_id, start, stop, readset_id, tag=0):
+        """Queues one region; returns its ticket.  PilonError with status
+        PL_E_REJECTED (alone over the budget) or PL_E_BUSY (queue full)."""
+        t = C.c_int64()
+        _check(lib().pl_serve_submit(self.h, C.byref(PlRegionDesc(contig_id, start, stop, readset_id)), int(tag),
+                                     C.byref(t)))
+        self.sizes[t.value] = stop - start + 1
+        return t.value
+
+    def poll(self, max_n=4096, wait_ms=0):
+        buf = (PlServeResult * max_n)()
+        n = C.c_int32()
+        _check(lib().pl_serve_poll(self.h, buf, max_n, int(wait_ms), C.byref(n)))
+        return [buf[i].as_dict() for i in range(n.value)]
+
+    def fetch(self, ticket):
+        G = self.sizes.pop(ticket)
+        pile = np.zeros((NFIELDS, G), np.int32)
+        calls = np.zeros(G, call_dtype())
+        _check(lib().pl_serve_fetch(self.h, int(ticket), pile.ctypes.data, calls.ctypes.data))
+        return pile, calls
+
+    def drain(self):
+        _check(lib().pl_serve_drain(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pl_serve_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class BaseSum:

@@ -192,3 +192,25 @@ class SynthData:
                          arr(s.bases2, (nb + 3) // 4 + 8, C.c_uint8, np.uint8),
                          arr(s.quals, nb + 8, C.c_int8, np.int8), ex_r, ex_o,
                          arr(s.nib, nib_len, C.c_uint8, np.uint8))
+
+
+def serve_requests(data, n, lo=2_000, hi=1_000_000, seed=1004):
+    """A polish request stream: n regions of log-uniform length in [lo, hi]
+    (capped at the contig length) at a uniform random offset of a uniform
+    random contig; returns a list of (contig index, start, stop), 1-based."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        ci = int(rng.integers(0, data.n))
+        L = len(data.draft(ci))
+        ln = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        ln = max(1, min(ln, L))
+        s = int(rng.integers(1, L - ln + 2))
+        out.append((ci, s, s + ln - 1))
+    return out
+
+
+def poisson_arrivals(n, rate, seed=1004):
+    """Arrival times (seconds from 0) of n requests at `rate` per second."""
+    rng = np.random.default_rng(seed + 1)
+    return np.cumsum(rng.exponential(1.0 / rate, n))

@@ -9,5 +9,8 @@ int main(void) {
          sizeof(pl_region_desc), sizeof(pl_batch_opts), sizeof(pl_batch_info));
   P(pl_config, min_depth); P(pl_config, duplicates); P(pl_read_soa, nib); P(pl_read_soa, n_exotic);
   P(pl_region_stats, aligned_bases); P(pl_batch_info, window_ms); P(pl_call, q);
+  printf("pl_serve_opts %zu\npl_serve_result %zu\n", sizeof(pl_serve_opts), sizeof(pl_serve_result));
+  P(pl_serve_opts, mem_budget); P(pl_serve_opts, copy_out); P(pl_serve_result, stats);
+  P(pl_serve_result, est_bytes); P(pl_serve_result, batch_device_ms);
   return 0;
 }

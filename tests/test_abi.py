@@ -60,7 +60,8 @@ def test_struct_layouts_match_c(tmp_path):
     out = dict(l.rsplit(" ", 1) for l in subprocess.check_output([str(exe)]).decode().strip().split("\n"))
     sizes = {"pl_config": abi.PlConfig, "pl_read_soa": abi.PlReadSoa, "pl_call": abi.PlCall,
              "pl_region_stats": abi.PlRegionStats, "pl_region_desc": abi.PlRegionDesc,
-             "pl_batch_opts": abi.PlBatchOpts, "pl_batch_info": abi.PlBatchInfo}
+             "pl_batch_opts": abi.PlBatchOpts, "pl_batch_info": abi.PlBatchInfo,
+             "pl_serve_opts": abi.PlServeOpts, "pl_serve_result": abi.PlServeResult}
     for name, cls in sizes.items():
         assert int(out[name]) == C.sizeof(cls), name
     for key, val in out.items():
