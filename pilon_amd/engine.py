@@ -97,7 +97,7 @@ class Context:
         self.close()
 
     def ref_upload(self, contig_id, name, bases):
-        bases = bytes(bases)
+        bases = bases.encode("ascii") if isinstance(bases, str) else bytes(bases)
         _check(lib().pl_ref_upload(self.h, contig_id, name.encode(), bases, len(bases)))
         self.contigs[contig_id] = (name, bases)
 
