@@ -432,7 +432,7 @@ pl_status enqueue_batch(Lane* L, EngineArgs& A, int persistent, int block, int g
 }
 
 // LDS the window kernel needs for a window of W positions (dynamic part)
-size_t window_lds(int32_t W) { return 72 * (size_t)W + 1024; }
+size_t window_lds(int32_t W) { return PL_WIN_LDS_PER_POS * (size_t)W + 1024; }
 // ... and per workgroup in total, with the per-wave run tables
 size_t window_lds_total(int32_t W, int block) { return window_lds(W) + 32 * (size_t)block + 1024; }
 

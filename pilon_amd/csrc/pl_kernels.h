@@ -14,3 +14,4 @@
 #endif
 #define PL_DEFAULT_WINDOW 512     // plain region form and autotuner fallback
 #define PL_DEFAULT_BLOCK 256
+#define PL_WIN_LDS_PER_POS 76     // window LDS bytes per position (counters + difference arrays)

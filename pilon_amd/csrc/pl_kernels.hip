@@ -343,6 +343,8 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
 // which consecutive lanes take consecutive bases — coalesced loads and
 // conflict-free LDS atomics.  Reads with clips, indels or exotic bases go
 // through walk_read, one wave per read.
+static_assert(PL_UNROLL >= 1 && PL_UNROLL <= 8, "the run lookup packs PL_UNROLL 4-bit counts in 32 bits");
+
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A, int persistent) {
   constexpr int NW = BLOCK / 64;
@@ -356,14 +358,15 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
   uint32_t* clpA = badA + W;
   int32_t* pcA = (int32_t*)(clpA + W);
   int32_t* isA = pcA + W;
-  uint32_t* hist = (uint32_t*)(isA + W);
+  int32_t* bdA = isA + W;   // badPair differences of bad runs (clips, invalid reads' M bases)
+  uint32_t* hist = (uint32_t*)(bdA + W);
   __shared__ RunL s_run[NW * (2 * 64 + 1)];
   __shared__ int32_t s_w;
-  __shared__ int32_t ws_a[NW], ws_b[NW];
+  __shared__ int32_t ws_a[NW], ws_b[NW], ws_c[NW];
   __shared__ unsigned long long red[2][NW];
   __shared__ int32_t s_ovf;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lds_words = (72 * W) / 8 + 256 / 2;
+  const int lds_words = (PL_WIN_LDS_PER_POS * W) / 8 + 256 / 2;
   const KConf& cf = A.cf;
 
   for (int iter = 0;; iter++) {
@@ -435,6 +438,17 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
           }
         }
       }
+      // runs that only add badPair over a range (soft clips, M bases of
+      // invalid reads) go to the difference array, not to the base loop
+      if (nrun > 1 && (ru1.fl & EF_BAD)) {
+        if (ru1.n > 0) { atomicAdd(&bdA[ru1.p0], 1); if (ru1.p0 + ru1.n < nW) atomicAdd(&bdA[ru1.p0 + ru1.n], -1); }
+        nrun = 1;
+      }
+      if (nrun > 0 && (ru0.fl & EF_BAD)) {
+        if (ru0.n > 0) { atomicAdd(&bdA[ru0.p0], 1); if (ru0.p0 + ru0.n < nW) atomicAdd(&bdA[ru0.p0 + ru0.n], -1); }
+        if (nrun > 1) ru0 = ru1;
+        nrun--;
+      }
       // ---- wave-local exclusive scan of (runs, bases)
       const uint64_t v = (nrun > 0 ? (1ull << 32) + (uint64_t)ru0.n : 0ull) +
                          (nrun > 1 ? (1ull << 32) + (uint64_t)ru1.n : 0ull);
@@ -470,27 +484,53 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
         for (int32_t base = 0; base < k1; base += 64 * PL_UNROLL) {
           // run of each k = the last run whose first flattened base is <= k.
           // The run starts inside this iteration's span are read once, 64 at
-          // a time (one LDS read per lane), and counted per k with scalar
-          // compares: no dependent LDS round trips per base.
+          // a time (one LDS read per lane); each one adds 1 to the counts of
+          // every unrolled k at or past it, all U counts packed 4 bits apart
+          // in one register (one shift and add per start).  More than 15
+          // starts in one span (runs shorter than ~34 bases) take the plain
+          // per-k compares.
           const int32_t lim = (k1 < base + 64 * PL_UNROLL) ? k1 : base + 64 * PL_UNROLL;
           int rr[PL_UNROLL];
-#pragma unroll
-          for (int j = 0; j < PL_UNROLL; j++) rr[j] = cur;
-          for (int c = cur;;) {
-            const int ib = c + 1 + lane;
-            const int32_t bnd = ib <= E ? wrun[ib].off : INT32_MAX;
-            const uint64_t mk = __ballot(bnd < lim);
-            const int nb = __popcll(mk);
-            for (int i = 0; i < nb; i++) {
-              const int32_t b = __builtin_amdgcn_readlane(bnd, i);
-#pragma unroll
-              for (int j = 0; j < PL_UNROLL; j++) rr[j] += (base + j * 64 + lane >= b) ? 1 : 0;
+          {
+            uint32_t cntp = 0u;
+            int total = 0;
+            for (int c = cur;;) {
+              const int ib = c + 1 + lane;
+              const int32_t bnd = ib <= E ? wrun[ib].off : INT32_MAX;
+              const int nb = __popcll(__ballot(bnd < lim));
+              total += nb;
+              if (total <= 15) {
+                for (int i = 0; i < nb; i++) {
+                  const int32_t t = __builtin_amdgcn_readlane(bnd, i) - base - lane;   // first k index >= start
+                  const int j0 = t <= 0 ? 0 : (t + 63) >> 6;
+                  cntp += j0 < PL_UNROLL ? (0x11111111u << (4 * j0)) : 0u;
+                }
+              }
+              if (nb < 64) break;
+              c += 64;
             }
-            if (nb < 64) break;
-            c += 64;
+            if (total <= 15) {
+#pragma unroll
+              for (int j = 0; j < PL_UNROLL; j++) rr[j] = cur + (int)((cntp >> (4 * j)) & 15u);
+            } else {
+#pragma unroll
+              for (int j = 0; j < PL_UNROLL; j++) rr[j] = cur;
+              for (int c = cur;;) {
+                const int ib = c + 1 + lane;
+                const int32_t bnd = ib <= E ? wrun[ib].off : INT32_MAX;
+                const int nb = __popcll(__ballot(bnd < lim));
+                for (int i = 0; i < nb; i++) {
+                  const int32_t b = __builtin_amdgcn_readlane(bnd, i);
+#pragma unroll
+                  for (int j = 0; j < PL_UNROLL; j++) rr[j] += (base + j * 64 + lane >= b) ? 1 : 0;
+                }
+                if (nb < 64) break;
+                c += 64;
+              }
+            }
           }
           int32_t pj[PL_UNROLL];
-          int32_t qj[PL_UNROLL];
+          uint32_t qj[PL_UNROLL];
           uint32_t prj[PL_UNROLL];
           bool act[PL_UNROLL];
 #pragma unroll
@@ -498,33 +538,27 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
             const int32_t k = base + j * 64 + lane;
             act[j] = k < k1;
             const RunL ru = wrun[act[j] ? rr[j] : cur];
-            prj[j] = act[j] ? ru.pi : 0u;
+            prj[j] = ru.pi;
             const int32_t d = k - ru.off;
             pj[j] = (int)(ru.pi & 0xffffu) + d;
-            qj[j] = ru.q0 + d;
+            qj[j] = act[j] ? (uint32_t)(ru.q0 + d) : 0u;   // inactive lanes read the chunk's first byte
           }
           int32_t qv[PL_UNROLL];
           uint32_t bv[PL_UNROLL];
 #pragma unroll
           for (int j = 0; j < PL_UNROLL; j++) {
-            const bool ld = act[j] && (prj[j] & EF_VALID);
-            qv[j] = ld ? (int32_t)tq[qj[j]] : 0;
-            bv[j] = ld ? (uint32_t)tb[(uint32_t)qj[j] >> 2] : 0u;
+            qv[j] = (int32_t)tq[qj[j]];
+            bv[j] = (uint32_t)tb[qj[j] >> 2];
           }
 #pragma unroll
           for (int j = 0; j < PL_UNROLL; j++) {
-            if (!act[j]) continue;
             const uint32_t pr = prj[j];
-            if (pr & EF_VALID) {
-              const int32_t qual = (pr & EF_NOQUAL) ? dq : qv[j];
-              const int b = (int)((bv[j] >> (2 * (qj[j] & 3))) & 3u);
-              if (qual >= mq) {
-                const int32_t mq1 = (int32_t)((pr >> 16) & 0x1ffu);
-                atomicAdd(&bs[b * W + pj[j]], (unsigned long long)pack_base(qual, mq1));
-                atomicAdd(&mqq[pj[j]], (unsigned long long)pack_mq_q(mq1, qual));
-              }
-            } else {
-              atomicAdd(&badA[pj[j]], 1u);
+            const int32_t qual = (pr & EF_NOQUAL) ? dq : qv[j];
+            if (act[j] && qual >= mq) {
+              const int b = (int)((bv[j] >> (2 * (qj[j] & 3u))) & 3u);
+              const int32_t mq1 = (int32_t)((pr >> 16) & 0x1ffu);
+              atomicAdd(&bs[b * W + pj[j]], (unsigned long long)pack_base(qual, mq1));
+              atomicAdd(&mqq[pj[j]], (unsigned long long)pack_mq_q(mq1, qual));
             }
           }
           cur = __shfl(rr[PL_UNROLL - 1], 63, 64);
@@ -547,25 +581,31 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     {
       const int per = (W + BLOCK - 1) / BLOCK;
       const int b0 = tid * per;
-      int32_t spc = 0, sis = 0;
+      int32_t spc = 0, sis = 0, sbd = 0;
       for (int i = 0; i < per; i++)
-        if (b0 + i < W) { spc = j_add(spc, pcA[b0 + i]); sis = j_add(sis, isA[b0 + i]); }
-      int32_t ipc = spc, iis = sis;
+        if (b0 + i < W) {
+          spc = j_add(spc, pcA[b0 + i]); sis = j_add(sis, isA[b0 + i]); sbd = j_add(sbd, bdA[b0 + i]);
+        }
+      int32_t ipc = spc, iis = sis, ibd = sbd;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
-        int32_t o1 = __shfl_up(ipc, d, 64), o2 = __shfl_up(iis, d, 64);
-        if (lane >= d) { ipc = j_add(ipc, o1); iis = j_add(iis, o2); }
+        int32_t o1 = __shfl_up(ipc, d, 64), o2 = __shfl_up(iis, d, 64), o3 = __shfl_up(ibd, d, 64);
+        if (lane >= d) { ipc = j_add(ipc, o1); iis = j_add(iis, o2); ibd = j_add(ibd, o3); }
       }
-      if (lane == 63) { ws_a[wave] = ipc; ws_b[wave] = iis; }
+      if (lane == 63) { ws_a[wave] = ipc; ws_b[wave] = iis; ws_c[wave] = ibd; }
       __syncthreads();
-      int32_t cpc = j_sub(ipc, spc), cis = j_sub(iis, sis);
-      for (int w2 = 0; w2 < wave; w2++) { cpc = j_add(cpc, ws_a[w2]); cis = j_add(cis, ws_b[w2]); }
+      int32_t cpc = j_sub(ipc, spc), cis = j_sub(iis, sis), cbd = j_sub(ibd, sbd);
+      for (int w2 = 0; w2 < wave; w2++) {
+        cpc = j_add(cpc, ws_a[w2]); cis = j_add(cis, ws_b[w2]); cbd = j_add(cbd, ws_c[w2]);
+      }
       for (int i = 0; i < per; i++) {
         if (b0 + i >= W) break;
         cpc = j_add(cpc, pcA[b0 + i]);
         cis = j_add(cis, isA[b0 + i]);
+        cbd = j_add(cbd, bdA[b0 + i]);
         pcA[b0 + i] = cpc;
         isA[b0 + i] = cis;
+        badA[b0 + i] += (uint32_t)cbd;
       }
     }
     __syncthreads();

@@ -51,7 +51,7 @@ def test_c1_full_region(engine):
     assert ref["stats"]["snps"] == 200
 
 
-@pytest.mark.parametrize("window,block", [(256, 256), (512, 512), (896, 512), (1024, 1024), (2048, 256), (64, 512)])
+@pytest.mark.parametrize("window,block", [(256, 256), (512, 512), (896, 512), (1024, 1024), (1920, 256), (64, 512)])
 @pytest.mark.parametrize("persistent", [0, 1])
 def test_c2_small_batch_windows(engine, window, block, persistent):
     """Scaled c2 (clips, improper pairs, MAPQ-0 repeats, error indels, N bases)
@@ -129,7 +129,7 @@ def test_deterministic(engine):
     rid = engine.readset_upload(5, b)
     L = len(draft)
     out1 = engine.batch_run([(5, 1, L, rid)], window=512, persistent=1, block=256)
-    out2 = engine.batch_run([(5, 1, L, rid)], window=2048, persistent=0, block=256)
+    out2 = engine.batch_run([(5, 1, L, rid)], window=1920, persistent=0, block=256)
     assert np.array_equal(out1[0], out2[0])
     assert np.array_equal(out1[1].view(np.uint64), out2[1].view(np.uint64))
 
