@@ -166,14 +166,19 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
         if ((m.info & RI_OUTLIER) || !read_in_query(m, J)) continue;
         frag_event(m, J, L0, L1, sk);
         if (m.fp_hi < L0 || m.fp_lo >= L1) continue;
-        Run rr[2];
-        int got = -1;
-        if (m.info & RI_SIMPLE) got = simple_run(m, R.seq_off[r], cf, L0, L1, rr, &sk.aligned_n, &sk.bcount);
-        else if (!(m.info & RI_EXOTIC) && ((m.info & RI_SEG_MASK) >> RI_SEG_SHIFT) < RI_SEG_MAX)
-          got = stage_segs(R, J, cf, r, m, R.seq_off[r], (int32_t)(R.seq_off[r + 1] - R.seq_off[r]), L0, L1, rr[0], rr[1],
-                           &sk.aligned_n, &sk.bcount, sk);
-        if (got < 0) cplx.push_back(r);
-        for (int k = 0; k < got; k++) runs.push_back(rr[k]);
+        Run rr[1];
+        const int ns = (int)((m.info & RI_SEG_MASK) >> RI_SEG_SHIFT);
+        if (m.info & RI_SIMPLE) {
+          if (simple_run(m, R.seq_off[r], cf, L0, L1, rr, &sk.aligned_n, &sk.bcount)) runs.push_back(rr[0]);
+        } else if (!(m.info & RI_EXOTIC) && ns < RI_SEG_MAX) {
+          // each segment on its own, as the kernel's lanes apply them
+          const int32_t L = (int32_t)(R.seq_off[r + 1] - R.seq_off[r]);
+          for (int i = 0; i < ns; i++)
+            if (seg_apply(J, cf, r, m.info, R.seq_off[r], L, L0, L1, R.segs[m.op0 + i], rr[0], &sk.aligned_n, &sk.bcount, sk))
+              runs.push_back(rr[0]);
+        } else {
+          cplx.push_back(r);
+        }
       }
       for (const Run& ru : runs)
         for (int32_t i = 0; i < ru.n; i++) {

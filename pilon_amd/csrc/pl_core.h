@@ -421,103 +421,6 @@ PL_HD int simple_run(const RMeta& m, int64_t seq0, const KConf& cf, int32_t L0, 
   return 1;
 }
 
-// A multi-op read in one thread: at most two runs (M segments or clips) go to
-// the flattened loop, clips and indel events are applied here.  Returns the
-// number of runs, or -1 (nothing done) when the read needs more than two.
-template <class Sink>
-PL_HD int stage_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r, const RMeta& m, int32_t L0,
-                     int32_t L1, Run& run0, Run& run1, int64_t* aligned, int64_t* bcount, Sink& s) {
-  const int64_t c0 = R.cigar_off[r], c1 = R.cigar_off[r + 1];
-  const int64_t seq0 = R.seq_off[r];
-  const int32_t L = (int32_t)(R.seq_off[r + 1] - seq0);
-  const int32_t flank = cf.flank;
-  const int32_t pos = m.pos;
-  int cnt = 0;
-  {
-    int32_t readOffset = 0, refOffset = 0;
-    for (int64_t k = c0; k < c1; k++) {
-      const uint32_t w = R.cigar[k];
-      const int op = (int)(w & 15);
-      const int32_t len = (int32_t)(w >> 4);
-      const int32_t locus = pos + refOffset;
-      if (op == 0 || op == 7 || op == 8) {
-        int32_t i0 = L0 - locus, i1 = len;
-        if (i0 < 0) i0 = 0;
-        if (flank - readOffset > i0) i0 = flank - readOffset;
-        if (L1 - locus < i1) i1 = L1 - locus;
-        if (L - flank - readOffset < i1) i1 = L - flank - readOffset;
-        if (i1 > i0) cnt++;
-      } else if (op == 4) {
-        const int32_t cs = readOffset == 0 ? locus - len : locus;
-        int32_t i0 = L0 - cs, i1 = L1 - cs;
-        if (i0 < 0) i0 = 0;
-        if (i1 > len) i1 = len;
-        if (i1 > i0) cnt++;
-      }
-      if (consumes_read(op)) readOffset += len;
-      if (consumes_ref(op)) refOffset += len;
-    }
-  }
-  if (cnt > 2) return -1;
-  const uint32_t info = m.info;
-  const bool valid = (info & RI_VALID) != 0;
-  const int32_t mq1 = (int32_t)(info & RI_ADJMQ_MASK) + 1;
-  const ReadAcc a = read_acc(R, r, info, cf.default_qual);
-  const uint32_t fl = valid ? (((uint32_t)mq1 << 16) | EF_VALID | ((info & RI_NOQUAL) ? EF_NOQUAL : 0u)) : EF_BAD;
-  int nr = 0;
-  int32_t readOffset = 0, refOffset = 0;
-  for (int64_t k = c0; k < c1; k++) {
-    const uint32_t w = R.cigar[k];
-    const int op = (int)(w & 15);
-    const int32_t len = (int32_t)(w >> 4);
-    const int32_t locus = pos + refOffset;
-    if (op == 0 || op == 7 || op == 8) {
-      const int32_t ab = locus > L0 ? locus : L0, ae = locus + len < L1 ? locus + len : L1;
-      if (ae > ab) *aligned += ae - ab;
-      int32_t i0 = L0 - locus, i1 = len;
-      if (i0 < 0) i0 = 0;
-      if (flank - readOffset > i0) i0 = flank - readOffset;
-      if (L1 - locus < i1) i1 = L1 - locus;
-      if (L - flank - readOffset < i1) i1 = L - flank - readOffset;
-      if (i1 > i0) {
-        Run& ru = nr == 0 ? run0 : run1;
-        ru.p0 = locus + i0 - L0;
-        ru.n = i1 - i0;
-        ru.q = seq0 + readOffset + i0;
-        ru.fl = fl;
-        nr++;
-        if (valid) *bcount += i1 - i0;
-      }
-    } else if (op == 4) {
-      const int32_t cs = readOffset == 0 ? locus - len : locus;
-      const int32_t ce = cs + len - 1;
-      if (cs >= L0 && cs < L1) s.clip(cs - L0);
-      if (ce >= L0 && ce < L1) s.clip(ce - L0);
-      int32_t i0 = L0 - cs, i1 = L1 - cs;
-      if (i0 < 0) i0 = 0;
-      if (i1 > len) i1 = len;
-      if (i1 > i0) {
-        Run& ru = nr == 0 ? run0 : run1;
-        ru.p0 = cs + i0 - L0;
-        ru.n = i1 - i0;
-        ru.q = seq0;
-        ru.fl = EF_BAD;
-        nr++;
-      }
-    } else if (op == 1) {
-      const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
-      if (valid && tr && locus >= J.start && locus <= J.stop)
-        ins_event(R, J, a, r, locus, readOffset, len, mq1, L0, L1, s);
-    } else if (op == 2) {
-      const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
-      if (valid && tr && locus >= J.start && locus <= J.stop && locus + len - 1 >= J.start && locus + len - 1 <= J.stop)
-        del_event(R, J, cf, a, r, locus, readOffset, len, L, mq1, L0, L1, s);
-    }
-    if (consumes_read(op)) readOffset += len;
-    if (consumes_ref(op)) refOffset += len;
-  }
-  return nr;
-}
 
 // ------------------------------------------------------------ segment table
 // The region-independent part of a multi-op read, computed once per read set:
@@ -585,7 +488,7 @@ PL_HD int build_segs(const RSDev& R, int64_t r, uint32_t info, const KConf& cf, 
           if (out) {
             Seg& g = out[n];
             g.kq = SEG_A | (a.qual(rl) << 8); g.locus = dl; g.roff = rl; g.len = len;
-            g.fin = a.bi(rl); g.hash = 0; g.rotn = 0; g.pad = 0;
+            g.fin = a.bi(rl); g.hash = 0; g.rotn = 0; g.pad = locus;   // pad: the deletion's own locus (its gate)
           }
           n++;
         }
@@ -604,109 +507,99 @@ PL_HD int build_segs(const RSDev& R, int64_t r, uint32_t info, const KConf& cf, 
   return n;
 }
 
-// A multi-op read at window time from its segments: at most two runs go to
-// the flattened loop; clips and indel events are applied here with the
-// region gates of PileUpRegion.addRead.  Returns the number of runs, or -1
-// (nothing done) when the read needs more than two.
+// One segment of a multi-op read at window time, applied on its own: every
+// segment carries what its gates need (a slide add carries its deletion's
+// locus), so the segments of a read can be applied in any order or in
+// parallel.  Clips, indel events and slide adds go to the sink with the
+// region gates of PileUpRegion.addRead; an M segment with trusted in-window
+// bases, or an in-window clip, returns 1 and its run in ru (clip runs and
+// runs of invalid reads carry EF_BAD: badPair only).
 template <class Sink>
-PL_HD int stage_segs(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r, const RMeta& m, int64_t seq0,
-                     int32_t L, int32_t L0, int32_t L1, Run& run0, Run& run1, int64_t* aligned, int64_t* bcount,
-                     Sink& s) {
-  const int ns = (int)((m.info & RI_SEG_MASK) >> RI_SEG_SHIFT);
-  const Seg* sg = R.segs + m.op0;
-  const int32_t flank = cf.flank;
-  int cnt = 0;
-  for (int i = 0; i < ns; i++) {
-    const Seg g = sg[i];
-    const int kd = seg_kind(g);
-    if (kd == SEG_M) {
-      int32_t i0 = L0 - g.locus, i1 = g.len;
-      if (i0 < 0) i0 = 0;
-      if (flank - g.roff > i0) i0 = flank - g.roff;
-      if (L1 - g.locus < i1) i1 = L1 - g.locus;
-      if (L - flank - g.roff < i1) i1 = L - flank - g.roff;
-      if (i1 > i0) cnt++;
-    } else if (kd == SEG_S) {
-      int32_t i0 = L0 - g.locus, i1 = L1 - g.locus;
-      if (i0 < 0) i0 = 0;
-      if (i1 > g.len) i1 = g.len;
-      if (i1 > i0) cnt++;
-    }
-  }
-  if (cnt > 2) return -1;
-  const uint32_t info = m.info;
+PL_HD int seg_apply(const JobDev& J, const KConf& cf, int64_t r, uint32_t info, int64_t seq0, int32_t L,
+                    int32_t L0, int32_t L1, const Seg& g, Run& ru, int64_t* aligned, int64_t* bcount, Sink& s) {
   const bool valid = (info & RI_VALID) != 0;
   const int32_t mq1 = (int32_t)(info & RI_ADJMQ_MASK) + 1;
-  const uint32_t fl = valid ? (((uint32_t)mq1 << 16) | EF_VALID | ((info & RI_NOQUAL) ? EF_NOQUAL : 0u)) : EF_BAD;
-  int nr = 0;
-  bool dgate = false;
-  for (int i = 0; i < ns; i++) {
-    const Seg g = sg[i];
-    const int kd = seg_kind(g);
-    if (kd == SEG_M) {
-      const int32_t ab = g.locus > L0 ? g.locus : L0, ae = g.locus + g.len < L1 ? g.locus + g.len : L1;
-      if (ae > ab) *aligned += ae - ab;
-      int32_t i0 = L0 - g.locus, i1 = g.len;
-      if (i0 < 0) i0 = 0;
-      if (flank - g.roff > i0) i0 = flank - g.roff;
-      if (L1 - g.locus < i1) i1 = L1 - g.locus;
-      if (L - flank - g.roff < i1) i1 = L - flank - g.roff;
-      if (i1 > i0) {
-        Run& ru = nr == 0 ? run0 : run1;
-        ru.p0 = g.locus + i0 - L0; ru.n = i1 - i0; ru.q = seq0 + g.roff + i0; ru.fl = fl;
-        nr++;
-        if (valid) *bcount += i1 - i0;
-      }
-    } else if (kd == SEG_S) {
-      const int32_t cs = g.locus, ce = cs + g.len - 1;
-      if (cs >= L0 && cs < L1) s.clip(cs - L0);
-      if (ce >= L0 && ce < L1) s.clip(ce - L0);
-      int32_t i0 = L0 - cs, i1 = L1 - cs;
-      if (i0 < 0) i0 = 0;
-      if (i1 > g.len) i1 = g.len;
-      if (i1 > i0) {
-        Run& ru = nr == 0 ? run0 : run1;
-        ru.p0 = cs + i0 - L0; ru.n = i1 - i0; ru.q = seq0; ru.fl = EF_BAD;
-        nr++;
-      }
-    } else if (kd == SEG_I) {                                // (:150-162)
-      if (g.locus >= J.start && g.locus <= J.stop) {
-        const int32_t il = g.fin;
-        if (il < J.start) s.edge(r);
-        else if (il >= L0 && il < L1) {
-          IndelRec e;
-          e.key = (uint32_t)(((uint64_t)(J.out + (il - J.start)) << 1) | 0u);
-          e.len = g.len; e.qual = (int16_t)seg_qual(g); e.mq1 = (uint16_t)mq1;
-          e.hash = g.hash; e.rot = g.rotn & 0x3fffffff; e.flags = (g.rotn >> 30) & 1 ? 2u : 0u;
-          e.src = seq0 + g.roff;
-          s.ins(il - L0, seg_qual(g), mq1, e);
-        }
-      }
-    } else if (kd == SEG_D) {                                // (:163-183)
-      const int32_t e1 = g.locus + g.len - 1;
-      dgate = g.locus >= J.start && g.locus <= J.stop && e1 >= J.start && e1 <= J.stop;
-      if (dgate) {
-        const int32_t dl = g.fin;
-        if (dl < J.start) s.edge(r);
-        else if (dl >= L0 && dl < L1) {
-          IndelRec e;
-          e.key = (uint32_t)(((uint64_t)(J.out + (dl - J.start)) << 1) | 1u);
-          e.len = g.len; e.qual = (int16_t)seg_qual(g); e.mq1 = (uint16_t)mq1;
-          e.hash = g.hash; e.rot = 0; e.flags = (g.rotn >> 30) & 1 ? 2u : 0u;
-          e.src = (int64_t)dl - 1;
-          s.del(dl - L0, seg_qual(g), mq1, e);
-        }
-      }
-    } else if (kd == SEG_A) {                                // slide add (:174-178)
-      const int32_t dl = g.locus, t = g.locus + g.len;
-      if (dgate && dl >= J.start && dl <= J.stop && t >= J.start && t <= J.stop && t >= L0 && t < L1) {
-        const int32_t q = seg_qual(g);
-        if (g.fin >= 0 && q >= cf.min_qual) s.base(t - L0, g.fin, q, mq1);
-        s.count_base();
+  const int32_t flank = cf.flank;
+  const int kd = seg_kind(g);
+  if (kd == SEG_M) {                                       // (:184-193)
+    const int32_t ab = g.locus > L0 ? g.locus : L0, ae = g.locus + g.len < L1 ? g.locus + g.len : L1;
+    if (ae > ab) *aligned += ae - ab;
+    int32_t i0 = L0 - g.locus, i1 = g.len;
+    if (i0 < 0) i0 = 0;
+    if (flank - g.roff > i0) i0 = flank - g.roff;
+    if (L1 - g.locus < i1) i1 = L1 - g.locus;
+    if (L - flank - g.roff < i1) i1 = L - flank - g.roff;
+    if (i1 <= i0) return 0;
+    ru.p0 = g.locus + i0 - L0; ru.n = i1 - i0; ru.q = seq0 + g.roff + i0;
+    ru.fl = valid ? (((uint32_t)mq1 << 16) | EF_VALID | ((info & RI_NOQUAL) ? EF_NOQUAL : 0u)) : EF_BAD;
+    if (valid) *bcount += i1 - i0;
+    return 1;
+  }
+  if (kd == SEG_S) {                                       // (:194-206)
+    const int32_t cs = g.locus, ce = cs + g.len - 1;
+    if (cs >= L0 && cs < L1) s.clip(cs - L0);
+    if (ce >= L0 && ce < L1) s.clip(ce - L0);
+    int32_t i0 = L0 - cs, i1 = L1 - cs;
+    if (i0 < 0) i0 = 0;
+    if (i1 > g.len) i1 = g.len;
+    if (i1 <= i0) return 0;
+    ru.p0 = cs + i0 - L0; ru.n = i1 - i0; ru.q = seq0; ru.fl = EF_BAD;
+    return 1;
+  }
+  if (kd == SEG_I) {                                       // (:150-162)
+    if (g.locus >= J.start && g.locus <= J.stop) {
+      const int32_t il = g.fin;
+      if (il < J.start) s.edge(r);
+      else if (il >= L0 && il < L1) {
+        IndelRec e;
+        e.key = (uint32_t)(((uint64_t)(J.out + (il - J.start)) << 1) | 0u);
+        e.len = g.len; e.qual = (int16_t)seg_qual(g); e.mq1 = (uint16_t)mq1;
+        e.hash = g.hash; e.rot = g.rotn & 0x3fffffff; e.flags = (g.rotn >> 30) & 1 ? 2u : 0u;
+        e.src = seq0 + g.roff;
+        s.ins(il - L0, seg_qual(g), mq1, e);
       }
     }
+    return 0;
   }
-  return nr;
+  if (kd == SEG_D) {                                       // (:163-183)
+    const int32_t e1 = g.locus + g.len - 1;
+    if (g.locus >= J.start && g.locus <= J.stop && e1 >= J.start && e1 <= J.stop) {
+      const int32_t dl = g.fin;
+      if (dl < J.start) s.edge(r);
+      else if (dl >= L0 && dl < L1) {
+        IndelRec e;
+        e.key = (uint32_t)(((uint64_t)(J.out + (dl - J.start)) << 1) | 1u);
+        e.len = g.len; e.qual = (int16_t)seg_qual(g); e.mq1 = (uint16_t)mq1;
+        e.hash = g.hash; e.rot = 0; e.flags = (g.rotn >> 30) & 1 ? 2u : 0u;
+        e.src = (int64_t)dl - 1;
+        s.del(dl - L0, seg_qual(g), mq1, e);
+      }
+    }
+    return 0;
+  }
+  if (kd == SEG_A) {                                       // slide add (:174-178)
+    const int32_t d0 = g.pad, d1 = g.pad + g.len - 1;       // the deletion's gate
+    const bool dgate = d0 >= J.start && d0 <= J.stop && d1 >= J.start && d1 <= J.stop;
+    const int32_t dl = g.locus, t = g.locus + g.len;
+    if (dgate && dl >= J.start && dl <= J.stop && t >= J.start && t <= J.stop && t >= L0 && t < L1) {
+      const int32_t q = seg_qual(g);
+      if (g.fin >= 0 && q >= cf.min_qual) s.base(t - L0, g.fin, q, mq1);
+      s.count_base();
+    }
+  }
+  return 0;
+}
+
+// Whether seg_apply would return a valid (base loop) run for g: lets a caller
+// reserve run-table space before applying any segment of the read.
+PL_HD bool seg_has_run(const KConf& cf, uint32_t info, int32_t L, int32_t L0, int32_t L1, const Seg& g) {
+  if (seg_kind(g) != SEG_M || !(info & RI_VALID)) return false;
+  int32_t i0 = L0 - g.locus, i1 = g.len;
+  if (i0 < 0) i0 = 0;
+  if (cf.flank - g.roff > i0) i0 = cf.flank - g.roff;
+  if (L1 - g.locus < i1) i1 = L1 - g.locus;
+  if (L - cf.flank - g.roff < i1) i1 = L - cf.flank - g.roff;
+  return i1 > i0;
 }
 
 // ------------------------------------------------------------ column call

@@ -409,65 +409,114 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     RunL* wrun = s_run + wave * (2 * 64 + 1);
     for (int64_t t0 = win.lo + 64 * wave; t0 < win.hi; t0 += BLOCK) {
       const int64_t qbase = gso[t0];
-      // ---- stage: one read per lane -> up to two runs
+      // ---- stage: one read per lane.  A simple read gives at most one run;
+      // a read with clips or indels is applied below one segment per lane;
+      // the rest (non-ACGT bases, too many segments) are walked.
       const int64_t r = t0 + lane;
-      Run ru0, ru1;
-      ru0.n = 0; ru1.n = 0;
+      const int32_t qsh = (int32_t)(qbase & 3);
+      Run ru0;
+      ru0.n = 0;
       int nrun = 0;
-      bool cplx = false;
+      bool cplx = false, segr = false;
+      uint32_t rinfo = 0, rop0 = 0;   // facts of a segmented read, for the lanes that apply its segments
+      int64_t rseq0 = 0;
       if (r < win.hi) {
         const RMeta m = gm[r];
         const int64_t seq0 = gso[r];
         if (!(m.info & RI_OUTLIER) && read_in_query(m, J)) {
           frag_event(m, J, L0, L1, sk);
           if (m.fp_hi >= L0 && m.fp_lo < L1) {
-            if (seq0 - qbase >= PL_TILE_SPAN) {
-              cplx = true;
-            } else if (m.info & RI_SIMPLE) {
-              nrun = simple_run(m, seq0, cf, L0, L1, &ru0, &sk.aligned_n, &sk.bcount);
+            if (seq0 - qbase >= PL_TILE_SPAN) cplx = true;
+            else if (m.info & RI_SIMPLE) nrun = simple_run(m, seq0, cf, L0, L1, &ru0, &sk.aligned_n, &sk.bcount);
+            else if (!(m.info & RI_EXOTIC) && ((m.info & RI_SEG_MASK) >> RI_SEG_SHIFT) < RI_SEG_MAX) {
+              segr = true;
+              rinfo = m.info; rop0 = m.op0; rseq0 = seq0;
             } else {
-              int got = -1;
-              const uint32_t nseg = (m.info & RI_SEG_MASK) >> RI_SEG_SHIFT;
-              if (!(m.info & RI_EXOTIC) && nseg < RI_SEG_MAX) {
-                const int32_t L = (int32_t)(gso[r + 1] - seq0);
-                got = stage_segs(R, J, cf, r, m, seq0, L, L0, L1, ru0, ru1, &sk.aligned_n, &sk.bcount, sk);
-              }
-              if (got < 0) cplx = true;
-              else nrun = got;
+              cplx = true;
             }
           }
         }
       }
-      // runs that only add badPair over a range (soft clips, M bases of
-      // invalid reads) go to the difference array, not to the base loop
-      if (nrun > 1 && (ru1.fl & EF_BAD)) {
-        if (ru1.n > 0) { atomicAdd(&bdA[ru1.p0], 1); if (ru1.p0 + ru1.n < nW) atomicAdd(&bdA[ru1.p0 + ru1.n], -1); }
-        nrun = 1;
-      }
+      const int32_t rlen = segr ? (int32_t)(gso[r + 1] - rseq0) : 0;
+      // a run that only adds badPair over a range (M bases of an invalid
+      // read) goes to the difference array, not to the base loop
       if (nrun > 0 && (ru0.fl & EF_BAD)) {
         if (ru0.n > 0) { atomicAdd(&bdA[ru0.p0], 1); if (ru0.p0 + ru0.n < nW) atomicAdd(&bdA[ru0.p0 + ru0.n], -1); }
-        if (nrun > 1) ru0 = ru1;
-        nrun--;
+        nrun = 0;
       }
-      // ---- wave-local exclusive scan of (runs, bases)
-      const uint64_t v = (nrun > 0 ? (1ull << 32) + (uint64_t)ru0.n : 0ull) +
-                         (nrun > 1 ? (1ull << 32) + (uint64_t)ru1.n : 0ull);
+      // ---- reads with clips or indels, one lane per segment: each round
+      // loads and applies the segments of as many pending reads as fit the
+      // 64 lanes (usually all of the chunk's in one round).  Base-loop runs
+      // are staged at wrun[64 ...] and appended after the simple runs; a
+      // round whose runs would not fit, or a read of more than 64 segments,
+      // joins the walked reads instead.
+      int32_t ncr = 0;
+      for (uint64_t pend = __ballot(segr); pend;) {
+        const bool inp = (pend >> lane) & 1ull;
+        const int nsg = inp ? (int)((rinfo & RI_SEG_MASK) >> RI_SEG_SHIFT) : 0;
+        const int sinc = wave_incl_scan(nsg);
+        const uint64_t take = __ballot(inp && sinc <= 64);
+        if (!take) {                         // over 64 segments: walked with the other complex reads
+          if (lane == __ffsll((unsigned long long)pend) - 1) cplx = true;
+          pend &= pend - 1;
+          continue;
+        }
+        pend &= ~take;
+        int own = -1, sidx = 0;
+        for (uint64_t sm = take; sm; sm &= sm - 1) {
+          const int o = __ffsll((unsigned long long)sm) - 1;
+          const int no = __builtin_amdgcn_readlane(nsg, o);
+          const int bo = __builtin_amdgcn_readlane(sinc, o) - no;
+          if (lane >= bo && lane < bo + no) { own = o; sidx = lane - bo; }
+        }
+        const int src = own < 0 ? lane : own;
+        const uint32_t info = (uint32_t)__shfl((int)rinfo, src, 64);
+        const uint32_t op0 = (uint32_t)__shfl((int)rop0, src, 64);
+        const int64_t s0 = ((int64_t)__shfl((int)(rseq0 >> 32), src, 64) << 32) |
+                           (uint32_t)__shfl((int)(uint32_t)rseq0, src, 64);
+        const int32_t Lc = __shfl(rlen, src, 64);
+        const bool has = own >= 0;
+        Seg g;
+        if (has) g = R.segs[op0 + sidx];
+        if (ncr + __popcll(__ballot(has && seg_has_run(cf, info, Lc, L0, L1, g))) > 64) {
+          if ((take >> lane) & 1ull) cplx = true;   // runs would not fit: walked instead
+          continue;
+        }
+        Run ru;
+        int got = has ? seg_apply(J, cf, t0 + own, info, s0, Lc, L0, L1, g, ru, &sk.aligned_n, &sk.bcount, sk) : 0;
+        if (got && (ru.fl & EF_BAD)) {
+          atomicAdd(&bdA[ru.p0], 1);
+          if (ru.p0 + ru.n < nW) atomicAdd(&bdA[ru.p0 + ru.n], -1);
+          got = 0;
+        }
+        const uint64_t gmask = __ballot(got != 0);
+        if (got) wrun[64 + ncr + __popcll(gmask & lanemask_lt())] =
+            RunL{ru.n, (uint32_t)ru.p0 | ru.fl, (int32_t)(ru.q - qbase) + qsh, 0};
+        ncr += __popcll(gmask);
+      }
+      // ---- run table: simple runs (wave scan), then the staged runs
+      const uint64_t v = nrun > 0 ? (1ull << 32) + (uint64_t)ru0.n : 0ull;
       const uint64_t inc = wave_incl_scan(v);
       const uint64_t tot = __shfl(inc, 63, 64);
-      {
+      if (nrun > 0) {
         const uint64_t ex = inc - v;
-        int32_t ei = (int32_t)(ex >> 32), off = (int32_t)(uint32_t)ex;
-        const int32_t qsh = (int32_t)(qbase & 3);
-        if (nrun > 0) {
-          wrun[ei] = RunL{off, (uint32_t)ru0.p0 | ru0.fl, (int32_t)(ru0.q - qbase) + qsh, 0};
-          off += ru0.n; ei++;
-        }
-        if (nrun > 1) {
-          wrun[ei] = RunL{off, (uint32_t)ru1.p0 | ru1.fl, (int32_t)(ru1.q - qbase) + qsh, 0};
-        }
+        wrun[(int32_t)(ex >> 32)] = RunL{(int32_t)(uint32_t)ex, (uint32_t)ru0.p0 | ru0.fl, (int32_t)(ru0.q - qbase) + qsh, 0};
       }
-      const int32_t E = (int32_t)(tot >> 32);
-      const int32_t B = (int32_t)(uint32_t)tot;
+      const int32_t Es = (int32_t)(tot >> 32);
+      const int32_t Bs = (int32_t)(uint32_t)tot;
+      int32_t Bc = 0;
+      if (ncr > 0) {
+        // every lane reads its staged run before any lane writes (one wave,
+        // LDS operations in order)
+        RunL cr = RunL{0, 0u, 0, 0};
+        if (lane < ncr) cr = wrun[64 + lane];
+        const int32_t cn = lane < ncr ? cr.off : 0;
+        const int32_t ci = wave_incl_scan(cn);
+        Bc = __shfl(ci, 63, 64);
+        if (lane < ncr) wrun[Es + lane] = RunL{Bs + ci - cn, cr.pi, cr.q0, 0};
+      }
+      const int32_t E = Es + ncr;
+      const int32_t B = Bs + Bc;
       if (lane == 0) wrun[E] = RunL{B, 0u, 0, 0};
       // LDS operations of one wave complete in order: the table written
       // above is what the loop below reads
