@@ -6,8 +6,8 @@
 #ifndef PL_WIN_BLOCK
 #define PL_WIN_BLOCK 256          // threads per window workgroup
 #endif
-#ifndef PL_UNROLL
-#define PL_UNROLL 8               // bases per lane per iteration of the flattened loop
+#ifndef PL_CHUNK
+#define PL_CHUNK 17               // consecutive bases of one run per lane in the base loop (odd: LDS banks)
 #endif
 #ifndef PL_WIN_WAVES_EU
 #define PL_WIN_WAVES_EU 1         // minimum waves per SIMD the window kernel is compiled for

@@ -78,6 +78,13 @@ __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     PrepOut o = prep_read(R, r, cf);
     RMeta m = o.m;
+    // a read without QUAL piles defaultQual at every base (PileUpRegion.scala:114-115):
+    // written into its quality bytes once, so the window kernel reads quals unconditionally
+    if (m.info & RI_NOQUAL) {
+      int8_t* qd = (int8_t*)R.quals + R.seq_off[r];
+      const int64_t L = R.seq_off[r + 1] - R.seq_off[r];
+      for (int64_t k = 0; k < L; k++) qd[k] = (int8_t)cf.default_qual;
+    }
     segcount[r] = wants_segs(m.info) ? (uint32_t)build_segs(R, r, m.info, cf, nullptr) : 0u;
     const bool keep = (m.info & RI_KEEP) != 0;
     if (keep) {
@@ -319,12 +326,13 @@ struct LdsSink {
 // counter the atomics wait on
 typedef const int8_t __attribute__((address_space(1))) gi8;
 typedef const uint8_t __attribute__((address_space(1))) gu8;
+typedef const uint32_t __attribute__((address_space(1))) gu32;
 
-struct __attribute__((aligned(16))) RunL {  // one run of a tile, one 16-byte LDS read
-  int32_t off;     // first flattened base of the run
+struct __attribute__((aligned(16))) RunL {  // one run of a chunk of reads, one 16-byte LDS read
+  int32_t off;     // first chunk of the run in the chunk stream
   uint32_t pi;     // window position | EF_* flags | mq1 << 16
-  int32_t q0;      // sequence offset of the run's first base, from the tile's 4-aligned base
-  int32_t pad;
+  int32_t q0;      // sequence offset of the run's first base, from the reads' 4-aligned base
+  int32_t pad;     // run length in bases
 };
 
 __device__ __forceinline__ int64_t uniform64(int64_t v) {
@@ -343,7 +351,7 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
 // which consecutive lanes take consecutive bases — coalesced loads and
 // conflict-free LDS atomics.  Reads with clips, indels or exotic bases go
 // through walk_read, one wave per read.
-static_assert(PL_UNROLL >= 1 && PL_UNROLL <= 8, "the run lookup packs PL_UNROLL 4-bit counts in 32 bits");
+static_assert(PL_CHUNK % 2 == 1 && PL_CHUNK <= 31, "an odd chunk of at most 31 bases (two 16-base words)");
 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A, int persistent) {
@@ -494,123 +502,95 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
             RunL{ru.n, (uint32_t)ru.p0 | ru.fl, (int32_t)(ru.q - qbase) + qsh, 0};
         ncr += __popcll(gmask);
       }
-      // ---- run table: simple runs (wave scan), then the staged runs
-      const uint64_t v = nrun > 0 ? (1ull << 32) + (uint64_t)ru0.n : 0ull;
+      // ---- run table: simple runs (wave scan), then the staged runs.  Runs
+      // are cut in chunks of PL_CHUNK bases; .off is the run's first chunk in
+      // the chunk stream, .n its length in bases.
+      const uint64_t v = nrun > 0 ? (1ull << 32) + (uint64_t)((ru0.n + PL_CHUNK - 1) / PL_CHUNK) : 0ull;
       const uint64_t inc = wave_incl_scan(v);
       const uint64_t tot = __shfl(inc, 63, 64);
       if (nrun > 0) {
         const uint64_t ex = inc - v;
-        wrun[(int32_t)(ex >> 32)] = RunL{(int32_t)(uint32_t)ex, (uint32_t)ru0.p0 | ru0.fl, (int32_t)(ru0.q - qbase) + qsh, 0};
+        wrun[(int32_t)(ex >> 32)] =
+            RunL{(int32_t)(uint32_t)ex, (uint32_t)ru0.p0 | ru0.fl, (int32_t)(ru0.q - qbase) + qsh, ru0.n};
       }
       const int32_t Es = (int32_t)(tot >> 32);
-      const int32_t Bs = (int32_t)(uint32_t)tot;
-      int32_t Bc = 0;
+      const int32_t Cs = (int32_t)(uint32_t)tot;
+      int32_t Cc = 0;
       if (ncr > 0) {
         // every lane reads its staged run before any lane writes (one wave,
         // LDS operations in order)
         RunL cr = RunL{0, 0u, 0, 0};
         if (lane < ncr) cr = wrun[64 + lane];
-        const int32_t cn = lane < ncr ? cr.off : 0;
+        const int32_t cn = lane < ncr ? (cr.off + PL_CHUNK - 1) / PL_CHUNK : 0;
         const int32_t ci = wave_incl_scan(cn);
-        Bc = __shfl(ci, 63, 64);
-        if (lane < ncr) wrun[Es + lane] = RunL{Bs + ci - cn, cr.pi, cr.q0, 0};
+        Cc = __shfl(ci, 63, 64);
+        if (lane < ncr) wrun[Es + lane] = RunL{Cs + ci - cn, cr.pi, cr.q0, cr.off};
       }
       const int32_t E = Es + ncr;
-      const int32_t B = Bs + Bc;
-      if (lane == 0) wrun[E] = RunL{B, 0u, 0, 0};
+      const int32_t CT = Cs + Cc;
+      if (lane == 0) wrun[E] = RunL{CT, 0u, 0, 0};
       // LDS operations of one wave complete in order: the table written
       // above is what the loop below reads
       __builtin_amdgcn_wave_barrier();
-      // ---- flattened bases of all runs of the chunk
-      if (B > 0) {
-        const int32_t k1 = B;
+      // ---- the bases of all runs: one chunk of PL_CHUNK consecutive bases of
+      // one run per lane.  A chunk's quals come in as 6 dwords and its 2-bit
+      // bases as 3, realigned with funnel shifts; each base then costs a
+      // byte and a 2-bit extract and two LDS atomics whose step offset is the
+      // instruction's immediate.  Lanes of one run are PL_CHUNK (odd)
+      // positions apart, so their 64-bit atomics spread over all LDS banks.
+      if (CT > 0) {
+        const int32_t mq = cf.min_qual;
+        const int64_t qb4 = qbase & ~(int64_t)3, qb16 = qbase & ~(int64_t)15;
+        const gu32* tq32 = (const gu32*)(gq + qb4);
+        const gu32* tb32 = (const gu32*)(gb + (qb16 >> 2));
+        const uint32_t qd = (uint32_t)(qb4 - qb16);
         int cur = 0;
-        const int32_t dq = cf.default_qual, mq = cf.min_qual;
-        // chunk-relative global pointers: 32-bit lane offsets from a uniform base
-        const int64_t qb4 = qbase & ~(int64_t)3;
-        const gi8* tq = (const gi8*)(gq + qb4);
-        const gu8* tb = (const gu8*)(gb + (qb4 >> 2));
-        for (int32_t base = 0; base < k1; base += 64 * PL_UNROLL) {
-          // run of each k = the last run whose first flattened base is <= k.
-          // The run starts inside this iteration's span are read once, 64 at
-          // a time (one LDS read per lane); each one adds 1 to the counts of
-          // every unrolled k at or past it, all U counts packed 4 bits apart
-          // in one register (one shift and add per start).  More than 15
-          // starts in one span (runs shorter than ~34 bases) take the plain
-          // per-k compares.
-          const int32_t lim = (k1 < base + 64 * PL_UNROLL) ? k1 : base + 64 * PL_UNROLL;
-          int rr[PL_UNROLL];
-          {
-            uint32_t cntp = 0u;
-            int total = 0;
-            for (int c = cur;;) {
-              const int ib = c + 1 + lane;
-              const int32_t bnd = ib <= E ? wrun[ib].off : INT32_MAX;
-              const int nb = __popcll(__ballot(bnd < lim));
-              total += nb;
-              if (total <= 15) {
-                for (int i = 0; i < nb; i++) {
-                  const int32_t t = __builtin_amdgcn_readlane(bnd, i) - base - lane;   // first k index >= start
-                  const int j0 = t <= 0 ? 0 : (t + 63) >> 6;
-                  cntp += j0 < PL_UNROLL ? (0x11111111u << (4 * j0)) : 0u;
-                }
-              }
-              if (nb < 64) break;
-              c += 64;
-            }
-            if (total <= 15) {
+        for (int32_t cb = 0; cb < CT; cb += 64) {
+          const int32_t C = cb + lane;
+          const int32_t lim = CT < cb + 64 ? CT : cb + 64;
+          // run of chunk C = the last run whose first chunk is <= C
+          int rr = cur;
+          for (int c = cur;;) {
+            const int ib = c + 1 + lane;
+            const int32_t bnd = ib <= E ? wrun[ib].off : INT32_MAX;
+            const int nb = __popcll(__ballot(bnd < lim));
+            for (int i = 0; i < nb; i++) rr += (C >= __builtin_amdgcn_readlane(bnd, i)) ? 1 : 0;
+            if (nb < 64) break;
+            c += 64;
+          }
+          const bool act = C < CT;
+          const RunL ru = wrun[act ? rr : cur];
+          const int32_t d = PL_CHUNK * (C - ru.off);
+          int32_t cnt = ru.pad - d;
+          cnt = act ? (cnt < PL_CHUNK ? cnt : PL_CHUNK) : 0;
+          const int32_t p = (int32_t)(ru.pi & 0xffffu) + (act ? d : 0);
+          const uint32_t q = act ? (uint32_t)(ru.q0 + d) : 0u;
+          const int32_t mq1 = (int32_t)((ru.pi >> 16) & 0x1ffu);
+          uint32_t qw[PL_CHUNK / 4 + 2];
 #pragma unroll
-              for (int j = 0; j < PL_UNROLL; j++) rr[j] = cur + (int)((cntp >> (4 * j)) & 15u);
-            } else {
+          for (int k = 0; k < PL_CHUNK / 4 + 2; k++) qw[k] = tq32[(q >> 2) + k];
+          const uint32_t qq = q + qd;
+          uint32_t bw[PL_CHUNK / 16 + 2];
 #pragma unroll
-              for (int j = 0; j < PL_UNROLL; j++) rr[j] = cur;
-              for (int c = cur;;) {
-                const int ib = c + 1 + lane;
-                const int32_t bnd = ib <= E ? wrun[ib].off : INT32_MAX;
-                const int nb = __popcll(__ballot(bnd < lim));
-                for (int i = 0; i < nb; i++) {
-                  const int32_t b = __builtin_amdgcn_readlane(bnd, i);
+          for (int k = 0; k < PL_CHUNK / 16 + 2; k++) bw[k] = tb32[(qq >> 4) + k];
+          uint32_t w[PL_CHUNK / 4 + 1];
 #pragma unroll
-                  for (int j = 0; j < PL_UNROLL; j++) rr[j] += (base + j * 64 + lane >= b) ? 1 : 0;
-                }
-                if (nb < 64) break;
-                c += 64;
-              }
+          for (int k = 0; k < PL_CHUNK / 4 + 1; k++) w[k] = __builtin_amdgcn_alignbyte(qw[k + 1], qw[k], q & 3u);
+          uint32_t x[PL_CHUNK / 16 + 1];
+#pragma unroll
+          for (int k = 0; k < PL_CHUNK / 16 + 1; k++) x[k] = __builtin_amdgcn_alignbit(bw[k + 1], bw[k], 2u * (qq & 15u));
+          unsigned long long* bsp = bs + p;
+          unsigned long long* mqp = mqq + p;
+#pragma unroll
+          for (int t = 0; t < PL_CHUNK; t++) {
+            const int32_t qual = (int32_t)(int8_t)(w[t >> 2] >> (8 * (t & 3)));
+            const int b = (int)((x[t >> 4] >> (2 * (t & 15))) & 3u);
+            if (t < cnt && qual >= mq) {
+              atomicAdd(bsp + b * W + t, (unsigned long long)pack_base(qual, mq1));
+              atomicAdd(mqp + t, (unsigned long long)pack_mq_q(mq1, qual));
             }
           }
-          int32_t pj[PL_UNROLL];
-          uint32_t qj[PL_UNROLL];
-          uint32_t prj[PL_UNROLL];
-          bool act[PL_UNROLL];
-#pragma unroll
-          for (int j = 0; j < PL_UNROLL; j++) {
-            const int32_t k = base + j * 64 + lane;
-            act[j] = k < k1;
-            const RunL ru = wrun[act[j] ? rr[j] : cur];
-            prj[j] = ru.pi;
-            const int32_t d = k - ru.off;
-            pj[j] = (int)(ru.pi & 0xffffu) + d;
-            qj[j] = act[j] ? (uint32_t)(ru.q0 + d) : 0u;   // inactive lanes read the chunk's first byte
-          }
-          int32_t qv[PL_UNROLL];
-          uint32_t bv[PL_UNROLL];
-#pragma unroll
-          for (int j = 0; j < PL_UNROLL; j++) {
-            qv[j] = (int32_t)tq[qj[j]];
-            bv[j] = (uint32_t)tb[qj[j] >> 2];
-          }
-#pragma unroll
-          for (int j = 0; j < PL_UNROLL; j++) {
-            const uint32_t pr = prj[j];
-            const int32_t qual = (pr & EF_NOQUAL) ? dq : qv[j];
-            if (act[j] && qual >= mq) {
-              const int b = (int)((bv[j] >> (2 * (qj[j] & 3u))) & 3u);
-              const int32_t mq1 = (int32_t)((pr >> 16) & 0x1ffu);
-              atomicAdd(&bs[b * W + pj[j]], (unsigned long long)pack_base(qual, mq1));
-              atomicAdd(&mqq[pj[j]], (unsigned long long)pack_mq_q(mq1, qual));
-            }
-          }
-          cur = __shfl(rr[PL_UNROLL - 1], 63, 64);
+          cur = __shfl(rr, 63, 64);
         }
       }
       // ---- everything else about the chunk's complex reads, one at a time
