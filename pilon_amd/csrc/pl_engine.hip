@@ -434,7 +434,9 @@ pl_status enqueue_batch(Lane* L, EngineArgs& A, int persistent, int block, int g
 // LDS the window kernel needs for a window of W positions (dynamic part)
 size_t window_lds(int32_t W) { return PL_WIN_LDS_PER_POS * (size_t)W + 1024; }
 // ... and per workgroup in total, with the per-wave run tables
-size_t window_lds_total(int32_t W, int block) { return window_lds(W) + 32 * (size_t)block + 1024; }
+size_t window_lds_total(int32_t W, int block) {
+  return window_lds(W) + 32 * (size_t)block + 1024 + sizeof(IndelRec) * PL_WIN_RECS;
+}
 
 pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t W, int persistent, int block, bool use_graph) {
   if (reqs.empty()) return fail(PL_E_INVAL, "empty batch");

@@ -10,8 +10,9 @@
 #define PL_CHUNK 17               // consecutive bases of one run per lane in the base loop (odd: LDS banks)
 #endif
 #ifndef PL_WIN_WAVES_EU
-#define PL_WIN_WAVES_EU 1         // minimum waves per SIMD the window kernel is compiled for
+#define PL_WIN_WAVES_EU 3         // minimum waves per SIMD the window kernel is compiled for
 #endif
 #define PL_DEFAULT_WINDOW 512     // plain region form and autotuner fallback
 #define PL_DEFAULT_BLOCK 256
 #define PL_WIN_LDS_PER_POS 76     // window LDS bytes per position (counters + difference arrays)
+#define PL_WIN_RECS 128           // indel evidence records a window buffers in LDS

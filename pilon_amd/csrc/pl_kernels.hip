@@ -291,6 +291,8 @@ struct LdsSink {
   IndelRec* rec;
   uint32_t* rec_n;
   uint32_t rec_cap;
+  IndelRec* lrec;        // the window's record buffer in LDS (PL_WIN_RECS)
+  uint32_t* lrec_n;
   int32_t* overflow;
   JobStat* st;
   int64_t bcount, aligned_n;
@@ -303,7 +305,11 @@ struct LdsSink {
   __device__ void count_base() { bcount++; }
   __device__ void bad(int p, int n) { atomicAdd(&bad_[p], (uint32_t)n); }
   __device__ void clip(int p) { atomicAdd(&clp_[p], 1u); }
+  // records collect in LDS and reach HBM with one atomic per window; a
+  // window with more than PL_WIN_RECS writes the rest directly
   __device__ void emit(const IndelRec& e) {
+    const uint32_t j = atomicAdd(lrec_n, 1u);
+    if (j < PL_WIN_RECS) { lrec[j] = e; return; }
     uint32_t i = atomicAdd(rec_n, 1u);
     if (i < rec_cap) rec[i] = e;
     else *overflow = 1;
@@ -373,6 +379,8 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
   __shared__ int32_t ws_a[NW], ws_b[NW], ws_c[NW];
   __shared__ unsigned long long red[2][NW];
   __shared__ int32_t s_ovf;
+  __shared__ IndelRec s_rec[PL_WIN_RECS];
+  __shared__ uint32_t s_nrec, s_recbase;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lds_words = (PL_WIN_LDS_PER_POS * W) / 8 + 256 / 2;
   const KConf& cf = A.cf;
@@ -403,20 +411,35 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     const int64_t* __restrict__ gso = R.seq_off;
     const RMeta* __restrict__ gm = R.meta;
     for (int k = tid; k < lds_words; k += BLOCK) lds[k] = 0ull;
-    if (tid == 0) s_ovf = 0;
+    if (tid == 0) { s_ovf = 0; s_nrec = 0; }
     __syncthreads();
 
     LdsSink sk;
     sk.W = W; sk.bs = bs; sk.mqq = mqq; sk.ins_ = insA; sk.del_ = delA; sk.bad_ = badA; sk.clp_ = clpA;
     sk.pc = pcA; sk.is = isA; sk.rec = A.rec; sk.rec_n = A.rec_n; sk.rec_cap = A.rec_cap;
+    sk.lrec = s_rec; sk.lrec_n = &s_nrec;
     sk.overflow = A.overflow; sk.st = &A.st[win.job]; sk.bcount = 0; sk.aligned_n = 0;
 
     // Each wave takes its own chunks of 64 reads (wave-strided over the
     // window's read range): staging, the run scan and the base loop are
     // wave-local, so the read loop has no workgroup barrier.
     RunL* wrun = s_run + wave * (2 * 64 + 1);
-    for (int64_t t0 = win.lo + 64 * wave; t0 < win.hi; t0 += BLOCK) {
-      const int64_t qbase = gso[t0];
+    // the next chunk's read facts are loaded while the current one is piled
+    int64_t t0 = win.lo + 64 * wave;
+    RMeta mN;
+    int64_t sN = 0, eN = 0, qN = 0;
+    mN.info = 0;
+    if (t0 + lane < win.hi) { mN = gm[t0 + lane]; sN = gso[t0 + lane]; eN = gso[t0 + lane + 1]; }
+    if (t0 < win.hi) qN = gso[t0];
+    for (; t0 < win.hi; t0 += BLOCK) {
+      const int64_t qbase = qN;
+      const RMeta m = mN;
+      const int64_t seq0 = sN, seqE = eN;
+      {
+        const int64_t tn = t0 + BLOCK;
+        if (tn + lane < win.hi) { mN = gm[tn + lane]; sN = gso[tn + lane]; eN = gso[tn + lane + 1]; }
+        if (tn < win.hi) qN = gso[tn];
+      }
       // ---- stage: one read per lane.  A simple read gives at most one run;
       // a read with clips or indels is applied below one segment per lane;
       // the rest (non-ACGT bases, too many segments) are walked.
@@ -429,8 +452,6 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
       uint32_t rinfo = 0, rop0 = 0;   // facts of a segmented read, for the lanes that apply its segments
       int64_t rseq0 = 0;
       if (r < win.hi) {
-        const RMeta m = gm[r];
-        const int64_t seq0 = gso[r];
         if (!(m.info & RI_OUTLIER) && read_in_query(m, J)) {
           frag_event(m, J, L0, L1, sk);
           if (m.fp_hi >= L0 && m.fp_lo < L1) {
@@ -445,7 +466,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
           }
         }
       }
-      const int32_t rlen = segr ? (int32_t)(gso[r + 1] - rseq0) : 0;
+      const int32_t rlen = segr ? (int32_t)(seqE - rseq0) : 0;
       // a run that only adds badPair over a range (M bases of an invalid
       // read) goes to the difference array, not to the base loop
       if (nrun > 0 && (ru0.fl & EF_BAD)) {
@@ -639,6 +660,17 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     }
     __syncthreads();
 
+    // the window's evidence records: one atomic, coalesced 32-byte stores
+    {
+      const uint32_t nl = s_nrec < PL_WIN_RECS ? s_nrec : PL_WIN_RECS;
+      if (tid == 0 && nl) s_recbase = atomicAdd(A.rec_n, nl);
+      __syncthreads();
+      for (uint32_t t = tid; t < nl; t += BLOCK) {
+        const uint32_t k = s_recbase + t;
+        if (k < A.rec_cap) A.rec[k] = s_rec[t];
+        else *A.overflow = 1;
+      }
+    }
     const int64_t G = A.G;
     for (int p = tid; p < nW; p += BLOCK) {
       const uint64_t b4[4] = {bs[p], bs[W + p], bs[2 * W + p], bs[3 * W + p]};
