@@ -809,21 +809,126 @@ __device__ __forceinline__ int32_t job_of(const JobDev* jobs, int32_t n_jobs, in
 }
 }  // namespace
 
+namespace {
+
+// first index in keys[0, n) whose key is >= x, by the whole wave: 64 probes
+// per round cut the range 65-fold (4 dependent loads for a million keys)
+__device__ int64_t wave_lower_bound(const uint32_t* keys, int64_t n, uint32_t x, int lane) {
+  int64_t lo = 0, hi = n;
+  while (hi - lo > 64) {
+    const int64_t m = hi - lo;
+    const int64_t probe = lo + ((int64_t)(lane + 1) * m) / 65;
+    const int k = __popcll(__ballot(keys[probe] < x));   // probes ascend: a prefix of lanes
+    const int64_t plo = lo + ((int64_t)k * m) / 65;       // probe of lane k-1 (lo when k = 0)
+    const int64_t phi = lo + ((int64_t)(k + 1) * m) / 65; // probe of lane k (hi when k = 64)
+    if (k > 0) lo = plo + 1;
+    hi = k < 64 ? phi : hi;
+  }
+  const int64_t i = lo + lane;
+  return lo + __popcll(__ballot(i < hi && keys[i] < x));
+}
+
+// strict majority over one (position, kind) group of sorted records, by the
+// whole wave: each lane counts the records equal to its own in (length,
+// hash) — an upper bound of the exact count — and the largest group's string
+// is confirmed by exact comparison; a hash collision falls back to the serial
+// scan.  Same result as group_majority.
+__device__ Majority wave_majority(const RSDev& R, const IndelRec* recs, const uint32_t* idx, int64_t a, int64_t b,
+                                  int lane) {
+  Majority m;
+  m.n = (int32_t)(b - a);
+  m.win = 0;
+  m.rec = -1;
+  if (b <= a) return m;
+  if (b - a > 64) return group_majority(R, recs, idx, a, b);
+  const int n = (int)(b - a);
+  const bool in = lane < n;
+  const int64_t ri = in ? (int64_t)idx[a + lane] : 0;
+  IndelRec e;
+  e.len = -1; e.hash = 0; e.key = 0; e.rot = 0; e.flags = 0; e.src = 0;
+  if (in) e = recs[ri];
+  int hc = 0;
+  for (int j = 0; j < n; j++) {
+    const int32_t lj = __builtin_amdgcn_readlane(e.len, j);
+    const uint32_t hj = __builtin_amdgcn_readlane(e.hash, j);
+    hc += (in && e.len == lj && e.hash == hj) ? 1 : 0;
+  }
+  int best = in ? hc * 64 + (63 - lane) : -1;
+  for (int d = 32; d >= 1; d >>= 1) { const int o = __shfl_xor(best, d, 64); best = o > best ? o : best; }
+  const int bc = best >> 6, bl = 63 - (best & 63);
+  if (2 * bc <= n) { m.win = bc; return m; }          // no string can have a strict majority
+  IndelRec c;
+  c.len = __shfl(e.len, bl, 64); c.hash = (uint32_t)__shfl((int)e.hash, bl, 64); c.key = (uint32_t)__shfl((int)e.key, bl, 64);
+  c.rot = __shfl(e.rot, bl, 64); c.flags = (uint32_t)__shfl((int)e.flags, bl, 64);
+  c.src = ((int64_t)__shfl((int)(e.src >> 32), bl, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)e.src, bl, 64);
+  const bool same = in && rec_same(R, e, c);
+  const int exact = __popcll(__ballot(same));
+  if (2 * exact > n) { m.win = exact; m.rec = __shfl((long long)ri, bl, 64); return m; }
+  return group_majority(R, recs, idx, a, b);
+}
+
+// the region (job) holding batch position g: the last job whose first
+// position is <= g, by the whole wave as in wave_lower_bound
+__device__ int32_t wave_job_of(const JobDev* jobs, int32_t n_jobs, int64_t g, int lane) {
+  int32_t lo = 0, hi = n_jobs;                          // first job with out > g lies in [lo, hi]
+  while (hi - lo > 64) {
+    const int64_t m = hi - lo;
+    const int32_t probe = lo + (int32_t)(((int64_t)(lane + 1) * m) / 65);
+    const int k = __popcll(__ballot(jobs[probe].out <= g));
+    const int32_t plo = lo + (int32_t)(((int64_t)k * m) / 65);
+    const int32_t phi = lo + (int32_t)(((int64_t)(k + 1) * m) / 65);
+    if (k > 0) lo = plo + 1;
+    if (k < 64) hi = phi;
+  }
+  const int32_t i = lo + lane;
+  return lo + __popcll(__ballot(i < hi && jobs[i].out <= g)) - 1;
+}
+
+}  // namespace
+
+// One wave per listed column: the full BaseCall with the column's evidence
+// records (hetIndelCall), lane 0 writing the results.
 // mode 0: deferred columns (writes call, appends candidates, counts confirmed)
 // mode 1: columns changed by the deletion pass (keeps disposition bits)
-__global__ void k_indel_pass(EngineArgs A, const uint32_t* list, const uint32_t* list_n, uint32_t cap, int mode) {
+__global__ void __launch_bounds__(256) k_indel_pass(EngineArgs A, const uint32_t* list, const uint32_t* list_n,
+                                                     uint32_t cap, int mode) {
   const uint32_t n = *list_n < cap ? *list_n : cap;
   const uint32_t nrec = *A.rec_n < A.rec_cap ? *A.rec_n : A.rec_cap;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n; i += nw) {
     const uint32_t g = list[i];
-    const int32_t j = job_of(A.jobs, A.n_jobs, g);
+    const int32_t j = wave_job_of(A.jobs, A.n_jobs, g, lane);
     const JobDev& J = A.jobs[j];
     const RSDev& R = A.rs[J.rs];
-    Col c = load_col(A.pile, A.G, g);
-    int32_t ilen;
-    int64_t wrec;
+    const Col c = load_col(A.pile, A.G, g);
     const uint8_t refb = R.ref[J.start + (int64_t)(g - J.out) - 1];
-    pl_call cl = full_call(A.rs, J, c, refb, A.rec, A.rkey, A.rval, nrec, g, A.cf, &ilen, &wrec);
+    const BaseFacts bf = base_facts(c);
+    int32_t pct = 0;
+    const int need = indel_kind_needed(c, A.cf, &pct);
+    int indel = 0;
+    bool homo = true;
+    int32_t ilen = 0;
+    int64_t wrec = -1;
+    if (need) {
+      const uint32_t key = (g << 1) | (uint32_t)(need - 1);
+      const int64_t a = wave_lower_bound(A.rkey, nrec, key, lane);
+      int64_t b = a;
+      for (;;) {
+        const int64_t t = b + lane;
+        const int cnt = __popcll(__ballot(t < (int64_t)nrec && A.rkey[t] == key));
+        b += cnt;
+        if (cnt < 64) break;
+      }
+      const Majority mj = wave_majority(R, A.rec, A.rval, a, b, lane);
+      if (mj.rec >= 0) {
+        const IndelRec& e = A.rec[mj.rec];
+        const int d = het_indel_decide(mj.n, mj.win, e.len, (e.flags & 2u) != 0, pct, A.cf);
+        if (d) { indel = need; homo = d == 2; ilen = e.len; wrec = mj.rec; }
+      }
+    }
+    if (lane != 0) continue;
+    const pl_call cl = make_call(bf, indel, homo, refb, A.cf);
     if (mode == 0) {
       A.calls[g] = cl;
       const int d = call_disp(cl);

@@ -125,10 +125,10 @@ class Context:
             descs = self.batch_descs(regions)
         opts = PlBatchOpts(window=window, block=block, persistent=persistent, use_graph=int(bool(use_graph)),
                            copy_out=int(copy_out))
-        G = sum(e - s + 1 for _, s, e, _ in regions)
-        stats = (PlRegionStats * n)()
+        stats = (PlRegionStats * n)() if want_stats else None
         pile = calls = None
         if copy_out:
+            G = sum(e - s + 1 for _, s, e, _ in regions)
             pile = np.zeros((NFIELDS, G), np.int32)
             calls = np.zeros(G, call_dtype())
         _check(lib().pl_batch_run(self.h, descs, n, C.byref(opts),
