@@ -82,6 +82,11 @@ def main():
     ap.add_argument("--graph", type=int, default=1, help="time the steps as hipGraph replays")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-region comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU oracle sample")
+    ap.add_argument("--serve", type=int, default=1, help="also run the serving workload (c4 request stream)")
+    ap.add_argument("--serve-scale", type=float, default=1.0, help="genome scale of the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-rate", type=float, default=20000.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-lanes", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path would shard "
@@ -187,6 +192,12 @@ def main():
                          f"{len(regions)} regions, {done_bases} aligned bases in {spent:.1f}s"}
         log(f"bench: CPU oracle {cpu['value'] / 1e6:.1f} M bases/s on {nreg} regions")
 
+    serving = None
+    if args.serve:
+        ctx.close()
+        del data
+        serving = serve_workload(cfg, args)
+
     line = {
         "metric": METRIC, "value": value, "unit": "bases/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -200,10 +211,83 @@ def main():
                      "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms},
         "device_pipeline_ms": pipe_ms,
         "cpu_baseline": cpu,
+        "p99_latency_ms": serving["p99_ms"] if serving else None,
+        "serving": serving,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
+    if not args.serve:
+        ctx.close()
+
+
+def serve_workload(cfg, args):
+    """The serving form on the c4 configuration: the genome and its reads
+    resident, a stream of polish requests (log-uniform 2 kb - 1 Mb at uniform
+    offsets) submitted at Poisson arrival times, admitted by estimated device
+    memory and run with continuous batching.  Latency = submit to result
+    ready (regions' stats on the host); throughput = aligned bases of all
+    requests over the time from the first submission to the last result."""
+    from pilon_amd.engine import Context, Server
+    from pilon_amd.synth import SynthData, poisson_arrivals, preset, serve_requests
+    t0 = time.time()
+    data = SynthData(preset("c4", scale=args.serve_scale))
+    ctx = Context(cfg, device=0)
+    rids = []
+    for ci in range(data.n):
+        ctx.ref_upload(ci, data.names[ci], data.draft(ci))
+        rids.append(ctx.readset_upload(ci, data.soa(ci)))
+    G_res = sum(len(data.draft(ci)) for ci in range(data.n))
+    log(f"serve: c4 x{args.serve_scale}: {data.n} contigs, {G_res} bp resident, ready in {time.time() - t0:.1f}s")
+    reqs = serve_requests(data, args.serve_requests, 2_000, 1_000_000, seed=1004)
+    arrive = poisson_arrivals(len(reqs), args.serve_rate, seed=1004)
+    srv = Server(ctx, lanes=args.serve_lanes, max_batch=64)
+    # warm the lanes (first batches allocate their buffers)
+    for ci, s, e in reqs[:2 * args.serve_lanes]:
+        srv.submit(ci, s, e, rids[ci], tag=-1)
+    srv.drain()
+    srv.poll(max_n=4096)
+
+    def stream(times):
+        """submit every request at its arrival time (all at once if None)"""
+        done = []
+        t_start = time.perf_counter()
+        for i, (ci, s, e) in enumerate(reqs):
+            if times is not None:
+                while time.perf_counter() - t_start < times[i]:
+                    pass
+            srv.submit(ci, s, e, rids[ci], tag=i)
+            if i % 64 == 0:
+                done += srv.poll(max_n=4096)
+        while len(done) < len(reqs):
+            done += srv.poll(max_n=4096, wait_ms=100)
+        t_end = time.perf_counter()
+        # PL_E_EDGE (4): regions on which Pilon itself throws (an indel
+        # normalised before the region start); the request ran, its result is that error
+        bad = [r for r in done if r["status"] not in (0, 4)]
+        assert not bad, bad[:3]
+        lat = np.array([r["latency_ms"] for r in done])
+        bases = sum(r["stats"]["aligned_bases"] for r in done)
+        return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "bases_per_s": bases / (t_end - t_start), "makespan_s": t_end - t_start,
+                "mean_batch": float(np.mean([r["batch_size"] for r in done])),
+                "edge_status": sum(1 for r in done if r["status"] == 4)}
+
+    cap = stream(None)
+    log(f"serve: capacity (all {len(reqs)} requests at once): {cap['bases_per_s'] / 1e9:.1f} G bases/s, "
+        f"mean batch {cap['mean_batch']:.1f}")
+    at = stream(arrive)
+    srv.close()
     ctx.close()
+    out = {"workload": "c4: 50 x 4 Mb contigs resident, 2x150 bp at 50x; 10,000 requests log-uniform 2 kb-1 Mb, "
+                       "Poisson arrivals",
+           "scale": args.serve_scale, "requests": len(reqs), "rate_per_s": args.serve_rate, "lanes": args.serve_lanes,
+           "p50_ms": at["p50_ms"], "p99_ms": at["p99_ms"], "mean_batch": at["mean_batch"],
+           "bases_per_s": at["bases_per_s"], "makespan_s": at["makespan_s"], "edge_status": at["edge_status"],
+           "capacity_bases_per_s": cap["bases_per_s"], "capacity_mean_batch": cap["mean_batch"],
+           "positions": int(sum(e - s + 1 for _, s, e in reqs))}
+    log(f"serve: {len(reqs)} requests at {args.serve_rate:.0f}/s: p50 {out['p50_ms']:.2f} ms, p99 {out['p99_ms']:.2f} ms, "
+        f"{out['bases_per_s'] / 1e9:.1f} G bases/s, mean batch {out['mean_batch']:.1f}")
+    return out
 
 
 if __name__ == "__main__":

@@ -115,6 +115,9 @@ struct Lane {
   std::vector<WinDev> hwins;
   std::vector<JobStat> hst;
   std::vector<Cand> hcand;
+  std::vector<pl_call> hccall;     // calls of the candidate columns
+  std::vector<int64_t> hins_off;   // insertion string of candidate k in hins_pool (-1: none)
+  std::string hins_pool;
   int64_t G = 0;
   int32_t W = 0;
   int32_t n_wins = 0;
@@ -675,6 +678,9 @@ pl_status autotune(pl_ctx* c, Lane* L, const std::vector<JobReq>& q, int persist
 }
 
 // copy stats (and candidates) of the last batch to the host
+// Stats, candidates and what the change lists need of them (the calls of
+// the candidate columns, the insertion strings) for every job of the last
+// batch: a fixed number of launches and copies whatever the batch size.
 pl_status fetch_batch_meta(Lane* L) {
   const size_t nj = L->hjobs.size();
   L->hst.resize(nj);
@@ -684,8 +690,40 @@ pl_status fetch_batch_meta(Lane* L) {
   HIPCHK(hipStreamSynchronize(L->stream));
   cn = std::min(cn, L->cand_cap);
   L->hcand.resize(cn);
-  if (cn) HIPCHK(hipMemcpyAsync(L->hcand.data(), L->cands.p, sizeof(Cand) * cn, hipMemcpyDeviceToHost, L->stream));
-  HIPCHK(hipStreamSynchronize(L->stream));
+  L->hccall.resize(cn);
+  L->hins_off.assign(cn, -1);
+  L->hins_pool.clear();
+  if (cn) {
+    HIPCHK(hipMemcpyAsync(L->hcand.data(), L->cands.p, sizeof(Cand) * cn, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(L->gath.ensure(sizeof(pl_call) * cn));
+    k_cand_calls<<<dim3((unsigned)std::min<uint32_t>(1024, (cn + 255) / 256)), dim3(256), 0, L->stream>>>(
+        L->cands.as<Cand>(), cn, L->calls.as<pl_call>(), L->gath.as<pl_call>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(L->hccall.data(), L->gath.p, sizeof(pl_call) * cn, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    // insertion strings of every INS candidate, decoded on the device at once
+    std::vector<int64_t> idx, off(1, 0);
+    for (uint32_t k = 0; k < cn; k++)
+      if (L->hcand[k].kind == PL_D_INS) {
+        L->hins_off[k] = off.back();
+        idx.push_back(k);
+        off.push_back(off.back() + L->hcand[k].len);
+      }
+    if (!idx.empty()) {
+      HIPCHK(L->stridx.ensure(8 * idx.size()));
+      HIPCHK(L->stroff.ensure(8 * off.size()));
+      HIPCHK(L->strpool.ensure((size_t)off.back() + 16));
+      HIPCHK(hipMemcpyAsync(L->stridx.p, idx.data(), 8 * idx.size(), hipMemcpyHostToDevice, L->stream));
+      HIPCHK(hipMemcpyAsync(L->stroff.p, off.data(), 8 * off.size(), hipMemcpyHostToDevice, L->stream));
+      k_ins_strings<<<dim3((unsigned)((idx.size() + 63) / 64)), dim3(64), 0, L->stream>>>(
+          L->args, L->cands.as<Cand>(), L->stridx.as<int64_t>(), L->stroff.as<int64_t>(), (int32_t)idx.size(),
+          L->strpool.as<uint8_t>());
+      HIPCHK(hipGetLastError());
+      L->hins_pool.resize((size_t)off.back());
+      HIPCHK(hipMemcpyAsync(&L->hins_pool[0], L->strpool.p, (size_t)off.back(), hipMemcpyDeviceToHost, L->stream));
+      HIPCHK(hipStreamSynchronize(L->stream));
+    }
+  }
   return PL_OK;
 }
 
@@ -706,7 +744,6 @@ pl_status job_changes(pl_ctx* c, Lane* L, size_t j, pl_region_stats* out, std::v
   const JobStat& S = L->hst[j];
   const Contig& ct = c->contigs.at(c->rsets[J.rs]->contig);
   std::vector<plh::Change> ch;
-  std::vector<int64_t> ins_idx;
   for (int32_t k = S.cand_lo; k < S.cand_hi && k < (int32_t)L->hcand.size(); k++) {
     const Cand& cd = L->hcand[k];
     if (!cd.kind) continue;
@@ -714,39 +751,11 @@ pl_status job_changes(pl_ctx* c, Lane* L, size_t j, pl_region_stats* out, std::v
     x.loc = J.start + (int32_t)(cd.gpos - J.out);
     x.kind = cd.kind;
     x.ref = to_upper(ct.host[x.loc - 1]);
-    x.base = 4; x.alt = 0;
+    x.base = L->hccall[k].base;
+    x.alt = L->hccall[k].alt;
     if (cd.kind == PL_D_DEL) x.indel.assign((const char*)ct.host.data() + x.loc - 1, (size_t)cd.len);
-    if (cd.kind == PL_D_INS) ins_idx.push_back(k);
+    if (cd.kind == PL_D_INS) x.indel = L->hins_pool.substr((size_t)L->hins_off[k], (size_t)cd.len);
     ch.push_back(x);
-  }
-  // base/alt of the changed columns
-  std::vector<pl_call> cl(ch.size());
-  for (size_t i = 0; i < ch.size(); i++)
-    HIPCHK(hipMemcpyAsync(&cl[i], L->calls.as<pl_call>() + J.out + (ch[i].loc - J.start), sizeof(pl_call),
-                          hipMemcpyDeviceToHost, L->stream));
-  // insertion strings decoded on the device
-  std::string pool;
-  std::vector<int64_t> off(ins_idx.size() + 1, 0);
-  if (!ins_idx.empty()) {
-    for (size_t i = 0; i < ins_idx.size(); i++) off[i + 1] = off[i] + L->hcand[ins_idx[i]].len;
-    HIPCHK(L->stridx.ensure(8 * ins_idx.size()));
-    HIPCHK(L->stroff.ensure(8 * off.size()));
-    HIPCHK(L->strpool.ensure((size_t)off.back() + 16));
-    HIPCHK(hipMemcpyAsync(L->stridx.p, ins_idx.data(), 8 * ins_idx.size(), hipMemcpyHostToDevice, L->stream));
-    HIPCHK(hipMemcpyAsync(L->stroff.p, off.data(), 8 * off.size(), hipMemcpyHostToDevice, L->stream));
-    k_ins_strings<<<dim3((unsigned)((ins_idx.size() + 63) / 64)), dim3(64), 0, L->stream>>>(
-        L->args, L->cands.as<Cand>(), L->stridx.as<int64_t>(), L->stroff.as<int64_t>(), (int32_t)ins_idx.size(),
-        L->strpool.as<uint8_t>());
-    HIPCHK(hipGetLastError());
-    pool.resize((size_t)off.back());
-    HIPCHK(hipMemcpyAsync(&pool[0], L->strpool.p, (size_t)off.back(), hipMemcpyDeviceToHost, L->stream));
-  }
-  HIPCHK(hipStreamSynchronize(L->stream));
-  size_t ii = 0;
-  for (size_t i = 0; i < ch.size(); i++) {
-    ch[i].base = cl[i].base;
-    ch[i].alt = cl[i].alt;
-    if (ch[i].kind == PL_D_INS) { ch[i].indel = pool.substr((size_t)off[ii], (size_t)(off[ii + 1] - off[ii])); ii++; }
   }
   if (out) {
     std::memset(out, 0, sizeof *out);

@@ -985,6 +985,11 @@ __global__ void k_final_flags(EngineArgs A) {
 }
 
 // insertion strings of INS changes into a byte pool
+// the calls of the candidate columns, in candidate order (one copy to the host)
+__global__ void k_cand_calls(const Cand* c, uint32_t n, const pl_call* calls, pl_call* out) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = calls[c[k].gpos];
+}
+
 __global__ void k_ins_strings(EngineArgs A, const Cand* sorted, const int64_t* idx, const int64_t* off, int32_t n, uint8_t* pool) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
