@@ -427,7 +427,7 @@ pl_status enqueue_batch(Lane* L, EngineArgs& A, int persistent, int block, int g
                  A.cand_n, A.cand_cap, bits_for((uint64_t)G), &ck, &cv);
   if (s) return s;
   k_cand_gather<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, cv, L->cands.as<Cand>());
-  k_finalize<<<dim3((n_jobs + 63) / 64), dim3(64), 0, st>>>(A, L->cands.as<Cand>(), A.cand_n, A.cand_cap, L->cands.as<Cand>());
+  k_finalize<<<dim3(n_jobs), dim3(64), 0, st>>>(A, L->cands.as<Cand>(), A.cand_n, A.cand_cap, L->cands.as<Cand>());
   const uint32_t gm = (uint32_t)std::min<int64_t>(2048, std::max<int64_t>(1, ((int64_t)A.mutated_cap + 3) / 4));
   k_indel_pass<<<dim3(gm), dim3(256), 0, st>>>(A, A.mutated, A.mutated_n, A.mutated_cap, 1);
   HIPCHK(hipGetLastError());

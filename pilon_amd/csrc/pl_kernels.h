@@ -16,3 +16,4 @@
 #define PL_DEFAULT_BLOCK 256
 #define PL_WIN_LDS_PER_POS 76     // window LDS bytes per position (counters + difference arrays)
 #define PL_WIN_RECS 128           // indel evidence records a window buffers in LDS
+#define PL_FIN_DELS 1024          // active deletion calls one region keeps in LDS in Pass 1

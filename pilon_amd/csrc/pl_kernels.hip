@@ -948,21 +948,112 @@ __global__ void __launch_bounds__(256) k_indel_pass(EngineArgs A, const uint32_t
 }
 
 // ------------------------------------------------------------------ Pass 1
-__global__ void k_finalize(EngineArgs A, const Cand* sorted, const uint32_t* cand_n, uint32_t cap, Cand* sorted_mut) {
-  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+// GenomeRegion.postProcess Pass 1 (GenomeRegion.scala:221-272) for one
+// region per wave.  Only deletion calls carry state along the scan (a
+// called deletion suppresses the candidates under it), so: the region's
+// deletion candidates are scanned in order to find the active ones (few);
+// every candidate is then cleared at once if too shallow or under an active
+// deletion; each active deletion's positions are then mutated by the lanes.
+// Same result as finalize_job, which a region with more than PL_FIN_DELS
+// active deletions runs serially.
+__global__ void __launch_bounds__(64) k_finalize(EngineArgs A, const Cand* sorted, const uint32_t* cand_n, uint32_t cap,
+                                                 Cand* cand) {
+  __shared__ int64_t s_dg[PL_FIN_DELS];
+  __shared__ int32_t s_dl[PL_FIN_DELS];
+  const int32_t j = blockIdx.x;
   if (j >= A.n_jobs) return;
+  const int lane = threadIdx.x;
   const uint32_t n = *cand_n < cap ? *cand_n : cap;
   const JobDev& J = A.jobs[j];
-  // candidates of this job: gpos in [out, out+size)
-  int64_t lo = 0, hi = n;
-  while (lo < hi) { int64_t m = (lo + hi) >> 1; if ((int64_t)sorted[m].gpos < J.out) lo = m + 1; else hi = m; }
-  int64_t a = lo;
-  hi = n;
-  while (lo < hi) { int64_t m = (lo + hi) >> 1; if ((int64_t)sorted[m].gpos < J.out + J.size) lo = m + 1; else hi = m; }
-  const int64_t b = lo;
-  A.st[j].cand_lo = (int32_t)a;
-  A.st[j].cand_hi = (int32_t)b;
-  finalize_job(A.cf, J, A.st[j], A.pile, A.G, A.calls, sorted_mut, a, b, A.mutated, A.mutated_n, A.mutated_cap, A.overflow);
+  JobStat& st = A.st[j];
+  // candidates of this region: gpos in [out, out + size)
+  int64_t a, b;
+  {
+    int64_t lo = 0, hi = n;
+    while (hi - lo > 64) {
+      const int64_t m = hi - lo, pr = lo + ((int64_t)(lane + 1) * m) / 65;
+      const int k = __popcll(__ballot((int64_t)sorted[pr].gpos < J.out));
+      const int64_t plo = lo + ((int64_t)k * m) / 65, phi = lo + ((int64_t)(k + 1) * m) / 65;
+      if (k > 0) lo = plo + 1;
+      if (k < 64) hi = phi;
+    }
+    a = lo + __popcll(__ballot(lo + lane < hi && (int64_t)sorted[lo + lane].gpos < J.out));
+    lo = a; hi = n;
+    while (hi - lo > 64) {
+      const int64_t m = hi - lo, pr = lo + ((int64_t)(lane + 1) * m) / 65;
+      const int k = __popcll(__ballot((int64_t)sorted[pr].gpos < J.out + J.size));
+      const int64_t plo = lo + ((int64_t)k * m) / 65, phi = lo + ((int64_t)(k + 1) * m) / 65;
+      if (k > 0) lo = plo + 1;
+      if (k < 64) hi = phi;
+    }
+    b = lo + __popcll(__ballot(lo + lane < hi && (int64_t)sorted[lo + lane].gpos < J.out + J.size));
+  }
+  if (lane == 0) { st.cand_lo = (int32_t)a; st.cand_hi = (int32_t)b; }
+  const int32_t md = min_depth_of(A.cf, st.base_count, J.size);
+  const bool none = st.read_count == 0;
+  // active deletions, in order
+  int nd = 0;
+  int64_t delEnd = -1;
+  bool serial = false;
+  for (int64_t c0 = a; c0 < b && !serial; c0 += 64) {
+    const int64_t k = c0 + lane;
+    const bool isd = k < b && !none && cand[k].kind == PL_D_DEL && cand[k].depth >= md;
+    for (uint64_t m = __ballot(isd); m; m &= m - 1) {
+      const int l = __ffsll((unsigned long long)m) - 1;
+      const int64_t g = (int64_t)cand[c0 + l].gpos;
+      const int32_t len = cand[c0 + l].len;
+      if (g <= delEnd) continue;
+      if (nd == PL_FIN_DELS) { serial = true; break; }
+      if (lane == 0) { s_dg[nd] = g; s_dl[nd] = len; }
+      nd++;
+      delEnd = g + len - 1;
+    }
+  }
+  __syncthreads();
+  if (serial) {
+    if (lane == 0)
+      finalize_job(A.cf, J, st, A.pile, A.G, A.calls, cand, a, b, A.mutated, A.mutated_n, A.mutated_cap, A.overflow);
+    return;
+  }
+  // every candidate at once: shallow, or under an active deletion (not its own)
+  for (int64_t c0 = a; c0 < b; c0 += 64) {
+    const int64_t k = c0 + lane;
+    if (k >= b) continue;
+    const int64_t g = (int64_t)cand[k].gpos;
+    int lo = 0, hi = nd;                                  // active deletions starting before g
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if (s_dg[mid] < g) lo = mid + 1; else hi = mid; }
+    const bool shadowed = lo > 0 && g <= s_dg[lo - 1] + s_dl[lo - 1] - 1;
+    if (none || shadowed || cand[k].depth < md) cand[k].kind = 0;
+  }
+  // the active deletions: deleted positions take the deletion count (Pass 1
+  // mutation, GenomeRegion.scala:262-264), confirmed positions under them leave the count
+  int64_t dropped = 0;
+  for (int i = 0; i < nd; i++) {
+    const int64_t g = s_dg[i];
+    const int32_t len = s_dl[i];
+    const int32_t dcount = A.pile[(int64_t)PL_F_DELETIONS * A.G + g];
+    for (int32_t q = 1 + lane; q < len; q += 64) {
+      const int64_t gj = g + q;
+      int64_t dep = 0;
+      for (int bb = 0; bb < 4; bb++) dep += (int64_t)(uint32_t)A.pile[(int64_t)(PL_F_COUNT_A + bb) * A.G + gj];
+      dep += A.pile[(int64_t)PL_F_DELETIONS * A.G + gj];
+      if (call_disp(A.calls[gj]) == PL_D_CONFIRMED && dep >= md) dropped++;
+      A.pile[(int64_t)PL_F_DELETIONS * A.G + gj] = j_add(A.pile[(int64_t)PL_F_DELETIONS * A.G + gj], dcount);
+      A.calls[gj].flags |= PL_CF_DELETED;
+      const uint32_t slot = atomicAdd(A.mutated_n, 1u);
+      if (slot < A.mutated_cap) A.mutated[slot] = (uint32_t)gj;
+      else *A.overflow = 1;
+    }
+  }
+  dropped = wave_sum(dropped);
+  if (lane == 0) {
+    st.min_depth = md;
+    int64_t nconf = 0;
+    if (md <= 255) for (int d = md; d < 256; d++) nconf += st.hist[d];
+    else nconf = -1;
+    if (nconf > 0) nconf = nconf - dropped > 0 ? nconf - dropped : 0;   // the serial scan stops decrementing at 0
+    st.n_confirmed = (int32_t)nconf;
+  }
 }
 
 // ------------------------------------------------------------------ export
