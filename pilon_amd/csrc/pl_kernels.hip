@@ -829,10 +829,12 @@ __device__ int64_t wave_lower_bound(const uint32_t* keys, int64_t n, uint32_t x,
 }
 
 // strict majority over one (position, kind) group of sorted records, by the
-// whole wave: each lane counts the records equal to its own in (length,
-// hash) — an upper bound of the exact count — and the largest group's string
-// is confirmed by exact comparison; a hash collision falls back to the serial
-// scan.  Same result as group_majority.
+// whole wave.  Each lane runs a Boyer-Moore vote over its strided share of
+// the records, comparing (length, hash) classes; the 64 partial votes are
+// merged pairwise (a merge keeps a majority class if there is one), and the
+// winning class's string is counted exactly.  A class with a strict
+// majority whose representative string falls short (a hash collision) is
+// resolved by the serial scan.  Same result as group_majority.
 __device__ Majority wave_majority(const RSDev& R, const IndelRec* recs, const uint32_t* idx, int64_t a, int64_t b,
                                   int lane) {
   Majority m;
@@ -840,31 +842,42 @@ __device__ Majority wave_majority(const RSDev& R, const IndelRec* recs, const ui
   m.win = 0;
   m.rec = -1;
   if (b <= a) return m;
-  if (b - a > 64) return group_majority(R, recs, idx, a, b);
-  const int n = (int)(b - a);
-  const bool in = lane < n;
-  const int64_t ri = in ? (int64_t)idx[a + lane] : 0;
-  IndelRec e;
-  e.len = -1; e.hash = 0; e.key = 0; e.rot = 0; e.flags = 0; e.src = 0;
-  if (in) e = recs[ri];
-  int hc = 0;
-  for (int j = 0; j < n; j++) {
-    const int32_t lj = __builtin_amdgcn_readlane(e.len, j);
-    const uint32_t hj = __builtin_amdgcn_readlane(e.hash, j);
-    hc += (in && e.len == lj && e.hash == hj) ? 1 : 0;
+  int32_t vl = -1, vc = 0;
+  uint32_t vh = 0;
+  int64_t vr = -1;
+  for (int64_t i = a + lane; i < b; i += 64) {
+    const int64_t r = idx[i];
+    const int32_t len = recs[r].len;
+    const uint32_t h = recs[r].hash;
+    if (vc == 0) { vl = len; vh = h; vr = r; vc = 1; }
+    else if (len == vl && h == vh) vc++;
+    else vc--;
   }
-  int best = in ? hc * 64 + (63 - lane) : -1;
-  for (int d = 32; d >= 1; d >>= 1) { const int o = __shfl_xor(best, d, 64); best = o > best ? o : best; }
-  const int bc = best >> 6, bl = 63 - (best & 63);
-  if (2 * bc <= n) { m.win = bc; return m; }          // no string can have a strict majority
-  IndelRec c;
-  c.len = __shfl(e.len, bl, 64); c.hash = (uint32_t)__shfl((int)e.hash, bl, 64); c.key = (uint32_t)__shfl((int)e.key, bl, 64);
-  c.rot = __shfl(e.rot, bl, 64); c.flags = (uint32_t)__shfl((int)e.flags, bl, 64);
-  c.src = ((int64_t)__shfl((int)(e.src >> 32), bl, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)e.src, bl, 64);
-  const bool same = in && rec_same(R, e, c);
-  const int exact = __popcll(__ballot(same));
-  if (2 * exact > n) { m.win = exact; m.rec = __shfl((long long)ri, bl, 64); return m; }
-  return group_majority(R, recs, idx, a, b);
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t ol = __shfl_xor(vl, d, 64), oc = __shfl_xor(vc, d, 64);
+    const uint32_t oh = (uint32_t)__shfl_xor((int)vh, d, 64);
+    const int64_t orr = __shfl_xor((long long)vr, d, 64);
+    if (ol == vl && oh == vh) vc += oc;
+    else if (vc >= oc) vc -= oc;
+    else { vl = ol; vh = oh; vr = orr; vc = oc - vc; }
+  }
+  const int64_t cr = __shfl((long long)vr, 0, 64);
+  if (cr < 0) return m;
+  const IndelRec c = recs[cr];
+  int32_t cls = 0, same = 0;
+  for (int64_t i = a + lane; i < b; i += 64) {
+    const IndelRec& e = recs[idx[i]];
+    if (e.len == c.len && e.hash == c.hash) {
+      cls++;
+      if (rec_same(R, e, c)) same++;
+    }
+  }
+  cls = wave_sum(cls);
+  same = wave_sum(same);
+  if (2 * same > m.n) { m.win = same; m.rec = cr; return m; }
+  if (2 * cls > m.n) return group_majority(R, recs, idx, a, b);   // collision inside the majority class
+  m.win = same;                                                     // no strict majority
+  return m;
 }
 
 // the region (job) holding batch position g: the last job whose first
