@@ -40,6 +40,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def kernel_sources_sha256():
+    """Hash of the window kernel's sources (a traffic profile is valid for these only)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("pl_common.h", "pl_core.h", "pl_kernels.h", "pl_kernels.hip"):
+        with open(os.path.join(ROOT, "pilon_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(path, workload, window, block):
+    """HBM bytes per k_window launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE profile, when it was taken on these sources and this launch."""
+    import re
+    try:
+        text = open(path).read()
+    except OSError:
+        return None, None
+    sha = re.search(r"kernel_sources_sha256: (\w+)", text)
+    cfg = re.search(r"launch: (\w+) window=(\d+) block=(\d+)", text)
+    tb = re.search(r"traffic_bytes_per_launch: (\d+)", text)
+    if not (sha and cfg and tb) or sha.group(1) != kernel_sources_sha256():
+        return None, None
+    if (cfg.group(1), int(cfg.group(2)), int(cfg.group(3))) != (workload, window, block):
+        return None, None
+    return int(tb.group(1)), os.path.relpath(path, ROOT)
+
+
 def dev_sync():
     import torch
     if torch.cuda.is_available():
@@ -87,7 +115,19 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=20000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB on the device three times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_window_c3_traffic.md"))
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied 3 x {4 << 28} bytes", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path would shard "
               "by region with no communication — see DESIGN.md", file=sys.stderr)
@@ -156,6 +196,7 @@ def main():
     win_ms = float(np.mean(win))
     b_alg = alg_read_bytes + G * 80.25 + 32 * info["n_records"]
     achieved = b_alg / (win_ms / 1000.0) / 1e9
+    traffic, traffic_src = measured_traffic(args.traffic_profile, args.config, W, B) if args.scale == 1.0 else (None, None)
     log(f"bench: {ms_per_step:.3f} ms/step wall ({'graph' if args.graph else 'eager'}), device pipeline {pipe_ms:.3f} ms, "
         f"window kernel {win_ms:.3f} ms, {value / 1e9:.2f} G bases/s, window kernel {achieved:.0f} GB/s algorithmic")
 
@@ -207,8 +248,8 @@ def main():
                    "window": W, "block": B, "autotuned": not (args.window and args.block),
                    "persistent": args.persistent, "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_window",
-                     "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "k_window", "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms},
         "device_pipeline_ms": pipe_ms,
         "cpu_baseline": cpu,
         "p99_latency_ms": serving["p99_ms"] if serving else None,
