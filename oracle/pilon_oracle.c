@@ -10,6 +10,19 @@
  * time, with Java's integer semantics (Int wraps at 32 bits, Long at 64,
  * division truncates).  Every function names the reference lines it restates.
  * Reference: broadinstitute/pilon v1.24, src/main/scala/org/broadinstitute/pilon/
+ *
+ * Pins (tests/test_oracle_kat.py): hand-derived known answers per addRead
+ * rule, the physCov closed form, the flank-0 count identity, Math.round
+ * corner cases, stable order ties, BaseCall scores, the hetIndelCall
+ * threshold table, the oldIndel branch, strict majority and N rejection, the
+ * AMB disposition and its change line with and without --iupac, planted
+ * variants fixed.
+ * Unreachable here: the overlap branch of fixFixList (GenomeRegion.scala:
+ * 557-593).  Pass 1 adds at most one change per locus and marks the loci a
+ * deletion covers as deleted, so no two fixes it makes overlap.
+ * Parity unpinned (no independent check of this reading of the reference):
+ * the htsjdk decoding of non-ACGT SEQ codes (the host packer's side of the
+ * boundary).
  */
 #include <stdint.h>
 #include <stdlib.h>

@@ -162,8 +162,54 @@ def case_no_qual():
     return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp)
 
 
+def case_ambiguous(iupac=False):
+    """GenomeRegion.scala:255-271,326-341; PileUp.scala:132-167; Bases.scala:67-90.
+    Ten 30M reads at 5 (trusted loci 15..24) carry C (five) or G (five) at
+    locus 20 (ref T), q30, MAPQ 60.  qualSum C = G = 5*30*61 = 9150; the
+    stable order keeps C before G (base C, alt G); total 18300, homoScore 0,
+    heteroScore 18300 -> not homo; score = 18300*10/610 = 300, q 30.
+    b != r, score > 0, not homo, alt != r -> AMB.  The fix is (20, T, C); with
+    --iupac it goes to the small fix list as toIUPAC(C, G) = S."""
+    reads = []
+    for k in range(10):
+        seq = REF_A[4:19] + ("C" if k < 5 else "G") + REF_A[20:34]
+        reads.append(read(5, "30M", seq, [30] * 30))
+    base = "S" if iupac else "C"
+    exp = {20: {"count_C": 5, "count_G": 5, "qual_C": 9150, "qual_G": 9150, "mqSum": 610, "qSum": 300}}
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect=exp, changes=f"ctg:20 ctg_pilon:20 T {base}\n",
+                bases=REF_A[:19] + base + REF_A[20:], disp={20: 5}, stats={"amb": 1, "snps": 0},
+                cfg={"iupac": 1} if iupac else {})
+
+
+def case_ambiguous_iupac():
+    return case_ambiguous(iupac=True)
+
+
+def case_het_deletion(old_indel=False):
+    """PileUp.scala:209-246; GenomeRegion.scala:255-271.  Six reads 14M2D16M
+    delete loci 15-16 (as in case_deletion_call) beside twelve 30M reads of
+    the reference.  At 15: count 12 (A), deletions 6, mqSum 12*61 + 6*61 =
+    1098, delQual 366, qSum 12*30 + 6*30 = 540; delPct = max(roundDiv(36600,
+    1098), roundDiv(600, 18)) = 33.  New heuristics (L=2: middle 43, low 21,
+    high 65): het deletion -> not homoIndel, no DEL, not confirmed (indel) ->
+    disposition NONE.  --oldindel: 33 >= 33 but 33 < 50-2 -> no indel call;
+    homo A, q = 21960*12/1098/12 = 20 -> CONFIRMED.  No change either way."""
+    seq = REF_C[0:14] + REF_C[16:32]
+    reads = [read(1, "14M2D16M", seq, [30] * 30) for _ in range(6)]
+    reads += [read(1, "30M", REF_C[0:30], [30] * 30) for _ in range(12)]
+    exp = {15: {"count_A": 12, "deletions": 6, "delQual": 366, "mqSum": 1098, "qSum": 540}}
+    return dict(ref=REF_C, start=1, stop=40, reads=reads, expect=exp, changes="", bases=REF_C,
+                disp={15: 1 if old_indel else 0}, stats={"dels": 0},
+                cfg={"old_indel": 1} if old_indel else {})
+
+
+def case_het_deletion_oldindel():
+    return case_het_deletion(old_indel=True)
+
+
 ALL = [case_trusted_flank, case_soft_clip, case_invalid_pair, case_deletion_slide, case_insertion_rotation,
-       case_deletion_call, case_physcov_pairs, case_n_base, case_no_qual]
+       case_deletion_call, case_physcov_pairs, case_n_base, case_no_qual, case_ambiguous, case_ambiguous_iupac,
+       case_het_deletion, case_het_deletion_oldindel]
 
 
 def batch_of(case):

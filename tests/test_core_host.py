@@ -19,7 +19,7 @@ def oracle():
 def test_kat_core_host(make, W):
     case = make()
     b = K.batch_of(case)
-    cfg = default_config()
+    cfg = default_config(**case.get("cfg", {}))
     ref = oracle().run_region(cfg, case["ref"], case["start"], case["stop"], [b], name="ctg")
     got = CH.run(cfg, case["ref"].encode(), b, [(case["start"], case["stop"])], W=W, name="ctg")[0]
     assert got["status"] == 0

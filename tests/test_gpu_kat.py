@@ -23,6 +23,10 @@ def test_kat_gpu(engine, make):
     from oracle import oracle as O
     case = make()
     print(f"case: {make.__name__}", flush=True)
+    cfg = default_config(**case.get("cfg", {}))
+    if case.get("cfg"):
+        from pilon_amd.engine import Context
+        engine = Context(cfg, device=0)
     engine.ref_upload(100, "ctg", case["ref"].encode())
     pr = PileUpRegion(engine, 100, case["start"], case["stop"])
     pr.addReads(K.batch_of(case))
@@ -30,7 +34,7 @@ def test_kat_gpu(engine, make):
     changes, bases = pr.fix()
     res = {"status": 0, "pile": pr.pile, "calls": pr.calls, "stats": pr.stats, "changes": changes, "bases": bases}
     check_case(res, case)
-    ref = O.run_region(default_config(), case["ref"], case["start"], case["stop"], [K.batch_of(case)], name="ctg")
+    ref = O.run_region(cfg, case["ref"], case["start"], case["stop"], [K.batch_of(case)], name="ctg")
     assert_same_region(pr.pile, pr.calls, ref, make.__name__)
     assert_same_stats(pr.stats, ref["stats"])
 

@@ -36,7 +36,8 @@ def check_case(res, case):
 @pytest.mark.parametrize("make", K.ALL, ids=[f.__name__ for f in K.ALL])
 def test_kat_oracle(make):
     case = make()
-    res = oracle().run_region(default_config(), case["ref"], case["start"], case["stop"], [K.batch_of(case)])
+    res = oracle().run_region(default_config(**case.get("cfg", {})), case["ref"], case["start"], case["stop"],
+                              [K.batch_of(case)])
     check_case(res, case)
 
 
@@ -106,6 +107,19 @@ def test_het_indel_thresholds(count, flags_want):
     else:
         assert c["flags"] & PL_CF_DELETION and L == 3
         assert bool(c["flags"] & PL_CF_HOMOINDEL) == (flags_want == "homo")
+
+
+@pytest.mark.parametrize("count,called", [(5, True), (20, False), (40, False)])
+def test_old_indel_branch(count, called):
+    """hetIndelCall with --oldindel (PileUp.scala:223-228) calls homozygous
+    indels only, when pct >= 33 and pct >= 50 - L: ten deletions of length 3
+    with count 5 give delPct 67 (called, homozygous); count 20 gives 33,
+    which the new heuristics call het but 33 < 47 here."""
+    c, L, _ = oracle().basecall(default_config(old_indel=1), bc=[count, 0, 0, 0], qs=[count * 30, 0, 0, 0],
+                                mqSum=100000, qSum=1, deletions=10, del_list=["AAA"] * 10)
+    assert bool(c["flags"] & PL_CF_INDEL) == called
+    if called:
+        assert c["flags"] & PL_CF_DELETION and c["flags"] & PL_CF_HOMOINDEL and L == 3
 
 
 def test_het_indel_majority_and_n():
