@@ -77,7 +77,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
   R.exo_read = s->exo_read; R.exo_off = s->exo_off; R.nib = s->nib;
   R.meta = meta.data(); R.ref = ref; R.ref_len = ref_len;
   // ---- read-set prep (k_readset_prep)
-  int32_t max_back = 0, max_lead = 0;
+  int32_t max_back = 0, max_lead = 0, max_back_fp = 0;
   std::vector<int64_t> outliers;
   for (int64_t r = 0; r < n; r++) meta[r] = prep_read(R, r, cf).m;
   // segment table (k_prep counts, k_segfill fills)
@@ -105,6 +105,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
     if (lead > PL_MAX_LEAD || back > PL_MAX_BACK || R.seq_off[r + 1] - R.seq_off[r] >= PL_MAX_SEQ) { outliers.push_back(r); meta[r].info |= RI_OUTLIER; continue; }
     max_back = std::max<int32_t>(max_back, (int32_t)std::max<int64_t>(back, 0));
     max_lead = std::max<int32_t>(max_lead, (int32_t)std::max<int64_t>(lead, 0));
+    if (m.fp_lo <= m.fp_hi && m.fp_hi > m.pos) max_back_fp = std::max<int32_t>(max_back_fp, m.fp_hi - m.pos);
   }
   // ---- jobs and windows
   int64_t G = 0;
@@ -157,11 +158,19 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
       sk.bad_ = badA.data(); sk.clp_ = clpA.data(); sk.pc = pcA.data(); sk.is = isA.data(); sk.recs = &recs;
       const int64_t lo = lower_pos(R.pos, n, (int64_t)L0 - max_back);
       const int64_t hi = lower_pos(R.pos, n, (int64_t)L1 + max_lead);
+      // fragment-only reads from their tails, as the window kernel does
+      const int64_t plo = fragment_only_end(R.pos, lo, hi, J, L0, max_back, max_back_fp);
+      int32_t c0 = 0, s0 = 0;
+      for (int64_t r = lo; r < plo; r++) {
+        const FragTail t = frag_tail(meta[r]);
+        if (frag_tail_event(t, L0, L1, sk)) { c0++; s0 = j_add(s0, t.size); }
+      }
+      if (c0) sk.frag(0, c0, s0);
       // same staging as the window kernel: runs for the flattened loop,
       // one-wave walks for the rest
       std::vector<Run> runs;
       std::vector<int64_t> cplx;
-      for (int64_t r = lo; r < hi; r++) {
+      for (int64_t r = plo; r < hi; r++) {
         const RMeta m = meta[r];
         if ((m.info & RI_OUTLIER) || !read_in_query(m, J)) continue;
         frag_event(m, J, L0, L1, sk);

@@ -44,6 +44,15 @@ struct __attribute__((aligned(16))) RMeta {
   uint32_t op0;              // first CIGAR word (0 if none)
 };
 
+// The part of physCovIncr a window needs from a read whose pileup footprint
+// ends before the window and whose fragment starts before it: the fragment's
+// end and size.  hi = INT32_MIN when the read adds no fragment (no fragment,
+// not kept, malformed, or handled from the outlier list).
+struct FragTail {
+  int32_t hi;                // fragment end (exclusive)
+  int32_t size;              // end - start, Int arithmetic
+};
+
 struct RSDev {               // one resident read set (one contig, sorted by pos)
   int64_t n;
   const int32_t* pos;
@@ -66,6 +75,8 @@ struct RSDev {               // one resident read set (one contig, sorted by pos
   int64_t n_outlier;
   int64_t* outliers;         // ascending read indices
   int32_t* scal;             // [0]=max_back [1]=max_lead [2]=max_qspan [3]=n_outlier(int)
+                             // [6]=max_back of the pileup footprints alone
+  FragTail* ftail;           // [n] per read
 };
 
 struct JobDev {              // one region of a batch
@@ -83,6 +94,8 @@ struct WinDev {
   int32_t p0, p1;            // region-relative [p0, p1)
   int32_t order;             // position in the work queue
   int64_t lo, hi;            // reads whose pos is in [locus(p0)-back, locus(p1)+lead)
+  int64_t plo;               // reads in [lo, plo) reach the window only through their fragments
+  int64_t pad_;
 };
 
 struct JobStat {             // device-written per job

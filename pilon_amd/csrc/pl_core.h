@@ -313,6 +313,47 @@ PL_HD void frag_event(const RMeta& m, const JobDev& J, int32_t L0, int32_t L1, S
   }
 }
 
+// What frag_event needs of a read that reaches a window only through its
+// fragment (computed once per read set, after the outlier decision).
+PL_HD FragTail frag_tail(const RMeta& m) {
+  FragTail t;
+  t.hi = INT32_MIN;
+  t.size = 0;
+  if ((m.info & RI_KEEP) && !(m.info & (RI_BAD | RI_OUTLIER)) && m.fr_lo < m.fr_hi) {
+    t.hi = m.fr_hi;
+    t.size = j_sub(m.fr_hi, m.fr_lo);
+  }
+  return t;
+}
+
+// End of the window's fragment-only reads: [lo, end) holds the reads with
+// pos < L0 - back_fp.  Their pileup footprint ends before L0 (fp_hi <= pos +
+// back_fp), and their fragment starts at or before pos (fr_lo <= pos), so
+// frag_event reduces to +1/+size at L0 and -1/-size at the fragment end,
+// which frag_tail holds.  read_in_query reduces to the kept/valid flags when
+// every such read has pos - 1 >= qs (spanEnd >= pos - 1) and pos < L0 <= qe;
+// otherwise the window takes every read through the full path.
+PL_HD int64_t fragment_only_end(const int32_t* pos, int64_t lo, int64_t hi, const JobDev& J, int32_t L0,
+                                int32_t back, int32_t back_fp) {
+  if ((int64_t)L0 - back - 1 < (int64_t)J.qs || back_fp >= back) return lo;
+  const int64_t x = (int64_t)L0 - back_fp;
+  while (lo < hi) {
+    const int64_t m = (lo + hi) >> 1;
+    if ((int64_t)pos[m] < x) lo = m + 1; else hi = m;
+  }
+  return lo;
+}
+
+// frag_event of a fragment-only read from its tail: returns 1 when it adds
+// +1/+size at window position 0 (the caller sums those), and applies the
+// -1/-size at the fragment end when that falls inside the window.
+template <class Sink>
+PL_HD int frag_tail_event(const FragTail& t, int32_t L0, int32_t L1, Sink& s) {
+  if (t.hi <= L0) return 0;
+  if (t.hi < L1) s.frag(t.hi - L0, -1, j_sub(0, t.size));
+  return 1;
+}
+
 // PileUpRegion.addRead restricted to the positions of one window [L0, L1)
 // (a slice of region [J.start, J.stop]).  Lanes stride over the bases of an
 // op; lane 0 handles the per-op events.  The Sink receives integer

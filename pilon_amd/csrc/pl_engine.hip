@@ -289,6 +289,7 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   d.ref_len = it->second.len;
   HIPCHK(hipMalloc(&p, sizeof(RMeta) * (size_t)(n + 1))); rs->mem.push_back(p); d.meta = (RMeta*)p;
   HIPCHK(hipMalloc(&p, 64)); rs->mem.push_back(p); d.scal = (int32_t*)p;
+  HIPCHK(hipMalloc(&p, sizeof(FragTail) * (size_t)(n + 1))); rs->mem.push_back(p); d.ftail = (FragTail*)p;
   uint32_t *keepflag, *segcount, *segoff;
   HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); keepflag = (uint32_t*)p;
   HIPCHK(hipMalloc(&p, 4 * (size_t)(n + 1))); segcount = (uint32_t*)p;
@@ -338,7 +339,7 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   (void)hipFree(keepflag);
   (void)hipFree(segcount);
   (void)hipFree(segoff);
-  rs->bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 32 * n + 8 * n;
+  rs->bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 32 * n + 8 * n + 8 * n;
   c->rsets.push_back(std::move(rs));
   c->rs_version++;
   *out_id = (int32_t)c->rsets.size() - 1;

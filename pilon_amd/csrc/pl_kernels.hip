@@ -103,6 +103,7 @@ __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount
         } else {
           atomicMax(&scal[0], (int32_t)(back > 0 ? back : 0));
           atomicMax(&scal[1], (int32_t)(lead > 0 ? lead : 0));
+          if (m.fp_lo <= m.fp_hi && m.fp_hi > m.pos) atomicMax(&scal[6], m.fp_hi - m.pos);
         }
       }
       const int32_t spanEnd = m.aend != 0 ? m.aend : m.pos;
@@ -111,6 +112,7 @@ __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount
       if (m.info & (RI_BAD | RI_EDGE_CLIP)) atomicOr(&scal[5], 1);
     }
     R.meta[r] = m;
+    R.ftail[r] = frag_tail(m);
     keepflag[r] = keep ? 1u : 0u;
   }
 }
@@ -203,8 +205,9 @@ __global__ void k_plan(const RSDev* rsets, const JobDev* jobs, WinDev* wins, int
   const int32_t back = R.scal[0], lead = R.scal[1];
   win.lo = lower_pos_dev(R.pos, 0, R.n, L0 - back);
   win.hi = lower_pos_dev(R.pos, win.lo, R.n, L1 + lead);
+  win.plo = fragment_only_end(R.pos, win.lo, win.hi, J, L0, back, R.scal[6]);
   // descending-cost order key for the work queue (largest windows first)
-  const int64_t c = (win.hi - win.lo) + (int64_t)(win.p1 - win.p0) / 16;
+  const int64_t c = (win.hi - win.plo) + (win.plo - win.lo) / 8 + (int64_t)(win.p1 - win.p0) / 16;
   cost[w] = 0xffffffffu - (uint32_t)(c > 0xfffffffll ? 0xfffffffll : c);
 }
 
@@ -425,7 +428,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     // wave-local, so the read loop has no workgroup barrier.
     RunL* wrun = s_run + wave * (2 * 64 + 1);
     // the next chunk's read facts are loaded while the current one is piled
-    int64_t t0 = win.lo + 64 * wave;
+    int64_t t0 = win.plo + 64 * wave;
     RMeta mN;
     int64_t sN = 0, eN = 0, qN = 0;
     mN.info = 0;
@@ -561,6 +564,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
       // positions apart, so their 64-bit atomics spread over all LDS banks.
       if (CT > 0) {
         const int32_t mq = cf.min_qual;
+        const uint32_t W8 = 8u * (uint32_t)W;
         const int64_t qb4 = qbase & ~(int64_t)3, qb16 = qbase & ~(int64_t)15;
         const gu32* tq32 = (const gu32*)(gq + qb4);
         const gu32* tb32 = (const gu32*)(gb + (qb16 >> 2));
@@ -602,15 +606,27 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
           for (int k = 0; k < PL_CHUNK / 16 + 1; k++) x[k] = __builtin_amdgcn_alignbit(bw[k + 1], bw[k], 2u * (qq & 15u));
           unsigned long long* bsp = bs + p;
           unsigned long long* mqp = mqq + p;
+          // The base's counter array is b*W entries on: a 24-bit multiply-add
+          // (b < 4, 8*W < 2^24).  With minQual >= 0 every qual that passes is
+          // non-negative, so qual*mq1 >= 0 and the count word is the constant
+          // 1 (pack_base without its sign extension).
+          auto pile = [&](auto nonneg) {
 #pragma unroll
-          for (int t = 0; t < PL_CHUNK; t++) {
-            const int32_t qual = (int32_t)(int8_t)(w[t >> 2] >> (8 * (t & 3)));
-            const int b = (int)((x[t >> 4] >> (2 * (t & 15))) & 3u);
-            if (t < cnt && qual >= mq) {
-              atomicAdd(bsp + b * W + t, (unsigned long long)pack_base(qual, mq1));
-              atomicAdd(mqp + t, (unsigned long long)pack_mq_q(mq1, qual));
+            for (int t = 0; t < PL_CHUNK; t++) {
+              const int32_t qual = (int32_t)(int8_t)(w[t >> 2] >> (8 * (t & 3)));
+              const uint32_t b = (x[t >> 4] >> (2 * (t & 15))) & 3u;
+              if (t < cnt && qual >= mq) {
+                unsigned long long* cell = (unsigned long long*)((char*)bsp + __umul24(b, W8)) + t;
+                const unsigned long long v = decltype(nonneg)::value
+                                                 ? ((1ull << 32) | (uint32_t)(qual * mq1))
+                                                 : (unsigned long long)pack_base(qual, mq1);
+                atomicAdd(cell, v);
+                atomicAdd(mqp + t, (unsigned long long)pack_mq_q(mq1, qual));
+              }
             }
-          }
+          };
+          if (mq >= 0) pile(std::integral_constant<bool, true>{});
+          else pile(std::integral_constant<bool, false>{});
           cur = __shfl(rr, 63, 64);
         }
       }
@@ -620,6 +636,22 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
         walk_read(R, J, cf, t0 + l, L0, L1, lane, 64, sk, false);
       }
       __builtin_amdgcn_wave_barrier();
+    }
+    // reads that reach the window only through their fragments: 8 bytes
+    // each, +1/+size at position 0 summed per wave
+    {
+      // read the descriptors again here rather than hold them through the read loop
+      asm volatile("" ::: "memory");
+      const FragTail* __restrict__ ft = R.ftail;
+      const int64_t flo = win.lo, fhi = win.plo;
+      uint32_t c0 = 0, s0 = 0;
+      for (int64_t r = flo + tid; r < fhi; r += BLOCK) {
+        const FragTail t = ft[r];
+        if (frag_tail_event(t, L0, L1, sk)) { c0++; s0 += (uint32_t)t.size; }
+      }
+      c0 = wave_sum(c0);
+      s0 = wave_sum(s0);
+      if (lane == 0 && c0) sk.frag(0, (int32_t)c0, (int32_t)s0);
     }
     for (int64_t k = wave; k < R.n_outlier; k += NW) {
       const int64_t r = uniform64(R.outliers[k]);

@@ -24,14 +24,20 @@ def test_kat_gpu(engine, make):
     case = make()
     print(f"case: {make.__name__}", flush=True)
     cfg = default_config(**case.get("cfg", {}))
+    own = None
     if case.get("cfg"):
         from pilon_amd.engine import Context
-        engine = Context(cfg, device=0)
-    engine.ref_upload(100, "ctg", case["ref"].encode())
-    pr = PileUpRegion(engine, 100, case["start"], case["stop"])
-    pr.addReads(K.batch_of(case))
-    pr.postProcess()
-    changes, bases = pr.fix()
+        engine = own = Context(cfg, device=0)
+    try:
+        engine.ref_upload(100, "ctg", case["ref"].encode())
+        pr = PileUpRegion(engine, 100, case["start"], case["stop"])
+        pr.addReads(K.batch_of(case))
+        pr.postProcess()
+        changes, bases = pr.fix()
+        pr.close()
+    finally:
+        if own is not None:
+            own.close()
     res = {"status": 0, "pile": pr.pile, "calls": pr.calls, "stats": pr.stats, "changes": changes, "bases": bases}
     check_case(res, case)
     ref = O.run_region(cfg, case["ref"], case["start"], case["stop"], [K.batch_of(case)], name="ctg")
