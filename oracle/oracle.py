@@ -111,6 +111,20 @@ def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, of
     return out
 
 
+def run_region_stats(cfg, contig, start, stop, readsets, name="ctg"):
+    """run_region without copying the per-position results out: (status, stats)."""
+    L = lib()
+    contig = contig.encode() if isinstance(contig, str) else bytes(contig)
+    soas = [rs.soa() for rs in readsets]
+    arr = (C.POINTER(PlReadSoa) * max(1, len(soas)))(*[C.pointer(s) for s in soas])
+    rp = L.orc_region_run(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
+                          arr, len(soas), (name + "_pilon").encode(), 0)
+    r = rp.contents
+    status, stats = r.status, (r.stats.as_dict() if r.status == 0 else None)
+    L.orc_result_free(rp)
+    return status, stats
+
+
 def java_round(x):
     return lib().orc_java_round(float(x))
 
