@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=20000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--bam", type=int, default=1,
+                    help="also run the BAM producer: the workload written as a BAM, decoded, uploaded and piled")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_window_c3_traffic.md"))
@@ -273,6 +275,13 @@ def main():
         cpu = cpu_baseline(cfg, data, regions, args.cpu_seconds)
         log(f"bench: CPU oracle {cpu['value'] / 1e6:.1f} M bases/s on {cpu['cores']} threads ({cpu['sample']})")
 
+    producer = None
+    if args.bam:
+        producer = bam_producer(data, ctx, args, W, B, aligned)
+        log(f"bench: BAM producer: {producer['records']} records decoded in {producer['decode_s']:.2f}s on "
+            f"{producer['threads']} threads ({producer['decode_bases_per_s'] / 1e9:.2f} G aligned bases/s), "
+            f"file to pileup {producer['end_to_end_bases_per_s'] / 1e9:.2f} G bases/s")
+
     serving = None
     if args.serve:
         ctx.close()
@@ -294,11 +303,57 @@ def main():
         "cpu_baseline": cpu,
         "p99_latency_ms": serving["p99_ms"] if serving else None,
         "serving": serving,
+        "bam_producer": producer,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
     if not args.serve:
         ctx.close()
+
+
+def bam_producer(data, ctx, args, W, B, aligned):
+    """The BAM -> read batch producer on this workload: its reads written as a
+    coordinate-sorted BAM (BGZF, deflate level 1, one reference per contig),
+    decoded on the host's threads (libpilonbam), uploaded as read sets and
+    piled in one batched pass.  End to end = decode + upload + pile, from a
+    file in the page cache."""
+    import shutil
+    import tempfile
+    from pilon_amd.bam import BamFile
+    nth = max(1, min(16, len(os.sched_getaffinity(0))))
+    d = tempfile.mkdtemp(prefix="pilon_bam_")
+    path = os.path.join(d, f"{args.config}.bam")
+    try:
+        t0 = time.perf_counter()
+        stream = data.write_bam(path, level=1)
+        t_write = time.perf_counter() - t0
+        fsize = os.path.getsize(path)
+        t0 = time.perf_counter()
+        bam = BamFile(path, threads=nth)
+        t_dec = time.perf_counter() - t0
+        st = bam.stats
+        t0 = time.perf_counter()
+        rids = [ctx.readset_upload(ci, bam.reads(ci)) for ci in range(data.n)]
+        dev_sync()
+        t_up = time.perf_counter() - t0
+        regions = regions_for(args.config, data, rids)
+        t0 = time.perf_counter()
+        _, _, stats = ctx.batch_run(regions, window=W, block=B, persistent=1, copy_out=False, want_stats=True)
+        t_pile = time.perf_counter() - t0
+        got = sum(x["aligned_bases"] for x in stats)
+        assert got == aligned, ("aligned bases through the BAM differ", got, aligned)
+        bam.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    e2e = t_dec + t_up + t_pile
+    return {"file_bytes": fsize, "bam_bytes": stream, "records": st["records"], "threads": nth,
+            "write_s": t_write, "decode_s": t_dec, "inflate_s": st["inflate_s"], "parse_s": st["decode_s"],
+            "upload_s": t_up, "pile_s": t_pile,
+            "decode_records_per_s": st["records"] / t_dec, "decode_bases_per_s": aligned / t_dec,
+            "decode_compressed_MB_per_s": fsize / t_dec / 1e6,
+            "end_to_end_bases_per_s": aligned / e2e,
+            "note": "aligned bases per second of the whole path from a BAM in the page cache: host decode, "
+                    "upload, one batched pileup (not a graph replay)"}
 
 
 def serve_workload(cfg, args):

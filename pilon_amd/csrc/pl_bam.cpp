@@ -1,0 +1,566 @@
+// pl_bam.cpp — BAM reader producing the engine's read batches (libpilonbam.so).
+//
+// Host C++ with libdeflate and std::thread.  The reading side of Pilon's BamFile
+// (pilon/BamFile.scala:38-60,108-148) goes through htsjdk; what the pileup
+// path takes from each SAMRecord is restated here from the SAMv1
+// specification (BGZF section 4.1, BAM records section 4.2):
+//
+//   * a BGZF block is a gzip member whose extra field carries BC (BSIZE);
+//     its CDATA is raw deflate, followed by CRC32 and ISIZE of the data;
+//   * the BAM stream is "BAM\1", l_text, text, n_ref, {l_name, name, l_ref}
+//     and then records of block_size + 32 fixed bytes + name, CIGAR, SEQ
+//     (4-bit codes "=ACMGRSVTWYHKDBN", two per byte, first in the high
+//     nibble), QUAL (0xFF when absent) and tags;
+//   * a CIGAR of more than 65535 operations is stored as kSmN in the record
+//     and in full in the CG:B,I tag.
+//
+// Each slice of the file (about 64 MiB of inflated BAM) goes through three
+// steps: the blocks are inflated in parallel, the records are delimited and
+// assigned their output offsets in one sequential pass, and the records are
+// decoded into the per-reference batches in parallel.  Non-ACGT reads get
+// their nibbles copied afterwards, in file order.
+#include "../../include/pilon_bam.h"
+
+#include <libdeflate.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <thread>
+#include <utility>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+pl_status fail(pl_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return s;
+}
+
+// BAM and BGZF integers are little-endian, as is the host
+inline uint16_t rd16(const uint8_t* p) { uint16_t v; std::memcpy(&v, p, 2); return v; }
+inline uint32_t rd32(const uint8_t* p) { uint32_t v; std::memcpy(&v, p, 4); return v; }
+inline int32_t rdi32(const uint8_t* p) { int32_t v; std::memcpy(&v, p, 4); return v; }
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// the end-of-file marker block of the SAMv1 specification (section 4.1.2)
+const uint8_t kEofBlock[28] = {0x1f, 0x8b, 0x08, 0x04, 0x00, 0x00, 0x00, 0x00, 0x00, 0xff, 0x06, 0x00, 0x42, 0x43,
+                               0x02, 0x00, 0x1b, 0x00, 0x03, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00};
+
+constexpr size_t kSliceBytes = (size_t)64 << 20;   // inflated bytes per slice (PILON_BAM_SLICE_KIB overrides)
+constexpr size_t kReadBytes = (size_t)16 << 20;    // compressed bytes per file read
+
+size_t slice_bytes() {
+  const char* e = std::getenv("PILON_BAM_SLICE_KIB");
+  const long v = e ? std::strtol(e, nullptr, 10) : 0;
+  return v > 0 ? (size_t)v << 10 : kSliceBytes;
+}
+
+struct Block {
+  size_t cdata, clen;      // raw deflate data within the compressed buffer
+  uint32_t isize, crc;
+  size_t uoff;             // destination within the slice
+};
+
+// 2-bit code of a BAM base nibble (A=1 C=2 G=4 T=8), -1 for any other code
+const int8_t kNib2[16] = {-1, 0, 1, -1, 2, -1, -1, -1, 3, -1, -1, -1, -1, -1, -1, -1};
+
+// Vectors whose growth leaves the new elements uninitialised: the parallel
+// decode writes every one of them (and touches its pages first).
+template <class T>
+struct NoInit : std::allocator<T> {
+  template <class U> struct rebind { using other = NoInit<U>; };
+  NoInit() = default;
+  template <class U> NoInit(const NoInit<U>&) {}
+  template <class U> void construct(U* p) { ::new ((void*)p) U; }
+  template <class U, class... A> void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+};
+template <class T> using Buf = std::vector<T, NoInit<T>>;
+
+struct RefReads {
+  Buf<int32_t> pos, tlen;
+  Buf<uint16_t> flag;
+  Buf<uint8_t> mapq;
+  Buf<int64_t> cigar_off{0}, seq_off{0};
+  Buf<uint32_t> cigar;
+  Buf<uint8_t> bases2;
+  Buf<int8_t> quals;
+  std::vector<int64_t> exo_read, exo_off;
+  std::vector<uint8_t> nib;
+  int64_t n = 0;
+  int32_t last_pos = INT32_MIN;
+};
+
+struct Rec {                // one placed record of the current slice
+  size_t u;                 // record start (its block_size field) in the slice
+  int32_t ref;
+  int64_t i;                // index within the reference's batch
+  int64_t coff, soff;       // CIGAR word and base offsets within the batch
+  uint32_t ncig, lseq;
+  size_t cig;               // CIGAR words in the slice (record body or CG tag)
+};
+
+template <class F>
+void parallel_for(int nth, int64_t n, const F& f) {
+  if (nth <= 1 || n < 2) { f(0, (int64_t)0, n); return; }
+  const int t_n = (int)std::min<int64_t>(nth, n);
+  std::vector<std::thread> th;
+  th.reserve((size_t)t_n);
+  for (int t = 0; t < t_n; t++) th.emplace_back([&f, t, t_n, n] { f(t, n * t / t_n, n * (t + 1) / t_n); });
+  for (auto& x : th) x.join();
+}
+
+// The CG:B,I tag of a record's tag area [p, e): word count and first word
+// offset, or false.  Tag value sizes follow SAMv1 section 4.2.4.
+bool find_cg(const uint8_t* base, size_t p, size_t e, uint32_t* count, size_t* words) {
+  while (p + 3 <= e) {
+    const uint8_t t0 = base[p], t1 = base[p + 1], ty = base[p + 2];
+    p += 3;
+    size_t sz = 0;
+    switch (ty) {
+      case 'A': case 'c': case 'C': sz = 1; break;
+      case 's': case 'S': sz = 2; break;
+      case 'i': case 'I': case 'f': sz = 4; break;
+      case 'Z': case 'H': {
+        while (p < e && base[p]) p++;
+        if (p >= e) return false;
+        p++;
+        continue;
+      }
+      case 'B': {
+        if (p + 5 > e) return false;
+        const uint8_t sub = base[p];
+        const uint32_t cnt = rd32(base + p + 1);
+        size_t es = 0;
+        switch (sub) {
+          case 'c': case 'C': es = 1; break;
+          case 's': case 'S': es = 2; break;
+          case 'i': case 'I': case 'f': es = 4; break;
+          default: return false;
+        }
+        if (t0 == 'C' && t1 == 'G' && (sub == 'I' || sub == 'i')) {
+          if (p + 5 + (size_t)cnt * 4 > e) return false;
+          *count = cnt;
+          *words = p + 5;
+          return true;
+        }
+        sz = 5 + (size_t)cnt * es;
+        break;
+      }
+      default: return false;
+    }
+    p += sz;
+  }
+  return false;
+}
+
+}  // namespace
+
+struct pl_bam {
+  std::vector<std::string> names;
+  std::vector<int64_t> lens;
+  std::vector<RefReads> refs;
+  pl_bam_stats st;
+};
+
+namespace {
+
+// Header at the start of the inflated stream: bytes consumed, 0 if the
+// stream does not hold all of it yet, -1 if it is not a BAM header.
+int64_t parse_header(const uint8_t* u, size_t n, pl_bam* B) {
+  if (n < 12) return 0;
+  if (std::memcmp(u, "BAM\1", 4) != 0) return -1;
+  const int32_t l_text = rdi32(u + 4);
+  if (l_text < 0) return -1;
+  size_t p = 8 + (size_t)l_text;
+  if (p + 4 > n) return 0;
+  const int32_t n_ref = rdi32(u + p);
+  if (n_ref < 0) return -1;
+  p += 4;
+  std::vector<std::string> names;
+  std::vector<int64_t> lens;
+  for (int32_t r = 0; r < n_ref; r++) {
+    if (p + 4 > n) return 0;
+    const int32_t l_name = rdi32(u + p);
+    if (l_name < 1) return -1;
+    if (p + 4 + (size_t)l_name + 4 > n) return 0;
+    names.emplace_back((const char*)u + p + 4, strnlen((const char*)u + p + 4, (size_t)l_name));
+    lens.push_back(rdi32(u + p + 4 + l_name));
+    p += 8 + (size_t)l_name;
+  }
+  B->names = std::move(names);
+  B->lens = std::move(lens);
+  B->refs.resize((size_t)n_ref);
+  return (int64_t)p;
+}
+
+// Inflate the slice's blocks into ub (after `carry` bytes already there).
+pl_status inflate_blocks(const std::vector<uint8_t>& cb, std::vector<Block>& blocks, std::vector<uint8_t>& ub,
+                         size_t carry, int nth) {
+  std::atomic<int64_t> bad{-1};
+  parallel_for(nth, (int64_t)blocks.size(), [&](int, int64_t lo, int64_t hi) {
+    libdeflate_decompressor* dz = libdeflate_alloc_decompressor();
+    if (!dz) { bad = lo; return; }
+    for (int64_t k = lo; k < hi; k++) {
+      const Block& b = blocks[(size_t)k];
+      uint8_t none = 0;
+      uint8_t* dst = b.isize ? ub.data() + carry + b.uoff : &none;   // an empty block (the EOF marker) still inflates
+      size_t got = 0;
+      const libdeflate_result rc = libdeflate_deflate_decompress(dz, cb.data() + b.cdata, b.clen, dst,
+                                                                 b.isize ? b.isize : 1, &got);
+      if (rc != LIBDEFLATE_SUCCESS || got != b.isize || libdeflate_crc32(0, dst, b.isize) != b.crc) {
+        int64_t e = -1;
+        bad.compare_exchange_strong(e, k);
+      }
+    }
+    libdeflate_free_decompressor(dz);
+  });
+  if (bad >= 0) return fail(PL_E_INVAL, "corrupt BGZF block (inflate, ISIZE or CRC32 mismatch)");
+  return PL_OK;
+}
+
+// Delimit the records of ub[u0, n) and decode every placed one into its
+// reference's batch; returns the offset of the first incomplete record.
+pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, size_t n, int nth, size_t* used) {
+  const uint8_t* u = ub.data();
+  const size_t nref = B->refs.size();
+  // record starts: the block_size chain
+  std::vector<size_t> offs;
+  size_t p = u0;
+  while (p + 4 <= n) {
+    const uint32_t bs = rd32(u + p);
+    if (bs < 32) return fail(PL_E_INVAL, "BAM record %lld: block_size %u below the fixed 32 bytes",
+                             (long long)(B->st.records + (int64_t)offs.size()), bs);
+    if (p + 4 + (size_t)bs > n) break;
+    offs.push_back(p);
+    p += 4 + (size_t)bs;
+  }
+  *used = p;
+  // per-record facts, in parallel: lengths checked, the CG tag resolved
+  const int64_t N = (int64_t)offs.size();
+  std::vector<Rec> all((size_t)N);
+  std::atomic<int64_t> bad{INT64_MAX};
+  parallel_for(nth, N, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; k++) {
+      const size_t q = offs[(size_t)k];
+      const uint8_t* r = u + q;
+      const uint32_t bs = rd32(r);
+      const uint32_t l_name = r[12];
+      uint32_t ncig = rd16(r + 16);
+      const uint32_t lseq = rd32(r + 20);
+      const size_t body = 36 + (size_t)l_name + 4 * (size_t)ncig;
+      if (lseq > bs || body + ((size_t)lseq + 1) / 2 + lseq > 4 + (size_t)bs) {
+        int64_t e = bad.load();
+        while (k < e && !bad.compare_exchange_weak(e, k)) {}
+        continue;
+      }
+      Rec& rc = all[(size_t)k];
+      rc.u = q;
+      rc.ref = rdi32(r + 4);
+      rc.i = rdi32(r + 8);            // pos (0-based) until the offsets are assigned
+      size_t cig = q + 36 + l_name;
+      // kSmN placeholder: the real CIGAR is in the CG:B,I tag
+      if (ncig == 2 && rd32(u + cig) == ((lseq << 4) | 4u) && (rd32(u + cig + 4) & 15u) == 3u) {
+        uint32_t cnt = 0;
+        size_t words = 0;
+        if (find_cg(u, q + body + ((size_t)lseq + 1) / 2 + lseq, q + 4 + bs, &cnt, &words)) { ncig = cnt; cig = words; }
+      }
+      rc.ncig = ncig;
+      rc.lseq = lseq;
+      rc.cig = cig;
+    }
+  });
+  if (bad.load() != INT64_MAX)
+    return fail(PL_E_INVAL, "BAM record %lld: name, CIGAR, SEQ and QUAL exceed block_size",
+                (long long)(B->st.records + bad.load()));
+  // output offsets per reference, in file order; coordinate order checked
+  std::vector<Rec> recs;
+  recs.reserve((size_t)N);
+  std::vector<int64_t> add_n(nref, 0), add_c(nref, 0), add_s(nref, 0);
+  std::vector<int32_t> touched;
+  for (int64_t k = 0; k < N; k++) {
+    Rec rc = all[(size_t)k];
+    const int32_t ref = rc.ref;
+    if (ref >= (int32_t)nref || ref < -1)
+      return fail(PL_E_INVAL, "BAM record %lld: refID %d outside the dictionary", (long long)(B->st.records + k), ref);
+    if (ref < 0) continue;
+    RefReads& R = B->refs[(size_t)ref];
+    const int32_t pos1 = (int32_t)rc.i + 1;
+    if (pos1 < R.last_pos)
+      return fail(PL_E_INVAL, "BAM is not coordinate-sorted: %s pos %d after %d", B->names[(size_t)ref].c_str(),
+                  pos1, R.last_pos);
+    R.last_pos = pos1;
+    if (add_n[(size_t)ref] == 0) touched.push_back(ref);
+    rc.i = R.n + add_n[(size_t)ref]++;
+    rc.coff = R.cigar_off.back() + add_c[(size_t)ref];
+    rc.soff = R.seq_off.back() + add_s[(size_t)ref];
+    add_c[(size_t)ref] += rc.ncig;
+    add_s[(size_t)ref] += rc.lseq;
+    recs.push_back(rc);
+  }
+  B->st.records += N;
+  // grow the batches of the references this slice touches
+  std::vector<size_t> old_b2(nref, 0);
+  for (int32_t ref : touched) {
+    RefReads& R = B->refs[(size_t)ref];
+    old_b2[(size_t)ref] = R.bases2.size();
+    const size_t nn = (size_t)(R.n + add_n[(size_t)ref]);
+    const int64_t cn = R.cigar_off.back() + add_c[(size_t)ref];
+    const int64_t sn = R.seq_off.back() + add_s[(size_t)ref];
+    R.pos.resize(nn); R.tlen.resize(nn); R.flag.resize(nn); R.mapq.resize(nn);
+    R.cigar_off.resize(nn + 1); R.seq_off.resize(nn + 1);
+    R.cigar.resize((size_t)cn);
+    R.quals.resize((size_t)sn);
+    R.bases2.resize((size_t)(sn + 3) / 4);
+  }
+  // bytes of bases2 that two reads share are OR-ed by both: clear the first
+  // and last byte of every read's range that this slice added (the bytes
+  // wholly inside a read are stored whole; the byte a read of an earlier
+  // slice ended in is already set up)
+  for (const Rec& rc : recs) {
+    if (!rc.lseq) continue;
+    RefReads& R = B->refs[(size_t)rc.ref];
+    const size_t b0 = (size_t)(rc.soff >> 2), b1 = (size_t)((rc.soff + rc.lseq - 1) >> 2);
+    if (b0 >= old_b2[(size_t)rc.ref]) R.bases2[b0] = 0;
+    if (b1 >= old_b2[(size_t)rc.ref]) R.bases2[b1] = 0;
+  }
+  // decode in parallel; bytes of bases2 shared by two reads are OR-ed atomically
+  std::vector<uint8_t> exotic(recs.size(), 0);
+  parallel_for(nth, (int64_t)recs.size(), [&](int, int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; k++) {
+      const Rec& rc = recs[(size_t)k];
+      RefReads& R = B->refs[(size_t)rc.ref];
+      const uint8_t* r = u + rc.u;
+      const uint32_t l_name = r[12];
+      const uint16_t flag = rd16(r + 18);
+      const int32_t next_ref = rdi32(r + 24);
+      const size_t i = (size_t)rc.i;
+      R.pos[i] = rdi32(r + 8) + 1;
+      R.tlen[i] = rdi32(r + 32);
+      R.mapq[i] = r[13];
+      R.cigar_off[i + 1] = rc.coff + rc.ncig;
+      R.seq_off[i + 1] = rc.soff + rc.lseq;
+      std::memcpy(R.cigar.data() + rc.coff, u + rc.cig, 4 * (size_t)rc.ncig);
+      const uint8_t* seq = r + 36 + l_name + 4 * (size_t)rd16(r + 16);
+      const uint8_t* qual = seq + ((size_t)rc.lseq + 1) / 2;
+      const bool noqual = rc.lseq == 0 || qual[0] == 0xFF;
+      std::memcpy(R.quals.data() + rc.soff, qual, rc.lseq);
+      bool exo = false;
+      for (uint32_t b = 0; b < rc.lseq && !exo; b++) exo = kNib2[(seq[b >> 1] >> (4 * (1 - (b & 1)))) & 15] < 0;
+      if (!exo && rc.lseq) {
+        // 2-bit codes, four to a byte; the first and last byte may hold
+        // bases of the neighbouring reads
+        const int64_t s0 = rc.soff, s1 = rc.soff + rc.lseq;
+        uint8_t* out = R.bases2.data();
+        int64_t g = s0;
+        uint32_t acc = 0;
+        int nacc = 0;
+        for (uint32_t b = 0; b < rc.lseq; b++, g++) {
+          const int c = kNib2[(seq[b >> 1] >> (4 * (1 - (b & 1)))) & 15];
+          acc |= (uint32_t)c << (2 * (g & 3));
+          nacc++;
+          if ((g & 3) == 3 || g + 1 == s1) {
+            const int64_t byte = g >> 2;
+            const bool whole = (byte << 2) >= s0 && (byte << 2) + 4 <= s1;
+            if (whole) out[byte] = (uint8_t)acc;
+            else __atomic_fetch_or(&out[byte], (uint8_t)acc, __ATOMIC_RELAXED);
+            acc = 0;
+            nacc = 0;
+          }
+        }
+        (void)nacc;
+      }
+      exotic[(size_t)k] = exo ? 1 : 0;
+      R.flag[i] = (uint16_t)((flag & 0x0FFFu) | (noqual ? PL_RF_NO_QUAL : 0u) |
+                             (next_ref == rc.ref ? PL_RF_MATE_SAME_REF : 0u) | (exo ? PL_RF_EXOTIC : 0u));
+    }
+  });
+  // nibbles of the non-ACGT reads, in file order, each from an even nibble
+  for (size_t k = 0; k < recs.size(); k++) {
+    if (!exotic[k]) continue;
+    const Rec& rc = recs[k];
+    RefReads& R = B->refs[(size_t)rc.ref];
+    const uint8_t* r = u + rc.u;
+    const uint8_t* seq = r + 36 + r[12] + 4 * (size_t)rd16(r + 16);
+    R.exo_read.push_back(rc.i);
+    R.exo_off.push_back((int64_t)R.nib.size() * 2);
+    R.nib.insert(R.nib.end(), seq, seq + ((size_t)rc.lseq + 1) / 2);
+    if (rc.lseq & 1) R.nib.back() &= 0xF0;
+    B->st.exotic++;
+  }
+  for (int32_t ref : touched) B->refs[(size_t)ref].n += add_n[(size_t)ref];
+  B->st.placed += (int64_t)recs.size();
+  return PL_OK;
+}
+
+pl_status open_impl(const char* path, int nth, pl_bam* B) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(PL_E_INVAL, "cannot open %s", path);
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(f, std::fclose);
+  std::vector<uint8_t> cb;       // compressed bytes not yet inflated
+  std::vector<uint8_t> ub;       // inflated bytes not yet decoded
+  std::vector<uint8_t> last_block;
+  size_t carry = 0;
+  const size_t slice = slice_bytes();
+  bool header = false, file_eof = false;
+  for (;;) {
+    // top up the compressed buffer
+    if (!file_eof && cb.size() < kReadBytes) {
+      const size_t old = cb.size();
+      cb.resize(old + kReadBytes);
+      const size_t got = std::fread(cb.data() + old, 1, kReadBytes, f);
+      cb.resize(old + got);
+      B->st.compressed_bytes += (int64_t)got;
+      if (got == 0) file_eof = true;
+    }
+    // delimit whole blocks, up to one slice of inflated bytes
+    std::vector<Block> blocks;
+    size_t p = 0, utot = 0, last_at = 0;
+    while (p + 18 <= cb.size() && utot < slice) {
+      const uint8_t* h = cb.data() + p;
+      if (h[0] != 0x1f || h[1] != 0x8b || h[2] != 8 || !(h[3] & 4))
+        return fail(PL_E_INVAL, "not a BGZF block at byte %lld", (long long)(B->st.compressed_bytes - cb.size() + p));
+      const size_t xlen = rd16(h + 10);
+      if (p + 12 + xlen > cb.size()) break;
+      size_t bsize = 0;
+      for (size_t x = 12; x + 4 <= 12 + xlen;) {
+        const size_t slen = rd16(h + x + 2);
+        if (h[x] == 66 && h[x + 1] == 67 && slen == 2) bsize = (size_t)rd16(h + x + 4) + 1;
+        x += 4 + slen;
+      }
+      if (!bsize || bsize < 20 + xlen) return fail(PL_E_INVAL, "BGZF block without a valid BC (BSIZE) subfield");
+      if (p + bsize > cb.size()) break;
+      Block b;
+      b.cdata = p + 12 + xlen;
+      b.clen = bsize - xlen - 20;
+      b.crc = rd32(h + bsize - 8);
+      b.isize = rd32(h + bsize - 4);
+      b.uoff = utot;
+      if (b.isize > 65536) return fail(PL_E_INVAL, "BGZF block ISIZE %u above 64 KiB", b.isize);
+      utot += b.isize;
+      blocks.push_back(b);
+      last_at = p;
+      p += bsize;
+    }
+    if (!blocks.empty()) last_block.assign(cb.data() + last_at, cb.data() + p);
+    if (blocks.empty()) {
+      if (file_eof) {
+        if (!cb.empty()) return fail(PL_E_INVAL, "truncated BGZF block at the end of the file");
+        break;
+      }
+      continue;
+    }
+    B->st.blocks += (int64_t)blocks.size();
+    ub.resize(carry + utot);
+    double t0 = now_s();
+    pl_status s = inflate_blocks(cb, blocks, ub, carry, nth);
+    B->st.inflate_s += now_s() - t0;
+    if (s) return s;
+    B->st.bam_bytes += (int64_t)utot;
+    cb.erase(cb.begin(), cb.begin() + (std::ptrdiff_t)p);
+    size_t un = carry + utot, u0 = 0;
+    if (!header) {
+      const int64_t h = parse_header(ub.data(), un, B);
+      if (h < 0) return fail(PL_E_INVAL, "not a BAM file (bad magic or header)");
+      if (h == 0) { carry = un; continue; }
+      header = true;
+      u0 = (size_t)h;
+    }
+    t0 = now_s();
+    size_t used = u0;
+    s = decode_records(B, ub, u0, un, nth, &used);
+    B->st.decode_s += now_s() - t0;
+    if (s) return s;
+    carry = un - used;
+    if (carry) std::memmove(ub.data(), ub.data() + used, carry);
+  }
+  if (!header) return fail(PL_E_INVAL, "not a BAM file (no complete header)");
+  if (carry) return fail(PL_E_INVAL, "truncated BAM record at the end of the file");
+  B->st.eof_marker = last_block.size() == sizeof kEofBlock &&
+                     std::memcmp(last_block.data(), kEofBlock, sizeof kEofBlock) == 0;
+  return PL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pl_status pl_bam_open(const char* path, int32_t n_threads, pl_bam** out) {
+  if (!path || !out) return fail(PL_E_INVAL, "null argument");
+  *out = nullptr;
+  int nth = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
+  nth = std::max(1, std::min(nth, 64));
+  std::unique_ptr<pl_bam> B(new (std::nothrow) pl_bam());
+  if (!B) return fail(PL_E_OOM, "host allocation failed");
+  std::memset(&B->st, 0, sizeof B->st);
+  B->st.threads = nth;
+  try {
+    pl_status s = open_impl(path, nth, B.get());
+    if (s) return s;
+  } catch (const std::bad_alloc&) {
+    return fail(PL_E_OOM, "host allocation failed while decoding %s", path);
+  }
+  *out = B.release();
+  return PL_OK;
+}
+
+int32_t pl_bam_n_refs(const pl_bam* bam) { return bam ? (int32_t)bam->names.size() : 0; }
+
+const char* pl_bam_ref_name(const pl_bam* bam, int32_t ref) {
+  if (!bam || ref < 0 || ref >= (int32_t)bam->names.size()) return nullptr;
+  return bam->names[(size_t)ref].c_str();
+}
+
+int64_t pl_bam_ref_len(const pl_bam* bam, int32_t ref) {
+  if (!bam || ref < 0 || ref >= (int32_t)bam->lens.size()) return -1;
+  return bam->lens[(size_t)ref];
+}
+
+pl_status pl_bam_ref_reads(const pl_bam* bam, int32_t ref, pl_read_soa* out) {
+  if (!bam || !out) return fail(PL_E_INVAL, "null argument");
+  if (ref < 0 || ref >= (int32_t)bam->refs.size()) return fail(PL_E_INVAL, "refID %d outside the dictionary", ref);
+  const RefReads& R = bam->refs[(size_t)ref];
+  std::memset(out, 0, sizeof *out);
+  out->n_reads = R.n;
+  out->pos = R.pos.data();
+  out->tlen = R.tlen.data();
+  out->flag = R.flag.data();
+  out->mapq = R.mapq.data();
+  out->cigar_off = R.cigar_off.data();
+  out->cigar = R.cigar.data();
+  out->seq_off = R.seq_off.data();
+  out->bases2 = R.bases2.data();
+  out->quals = R.quals.data();
+  out->n_exotic = (int64_t)R.exo_read.size();
+  out->exo_read = R.exo_read.data();
+  out->exo_off = R.exo_off.data();
+  out->nib = R.nib.data();
+  return PL_OK;
+}
+
+void pl_bam_get_stats(const pl_bam* bam, pl_bam_stats* out) {
+  if (bam && out) *out = bam->st;
+}
+
+void pl_bam_close(pl_bam* bam) { delete bam; }
+
+const char* pl_bam_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

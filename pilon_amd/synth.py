@@ -39,7 +39,7 @@ class SynthParams(C.Structure):
 
 def build():
     subprocess.check_call(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall",
-                           "-o", LIB, SRC])
+                           "-o", LIB, SRC, "-lz"])
 
 
 _lib = None
@@ -61,6 +61,8 @@ def lib():
         L.synth_variants.restype = C.c_int64
         L.synth_variants.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.synth_reads.argtypes = [C.c_void_p, C.c_int32, C.POINTER(PlReadSoa)]
+        L.synth_write_bam.restype = C.c_int64
+        L.synth_write_bam.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int32]
         _lib = L
     return _lib
 
@@ -163,6 +165,15 @@ class SynthData:
         s = PlReadSoa()
         lib().synth_reads(self.h, i, C.byref(s))
         return s
+
+    def write_bam(self, path, level=1):
+        """The workload as a coordinate-sorted BAM (one reference per contig);
+        returns the size of the inflated BAM stream."""
+        names = (C.c_char_p * self.n)(*[n.encode() for n in self.names])
+        n = lib().synth_write_bam(self.h, os.fsencode(path), names, level)
+        if n < 0:
+            raise OSError(f"writing {path} failed")
+        return n
 
     def n_reads(self, i):
         return self.soa(i).n_reads

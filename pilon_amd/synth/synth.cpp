@@ -17,6 +17,9 @@
 #include <numeric>
 #include <vector>
 #include <omp.h>
+#include <zlib.h>
+#include <cstdio>
+#include <string>
 
 #include "../../include/pilon_hip.h"
 
@@ -545,3 +548,134 @@ void synth_reads(void* h, int32_t ci, pl_read_soa* s) {
 }
 
 }  // extern "C"
+
+// ---- the workload as a BAM file (SAMv1 sections 4.1-4.2): one record per
+// read, named r<index>, the mate on the same reference iff
+// PL_RF_MATE_SAME_REF, the SAM flag bits only; BGZF blocks of 0xFF00 bytes,
+// raw deflate at `level`, and the end-of-file marker.  Returns the size of
+// the inflated BAM stream, or -1.
+namespace {
+void put32(std::vector<uint8_t>& v, uint32_t x) { for (int k = 0; k < 4; k++) v.push_back((uint8_t)(x >> (8 * k))); }
+void put16(std::vector<uint8_t>& v, uint16_t x) { v.push_back((uint8_t)x); v.push_back((uint8_t)(x >> 8)); }
+uint32_t reg2bin(int64_t beg, int64_t end) {
+  --end;
+  if (beg >> 14 == end >> 14) return (uint32_t)(((1 << 15) - 1) / 7 + (beg >> 14));
+  if (beg >> 17 == end >> 17) return (uint32_t)(((1 << 12) - 1) / 7 + (beg >> 17));
+  if (beg >> 20 == end >> 20) return (uint32_t)(((1 << 9) - 1) / 7 + (beg >> 20));
+  if (beg >> 23 == end >> 23) return (uint32_t)(((1 << 6) - 1) / 7 + (beg >> 23));
+  if (beg >> 26 == end >> 26) return (uint32_t)(((1 << 3) - 1) / 7 + (beg >> 26));
+  return 0;
+}
+void contig_records(const ContigOut& c, int32_t ref, std::vector<uint8_t>& out) {
+  static const uint8_t kNib[4] = {1, 2, 4, 8};
+  const size_t n = c.pos.size();
+  size_t e = 0;
+  for (size_t i = 0; i < n; i++) {
+    const int64_t s0 = c.seq_off[i], L = c.seq_off[i + 1] - s0;
+    const int64_t c0 = c.cigar_off[i], nc = c.cigar_off[i + 1] - c0;
+    const std::string name = "r" + std::to_string(i);
+    int64_t rlen = 0;
+    for (int64_t k = 0; k < nc; k++) {
+      const uint32_t w = c.cigar[(size_t)(c0 + k)], op = w & 15u;
+      if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rlen += w >> 4;
+    }
+    const int32_t pos0 = c.pos[i] - 1;
+    const int64_t b0 = pos0 > 0 ? pos0 : 0;
+    const uint16_t f = c.flag[i];
+    put32(out, (uint32_t)(32 + name.size() + 1 + 4 * nc + (L + 1) / 2 + L));
+    put32(out, (uint32_t)ref);
+    put32(out, (uint32_t)pos0);
+    out.push_back((uint8_t)(name.size() + 1));
+    out.push_back(c.mapq[i]);
+    put16(out, (uint16_t)reg2bin(b0, b0 + (rlen > 1 ? rlen : 1)));
+    put16(out, (uint16_t)nc);
+    put16(out, (uint16_t)(f & 0x0FFFu));
+    put32(out, (uint32_t)L);
+    put32(out, (uint32_t)((f & PL_RF_MATE_SAME_REF) ? ref : -1));
+    put32(out, (uint32_t)-1);
+    put32(out, (uint32_t)c.tlen[i]);
+    out.insert(out.end(), name.begin(), name.end());
+    out.push_back(0);
+    for (int64_t k = 0; k < nc; k++) put32(out, c.cigar[(size_t)(c0 + k)]);
+    const bool exo = e < c.exo_read.size() && c.exo_read[e] == (int64_t)i;
+    for (int64_t k = 0; k < L; k += 2) {
+      uint8_t hi, lo = 0;
+      if (exo) {
+        const int64_t o = c.exo_off[e] + k;
+        hi = (uint8_t)((c.nib[(size_t)(o >> 1)] >> (4 * (1 - (o & 1)))) & 15);
+        if (k + 1 < L) lo = (uint8_t)((c.nib[(size_t)((o + 1) >> 1)] >> (4 * (1 - ((o + 1) & 1)))) & 15);
+      } else {
+        const int64_t q = s0 + k;
+        hi = kNib[(c.bases2[(size_t)(q >> 2)] >> (2 * (q & 3))) & 3];
+        if (k + 1 < L) lo = kNib[(c.bases2[(size_t)((q + 1) >> 2)] >> (2 * ((q + 1) & 3))) & 3];
+      }
+      out.push_back((uint8_t)((hi << 4) | lo));
+    }
+    if (exo) e++;
+    if (f & PL_RF_NO_QUAL) out.insert(out.end(), (size_t)L, (uint8_t)0xFF);
+    else out.insert(out.end(), (const uint8_t*)c.quals.data() + s0, (const uint8_t*)c.quals.data() + s0 + L);
+  }
+}
+bool deflate_block(const uint8_t* src, size_t n, int level, std::vector<uint8_t>& out) {
+  z_stream zs;
+  std::memset(&zs, 0, sizeof zs);
+  if (deflateInit2(&zs, level, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) return false;
+  std::vector<uint8_t> cd(deflateBound(&zs, (uLong)n) + 16);
+  zs.next_in = const_cast<Bytef*>(src);
+  zs.avail_in = (uInt)n;
+  zs.next_out = cd.data();
+  zs.avail_out = (uInt)cd.size();
+  const int rc = deflate(&zs, Z_FINISH);
+  const size_t clen = zs.total_out;
+  deflateEnd(&zs);
+  if (rc != Z_STREAM_END) return false;
+  const size_t bsize = 18 + clen + 8;
+  const uint8_t head[18] = {31, 139, 8, 4, 0, 0, 0, 0, 0, 255, 6, 0, 66, 67, 2, 0,
+                            (uint8_t)((bsize - 1) & 255), (uint8_t)((bsize - 1) >> 8)};
+  out.assign(head, head + 18);
+  out.insert(out.end(), cd.begin(), cd.begin() + (std::ptrdiff_t)clen);
+  put32(out, (uint32_t)crc32(0L, src, (uInt)n));
+  put32(out, (uint32_t)n);
+  return true;
+}
+}  // namespace
+
+extern "C" int64_t synth_write_bam(void* h, const char* path, const char* const* names, int32_t level) {
+  SynthResult& S = *(SynthResult*)h;
+  const int32_t nc = (int32_t)S.contigs.size();
+  std::vector<uint8_t> stream;
+  const std::string text = "@HD\tVN:1.6\tSO:coordinate\n";
+  stream.insert(stream.end(), {'B', 'A', 'M', 1});
+  put32(stream, (uint32_t)text.size());
+  stream.insert(stream.end(), text.begin(), text.end());
+  put32(stream, (uint32_t)nc);
+  for (int32_t ci = 0; ci < nc; ci++) {
+    const size_t ln = std::strlen(names[ci]);
+    put32(stream, (uint32_t)(ln + 1));
+    stream.insert(stream.end(), names[ci], names[ci] + ln);
+    stream.push_back(0);
+    put32(stream, (uint32_t)S.contigs[(size_t)ci].draft.size());
+  }
+  std::vector<std::vector<uint8_t>> rec((size_t)nc);
+#pragma omp parallel for schedule(dynamic)
+  for (int32_t ci = 0; ci < nc; ci++) contig_records(S.contigs[(size_t)ci], ci, rec[(size_t)ci]);
+  size_t total = stream.size();
+  for (auto& r : rec) total += r.size();
+  stream.reserve(total);
+  for (auto& r : rec) { stream.insert(stream.end(), r.begin(), r.end()); std::vector<uint8_t>().swap(r); }
+  const size_t B = 0xFF00, nb = (stream.size() + B - 1) / B;
+  std::vector<std::vector<uint8_t>> blk(nb + 1);
+  int ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int64_t k = 0; k < (int64_t)nb; k++) {
+    const size_t o = (size_t)k * B;
+    ok = deflate_block(stream.data() + o, std::min(B, stream.size() - o), level, blk[(size_t)k]) && ok;
+  }
+  if (!ok || !deflate_block(nullptr, 0, level, blk[nb])) return -1;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return -1;
+  for (auto& b : blk) if (std::fwrite(b.data(), 1, b.size(), f) != b.size()) { std::fclose(f); return -1; }
+  std::fclose(f);
+  return (int64_t)stream.size();
+}
+

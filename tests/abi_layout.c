@@ -2,6 +2,7 @@
 #include <stddef.h>
 #include <stdio.h>
 #include "../include/pilon_hip.h"
+#include "../include/pilon_bam.h"
 #define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f))
 int main(void) {
   printf("pl_config %zu\npl_read_soa %zu\npl_call %zu\npl_region_stats %zu\npl_region_desc %zu\npl_batch_opts %zu\npl_batch_info %zu\n",
@@ -12,5 +13,7 @@ int main(void) {
   printf("pl_serve_opts %zu\npl_serve_result %zu\n", sizeof(pl_serve_opts), sizeof(pl_serve_result));
   P(pl_serve_opts, mem_budget); P(pl_serve_opts, copy_out); P(pl_serve_result, stats);
   P(pl_serve_result, est_bytes); P(pl_serve_result, batch_device_ms);
+  printf("pl_bam_stats %zu\n", sizeof(pl_bam_stats));
+  P(pl_bam_stats, inflate_s); P(pl_bam_stats, threads); P(pl_bam_stats, eof_marker);
   return 0;
 }

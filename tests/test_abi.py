@@ -8,14 +8,15 @@ import subprocess
 import pytest
 
 from pilon_amd import abi
+from pilon_amd.bam import PlBamStats
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pilon_hip.h")
 LIB = os.path.join(ROOT, "pilon_amd", "libpilonhip.so")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(pl_[a-z_]+)\s*\(", text)))
 
@@ -31,6 +32,17 @@ def test_library_exports_every_declared_symbol():
     assert os.path.exists(LIB), "build first (__graft_entry__.build())"
     lib = C.CDLL(LIB)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bam_library_exports_every_declared_symbol():
+    """libpilonbam.so (the BAM reader) and include/pilon_bam.h."""
+    lib = os.path.join(ROOT, "pilon_amd", "libpilonbam.so")
+    assert os.path.exists(lib), "build first (__graft_entry__.build())"
+    names = declared_functions(os.path.join(ROOT, "include", "pilon_bam.h"))
+    assert "pl_bam_open" in names and "pl_bam_ref_reads" in names
+    L = C.CDLL(lib)
+    missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
 
@@ -61,7 +73,8 @@ def test_struct_layouts_match_c(tmp_path):
     sizes = {"pl_config": abi.PlConfig, "pl_read_soa": abi.PlReadSoa, "pl_call": abi.PlCall,
              "pl_region_stats": abi.PlRegionStats, "pl_region_desc": abi.PlRegionDesc,
              "pl_batch_opts": abi.PlBatchOpts, "pl_batch_info": abi.PlBatchInfo,
-             "pl_serve_opts": abi.PlServeOpts, "pl_serve_result": abi.PlServeResult}
+             "pl_serve_opts": abi.PlServeOpts, "pl_serve_result": abi.PlServeResult,
+             "pl_bam_stats": PlBamStats}
     for name, cls in sizes.items():
         assert int(out[name]) == C.sizeof(cls), name
     for key, val in out.items():

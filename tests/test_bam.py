@@ -1,0 +1,167 @@
+"""The BAM reader (libpilonbam.so) against records encoded literally from
+the SAMv1 specification (tests/bamspec.py): the decode rules the pileup path
+depends on, BGZF framing and error cases, and round trips of synthetic
+workloads that must give the oracle the same regions as the original
+batches."""
+import struct
+
+import numpy as np
+import pytest
+
+from pilon_amd.abi import PL_RF_EXOTIC, PL_RF_MATE_SAME_REF, PL_RF_NO_QUAL, default_config
+from pilon_amd.bam import BamError, BamFile
+from pilon_amd.reads import unpack_2bit
+from tests import bamspec as S
+
+
+def w(n, op):
+    return (n << 4) | "MIDNSHP=X".index(op)
+
+
+def known_records():
+    cg = b"CGBI" + struct.pack("<I", 3) + struct.pack("<3I", w(4, "M"), w(2, "D"), w(6, "M"))
+    recs = [
+        # ref 0: a proper pair mate (mate on the same reference)
+        S.record(0, 9, [w(5, "M")], "ACGTA", [30, 31, 32, 33, 34], flag=0x63, next_ref=0, tlen=200, name="a"),
+        # non-ACGT codes, QUAL absent, mate on another reference, a flag bit above 0x800
+        S.record(0, 9, [w(3, "M"), w(1, "I"), w(2, "M")], "ACNGTM", None, flag=0x1891, next_ref=1, tlen=-50,
+                 mapq=7, name="b"),
+        # placed but unmapped, SEQ "*"
+        S.record(0, 20, [], "", None, flag=0x4, next_ref=-1, name="c"),
+        # a CIGAR carried by the CG:B,I tag (kSmN placeholder in the record)
+        S.record(0, 30, [w(10, "S"), w(10, "N")], "ACGTACGTAC", [20] * 10, flag=0, name="d",
+                 tags=b"NMi" + struct.pack("<i", 1) + cg + b"XZZhello\0"),
+        # ref 1
+        S.record(1, 0, [w(2, "S"), w(3, "M")], "GGACG", [40] * 5, flag=0x10, name="e"),
+        # unplaced: never handed over
+        S.record(-1, -1, [], "ACGT", [10] * 4, flag=0x4, name="f"),
+    ]
+    return S.header([("chrA", 1000), ("chrB", 500)]) + b"".join(recs)
+
+
+def test_eof_block_is_the_spec_marker():
+    assert S.bgzf_block(b"") == S.SPEC_EOF
+
+
+@pytest.mark.parametrize("block,threads", [(0xFF00, 1), (7, 1), (7, 4), (1000, 3)])
+def test_known_records(tmp_path, block, threads):
+    p = tmp_path / "k.bam"
+    p.write_bytes(S.bgzf(known_records(), block=block))
+    with BamFile(p, threads=threads) as b:
+        assert b.names == ["chrA", "chrB"] and b.lens == [1000, 500]
+        st = b.stats
+        assert st["records"] == 6 and st["placed"] == 5 and st["exotic"] == 1 and st["eof_marker"] == 1
+        a = b.batch(0)
+        assert a.pos.tolist() == [10, 10, 21, 31]                      # getAlignmentStart = pos + 1
+        assert a.tlen.tolist() == [200, -50, 0, 0]
+        assert a.mapq.tolist() == [60, 7, 60, 60]
+        assert a.flag.tolist() == [0x63 | PL_RF_MATE_SAME_REF, 0x891 | PL_RF_NO_QUAL | PL_RF_EXOTIC,
+                                   0x4 | PL_RF_NO_QUAL, 0]
+        assert a.cigar_off.tolist() == [0, 1, 4, 4, 7]
+        assert a.cigar.tolist() == [w(5, "M"), w(3, "M"), w(1, "I"), w(2, "M"), w(4, "M"), w(2, "D"), w(6, "M")]
+        assert a.seq_off.tolist() == [0, 5, 11, 11, 21]
+        assert unpack_2bit(a.bases2, 0, 5).tolist() == [0, 1, 2, 3, 0]
+        assert unpack_2bit(a.bases2, 11, 10).tolist() == [0, 1, 2, 3, 0, 1, 2, 3, 0, 1]
+        assert a.quals[0:5].tolist() == [30, 31, 32, 33, 34]
+        assert a.quals[5:11].tolist() == [-1] * 6                      # 0xFF as a Java byte
+        assert a.exo_read.tolist() == [1] and a.exo_off.tolist() == [0]
+        # "ACNGTM": A=1 C=2 N=15 G=4 T=8 M=3, first base in the high nibble
+        assert a.nib[:3].tolist() == [0x12, 0xF4, 0x83]
+        e = b.batch(1)
+        assert e.pos.tolist() == [1] and e.flag.tolist() == [0x10]
+        assert unpack_2bit(e.bases2, 0, 5).tolist() == [2, 2, 0, 1, 2]
+
+
+def test_header_and_records_split_over_many_blocks(tmp_path):
+    text = b"@HD\tVN:1.6\tSO:coordinate\n" + b"@CO\t" + b"x" * 200000 + b"\n"
+    data = S.header([("c", 100)], text=text) + S.record(0, 4, [w(4, "M")], "TTGA", [9, 9, 9, 9])
+    p = tmp_path / "h.bam"
+    p.write_bytes(S.bgzf(data, block=5000))
+    with BamFile(p, threads=2) as b:
+        a = b.batch(0)
+        assert a.pos.tolist() == [5] and unpack_2bit(a.bases2, 0, 4).tolist() == [3, 3, 2, 0]
+
+
+def test_no_eof_marker_is_reported(tmp_path):
+    p = tmp_path / "n.bam"
+    p.write_bytes(S.bgzf(known_records(), eof=False))
+    with BamFile(p) as b:
+        assert b.stats["eof_marker"] == 0 and b.batch(0).n == 4
+
+
+def test_errors(tmp_path):
+    good = S.bgzf(known_records(), block=300)
+    cases = {}
+    bad = bytearray(good)
+    bad[40] ^= 0x55                                                    # inside the first block's deflate data
+    cases["crc"] = bytes(bad)
+    cases["truncated"] = good[:-40]
+    cases["not bgzf"] = b"\x1f\x8b\x08\x00" + b"\0" * 40
+    cases["not bam"] = S.bgzf(b"SAM\x01" + b"\0" * 64)
+    unsorted = S.header([("c", 100)]) + S.record(0, 50, [w(2, "M")], "AC", [9, 9]) + \
+        S.record(0, 10, [w(2, "M")], "AC", [9, 9])
+    cases["unsorted"] = S.bgzf(unsorted)
+    h = S.header([("c", 100)])
+    short = bytearray(h + S.record(0, 5, [w(8, "M")], "ACGTACGT", [9] * 8))
+    struct.pack_into("<I", short, len(h), 40)                          # block_size below its contents
+    cases["block_size"] = S.bgzf(bytes(short))
+    for name, blob in cases.items():
+        p = tmp_path / f"{name.replace(' ', '_')}.bam"
+        p.write_bytes(blob)
+        with pytest.raises(BamError) as e:
+            BamFile(p)
+        assert e.value.status == 1, name
+
+
+def _synth_roundtrip(tmp_path, name, scale, extra, threads, monkeypatch, slice_kib):
+    from oracle import oracle as O
+    from pilon_amd.synth import SynthData, preset
+    from tests.helpers import assert_same_region
+    p = preset(name, scale=scale)
+    p.update(extra)
+    d = SynthData(p)
+    refs = [(d.names[i], len(d.draft(i))) for i in range(d.n)]
+    batches = {i: d.batch(i) for i in range(d.n)}
+    path = tmp_path / f"{name}.bam"
+    nbytes = S.write_bam(path, refs, batches)
+    cfg = default_config()
+    monkeypatch.setenv("PILON_BAM_SLICE_KIB", str(slice_kib))
+    with BamFile(path, threads=threads) as b:
+        assert b.names == [r[0] for r in refs]
+        for i in range(d.n):
+            S.same_batch(batches[i], b.batch(i), f"{name} ref {i}")
+        # the oracle sees the same regions through the decoded batch
+        i = 0
+        L = len(d.draft(i))
+        ref = O.run_region(cfg, d.draft(i), 1, L, [batches[i]], name=d.names[i])
+        got = O.run_region(cfg, d.draft(i), 1, L, [b.reads(i)], name=d.names[i])
+        assert_same_region(got["pile"], got["calls"], ref, f"{name} via BAM")
+        assert got["changes"] == ref["changes"] and got["bases"] == ref["bases"]
+        return b.stats, nbytes
+
+
+def test_synthetic_roundtrip_c2(tmp_path, monkeypatch):
+    # slices of 256 KiB: records cut at slice ends are carried to the next
+    st, nbytes = _synth_roundtrip(tmp_path, "c2", 0.02, {"n_rate": 0.001}, 3, monkeypatch, 256)
+    assert st["exotic"] > 0 and st["placed"] > 10000
+    assert st["bam_bytes"] == nbytes and nbytes > 40 * (256 << 10)
+
+
+def test_synthetic_roundtrip_many_contigs(tmp_path, monkeypatch):
+    st, nbytes = _synth_roundtrip(tmp_path, "c3", 0.002, {}, 4, monkeypatch, 1024)
+    assert st["bam_bytes"] == nbytes
+
+
+def test_native_writer_matches_the_spec_writer(tmp_path):
+    """The generator's BAM writer (used for workload-size files) produces the
+    same bytes as the spec-literal writer at the same level and block size."""
+    from pilon_amd.synth import SynthData, preset
+    p = preset("c2", scale=0.004)
+    p.update({"n_rate": 0.002})
+    d = SynthData(p)
+    a, b = tmp_path / "native.bam", tmp_path / "spec.bam"
+    n1 = d.write_bam(a, level=1)
+    n2 = S.write_bam(b, [(d.names[i], len(d.draft(i))) for i in range(d.n)], {i: d.batch(i) for i in range(d.n)},
+                     level=1)
+    assert n1 == n2
+    assert a.read_bytes() == b.read_bytes()
