@@ -12,6 +12,7 @@
  *   PileUp fields + PileUp.baseCall        pilon/PileUp.scala:25-257
  *   GenomeRegion.identifyAndFixIssues      pilon/GenomeRegion.scala:307-380,413
  *   GenomeRegion.fixIssues/writeChanges    pilon/GenomeRegion.scala:557-657
+ *   GenomeRegion.writeVcf / Vcf            pilon/GenomeRegion.scala:623-643, pilon/Vcf.scala:28-163
  *
  * Conventions: plain C types only; 0 = success; the library owns device
  * memory, the caller owns host memory; one pl_ctx per host thread; objects
@@ -214,6 +215,34 @@ pl_status pl_batch_run(pl_ctx* ctx, const pl_region_desc* regions, int32_t n,
  * with HIP events on the library's stream, plus the share of the window
  * kernel alone. */
 pl_status pl_last_timing(pl_ctx* ctx, double* pipeline_ms, double* window_ms);
+
+/* ------------------------------------------------------------------ VCF */
+/* GenomeRegion.writeVcf (GenomeRegion.scala:623-643) for every region of
+ * the context's last batch (pl_batch_run, or the region that pl_region_finish
+ * ran), in batch order: Vcf.writeRecord (Vcf.scala:74-163) per locus, a
+ * deleted locus as an embedded "Del" record, an indel call followed by its
+ * second record.  Formatted on the device from the pileup, calls and indel
+ * evidence the batch left there; *text is malloc'ed (pl_free), *len bytes
+ * plus a terminating NUL.  Reassembly (writeFixRecord) and duplication
+ * (writeDup) records are not produced: those fixes are outside this path.
+ * device_ms (may be NULL) = device time of the formatting passes. */
+#define PL_VCF_DIPLOID 0x2u   /* --diploid: no "Amb" filter              */
+#define PL_VCF_QE      0x4u   /* --vcfqe: QE= (qualSum) instead of QP=  */
+pl_status pl_batch_vcf(pl_ctx* ctx, uint32_t flags, char** text, int64_t* len, double* device_ms);
+
+/* The same for one finished region (pl_region_finish).  A region fed by
+ * pl_region_add_reads keeps its reads on the device until pl_region_close
+ * for this; if the context has run other batches since, the region's batch
+ * runs again first.  pl_batch_vcf after pl_region_finish of such a region
+ * returns PL_E_INVAL. */
+pl_status pl_region_vcf(pl_region* reg, uint32_t flags, char** text, int64_t* len, double* device_ms);
+
+/* Vcf.writeHeader (Vcf.scala:28-70): the header lines for the contigs
+ * (names, lengths), with the caller's date (yyyyMMdd), source (Pilon's
+ * version string), command line and reference URI. */
+pl_status pl_vcf_header(const char* const* names, const int64_t* lengths, int32_t n, uint32_t flags,
+                        const char* date, const char* source, const char* command, const char* reference,
+                        char** text, int64_t* len);
 
 /* Sizes and HIP-event times of the last batch (pl_batch_run or
  * pl_region_finish). */

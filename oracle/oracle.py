@@ -30,6 +30,7 @@ class OrcResult(C.Structure):
                 ("insertSize", C.POINTER(C.c_int32)), ("weightedQual", C.POINTER(C.c_int8)),
                 ("weightedMq", C.POINTER(C.c_int8)), ("clips", C.POINTER(C.c_int16)),
                 ("stats", PlRegionStats), ("changes_text", C.c_char_p), ("changes_len", C.c_int64),
+                ("vcf_text", C.c_void_p), ("vcf_len", C.c_int64),
                 ("bases", C.POINTER(C.c_uint8)), ("bases_len", C.c_int64),
                 ("n_indel", C.c_int32), ("indel_pos", C.POINTER(C.c_int32)),
                 ("indel_off", C.POINTER(C.c_int64)), ("indel_str", C.POINTER(C.c_uint8))]
@@ -47,8 +48,9 @@ def lib():
         L.orc_region_run.restype = C.POINTER(OrcResult)
         L.orc_region_run.argtypes = [C.POINTER(PlConfig), C.c_char_p, C.c_int64, C.c_char_p,
                                      C.c_int32, C.c_int32, C.POINTER(C.POINTER(PlReadSoa)),
-                                     C.c_int32, C.c_char_p, C.c_int64]
+                                     C.c_int32, C.c_char_p, C.c_int64, C.c_int32]
         L.orc_result_free.argtypes = [C.POINTER(OrcResult)]
+        L.orc_fmt_af.argtypes = [C.c_float, C.c_char_p]
         L.orc_java_round.restype = C.c_int64
         L.orc_java_round.argtypes = [C.c_double]
         L.orc_roundDivI.restype = C.c_int32
@@ -71,8 +73,13 @@ def _arr(ptr, n, dtype):
     return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
 
 
-def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0):
-    """One GenomeRegion through the oracle; returns a dict of numpy arrays."""
+VCF_ON, VCF_DIPLOID, VCF_QE = 1, 2, 4
+
+
+def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0, vcf=0):
+    """One GenomeRegion through the oracle; returns a dict of numpy arrays.
+    vcf: VCF_ON (| VCF_DIPLOID | VCF_QE) adds the region's VCF record lines
+    (GenomeRegion.writeVcf) as out["vcf"]."""
     L = lib()
     if isinstance(contig, str):
         contig = contig.encode()
@@ -82,7 +89,7 @@ def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, of
     soas = [rs.soa() for rs in readsets]
     arr = (C.POINTER(PlReadSoa) * max(1, len(soas)))(*[C.pointer(s) for s in soas])
     rp = L.orc_region_run(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
-                          arr, len(soas), new_name.encode(), offset)
+                          arr, len(soas), new_name.encode(), offset, vcf)
     r = rp.contents
     out = {"status": r.status, "edge_read": r.edge_read, "edge_why": r.edge_why.decode()}
     if r.status == 0:
@@ -101,6 +108,8 @@ def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, of
         out["clips"] = _arr(r.clips, size, np.int16)
         out["stats"] = r.stats.as_dict()
         out["changes"] = C.string_at(r.changes_text, r.changes_len).decode() if r.changes_len else ""
+        if vcf:
+            out["vcf"] = C.string_at(r.vcf_text, r.vcf_len).decode() if r.vcf_len else ""
         out["bases"] = bytes(_arr(r.bases, r.bases_len, np.uint8))
         n = r.n_indel
         pos = _arr(r.indel_pos, n, np.int32)
@@ -118,11 +127,18 @@ def run_region_stats(cfg, contig, start, stop, readsets, name="ctg"):
     soas = [rs.soa() for rs in readsets]
     arr = (C.POINTER(PlReadSoa) * max(1, len(soas)))(*[C.pointer(s) for s in soas])
     rp = L.orc_region_run(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
-                          arr, len(soas), (name + "_pilon").encode(), 0)
+                          arr, len(soas), (name + "_pilon").encode(), 0, 0)
     r = rp.contents
     status, stats = r.status, (r.stats.as_dict() if r.status == 0 else None)
     L.orc_result_free(rp)
     return status, stats
+
+
+def fmt_af(af):
+    """Java's "%.2f".format of a Float widened to Double, as the oracle restates it."""
+    buf = C.create_string_buffer(32)
+    lib().orc_fmt_af(C.c_float(af), buf)
+    return buf.value.decode()
 
 
 def java_round(x):

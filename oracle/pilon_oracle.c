@@ -150,6 +150,7 @@ static void basesum_order(const int64_t s[4], int ord[4]) {
 /* --------------------------------------------------------- BaseCall */
 typedef struct {
   int base;              /* 0..3, 4 = 'N' */
+  int bI;                /* order(0): the index of baseSum even when base is 'N' */
   int alt;
   int homo;
   int64_t score, q;
@@ -214,6 +215,7 @@ static void baseCall(const PileUp* p, const pl_config* cfg, BaseCall* bc) {
   basesum_order(p->qSum > 0 ? p->qs : p->bc, ord);
   bc->base = n > 0 ? ord[0] : 4;
   int bI = ord[0], aI = ord[1];
+  bc->bI = bI;
   bc->alt = aI;
   int64_t baseSum = p->qs[bI], altBaseSum = p->qs[aI];
   int64_t total = p->qs[0] + p->qs[1] + p->qs[2] + p->qs[3];
@@ -557,6 +559,7 @@ typedef struct orc_result {
   int16_t* clips;
   pl_region_stats stats;
   char*    changes_text; int64_t changes_len;
+  char*    vcf_text; int64_t vcf_len;   /* Vcf.writeRecord lines of the region (when asked) */
   uint8_t* bases; int64_t bases_len;
   /* the called indel string at each INS/DEL change, packed: for change c,
    * indel_pos[c] (region index) and indel_str[indel_off[c] .. indel_off[c+1]) */
@@ -567,7 +570,7 @@ void orc_result_free(orc_result* r) {
   if (!r) return;
   free(r->pile); free(r->bcq); free(r->calls); free(r->coverage); free(r->badCoverage);
   free(r->physCoverage); free(r->insertSize); free(r->weightedQual); free(r->weightedMq);
-  free(r->clips); free(r->changes_text); free(r->bases);
+  free(r->clips); free(r->changes_text); free(r->vcf_text); free(r->bases);
   free(r->indel_pos); free(r->indel_off); free(r->indel_str);
   free(r);
 }
@@ -601,10 +604,139 @@ static uint8_t toIUPAC(int b1, int b2) {
  * postProcess Pass 1, identifyAndFixIssues (snps/indels), writeChanges.
  * contig: raw bytes, name: contig name, new_name/offset: writeChanges args.
  */
+/* ------------------------------------------------------------------ VCF */
+/* Java's "%.2f".format of a Float widened to Double (Vcf.scala:147): the
+ * Formatter rounds the decimal digits of the value half up.  A widened float
+ * has at most 24 significant bits, so no digit string shorter than its exact
+ * expansion lies on a tie that the exact value does not: rounding the exact
+ * binary value half up gives the same digits. */
+static void fmt_af(char* out, float af) {
+  int64_t q = 0;
+  if (af > 0.0f) {
+    int e;
+    double m = frexp((double)af, &e);                    /* af = m * 2^e, m in [0.5, 1) */
+    uint64_t mi = (uint64_t)ldexp(m, 24);                /* exact: 24 significant bits */
+    uint64_t t = mi * 100u;                              /* af * 100 = t * 2^(e-24) */
+    int sh = 24 - e;
+    if (sh <= 0) q = (int64_t)(t << -sh);
+    else if (sh < 63) {
+      q = (int64_t)(t >> sh);
+      if ((t & ((1ull << sh) - 1)) >= (1ull << (sh - 1))) q++;
+    }
+  }
+  sprintf(out, "%lld.%02lld", (long long)(q / 100), (long long)(q % 100));
+}
+
+typedef struct {
+  const pl_config* cfg; const uint8_t* contig; const char* name; int32_t start;
+  int32_t minDepth; int diploid, vcfqe;
+  char* buf; int64_t len, cap;
+} VcfOut;
+
+static void vcf_puts(VcfOut* o, const char* s) { sb_append(&o->buf, &o->len, &o->cap, s, (int64_t)strlen(s)); }
+static void vcf_putn(VcfOut* o, const uint8_t* s, int32_t n) { sb_append(&o->buf, &o->len, &o->cap, (const char*)s, n); }
+static void vcf_putl(VcfOut* o, int64_t v) { char t[32]; sprintf(t, "%lld", (long long)v); vcf_puts(o, t); }
+
+/* Vcf.writeRecord (Vcf.scala:74-163) for region index i */
+static void vcf_record(VcfOut* o, const PileUp* p, int32_t i, int embedded, int indelOkArg) {
+  BaseCall bc; baseCall(p, o->cfg, &bc);
+  const int indelOk = indelOkArg && i > 0;
+  const int32_t locus = o->start + i;
+  const uint8_t base = bc.base == 4 ? 'N' : (uint8_t)"ACGT"[bc.base];
+  const uint8_t altB = (uint8_t)"ACGT"[bc.alt];
+  /* bcString = callString(indelOk) (PileUp.scala:169-173) */
+  const uint8_t* bcs = &base; int32_t bcl = 1;
+  if (indelOk && (bc.isIns || bc.isDel)) { bcs = bc.indelStr; bcl = bc.indelLen; }
+  const int32_t baseDP = (int32_t)p->qs[bc.bI];        /* bc.baseSum.toInt */
+  const int32_t altBaseDP = (int32_t)p->qs[bc.alt];
+  const int64_t depth = pu_depth(p);
+  int32_t loc = locus;
+  uint8_t rBase;
+  uint8_t refS[2], altS[2];
+  const uint8_t *refStr, *altStr; int32_t refL, altL;  /* REF and the called allele (cBase) */
+  uint8_t* tmp = NULL;
+  const char* callType;
+  int32_t refDP, altDP;
+  if (indelOk && !embedded && bc.isDel) {
+    loc -= 1;
+    rBase = upper(o->contig[loc - 1]);
+    callType = bc.homoIndel ? "1/1" : "0/1";
+    int32_t pc = delPct(p);
+    tmp = (uint8_t*)malloc((size_t)bcl + 1); tmp[0] = rBase; memcpy(tmp + 1, bcs, (size_t)bcl);
+    refStr = tmp; refL = bcl + 1; altS[0] = rBase; altStr = altS; altL = 1;
+    refDP = 100 - pc; altDP = pc;
+  } else if (indelOk && !embedded && bc.isIns) {
+    loc -= 1;
+    rBase = upper(o->contig[loc - 1]);
+    callType = bc.homoIndel ? "1/1" : "0/1";
+    int32_t pc = insPct(p);
+    tmp = (uint8_t*)malloc((size_t)bcl + 1); tmp[0] = rBase; memcpy(tmp + 1, bcs, (size_t)bcl);
+    refS[0] = rBase; refStr = refS; refL = 1; altStr = tmp; altL = bcl + 1;
+    refDP = 100 - pc; altDP = pc;
+  } else if (bc.homo) {
+    rBase = upper(o->contig[loc - 1]);
+    refS[0] = rBase; refStr = refS; refL = 1; altS[0] = base; altStr = altS; altL = 1;
+    if (rBase == base || (bcl == 1 && bcs[0] == 'N')) { callType = "0/0"; refDP = baseDP; altDP = altBaseDP; }
+    else { callType = "1/1"; refDP = altBaseDP; altDP = baseDP; }
+  } else {
+    rBase = upper(o->contig[loc - 1]);
+    refS[0] = rBase; refStr = refS; refL = 1;
+    callType = "0/1";
+    if (rBase == base) { altS[0] = altB; refDP = baseDP; altDP = altBaseDP; }
+    else { altS[0] = base; refDP = altBaseDP; altDP = baseDP; }
+    altStr = altS; altL = 1;
+  }
+  /* filters: prepended, so the last added comes first */
+  const char* f[3]; int nf = 0;
+  if (embedded) f[nf++] = "Del";
+  if (!o->diploid && callType[0] == '0' && callType[2] == '1') f[nf++] = "Amb";
+  if (depth < o->minDepth) f[nf++] = "LowCov";
+  const int dotAlt = (altL == 1 && altStr[0] == 'N') || (altL == refL && memcmp(altStr, refStr, (size_t)refL) == 0);
+  const int ac = callType[0] == '0' ? (callType[2] == '0' ? 0 : 1) : 2;
+  const int32_t dsum = jadd(refDP, altDP);
+  const float af = (dsum > 0 && !dotAlt) ? (float)altDP / (float)dsum : 0.0f;
+  char afs[32]; fmt_af(afs, af);
+  const int64_t count = pu_count(p);
+  const int64_t qsum = p->qs[0] + p->qs[1] + p->qs[2] + p->qs[3];
+  const int64_t meanQual = roundDivL(qsum, roundDivL((int64_t)p->mqSum * count, depth));   /* PileUp.scala:64-67 */
+  const int64_t meanMq = roundDivL((int64_t)p->mqSum - depth, depth);                     /* :70-72 */
+  vcf_puts(o, o->name); vcf_puts(o, "\t"); vcf_putl(o, loc); vcf_puts(o, "\t.\t");
+  vcf_putn(o, refStr, refL); vcf_puts(o, "\t");
+  if (dotAlt) vcf_puts(o, "."); else vcf_putn(o, altStr, altL);
+  vcf_puts(o, "\t");
+  if (indelOk && bc.isDel) vcf_puts(o, "."); else vcf_putl(o, bc.score);
+  vcf_puts(o, "\t");
+  if (!nf) vcf_puts(o, "PASS");
+  for (int k = 0; k < nf; k++) { if (k) vcf_puts(o, ";"); vcf_puts(o, f[k]); }
+  vcf_puts(o, "\tDP="); vcf_putl(o, embedded ? count : depth);
+  vcf_puts(o, ";TD="); vcf_putl(o, depth + p->badPair);
+  vcf_puts(o, ";BQ="); vcf_putl(o, meanQual);
+  vcf_puts(o, ";MQ="); vcf_putl(o, meanMq);
+  vcf_puts(o, ";QD="); vcf_putl(o, bc.q);
+  vcf_puts(o, ";BC=");
+  for (int b = 0; b < 4; b++) { if (b) vcf_puts(o, ","); vcf_putl(o, p->bc[b]); }
+  if (o->vcfqe) {                                       /* BaseSum.toString (BaseSum.scala:73) */
+    vcf_puts(o, ";QE=");
+    for (int b = 0; b < 4; b++) { if (b) vcf_puts(o, ","); vcf_putl(o, p->qs[b]); }
+  } else {                                              /* BaseSum.toStringPct (BaseSum.scala:68-71) */
+    vcf_puts(o, ";QP=");
+    for (int b = 0; b < 4; b++) { if (b) vcf_puts(o, ","); vcf_putl(o, qsum == 0 ? 0 : ldiv64(100 * p->qs[b] + qsum / 2, qsum)); }
+  }
+  vcf_puts(o, ";PC="); vcf_putl(o, p->physCov);
+  vcf_puts(o, ";IC="); vcf_putl(o, p->insertions);
+  vcf_puts(o, ";DC="); vcf_putl(o, p->deletions);
+  vcf_puts(o, ";XC="); vcf_putl(o, p->clips);
+  vcf_puts(o, ";AC="); vcf_putl(o, ac);
+  vcf_puts(o, ";AF="); vcf_puts(o, afs);
+  vcf_puts(o, "\tGT\t"); vcf_puts(o, callType); vcf_puts(o, "\n");
+  free(tmp);
+  if (indelOk && bc.indel && !embedded) vcf_record(o, p, i, bc.isDel && bc.homoIndel, 0);
+}
+
 orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
                            const char* name, int32_t start, int32_t stop,
                            const pl_read_soa* const* sets, int32_t n_sets,
-                           const char* new_name, int64_t offset) {
+                           const char* new_name, int64_t offset, int32_t vcf_flags) {
   orc_result* res = (orc_result*)calloc(1, sizeof(orc_result));
   if (start < 1 || stop < start || stop > contig_len) { res->status = PL_E_INVAL; return res; }  /* GenomeRegion.scala:34 */
   PRegion R; memset(&R, 0, sizeof R);
@@ -851,6 +983,17 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
   res->stats.n_unknown_ops = (int32_t)R.unknownOps;
   res->stats.aligned_bases = R.alignedBases;
 
+  /* GenomeRegion.writeVcf (GenomeRegion.scala:623-643) without the
+   * reassembly and duplication records: every locus, embedded when deleted */
+  if (vcf_flags & 1) {
+    VcfOut o; memset(&o, 0, sizeof o);
+    o.cfg = cfg; o.contig = contig; o.name = name; o.start = start; o.minDepth = minDepth;
+    o.diploid = (vcf_flags & 2) != 0; o.vcfqe = (vcf_flags & 4) != 0;
+    sb_append(&o.buf, &o.len, &o.cap, "", 0);
+    for (int32_t i = 0; i < size; i++) vcf_record(&o, &R.pu[i], i, deleted[i], 1);
+    res->vcf_text = o.buf; res->vcf_len = o.len;
+  }
+
   free(tmp); free(kept); free(snpL.v); free(smallL.v); free(oneChar);
   free(confirmed); free(deleted); free(kindAt);
   for (int32_t i = 0; i < size; i++) { free(R.pu[i].ins.v); free(R.pu[i].del.v); }
@@ -861,6 +1004,7 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
 }
 
 /* Exposed for known-answer tests of the small helpers. */
+void orc_fmt_af(float af, char* out) { fmt_af(out, af); }
 int32_t orc_roundDivI(int32_t n, int32_t d) { return roundDivI(n, d); }
 int64_t orc_roundDivL(int64_t n, int64_t d) { return roundDivL(n, d); }
 

@@ -181,6 +181,10 @@ struct pl_region {
   bool finished = false;
   pl_region_stats stats;
   std::vector<plh::Change> changes;
+  // the read set pl_region_finish built from add_reads: out of the context's
+  // table once finished, kept for pl_region_vcf until the region is closed
+  std::unique_ptr<ReadSet> temp;
+  int32_t temp_id = -1;
 };
 
 namespace {
@@ -995,7 +999,11 @@ pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_regio
   if (!s) s = job_status(L, 0);
   if (!s) s = job_changes(c, L, 0, &r->stats, &r->changes);
   if (!s) s = job_export(L, 0, pile, calls);
-  if (temp) { c->rsets[id].reset(); c->rs_version++; }
+  if (temp) {
+    r->temp = std::move(c->rsets[id]);
+    r->temp_id = id;
+    c->rs_version++;
+  }
   if (s) return s;
   if (stats) *stats = r->stats;
   r->finished = true;
@@ -1098,3 +1106,4 @@ pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
 }  // extern "C"
 
 #include "pl_serve.hip"
+#include "pl_vcf.hip"

@@ -1135,3 +1135,5 @@ __global__ void k_ins_strings(EngineArgs A, const Cand* sorted, const int64_t* i
   const RSDev& R = A.rs[A.jobs[j].rs];
   for (int32_t q = 0; q < e.len; q++) pool[off[i] + q] = rec_char(R, e, q);
 }
+
+#include "pl_vcf_kernels.hip"

@@ -1,0 +1,204 @@
+// pl_vcf.hip — the VCF writer's host side (included at the end of
+// pl_engine.hip): Vcf.writeHeader (Vcf.scala:28-70) and, for the last batch
+// of a context, the records of GenomeRegion.writeVcf formatted on the device
+// by the k_vcf_* kernels.
+
+namespace {
+
+// device buffers of one VCF pass, freed on every exit
+struct VcfBufs {
+  std::vector<void*> p;
+  ~VcfBufs() { for (void* x : p) (void)hipFree(x); }
+  template <class T> pl_status get(T** out, size_t n) {
+    void* q = nullptr;
+    if (hipMalloc(&q, n ? n : 1) != hipSuccess) return fail(PL_E_OOM, "device allocation of %zu bytes failed", n);
+    p.push_back(q);
+    *out = (T*)q;
+    return PL_OK;
+  }
+};
+
+void add_line(std::string& s, const std::string& l) { s += l; s += '\n'; }
+
+}  // namespace
+
+extern "C" {
+
+pl_status pl_vcf_header(const char* const* names, const int64_t* lengths, int32_t n, uint32_t flags,
+                        const char* date, const char* source, const char* command, const char* reference,
+                        char** text, int64_t* len) {
+  if (!text || !len || (n > 0 && (!names || !lengths))) return fail(PL_E_INVAL, "null argument");
+  std::string s;
+  add_line(s, "##fileformat=VCFv4.1");
+  add_line(s, std::string("##fileDate=") + (date ? date : ""));
+  add_line(s, std::string("##source=\"") + (source ? source : "") + "\"");
+  add_line(s, std::string("##PILON=\"") + (command ? command : "") + "\"");
+  add_line(s, std::string("##reference=") + (reference ? reference : ""));
+  for (int32_t i = 0; i < n; i++)
+    add_line(s, std::string("##contig=<ID=") + names[i] + ",length=" + std::to_string(lengths[i]) + ">");
+  add_line(s, "##FILTER=<ID=LowCov,Description=\"Low Coverage of good reads at location\">");
+  add_line(s, "##FILTER=<ID=Amb,Description=\"Ambiguous evidence in haploid genome\">");
+  add_line(s, "##FILTER=<ID=Del,Description=\"This base is in a deletion or change event from another record\">");
+  add_line(s, "##INFO=<ID=DP,Number=1,Type=Integer,Description=\"Valid read depth; some reads may have been filtered\">");
+  add_line(s, "##INFO=<ID=TD,Number=1,Type=Integer,Description=\"Total read depth including bad pairs\">");
+  add_line(s, "##INFO=<ID=PC,Number=1,Type=Integer,Description=\"Physical coverage of valid inserts across locus\">");
+  add_line(s, "##INFO=<ID=BQ,Number=1,Type=Integer,Description=\"Mean base quality at locus\">");
+  add_line(s, "##INFO=<ID=MQ,Number=1,Type=Integer,Description=\"Mean read mapping quality at locus\">");
+  add_line(s, "##INFO=<ID=QD,Number=1,Type=Integer,Description=\"Variant confidence/quality by depth\">");
+  add_line(s, "##INFO=<ID=BC,Number=4,Type=Integer,Description=\"Count of As, Cs, Gs, Ts at locus\">");
+  if (flags & PL_VCF_QE)
+    add_line(s, "##INFO=<ID=QE,Number=4,Type=Integer,Description=\"Evidence for As, Cs, Gs, Ts weighted by Q & MQ at locus\">");
+  else
+    add_line(s, "##INFO=<ID=QP,Number=4,Type=Integer,Description=\"Percentage of As, Cs, Gs, Ts weighted by Q & MQ at locus\">");
+  add_line(s, "##INFO=<ID=IC,Number=1,Type=Integer,Description=\"Number of reads with insertion here\">");
+  add_line(s, "##INFO=<ID=DC,Number=1,Type=Integer,Description=\"Number of reads with deletion here\">");
+  add_line(s, "##INFO=<ID=XC,Number=1,Type=Integer,Description=\"Number of reads clipped here\">");
+  add_line(s, "##INFO=<ID=AC,Number=A,Type=Integer,Description=\"Allele count in genotypes, for each ALT allele, in the same order as listed\">");
+  add_line(s, "##INFO=<ID=AF,Number=A,Type=Float,Description=\"Fraction of evidence in support of alternate allele(s)\">");
+  add_line(s, "##INFO=<ID=SVTYPE,Number=1,Type=String,Description=\"Type of structural variant\">");
+  add_line(s, "##INFO=<ID=SVLEN,Number=.,Type=String,Description=\"Difference in length between REF and ALT alleles\">");
+  add_line(s, "##INFO=<ID=END,Number=1,Type=Integer,Description=\"End position of the variant described in this record\">");
+  add_line(s, "##INFO=<ID=IMPRECISE,Number=0,Type=Flag,Description=\"Imprecise change from local reassembly (ALT contains Ns)\">");
+  add_line(s, "##FORMAT=<ID=GT,Number=1,Type=String,Description=\"Genotype\">");
+  add_line(s, "##FORMAT=<ID=AD,Number=.,Type=String,Description=\"Allelic depths for the ref and alt alleles in the order listed\">");
+  add_line(s, "##FORMAT=<ID=DP,Number=1,Type=String,Description=\"Approximate read depth; some reads may have been filtered\">");
+  add_line(s, "##ALT=<ID=DUP,Description=\"Possible segmental duplication\">");
+  add_line(s, "#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\tSAMPLE");
+  char* out = (char*)malloc(s.size() + 1);
+  if (!out) return fail(PL_E_OOM, "host allocation failed");
+  std::memcpy(out, s.data(), s.size() + 1);
+  *text = out;
+  *len = (int64_t)s.size();
+  return PL_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The VCF records of the lane's last batch; names[j] is region j's name.
+pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, uint32_t flags, char** text,
+                      int64_t* len, double* device_ms) {
+  *text = nullptr;
+  *len = 0;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = L->stream;
+  const int64_t G = L->G;
+  const int32_t nj = (int32_t)L->hjobs.size();
+  std::string names;
+  std::vector<int64_t> noff(1, 0);
+  for (int32_t j = 0; j < nj; j++) {
+    names += rnames[(size_t)j];
+    noff.push_back((int64_t)names.size());
+  }
+  VcfBufs B;
+  VcfArgs V;
+  std::memset(&V, 0, sizeof V);
+  V.A = L->args;
+  V.flags = flags;
+  char* dnames;
+  int64_t* dnoff;
+  pl_status s;
+  if ((s = B.get(&dnames, names.size())) || (s = B.get(&dnoff, 8 * noff.size()))) return s;
+  HIPCHK(hipMemcpyAsync(dnames, names.data(), names.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dnoff, noff.data(), 8 * noff.size(), hipMemcpyHostToDevice, st));
+  V.names = dnames;
+  V.name_off = dnoff;
+  if ((s = B.get(&V.ilist_n, 4)) || (s = B.get(&V.imap, 4 * (size_t)G)) || (s = B.get(&V.len, 4 * (size_t)G)))
+    return s;
+  uint64_t* off;
+  if ((s = B.get(&off, 8 * (size_t)(G + 1)))) return s;
+  V.off = off;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  // indel-call columns, grown once if the first capacity was short
+  uint32_t icap = (uint32_t)std::min<int64_t>(G, 1 << 20) + 64, ni = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    if ((s = B.get(&V.ilist, 4 * (size_t)icap)) || (s = B.get(&V.irec, 8 * (size_t)icap))) return s;
+    V.icap = icap;
+    HIPCHK(hipMemsetAsync(V.ilist_n, 0, 4, st));
+    k_vcf_mark<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(V);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&ni, V.ilist_n, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (ni <= icap) break;
+    icap = ni + 64;
+  }
+  if (ni) {
+    k_vcf_indel<<<dim3((unsigned)std::min<uint32_t>(4096, (ni + 3) / 4)), dim3(256), 0, st>>>(V);
+    HIPCHK(hipGetLastError());
+  }
+  k_vcf_lines<false><<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(V);
+  HIPCHK(hipGetLastError());
+  const int64_t tiles = std::max<int64_t>(1, (G + SCAN_TILE - 1) / SCAN_TILE);
+  uint64_t* tsum;
+  if ((s = B.get(&tsum, 8 * (size_t)tiles))) return s;
+  k_scan64_tiles<<<dim3((unsigned)tiles), dim3(256), 0, st>>>(V.len, G, tsum);
+  k_scan64_top<<<dim3(1), dim3(1024), 0, st>>>(tsum, tiles);
+  k_scan64_apply<<<dim3((unsigned)tiles), dim3(256), 0, st>>>(V.len, G, tsum, off);
+  HIPCHK(hipGetLastError());
+  uint64_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, off + G, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  char* dtext;
+  if ((s = B.get(&dtext, (size_t)total))) return s;
+  V.text = dtext;
+  k_vcf_lines<true><<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(V);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(e1, st));
+  char* out = (char*)malloc((size_t)total + 1);
+  if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
+  HIPCHK(hipMemcpyAsync(out, dtext, (size_t)total, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  out[total] = 0;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (device_ms) *device_ms = ms;
+  *text = out;
+  *len = (int64_t)total;
+  return PL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pl_status pl_batch_vcf(pl_ctx* c, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!c || !text || !len) return fail(PL_E_INVAL, "null argument");
+  Lane* L = &c->main;
+  if (!L->tables_valid || L->hjobs.empty() || L->n_wins <= 0) return fail(PL_E_INVAL, "no batch to write");
+  std::vector<std::string> names;
+  for (const JobDev& J : L->hjobs) {
+    if (J.rs < 0 || J.rs >= (int32_t)c->rsets.size() || !c->rsets[J.rs])
+      return fail(PL_E_INVAL, "the last batch's reads were released (a finished region: use pl_region_vcf)");
+    names.push_back(c->contigs.at(c->rsets[J.rs]->contig).name);
+  }
+  return vcf_of_lane(c, L, names, flags, text, len, device_ms);
+}
+
+pl_status pl_region_vcf(pl_region* r, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
+  if (!r->finished) return fail(PL_E_INVAL, "region not finished (pl_region_finish first)");
+  pl_ctx* c = r->ctx;
+  Lane* L = &c->main;
+  const int32_t id = r->readset >= 0 ? r->readset : r->temp_id;
+  const std::vector<JobReq> q{{id, r->start, r->stop}};
+  const bool current = L->tables_valid && L->last_reqs == q && L->hjobs.size() == 1;
+  std::vector<std::string> names{c->contigs.at(r->contig).name};
+  if (current) return vcf_of_lane(c, L, names, flags, text, len, device_ms);
+  // the lane has run other batches since: run this region again (its read
+  // set goes back into the context's table for the run)
+  HIPCHK(hipSetDevice(c->device));
+  if (r->temp) { c->rsets[id] = std::move(r->temp); c->rs_version++; }
+  pl_status s = run_jobs(c, L, q, PL_DEFAULT_WINDOW, 0, PL_DEFAULT_BLOCK, false);
+  if (!s) s = fetch_batch_meta(L);
+  if (!s) s = vcf_of_lane(c, L, names, flags, text, len, device_ms);
+  if (r->temp_id >= 0 && !r->temp) { r->temp = std::move(c->rsets[id]); c->rs_version++; }
+  return s;
+}
+
+}  // extern "C"

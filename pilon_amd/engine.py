@@ -26,6 +26,23 @@ LIB = os.path.join(HERE, "libpilonhip.so")
 _lib = None
 
 
+VCF_DIPLOID, VCF_QE = 0x2, 0x4
+
+
+def vcf_header(contigs, date, source, command, reference, vcfqe=False):
+    """Vcf.writeHeader (Vcf.scala:28-70) for [(name, length)] contigs."""
+    n = len(contigs)
+    names = (C.c_char_p * max(1, n))(*[c[0].encode() for c in contigs])
+    lens = (C.c_int64 * max(1, n))(*[c[1] for c in contigs])
+    t, ln = C.c_void_p(), C.c_int64()
+    _check(lib().pl_vcf_header(names, lens, n, VCF_QE if vcfqe else 0, date.encode(), source.encode(),
+                               command.encode(), reference.encode(), C.byref(t), C.byref(ln)))
+    try:
+        return C.string_at(t, ln.value).decode()
+    finally:
+        lib().pl_free(t)
+
+
 class PilonError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -64,6 +81,10 @@ def lib():
             "pl_serve_fetch": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
             "pl_serve_drain": (C.c_int, [C.c_void_p]),
             "pl_serve_close": (None, [C.c_void_p]),
+            "pl_batch_vcf": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
+            "pl_region_vcf": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
+            "pl_vcf_header": (C.c_int, [P(C.c_char_p), P(C.c_int64), C.c_int32, C.c_uint32, C.c_char_p, C.c_char_p,
+                                        C.c_char_p, C.c_char_p, P(C.c_void_p), P(C.c_int64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -135,6 +156,19 @@ class Context:
                                   pile.ctypes.data if copy_out else None,
                                   calls.ctypes.data if copy_out else None, stats if want_stats else None))
         return pile, calls, ([stats[i].as_dict() for i in range(n)] if want_stats else None)
+
+    def batch_vcf(self, diploid=False, vcfqe=False):
+        """GenomeRegion.writeVcf for every region of the last batch, in batch
+        order (Vcf.writeRecord lines, formatted on the device); returns
+        (text, device_ms)."""
+        flags = (VCF_DIPLOID if diploid else 0) | (VCF_QE if vcfqe else 0)
+        t, n, ms = C.c_void_p(), C.c_int64(), C.c_double()
+        _check(lib().pl_batch_vcf(self.h, flags, C.byref(t), C.byref(n), C.byref(ms)))
+        try:
+            text = C.string_at(t, n.value).decode()
+        finally:
+            lib().pl_free(t)
+        return text, ms.value
 
     def batch_info(self):
         info = PlBatchInfo()
@@ -293,6 +327,16 @@ class PileUpRegion:
     def __call__(self, i):
         """apply(i): the PileUp at region index i (PileUpRegion.scala:233)."""
         return PileUp(self.pile[:, i], self.calls[i])
+
+    def vcf(self, diploid=False, vcfqe=False):
+        """GenomeRegion.writeVcf for this finished region: (text, device_ms)."""
+        flags = (VCF_DIPLOID if diploid else 0) | (VCF_QE if vcfqe else 0)
+        t, n, ms = C.c_void_p(), C.c_int64(), C.c_double()
+        _check(lib().pl_region_vcf(self.h, flags, C.byref(t), C.byref(n), C.byref(ms)))
+        try:
+            return C.string_at(t, n.value).decode(), ms.value
+        finally:
+            lib().pl_free(t)
 
     def fix(self, new_name=None, offset=0):
         """identifyAndFixIssues + writeChanges: returns (changes text, fixed bases)."""
