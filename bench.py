@@ -169,6 +169,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=20000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--vcf", type=int, default=1, help="also format the batch's VCF records on the device")
     ap.add_argument("--bam", type=int, default=1,
                     help="also run the BAM producer: the workload written as a BAM, decoded, uploaded and piled")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -275,6 +276,21 @@ def main():
         cpu = cpu_baseline(cfg, data, regions, args.cpu_seconds)
         log(f"bench: CPU oracle {cpu['value'] / 1e6:.1f} M bases/s on {cpu['cores']} threads ({cpu['sample']})")
 
+    vcf = None
+    if args.vcf:
+        # the last batch on the context is the timed one (all regions)
+        ctx.batch_run(regions, window=W, block=B, persistent=1, copy_out=False, want_stats=False)
+        t0 = time.perf_counter()
+        nbytes, lines, vms = ctx.batch_vcf(decode=False)
+        wall = time.perf_counter() - t0
+        vcf = {"records": lines, "bytes": nbytes, "device_ms": vms, "wall_ms": wall * 1000.0,
+               "records_per_s": lines / (vms / 1000.0), "text_GB_per_s": nbytes / (vms / 1000.0) / 1e9,
+               "note": "Vcf.writeRecord lines of every locus (GenomeRegion.writeVcf) formatted on the device; "
+                       "device_ms covers the mark, indel, length, scan and write passes, wall_ms adds the copy "
+                       "of the text to host memory"}
+        log(f"bench: VCF: {lines} records, {nbytes / 1e9:.2f} GB in {vms:.1f} ms on the device "
+            f"({vcf['records_per_s'] / 1e9:.2f} G records/s), {wall * 1000:.0f} ms with the copy out")
+
     producer = None
     if args.bam:
         producer = bam_producer(data, ctx, args, W, B, aligned)
@@ -304,6 +320,7 @@ def main():
         "p99_latency_ms": serving["p99_ms"] if serving else None,
         "serving": serving,
         "bam_producer": producer,
+        "vcf": vcf,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

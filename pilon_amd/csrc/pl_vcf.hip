@@ -145,7 +145,7 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   char* dtext;
   if ((s = B.get(&dtext, (size_t)total))) return s;
   V.text = dtext;
-  k_vcf_lines<true><<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(V);
+  k_vcf_write<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(V);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(e1, st));
   char* out = (char*)malloc((size_t)total + 1);

@@ -10,9 +10,9 @@
 //                 prints the indel allele);
 //   k_vcf_indel   finds each listed column's winning evidence record again
 //                 (the strict majority of the indel pass);
-//   k_vcf_lines   measures (pass 1) or writes (pass 2) every locus's
-//                 record lines; an exclusive 64-bit scan of the lengths
-//                 between the passes gives each locus its byte offset.
+//   k_vcf_lines   measures every locus's record lines; an exclusive
+//                 64-bit scan of the lengths gives each locus its offset;
+//   k_vcf_write   formats them again, staged in LDS, and stores them.
 
 struct VcfArgs {
   EngineArgs A;
@@ -278,6 +278,7 @@ __global__ void __launch_bounds__(256) k_vcf_indel(VcfArgs V) {
   }
 }
 
+// the length pass (WRITE false); the write pass is k_vcf_write
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_vcf_lines(VcfArgs V) {
   const WinDev win = V.A.wins[blockIdx.x];
@@ -290,6 +291,48 @@ __global__ void __launch_bounds__(256) k_vcf_lines(VcfArgs V) {
     const int64_t g = J.out + p;
     if (WRITE) vcf_locus<true>(V.text + V.off[g], V, J, R, name, name_len, minDepth, g, p);
     else V.len[g] = vcf_locus<false>(nullptr, V, J, R, name, name_len, minDepth, g, p);
+  }
+}
+
+// The write pass.  A chunk of 256 loci (one per thread) is formatted into
+// LDS at the loci's relative offsets, then the chunk's bytes, contiguous in
+// the output, leave LDS as aligned dword stores of consecutive lanes.  A
+// chunk whose text exceeds the stage is written straight to HBM.
+#define PL_VCF_STAGE 57344
+__global__ void __launch_bounds__(256) k_vcf_write(VcfArgs V) {
+  __shared__ char stage[PL_VCF_STAGE];
+  const WinDev win = V.A.wins[blockIdx.x];
+  const JobDev& J = V.A.jobs[win.job];
+  const RSDev& R = V.A.rs[J.rs];
+  const char* name = V.names + V.name_off[win.job];
+  const int64_t name_len = V.name_off[win.job + 1] - V.name_off[win.job];
+  const int32_t minDepth = V.A.st[win.job].min_depth;
+  const int tid = threadIdx.x;
+  for (int c0 = win.p0; c0 < win.p1; c0 += 256) {
+    const int c1 = c0 + 256 < win.p1 ? c0 + 256 : win.p1;
+    const int p = c0 + tid;
+    const int64_t g = J.out + p;
+    const uint64_t base = V.off[J.out + c0];
+    const uint64_t S = V.off[J.out + c1] - base;
+    if (S > PL_VCF_STAGE) {
+      if (p < c1) vcf_locus<true>(V.text + V.off[g], V, J, R, name, name_len, minDepth, g, p);
+      continue;
+    }
+    if (p < c1) vcf_locus<true>(stage + (V.off[g] - base), V, J, R, name, name_len, minDepth, g, p);
+    __syncthreads();
+    char* dst = V.text + base;
+    const uint32_t head = (uint32_t)((4u - (base & 3u)) & 3u) < S ? (uint32_t)((4u - (base & 3u)) & 3u) : (uint32_t)S;
+    if ((uint32_t)tid < head) dst[tid] = stage[tid];
+    const uint32_t nw = (uint32_t)((S - head) >> 2);
+    uint32_t* dw = (uint32_t*)(dst + head);                   // 4-byte aligned
+    for (uint32_t w = tid; w < nw; w += 256) {
+      const uint32_t k = head + 4 * w;
+      dw[w] = (uint32_t)(uint8_t)stage[k] | ((uint32_t)(uint8_t)stage[k + 1] << 8) |
+              ((uint32_t)(uint8_t)stage[k + 2] << 16) | ((uint32_t)(uint8_t)stage[k + 3] << 24);
+    }
+    const uint32_t t0 = head + 4 * nw;
+    if ((uint64_t)(t0 + tid) < S) dst[t0 + tid] = stage[t0 + tid];   // at most 3 tail bytes
+    __syncthreads();
   }
 }
 
@@ -355,4 +398,3 @@ __global__ void __launch_bounds__(256) k_scan64_apply(const uint32_t* in, int64_
 }
 
 template __global__ void k_vcf_lines<false>(VcfArgs);
-template __global__ void k_vcf_lines<true>(VcfArgs);

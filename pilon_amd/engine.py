@@ -157,14 +157,18 @@ class Context:
                                   calls.ctypes.data if copy_out else None, stats if want_stats else None))
         return pile, calls, ([stats[i].as_dict() for i in range(n)] if want_stats else None)
 
-    def batch_vcf(self, diploid=False, vcfqe=False):
+    def batch_vcf(self, diploid=False, vcfqe=False, decode=True):
         """GenomeRegion.writeVcf for every region of the last batch, in batch
         order (Vcf.writeRecord lines, formatted on the device); returns
-        (text, device_ms)."""
+        (text, device_ms), or (bytes of text, lines, device_ms) when decode
+        is False."""
         flags = (VCF_DIPLOID if diploid else 0) | (VCF_QE if vcfqe else 0)
         t, n, ms = C.c_void_p(), C.c_int64(), C.c_double()
         _check(lib().pl_batch_vcf(self.h, flags, C.byref(t), C.byref(n), C.byref(ms)))
         try:
+            if not decode:
+                buf = np.ctypeslib.as_array(C.cast(t, C.POINTER(C.c_uint8)), shape=(n.value,))
+                return n.value, int(np.count_nonzero(buf == 10)), ms.value
             text = C.string_at(t, n.value).decode()
         finally:
             lib().pl_free(t)
