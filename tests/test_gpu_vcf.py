@@ -97,3 +97,35 @@ def test_vcf_synthetic_batch(engine, name, scale, extra, diploid, qe):
     ref = "".join(parts)
     assert got == ref, first_diff(got, ref)
     assert ms > 0
+
+
+def test_vcf_long_names_and_empty_regions(engine):
+    """A contig name of 400 characters makes each 256-locus chunk's text
+    larger than the write pass's LDS stage (the direct-store path); a region
+    with no reads prints LowCov records of empty pileups."""
+    from oracle import oracle as O
+    from pilon_amd.synth import SynthData, preset
+    print("case: vcf long contig name + empty region", flush=True)
+    d = SynthData(preset("c1", scale=0.2))
+    long_name = "contig_" + "x" * 393
+    draft = d.draft(0)
+    engine.ref_upload(500, long_name, draft)
+    rid = engine.readset_upload(500, d.batch(0))
+    from pilon_amd.reads import ReadBatch
+    import numpy as np
+    empty = ReadBatch(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.uint16), np.zeros(0, np.uint8),
+                      np.zeros(1, np.int64), np.zeros(0, np.uint32), np.zeros(1, np.int64), np.zeros(8, np.uint8),
+                      np.zeros(8, np.int8))
+    engine.ref_upload(501, "empty", draft)
+    rid_e = engine.readset_upload(501, empty)
+    L = len(draft)
+    regions = [(500, 1, L, rid), (501, 1, 3000, rid_e), (500, 5000, L, rid)]
+    engine.batch_run(regions, window=0, block=0, persistent=1, copy_out=False, want_stats=False)
+    got, _ = engine.batch_vcf()
+    ref = "".join([
+        O.run_region(engine.config, draft, 1, L, [d.batch(0)], name=long_name, vcf=O.VCF_ON)["vcf"],
+        O.run_region(engine.config, draft, 1, 3000, [empty], name="empty", vcf=O.VCF_ON)["vcf"],
+        O.run_region(engine.config, draft, 5000, L, [d.batch(0)], name=long_name, vcf=O.VCF_ON)["vcf"],
+    ])
+    assert got == ref, first_diff(got, ref)
+    assert max(len(l) for l in got.splitlines()) > 400
