@@ -30,7 +30,7 @@ class OrcResult(C.Structure):
                 ("insertSize", C.POINTER(C.c_int32)), ("weightedQual", C.POINTER(C.c_int8)),
                 ("weightedMq", C.POINTER(C.c_int8)), ("clips", C.POINTER(C.c_int16)),
                 ("stats", PlRegionStats), ("changes_text", C.c_char_p), ("changes_len", C.c_int64),
-                ("vcf_text", C.c_void_p), ("vcf_len", C.c_int64),
+                ("vcf_text", C.c_void_p), ("vcf_len", C.c_int64), ("tracks", C.POINTER(C.c_int32)),
                 ("bases", C.POINTER(C.c_uint8)), ("bases_len", C.c_int64),
                 ("n_indel", C.c_int32), ("indel_pos", C.POINTER(C.c_int32)),
                 ("indel_off", C.POINTER(C.c_int64)), ("indel_str", C.POINTER(C.c_uint8))]
@@ -51,6 +51,8 @@ def lib():
                                      C.c_int32, C.c_char_p, C.c_int64, C.c_int32]
         L.orc_result_free.argtypes = [C.POINTER(OrcResult)]
         L.orc_fmt_af.argtypes = [C.c_float, C.c_char_p]
+        L.orc_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        L.orc_gc.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p]
         L.orc_java_round.restype = C.c_int64
         L.orc_java_round.argtypes = [C.c_double]
         L.orc_roundDivI.restype = C.c_int32
@@ -73,7 +75,9 @@ def _arr(ptr, n, dtype):
     return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
 
 
-VCF_ON, VCF_DIPLOID, VCF_QE = 1, 2, 4
+VCF_ON, VCF_DIPLOID, VCF_QE, TRACKS = 1, 2, 4, 8
+TRACK_NAMES = ["Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverage", "Pct Bad", "Delta Coverage",
+               "Dip Coverage", "Physical Coverage", "Clipped Alignments", "Weighted Qual", "Weighted MQ", "GC"]
 
 
 def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0, vcf=0):
@@ -108,8 +112,10 @@ def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, of
         out["clips"] = _arr(r.clips, size, np.int16)
         out["stats"] = r.stats.as_dict()
         out["changes"] = C.string_at(r.changes_text, r.changes_len).decode() if r.changes_len else ""
-        if vcf:
+        if vcf & VCF_ON:
             out["vcf"] = C.string_at(r.vcf_text, r.vcf_len).decode() if r.vcf_len else ""
+        if vcf & TRACKS:
+            out["tracks"] = _arr(r.tracks, len(TRACK_NAMES) * size, np.int32).reshape(len(TRACK_NAMES), size)
         out["bases"] = bytes(_arr(r.bases, r.bases_len, np.uint8))
         n = r.n_indel
         pos = _arr(r.indel_pos, n, np.int32)
@@ -139,6 +145,20 @@ def fmt_af(af):
     buf = C.create_string_buffer(32)
     lib().orc_fmt_af(C.c_float(af), buf)
     return buf.value.decode()
+
+
+def smooth(values, window=200):
+    v = np.ascontiguousarray(values, dtype=np.int32)
+    out = np.zeros(len(v), np.int32)
+    lib().orc_smooth(v.ctypes.data, len(v), window, out.ctypes.data)
+    return out
+
+
+def gc_track(contig, start, stop):
+    contig = contig.encode() if isinstance(contig, str) else bytes(contig)
+    out = np.zeros(stop - start + 1, np.int8)
+    lib().orc_gc(contig, len(contig), start, stop, out.ctypes.data)
+    return out
 
 
 def java_round(x):

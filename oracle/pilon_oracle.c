@@ -560,6 +560,7 @@ typedef struct orc_result {
   pl_region_stats stats;
   char*    changes_text; int64_t changes_len;
   char*    vcf_text; int64_t vcf_len;   /* Vcf.writeRecord lines of the region (when asked) */
+  int32_t* tracks;           /* [ORC_NTRACKS*size] Tracks.standardTracks values (when asked) */
   uint8_t* bases; int64_t bases_len;
   /* the called indel string at each INS/DEL change, packed: for change c,
    * indel_pos[c] (region index) and indel_str[indel_off[c] .. indel_off[c+1]) */
@@ -570,7 +571,7 @@ void orc_result_free(orc_result* r) {
   if (!r) return;
   free(r->pile); free(r->bcq); free(r->calls); free(r->coverage); free(r->badCoverage);
   free(r->physCoverage); free(r->insertSize); free(r->weightedQual); free(r->weightedMq);
-  free(r->clips); free(r->changes_text); free(r->vcf_text); free(r->bases);
+  free(r->clips); free(r->changes_text); free(r->vcf_text); free(r->tracks); free(r->bases);
   free(r->indel_pos); free(r->indel_off); free(r->indel_str);
   free(r);
 }
@@ -733,6 +734,56 @@ static void vcf_record(VcfOut* o, const PileUp* p, int32_t i, int embedded, int 
   if (indelOk && bc.indel && !embedded) vcf_record(o, p, i, bc.isDel && bc.homoIndel, 0);
 }
 
+/* --------------------------------------------------------------- tracks */
+/* The per-locus tracks of Tracks.standardTracks (Tracks.scala:33-56), in
+ * that order; the Pilon.bed issue track is not among them. */
+#define ORC_NTRACKS 13
+
+/* GenomeRegion.smooth (GenomeRegion.scala:190-210), Int arithmetic.  The
+ * window sum keeps input(0) (the first subtraction drops input(1)), and
+ * result(window - half) is never written before it fills the left edge. */
+static void smooth_int(const int32_t* in, int32_t n, int32_t window, int32_t* out) {
+  const int32_t half = window / 2;
+  int32_t accum = 0;
+  memset(out, 0, sizeof(int32_t) * (size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    accum = jadd(accum, in[i]);
+    if (i > window) {
+      accum = jsub(accum, in[i - window]);
+      out[i - half] = jdiv(jadd(accum, half), window);
+    }
+  }
+  if (n > window) {
+    for (int32_t i = 0; i < window - half; i++) out[i] = out[window - half];
+    for (int32_t i = n - half; i < n; i++) out[i] = out[n - half - 1];
+  } else {
+    for (int32_t i = 0; i < n; i++) out[i] = jdiv(accum, n);
+  }
+}
+
+/* GenomeRegion.computeGc (GenomeRegion.scala:158-188): GC count of a
+ * 100-base circular buffer over the whole contig, stored at the window's
+ * center (and at the last half window's own loci).  The scan runs over
+ * 0-based contig indices but tests and indexes them with the region's
+ * 1-based bounds (inRegion, index), so start/stop here are the 1-based
+ * numbers and gc[k] holds the count when center == start + k. */
+static void compute_gc(const uint8_t* contig, int64_t contig_len, int32_t start, int32_t stop, int8_t* gc) {
+  const int window = 100, halfWindow = (window + 1) / 2;
+  uint8_t buf[100];
+  int32_t count = 0;
+  for (int i = 0; i < window; i++) { buf[i] = (i & 1) == 0; count += buf[i]; }
+  for (int64_t locus = 0; locus < contig_len; locus++) {
+    const int64_t center = locus >= halfWindow ? locus - halfWindow : locus;
+    const int bi = (int)(locus % window);
+    const uint8_t c = contig[locus];
+    const uint8_t g = (c == 'G' || c == 'C') ? 1 : (c == 'A' || c == 'T') ? 0 : buf[bi];
+    count += (int32_t)g - (int32_t)buf[bi];
+    buf[bi] = g;
+    if (center >= start && center <= stop) gc[center - start] = (int8_t)count;
+    if (locus >= start && locus <= stop && locus >= contig_len - halfWindow) gc[locus - start] = (int8_t)count;
+  }
+}
+
 orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
                            const char* name, int32_t start, int32_t stop,
                            const pl_read_soa* const* sets, int32_t n_sets,
@@ -787,8 +838,12 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
     return res;
   }
 
+  /* processBam's fragCoverage (GenomeRegion.scala:287-300): each BAM's depth
+   * increment, before Pass 1 mutates any deletion count */
+  int32_t* fragCov = (int32_t*)calloc((size_t)size, 4);
   /* GenomeRegion.postProcess (GenomeRegion.scala:214-272) */
   pr_computePhysCov(&R);
+  for (int32_t i = 0; i < size; i++) fragCov[i] = (int32_t)pu_depth(&R.pu[i]);
   int64_t meanCoverage = roundDivL(R.baseCount, (int64_t)size);          /* PileUpRegion.coverage */
   int32_t minDepth;
   if (cfg->min_depth >= 1) minDepth = (int32_t)cfg->min_depth;
@@ -837,6 +892,44 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
         }
       }
     }
+  }
+
+  /* Pass 2 (GenomeRegion.scala:273-283) and the tracks (Tracks.scala:33-190) */
+  if (vcf_flags & 8) {
+    int32_t* t = res->tracks = (int32_t*)calloc((size_t)ORC_NTRACKS * (size_t)size, 4);
+    int16_t* copyNumber = (int16_t*)calloc((size_t)size, 2);
+    if (R.readCount != 0) {                                     /* Pass 2 runs after the early return */
+      double sum = 0.0;
+      for (int32_t i = 0; i < size; i++) sum += (double)fragCov[i];   /* NormalDistribution.mean */
+      const double baseCov = sum / (double)size;
+      int32_t* sm = (int32_t*)malloc(sizeof(int32_t) * (size_t)size);
+      smooth_int(fragCov, size, 200, sm);
+      for (int32_t i = 0; i < size; i++)
+        copyNumber[i] = baseCov > 0 ? (int16_t)orc_java_round((double)sm[i] / baseCov) : 0;
+      free(sm);
+    }
+    int8_t* gc = (int8_t*)calloc((size_t)size, 1);
+    compute_gc(contig, contig_len, start, stop, gc);
+    for (int32_t i = 0; i < size; i++) {
+      const int32_t cov = res->coverage[i], bad = res->badCoverage[i];
+      const int32_t l = fragCov[i > 100 ? i - 100 : 0], r = fragCov[i + 100 < size - 1 ? i + 100 : size - 1];
+      const int32_t c = fragCov[i];
+      const int32_t d = jsub(l, r);
+      t[0 * (size_t)size + i] = (kindAt[i] == K_SNP || kindAt[i] == K_INS || kindAt[i] == K_DEL);   /* changed */
+      t[1 * (size_t)size + i] = confirmed[i] ? 0 : 1;
+      t[2 * (size_t)size + i] = (int32_t)copyNumber[i] - 1;
+      t[3 * (size_t)size + i] = cov;
+      t[4 * (size_t)size + i] = bad;
+      t[5 * (size_t)size + i] = jadd(cov, bad) > 0 ? jdiv(jmul(bad, 100), jadd(cov, bad)) : 0;
+      t[6 * (size_t)size + i] = d < 0 ? jsub(0, d) : d;                                        /* delta */
+      t[7 * (size_t)size + i] = jadd(jsub(l, c), jsub(r, c));                                  /* dip */
+      t[8 * (size_t)size + i] = res->physCoverage[i];
+      t[9 * (size_t)size + i] = res->clips[i];
+      t[10 * (size_t)size + i] = res->weightedQual[i];
+      t[11 * (size_t)size + i] = res->weightedMq[i];
+      t[12 * (size_t)size + i] = gc[i];
+    }
+    free(gc); free(copyNumber);
   }
 
   /* final PileUp state and BaseCall per position */
@@ -995,7 +1088,7 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
   }
 
   free(tmp); free(kept); free(snpL.v); free(smallL.v); free(oneChar);
-  free(confirmed); free(deleted); free(kindAt);
+  free(confirmed); free(deleted); free(kindAt); free(fragCov);
   for (int32_t i = 0; i < size; i++) { free(R.pu[i].ins.v); free(R.pu[i].del.v); }
   free(R.pu);
   /* indel strings point into the arena: copied above, so it can go */
@@ -1004,6 +1097,8 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
 }
 
 /* Exposed for known-answer tests of the small helpers. */
+void orc_smooth(const int32_t* in, int32_t n, int32_t window, int32_t* out) { smooth_int(in, n, window, out); }
+void orc_gc(const uint8_t* contig, int64_t len, int32_t start, int32_t stop, int8_t* gc) { compute_gc(contig, len, start, stop, gc); }
 void orc_fmt_af(float af, char* out) { fmt_af(out, af); }
 int32_t orc_roundDivI(int32_t n, int32_t d) { return roundDivI(n, d); }
 int64_t orc_roundDivL(int64_t n, int64_t d) { return roundDivL(n, d); }
