@@ -170,6 +170,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20000.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--vcf", type=int, default=1, help="also format the batch's VCF records on the device")
+    ap.add_argument("--tracks", type=int, default=1, help="also compute the batch's 13 per-locus tracks on the device")
     ap.add_argument("--bam", type=int, default=1,
                     help="also run the BAM producer: the workload written as a BAM, decoded, uploaded and piled")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -291,6 +292,28 @@ def main():
         log(f"bench: VCF: {lines} records, {nbytes / 1e9:.2f} GB in {vms:.1f} ms on the device "
             f"({vcf['records_per_s'] / 1e9:.2f} G records/s), {wall * 1000:.0f} ms with the copy out")
 
+    tracks = None
+    if args.tracks:
+        # the last batch on the context is still the timed one; the first
+        # call also builds each contig's GC buffer values (once per contig)
+        t0 = time.perf_counter()
+        ctx.batch_tracks(copy=False)
+        first_ms = (time.perf_counter() - t0) * 1000.0
+        tms = [ctx.batch_tracks(copy=False)[1] for _ in range(3)]
+        t0 = time.perf_counter()
+        wig, wms = ctx.batch_wig(3)
+        wall = time.perf_counter() - t0
+        tracks = {"loci": G, "device_ms": float(np.mean(tms)), "first_call_wall_ms": first_ms,
+                  "loci_per_s": G / (float(np.mean(tms)) / 1000.0), "wig_coverage_bytes": len(wig),
+                  "wig_device_ms": wms, "wig_wall_ms": wall * 1000.0,
+                  "note": "Tracks.standardTracks values (13 per locus, copy number and GC included) on the device; "
+                          "device_ms covers the fragment-coverage, prefix-sum and value passes; the wig figures "
+                          "are the Coverage track's text (values again, then length, scan and write passes)"}
+        del wig
+        log(f"bench: tracks: {G} loci x 13 in {tracks['device_ms']:.2f} ms on the device "
+            f"({tracks['loci_per_s'] / 1e9:.2f} G loci/s; first call {first_ms:.1f} ms wall with the GC buffers), "
+            f"Coverage.wig {tracks['wig_coverage_bytes'] / 1e6:.0f} MB in {wms:.1f} ms")
+
     producer = None
     if args.bam:
         producer = bam_producer(data, ctx, args, W, B, aligned)
@@ -321,6 +344,7 @@ def main():
         "serving": serving,
         "bam_producer": producer,
         "vcf": vcf,
+        "tracks": tracks,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

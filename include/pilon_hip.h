@@ -244,6 +244,44 @@ pl_status pl_vcf_header(const char* const* names, const int64_t* lengths, int32_
                         const char* date, const char* source, const char* command, const char* reference,
                         char** text, int64_t* len);
 
+/* --------------------------------------------------------------- tracks */
+/* The per-locus tracks of Tracks.standardTracks (Tracks.scala:33-56), in
+ * that order, with Pass 2 of GenomeRegion.postProcess (copy number,
+ * GenomeRegion.scala:273-283) and computeGc (GenomeRegion.scala:160-188)
+ * for every region of the context's last batch.  The Pilon.bed issue track
+ * is not among them (its regions come from the reassembly pass). */
+enum pl_track {
+  PL_TRACK_CHANGES = 0,        /* Changes.wig            */
+  PL_TRACK_UNCONFIRMED,        /* Unconfirmed.wig        */
+  PL_TRACK_COPY_NUMBER,        /* CopyNumber.wig         */
+  PL_TRACK_COVERAGE,           /* Coverage.wig           */
+  PL_TRACK_BAD_COVERAGE,       /* BadCoverage.wig        */
+  PL_TRACK_PCT_BAD,            /* PctBad.wig             */
+  PL_TRACK_DELTA_COVERAGE,     /* DeltaCoverage.wig      */
+  PL_TRACK_DIP_COVERAGE,       /* DipCoverage.wig        */
+  PL_TRACK_PHYSICAL_COVERAGE,  /* PhysicalCoverage.wig   */
+  PL_TRACK_CLIPPED_ALIGNMENTS, /* ClippedAlignments.wig  */
+  PL_TRACK_WEIGHTED_QUAL,      /* WeightedQual.wig       */
+  PL_TRACK_WEIGHTED_MQ,        /* WeightedMq.wig         */
+  PL_TRACK_GC,                 /* GC.wig                 */
+  PL_NTRACKS
+};
+
+/* values[t * G + i] = track t at position i of the batch (G = the batch's
+ * positions, regions in batch order); n_values >= PL_NTRACKS * G.  values
+ * may be NULL to run the passes without copying (device_ms still set). */
+pl_status pl_batch_tracks(pl_ctx* ctx, int32_t* values, int64_t n_values, double* device_ms);
+/* The same for one finished region (as pl_region_vcf). */
+pl_status pl_region_tracks(pl_region* reg, int32_t* values, int64_t n_values, double* device_ms);
+
+/* Tracks.makeTrack (Tracks.scala:160-177) over the last batch's regions:
+ * "fixedStep chrom=<contig> start=<start> step=1" per region, then one value
+ * per line, formatted on the device; PL_WIG_TRACK_LINE puts the file's
+ * "track type=wiggle_0 ..." line first.  *text is malloc'ed (pl_free). */
+#define PL_WIG_TRACK_LINE 0x1u
+pl_status pl_batch_wig(pl_ctx* ctx, int32_t track, uint32_t flags, char** text, int64_t* len, double* device_ms);
+pl_status pl_region_wig(pl_region* reg, int32_t track, uint32_t flags, char** text, int64_t* len, double* device_ms);
+
 /* Sizes and HIP-event times of the last batch (pl_batch_run or
  * pl_region_finish). */
 typedef struct pl_batch_info {

@@ -66,6 +66,7 @@ struct Contig {
   std::vector<uint8_t> host;
   uint8_t* dev = nullptr;
   int64_t len = 0;
+  uint8_t* gcv = nullptr;         // computeGc's buffer value per locus (built on first use)
 };
 
 struct ReadSet {
@@ -915,7 +916,10 @@ void pl_ctx_destroy(pl_ctx* c) {
   (void)hipSetDevice(c->device);
   lane_destroy(&c->main);
   c->rsets.clear();
-  for (auto& kv : c->contigs) if (kv.second.dev) (void)hipFree(kv.second.dev);
+  for (auto& kv : c->contigs) {
+    if (kv.second.dev) (void)hipFree(kv.second.dev);
+    if (kv.second.gcv) (void)hipFree(kv.second.gcv);
+  }
   delete c;
 }
 
@@ -927,6 +931,7 @@ pl_status pl_ref_upload(pl_ctx* c, int32_t contig_id, const char* name, const ui
   HIPCHK(hipSetDevice(c->device));
   Contig& ct = c->contigs[contig_id];
   if (ct.dev) { HIPCHK(hipStreamSynchronize(L->stream)); HIPCHK(hipFree(ct.dev)); ct.dev = nullptr; }
+  if (ct.gcv) { HIPCHK(hipFree(ct.gcv)); ct.gcv = nullptr; }
   ct.name = name ? name : "";
   ct.host.assign(ascii, ascii + len);
   ct.len = len;
@@ -1107,3 +1112,4 @@ pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
 
 #include "pl_serve.hip"
 #include "pl_vcf.hip"
+#include "pl_track.hip"

@@ -1137,3 +1137,4 @@ __global__ void k_ins_strings(EngineArgs A, const Cand* sorted, const int64_t* i
 }
 
 #include "pl_vcf_kernels.hip"
+#include "pl_track_kernels.hip"

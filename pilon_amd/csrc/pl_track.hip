@@ -1,0 +1,221 @@
+// pl_track.hip — the tracks' host side (included at the end of
+// pl_engine.hip, after pl_vcf.hip): Tracks.standardTracks values and
+// Tracks.makeTrack text for the last batch of a lane, computed by the
+// k_gcv_*, k_trk_* and k_wig_lines kernels.
+
+namespace {
+
+const char* const kTrackNames[PL_NTRACKS] = {
+    "Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverage", "Pct Bad", "Delta Coverage",
+    "Dip Coverage", "Physical Coverage", "Clipped Alignments", "Weighted Qual", "Weighted MQ", "GC"};
+
+// computeGc's buffer value per contig locus, built once per uploaded contig
+pl_status contig_gcv(Contig& ct, hipStream_t st) {
+  if (ct.gcv) return PL_OK;
+  const int64_t n = ct.len;
+  const int64_t rows = (n + 99) / 100;
+  const int64_t chunks = (rows + PL_GCV_ROWS - 1) / PL_GCV_ROWS;
+  uint8_t* gcv = nullptr;
+  uint8_t* carry = nullptr;
+  if (hipMalloc((void**)&gcv, (size_t)n) != hipSuccess) return fail(PL_E_OOM, "device allocation of %lld bytes failed", (long long)n);
+  if (hipMalloc((void**)&carry, (size_t)chunks * 100) != hipSuccess) {
+    (void)hipFree(gcv);
+    return fail(PL_E_OOM, "device allocation failed");
+  }
+  const unsigned blocks = (unsigned)((chunks * 100 + 255) / 256);
+  k_gcv_carry<<<dim3(blocks), dim3(256), 0, st>>>(ct.dev, n, chunks, carry);
+  k_gcv_top<<<dim3(1), dim3(128), 0, st>>>(carry, chunks);
+  k_gcv_fill<<<dim3(blocks), dim3(256), 0, st>>>(ct.dev, n, chunks, carry, gcv);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(carry);
+  if (e != hipSuccess) {
+    (void)hipFree(gcv);
+    return fail(PL_E_DEVICE, "GC buffer pass: %s", hipGetErrorString(e));
+  }
+  ct.gcv = gcv;
+  return PL_OK;
+}
+
+// The track values of the lane's last batch into T.vals (device), in B's
+// buffers; contigs[j] is region j's contig.  ms accumulates the device time.
+pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs, VcfBufs& B, TrackArgs& T,
+                         double* ms) {
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = L->stream;
+  const int64_t G = L->G;
+  const int32_t nj = (int32_t)L->hjobs.size();
+  std::memset(&T, 0, sizeof T);
+  T.A = L->args;
+  // the contigs' GC buffers (once per contig) and each region's pointer
+  std::vector<const uint8_t*> gp((size_t)nj);
+  if ((int32_t)contigs.size() != nj) return fail(PL_E_INVAL, "%d contigs for %d regions", (int)contigs.size(), nj);
+  for (int32_t j = 0; j < nj; j++) {
+    Contig& ct = c->contigs.at(contigs[(size_t)j]);
+    if (pl_status s = contig_gcv(ct, st)) return s;
+    gp[(size_t)j] = ct.gcv;
+  }
+  const uint8_t** dgp;
+  pl_status s;
+  if ((s = B.get(&dgp, sizeof(void*) * (size_t)nj)) || (s = B.get(&T.frag, 4 * (size_t)G)) ||
+      (s = B.get(&T.jsum, 8 * (size_t)nj)) || (s = B.get(&T.vals, 4 * (size_t)PL_NTRACKS * (size_t)G)))
+    return s;
+  uint64_t* fpre;
+  if ((s = B.get(&fpre, 8 * (size_t)(G + 1)))) return s;
+  const int64_t tiles = std::max<int64_t>(1, (G + SCAN_TILE - 1) / SCAN_TILE);
+  uint64_t* tsum;
+  if ((s = B.get(&tsum, 8 * (size_t)tiles))) return s;
+  T.gcv = dgp;
+  T.fpre = fpre;
+  HIPCHK(hipMemcpyAsync(dgp, gp.data(), sizeof(void*) * (size_t)nj, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(T.jsum, 0, 8 * (size_t)nj, st));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  k_trk_frag<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
+  k_scan64_tiles<<<dim3((unsigned)tiles), dim3(256), 0, st>>>((const uint32_t*)T.frag, G, tsum);
+  k_scan64_top<<<dim3(1), dim3(1024), 0, st>>>(tsum, tiles);
+  k_scan64_apply<<<dim3((unsigned)tiles), dim3(256), 0, st>>>((const uint32_t*)T.frag, G, tsum, fpre);
+  k_trk_values<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms += t;
+  return PL_OK;
+}
+
+pl_status tracks_copy(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs, int32_t* values, int64_t n_values,
+                      double* device_ms) {
+  const int64_t G = L->G;
+  if (values && n_values < (int64_t)PL_NTRACKS * G)
+    return fail(PL_E_INVAL, "values holds %lld entries, the batch needs %lld", (long long)n_values,
+                (long long)PL_NTRACKS * G);
+  VcfBufs B;
+  TrackArgs T;
+  double ms = 0;
+  if (pl_status s = tracks_of_lane(c, L, contigs, B, T, &ms)) return s;
+  if (values) {
+    HIPCHK(hipMemcpyAsync(values, T.vals, 4 * (size_t)PL_NTRACKS * (size_t)G, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+  }
+  if (device_ms) *device_ms = ms;
+  return PL_OK;
+}
+
+pl_status wig_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, const std::vector<int32_t>& contigs,
+                      int32_t track, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  *text = nullptr;
+  *len = 0;
+  if (track < 0 || track >= PL_NTRACKS) return fail(PL_E_INVAL, "track %d is not in [0, %d)", track, PL_NTRACKS);
+  const int64_t G = L->G;
+  const int32_t nj = (int32_t)L->hjobs.size();
+  hipStream_t st = L->stream;
+  VcfBufs B;
+  TrackArgs T;
+  double ms = 0;
+  if (pl_status s = tracks_of_lane(c, L, contigs, B, T, &ms)) return s;
+  std::string names;
+  std::vector<int64_t> noff(1, 0);
+  for (int32_t j = 0; j < nj; j++) {
+    names += rnames[(size_t)j];
+    noff.push_back((int64_t)names.size());
+  }
+  char* dnames;
+  int64_t* dnoff;
+  uint64_t* off;
+  pl_status s;
+  if ((s = B.get(&dnames, names.size())) || (s = B.get(&dnoff, 8 * noff.size())) || (s = B.get(&T.len, 4 * (size_t)G)) ||
+      (s = B.get(&off, 8 * (size_t)(G + 1))))
+    return s;
+  const int64_t tiles = std::max<int64_t>(1, (G + SCAN_TILE - 1) / SCAN_TILE);
+  uint64_t* tsum;
+  if ((s = B.get(&tsum, 8 * (size_t)tiles))) return s;
+  HIPCHK(hipMemcpyAsync(dnames, names.data(), names.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dnoff, noff.data(), 8 * noff.size(), hipMemcpyHostToDevice, st));
+  T.track = track;
+  T.names = dnames;
+  T.name_off = dnoff;
+  T.off = off;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  k_wig_lines<false><<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
+  k_scan64_tiles<<<dim3((unsigned)tiles), dim3(256), 0, st>>>(T.len, G, tsum);
+  k_scan64_top<<<dim3(1), dim3(1024), 0, st>>>(tsum, tiles);
+  k_scan64_apply<<<dim3((unsigned)tiles), dim3(256), 0, st>>>(T.len, G, tsum, off);
+  HIPCHK(hipGetLastError());
+  uint64_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, off + G, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  char* dtext;
+  if ((s = B.get(&dtext, (size_t)total))) return s;
+  T.text = dtext;
+  k_wig_lines<true><<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(e1, st));
+  std::string head;
+  if (flags & PL_WIG_TRACK_LINE) {
+    head = std::string("track type=wiggle_0 graphType=line color=0,0,255 altColor=255,0,0 name=\"") +
+           kTrackNames[track] + "\"";
+    if (track == PL_TRACK_GC) head += " graphType=heatmap midRange=35:65 midColor=0,255,0";
+    head += '\n';
+  }
+  char* out = (char*)malloc(head.size() + (size_t)total + 1);
+  if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
+  std::memcpy(out, head.data(), head.size());
+  HIPCHK(hipMemcpyAsync(out + head.size(), dtext, (size_t)total, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  out[head.size() + total] = 0;
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (device_ms) *device_ms = ms + t;
+  *text = out;
+  *len = (int64_t)(head.size() + total);
+  return PL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pl_status pl_batch_tracks(pl_ctx* c, int32_t* values, int64_t n_values, double* device_ms) {
+  if (!c) return fail(PL_E_INVAL, "null argument");
+  std::vector<std::string> names;
+  std::vector<int32_t> contigs;
+  if (pl_status s = batch_names(c, &c->main, &names, &contigs)) return s;
+  return tracks_copy(c, &c->main, contigs, values, n_values, device_ms);
+}
+
+pl_status pl_region_tracks(pl_region* r, int32_t* values, int64_t n_values, double* device_ms) {
+  if (!r) return fail(PL_E_INVAL, "null argument");
+  return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>&,
+                               const std::vector<int32_t>& contigs) {
+    return tracks_copy(c, L, contigs, values, n_values, device_ms);
+  });
+}
+
+pl_status pl_batch_wig(pl_ctx* c, int32_t track, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!c || !text || !len) return fail(PL_E_INVAL, "null argument");
+  std::vector<std::string> names;
+  std::vector<int32_t> contigs;
+  if (pl_status s = batch_names(c, &c->main, &names, &contigs)) return s;
+  return wig_of_lane(c, &c->main, names, contigs, track, flags, text, len, device_ms);
+}
+
+pl_status pl_region_wig(pl_region* r, int32_t track, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
+  return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
+                               const std::vector<int32_t>& contigs) {
+    return wig_of_lane(c, L, names, contigs, track, flags, text, len, device_ms);
+  });
+}
+
+}  // extern "C"

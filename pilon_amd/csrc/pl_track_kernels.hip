@@ -1,0 +1,296 @@
+// pl_track_kernels.hip — the per-locus tracks of Tracks.standardTracks
+// (Tracks.scala:33-56) with Pass 2 of GenomeRegion.postProcess
+// (GenomeRegion.scala:273-283) for a batch's regions, on the device.
+// Included at the end of pl_kernels.hip.
+//
+//   k_gcv_carry/top/fill  once per contig: the value each contig locus
+//                 leaves in computeGc's 100-slot circular buffer (1 for G/C,
+//                 0 for A/T, else the slot's previous value: the value of
+//                 the locus 100 before, or the buffer's initial 1,0,1,0...),
+//                 as a scan down each of the 100 residue columns;
+//   k_trk_frag    fragment coverage per locus (processBam's depth before
+//                 Pass 1 added a called deletion's count to the loci under
+//                 it) and its sum per region;
+//   k_trk_values  the 13 values per locus: copy number from the smoothed
+//                 fragment coverage (its 64-bit prefix sums) over the
+//                 region's mean, GC as a window sum of the buffer values
+//                 (prefix sums staged in LDS), the rest from the pileup;
+//   k_wig_lines   one track's text (length pass, then write pass).
+
+#define PL_GCV_ROWS 1024     // contig rows (of 100 loci) per column-scan chunk
+#define PL_GC_STAGE (2048 + 149)
+
+struct TrackArgs {
+  EngineArgs A;
+  int32_t* frag;               // [G] fragment coverage
+  const uint64_t* fpre;        // [G + 1] exclusive prefix sums of frag (mod 2^64)
+  int64_t* jsum;               // [n_jobs] sum of frag per region
+  const uint8_t* const* gcv;   // [n_jobs] the region's contig buffer values
+  int32_t* vals;               // [PL_NTRACKS][G]
+  // one track's wig text
+  int32_t track;
+  const char* names;           // region (contig) names, concatenated
+  const int64_t* name_off;     // [n_jobs + 1]
+  uint32_t* len;               // [G] bytes per locus
+  const uint64_t* off;         // [G + 1]
+  char* text;
+};
+
+namespace {
+
+// 1 for G/C, 0 for A/T, 2 for any other byte (GenomeRegion.scala:173-179)
+__device__ __forceinline__ uint32_t gc_class(uint8_t c) {
+  return (c == 'G' || c == 'C') ? 1u : (c == 'A' || c == 'T') ? 0u : 2u;
+}
+
+// java.lang.Math.round(double) for finite arguments within Long range
+__device__ __forceinline__ int64_t java_round_dev(double x) {
+  const double f = floor(x);
+  return (int64_t)f + ((x - f) >= 0.5 ? 1 : 0);
+}
+
+// GenomeRegion.smooth(fragCoverage, 200)(r) (GenomeRegion.scala:190-210)
+// from prefix sums, Int arithmetic: result(r) for 101 <= r <= n-101 is
+// (input(0) + sum input(r-99 .. r+100) + 100) / 200 (the first subtraction
+// drops input(1), so input(0) stays in the sum); result(0..100) is never
+// written (0), result(n-100 .. n-1) repeat result(n-101); n <= 200 gives
+// sum / n everywhere.
+__device__ __forceinline__ int32_t smooth_at(const TrackArgs& T, const JobDev& J, int32_t r) {
+  const int32_t n = J.size;
+  const uint64_t* P = T.fpre + J.out;
+  if (n <= 200) return j_div((int32_t)(uint32_t)(P[n] - P[0]), n);
+  if (r >= n - 100) r = n - 101;
+  if (r <= 100) return 0;
+  const int32_t acc = j_add(T.frag[J.out], (int32_t)(uint32_t)(P[r + 101] - P[r - 99]));
+  return j_div(j_add(acc, 100), 200);
+}
+
+template <bool WRITE>
+struct WOut {
+  char* p;
+  uint32_t n;
+  __device__ void c(char x) {
+    if (WRITE) p[n] = x;
+    n++;
+  }
+  __device__ void s(const char* t) {
+    for (; *t; t++) c(*t);
+  }
+  __device__ void i(int32_t v) {                           // Integer.toString
+    uint32_t u = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+    if (v < 0) c('-');
+    uint64_t packed = 0;
+    int nd = 0;
+    do {
+      packed |= (uint64_t)(u % 10u) << (4 * nd);
+      u /= 10u;
+      nd++;
+    } while (u);
+    for (int k = nd - 1; k >= 0; k--) c((char)('0' + ((packed >> (4 * k)) & 15u)));
+  }
+};
+
+// one locus of a wig track, preceded at region index 0 by the region's
+// fixedStep line (Tracks.makeTrack, Tracks.scala:168-176)
+template <bool WRITE>
+__device__ uint32_t wig_locus(char* out, const TrackArgs& T, int32_t job, const JobDev& J, int64_t g, int32_t r) {
+  WOut<WRITE> o;
+  o.p = out;
+  o.n = 0;
+  if (r == 0) {
+    o.s("fixedStep chrom=");
+    for (int64_t k = T.name_off[job]; k < T.name_off[job + 1]; k++) o.c(T.names[k]);
+    o.s(" start=");
+    o.i(J.start);
+    o.s(" step=1\n");
+  }
+  o.i(T.vals[(int64_t)T.track * T.A.G + g]);
+  o.c('\n');
+  return o.n;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- GC buffer
+// Locus k of the contig is column k % 100, row k / 100.  One thread per
+// (chunk of PL_GCV_ROWS rows, column): adjacent threads read adjacent loci.
+// carry = the chunk's last defined value in the column (2: none).
+__global__ void __launch_bounds__(256) k_gcv_carry(const uint8_t* ref, int64_t n, int64_t chunks, uint8_t* carry) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= chunks * 100) return;
+  const int64_t ch = t / 100, col = t % 100;
+  const int64_t rows = (n + 99) / 100;
+  const int64_t r0 = ch * PL_GCV_ROWS, r1 = r0 + PL_GCV_ROWS < rows ? r0 + PL_GCV_ROWS : rows;
+  uint32_t v = 2;
+  for (int64_t r = r1 - 1; r >= r0 && v == 2; r--) {
+    const int64_t k = r * 100 + col;
+    if (k < n) v = gc_class(ref[k]);
+  }
+  carry[t] = (uint8_t)v;
+}
+
+// one thread per column: the value entering each chunk, starting from the
+// buffer's initial value (slot `col` holds 1 when col is even)
+__global__ void __launch_bounds__(128) k_gcv_top(uint8_t* carry, int64_t chunks) {
+  const int col = threadIdx.x;
+  if (col >= 100) return;
+  uint8_t run = (col & 1) == 0;
+  for (int64_t ch = 0; ch < chunks; ch++) {
+    const uint8_t c = carry[ch * 100 + col];
+    carry[ch * 100 + col] = run;
+    if (c != 2) run = c;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gcv_fill(const uint8_t* ref, int64_t n, int64_t chunks, const uint8_t* carry,
+                                                  uint8_t* gcv) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= chunks * 100) return;
+  const int64_t ch = t / 100, col = t % 100;
+  const int64_t rows = (n + 99) / 100;
+  const int64_t r0 = ch * PL_GCV_ROWS, r1 = r0 + PL_GCV_ROWS < rows ? r0 + PL_GCV_ROWS : rows;
+  uint32_t run = carry[t];
+  for (int64_t r = r0; r < r1; r++) {
+    const int64_t k = r * 100 + col;
+    if (k >= n) break;
+    const uint32_t v = gc_class(ref[k]);
+    if (v != 2) run = v;
+    gcv[k] = (uint8_t)run;
+  }
+}
+
+// ------------------------------------------------------------ fragment cov
+__global__ void __launch_bounds__(256) k_trk_frag(TrackArgs T) {
+  const EngineArgs& A = T.A;
+  const WinDev win = A.wins[blockIdx.x];
+  const JobDev& J = A.jobs[win.job];
+  const int64_t G = A.G;
+  int64_t s = 0;
+  for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
+    const int64_t g = J.out + p;
+    int64_t d = 0;
+    for (int b = 0; b < 4; b++) d += (uint32_t)A.pile[(int64_t)(PL_F_COUNT_A + b) * G + g];
+    int32_t fc = (int32_t)(d + A.pile[(int64_t)PL_F_DELETIONS * G + g]);
+    if (A.calls[g].flags & PL_CF_DELETED) {
+      // the deletion's caller: the nearest locus to the left that Pass 1 did
+      // not delete (GenomeRegion.scala:259-264); its count was added here
+      int64_t k = g - 1;
+      while (k > J.out && (A.calls[k].flags & PL_CF_DELETED)) k--;
+      fc = j_sub(fc, A.pile[(int64_t)PL_F_DELETIONS * G + k]);
+    }
+    T.frag[g] = fc;
+    s += fc;
+  }
+  s = wave_sum(s);
+  __shared__ int64_t ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t t = ws[0] + ws[1] + ws[2] + ws[3];
+    if (t) atomicAdd((unsigned long long*)&T.jsum[win.job], (unsigned long long)t);
+  }
+}
+
+// ------------------------------------------------------------ track values
+__global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
+  __shared__ int32_t P[PL_GC_STAGE + 1];
+  __shared__ int32_t ws[4];
+  const EngineArgs& A = T.A;
+  const WinDev win = A.wins[blockIdx.x];
+  const JobDev& J = A.jobs[win.job];
+  const JobStat& S = A.st[win.job];
+  const RSDev& R = A.rs[J.rs];
+  const uint8_t* gcv = T.gcv[win.job];
+  const int64_t n = R.ref_len;
+  const int64_t G = A.G;
+
+  // computeGc (GenomeRegion.scala:160-188) tests and indexes the 0-based
+  // contig loci with the region's 1-based bounds, so region index p holds
+  // the count for center c = start + p (0-based): the buffer after locus
+  // c + 50 (after locus c itself in the contig's last 50 loci; none when
+  // c is past the contig).  The counts need buffer values at loci
+  // [start + p0 - 99, start + p1 + 50); their prefix sums go to LDS.
+  const int64_t base = (int64_t)J.start + win.p0 - 99;
+  const int32_t SN = (win.p1 - win.p0) + 149;
+  {
+    const int32_t per = (SN + 255) / 256;
+    const int32_t i0 = threadIdx.x * per, i1 = i0 + per < SN ? i0 + per : SN;
+    int32_t s = 0;
+    for (int32_t i = i0; i < i1; i++) {
+      const int64_t k = base + i;
+      s += k < 0 ? (int32_t)((k & 1) == 0) : k < n ? (int32_t)gcv[k] : 0;
+    }
+    const int32_t inc = wave_incl_scan(s);
+    if ((threadIdx.x & 63) == 63) ws[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int32_t acc = inc - s;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) acc += ws[w];
+    if (threadIdx.x == 0) P[0] = 0;
+    for (int32_t i = i0; i < i1; i++) {
+      const int64_t k = base + i;
+      acc += k < 0 ? (int32_t)((k & 1) == 0) : k < n ? (int32_t)gcv[k] : 0;
+      P[i + 1] = acc;
+    }
+    __syncthreads();
+  }
+
+  const bool reads = S.read_count != 0;
+  const double baseCov = (double)T.jsum[win.job] / (double)J.size;       // NormalDistribution.mean
+  for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
+    const int64_t g = J.out + p;
+    const Col c = load_col(A.pile, G, g);
+    const uint16_t f = A.calls[g].flags;
+    const int64_t dep = c.depth();
+    // the final disposition (as k_final_flags)
+    int disp = (f & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT;
+    if (!reads || (f & PL_CF_DELETED) || dep < S.min_depth) disp = PL_D_NONE;
+    // processBam leaves the pileup tracks at 0 in a region without reads
+    const int32_t cov = reads ? (int32_t)dep : 0;
+    const int32_t bad = reads ? A.pile[(int64_t)PL_F_BADPAIR * G + g] : 0;
+    int16_t cn = 0;
+    if (reads && baseCov > 0) cn = (int16_t)java_round_dev((double)smooth_at(T, J, p) / baseCov);
+    const int32_t sz = J.size;
+    // deltaCoverage / dipCoverage at radius 100 (GenomeRegion.scala:672-693)
+    const int32_t l = T.frag[J.out + (p > 100 ? p - 100 : 0)];
+    const int32_t rr = T.frag[J.out + (p + 100 < sz - 1 ? p + 100 : sz - 1)];
+    const int32_t ce = T.frag[g];
+    const int32_t d = j_sub(l, rr);
+    const int64_t qsum = c.qs[0] + c.qs[1] + c.qs[2] + c.qs[3];
+    const int64_t cc = (int64_t)J.start + p;
+    int32_t gc = 0;
+    if (cc < n) {
+      const int64_t L = cc >= n - 50 ? cc : cc + 50;
+      gc = (int32_t)(int8_t)(P[L - base + 1] - P[L - 99 - base]);
+    }
+    int32_t v[PL_NTRACKS];
+    v[0] = (disp == PL_D_SNP || disp == PL_D_INS || disp == PL_D_DEL) ? 1 : 0;   // Changes
+    v[1] = disp == PL_D_CONFIRMED ? 0 : 1;                                       // Unconfirmed
+    v[2] = (int32_t)cn - 1;                                                      // Copy Number
+    v[3] = cov;                                                                  // Coverage
+    v[4] = bad;                                                                  // Bad Coverage
+    v[5] = j_add(cov, bad) > 0 ? j_div(j_mul(bad, 100), j_add(cov, bad)) : 0;   // Pct Bad
+    v[6] = d < 0 ? j_sub(0, d) : d;                                              // Delta Coverage
+    v[7] = j_add(j_sub(l, ce), j_sub(rr, ce));                                   // Dip Coverage
+    v[8] = reads ? A.pile[(int64_t)PL_F_PHYSCOV * G + g] : 0;                    // Physical Coverage
+    v[9] = reads ? (int32_t)(int16_t)A.pile[(int64_t)PL_F_CLIPS * G + g] : 0;    // Clipped Alignments
+    v[10] = reads ? (int32_t)(int8_t)round_div_l(qsum, (int64_t)c.mqSum) : 0;   // Weighted Qual (PileUp.scala:60-62)
+    v[11] = reads ? (int32_t)(int8_t)round_div_l(qsum, (int64_t)c.qSum) : 0;    // Weighted MQ (PileUp.scala:56-58)
+    v[12] = gc;                                                                  // GC
+#pragma unroll
+    for (int t = 0; t < PL_NTRACKS; t++) T.vals[(int64_t)t * G + g] = v[t];
+  }
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(256) k_wig_lines(TrackArgs T) {
+  const WinDev win = T.A.wins[blockIdx.x];
+  const JobDev& J = T.A.jobs[win.job];
+  for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
+    const int64_t g = J.out + p;
+    if (WRITE) wig_locus<true>(T.text + T.off[g], T, win.job, J, g, p);
+    else T.len[g] = wig_locus<false>(nullptr, T, win.job, J, g, p);
+  }
+}
+
+template __global__ void k_wig_lines<false>(TrackArgs);
+template __global__ void k_wig_lines<true>(TrackArgs);

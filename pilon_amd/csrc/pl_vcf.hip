@@ -163,25 +163,12 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   return PL_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-pl_status pl_batch_vcf(pl_ctx* c, uint32_t flags, char** text, int64_t* len, double* device_ms) {
-  if (!c || !text || !len) return fail(PL_E_INVAL, "null argument");
-  Lane* L = &c->main;
-  if (!L->tables_valid || L->hjobs.empty() || L->n_wins <= 0) return fail(PL_E_INVAL, "no batch to write");
-  std::vector<std::string> names;
-  for (const JobDev& J : L->hjobs) {
-    if (J.rs < 0 || J.rs >= (int32_t)c->rsets.size() || !c->rsets[J.rs])
-      return fail(PL_E_INVAL, "the last batch's reads were released (a finished region: use pl_region_vcf)");
-    names.push_back(c->contigs.at(c->rsets[J.rs]->contig).name);
-  }
-  return vcf_of_lane(c, L, names, flags, text, len, device_ms);
-}
-
-pl_status pl_region_vcf(pl_region* r, uint32_t flags, char** text, int64_t* len, double* device_ms) {
-  if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
+// f(ctx, lane, region names, region contigs) on a lane whose last batch is the finished
+// region r: the context's own lane if it still holds r's batch, else r's
+// batch runs there again (its read set goes back into the context's table
+// for the run).
+template <class F>
+pl_status on_region_lane(pl_region* r, F&& f) {
   if (!r->finished) return fail(PL_E_INVAL, "region not finished (pl_region_finish first)");
   pl_ctx* c = r->ctx;
   Lane* L = &c->main;
@@ -189,16 +176,47 @@ pl_status pl_region_vcf(pl_region* r, uint32_t flags, char** text, int64_t* len,
   const std::vector<JobReq> q{{id, r->start, r->stop}};
   const bool current = L->tables_valid && L->last_reqs == q && L->hjobs.size() == 1;
   std::vector<std::string> names{c->contigs.at(r->contig).name};
-  if (current) return vcf_of_lane(c, L, names, flags, text, len, device_ms);
-  // the lane has run other batches since: run this region again (its read
-  // set goes back into the context's table for the run)
+  const std::vector<int32_t> contigs{r->contig};
+  if (current) return f(c, L, names, contigs);
   HIPCHK(hipSetDevice(c->device));
   if (r->temp) { c->rsets[id] = std::move(r->temp); c->rs_version++; }
   pl_status s = run_jobs(c, L, q, PL_DEFAULT_WINDOW, 0, PL_DEFAULT_BLOCK, false);
   if (!s) s = fetch_batch_meta(L);
-  if (!s) s = vcf_of_lane(c, L, names, flags, text, len, device_ms);
+  if (!s) s = f(c, L, names, contigs);
   if (r->temp_id >= 0 && !r->temp) { r->temp = std::move(c->rsets[id]); c->rs_version++; }
   return s;
+}
+
+// the region names and contigs of the lane's last batch (PL_E_INVAL when a
+// finished region's reads were released)
+pl_status batch_names(pl_ctx* c, Lane* L, std::vector<std::string>* names, std::vector<int32_t>* contigs = nullptr) {
+  if (!L->tables_valid || L->hjobs.empty() || L->n_wins <= 0) return fail(PL_E_INVAL, "no batch on the device");
+  for (const JobDev& J : L->hjobs) {
+    if (J.rs < 0 || J.rs >= (int32_t)c->rsets.size() || !c->rsets[J.rs])
+      return fail(PL_E_INVAL, "the last batch's reads were released (a finished region: use the pl_region_* form)");
+    names->push_back(c->contigs.at(c->rsets[J.rs]->contig).name);
+    if (contigs) contigs->push_back(c->rsets[J.rs]->contig);
+  }
+  return PL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pl_status pl_batch_vcf(pl_ctx* c, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!c || !text || !len) return fail(PL_E_INVAL, "null argument");
+  std::vector<std::string> names;
+  if (pl_status s = batch_names(c, &c->main, &names)) return s;
+  return vcf_of_lane(c, &c->main, names, flags, text, len, device_ms);
+}
+
+pl_status pl_region_vcf(pl_region* r, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
+  return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
+                                const std::vector<int32_t>&) {
+    return vcf_of_lane(c, L, names, flags, text, len, device_ms);
+  });
 }
 
 }  // extern "C"

@@ -28,6 +28,15 @@ _lib = None
 
 VCF_DIPLOID, VCF_QE = 0x2, 0x4
 
+# Tracks.standardTracks (Tracks.scala:33-56): name and file of each track
+TRACK_NAMES = ("Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverage", "Pct Bad", "Delta Coverage",
+               "Dip Coverage", "Physical Coverage", "Clipped Alignments", "Weighted Qual", "Weighted MQ", "GC")
+TRACK_FILES = ("Changes.wig", "Unconfirmed.wig", "CopyNumber.wig", "Coverage.wig", "BadCoverage.wig", "PctBad.wig",
+               "DeltaCoverage.wig", "DipCoverage.wig", "PhysicalCoverage.wig", "ClippedAlignments.wig",
+               "WeightedQual.wig", "WeightedMq.wig", "GC.wig")
+NTRACKS = len(TRACK_NAMES)
+WIG_TRACK_LINE = 0x1
+
 
 def vcf_header(contigs, date, source, command, reference, vcfqe=False):
     """Vcf.writeHeader (Vcf.scala:28-70) for [(name, length)] contigs."""
@@ -39,6 +48,15 @@ def vcf_header(contigs, date, source, command, reference, vcfqe=False):
                                command.encode(), reference.encode(), C.byref(t), C.byref(ln)))
     try:
         return C.string_at(t, ln.value).decode()
+    finally:
+        lib().pl_free(t)
+
+
+def _wig(fn, h, track, track_line):
+    t, n, ms = C.c_void_p(), C.c_int64(), C.c_double()
+    _check(fn(h, int(track), WIG_TRACK_LINE if track_line else 0, C.byref(t), C.byref(n), C.byref(ms)))
+    try:
+        return C.string_at(t, n.value).decode(), ms.value
     finally:
         lib().pl_free(t)
 
@@ -83,6 +101,12 @@ def lib():
             "pl_serve_close": (None, [C.c_void_p]),
             "pl_batch_vcf": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
             "pl_region_vcf": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
+            "pl_batch_tracks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_double)]),
+            "pl_region_tracks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_double)]),
+            "pl_batch_wig": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(C.c_void_p), P(C.c_int64),
+                                       P(C.c_double)]),
+            "pl_region_wig": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(C.c_void_p), P(C.c_int64),
+                                        P(C.c_double)]),
             "pl_vcf_header": (C.c_int, [P(C.c_char_p), P(C.c_int64), C.c_int32, C.c_uint32, C.c_char_p, C.c_char_p,
                                         C.c_char_p, C.c_char_p, P(C.c_void_p), P(C.c_int64)]),
         }
@@ -173,6 +197,22 @@ class Context:
         finally:
             lib().pl_free(t)
         return text, ms.value
+
+    def batch_tracks(self, copy=True):
+        """Tracks.standardTracks values of the last batch: ([NTRACKS, G] int32
+        in batch order, or None when copy is False; device_ms)."""
+        ms = C.c_double()
+        vals = None
+        if copy:
+            vals = np.empty((NTRACKS, self.batch_info()["positions"]), np.int32)
+        _check(lib().pl_batch_tracks(self.h, vals.ctypes.data if copy else None, vals.size if copy else 0,
+                                     C.byref(ms)))
+        return vals, ms.value
+
+    def batch_wig(self, track, track_line=True):
+        """Tracks.makeTrack text of one track over the last batch's regions:
+        (text, device_ms)."""
+        return _wig(lib().pl_batch_wig, self.h, track, track_line)
 
     def batch_info(self):
         info = PlBatchInfo()
@@ -341,6 +381,18 @@ class PileUpRegion:
             return C.string_at(t, n.value).decode(), ms.value
         finally:
             lib().pl_free(t)
+
+    def tracks(self):
+        """Tracks.standardTracks values of this finished region: ([NTRACKS,
+        size] int32, device_ms)."""
+        vals = np.empty((NTRACKS, self.stop - self.start + 1), np.int32)
+        ms = C.c_double()
+        _check(lib().pl_region_tracks(self.h, vals.ctypes.data, vals.size, C.byref(ms)))
+        return vals, ms.value
+
+    def wig(self, track, track_line=True):
+        """Tracks.makeTrack text of one track over this region: (text, device_ms)."""
+        return _wig(lib().pl_region_wig, self.h, track, track_line)
 
     def fix(self, new_name=None, offset=0):
         """identifyAndFixIssues + writeChanges: returns (changes text, fixed bases)."""
