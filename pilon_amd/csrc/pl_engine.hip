@@ -66,7 +66,10 @@ struct Contig {
   std::vector<uint8_t> host;
   uint8_t* dev = nullptr;
   int64_t len = 0;
-  uint8_t* gcv = nullptr;         // computeGc's buffer value per locus (built on first use)
+  // computeGc's buffer value per locus (built on first use), in a device
+  // block shared by the contigs built together
+  std::shared_ptr<uint8_t> gcv_mem;
+  uint8_t* gcv = nullptr;
 };
 
 struct ReadSet {
@@ -916,10 +919,8 @@ void pl_ctx_destroy(pl_ctx* c) {
   (void)hipSetDevice(c->device);
   lane_destroy(&c->main);
   c->rsets.clear();
-  for (auto& kv : c->contigs) {
-    if (kv.second.dev) (void)hipFree(kv.second.dev);
-    if (kv.second.gcv) (void)hipFree(kv.second.gcv);
-  }
+  for (auto& kv : c->contigs) if (kv.second.dev) (void)hipFree(kv.second.dev);
+  c->contigs.clear();
   delete c;
 }
 
@@ -931,7 +932,8 @@ pl_status pl_ref_upload(pl_ctx* c, int32_t contig_id, const char* name, const ui
   HIPCHK(hipSetDevice(c->device));
   Contig& ct = c->contigs[contig_id];
   if (ct.dev) { HIPCHK(hipStreamSynchronize(L->stream)); HIPCHK(hipFree(ct.dev)); ct.dev = nullptr; }
-  if (ct.gcv) { HIPCHK(hipFree(ct.gcv)); ct.gcv = nullptr; }
+  ct.gcv_mem.reset();
+  ct.gcv = nullptr;
   ct.name = name ? name : "";
   ct.host.assign(ascii, ascii + len);
   ct.len = len;

@@ -9,31 +9,57 @@ const char* const kTrackNames[PL_NTRACKS] = {
     "Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverage", "Pct Bad", "Delta Coverage",
     "Dip Coverage", "Physical Coverage", "Clipped Alignments", "Weighted Qual", "Weighted MQ", "GC"};
 
-// computeGc's buffer value per contig locus, built once per uploaded contig
-pl_status contig_gcv(Contig& ct, hipStream_t st) {
-  if (ct.gcv) return PL_OK;
-  const int64_t n = ct.len;
-  const int64_t rows = (n + 99) / 100;
-  const int64_t chunks = (rows + PL_GCV_ROWS - 1) / PL_GCV_ROWS;
-  uint8_t* gcv = nullptr;
-  uint8_t* carry = nullptr;
-  if (hipMalloc((void**)&gcv, (size_t)n) != hipSuccess) return fail(PL_E_OOM, "device allocation of %lld bytes failed", (long long)n);
-  if (hipMalloc((void**)&carry, (size_t)chunks * 100) != hipSuccess) {
-    (void)hipFree(gcv);
-    return fail(PL_E_OOM, "device allocation failed");
+// computeGc's buffer values of the listed contigs that have none yet, built
+// together in one device block
+pl_status build_gcv(pl_ctx* c, const std::vector<int32_t>& ids, hipStream_t st) {
+  std::vector<int32_t> miss;
+  for (int32_t id : ids)
+    if (!c->contigs.at(id).gcv) miss.push_back(id);
+  std::sort(miss.begin(), miss.end());
+  miss.erase(std::unique(miss.begin(), miss.end()), miss.end());
+  if (miss.empty()) return PL_OK;
+  std::vector<GcvJob> jobs;
+  std::vector<size_t> off;
+  size_t bytes = 0;
+  int64_t nt = 0;
+  for (int32_t id : miss) {
+    const Contig& ct = c->contigs.at(id);
+    GcvJob g;
+    g.ref = ct.dev;
+    g.gcv = nullptr;
+    g.n = ct.len;
+    const int64_t rows = (ct.len + 99) / 100;
+    g.rpc = std::max<int64_t>(PL_GCV_MIN_ROWS, (rows + PL_GCV_MAX_CHUNKS - 1) / PL_GCV_MAX_CHUNKS);
+    g.chunks = (rows + g.rpc - 1) / g.rpc;
+    g.t0 = nt;
+    nt += g.chunks * 100;
+    off.push_back(bytes);
+    bytes += ((size_t)ct.len + 255) & ~(size_t)255;
+    jobs.push_back(g);
   }
-  const unsigned blocks = (unsigned)((chunks * 100 + 255) / 256);
-  k_gcv_carry<<<dim3(blocks), dim3(256), 0, st>>>(ct.dev, n, chunks, carry);
-  k_gcv_top<<<dim3(1), dim3(128), 0, st>>>(carry, chunks);
-  k_gcv_fill<<<dim3(blocks), dim3(256), 0, st>>>(ct.dev, n, chunks, carry, gcv);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  (void)hipFree(carry);
-  if (e != hipSuccess) {
-    (void)hipFree(gcv);
-    return fail(PL_E_DEVICE, "GC buffer pass: %s", hipGetErrorString(e));
+  uint8_t* block = nullptr;
+  if (hipMalloc((void**)&block, bytes) != hipSuccess)
+    return fail(PL_E_OOM, "device allocation of %zu bytes failed", bytes);
+  std::shared_ptr<uint8_t> mem(block, [](uint8_t* p) { (void)hipFree(p); });
+  for (size_t i = 0; i < jobs.size(); i++) jobs[i].gcv = block + off[i];
+  VcfBufs B;
+  uint8_t* carry;
+  GcvJob* dj;
+  pl_status s;
+  if ((s = B.get(&carry, (size_t)nt)) || (s = B.get(&dj, sizeof(GcvJob) * jobs.size()))) return s;
+  HIPCHK(hipMemcpyAsync(dj, jobs.data(), sizeof(GcvJob) * jobs.size(), hipMemcpyHostToDevice, st));
+  const int32_t nj = (int32_t)jobs.size();
+  const unsigned blocks = (unsigned)((nt + 255) / 256);
+  k_gcv_carry<<<dim3(blocks), dim3(256), 0, st>>>(dj, nj, nt, carry);
+  k_gcv_top<<<dim3((unsigned)(((int64_t)nj * 100 + 255) / 256)), dim3(256), 0, st>>>(dj, nj, carry);
+  k_gcv_fill<<<dim3(blocks), dim3(256), 0, st>>>(dj, nj, nt, carry);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  for (size_t i = 0; i < miss.size(); i++) {
+    Contig& ct = c->contigs.at(miss[i]);
+    ct.gcv_mem = mem;
+    ct.gcv = block + off[i];
   }
-  ct.gcv = gcv;
   return PL_OK;
 }
 
@@ -48,13 +74,10 @@ pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs
   std::memset(&T, 0, sizeof T);
   T.A = L->args;
   // the contigs' GC buffers (once per contig) and each region's pointer
-  std::vector<const uint8_t*> gp((size_t)nj);
   if ((int32_t)contigs.size() != nj) return fail(PL_E_INVAL, "%d contigs for %d regions", (int)contigs.size(), nj);
-  for (int32_t j = 0; j < nj; j++) {
-    Contig& ct = c->contigs.at(contigs[(size_t)j]);
-    if (pl_status s = contig_gcv(ct, st)) return s;
-    gp[(size_t)j] = ct.gcv;
-  }
+  if (pl_status s = build_gcv(c, contigs, st)) return s;
+  std::vector<const uint8_t*> gp((size_t)nj);
+  for (int32_t j = 0; j < nj; j++) gp[(size_t)j] = c->contigs.at(contigs[(size_t)j]).gcv;
   const uint8_t** dgp;
   pl_status s;
   if ((s = B.get(&dgp, sizeof(void*) * (size_t)nj)) || (s = B.get(&T.frag, 4 * (size_t)G)) ||

@@ -17,7 +17,8 @@
 //                 (prefix sums staged in LDS), the rest from the pileup;
 //   k_wig_lines   one track's text (length pass, then write pass).
 
-#define PL_GCV_ROWS 1024     // contig rows (of 100 loci) per column-scan chunk
+#define PL_GCV_MIN_ROWS 64     // contig rows (of 100 loci) per column-scan chunk, at least
+#define PL_GCV_MAX_CHUNKS 2048 // chunks per contig, at most (the serial pass between)
 #define PL_GC_STAGE (2048 + 149)
 
 struct TrackArgs {
@@ -112,50 +113,88 @@ __device__ uint32_t wig_locus(char* out, const TrackArgs& T, int32_t job, const 
 }  // namespace
 
 // ---------------------------------------------------------------- GC buffer
-// Locus k of the contig is column k % 100, row k / 100.  One thread per
-// (chunk of PL_GCV_ROWS rows, column): adjacent threads read adjacent loci.
-// carry = the chunk's last defined value in the column (2: none).
-__global__ void __launch_bounds__(256) k_gcv_carry(const uint8_t* ref, int64_t n, int64_t chunks, uint8_t* carry) {
+// Locus k of a contig is column k % 100, row k / 100; a contig's rows are
+// cut into chunks of `rpc` rows.  The contigs still without buffer values
+// are built together: one thread per (contig, chunk, column) in the carry
+// and fill passes (adjacent threads read adjacent loci), one per (contig,
+// column) in the pass between them.
+struct GcvJob {
+  const uint8_t* ref;
+  uint8_t* gcv;
+  int64_t n;                 // contig length
+  int64_t rpc;               // rows per chunk
+  int64_t chunks;
+  int64_t t0;                // first thread (chunk * 100 + column) of this contig
+};
+
+__device__ __forceinline__ int32_t gcv_job_of(const GcvJob* J, int32_t nj, int64_t t) {
+  int32_t lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int32_t m = (lo + hi + 1) >> 1;
+    if (J[m].t0 <= t) lo = m; else hi = m - 1;
+  }
+  return lo;
+}
+
+// carry = the chunk's last defined value in the column (2: none)
+__global__ void __launch_bounds__(256) k_gcv_carry(const GcvJob* J, int32_t nj, int64_t nt, uint8_t* carry) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= chunks * 100) return;
-  const int64_t ch = t / 100, col = t % 100;
-  const int64_t rows = (n + 99) / 100;
-  const int64_t r0 = ch * PL_GCV_ROWS, r1 = r0 + PL_GCV_ROWS < rows ? r0 + PL_GCV_ROWS : rows;
+  if (t >= nt) return;
+  const GcvJob& g = J[gcv_job_of(J, nj, t)];
+  const int64_t u = t - g.t0, ch = u / 100, col = u % 100;
+  const int64_t rows = (g.n + 99) / 100;
+  const int64_t r0 = ch * g.rpc, r1 = r0 + g.rpc < rows ? r0 + g.rpc : rows;
   uint32_t v = 2;
   for (int64_t r = r1 - 1; r >= r0 && v == 2; r--) {
     const int64_t k = r * 100 + col;
-    if (k < n) v = gc_class(ref[k]);
+    if (k < g.n) v = gc_class(g.ref[k]);
   }
   carry[t] = (uint8_t)v;
 }
 
-// one thread per column: the value entering each chunk, starting from the
-// buffer's initial value (slot `col` holds 1 when col is even)
-__global__ void __launch_bounds__(128) k_gcv_top(uint8_t* carry, int64_t chunks) {
-  const int col = threadIdx.x;
-  if (col >= 100) return;
+// one thread per (contig, column): the value entering each chunk, starting
+// from the buffer's initial value (slot `col` holds 1 when col is even)
+__global__ void __launch_bounds__(256) k_gcv_top(const GcvJob* J, int32_t nj, uint8_t* carry) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nj * 100) return;
+  const GcvJob& g = J[t / 100];
+  const int64_t col = t % 100;
   uint8_t run = (col & 1) == 0;
-  for (int64_t ch = 0; ch < chunks; ch++) {
-    const uint8_t c = carry[ch * 100 + col];
-    carry[ch * 100 + col] = run;
+  for (int64_t ch = 0; ch < g.chunks; ch++) {
+    const int64_t i = g.t0 + ch * 100 + col;
+    const uint8_t c = carry[i];
+    carry[i] = run;
     if (c != 2) run = c;
   }
 }
 
-__global__ void __launch_bounds__(256) k_gcv_fill(const uint8_t* ref, int64_t n, int64_t chunks, const uint8_t* carry,
-                                                  uint8_t* gcv) {
+__global__ void __launch_bounds__(256) k_gcv_fill(const GcvJob* J, int32_t nj, int64_t nt, const uint8_t* carry) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= chunks * 100) return;
-  const int64_t ch = t / 100, col = t % 100;
-  const int64_t rows = (n + 99) / 100;
-  const int64_t r0 = ch * PL_GCV_ROWS, r1 = r0 + PL_GCV_ROWS < rows ? r0 + PL_GCV_ROWS : rows;
+  if (t >= nt) return;
+  const GcvJob& g = J[gcv_job_of(J, nj, t)];
+  const int64_t u = t - g.t0, ch = u / 100, col = u % 100;
+  const int64_t rows = (g.n + 99) / 100;
+  const int64_t r0 = ch * g.rpc, r1 = r0 + g.rpc < rows ? r0 + g.rpc : rows;
+  const uint8_t* __restrict__ ref = g.ref;
+  uint8_t* __restrict__ out = g.gcv;
   uint32_t run = carry[t];
-  for (int64_t r = r0; r < r1; r++) {
-    const int64_t k = r * 100 + col;
-    if (k >= n) break;
-    const uint32_t v = gc_class(ref[k]);
-    if (v != 2) run = v;
-    gcv[k] = (uint8_t)run;
+  // eight loads in flight before the dependent selects
+  for (int64_t r = r0; r < r1; r += 8) {
+    uint8_t c[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int64_t k = (r + q) * 100 + col;
+      c[q] = (r + q < r1 && k < g.n) ? ref[k] : (uint8_t)0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int64_t k = (r + q) * 100 + col;
+      if (r + q < r1 && k < g.n) {
+        const uint32_t v = gc_class(c[q]);
+        if (v != 2) run = v;
+        out[k] = (uint8_t)run;
+      }
+    }
   }
 }
 
@@ -238,9 +277,17 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
   const double baseCov = (double)T.jsum[win.job] / (double)J.size;       // NormalDistribution.mean
   for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
     const int64_t g = J.out + p;
-    const Col c = load_col(A.pile, G, g);
+    // the pileup fields the tracks read (10 of the 18)
+    int64_t cnt = 0, qsum = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      cnt += (uint32_t)A.pile[(int64_t)(PL_F_COUNT_A + b) * G + g];
+      qsum += A.pile[(int64_t)(PL_F_QUAL_A + b) * G + g];
+    }
+    const int32_t mqSum = A.pile[(int64_t)PL_F_MQSUM * G + g];
+    const int32_t qSum = A.pile[(int64_t)PL_F_QSUM * G + g];
     const uint16_t f = A.calls[g].flags;
-    const int64_t dep = c.depth();
+    const int64_t dep = cnt + A.pile[(int64_t)PL_F_DELETIONS * G + g];
     // the final disposition (as k_final_flags)
     int disp = (f & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT;
     if (!reads || (f & PL_CF_DELETED) || dep < S.min_depth) disp = PL_D_NONE;
@@ -255,7 +302,6 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
     const int32_t rr = T.frag[J.out + (p + 100 < sz - 1 ? p + 100 : sz - 1)];
     const int32_t ce = T.frag[g];
     const int32_t d = j_sub(l, rr);
-    const int64_t qsum = c.qs[0] + c.qs[1] + c.qs[2] + c.qs[3];
     const int64_t cc = (int64_t)J.start + p;
     int32_t gc = 0;
     if (cc < n) {
@@ -273,8 +319,8 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
     v[7] = j_add(j_sub(l, ce), j_sub(rr, ce));                                   // Dip Coverage
     v[8] = reads ? A.pile[(int64_t)PL_F_PHYSCOV * G + g] : 0;                    // Physical Coverage
     v[9] = reads ? (int32_t)(int16_t)A.pile[(int64_t)PL_F_CLIPS * G + g] : 0;    // Clipped Alignments
-    v[10] = reads ? (int32_t)(int8_t)round_div_l(qsum, (int64_t)c.mqSum) : 0;   // Weighted Qual (PileUp.scala:60-62)
-    v[11] = reads ? (int32_t)(int8_t)round_div_l(qsum, (int64_t)c.qSum) : 0;    // Weighted MQ (PileUp.scala:56-58)
+    v[10] = reads ? (int32_t)(int8_t)round_div_l(qsum, (int64_t)mqSum) : 0;   // Weighted Qual (PileUp.scala:60-62)
+    v[11] = reads ? (int32_t)(int8_t)round_div_l(qsum, (int64_t)qSum) : 0;    // Weighted MQ (PileUp.scala:56-58)
     v[12] = gc;                                                                  // GC
 #pragma unroll
     for (int t = 0; t < PL_NTRACKS; t++) T.vals[(int64_t)t * G + g] = v[t];

@@ -103,3 +103,37 @@ def test_tracks_on_c1_are_consistent():
     assert np.array_equal(t[12], O.gc_track(d.draft(0), 1, L))
     # uniform 40x coverage: copy number 1 (track 0) away from the zeroed left edge
     assert (t[2][:101] == -1).all() and (t[2][200:-200] == 0).mean() > 0.99
+
+
+def test_fragment_tracks_from_the_final_pileup():
+    """Delta, dip and copy number of c1 (which has Pass-1 deletions) from
+    fragment coverage rebuilt another way: the final depth, less, at a
+    deleted locus, the deletion count of the nearest undeleted locus to its
+    left (the one whose call added it).  The oracle keeps processBam's
+    depth from before Pass 1 instead."""
+    O = oracle()
+    from pilon_amd.abi import F
+    from pilon_amd.synth import SynthData, preset
+    d = SynthData(preset("c1"))
+    L = len(d.draft(0))
+    r = O.run_region(default_config(), d.draft(0), 1, L, [d.batch(0)], name="c", vcf=O.TRACKS)
+    t, pile = r["tracks"], r["pile"].astype(np.int64)
+    deleted = (r["calls"]["flags"] & 0x0800) != 0
+    assert deleted.sum() > 0
+    depth = pile[F["count_A"]] + pile[F["count_C"]] + pile[F["count_G"]] + pile[F["count_T"]] + pile[F["deletions"]]
+    frag = depth.copy()
+    for i in np.nonzero(deleted)[0]:
+        k = i - 1
+        while deleted[k]:
+            k -= 1
+        frag[i] -= pile[F["deletions"]][k]
+    assert (frag != depth).any()
+    n = L
+    idx = np.arange(n)
+    left, right = frag[np.maximum(0, idx - 100)], frag[np.minimum(n - 1, idx + 100)]
+    assert np.array_equal(t[6], np.abs(left - right))
+    assert np.array_equal(t[7], (left - frag) + (right - frag))
+    mean = frag.sum() / n
+    x = O.smooth(frag.astype(np.int32)) / mean
+    cn = (np.floor(x) + ((x - np.floor(x)) >= 0.5)).astype(np.int64)
+    assert np.array_equal(t[2], cn - 1)
