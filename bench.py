@@ -309,10 +309,17 @@ def main():
                   "note": "Tracks.standardTracks values (13 per locus, copy number and GC included) on the device; "
                           "device_ms covers the fragment-coverage, prefix-sum and value passes; the wig figures "
                           "are the Coverage track's text (values again, then length, scan and write passes)"}
-        del wig
+        t0 = time.perf_counter()
+        bed, bms = ctx.batch_bed()
+        bwall = time.perf_counter() - t0
+        tracks.update({"bed_lines": bed.count("\n") - 1, "bed_device_ms": bms, "bed_wall_ms": bwall * 1000.0})
+        tracks["note"] += ("; Pilon.bed: the issue bits of every locus on the device with the values, their runs "
+                           "summarised on the host's threads (bed_wall_ms includes the mask copy and the summary)")
+        del wig, bed
         log(f"bench: tracks: {G} loci x 13 in {tracks['device_ms']:.2f} ms on the device "
             f"({tracks['loci_per_s'] / 1e9:.2f} G loci/s; first call {first_ms:.1f} ms wall with the GC buffers), "
-            f"Coverage.wig {tracks['wig_coverage_bytes'] / 1e6:.0f} MB in {wms:.1f} ms")
+            f"Coverage.wig {tracks['wig_coverage_bytes'] / 1e6:.0f} MB in {wms:.1f} ms, "
+            f"Pilon.bed {tracks['bed_lines']} lines in {bwall * 1000:.1f} ms wall")
 
     producer = None
     if args.bam:

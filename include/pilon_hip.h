@@ -13,6 +13,8 @@
  *   GenomeRegion.identifyAndFixIssues      pilon/GenomeRegion.scala:307-380,413
  *   GenomeRegion.fixIssues/writeChanges    pilon/GenomeRegion.scala:557-657
  *   GenomeRegion.writeVcf / Vcf            pilon/GenomeRegion.scala:623-643, pilon/Vcf.scala:28-163
+ *   Tracks.standardTracks, makeTrack,      pilon/Tracks.scala:33-216,
+ *   makeBedTrack; Pass 2, computeGc        pilon/GenomeRegion.scala:158-210,273-283,696-763
  *
  * Conventions: plain C types only; 0 = success; the library owns device
  * memory, the caller owns host memory; one pl_ctx per host thread; objects
@@ -249,7 +251,7 @@ pl_status pl_vcf_header(const char* const* names, const int64_t* lengths, int32_
  * that order, with Pass 2 of GenomeRegion.postProcess (copy number,
  * GenomeRegion.scala:273-283) and computeGc (GenomeRegion.scala:160-188)
  * for every region of the context's last batch.  The Pilon.bed issue track
- * is not among them (its regions come from the reassembly pass). */
+ * is pl_batch_bed below. */
 enum pl_track {
   PL_TRACK_CHANGES = 0,        /* Changes.wig            */
   PL_TRACK_UNCONFIRMED,        /* Unconfirmed.wig        */
@@ -278,9 +280,20 @@ pl_status pl_region_tracks(pl_region* reg, int32_t* values, int64_t n_values, do
  * "fixedStep chrom=<contig> start=<start> step=1" per region, then one value
  * per line, formatted on the device; PL_WIG_TRACK_LINE puts the file's
  * "track type=wiggle_0 ..." line first.  *text is malloc'ed (pl_free). */
-#define PL_WIG_TRACK_LINE 0x1u
+#define PL_WIG_TRACK_LINE 0x1u   /* also: pl_batch_bed's track line */
 pl_status pl_batch_wig(pl_ctx* ctx, int32_t track, uint32_t flags, char** text, int64_t* len, double* device_ms);
 pl_status pl_region_wig(pl_region* reg, int32_t track, uint32_t flags, char** text, int64_t* len, double* device_ms);
+
+/* Tracks.makeBedTrack (Tracks.scala:194-216) over the last batch's regions:
+ * per region its unconfirmed ("?"), change ("X"), low-coverage ("LowCov"),
+ * collapsed-repeat ("Copy#"), duplication, gap and break regions
+ * (GenomeRegion.scala:696-763) as BED lines.  The per-locus tests run on the
+ * device with the tracks; the runs are summarised on the host.  The
+ * reassembly outcomes Pilon appends per region are not produced (no
+ * reassembly on this path).  PL_WIG_TRACK_LINE puts the file's
+ * "track description=..." line first.  *text is malloc'ed (pl_free). */
+pl_status pl_batch_bed(pl_ctx* ctx, uint32_t flags, char** text, int64_t* len, double* device_ms);
+pl_status pl_region_bed(pl_region* reg, uint32_t flags, char** text, int64_t* len, double* device_ms);
 
 /* Sizes and HIP-event times of the last batch (pl_batch_run or
  * pl_region_finish). */

@@ -30,7 +30,8 @@ class OrcResult(C.Structure):
                 ("insertSize", C.POINTER(C.c_int32)), ("weightedQual", C.POINTER(C.c_int8)),
                 ("weightedMq", C.POINTER(C.c_int8)), ("clips", C.POINTER(C.c_int16)),
                 ("stats", PlRegionStats), ("changes_text", C.c_char_p), ("changes_len", C.c_int64),
-                ("vcf_text", C.c_void_p), ("vcf_len", C.c_int64), ("tracks", C.POINTER(C.c_int32)),
+                ("vcf_text", C.c_void_p), ("vcf_len", C.c_int64), ("bed_text", C.c_void_p), ("bed_len", C.c_int64),
+                ("tracks", C.POINTER(C.c_int32)),
                 ("bases", C.POINTER(C.c_uint8)), ("bases_len", C.c_int64),
                 ("n_indel", C.c_int32), ("indel_pos", C.POINTER(C.c_int32)),
                 ("indel_off", C.POINTER(C.c_int64)), ("indel_str", C.POINTER(C.c_uint8))]
@@ -53,6 +54,8 @@ def lib():
         L.orc_fmt_af.argtypes = [C.c_float, C.c_char_p]
         L.orc_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.orc_gc.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p]
+        L.orc_summary_regions.restype = C.c_int32
+        L.orc_summary_regions.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32]
         L.orc_java_round.restype = C.c_int64
         L.orc_java_round.argtypes = [C.c_double]
         L.orc_roundDivI.restype = C.c_int32
@@ -75,7 +78,7 @@ def _arr(ptr, n, dtype):
     return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
 
 
-VCF_ON, VCF_DIPLOID, VCF_QE, TRACKS = 1, 2, 4, 8
+VCF_ON, VCF_DIPLOID, VCF_QE, TRACKS, BED = 1, 2, 4, 8, 16
 TRACK_NAMES = ["Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverage", "Pct Bad", "Delta Coverage",
                "Dip Coverage", "Physical Coverage", "Clipped Alignments", "Weighted Qual", "Weighted MQ", "GC"]
 
@@ -83,7 +86,8 @@ TRACK_NAMES = ["Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverag
 def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0, vcf=0):
     """One GenomeRegion through the oracle; returns a dict of numpy arrays.
     vcf: VCF_ON (| VCF_DIPLOID | VCF_QE) adds the region's VCF record lines
-    (GenomeRegion.writeVcf) as out["vcf"]."""
+    (GenomeRegion.writeVcf) as out["vcf"]; TRACKS the 13 per-locus tracks as
+    out["tracks"]; BED the region's Pilon.bed lines as out["bed"]."""
     L = lib()
     if isinstance(contig, str):
         contig = contig.encode()
@@ -114,7 +118,9 @@ def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, of
         out["changes"] = C.string_at(r.changes_text, r.changes_len).decode() if r.changes_len else ""
         if vcf & VCF_ON:
             out["vcf"] = C.string_at(r.vcf_text, r.vcf_len).decode() if r.vcf_len else ""
-        if vcf & TRACKS:
+        if vcf & BED:
+            out["bed"] = C.string_at(r.bed_text, r.bed_len).decode() if r.bed_len else ""
+        if vcf & (TRACKS | BED):
             out["tracks"] = _arr(r.tracks, len(TRACK_NAMES) * size, np.int32).reshape(len(TRACK_NAMES), size)
         out["bases"] = bytes(_arr(r.bases, r.bases_len, np.uint8))
         n = r.n_indel
@@ -180,3 +186,13 @@ def basecall(cfg, bc, qs, mqSum=0, qSum=0, deletions=0, insertions=0, insQual=0,
                    len(ins_list), il, "".join(ins_list).encode(), len(del_list), dl,
                    "".join(del_list).encode(), C.byref(out), C.byref(ilen), C.byref(score))
     return {"base": out.base, "alt": out.alt, "flags": out.flags, "q": out.q}, ilen.value, score.value
+
+
+def summary_regions(test, start, slop=100):
+    """GenomeRegion.summaryRegions over a boolean array of a region starting
+    at locus `start`: [(start, stop)] loci, nearEdge regions dropped."""
+    t = np.ascontiguousarray(np.asarray(test, dtype=np.uint8))
+    cap = len(t) + 1
+    out = np.zeros(2 * cap, np.int32)
+    n = lib().orc_summary_regions(t.ctypes.data, len(t), start, slop, out.ctypes.data, cap)
+    return [(int(out[2 * k]), int(out[2 * k + 1])) for k in range(n)]

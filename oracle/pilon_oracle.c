@@ -560,6 +560,7 @@ typedef struct orc_result {
   pl_region_stats stats;
   char*    changes_text; int64_t changes_len;
   char*    vcf_text; int64_t vcf_len;   /* Vcf.writeRecord lines of the region (when asked) */
+  char*    bed_text; int64_t bed_len;   /* the region's Pilon.bed lines (when asked) */
   int32_t* tracks;           /* [ORC_NTRACKS*size] Tracks.standardTracks values (when asked) */
   uint8_t* bases; int64_t bases_len;
   /* the called indel string at each INS/DEL change, packed: for change c,
@@ -571,7 +572,7 @@ void orc_result_free(orc_result* r) {
   if (!r) return;
   free(r->pile); free(r->bcq); free(r->calls); free(r->coverage); free(r->badCoverage);
   free(r->physCoverage); free(r->insertSize); free(r->weightedQual); free(r->weightedMq);
-  free(r->clips); free(r->changes_text); free(r->vcf_text); free(r->tracks); free(r->bases);
+  free(r->clips); free(r->changes_text); free(r->vcf_text); free(r->bed_text); free(r->tracks); free(r->bases);
   free(r->indel_pos); free(r->indel_off); free(r->indel_str);
   free(r);
 }
@@ -784,6 +785,138 @@ static void compute_gc(const uint8_t* contig, int64_t contig_len, int32_t start,
   }
 }
 
+/* ------------------------------------------------------------ Pilon.bed */
+typedef struct { int32_t start, stop; } IssueRegion;      /* Region(name, start, stop), loci */
+typedef struct { IssueRegion* v; int32_t n, cap; } RegionList;
+
+static void rl_push(RegionList* l, int32_t a, int32_t b) {
+  if (l->n == l->cap) { l->cap = 2 * l->cap + 16; l->v = (IssueRegion*)realloc(l->v, sizeof(IssueRegion) * (size_t)l->cap); }
+  l->v[l->n].start = a; l->v[l->n].stop = b; l->n++;
+}
+
+/* GenomeRegion.summaryRegions (GenomeRegion.scala:740-763): runs of loci
+ * passing the test, a run closing at the first failing index more than
+ * `slop` past its last passing one; regions within 100 of either end of the
+ * GenomeRegion are dropped (nearEdge, :691). */
+static RegionList summary_regions(const uint8_t* test, int32_t size, int32_t start, int32_t stop, int32_t slop) {
+  RegionList all = {0}, out = {0};
+  int32_t first = -1, last = -1;
+  for (int32_t i = 0; i < size; i++) {
+    if (test[i]) {
+      last = i;
+      if (first < 0) first = i;
+    } else if (last >= 0 && i > last + slop) {
+      rl_push(&all, start + first, start + last);
+      first = -1;
+      last = -1;
+    }
+  }
+  if (last >= 0) rl_push(&all, start + first, start + last);
+  for (int32_t k = 0; k < all.n; k++)
+    if (!(all.v[k].start - start < 100 || stop - all.v[k].stop < 100)) rl_push(&out, all.v[k].start, all.v[k].stop);
+  free(all.v);
+  return out;
+}
+
+/* Tracks.regionsToBed (Tracks.scala:179-183) */
+static void bed_append(char** buf, int64_t* len, int64_t* cap, const char* name, const RegionList* l,
+                       const char* label, const char* rgb) {
+  char line[512];
+  for (int32_t k = 0; k < l->n; k++) {
+    int n = snprintf(line, sizeof line, "%s\t%d\t%d\t%s\t0\t+\t%d\t%d\t%s\n", name, l->v[k].start - 1, l->v[k].stop,
+                     label, l->v[k].start - 1, l->v[k].stop, rgb);
+    if (n >= (int)sizeof line) {                            /* a long contig name */
+      char* big = (char*)malloc((size_t)n + 1);
+      snprintf(big, (size_t)n + 1, "%s\t%d\t%d\t%s\t0\t+\t%d\t%d\t%s\n", name, l->v[k].start - 1, l->v[k].stop, label,
+               l->v[k].start - 1, l->v[k].stop, rgb);
+      sb_append(buf, len, cap, big, n);
+      free(big);
+    } else {
+      sb_append(buf, len, cap, line, n);
+    }
+  }
+}
+
+/* The region's lines of Tracks.makeBedTrack (Tracks.scala:194-216): its
+ * unconfirmed, change, low-coverage, collapsed-repeat, duplication, gap and
+ * break regions (GenomeRegion.scala:696-738) in that order.  The
+ * reassembly outcomes that follow them are not produced (no reassembly on
+ * this path). */
+static void bed_lines(const pl_config* cfg, const char* name, int32_t start, int32_t stop, const uint8_t* contig,
+                      const int32_t* coverage, const int32_t* badCoverage, const int16_t* clips,
+                      const uint8_t* confirmed, const uint8_t* kindAt, const uint8_t* deleted,
+                      const int16_t* copyNumber, const int32_t* fragCov, double fragMean, char** text, int64_t* len) {
+  const int32_t size = stop + 1 - start;
+  uint8_t* t = (uint8_t*)calloc((size_t)size, 1);
+  int64_t totalBad = 0, totalGood = 0;                      /* pctBadOverall (:99-113), Long */
+  for (int32_t i = 0; i < size; i++) { totalBad += badCoverage[i]; totalGood += coverage[i]; }
+  const int64_t pctBadOverall = roundDivL(100 * totalBad, totalGood + totalBad);
+  int64_t cap = 0;
+  *text = NULL;
+  *len = 0;
+  RegionList L[7];
+  for (int32_t i = 0; i < size; i++) t[i] = !confirmed[i];
+  L[0] = summary_regions(t, size, start, stop, 100);                                    /* unConfirmedRegions */
+  for (int32_t i = 0; i < size; i++) t[i] = kindAt[i] == K_SNP || kindAt[i] == K_INS || kindAt[i] == K_DEL || deleted[i];
+  L[1] = summary_regions(t, size, start, stop, 1);                                      /* changeRegions */
+  for (int32_t i = 0; i < size; i++)                                                    /* lowCoverage (:700) */
+    t[i] = coverage[i] < cfg->min_min_depth && upper(contig[(int64_t)start + i - 1]) != 'N';
+  L[2] = summary_regions(t, size, start, stop, 100);
+  for (int32_t i = 0; i < size; i++) t[i] = copyNumber[i] > 1;
+  RegionList hcn = summary_regions(t, size, start, stop, 100);                          /* highCopyNumberRegions */
+  L[3] = (RegionList){0};
+  for (int32_t k = 0; k < hcn.n; k++) {                     /* possibleCollapsedRepeats (:724-729) */
+    const int32_t a = hcn.v[k].start - start, b = hcn.v[k].stop - start;
+    int32_t d[2];
+    for (int e = 0; e < 2; e++) {
+      const int32_t i = e ? b : a;
+      const int32_t l = fragCov[i > 100 ? i - 100 : 0], r = fragCov[i + 100 < size - 1 ? i + 100 : size - 1];
+      const int32_t x = jsub(l, r);
+      d[e] = x < 0 ? jsub(0, x) : x;                        /* delta (:672-678) */
+    }
+    if ((double)d[0] / fragMean >= 0.5 && (double)d[1] / fragMean >= 0.5) rl_push(&L[3], hcn.v[k].start, hcn.v[k].stop);
+  }
+  free(hcn.v);
+  RegionList dup = summary_regions(t, size, start, stop, 2000);                        /* duplicationEvents (:731-737) */
+  L[4] = (RegionList){0};
+  for (int32_t k = 0; k < dup.n; k++)
+    if (dup.v[k].stop + 1 - dup.v[k].start > 10000) rl_push(&L[4], dup.v[k].start, dup.v[k].stop);
+  free(dup.v);
+  for (int32_t i = 0; i < size; i++) t[i] = upper(contig[(int64_t)start + i - 1]) == 'N';
+  RegionList g = summary_regions(t, size, start, stop, 100);                            /* gaps (:698) */
+  L[5] = (RegionList){0};
+  for (int32_t k = 0; k < g.n; k++)
+    if (g.v[k].stop + 1 - g.v[k].start >= 10) rl_push(&L[5], g.v[k].start, g.v[k].stop);
+  free(g.v);
+  for (int32_t i = 0; i < size; i++) {                      /* breakp (:719) */
+    const int32_t cov = coverage[i], bad = badCoverage[i];
+    const int lowCov = cov < cfg->min_min_depth && upper(contig[(int64_t)start + i - 1]) != 'N';
+    const int highClip = cov >= cfg->min_min_depth && pctI(clips[i], cov) >= 33;
+    const int tooBad = (int64_t)pctI(bad, jadd(cov, bad)) > pctBadOverall + 20;
+    const int32_t l = fragCov[i > 100 ? i - 100 : 0], r = fragCov[i + 100 < size - 1 ? i + 100 : size - 1];
+    const int32_t dip = jadd(jsub(l, fragCov[i]), jsub(r, fragCov[i]));
+    t[i] = lowCov || highClip || tooBad || (double)dip / fragMean >= 1.5;
+  }
+  RegionList br = summary_regions(t, size, start, stop, 200);                           /* possibleBreaks (:720) */
+  L[6] = (RegionList){0};
+  for (int32_t k = 0; k < br.n; k++) {                      /* filter !nearAny(gaps, 300) (Region.scala:37-40) */
+    int near = 0;
+    for (int32_t q = 0; q < L[5].n && !near; q++) {
+      const IssueRegion G = L[5].v[q], B = br.v[k];
+      near = (B.start <= G.stop && B.stop >= G.start) || abs(B.stop - G.start) <= 300 || abs(B.start - G.stop) <= 300;
+    }
+    if (!near) rl_push(&L[6], br.v[k].start, br.v[k].stop);
+  }
+  free(br.v);
+  static const char* labels[7] = {"?", "X", "LowCov", "Copy#", "Duplication", "Gap", "Break"};
+  static const char* rgbs[7] = {"0,128,128", "255,0,0", "128,0,128", "128,128,0", "255,128,0", "0,0,0", "255,0,255"};
+  for (int k = 0; k < 7; k++) {
+    bed_append(text, len, &cap, name, &L[k], labels[k], rgbs[k]);
+    free(L[k].v);
+  }
+  free(t);
+}
+
 orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
                            const char* name, int32_t start, int32_t stop,
                            const pl_read_soa* const* sets, int32_t n_sets,
@@ -895,7 +1028,7 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
   }
 
   /* Pass 2 (GenomeRegion.scala:273-283) and the tracks (Tracks.scala:33-190) */
-  if (vcf_flags & 8) {
+  if (vcf_flags & (8 | 16)) {
     int32_t* t = res->tracks = (int32_t*)calloc((size_t)ORC_NTRACKS * (size_t)size, 4);
     int16_t* copyNumber = (int16_t*)calloc((size_t)size, 2);
     if (R.readCount != 0) {                                     /* Pass 2 runs after the early return */
@@ -928,6 +1061,12 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
       t[10 * (size_t)size + i] = res->weightedQual[i];
       t[11 * (size_t)size + i] = res->weightedMq[i];
       t[12 * (size_t)size + i] = gc[i];
+    }
+    if (vcf_flags & 16) {
+      double fsum = 0.0;
+      for (int32_t i = 0; i < size; i++) fsum += (double)fragCov[i];
+      bed_lines(cfg, name, start, stop, contig, res->coverage, res->badCoverage, res->clips, confirmed, kindAt,
+                deleted, copyNumber, fragCov, fsum / (double)size, &res->bed_text, &res->bed_len);
     }
     free(gc); free(copyNumber);
   }
@@ -1098,6 +1237,13 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
 
 /* Exposed for known-answer tests of the small helpers. */
 void orc_smooth(const int32_t* in, int32_t n, int32_t window, int32_t* out) { smooth_int(in, n, window, out); }
+int32_t orc_summary_regions(const uint8_t* test, int32_t size, int32_t start, int32_t slop, int32_t* out, int32_t cap) {
+  RegionList l = summary_regions(test, size, start, start + size - 1, slop);
+  const int32_t n = l.n;
+  for (int32_t k = 0; k < n && k < cap; k++) { out[2 * k] = l.v[k].start; out[2 * k + 1] = l.v[k].stop; }
+  free(l.v);
+  return n;
+}
 void orc_gc(const uint8_t* contig, int64_t len, int32_t start, int32_t stop, int8_t* gc) { compute_gc(contig, len, start, stop, gc); }
 void orc_fmt_af(float af, char* out) { fmt_af(out, af); }
 int32_t orc_roundDivI(int32_t n, int32_t d) { return roundDivI(n, d); }

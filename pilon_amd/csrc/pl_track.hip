@@ -66,7 +66,7 @@ pl_status build_gcv(pl_ctx* c, const std::vector<int32_t>& ids, hipStream_t st) 
 // The track values of the lane's last batch into T.vals (device), in B's
 // buffers; contigs[j] is region j's contig.  ms accumulates the device time.
 pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs, VcfBufs& B, TrackArgs& T,
-                         double* ms) {
+                         double* ms, bool issues = false) {
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = L->stream;
   const int64_t G = L->G;
@@ -88,6 +88,12 @@ pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs
   const int64_t tiles = std::max<int64_t>(1, (G + SCAN_TILE - 1) / SCAN_TILE);
   uint64_t* tsum;
   if ((s = B.get(&tsum, 8 * (size_t)tiles))) return s;
+  if (issues) {
+    if ((s = B.get(&T.mask, (size_t)G)) || (s = B.get(&T.jbad, 8 * (size_t)nj)) || (s = B.get(&T.jgood, 8 * (size_t)nj)))
+      return s;
+    HIPCHK(hipMemsetAsync(T.jbad, 0, 8 * (size_t)nj, st));
+    HIPCHK(hipMemsetAsync(T.jgood, 0, 8 * (size_t)nj, st));
+  }
   T.gcv = dgp;
   T.fpre = fpre;
   HIPCHK(hipMemcpyAsync(dgp, gp.data(), sizeof(void*) * (size_t)nj, hipMemcpyHostToDevice, st));
@@ -205,6 +211,119 @@ pl_status wig_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   return PL_OK;
 }
 
+// GenomeRegion.summaryRegions (GenomeRegion.scala:740-763) over the issue
+// bits of one region: runs of loci with the bit, a run ending where more
+// than `slop` loci without it follow; runs within 100 loci of either end
+// of the region are dropped (nearEdge, :691).  Regions are 1-based loci.
+struct Summary {
+  uint8_t bit;
+  int32_t slop;
+  int32_t first = -1, last = -1;
+  std::vector<std::pair<int32_t, int32_t>> out;
+};
+
+void summary_close(Summary& S, int32_t start, int32_t stop) {
+  const int32_t a = start + S.first, b = start + S.last;
+  if (!(a - start < 100 || stop - b < 100)) S.out.emplace_back(a, b);
+}
+
+// the region's lines of Tracks.makeBedTrack (Tracks.scala:194-216), from
+// its issue bits: unconfirmed, change, low-coverage, collapsed-repeat,
+// duplication, gap and break regions, in that order
+void bed_of_region(const uint8_t* m, int32_t start, int32_t stop, const std::string& name, std::string& out) {
+  const int32_t n = stop + 1 - start;
+  Summary S[7] = {{PL_ISSUE_UNCONFIRMED, 100}, {PL_ISSUE_CHANGED, 1}, {PL_ISSUE_LOWCOV, 100}, {PL_ISSUE_HIGHCN, 100},
+                  {PL_ISSUE_HIGHCN, 2000}, {PL_ISSUE_N, 100}, {PL_ISSUE_BREAK, 200}};
+  for (int32_t i = 0; i < n; i++) {
+    while (i + 8 <= n) {                       // skip loci without issues, eight at a time
+      uint64_t w;
+      std::memcpy(&w, m + i, 8);
+      if (w) break;
+      i += 8;
+    }
+    if (i >= n) break;
+    const uint8_t b = m[i];
+    if (!b) continue;
+    for (Summary& q : S) {
+      if (!(b & q.bit)) continue;
+      if (q.last >= 0 && i > q.last + q.slop + 1) { summary_close(q, start, stop); q.first = -1; }
+      if (q.first < 0) q.first = i;
+      q.last = i;
+    }
+  }
+  for (Summary& q : S)
+    if (q.last >= 0) summary_close(q, start, stop);
+  std::vector<std::pair<int32_t, int32_t>> lists[7];
+  lists[0] = std::move(S[0].out);
+  lists[1] = std::move(S[1].out);
+  lists[2] = std::move(S[2].out);
+  for (auto& r : S[3].out)                     // possibleCollapsedRepeats (:724-729)
+    if ((m[r.first - start] & PL_ISSUE_DELTA) && (m[r.second - start] & PL_ISSUE_DELTA)) lists[3].push_back(r);
+  for (auto& r : S[4].out)                     // duplicationEvents (:731-737)
+    if (r.second + 1 - r.first > 10000) lists[4].push_back(r);
+  for (auto& r : S[5].out)                     // gaps (:698)
+    if (r.second + 1 - r.first >= 10) lists[5].push_back(r);
+  for (auto& r : S[6].out) {                   // possibleBreaks: not near a gap (:720, Region.scala:37-40)
+    bool near = false;
+    for (auto& g : lists[5])
+      if ((r.first <= g.second && r.second >= g.first) || std::abs(r.second - g.first) <= 300 ||
+          std::abs(r.first - g.second) <= 300) { near = true; break; }
+    if (!near) lists[6].push_back(r);
+  }
+  static const char* const labels[7] = {"?", "X", "LowCov", "Copy#", "Duplication", "Gap", "Break"};
+  static const char* const rgbs[7] = {"0,128,128", "255,0,0", "128,0,128", "128,128,0", "255,128,0", "0,0,0", "255,0,255"};
+  for (int k = 0; k < 7; k++)
+    for (auto& r : lists[k]) {
+      const std::string a = std::to_string(r.first - 1), b = std::to_string(r.second);
+      out += name; out += '\t'; out += a; out += '\t'; out += b; out += '\t'; out += labels[k];
+      out += "\t0\t+\t"; out += a; out += '\t'; out += b; out += '\t'; out += rgbs[k]; out += '\n';
+    }
+}
+
+pl_status bed_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, const std::vector<int32_t>& contigs,
+                      uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  *text = nullptr;
+  *len = 0;
+  const int64_t G = L->G;
+  const int32_t nj = (int32_t)L->hjobs.size();
+  VcfBufs B;
+  TrackArgs T;
+  double ms = 0;
+  if (pl_status s = tracks_of_lane(c, L, contigs, B, T, &ms, true)) return s;
+  std::vector<uint8_t> mask((size_t)G);
+  HIPCHK(hipMemcpyAsync(mask.data(), T.mask, (size_t)G, hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  // regions in parallel on the host's cores, joined in batch order
+  std::vector<std::string> parts((size_t)nj);
+  const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::atomic<int32_t> next(0);
+  auto work = [&]() {
+    for (int32_t j; (j = next.fetch_add(1)) < nj;) {
+      const JobDev& J = L->hjobs[(size_t)j];
+      bed_of_region(mask.data() + J.out, J.start, J.stop, rnames[(size_t)j], parts[(size_t)j]);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int k = 1; k < std::min(nt, nj); k++) th.emplace_back(work);
+  work();
+  for (auto& t : th) t.join();
+  std::string head;
+  if (flags & PL_WIG_TRACK_LINE) head = "track description=\"Issues found by Pilon\" name=\"Pilon\"\n";
+  size_t total = head.size();
+  for (auto& p : parts) total += p.size();
+  char* out = (char*)malloc(total + 1);
+  if (!out) return fail(PL_E_OOM, "host allocation of %zu bytes failed", total);
+  size_t o = 0;
+  std::memcpy(out, head.data(), head.size());
+  o += head.size();
+  for (auto& p : parts) { std::memcpy(out + o, p.data(), p.size()); o += p.size(); }
+  out[total] = 0;
+  if (device_ms) *device_ms = ms;
+  *text = out;
+  *len = (int64_t)total;
+  return PL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -238,6 +357,22 @@ pl_status pl_region_wig(pl_region* r, int32_t track, uint32_t flags, char** text
   return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
                                const std::vector<int32_t>& contigs) {
     return wig_of_lane(c, L, names, contigs, track, flags, text, len, device_ms);
+  });
+}
+
+pl_status pl_batch_bed(pl_ctx* c, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!c || !text || !len) return fail(PL_E_INVAL, "null argument");
+  std::vector<std::string> names;
+  std::vector<int32_t> contigs;
+  if (pl_status s = batch_names(c, &c->main, &names, &contigs)) return s;
+  return bed_of_lane(c, &c->main, names, contigs, flags, text, len, device_ms);
+}
+
+pl_status pl_region_bed(pl_region* r, uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
+  return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
+                               const std::vector<int32_t>& contigs) {
+    return bed_of_lane(c, L, names, contigs, flags, text, len, device_ms);
   });
 }
 

@@ -17,6 +17,15 @@
 //                 (prefix sums staged in LDS), the rest from the pileup;
 //   k_wig_lines   one track's text (length pass, then write pass).
 
+// issue bits of a locus (GenomeRegion.scala:696-720), for Pilon.bed
+#define PL_ISSUE_UNCONFIRMED 0x01  // !confirmed(i)
+#define PL_ISSUE_CHANGED     0x02  // changed(i) || deleted(i)
+#define PL_ISSUE_LOWCOV      0x04  // lowCoverage(i)
+#define PL_ISSUE_HIGHCN      0x08  // copyNumber(i) > 1
+#define PL_ISSUE_N           0x10  // refBase(locus(i)) == 'N'
+#define PL_ISSUE_BREAK       0x20  // breakp(i)
+#define PL_ISSUE_DELTA       0x40  // deltaFraction(i) >= 0.5
+
 #define PL_GCV_MIN_ROWS 64     // contig rows (of 100 loci) per column-scan chunk, at least
 #define PL_GCV_MAX_CHUNKS 2048 // chunks per contig, at most (the serial pass between)
 #define PL_GC_STAGE (2048 + 149)
@@ -28,6 +37,10 @@ struct TrackArgs {
   int64_t* jsum;               // [n_jobs] sum of frag per region
   const uint8_t* const* gcv;   // [n_jobs] the region's contig buffer values
   int32_t* vals;               // [PL_NTRACKS][G]
+  // Pilon.bed: per-locus issue bits (null: not asked) and per-region sums
+  uint8_t* mask;               // [G] PL_ISSUE_* bits
+  int64_t* jbad;               // [n_jobs] sum of badCoverage
+  int64_t* jgood;              // [n_jobs] sum of coverage
   // one track's wig text
   int32_t track;
   const char* names;           // region (contig) names, concatenated
@@ -204,7 +217,8 @@ __global__ void __launch_bounds__(256) k_trk_frag(TrackArgs T) {
   const WinDev win = A.wins[blockIdx.x];
   const JobDev& J = A.jobs[win.job];
   const int64_t G = A.G;
-  int64_t s = 0;
+  const bool reads = A.st[win.job].read_count != 0;
+  int64_t s = 0, sgood = 0, sbad = 0;
   for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
     const int64_t g = J.out + p;
     int64_t d = 0;
@@ -219,14 +233,28 @@ __global__ void __launch_bounds__(256) k_trk_frag(TrackArgs T) {
     }
     T.frag[g] = fc;
     s += fc;
+    if (T.mask && reads) {                                   // pctBadOverall's Long sums (GenomeRegion.scala:99-113)
+      sgood += (int32_t)(d + A.pile[(int64_t)PL_F_DELETIONS * G + g]);
+      sbad += A.pile[(int64_t)PL_F_BADPAIR * G + g];
+    }
   }
   s = wave_sum(s);
-  __shared__ int64_t ws[4];
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __shared__ int64_t ws[3][4];
+  if ((threadIdx.x & 63) == 0) ws[0][threadIdx.x >> 6] = s;
+  if (T.mask) {
+    sgood = wave_sum(sgood);
+    sbad = wave_sum(sbad);
+    if ((threadIdx.x & 63) == 0) { ws[1][threadIdx.x >> 6] = sgood; ws[2][threadIdx.x >> 6] = sbad; }
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int64_t t = ws[0] + ws[1] + ws[2] + ws[3];
+    const int64_t t = ws[0][0] + ws[0][1] + ws[0][2] + ws[0][3];
     if (t) atomicAdd((unsigned long long*)&T.jsum[win.job], (unsigned long long)t);
+    if (T.mask) {
+      const int64_t tg = ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3], tb = ws[2][0] + ws[2][1] + ws[2][2] + ws[2][3];
+      if (tg) atomicAdd((unsigned long long*)&T.jgood[win.job], (unsigned long long)tg);
+      if (tb) atomicAdd((unsigned long long*)&T.jbad[win.job], (unsigned long long)tb);
+    }
   }
 }
 
@@ -275,6 +303,8 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
 
   const bool reads = S.read_count != 0;
   const double baseCov = (double)T.jsum[win.job] / (double)J.size;       // NormalDistribution.mean
+  int64_t pbo = 0;                                                       // pctBadOverall, Long
+  if (T.mask) pbo = round_div_l(100 * T.jbad[win.job], T.jgood[win.job] + T.jbad[win.job]);
   for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
     const int64_t g = J.out + p;
     // the pileup fields the tracks read (10 of the 18)
@@ -324,6 +354,26 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
     v[12] = gc;                                                                  // GC
 #pragma unroll
     for (int t = 0; t < PL_NTRACKS; t++) T.vals[(int64_t)t * G + g] = v[t];
+    if (T.mask) {
+      // the loci tests behind Pilon.bed's regions (GenomeRegion.scala:696-720)
+      const int32_t mmd = A.cf.min_min_depth;
+      uint8_t rb = R.ref[(int64_t)J.start + p - 1];
+      if (rb >= 'a' && rb <= 'z') rb = (uint8_t)(rb - 32);
+      const bool isN = rb == 'N';
+      const bool low = cov < mmd && !isN;
+      const bool clip = cov >= mmd && pct_i(v[9], cov) >= 33;
+      const bool bad_ = (int64_t)pct_i(bad, j_add(cov, bad)) > pbo + 20;
+      const bool dip = (double)v[7] / baseCov >= 1.5;
+      uint8_t m = 0;
+      if (disp != PL_D_CONFIRMED) m |= PL_ISSUE_UNCONFIRMED;
+      if (v[0] || (f & PL_CF_DELETED)) m |= PL_ISSUE_CHANGED;
+      if (low) m |= PL_ISSUE_LOWCOV;
+      if (cn > 1) m |= PL_ISSUE_HIGHCN;
+      if (isN) m |= PL_ISSUE_N;
+      if (low || clip || bad_ || dip) m |= PL_ISSUE_BREAK;
+      if ((double)v[6] / baseCov >= 0.5) m |= PL_ISSUE_DELTA;
+      T.mask[g] = m;
+    }
   }
 }
 

@@ -61,6 +61,15 @@ def _wig(fn, h, track, track_line):
         lib().pl_free(t)
 
 
+def _bed(fn, h, track_line):
+    t, n, ms = C.c_void_p(), C.c_int64(), C.c_double()
+    _check(fn(h, WIG_TRACK_LINE if track_line else 0, C.byref(t), C.byref(n), C.byref(ms)))
+    try:
+        return C.string_at(t, n.value).decode(), ms.value
+    finally:
+        lib().pl_free(t)
+
+
 class PilonError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -107,6 +116,8 @@ def lib():
                                        P(C.c_double)]),
             "pl_region_wig": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(C.c_void_p), P(C.c_int64),
                                         P(C.c_double)]),
+            "pl_batch_bed": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
+            "pl_region_bed": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
             "pl_vcf_header": (C.c_int, [P(C.c_char_p), P(C.c_int64), C.c_int32, C.c_uint32, C.c_char_p, C.c_char_p,
                                         C.c_char_p, C.c_char_p, P(C.c_void_p), P(C.c_int64)]),
         }
@@ -213,6 +224,11 @@ class Context:
         """Tracks.makeTrack text of one track over the last batch's regions:
         (text, device_ms)."""
         return _wig(lib().pl_batch_wig, self.h, track, track_line)
+
+    def batch_bed(self, track_line=True):
+        """Tracks.makeBedTrack (Pilon.bed) lines of the last batch's regions:
+        (text, device_ms)."""
+        return _bed(lib().pl_batch_bed, self.h, track_line)
 
     def batch_info(self):
         info = PlBatchInfo()
@@ -393,6 +409,10 @@ class PileUpRegion:
     def wig(self, track, track_line=True):
         """Tracks.makeTrack text of one track over this region: (text, device_ms)."""
         return _wig(lib().pl_region_wig, self.h, track, track_line)
+
+    def bed(self, track_line=True):
+        """Tracks.makeBedTrack (Pilon.bed) lines of this region: (text, device_ms)."""
+        return _bed(lib().pl_region_bed, self.h, track_line)
 
     def fix(self, new_name=None, offset=0):
         """identifyAndFixIssues + writeChanges: returns (changes text, fixed bases)."""
