@@ -98,22 +98,19 @@ pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs
   T.fpre = fpre;
   HIPCHK(hipMemcpyAsync(dgp, gp.data(), sizeof(void*) * (size_t)nj, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(T.jsum, 0, 8 * (size_t)nj, st));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, st));
+  EventPair ev;
+  HIPCHK(ev.create());
+  HIPCHK(hipEventRecord(ev.e0, st));
   k_trk_frag<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
   k_scan64_tiles<<<dim3((unsigned)tiles), dim3(256), 0, st>>>((const uint32_t*)T.frag, G, tsum);
   k_scan64_top<<<dim3(1), dim3(1024), 0, st>>>(tsum, tiles);
   k_scan64_apply<<<dim3((unsigned)tiles), dim3(256), 0, st>>>((const uint32_t*)T.frag, G, tsum, fpre);
   k_trk_values<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(e1, st));
-  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventRecord(ev.e1, st));
+  HIPCHK(hipEventSynchronize(ev.e1));
   float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIPCHK(hipEventElapsedTime(&t, ev.e0, ev.e1));
   *ms += t;
   return PL_OK;
 }
@@ -170,10 +167,9 @@ pl_status wig_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   T.names = dnames;
   T.name_off = dnoff;
   T.off = off;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, st));
+  EventPair ev;
+  HIPCHK(ev.create());
+  HIPCHK(hipEventRecord(ev.e0, st));
   k_wig_lines<false><<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
   k_scan64_tiles<<<dim3((unsigned)tiles), dim3(256), 0, st>>>(T.len, G, tsum);
   k_scan64_top<<<dim3(1), dim3(1024), 0, st>>>(tsum, tiles);
@@ -187,7 +183,7 @@ pl_status wig_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   T.text = dtext;
   k_wig_lines<true><<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventRecord(ev.e1, st));
   std::string head;
   if (flags & PL_WIG_TRACK_LINE) {
     head = std::string("track type=wiggle_0 graphType=line color=0,0,255 altColor=255,0,0 name=\"") +
@@ -198,13 +194,15 @@ pl_status wig_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   char* out = (char*)malloc(head.size() + (size_t)total + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
   std::memcpy(out, head.data(), head.size());
-  HIPCHK(hipMemcpyAsync(out + head.size(), dtext, (size_t)total, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  hipError_t ce = hipMemcpyAsync(out + head.size(), dtext, (size_t)total, hipMemcpyDeviceToHost, st);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+  if (ce != hipSuccess) {
+    free(out);
+    return fail(PL_E_DEVICE, "copy of the wig text: %s", hipGetErrorString(ce));
+  }
   out[head.size() + total] = 0;
   float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIPCHK(hipEventElapsedTime(&t, ev.e0, ev.e1));
   if (device_ms) *device_ms = ms + t;
   *text = out;
   *len = (int64_t)(head.size() + total);
@@ -304,7 +302,11 @@ pl_status bed_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
     }
   };
   std::vector<std::thread> th;
-  for (int k = 1; k < std::min(nt, nj); k++) th.emplace_back(work);
+  try {
+    for (int k = 1; k < std::min(nt, nj); k++) th.emplace_back(work);
+  } catch (const std::exception&) {
+    // no more threads: the ones started and this one take every region
+  }
   work();
   for (auto& t : th) t.join();
   std::string head;

@@ -20,6 +20,19 @@ struct VcfBufs {
 
 void add_line(std::string& s, const std::string& l) { s += l; s += '\n'; }
 
+// two HIP events bracketing a pass, destroyed on every exit
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&e0);
+    return e != hipSuccess ? e : hipEventCreate(&e1);
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -109,10 +122,9 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   uint64_t* off;
   if ((s = B.get(&off, 8 * (size_t)(G + 1)))) return s;
   V.off = off;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, st));
+  EventPair ev;
+  HIPCHK(ev.create());
+  HIPCHK(hipEventRecord(ev.e0, st));
   // indel-call columns, grown once if the first capacity was short
   uint32_t icap = (uint32_t)std::min<int64_t>(G, 1 << 20) + 64, ni = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -147,16 +159,18 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   V.text = dtext;
   k_vcf_write<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(V);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventRecord(ev.e1, st));
   char* out = (char*)malloc((size_t)total + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
-  HIPCHK(hipMemcpyAsync(out, dtext, (size_t)total, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  hipError_t ce = hipMemcpyAsync(out, dtext, (size_t)total, hipMemcpyDeviceToHost, st);
+  if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+  if (ce != hipSuccess) {
+    free(out);
+    return fail(PL_E_DEVICE, "copy of the VCF text: %s", hipGetErrorString(ce));
+  }
   out[total] = 0;
   float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIPCHK(hipEventElapsedTime(&ms, ev.e0, ev.e1));
   if (device_ms) *device_ms = ms;
   *text = out;
   *len = (int64_t)total;
