@@ -30,7 +30,13 @@ PL_HD int32_t j_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint
 PL_HD int32_t j_div(int32_t a, int32_t b) { return (a == INT_MIN && b == -1) ? a : a / b; }
 // Utils.roundDiv / pct (Utils.scala:23-26)
 PL_HD int32_t round_div_i(int32_t n, int32_t d) { return d > 0 ? j_div(j_add(n, d / 2), d) : 0; }
-PL_HD int64_t round_div_l(int64_t n, int64_t d) { return d > 0 ? (n + d / 2) / d : 0; }
+PL_HD int64_t round_div_l(int64_t n, int64_t d) {
+  if (d <= 0) return 0;
+  const int64_t x = n + d / 2;
+  // a non-negative quotient of 32-bit operands: one 32-bit division
+  if ((uint64_t)x <= 0xffffffffull && (uint64_t)d <= 0xffffffffull) return (int64_t)((uint32_t)x / (uint32_t)d);
+  return x / d;
+}
 PL_HD int32_t pct_i(int32_t n, int32_t d) { return round_div_i(j_mul(100, n), d); }
 
 // ------------------------------------------------------------ bases

@@ -47,7 +47,18 @@ struct VOut {
   __device__ void l(int64_t v) {
     uint64_t u = v < 0 ? (uint64_t)(-(v + 1)) + 1u : (uint64_t)v;
     if (v < 0) c('-');
-    if (u < 10000000000000000ull) {
+    if (u <= 0xffffffffull) {
+      // the common case: 32-bit divisions by 10 (a multiply-high each)
+      uint32_t w = (uint32_t)u;
+      uint64_t packed = 0;
+      int nd = 0;
+      do {
+        packed |= (uint64_t)(w % 10u) << (4 * nd);
+        w /= 10u;
+        nd++;
+      } while (w);
+      for (int k = nd - 1; k >= 0; k--) c((char)('0' + ((packed >> (4 * k)) & 15u)));
+    } else if (u < 10000000000000000ull) {
       // up to 16 digits, four bits each, generated with constant divisions
       uint64_t packed = 0;
       int nd = 0;
@@ -114,7 +125,7 @@ __device__ VcfCol load_vcf_col(const int32_t* pile, int64_t G, int64_t g) {
 // One Vcf.writeRecord call (Vcf.scala:74-163) for region index i at global
 // column g.  The indel string (callString) is rec's, when indelOk.
 template <bool WRITE>
-__device__ void vcf_record(VOut<WRITE>& o, const VcfArgs& V, const JobDev& J, const RSDev& R, const char* name,
+__device__ __forceinline__ void vcf_record(VOut<WRITE>& o, const VcfArgs& V, const JobDev& J, const RSDev& R, const char* name,
                            int64_t name_len, int32_t minDepth, const VcfCol& v, uint16_t cflags, int64_t rec,
                            int32_t i, bool embedded, bool indelOkArg) {
   const Col& c = v.c;
@@ -216,7 +227,7 @@ __device__ void vcf_record(VOut<WRITE>& o, const VcfArgs& V, const JobDev& J, co
 // all records of one locus: writeRecord(i, deleted(i)) and, for an indel
 // call, its second record (Vcf.scala:161)
 template <bool WRITE>
-__device__ uint32_t vcf_locus(char* out, const VcfArgs& V, const JobDev& J, const RSDev& R, const char* name,
+__device__ __forceinline__ uint32_t vcf_locus(char* out, const VcfArgs& V, const JobDev& J, const RSDev& R, const char* name,
                               int64_t name_len, int32_t minDepth, int64_t g, int32_t i) {
   VOut<WRITE> o;
   o.p = out;
@@ -295,12 +306,13 @@ __global__ void __launch_bounds__(256) k_vcf_lines(VcfArgs V) {
 }
 
 // The write pass.  A chunk of 256 loci (one per thread) is formatted into
-// LDS at the loci's relative offsets, then the chunk's bytes, contiguous in
-// the output, leave LDS as aligned dword stores of consecutive lanes.  A
-// chunk whose text exceeds the stage is written straight to HBM.
+// LDS at the loci's relative offsets, shifted so that LDS and output
+// addresses agree mod 4; the chunk's bytes, contiguous in the output, then
+// leave LDS as aligned dword loads and stores of consecutive lanes.  A chunk
+// whose text exceeds the stage is written straight to HBM.
 #define PL_VCF_STAGE 57344
 __global__ void __launch_bounds__(256) k_vcf_write(VcfArgs V) {
-  __shared__ char stage[PL_VCF_STAGE];
+  __shared__ __attribute__((aligned(16))) char stage[PL_VCF_STAGE];
   const WinDev win = V.A.wins[blockIdx.x];
   const JobDev& J = V.A.jobs[win.job];
   const RSDev& R = V.A.rs[J.rs];
@@ -314,24 +326,24 @@ __global__ void __launch_bounds__(256) k_vcf_write(VcfArgs V) {
     const int64_t g = J.out + p;
     const uint64_t base = V.off[J.out + c0];
     const uint64_t S = V.off[J.out + c1] - base;
-    if (S > PL_VCF_STAGE) {
+    const uint32_t sh = (uint32_t)(((uintptr_t)V.text + base) & 3u);   // output alignment of the chunk
+    if (S + sh > PL_VCF_STAGE) {
       if (p < c1) vcf_locus<true>(V.text + V.off[g], V, J, R, name, name_len, minDepth, g, p);
       continue;
     }
-    if (p < c1) vcf_locus<true>(stage + (V.off[g] - base), V, J, R, name, name_len, minDepth, g, p);
+    char* st = stage + sh;                                    // st and the output agree mod 4
+    if (p < c1) vcf_locus<true>(st + (V.off[g] - base), V, J, R, name, name_len, minDepth, g, p);
     __syncthreads();
     char* dst = V.text + base;
-    const uint32_t head = (uint32_t)((4u - (base & 3u)) & 3u) < S ? (uint32_t)((4u - (base & 3u)) & 3u) : (uint32_t)S;
-    if ((uint32_t)tid < head) dst[tid] = stage[tid];
+    const uint32_t h0 = (4u - sh) & 3u;
+    const uint32_t head = h0 < S ? h0 : (uint32_t)S;
+    if ((uint32_t)tid < head) dst[tid] = st[tid];
     const uint32_t nw = (uint32_t)((S - head) >> 2);
     uint32_t* dw = (uint32_t*)(dst + head);                   // 4-byte aligned
-    for (uint32_t w = tid; w < nw; w += 256) {
-      const uint32_t k = head + 4 * w;
-      dw[w] = (uint32_t)(uint8_t)stage[k] | ((uint32_t)(uint8_t)stage[k + 1] << 8) |
-              ((uint32_t)(uint8_t)stage[k + 2] << 16) | ((uint32_t)(uint8_t)stage[k + 3] << 24);
-    }
+    const uint32_t* sw = (const uint32_t*)(st + head);        // 4-byte aligned in LDS
+    for (uint32_t w = tid; w < nw; w += 256) dw[w] = sw[w];
     const uint32_t t0 = head + 4 * nw;
-    if ((uint64_t)(t0 + tid) < S) dst[t0 + tid] = stage[t0 + tid];   // at most 3 tail bytes
+    if ((uint64_t)(t0 + tid) < S) dst[t0 + tid] = st[t0 + tid];   // at most 3 tail bytes
     __syncthreads();
   }
 }
