@@ -82,6 +82,19 @@ struct Block {
 // 2-bit code of a BAM base nibble (A=1 C=2 G=4 T=8), -1 for any other code
 const int8_t kNib2[16] = {-1, 0, 1, -1, 2, -1, -1, -1, 3, -1, -1, -1, -1, -1, -1, -1};
 
+// the two bases of a SEQ byte (first base in the high nibble) as 4 bits,
+// first base in the low 2; 0xFF when either is not A/C/G/T
+struct PairTable {
+  uint8_t v[256];
+  PairTable() {
+    for (int b = 0; b < 256; b++) {
+      const int c0 = kNib2[b >> 4], c1 = kNib2[b & 15];
+      v[b] = (c0 < 0 || c1 < 0) ? 0xFF : (uint8_t)(c0 | (c1 << 2));
+    }
+  }
+};
+const PairTable kPair;
+
 // Vectors whose growth leaves the new elements uninitialised: the parallel
 // decode writes every one of them (and touches its pages first).
 template <class T>
@@ -362,30 +375,33 @@ pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, s
       const uint8_t* qual = seq + ((size_t)rc.lseq + 1) / 2;
       const bool noqual = rc.lseq == 0 || qual[0] == 0xFF;
       std::memcpy(R.quals.data() + rc.soff, qual, rc.lseq);
-      bool exo = false;
-      for (uint32_t b = 0; b < rc.lseq && !exo; b++) exo = kNib2[(seq[b >> 1] >> (4 * (1 - (b & 1)))) & 15] < 0;
+      // two bases per SEQ byte through kPair: any code but A/C/G/T makes the
+      // read exotic (the padding nibble of an odd length is not a base)
+      const uint32_t npair = rc.lseq >> 1;
+      uint8_t orp = 0;
+      for (uint32_t j = 0; j < npair; j++) orp |= kPair.v[seq[j]] == 0xFF ? 1 : 0;
+      bool exo = orp != 0 || ((rc.lseq & 1) && kNib2[seq[npair] >> 4] < 0);
       if (!exo && rc.lseq) {
-        // 2-bit codes, four to a byte; the first and last byte may hold
-        // bases of the neighbouring reads
+        // 2-bit codes, four to a byte, emitted a byte at a time; the first
+        // and last byte may hold bases of the neighbouring reads
         const int64_t s0 = rc.soff, s1 = rc.soff + rc.lseq;
         uint8_t* out = R.bases2.data();
-        int64_t g = s0;
+        int64_t byte = s0 >> 2;
         uint32_t acc = 0;
-        int nacc = 0;
-        for (uint32_t b = 0; b < rc.lseq; b++, g++) {
-          const int c = kNib2[(seq[b >> 1] >> (4 * (1 - (b & 1)))) & 15];
-          acc |= (uint32_t)c << (2 * (g & 3));
-          nacc++;
-          if ((g & 3) == 3 || g + 1 == s1) {
-            const int64_t byte = g >> 2;
-            const bool whole = (byte << 2) >= s0 && (byte << 2) + 4 <= s1;
-            if (whole) out[byte] = (uint8_t)acc;
-            else __atomic_fetch_or(&out[byte], (uint8_t)acc, __ATOMIC_RELAXED);
-            acc = 0;
-            nacc = 0;
-          }
+        int bits = 2 * (int)(s0 & 3);
+        auto emit = [&](uint8_t v) {
+          const bool whole = (byte << 2) >= s0 && (byte << 2) + 4 <= s1;
+          if (whole) out[byte] = v;
+          else __atomic_fetch_or(&out[byte], v, __ATOMIC_RELAXED);
+          byte++;
+        };
+        for (uint32_t j = 0; j < npair; j++) {
+          acc |= (uint32_t)kPair.v[seq[j]] << bits;
+          bits += 4;
+          if (bits >= 8) { emit((uint8_t)acc); acc >>= 8; bits -= 8; }
         }
-        (void)nacc;
+        if (rc.lseq & 1) { acc |= (uint32_t)kNib2[seq[npair] >> 4] << bits; bits += 2; }
+        while (bits > 0) { emit((uint8_t)acc); acc >>= 8; bits -= 8; }
       }
       exotic[(size_t)k] = exo ? 1 : 0;
       R.flag[i] = (uint16_t)((flag & 0x0FFFu) | (noqual ? PL_RF_NO_QUAL : 0u) |
