@@ -83,11 +83,6 @@ pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs
   if ((s = B.get(&dgp, sizeof(void*) * (size_t)nj)) || (s = B.get(&T.frag, 4 * (size_t)G)) ||
       (s = B.get(&T.jsum, 8 * (size_t)nj)) || (s = B.get(&T.vals, 4 * (size_t)PL_NTRACKS * (size_t)G)))
     return s;
-  uint64_t* fpre;
-  if ((s = B.get(&fpre, 8 * (size_t)(G + 1)))) return s;
-  const int64_t tiles = std::max<int64_t>(1, (G + SCAN_TILE - 1) / SCAN_TILE);
-  uint64_t* tsum;
-  if ((s = B.get(&tsum, 8 * (size_t)tiles))) return s;
   if (issues) {
     if ((s = B.get(&T.mask, (size_t)G)) || (s = B.get(&T.jbad, 8 * (size_t)nj)) || (s = B.get(&T.jgood, 8 * (size_t)nj)))
       return s;
@@ -95,16 +90,12 @@ pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs
     HIPCHK(hipMemsetAsync(T.jgood, 0, 8 * (size_t)nj, st));
   }
   T.gcv = dgp;
-  T.fpre = fpre;
   HIPCHK(hipMemcpyAsync(dgp, gp.data(), sizeof(void*) * (size_t)nj, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(T.jsum, 0, 8 * (size_t)nj, st));
   EventPair ev;
   HIPCHK(ev.create());
   HIPCHK(hipEventRecord(ev.e0, st));
   k_trk_frag<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
-  k_scan64_tiles<<<dim3((unsigned)tiles), dim3(256), 0, st>>>((const uint32_t*)T.frag, G, tsum);
-  k_scan64_top<<<dim3(1), dim3(1024), 0, st>>>(tsum, tiles);
-  k_scan64_apply<<<dim3((unsigned)tiles), dim3(256), 0, st>>>((const uint32_t*)T.frag, G, tsum, fpre);
   k_trk_values<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(T);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev.e1, st));

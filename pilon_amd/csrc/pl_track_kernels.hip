@@ -12,9 +12,9 @@
 //                 Pass 1 added a called deletion's count to the loci under
 //                 it) and its sum per region;
 //   k_trk_values  the 13 values per locus: copy number from the smoothed
-//                 fragment coverage (its 64-bit prefix sums) over the
-//                 region's mean, GC as a window sum of the buffer values
-//                 (prefix sums staged in LDS), the rest from the pileup;
+//                 fragment coverage over the region's mean, GC as a window
+//                 sum of the buffer values (both from prefix sums staged in
+//                 LDS per window), the rest from the pileup;
 //   k_wig_lines   one track's text (length pass, then write pass).
 
 // issue bits of a locus (GenomeRegion.scala:696-720), for Pilon.bed
@@ -28,12 +28,12 @@
 
 #define PL_GCV_MIN_ROWS 64     // contig rows (of 100 loci) per column-scan chunk, at least
 #define PL_GCV_MAX_CHUNKS 2048 // chunks per contig, at most (the serial pass between)
-#define PL_GC_STAGE (2048 + 149)
+#define PL_GC_STAGE (2048 + 149)     // GC buffer values a window needs, at most
+#define PL_FRAG_STAGE (2048 + 201)   // fragment coverage a window needs, at most
 
 struct TrackArgs {
   EngineArgs A;
   int32_t* frag;               // [G] fragment coverage
-  const uint64_t* fpre;        // [G + 1] exclusive prefix sums of frag (mod 2^64)
   int64_t* jsum;               // [n_jobs] sum of frag per region
   const uint8_t* const* gcv;   // [n_jobs] the region's contig buffer values
   int32_t* vals;               // [PL_NTRACKS][G]
@@ -64,18 +64,18 @@ __device__ __forceinline__ int64_t java_round_dev(double x) {
 }
 
 // GenomeRegion.smooth(fragCoverage, 200)(r) (GenomeRegion.scala:190-210)
-// from prefix sums, Int arithmetic: result(r) for 101 <= r <= n-101 is
+// in closed form, Int arithmetic: result(r) for 101 <= r <= n-101 is
 // (input(0) + sum input(r-99 .. r+100) + 100) / 200 (the first subtraction
 // drops input(1), so input(0) stays in the sum); result(0..100) is never
 // written (0), result(n-100 .. n-1) repeat result(n-101); n <= 200 gives
-// sum / n everywhere.
-__device__ __forceinline__ int32_t smooth_at(const TrackArgs& T, const JobDev& J, int32_t r) {
-  const int32_t n = J.size;
-  const uint64_t* P = T.fpre + J.out;
-  if (n <= 200) return j_div((int32_t)(uint32_t)(P[n] - P[0]), n);
+// sum / n everywhere.  F holds prefix sums (mod 2^32) of the region's
+// fragment coverage from index flo on; total is the region's sum.
+__device__ __forceinline__ int32_t smooth_at(const uint32_t* F, int32_t flo, int32_t n, int32_t in0, int64_t total,
+                                             int32_t r) {
+  if (n <= 200) return j_div((int32_t)(uint32_t)total, n);
   if (r >= n - 100) r = n - 101;
   if (r <= 100) return 0;
-  const int32_t acc = j_add(T.frag[J.out], (int32_t)(uint32_t)(P[r + 101] - P[r - 99]));
+  const int32_t acc = j_add(in0, (int32_t)(F[r + 100 - flo + 1] - F[r - 99 - flo]));
   return j_div(j_add(acc, 100), 200);
 }
 
@@ -261,7 +261,9 @@ __global__ void __launch_bounds__(256) k_trk_frag(TrackArgs T) {
 // ------------------------------------------------------------ track values
 __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
   __shared__ int32_t P[PL_GC_STAGE + 1];
+  __shared__ uint32_t F[PL_FRAG_STAGE + 1];
   __shared__ int32_t ws[4];
+  __shared__ uint32_t wf[4];
   const EngineArgs& A = T.A;
   const WinDev win = A.wins[blockIdx.x];
   const JobDev& J = A.jobs[win.job];
@@ -298,8 +300,34 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
       acc += k < 0 ? (int32_t)((k & 1) == 0) : k < n ? (int32_t)gcv[k] : 0;
       P[i + 1] = acc;
     }
+  }
+  // Fragment coverage of region indices [flo, fhi), as LDS prefix sums: the
+  // smoothing window (r-99 .. r+100, r clamped to n-101 at the right edge)
+  // and the +-100 neighbours of delta/dip of every locus of the window.
+  const int32_t nreg = J.size;
+  const int32_t edge = nreg - 101 > 0 ? nreg - 101 : 0;
+  const int32_t flo0 = (win.p0 < edge ? win.p0 : edge) - 100;
+  const int32_t flo = flo0 > 0 ? flo0 : 0;
+  const int32_t fhi = nreg < win.p1 + 101 ? nreg : win.p1 + 101;
+  {
+    const int32_t FN = fhi - flo;
+    const int32_t per = (FN + 255) / 256;
+    const int32_t i0 = threadIdx.x * per, i1 = i0 + per < FN ? i0 + per : FN;
+    uint32_t s = 0;
+    for (int32_t i = i0; i < i1; i++) s += (uint32_t)T.frag[J.out + flo + i];
+    const uint32_t inc = wave_incl_scan(s);
+    if ((threadIdx.x & 63) == 63) wf[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint32_t acc = inc - s;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) acc += wf[w];
+    if (threadIdx.x == 0) F[0] = 0;
+    for (int32_t i = i0; i < i1; i++) {
+      acc += (uint32_t)T.frag[J.out + flo + i];
+      F[i + 1] = acc;
+    }
     __syncthreads();
   }
+  const int32_t in0 = T.frag[J.out];
 
   const bool reads = S.read_count != 0;
   const double baseCov = (double)T.jsum[win.job] / (double)J.size;       // NormalDistribution.mean
@@ -325,12 +353,13 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
     const int32_t cov = reads ? (int32_t)dep : 0;
     const int32_t bad = reads ? A.pile[(int64_t)PL_F_BADPAIR * G + g] : 0;
     int16_t cn = 0;
-    if (reads && baseCov > 0) cn = (int16_t)java_round_dev((double)smooth_at(T, J, p) / baseCov);
-    const int32_t sz = J.size;
+    if (reads && baseCov > 0)
+      cn = (int16_t)java_round_dev((double)smooth_at(F, flo, nreg, in0, T.jsum[win.job], p) / baseCov);
     // deltaCoverage / dipCoverage at radius 100 (GenomeRegion.scala:672-693)
-    const int32_t l = T.frag[J.out + (p > 100 ? p - 100 : 0)];
-    const int32_t rr = T.frag[J.out + (p + 100 < sz - 1 ? p + 100 : sz - 1)];
-    const int32_t ce = T.frag[g];
+    const int32_t il = (p > 100 ? p - 100 : 0) - flo, ir = (p + 100 < nreg - 1 ? p + 100 : nreg - 1) - flo;
+    const int32_t l = (int32_t)(F[il + 1] - F[il]);
+    const int32_t rr = (int32_t)(F[ir + 1] - F[ir]);
+    const int32_t ce = (int32_t)(F[p - flo + 1] - F[p - flo]);
     const int32_t d = j_sub(l, rr);
     const int64_t cc = (int64_t)J.start + p;
     int32_t gc = 0;
