@@ -78,3 +78,51 @@ def test_c5_diploid_hotspot_shape(engine):
     # the hotspot is deep: some column holds far more than the mean depth
     depth = pile[0:4, -L:].sum(axis=0)
     assert depth.max() > 10 * np.median(depth)
+
+
+class _SoaOf:
+    """A read set handed to the oracle straight from the generator's buffers."""
+
+    def __init__(self, d, ci):
+        self.d, self.ci = d, ci
+
+    def soa(self):
+        return self.d.soa(self.ci)
+
+
+def test_c3_full_batch(engine):
+    """The many-contig batch at its full size (3,000 contigs, 60 Mb, 24 M
+    reads at 60x), as the benchmark runs it (one persistent launch,
+    autotuned shape): every field and call of every region against the
+    oracle, the regions checked on 16 host threads."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    from pilon_amd.synth import SynthData, preset
+    print("case: c3 full (3000 contigs, 60 Mb, 60x) batched vs oracle", flush=True)
+    d = SynthData(preset("c3"))
+    regions, rids = [], []
+    for ci in range(d.n):
+        engine.ref_upload(2000 + ci, d.names[ci], d.draft(ci))
+        rid = engine.readset_upload(2000 + ci, d.soa(ci))
+        regions.append((2000 + ci, 1, len(d.draft(ci)), rid))
+    pile, calls, stats = engine.batch_run(regions, window=0, block=0, persistent=1)
+    offs = np.concatenate([[0], np.cumsum([e - s + 1 for _, s, e, _ in regions])])
+    print(f"case: c3 full piled, G={int(offs[-1])}; checking", flush=True)
+
+    def check(k):
+        cid, s, e, _ = regions[k]
+        ci = cid - 2000
+        ref = O.run_region(engine.config, d.draft(ci), s, e, [_SoaOf(d, ci)], name=d.names[ci])
+        assert ref["status"] == 0, k
+        a, b = int(offs[k]), int(offs[k + 1])
+        assert_same_region(pile[:, a:b], calls[a:b], ref, f"c3 region {k}")
+        assert_same_stats(stats[k], ref["stats"], f"c3 region {k}")
+        return ref["stats"]["base_count"]
+
+    with ThreadPoolExecutor(16) as ex:
+        total = 0
+        for k, bc in enumerate(ex.map(check, range(len(regions)))):
+            total += bc
+            if k % 1000 == 999:
+                print(f"case: c3 full, {k + 1} regions match", flush=True)
+    assert total > 3_000_000_000

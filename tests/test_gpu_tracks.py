@@ -60,12 +60,20 @@ def test_tracks_kat(engine, make):
         pr.postProcess()
         got, _ = pr.tracks()
         wig, _ = pr.wig(3)
+        # after another batch on the context the region's batch runs again
+        ctx.ref_upload(161, "other", case["ref"].encode())
+        pr2 = PileUpRegion(ctx, 161, case["start"], case["stop"])
+        pr2.addReads(K.batch_of(case))
+        pr2.postProcess()
+        again, _ = pr.tracks()
+        pr2.close()
         pr.close()
     finally:
         if own is not None:
             own.close()
     ref = O.run_region(cfg, case["ref"], case["start"], case["stop"], [K.batch_of(case)], name="ctg", vcf=O.TRACKS)
     same_tracks(got, ref["tracks"], make.__name__)
+    assert np.array_equal(again, got)
     assert wig == wig_text(3, [("ctg", case["start"], ref["tracks"])])
 
 
