@@ -54,6 +54,11 @@ def lib():
         L.orc_fmt_af.argtypes = [C.c_float, C.c_char_p]
         L.orc_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.orc_gc.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p]
+        L.orc_kmer_graph.restype = C.c_int32
+        L.orc_kmer_graph.argtypes = [C.POINTER(PlConfig), C.POINTER(PlReadSoa), C.c_int32, C.c_int32,
+                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_int64)]
+        L.orc_free.argtypes = [C.c_void_p]
         L.orc_summary_regions.restype = C.c_int32
         L.orc_summary_regions.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32]
         L.orc_java_round.restype = C.c_int64
@@ -196,3 +201,24 @@ def summary_regions(test, start, slop=100):
     out = np.zeros(2 * cap, np.int32)
     n = lib().orc_summary_regions(t.ctypes.data, len(t), start, slop, out.ctypes.data, cap)
     return [(int(out[2 * k]), int(out[2 * k + 1])) for k in range(n)]
+
+
+def kmer_graph(cfg, batch, K=47, min_depth=5):
+    """Assembler.addReads + buildGraph over one read batch: (status, [(kmer,
+    kGraph base, altGraph base or None)] in byte order of the K-mers)."""
+    L = lib()
+    soa = batch.soa()
+    k, nx, al = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    n = C.c_int64()
+    st = L.orc_kmer_graph(C.byref(cfg), C.byref(soa), K, min_depth, C.byref(k), C.byref(nx), C.byref(al),
+                          C.byref(n))
+    out = []
+    if st == 0 and n.value:
+        ks = C.string_at(k, K * n.value)
+        ns = C.string_at(nx, n.value)
+        als = C.string_at(al, n.value)
+        out = [(ks[K * i:K * i + K].decode(), chr(ns[i]), chr(als[i]) if als[i] else None) for i in range(n.value)]
+    for p in (k, nx, al):
+        if p.value:
+            L.orc_free(p)
+    return st, out

@@ -1283,3 +1283,106 @@ void orc_basecall(const pl_config* cfg, const int64_t* bc4, const int64_t* qs4,
   *score = bc.score;
   free(p.ins.v); free(p.del.v);
 }
+
+/* ------------------------------------------------------ Assembler graph */
+/* Assembler.addReads + buildGraph (Assembler.scala:46-85,131-153): every
+ * K-mer of every read of length > K, on both strands (reverseComplement
+ * maps A/C/G/T and keeps every other letter, Bases.scala:47-60; quals
+ * reversed), piles the base after it with PileUp.add(base, qual, mq)
+ * (PileUp.scala:75-84: A/C/G/T only, qual >= minQual; mq is the read's
+ * MAPQ).  K-mers of depth >= minDepth get kGraph's edge (the called base)
+ * and, when the call is not homozygous, altGraph's (the alternative base).
+ * The pileups are aggregated by sorting the K-mer occurrences, which gives
+ * the same sums as the HashMap (integer sums are order-free).  With 100,000
+ * or more such reads Assembler.addRead prunes the map (:76), which this
+ * restatement does not do: it returns PL_E_INVAL instead.
+ * Output: the K-mers in byte order, K bytes each; next[i] the kGraph base,
+ * alt[i] the altGraph base or 0. */
+typedef struct { const uint8_t* k; uint8_t next; int8_t qual; int32_t mq; } KOcc;
+static int32_t g_kmer_k;
+static int cmp_kocc(const void* a, const void* b) {
+  return memcmp(((const KOcc*)a)->k, ((const KOcc*)b)->k, (size_t)g_kmer_k);
+}
+
+int32_t orc_kmer_graph(const pl_config* cfg, const pl_read_soa* R, int32_t K, int32_t minDepth,
+                       uint8_t** kmers, uint8_t** next, uint8_t** alt, int64_t* n_out) {
+  static const char NIB[] = "=ACMGRSVTWYHKDBN";
+  *kmers = NULL; *next = NULL; *alt = NULL; *n_out = 0;
+  if (K < 1 || K > 48) return PL_E_INVAL;
+  int64_t nocc = 0, nreads = 0, chars = 0;
+  for (int64_t i = 0; i < R->n_reads; i++) {
+    const int64_t L = R->seq_off[i + 1] - R->seq_off[i];
+    if (L > K) { nreads++; nocc += 2 * (L - K); chars += 2 * L; }
+  }
+  if (nreads >= 100000) return PL_E_INVAL;
+  uint8_t* strands = (uint8_t*)malloc((size_t)chars + 1);
+  KOcc* occ = (KOcc*)malloc(sizeof(KOcc) * (size_t)(nocc + 1));
+  int64_t co = 0, no = 0, ex = 0;
+  int8_t* q = NULL; int64_t qcap = 0;
+  for (int64_t i = 0; i < R->n_reads; i++) {
+    const int64_t s0 = R->seq_off[i], L = R->seq_off[i + 1] - s0;
+    while (ex < R->n_exotic && R->exo_read[ex] < i) ex++;
+    const int exotic = ex < R->n_exotic && R->exo_read[ex] == i;
+    if (L <= K) continue;
+    uint8_t* fw = strands + co;
+    uint8_t* rc = fw + L;
+    co += 2 * L;
+    if (L > qcap) { qcap = L; q = (int8_t*)realloc(q, (size_t)qcap); }
+    for (int64_t j = 0; j < L; j++) {                       /* getReadString */
+      uint8_t c;
+      if (exotic) {
+        const int64_t k = R->exo_off[ex] + j;
+        c = (uint8_t)NIB[(R->nib[k >> 1] >> ((k & 1) ? 0 : 4)) & 15];
+      } else {
+        const int64_t k = s0 + j;
+        c = (uint8_t)"ACGT"[(R->bases2[k >> 2] >> (2 * (k & 3))) & 3];
+      }
+      fw[j] = c;
+      q[j] = (R->flag[i] & PL_RF_NO_QUAL) ? (int8_t)cfg->default_qual : R->quals[s0 + j];
+    }
+    for (int64_t j = 0; j < L; j++) {                       /* Bases.reverseComplement */
+      const uint8_t c = fw[L - 1 - j];
+      rc[j] = c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : c == 'T' ? 'A' : c;
+    }
+    const int32_t mq = R->mapq[i];
+    for (int64_t o = 0; o + K < L; o++) {                   /* addToPileups, offsets 0 .. L-K-1 */
+      occ[no++] = (KOcc){fw + o, fw[o + K], q[o + K], mq};
+      occ[no++] = (KOcc){rc + o, rc[o + K], q[L - 1 - (o + K)], mq};
+    }
+  }
+  free(q);
+  g_kmer_k = K;
+  qsort(occ, (size_t)no, sizeof(KOcc), cmp_kocc);
+  uint8_t* ko = (uint8_t*)malloc((size_t)K * (size_t)(no + 1));
+  uint8_t* nx = (uint8_t*)malloc((size_t)no + 1);
+  uint8_t* al = (uint8_t*)malloc((size_t)no + 1);
+  int64_t n = 0;
+  for (int64_t a = 0; a < no;) {
+    int64_t b = a + 1;
+    while (b < no && memcmp(occ[a].k, occ[b].k, (size_t)K) == 0) b++;
+    PileUp p; memset(&p, 0, sizeof p);
+    for (int64_t t = a; t < b; t++) {                       /* PileUp.add */
+      const uint8_t c = occ[t].next;
+      const int bi = c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : -1;
+      if (bi >= 0 && occ[t].qual >= cfg->min_qual) {
+        const int32_t mq1 = occ[t].mq + 1;
+        p.bc[bi] += 1;
+        p.qs[bi] += (int64_t)jmul(occ[t].qual, mq1);
+        p.mqSum = jadd(p.mqSum, mq1);
+        p.qSum = jadd(p.qSum, occ[t].qual);
+      }
+    }
+    if (pu_depth(&p) >= minDepth) {                         /* buildGraph */
+      BaseCall bc; baseCall(&p, cfg, &bc);
+      memcpy(ko + (size_t)K * (size_t)n, occ[a].k, (size_t)K);
+      nx[n] = (uint8_t)"ACGTN"[bc.base];
+      al[n] = bc.homo ? 0 : (uint8_t)"ACGT"[bc.alt];
+      n++;
+    }
+    a = b;
+  }
+  free(occ); free(strands);
+  *kmers = ko; *next = nx; *alt = al; *n_out = n;
+  return PL_OK;
+}
+void orc_free(void* p) { free(p); }
