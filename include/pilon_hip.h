@@ -15,6 +15,7 @@
  *   GenomeRegion.writeVcf / Vcf            pilon/GenomeRegion.scala:623-643, pilon/Vcf.scala:28-163
  *   Tracks.standardTracks, makeTrack,      pilon/Tracks.scala:33-216,
  *   makeBedTrack; Pass 2, computeGc        pilon/GenomeRegion.scala:158-210,273-283,696-763
+ *   Assembler.addReads / buildGraph        pilon/Assembler.scala:46-85,131-153
  *
  * Conventions: plain C types only; 0 = success; the library owns device
  * memory, the caller owns host memory; one pl_ctx per host thread; objects
@@ -294,6 +295,19 @@ pl_status pl_region_wig(pl_region* reg, int32_t track, uint32_t flags, char** te
  * "track description=..." line first.  *text is malloc'ed (pl_free). */
 pl_status pl_batch_bed(pl_ctx* ctx, uint32_t flags, char** text, int64_t* len, double* device_ms);
 pl_status pl_region_bed(pl_region* reg, uint32_t flags, char** text, int64_t* len, double* device_ms);
+
+/* ----------------------------------------------------- assembler K-mers */
+/* Assembler.addReads + buildGraph (Assembler.scala:46-85,131-153), the
+ * K-mer pileups of Pilon's local reassembly: every K-mer of every read
+ * longer than K, on both strands, piles the base after it (PileUp.add, with
+ * the read's MAPQ); each K-mer of depth >= min_depth gets its kGraph edge
+ * (the called base, next[i]) and, when the call is heterozygous, its
+ * altGraph edge (alt[i], else 0).  *kmers holds *n K-mers of K bytes each,
+ * in byte order.  K in [1, 48]; PL_E_INVAL for 100,000 or more reads longer
+ * than K (Assembler.addRead prunes its map there, which this pass does not).
+ * The three arrays are malloc'ed (pl_free). */
+pl_status pl_kmer_graph(pl_ctx* ctx, const pl_read_soa* reads, int32_t K, int32_t min_depth, char** kmers,
+                        uint8_t** next, uint8_t** alt, int64_t* n, double* device_ms);
 
 /* Sizes and HIP-event times of the last batch (pl_batch_run or
  * pl_region_finish). */

@@ -1117,3 +1117,5 @@ pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
 #include "pl_serve.hip"
 #include "pl_vcf.hip"
 #include "pl_track.hip"
+#include "pl_kmer_kernels.hip"
+#include "pl_kmer.hip"

@@ -1,0 +1,159 @@
+// pl_kmer_kernels.hip — the K-mer pileups of Pilon's local assembler
+// (Assembler.addReads / addToPileups / buildGraph, Assembler.scala:46-85,
+// 131-153) on the device.  Included at the end of pl_kernels.hip.
+//
+//   k_kmer_emit   one thread per K-mer occurrence (read, strand, offset):
+//                 the K-mer as 4-bit BAM codes packed in six 32-bit words
+//                 (first base most significant), and the base after it
+//                 with its quality and the read's MAPQ;
+//   (a stable LSD radix sort of the occurrences by the six words)
+//   k_kmer_heads  the first occurrence of each distinct K-mer;
+//   k_kmer_agg    PileUp.add of every occurrence into its K-mer's sums;
+//   k_kmer_call   BaseCall of every K-mer with depth >= minDepth: kGraph's
+//                 base and, when not homozygous, altGraph's.
+
+#define PL_KMER_WORDS 6          // K <= 48: eight 4-bit codes per word
+
+struct KmerArgs {
+  int64_t n_reads;
+  const int64_t* seq_off;        // [n_reads + 1]
+  const uint8_t* bases2;
+  const int8_t* quals;
+  const uint16_t* flag;
+  const uint8_t* mapq;
+  const int64_t* exo_of;         // [n_reads] first nibble of a non-ACGT read, -1 otherwise
+  const uint8_t* nib;
+  const int64_t* occ_off;        // [n_reads + 1] first occurrence of each read
+  uint32_t n_occ;
+  int32_t K, words;
+  int32_t default_qual, min_qual, min_depth;
+  uint32_t* kw;                  // [PL_KMER_WORDS][n_occ]
+  uint32_t* val;                 // [n_occ] code | qual << 8 | mapq << 16
+  const uint32_t* perm;          // [n_occ] occurrences in K-mer order
+  uint32_t* head;                // [n_occ]
+  const uint32_t* seg;           // [n_occ] heads before each sorted occurrence (exclusive scan)
+  uint32_t n_seg;
+  uint32_t* cnt;                 // [4][n_seg]
+  unsigned long long* qsum;      // [4][n_seg]
+  int32_t* mqs;                  // [n_seg]
+  int32_t* qs;                   // [n_seg]
+  uint32_t* first;               // [n_seg] an occurrence of the K-mer
+  uint32_t* out_n;
+  uint32_t* out_kw;              // [PL_KMER_WORDS][n_seg]
+  uint8_t* out_next;
+  uint8_t* out_alt;
+};
+
+namespace {
+
+// 4-bit BAM code of base j of read r (getReadString)
+__device__ __forceinline__ uint32_t kmer_code(const KmerArgs& A, int64_t r, int64_t j) {
+  const int64_t x = A.exo_of[r];
+  if (x >= 0) {
+    const int64_t k = x + j;
+    return (A.nib[k >> 1] >> ((k & 1) ? 0 : 4)) & 15u;
+  }
+  const int64_t k = A.seq_off[r] + j;
+  return 1u << ((A.bases2[k >> 2] >> (2 * (k & 3))) & 3u);
+}
+
+// Bases.complement (Bases.scala:47-53): A<->T, C<->G, any other code kept
+__device__ __forceinline__ uint32_t kmer_comp(uint32_t c) {
+  return c == 1u ? 8u : c == 8u ? 1u : c == 2u ? 4u : c == 4u ? 2u : c;
+}
+
+__device__ __forceinline__ int32_t kmer_qual(const KmerArgs& A, int64_t r, int64_t j) {
+  return (A.flag[r] & PL_RF_NO_QUAL) ? A.default_qual : (int32_t)A.quals[A.seq_off[r] + j];
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_kmer_emit(KmerArgs A) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)A.n_occ) return;
+  int64_t lo = 0, hi = A.n_reads;                          // the read: last occ_off <= t
+  while (hi - lo > 1) {
+    const int64_t m = (lo + hi) >> 1;
+    if (A.occ_off[m] <= t) lo = m; else hi = m;
+  }
+  const int64_t r = lo;
+  const int64_t local = t - A.occ_off[r];
+  const bool rc = local & 1;
+  const int64_t o = local >> 1;
+  const int64_t L = A.seq_off[r + 1] - A.seq_off[r];
+  uint32_t w[PL_KMER_WORDS] = {0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < A.K; j++) {
+    const int64_t p = o + j;
+    const uint32_t c = rc ? kmer_comp(kmer_code(A, r, L - 1 - p)) : kmer_code(A, r, p);
+    w[j >> 3] |= c << (28 - 4 * (j & 7));
+  }
+  const int64_t pn = o + A.K;
+  const uint32_t next = rc ? kmer_comp(kmer_code(A, r, L - 1 - pn)) : kmer_code(A, r, pn);
+  const int32_t qual = rc ? kmer_qual(A, r, L - 1 - pn) : kmer_qual(A, r, pn);
+#pragma unroll
+  for (int k = 0; k < PL_KMER_WORDS; k++)
+    if (k < A.words) A.kw[(int64_t)k * A.n_occ + t] = w[k];
+  A.val[t] = next | ((uint32_t)(uint8_t)(int8_t)qual << 8) | ((uint32_t)A.mapq[r] << 16);
+}
+
+// keys[i] = word k of the i-th occurrence in the current order
+__global__ void __launch_bounds__(256) k_kmer_gather(const uint32_t* kw, const uint32_t* perm, uint32_t n, uint32_t* keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = kw[perm[i]];
+}
+
+__global__ void __launch_bounds__(256) k_kmer_heads(KmerArgs A) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n_occ) return;
+  uint32_t h = 1;
+  if (i > 0) {
+    const uint32_t a = A.perm[i], b = A.perm[i - 1];
+    h = 0;
+    for (int k = 0; k < A.words; k++) h |= A.kw[(int64_t)k * A.n_occ + a] != A.kw[(int64_t)k * A.n_occ + b];
+  }
+  A.head[i] = h;
+}
+
+// PileUp.add(base, qual, mq) (PileUp.scala:75-84) of every occurrence
+__global__ void __launch_bounds__(256) k_kmer_agg(KmerArgs A) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n_occ) return;
+  const uint32_t s = A.seg[i] + A.head[i] - 1u;             // seg is the exclusive scan of head
+  const uint32_t p = A.perm[i];
+  if (A.head[i]) A.first[s] = p;
+  const uint32_t v = A.val[p];
+  const uint32_t c = v & 15u;
+  const int bi = c == 1u ? 0 : c == 2u ? 1 : c == 4u ? 2 : c == 8u ? 3 : -1;
+  const int32_t qual = (int32_t)(int8_t)(uint8_t)(v >> 8);
+  const int32_t mq1 = (int32_t)(v >> 16) + 1;
+  if (bi >= 0 && qual >= A.min_qual) {
+    atomicAdd(&A.cnt[(int64_t)bi * A.n_seg + s], 1u);
+    atomicAdd(&A.qsum[(int64_t)bi * A.n_seg + s], (unsigned long long)(int64_t)j_mul(qual, mq1));
+    atomicAdd(&A.mqs[s], mq1);
+    atomicAdd(&A.qs[s], qual);
+  }
+}
+
+// buildGraph (Assembler.scala:131-150) for every K-mer
+__global__ void __launch_bounds__(256) k_kmer_call(KmerArgs A) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  Col c;
+  if (s < A.n_seg) {
+    for (int b = 0; b < 4; b++) {
+      c.bc[b] = A.cnt[(int64_t)b * A.n_seg + s];
+      c.qs[b] = (int64_t)A.qsum[(int64_t)b * A.n_seg + s];
+    }
+    c.mqSum = A.mqs[s];
+    c.qSum = A.qs[s];
+    c.deletions = c.insertions = c.insQual = c.delQual = 0;
+    keep = c.depth() >= A.min_depth;
+  }
+  const uint32_t k = wave_append(A.out_n, keep);
+  if (!keep) return;
+  const BaseFacts f = base_facts(c);
+  const uint32_t p = A.first[s];
+  for (int w = 0; w < A.words; w++) A.out_kw[(int64_t)w * A.n_seg + k] = A.kw[(int64_t)w * A.n_occ + p];
+  A.out_next[k] = (uint8_t)"ACGTN"[f.base];
+  A.out_alt[k] = f.homo ? (uint8_t)0 : (uint8_t)"ACGT"[f.alt];
+}

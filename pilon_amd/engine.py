@@ -116,6 +116,8 @@ def lib():
                                        P(C.c_double)]),
             "pl_region_wig": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, P(C.c_void_p), P(C.c_int64),
                                         P(C.c_double)]),
+            "pl_kmer_graph": (C.c_int, [C.c_void_p, P(PlReadSoa), C.c_int32, C.c_int32, P(C.c_void_p),
+                                        P(C.c_void_p), P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
             "pl_batch_bed": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
             "pl_region_bed": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
             "pl_vcf_header": (C.c_int, [P(C.c_char_p), P(C.c_int64), C.c_int32, C.c_uint32, C.c_char_p, C.c_char_p,
@@ -229,6 +231,28 @@ class Context:
         """Tracks.makeBedTrack (Pilon.bed) lines of the last batch's regions:
         (text, device_ms)."""
         return _bed(lib().pl_batch_bed, self.h, track_line)
+
+    def kmer_graph(self, reads, K=47, min_depth=5):
+        """Assembler.addReads + buildGraph on the device: ([(kmer, kGraph base,
+        altGraph base or None)] in byte order of the K-mers, device_ms)."""
+        soa = reads.soa() if hasattr(reads, "soa") else reads
+        k, nx, al = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        n, ms = C.c_int64(), C.c_double()
+        _check(lib().pl_kmer_graph(self.h, C.byref(soa), K, min_depth, C.byref(k), C.byref(nx), C.byref(al),
+                                   C.byref(n), C.byref(ms)))
+        try:
+            m = n.value
+            if not m:
+                return [], ms.value
+            ks = C.string_at(k, K * m)
+            ns = C.string_at(nx, m)
+            als = C.string_at(al, m)
+            return [(ks[K * i:K * i + K].decode(), chr(ns[i]), chr(als[i]) if als[i] else None)
+                    for i in range(m)], ms.value
+        finally:
+            for p in (k, nx, al):
+                if p.value:
+                    lib().pl_free(p)
 
     def batch_info(self):
         info = PlBatchInfo()
