@@ -171,6 +171,8 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--vcf", type=int, default=1, help="also format the batch's VCF records on the device")
     ap.add_argument("--tracks", type=int, default=1, help="also compute the batch's 13 per-locus tracks on the device")
+    ap.add_argument("--kmer", type=int, default=1,
+                    help="also build the assembler's K-mer graph of 99,999 reads of the largest contig on the device")
     ap.add_argument("--bam", type=int, default=1,
                     help="also run the BAM producer: the workload written as a BAM, decoded, uploaded and piled")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -321,6 +323,26 @@ def main():
             f"Coverage.wig {tracks['wig_coverage_bytes'] / 1e6:.0f} MB in {wms:.1f} ms, "
             f"Pilon.bed {tracks['bed_lines']} lines in {bwall * 1000:.1f} ms wall")
 
+    kmer = None
+    if args.kmer:
+        ci = max(range(data.n), key=lambda i: len(data.draft(i)))
+        b = data.batch(ci)
+        b = b.slice(0, min(99_999, b.n))
+        lens = np.diff(b.seq_off)
+        occ = int(2 * np.clip(lens - 47, 0, None).sum())
+        ctx.kmer_graph(b, K=47)
+        kms = []
+        for _ in range(3):
+            g, ms = ctx.kmer_graph(b, K=47)
+            kms.append(ms)
+        kmer = {"reads": int(b.n), "K": 47, "occurrences": occ, "graph_kmers": len(g),
+                "device_ms": float(np.mean(kms)), "occurrences_per_s": occ / (float(np.mean(kms)) / 1000.0),
+                "note": "Assembler.addReads + buildGraph (K-mer pileups of local reassembly) on the device; "
+                        "device_ms covers emission, the six radix sorts by K-mer, the sums and the calls"}
+        del g
+        log(f"bench: K-mer graph: {b.n} reads, {occ} K-mer occurrences -> {kmer['graph_kmers']} graph K-mers "
+            f"in {kmer['device_ms']:.2f} ms on the device ({kmer['occurrences_per_s'] / 1e9:.2f} G occurrences/s)")
+
     producer = None
     if args.bam:
         producer = bam_producer(data, ctx, args, W, B, aligned)
@@ -352,6 +374,7 @@ def main():
         "bam_producer": producer,
         "vcf": vcf,
         "tracks": tracks,
+        "kmer_graph": kmer,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

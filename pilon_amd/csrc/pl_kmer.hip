@@ -1,8 +1,8 @@
 // pl_kmer.hip — the host side of the assembler's K-mer graph (included at
 // the end of pl_engine.hip): the reads go to the device, the occurrences
 // are emitted, sorted by K-mer (six stable 32-bit radix sorts, last word
-// first), aggregated and called by the k_kmer_* kernels; the kept K-mers
-// come back in byte order.
+// first), summed and called per K-mer by the k_kmer_* kernels; the kept
+// K-mers come back in byte order.
 
 namespace {
 
@@ -98,17 +98,13 @@ pl_status kmer_graph(pl_ctx* c, const pl_read_soa* R, int32_t K, int32_t min_dep
   HIPCHK(hipStreamSynchronize(st));
   A.seg = seg;
   A.n_seg = ns;
-  if ((s = B.get(&A.cnt, 16 * (size_t)ns)) || (s = B.get(&A.qsum, 32 * (size_t)ns)) || (s = B.get(&A.mqs, 4 * (size_t)ns)) ||
-      (s = B.get(&A.qs, 4 * (size_t)ns)) || (s = B.get(&A.first, 4 * (size_t)ns)) || (s = B.get(&A.out_n, 4)) ||
+  if ((s = B.get(&A.starts, 4 * ((size_t)ns + 1))) || (s = B.get(&A.out_n, 4)) ||
       (s = B.get(&A.out_kw, 4 * (size_t)A.words * (size_t)ns)) || (s = B.get(&A.out_next, (size_t)ns)) ||
       (s = B.get(&A.out_alt, (size_t)ns)))
     return s;
-  HIPCHK(hipMemsetAsync(A.cnt, 0, 16 * (size_t)ns, st));
-  HIPCHK(hipMemsetAsync(A.qsum, 0, 32 * (size_t)ns, st));
-  HIPCHK(hipMemsetAsync(A.mqs, 0, 4 * (size_t)ns, st));
-  HIPCHK(hipMemsetAsync(A.qs, 0, 4 * (size_t)ns, st));
   HIPCHK(hipMemsetAsync(A.out_n, 0, 4, st));
-  k_kmer_agg<<<dim3(gb), dim3(256), 0, st>>>(A);
+  HIPCHK(hipMemcpyAsync(A.starts + ns, &nocc, 4, hipMemcpyHostToDevice, st));
+  k_kmer_starts<<<dim3(gb), dim3(256), 0, st>>>(A);
   k_kmer_call<<<dim3((ns + 255) / 256), dim3(256), 0, st>>>(A);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev.e1, st));

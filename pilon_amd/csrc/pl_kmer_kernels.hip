@@ -7,10 +7,12 @@
 //                 (first base most significant), and the base after it
 //                 with its quality and the read's MAPQ;
 //   (a stable LSD radix sort of the occurrences by the six words)
-//   k_kmer_heads  the first occurrence of each distinct K-mer;
-//   k_kmer_agg    PileUp.add of every occurrence into its K-mer's sums;
-//   k_kmer_call   BaseCall of every K-mer with depth >= minDepth: kGraph's
-//                 base and, when not homozygous, altGraph's.
+//   k_kmer_heads  the first occurrence of each distinct K-mer (then an
+//                 exclusive scan numbers the K-mers, k_kmer_starts lists
+//                 where each begins);
+//   k_kmer_call   one thread per K-mer: PileUp.add of its occurrences and,
+//                 at depth >= minDepth, BaseCall: kGraph's base and, when
+//                 not homozygous, altGraph's.
 
 #define PL_KMER_WORDS 6          // K <= 48: eight 4-bit codes per word
 
@@ -33,11 +35,7 @@ struct KmerArgs {
   uint32_t* head;                // [n_occ]
   const uint32_t* seg;           // [n_occ] heads before each sorted occurrence (exclusive scan)
   uint32_t n_seg;
-  uint32_t* cnt;                 // [4][n_seg]
-  unsigned long long* qsum;      // [4][n_seg]
-  int32_t* mqs;                  // [n_seg]
-  int32_t* qs;                   // [n_seg]
-  uint32_t* first;               // [n_seg] an occurrence of the K-mer
+  uint32_t* starts;              // [n_seg + 1] first sorted occurrence of each K-mer
   uint32_t* out_n;
   uint32_t* out_kw;              // [PL_KMER_WORDS][n_seg]
   uint8_t* out_next;
@@ -114,45 +112,43 @@ __global__ void __launch_bounds__(256) k_kmer_heads(KmerArgs A) {
   A.head[i] = h;
 }
 
-// PileUp.add(base, qual, mq) (PileUp.scala:75-84) of every occurrence
-__global__ void __launch_bounds__(256) k_kmer_agg(KmerArgs A) {
+// the first sorted occurrence of each K-mer (starts[n_seg] = n_occ is set by the host)
+__global__ void __launch_bounds__(256) k_kmer_starts(KmerArgs A) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n_occ) return;
-  const uint32_t s = A.seg[i] + A.head[i] - 1u;             // seg is the exclusive scan of head
-  const uint32_t p = A.perm[i];
-  if (A.head[i]) A.first[s] = p;
-  const uint32_t v = A.val[p];
-  const uint32_t c = v & 15u;
-  const int bi = c == 1u ? 0 : c == 2u ? 1 : c == 4u ? 2 : c == 8u ? 3 : -1;
-  const int32_t qual = (int32_t)(int8_t)(uint8_t)(v >> 8);
-  const int32_t mq1 = (int32_t)(v >> 16) + 1;
-  if (bi >= 0 && qual >= A.min_qual) {
-    atomicAdd(&A.cnt[(int64_t)bi * A.n_seg + s], 1u);
-    atomicAdd(&A.qsum[(int64_t)bi * A.n_seg + s], (unsigned long long)(int64_t)j_mul(qual, mq1));
-    atomicAdd(&A.mqs[s], mq1);
-    atomicAdd(&A.qs[s], qual);
-  }
+  if (i < A.n_occ && A.head[i]) A.starts[A.seg[i]] = i;
 }
 
-// buildGraph (Assembler.scala:131-150) for every K-mer
+// one thread per K-mer: PileUp.add (PileUp.scala:75-84) of its occurrences,
+// contiguous after the sort, then buildGraph's call (Assembler.scala:131-150)
 __global__ void __launch_bounds__(256) k_kmer_call(KmerArgs A) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   bool keep = false;
   Col c;
+  uint32_t i0 = 0;
   if (s < A.n_seg) {
-    for (int b = 0; b < 4; b++) {
-      c.bc[b] = A.cnt[(int64_t)b * A.n_seg + s];
-      c.qs[b] = (int64_t)A.qsum[(int64_t)b * A.n_seg + s];
+    for (int b = 0; b < 4; b++) { c.bc[b] = 0; c.qs[b] = 0; }
+    c.mqSum = c.qSum = c.deletions = c.insertions = c.insQual = c.delQual = 0;
+    i0 = A.starts[s];
+    const uint32_t i1 = A.starts[s + 1];
+    for (uint32_t i = i0; i < i1; i++) {
+      const uint32_t v = A.val[A.perm[i]];
+      const uint32_t code = v & 15u;
+      const int bi = code == 1u ? 0 : code == 2u ? 1 : code == 4u ? 2 : code == 8u ? 3 : -1;
+      const int32_t qual = (int32_t)(int8_t)(uint8_t)(v >> 8);
+      const int32_t mq1 = (int32_t)(v >> 16) + 1;
+      if (bi >= 0 && qual >= A.min_qual) {
+        c.bc[bi] += 1;
+        c.qs[bi] += (int64_t)j_mul(qual, mq1);
+        c.mqSum = j_add(c.mqSum, mq1);
+        c.qSum = j_add(c.qSum, qual);
+      }
     }
-    c.mqSum = A.mqs[s];
-    c.qSum = A.qs[s];
-    c.deletions = c.insertions = c.insQual = c.delQual = 0;
     keep = c.depth() >= A.min_depth;
   }
   const uint32_t k = wave_append(A.out_n, keep);
   if (!keep) return;
   const BaseFacts f = base_facts(c);
-  const uint32_t p = A.first[s];
+  const uint32_t p = A.perm[i0];
   for (int w = 0; w < A.words; w++) A.out_kw[(int64_t)w * A.n_seg + k] = A.kw[(int64_t)w * A.n_occ + p];
   A.out_next[k] = (uint8_t)"ACGTN"[f.base];
   A.out_alt[k] = f.homo ? (uint8_t)0 : (uint8_t)"ACGT"[f.alt];
