@@ -116,10 +116,8 @@ pl_status tracks_copy(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs, i
   TrackArgs T;
   double ms = 0;
   if (pl_status s = tracks_of_lane(c, L, contigs, B, T, &ms)) return s;
-  if (values) {
-    HIPCHK(hipMemcpyAsync(values, T.vals, 4 * (size_t)PL_NTRACKS * (size_t)G, hipMemcpyDeviceToHost, L->stream));
-    HIPCHK(hipStreamSynchronize(L->stream));
-  }
+  if (values)
+    if (pl_status cs = copy_to_host(values, T.vals, 4 * (size_t)PL_NTRACKS * (size_t)G, L->stream)) return cs;
   if (device_ms) *device_ms = ms;
   return PL_OK;
 }
@@ -185,11 +183,9 @@ pl_status wig_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   char* out = (char*)malloc(head.size() + (size_t)total + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
   std::memcpy(out, head.data(), head.size());
-  hipError_t ce = hipMemcpyAsync(out + head.size(), dtext, (size_t)total, hipMemcpyDeviceToHost, st);
-  if (ce == hipSuccess) ce = hipStreamSynchronize(st);
-  if (ce != hipSuccess) {
+  if (pl_status cs = copy_to_host(out + head.size(), dtext, (size_t)total, st)) {
     free(out);
-    return fail(PL_E_DEVICE, "copy of the wig text: %s", hipGetErrorString(ce));
+    return cs;
   }
   out[head.size() + total] = 0;
   float t = 0.f;

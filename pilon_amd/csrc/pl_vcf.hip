@@ -20,6 +20,55 @@ struct VcfBufs {
 
 void add_line(std::string& s, const std::string& l) { s += l; s += '\n'; }
 
+// A device-to-host copy into pageable memory through two pinned staging
+// buffers: the DMA of one chunk overlaps the host copy of the one before.
+// Small copies go directly.  Returns with the data in dst.
+pl_status copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  constexpr size_t kChunk = (size_t)32 << 20;
+  if (bytes <= 2 * kChunk) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PL_OK;
+  }
+  struct Stage {
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Stage() {
+      for (int k = 0; k < 2; k++) {
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        if (pin[k]) (void)hipHostFree(pin[k]);
+      }
+    }
+  } S;
+  for (int k = 0; k < 2; k++) {
+    if (hipHostMalloc(&S.pin[k], kChunk, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));   // no pinned memory: direct
+      HIPCHK(hipStreamSynchronize(st));
+      return PL_OK;
+    }
+    HIPCHK(hipEventCreate(&S.ev[k]));
+  }
+  size_t pend_off[2] = {0, 0}, pend_len[2] = {0, 0};
+  size_t off = 0;
+  for (int k = 0; off < bytes || pend_len[0] || pend_len[1]; k ^= 1) {
+    if (pend_len[k]) {                                        // drain this buffer's previous chunk
+      HIPCHK(hipEventSynchronize(S.ev[k]));
+      std::memcpy((char*)dst + pend_off[k], S.pin[k], pend_len[k]);
+      pend_len[k] = 0;
+    }
+    if (off < bytes) {
+      const size_t n = std::min(kChunk, bytes - off);
+      HIPCHK(hipMemcpyAsync(S.pin[k], (const char*)src + off, n, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(S.ev[k], st));
+      pend_off[k] = off;
+      pend_len[k] = n;
+      off += n;
+    }
+  }
+  return PL_OK;
+}
+
 // two HIP events bracketing a pass, destroyed on every exit
 struct EventPair {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -162,11 +211,9 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   HIPCHK(hipEventRecord(ev.e1, st));
   char* out = (char*)malloc((size_t)total + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
-  hipError_t ce = hipMemcpyAsync(out, dtext, (size_t)total, hipMemcpyDeviceToHost, st);
-  if (ce == hipSuccess) ce = hipStreamSynchronize(st);
-  if (ce != hipSuccess) {
+  if (pl_status cs = copy_to_host(out, dtext, (size_t)total, st)) {
     free(out);
-    return fail(PL_E_DEVICE, "copy of the VCF text: %s", hipGetErrorString(ce));
+    return cs;
   }
   out[total] = 0;
   float ms = 0.f;
