@@ -190,7 +190,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path would shard "
-              "by region with no communication — see DESIGN.md", file=sys.stderr)
+              "by region with no communication (regions are independent)", file=sys.stderr)
         sys.exit(2)
 
     from pilon_amd.abi import default_config
