@@ -17,6 +17,16 @@
  * threshold table, the oldIndel branch, strict majority and N rejection, the
  * AMB disposition and its change line with and without --iupac, planted
  * variants fixed.
+ * Further pins: the VCF records (tests/test_oracle_vcf.py: hand-derived
+ * lines, "%.2f" of the Float AF against decimal rounding half up), the
+ * tracks (tests/test_oracle_tracks.py: smooth's closed forms and edges,
+ * computeGc against a brute force, delta/dip/copy number from fragment
+ * coverage rebuilt from the final pileup), Pilon.bed
+ * (tests/test_oracle_bed.py: summaryRegions against run grouping, every
+ * region's lines against a second derivation), the assembler's K-mer graph
+ * (tests/test_oracle_kmer.py: against a dict-of-pileups restatement).
+ * Not restated: Assembler.addRead's prune at its 100,000th read (the K-mer
+ * graph refuses such inputs, PL_E_INVAL, as the device pass does).
  * Unreachable here: the overlap branch of fixFixList (GenomeRegion.scala:
  * 557-593).  Pass 1 adds at most one change per locus and marks the loci a
  * deletion covers as deleted, so no two fixes it makes overlap.
