@@ -145,10 +145,51 @@ def cpu_baseline(cfg, data, regions, seconds):
                     drafts[ci] = data.draft(ci)
                 live.add(pool.submit(one, nxt)); nxt += 1
         wall = time.perf_counter() - t0
-    return {"value": done / wall, "unit": "bases/s", "cores": nth, "kind": "port",
-            "sample": f"oracle (C restatement of Pilon's path) on the first {nreg} of {len(regions)} regions, "
-                      f"one region per task on {nth} threads: {done} aligned bases in {wall:.1f}s wall "
-                      f"({done / busy / 1e6:.1f} M bases/s per thread)"}
+    out = {"value": done / wall, "unit": "bases/s", "cores": nth, "kind": "port",
+           "sample": f"oracle (C restatement of Pilon's path) on the first {nreg} of {len(regions)} regions, "
+                     f"one region per task on {nth} threads: {done} aligned bases in {wall:.1f}s wall "
+                     f"({done / busy / 1e6:.1f} M bases/s per thread)"}
+    out["rows"] = cpu_rows(cfg, data, regions, nth, drafts)
+    return out
+
+
+def cpu_rows(cfg, data, regions, nth, drafts):
+    """The oracle on the widened rows beside the device figures: VCF records
+    and the 13 tracks with Pilon.bed of the first 200 regions (one region per
+    task on nth threads), and the assembler's K-mer graph of the reads the
+    K-mer leg uses (one thread)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    sample = regions[:200]
+    for ci, _, _, _ in sample:
+        if ci not in drafts:
+            drafts[ci] = data.draft(ci)
+
+    def one(k, flags):
+        ci, s, e, _ = sample[k]
+        r = O.run_region(cfg, drafts[ci], s, e, [_ContigReads(data, ci)], name=data.names[ci], vcf=flags)
+        assert r["status"] == 0
+        return r["vcf"].count("\n") if flags & O.VCF_ON else e - s + 1
+
+    rows = {}
+    for key, flags, unit in (("vcf", O.VCF_ON, "records/s"), ("tracks_bed", O.TRACKS | O.BED, "loci/s")):
+        with ThreadPoolExecutor(nth) as pool:
+            t0 = time.perf_counter()
+            n = sum(pool.map(lambda k: one(k, flags), range(len(sample))))
+            wall = time.perf_counter() - t0
+        rows[key] = {"value": n / wall, "unit": unit, "cores": nth,
+                     "sample": f"{len(sample)} regions, {n} {unit.split('/')[0]} in {wall:.2f}s wall (pileup included)"}
+    ci = max(range(data.n), key=lambda i: len(data.draft(i)))
+    b = data.batch(ci)
+    b = b.slice(0, min(99_999, b.n))
+    occ = int(2 * np.clip(np.diff(b.seq_off) - 47, 0, None).sum())
+    t0 = time.perf_counter()
+    st, g = O.kmer_graph(cfg, b, K=47, min_depth=5)
+    wall = time.perf_counter() - t0
+    assert st == 0
+    rows["kmer_graph"] = {"value": occ / wall, "unit": "occurrences/s", "cores": 1,
+                          "sample": f"{b.n} reads, {occ} K-mer occurrences, {len(g)} graph K-mers in {wall:.2f}s"}
+    return rows
 
 
 def main():
@@ -278,6 +319,8 @@ def main():
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(cfg, data, regions, args.cpu_seconds)
         log(f"bench: CPU oracle {cpu['value'] / 1e6:.1f} M bases/s on {cpu['cores']} threads ({cpu['sample']})")
+        for k, v in cpu["rows"].items():
+            log(f"bench: CPU oracle {k}: {v['value'] / 1e6:.2f} M {v['unit']} on {v['cores']} threads ({v['sample']})")
 
     vcf = None
     if args.vcf:
