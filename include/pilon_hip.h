@@ -183,7 +183,8 @@ pl_status pl_region_finish(pl_region* reg, int32_t* pileup, pl_call* calls,
                            pl_region_stats* stats);
 /* identifyAndFixIssues (snps/indels) + writeChanges + fixed bases.  The
  * library allocates *changes_text (the .changes lines) and *bases; free
- * them with pl_free.  new_name/offset are writeChanges' arguments. */
+ * them with pl_free.  new_name/offset are writeChanges' arguments (new_name
+ * NULL: the contig's polished name, as pl_pilon_name). */
 pl_status pl_region_fix(pl_region* reg, const char* new_name, int64_t offset,
                         char** changes_text, int64_t* changes_len,
                         uint8_t** bases, int64_t* bases_len);
@@ -309,6 +310,55 @@ pl_status pl_region_bed(pl_region* reg, uint32_t flags, char** text, int64_t* le
 pl_status pl_kmer_graph(pl_ctx* ctx, const pl_read_soa* reads, int32_t K, int32_t min_depth, char** kmers,
                         uint8_t** next, uint8_t** alt, int64_t* n, double* device_ms);
 
+/* ------------------------------------------------- fixes, FASTA, regions */
+/* GenomeFile.processRegions' output loop (GenomeFile.scala:135-162) over the
+ * context's last batch: per region, writeChanges (GenomeRegion.scala:645-657)
+ * after identifyAndFixIssues (snps/indels/amb, :307-380,413) and fixIssues
+ * (:595-621); per element, writeFastaElement (GenomeFile.scala:79-82) of the
+ * fixed bases of its regions concatenated.  An element is one entry of
+ * GenomeFile.regions: a maximal run of consecutive regions of one contig
+ * with the same element[i] (element NULL: the contig, as for the chunks of
+ * contigRegions; one element per --targets entry otherwise).  Its newName is
+ * the contig name + "_pilon", "|pilon" or "pilon" (a name holding '|', one
+ * ending in '|'), and the .changes offsets of a region accumulate the length
+ * changes of the element's earlier regions (:149-153).  *changes_text and
+ * *fasta_text (either may be NULL) are malloc'ed (pl_free). */
+pl_status pl_batch_fix(pl_ctx* ctx, const int32_t* element, int32_t n_element, char** changes_text,
+                       int64_t* changes_len, char** fasta_text, int64_t* fasta_len);
+
+/* GenomeFile.writeFastaElement (GenomeFile.scala:79-82): ">header", then the
+ * sequence in lines of 80 (the last shorter), each ended by '\n'.  *text is
+ * malloc'ed (pl_free). */
+pl_status pl_fasta_element(const char* header, const uint8_t* seq, int64_t len, char** text, int64_t* text_len);
+
+/* The polished contig name of GenomeFile.processRegions (GenomeFile.scala:
+ * 137-140).  *out is malloc'ed (pl_free). */
+pl_status pl_pilon_name(const char* name, char** out);
+
+/* GenomeFile.contigRegions (GenomeFile.scala:67-74) with Pilon.chunkSize
+ * (10,000,000 by default, Pilon.scala:52): nChunks = ceil(len / chunk),
+ * regions [1 + k*size, min(len, (k+1)*size)] of size ceil(len / nChunks), in
+ * Int arithmetic.  *n = the region count; starts/stops (cap entries; both
+ * NULL to count only) receive them. */
+pl_status pl_contig_regions(int64_t length, int32_t chunk_size, int32_t* starts, int32_t* stops, int32_t cap,
+                            int32_t* n);
+
+/* One --targets region: contig = index into the names given. */
+typedef struct pl_target {
+  int32_t contig;
+  int32_t start, stop;   /* 1-based inclusive */
+} pl_target;
+
+/* GenomeFile.parseTargets (GenomeFile.scala:215-250): the lines of the file
+ * named by targets when it reads and every line parses to regions, else the
+ * string itself; each comma-separated "contig:start-stop" (or a bare contig
+ * name: the whole contig) is one unchunked region, in order, duplicates
+ * kept.  Where Pilon throws (an unknown contig, a coordinate beyond Int,
+ * stop past the contig) or builds a region it cannot process (start < 1,
+ * stop < start) this returns PL_E_INVAL.  *out is malloc'ed (pl_free). */
+pl_status pl_parse_targets(const char* targets, const char* const* names, const int64_t* lengths, int32_t n_contigs,
+                           pl_target** out, int32_t* n_out);
+
 /* Sizes and HIP-event times of the last batch (pl_batch_run or
  * pl_region_finish). */
 typedef struct pl_batch_info {
@@ -346,8 +396,11 @@ typedef struct pl_serve_opts {
   int64_t mem_budget;      /* device bytes for in-flight requests; 0 = 1/4 of free     */
   int32_t max_queue;       /* waiting requests before PL_E_BUSY; 0 = 65536            */
   int32_t window, block;   /* launch shape; 0 = 512 / 256                             */
-  int32_t copy_out;        /* 1 = keep each result's pileup and calls for pl_serve_fetch */
+  int32_t copy_out;        /* PL_SERVE_KEEP_* bits: what each result keeps for fetching */
 } pl_serve_opts;
+
+#define PL_SERVE_KEEP_PILEUP 0x1u  /* pileup and calls, for pl_serve_fetch               */
+#define PL_SERVE_KEEP_FIX    0x2u  /* change list, for pl_serve_fetch_fix               */
 
 typedef struct pl_serve_result {
   int64_t ticket;
@@ -371,6 +424,13 @@ pl_status pl_serve_poll(pl_server* srv, pl_serve_result* out, int32_t max, int32
 /* Pileup (PL_NFIELDS*G, field-major) and calls of a polled request of a
  * copy_out server; the kept copy is freed afterwards. */
 pl_status pl_serve_fetch(pl_server* srv, int64_t ticket, int32_t* pileup, pl_call* calls);
+/* The .changes lines and fixed bases of a polled request of a server kept
+ * with PL_SERVE_KEEP_FIX: identifyAndFixIssues + fixIssues + writeChanges
+ * (GenomeRegion.scala:307-380,595-657) of its region, new_name NULL = the
+ * contig's polished name; the kept change list is freed afterwards.  Both
+ * outputs are malloc'ed (pl_free). */
+pl_status pl_serve_fetch_fix(pl_server* srv, int64_t ticket, const char* new_name, int64_t offset, char** changes_text,
+                             int64_t* changes_len, uint8_t** bases, int64_t* bases_len);
 /* Waits until every submitted request has finished. */
 pl_status pl_serve_drain(pl_server* srv);
 /* Drains the queue, stops the lanes and frees their device memory. */

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, never by the product (pilon_amd/).
 """
 import ctypes as C
+import re
 import os
 import subprocess
 
@@ -222,3 +223,117 @@ def kmer_graph(cfg, batch, K=47, min_depth=5):
         if p.value:
             L.orc_free(p)
     return st, out
+
+
+# ---------------------------------------------------------------- GenomeFile
+def _jint(x):
+    """Java Int wrap."""
+    return (x + 2 ** 31) % 2 ** 32 - 2 ** 31
+
+
+def contig_regions(length, chunk_size=10000000):
+    """GenomeFile.contigRegions (GenomeFile.scala:67-74), Int arithmetic:
+    [(start, stop)] of Range(1, len + 1, chunkSize)."""
+    n_chunks = int(_jint(length + chunk_size - 1) / chunk_size)      # Java division truncates
+    chunk = int(_jint(length + n_chunks - 1) / n_chunks)
+    return [(b, min(length, b + chunk - 1)) for b in range(1, length + 1, chunk)]
+
+
+def pilon_name(name):
+    """processRegions' newName (GenomeFile.scala:137-140)."""
+    sep = "_" if "|" not in name else ("" if name.endswith("|") else "|")
+    return name + sep + "pilon"
+
+
+def fasta_element(header, seq):
+    """writeFastaElement (GenomeFile.scala:79-82): println(">" + header), then
+    sequence.sliding(80, 80) foreach println (the last group shorter, no
+    group for an empty sequence)."""
+    out = [">" + header + "\n"]
+    for i in range(0, len(seq), 80):
+        out.append(seq[i:i + 80] + "\n")
+    return "".join(out)
+
+
+_TARGET = re.compile(rb"\s*([^\n\r]+):([0-9]+)-([0-9]+)\s*")
+
+
+def _java_split_comma(s):
+    parts = s.split(",")
+    if len(parts) > 1:
+        while parts and parts[-1] == "":
+            parts.pop()
+    return parts
+
+
+def parse_target_string(s, contigs):
+    """parseTargetString (GenomeFile.scala:215-236) over [(name, length)]:
+    [(contig index, start, stop)]; raises ValueError where Pilon throws
+    (or builds a region it cannot process)."""
+    index = {}
+    for i, (name, _) in enumerate(contigs):
+        index.setdefault(name, i)
+    out = []
+    for t in _java_split_comma(s):
+        m = _TARGET.fullmatch(t.encode())
+        if m:
+            name, a, b = m.group(1).decode(), int(m.group(2)), int(m.group(3))
+            if a >= 2 ** 31 or b >= 2 ** 31:
+                raise ValueError("NumberFormatException: " + t)
+        else:
+            name = t
+        if name not in index:
+            raise ValueError("NoSuchElementException: " + name)
+        length = contigs[index[name]][1]
+        if not m:
+            a, b = 1, length
+        if b > length or a < 1 or b < a:
+            raise ValueError("bad region " + t)
+        out.append((index[name], a, b))
+    return out
+
+
+def parse_targets(s, contigs):
+    """parseTargets (GenomeFile.scala:238-250): the file's lines if it reads
+    and parses to something, else the string."""
+    try:
+        with open(s, "rb") as f:
+            lines = f.read().decode().splitlines()
+        got = [t for line in lines for t in parse_target_string(line, contigs)]
+        if got:
+            return got
+    except (OSError, ValueError):
+        pass
+    return parse_target_string(s, contigs)
+
+
+def process_regions_output(cfg, contigs, regions, readsets_of, element=None):
+    """processRegions' output loop (GenomeFile.scala:135-162) over regions
+    [(contig index, start, stop)] with contigs [(name, bases)]: each region
+    through run_region with its element's newName and the offset of the
+    element's earlier regions (:149-153); returns (.changes text, FASTA
+    text).  readsets_of(i) gives region i's read batches; element (one id
+    per region, None = the contig) groups consecutive regions."""
+    changes, fasta = [], []
+    i = 0
+    while i < len(regions):
+        j = i
+        key = element[i] if element is not None else regions[i][0]
+        while j < len(regions) and regions[j][0] == regions[i][0] and \
+                (element[j] if element is not None else regions[j][0]) == key:
+            j += 1
+        name, bases = contigs[regions[i][0]]
+        new_name = pilon_name(name)
+        offset = 0
+        seq = []
+        for k in range(i, j):
+            _, s, e = regions[k]
+            r = run_region(cfg, bases, s, e, readsets_of(k), name=name, new_name=new_name, offset=offset)
+            if r["status"] != 0:
+                raise ValueError(f"region {k}: status {r['status']}")
+            changes.append(r["changes"])
+            seq.append(r["bases"].decode())
+            offset += len(r["bases"]) - (e - s + 1)
+        fasta.append(fasta_element(new_name, "".join(seq)))
+        i = j
+    return "".join(changes), "".join(fasta)

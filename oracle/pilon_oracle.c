@@ -989,7 +989,7 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
   for (int32_t i = 0; i < size; i++) fragCov[i] = (int32_t)pu_depth(&R.pu[i]);
   int64_t meanCoverage = roundDivL(R.baseCount, (int64_t)size);          /* PileUpRegion.coverage */
   int32_t minDepth;
-  if (cfg->min_depth >= 1) minDepth = (int32_t)cfg->min_depth;
+  if (cfg->min_depth >= 1) minDepth = cfg->min_depth >= 2147483647.0 ? INT32_MAX : (int32_t)cfg->min_depth;  /* toInt saturates */
   else {
     int32_t md = (int32_t)orc_java_round(cfg->min_depth * (double)meanCoverage);
     minDepth = md > cfg->min_min_depth ? md : cfg->min_min_depth;

@@ -116,3 +116,12 @@ class PlBatchInfo(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PlTarget(C.Structure):
+    """pl_target: one --targets region (contig = index into the names given)."""
+    _fields_ = [("contig", C.c_int32), ("start", C.c_int32), ("stop", C.c_int32)]
+
+
+SERVE_KEEP_PILEUP, SERVE_KEEP_FIX = 0x1, 0x2
+CHUNK_SIZE = 10000000   # Pilon.chunkSize (Pilon.scala:52)

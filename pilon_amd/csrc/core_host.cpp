@@ -290,6 +290,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
       for (int b2 = 0; b2 < 4; b2++) dep += (uint32_t)pile[(int64_t)(PL_F_COUNT_A + b2) * G + g];
       dep += pile[(int64_t)PL_F_DELETIONS * G + g];
       calls[g].flags = plh::final_flags(calls[g].flags, dep, st[j].min_depth, st[j].read_count);
+      if (st[j].min_depth > PL_CONF_HIST_MAX && call_disp(calls[g]) == PL_D_CONFIRMED) st[j].n_confirmed++;
     }
     for (int64_t k = a; k < b; k++) {
       const Cand& c = cands[k];

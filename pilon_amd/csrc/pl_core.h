@@ -782,9 +782,20 @@ PL_HD pl_call make_call(const BaseFacts& f, int indel, bool homo_indel, uint8_t 
 
 PL_HD int call_disp(const pl_call& c) { return (c.flags & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT; }
 
+// The confirmed count of Pass 1 (GenomeRegion.scala:255-256) is read from a
+// per-region histogram of confirmed candidates by depth, whose last bucket
+// holds depth 255 and above; a region whose minDepth is above that bucket
+// counts its confirmed positions exactly instead, after the deletion pass.
+#define PL_CONF_HIST_MAX 255
+// Is position g confirmed after Pass 1: a CONFIRMED candidate, not deleted,
+// depth >= minDepth, in a region with reads (GenomeRegion.scala:229-256)?
+PL_HD bool confirmed_final(const pl_call& c, int64_t depth, int32_t md, int64_t read_count) {
+  return read_count != 0 && !(c.flags & PL_CF_DELETED) && depth >= md && call_disp(c) == PL_D_CONFIRMED;
+}
+
 // Java Math.round on the double minDepth * meanCoverage (GenomeRegion.scala:221-224)
 PL_HD int32_t min_depth_of(const KConf& cf, int64_t base_count, int32_t size) {
-  if (cf.min_depth >= 1) return (int32_t)cf.min_depth;
+  if (cf.min_depth >= 1) return cf.min_depth >= 2147483647.0 ? INT32_MAX : (int32_t)cf.min_depth;   // Double.toInt saturates
   const int64_t cov = round_div_l(base_count, (int64_t)size);
   const double x = cf.min_depth * (double)cov;
   const double f = floor(x);
@@ -968,9 +979,10 @@ PL_HD void finalize_job(const KConf& cf, const JobDev& J, JobStat& st, int32_t* 
                         uint32_t* mutated, uint32_t* mut_n, uint32_t mut_cap, int32_t* overflow) {
   const int32_t md = min_depth_of(cf, st.base_count, J.size);
   st.min_depth = md;
+  // confirmed positions by depth, from the histogram when minDepth falls
+  // inside it; otherwise counted exactly after the pass (confirmed_exact)
   int64_t nconf = 0;
-  if (md <= 255) for (int d = md; d < 256; d++) nconf += st.hist[d];
-  else nconf = -1;
+  if (md <= PL_CONF_HIST_MAX) for (int d = md; d < 256; d++) nconf += st.hist[d];
   int64_t delEnd = -1;
   for (int64_t k = c_lo; k < c_hi; k++) {
     Cand& c = cand[k];
@@ -983,7 +995,7 @@ PL_HD void finalize_job(const KConf& cf, const JobDev& J, JobStat& st, int32_t* 
         int64_t dep = 0;
         for (int b = 0; b < 4; b++) dep += (int64_t)(uint32_t)pile[(int64_t)(PL_F_COUNT_A + b) * G + gj];
         dep += pile[(int64_t)PL_F_DELETIONS * G + gj];
-        if (call_disp(calls[gj]) == PL_D_CONFIRMED && dep >= md && nconf > 0) nconf--;
+        if (md <= PL_CONF_HIST_MAX && call_disp(calls[gj]) == PL_D_CONFIRMED && dep >= md && nconf > 0) nconf--;
         pile[(int64_t)PL_F_DELETIONS * G + gj] = j_add(pile[(int64_t)PL_F_DELETIONS * G + gj], dcount);
         calls[gj].flags |= PL_CF_DELETED;
         uint32_t slot = pl_bump(mut_n);

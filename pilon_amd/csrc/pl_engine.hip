@@ -145,6 +145,11 @@ struct Lane {
   hipGraphExec_t gexec = nullptr;
   std::vector<uint8_t> gkey;
   EngineArgs gargs;
+  // two pinned host buffers for device-to-host hand-offs in chunks (VCF,
+  // wig, track values), allocated on first use and kept until the lane is
+  // destroyed, so a hand-off allocates no pinned memory
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 
   size_t device_bytes() const {
     const DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st,
@@ -383,6 +388,12 @@ void lane_destroy(Lane* L) {
   if (L->stream) (void)hipStreamSynchronize(L->stream);
   drop_graph(L);
   L->free_buffers();
+  for (int k = 0; k < 2; k++) {
+    if (L->stage_ev[k]) (void)hipEventDestroy(L->stage_ev[k]);
+    if (L->stage[k]) (void)hipHostFree(L->stage[k]);
+    L->stage_ev[k] = nullptr;
+    L->stage[k] = nullptr;
+  }
   for (int i = 0; i < 4; i++) if (L->ev[i]) (void)hipEventDestroy(L->ev[i]);
   if (L->stream) (void)hipStreamDestroy(L->stream);
   L->stream = nullptr;
@@ -439,6 +450,7 @@ pl_status enqueue_batch(Lane* L, EngineArgs& A, int persistent, int block, int g
   if (s) return s;
   k_cand_gather<<<dim3(gc), dim3(256), 0, st>>>(A.cand, A.cand_n, A.cand_cap, cv, L->cands.as<Cand>());
   k_finalize<<<dim3(n_jobs), dim3(64), 0, st>>>(A, L->cands.as<Cand>(), A.cand_n, A.cand_cap, L->cands.as<Cand>());
+  k_confirm_exact<<<dim3((unsigned)std::min<int32_t>(n_wins, 1024)), dim3(256), 0, st>>>(A);
   const uint32_t gm = (uint32_t)std::min<int64_t>(2048, std::max<int64_t>(1, ((int64_t)A.mutated_cap + 3) / 4));
   k_indel_pass<<<dim3(gm), dim3(256), 0, st>>>(A, A.mutated, A.mutated_n, A.mutated_cap, 1);
   HIPCHK(hipGetLastError());
@@ -899,6 +911,8 @@ pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out) {
   if (cfg->longread) return fail(PL_E_INVAL, "long-read BAMs (addRead longRead > 0) are not supported");
   if (cfg->flank < 0) return fail(PL_E_INVAL, "flank must be >= 0");
   if (cfg->default_qual < -128 || cfg->default_qual > 127) return fail(PL_E_INVAL, "defaultQual is a Byte");
+  if (cfg->min_min_depth < 0) return fail(PL_E_INVAL, "minMinDepth must be >= 0");
+  if (!std::isfinite(cfg->min_depth)) return fail(PL_E_INVAL, "minDepth must be a finite number");
   int n = 0;
   HIPCHK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(PL_E_DEVICE, "no HIP device %d (have %d)", device, n);
@@ -1024,7 +1038,7 @@ pl_status pl_region_fix(pl_region* r, const char* new_name, int64_t offset, char
   if (!r || !r->finished) return fail(PL_E_INVAL, "region not finished");
   const Contig& ct = r->ctx->contigs[r->contig];
   std::string rb((const char*)ct.host.data() + r->start - 1, (size_t)(r->stop - r->start + 1));
-  std::string nn = new_name ? new_name : ct.name + "_pilon";
+  std::string nn = new_name ? new_name : plh::pilon_name(ct.name);
   plh::FixResult fr = plh::fix_region(r->ctx->pc, ct.name, r->start, rb, r->changes, nn, offset);
   if (changes_text) {
     *changes_text = (char*)malloc(fr.changes.size() + 1);
@@ -1119,3 +1133,4 @@ pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
 #include "pl_track.hip"
 #include "pl_kmer_kernels.hip"
 #include "pl_kmer.hip"
+#include "pl_fix.hip"

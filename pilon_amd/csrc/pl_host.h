@@ -62,18 +62,31 @@ inline std::vector<Fix> fix_fix_list(std::vector<Fix> in) {
   return out;
 }
 
-// GenomeRegion.fixIssues (GenomeRegion.scala:595-621): apply right to left.
+// GenomeRegion.fixIssues (GenomeRegion.scala:595-621).  Pilon applies the
+// kept fixes right to left; they are sorted and no two overlap (each starts
+// after the last base its predecessor replaces), so one left-to-right splice
+// gives the same bases in time linear in the region.
 inline void apply_fixes(std::string& bases, int32_t region_start, const std::vector<Fix>& list) {
-  std::vector<Fix> kept = fix_fix_list(list);
-  for (size_t k = kept.size(); k-- > 0;) {
-    const Fix& f = kept[k];
-    const size_t at = (size_t)(f.loc - region_start);
-    if (f.was.size() == f.patch.size()) {
-      for (size_t i = 0; i < f.was.size(); i++) bases[at + i] = f.patch[i];
-    } else {
-      bases = bases.substr(0, at) + f.patch + bases.substr(at + f.was.size());
-    }
+  const std::vector<Fix> kept = fix_fix_list(list);
+  if (kept.empty()) return;
+  std::string out;
+  out.reserve(bases.size() + 64);
+  size_t at = 0;
+  for (const Fix& f : kept) {
+    const size_t s = (size_t)(f.loc - region_start);
+    out.append(bases, at, s - at);
+    out += f.patch;
+    at = s + f.was.size();
   }
+  out.append(bases, at, std::string::npos);
+  bases.swap(out);
+}
+
+// GenomeFile.processRegions' newName (GenomeFile.scala:137-140): name + sep +
+// "pilon", sep "_" unless the name holds a '|' ("" when it ends with one).
+inline std::string pilon_name(const std::string& name) {
+  const char* sep = name.find('|') == std::string::npos ? "_" : (!name.empty() && name.back() == '|') ? "" : "|";
+  return name + sep + "pilon";
 }
 
 inline std::string region_string(const std::string& name, int64_t s, int64_t e) {  // Region.regionString

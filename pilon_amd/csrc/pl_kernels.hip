@@ -1093,11 +1093,41 @@ __global__ void __launch_bounds__(64) k_finalize(EngineArgs A, const Cand* sorte
   dropped = wave_sum(dropped);
   if (lane == 0) {
     st.min_depth = md;
-    int64_t nconf = 0;
-    if (md <= 255) for (int d = md; d < 256; d++) nconf += st.hist[d];
-    else nconf = -1;
+    int64_t nconf = 0;   // minDepth above the histogram: counted by k_confirm_exact
+    if (md <= PL_CONF_HIST_MAX) for (int d = md; d < 256; d++) nconf += st.hist[d];
     if (nconf > 0) nconf = nconf - dropped > 0 ? nconf - dropped : 0;   // the serial scan stops decrementing at 0
     st.n_confirmed = (int32_t)nconf;
+  }
+}
+
+// Exact confirmed count of the regions whose minDepth lies above the depth
+// histogram (a --mindepth of 256 or more, or a region averaging more than
+// about 2,555x): every position of their windows is tested after the
+// deletion pass.  Other regions' windows are skipped after one load, so the
+// launch costs a few microseconds when no region needs it.
+__global__ void __launch_bounds__(256) k_confirm_exact(EngineArgs A) {
+  __shared__ uint32_t s_n;
+  for (int32_t w = blockIdx.x; w < A.n_wins; w += gridDim.x) {
+    const WinDev win = A.wins[w];
+    JobStat& S = A.st[win.job];
+    const int32_t md = S.min_depth;
+    if (md <= PL_CONF_HIST_MAX || S.read_count == 0) continue;
+    const JobDev& J = A.jobs[win.job];
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    uint32_t n = 0;
+    for (int p = win.p0 + threadIdx.x; p < win.p1; p += blockDim.x) {
+      const int64_t g = J.out + p;
+      int64_t dep = 0;
+      for (int b = 0; b < 4; b++) dep += (uint32_t)A.pile[(int64_t)(PL_F_COUNT_A + b) * A.G + g];
+      dep += A.pile[(int64_t)PL_F_DELETIONS * A.G + g];
+      n += confirmed_final(A.calls[g], dep, md, S.read_count) ? 1u : 0u;
+    }
+    n = wave_sum(n);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&s_n, n);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) atomicAdd(&S.n_confirmed, (int32_t)s_n);
+    __syncthreads();
   }
 }
 

@@ -35,8 +35,11 @@ struct ServeReq {
 
 struct ServeDone {
   pl_serve_result res;
+  JobReq job;
+  bool keep_pile = false, keep_fix = false;   // parts not yet fetched
   std::vector<int32_t> pile;
   std::vector<pl_call> calls;
+  std::vector<plh::Change> changes;
 };
 
 // Device bytes one region of G positions can need inside a lane: pileup and
@@ -109,16 +112,20 @@ void serve_worker(pl_server* S, Lane* L) {
       d.res.est_bytes = batch[j].est;
       d.res.batch_size = (int32_t)batch.size();
       d.res.batch_device_ms = bs ? 0.0 : L->last_ms;
+      d.job = batch[j].job;
       pl_status s = bs;
       if (!s) s = job_status(L, j);
-      if (!s) s = job_changes(c, L, j, &d.res.stats, nullptr);
-      if (!s && S->o.copy_out) {
+      if (!s) s = job_changes(c, L, j, &d.res.stats, (S->o.copy_out & PL_SERVE_KEEP_FIX) ? &d.changes : nullptr);
+      d.keep_fix = !s && (S->o.copy_out & PL_SERVE_KEEP_FIX);
+      d.keep_pile = !s && (S->o.copy_out & PL_SERVE_KEEP_PILEUP);
+      if (!s && (S->o.copy_out & PL_SERVE_KEEP_PILEUP)) {
         const int64_t sz = L->hjobs[j].size;
         d.pile.resize((size_t)PL_NFIELDS * sz);
         d.calls.resize((size_t)sz);
         s = job_export(L, j, d.pile.data(), d.calls.data());
       }
       d.res.status = s;
+      if (s) d.keep_fix = d.keep_pile = false;
     }
     // keep a lane's buffers near what its batches need
     if ((int64_t)L->device_bytes() > 2 * est + ((int64_t)64 << 20)) {
@@ -226,7 +233,7 @@ pl_status pl_serve_poll(pl_server* S, pl_serve_result* out, int32_t max, int32_t
     auto it = S->done.find(t);
     if (it == S->done.end()) continue;
     out[n++] = it->second.res;
-    if (!S->o.copy_out || it->second.res.status != PL_OK) S->done.erase(it);
+    if (!it->second.keep_pile && !it->second.keep_fix) S->done.erase(it);
   }
   *n_out = n;
   return PL_OK;
@@ -235,13 +242,16 @@ pl_status pl_serve_poll(pl_server* S, pl_serve_result* out, int32_t max, int32_t
 pl_status pl_serve_fetch(pl_server* S, int64_t ticket, int32_t* pile, pl_call* calls) {
   if (!S) return fail(PL_E_INVAL, "null argument");
   std::lock_guard<std::mutex> lk(S->mu);
-  if (!S->o.copy_out) return fail(PL_E_INVAL, "server was opened without copy_out");
+  if (!(S->o.copy_out & PL_SERVE_KEEP_PILEUP)) return fail(PL_E_INVAL, "server was opened without PL_SERVE_KEEP_PILEUP");
   auto it = S->done.find(ticket);
-  if (it == S->done.end()) return fail(PL_E_INVAL, "no kept result for ticket %lld (not done, not polled, or fetched)",
-                                       (long long)ticket);
+  if (it == S->done.end() || !it->second.keep_pile)
+    return fail(PL_E_INVAL, "no kept result for ticket %lld (not done, not polled, or fetched)", (long long)ticket);
   if (pile) std::memcpy(pile, it->second.pile.data(), 4 * it->second.pile.size());
   if (calls) std::memcpy(calls, it->second.calls.data(), sizeof(pl_call) * it->second.calls.size());
-  S->done.erase(it);
+  it->second.keep_pile = false;
+  std::vector<int32_t>().swap(it->second.pile);
+  std::vector<pl_call>().swap(it->second.calls);
+  if (!it->second.keep_fix) S->done.erase(it);
   return PL_OK;
 }
 

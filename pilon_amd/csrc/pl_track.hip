@@ -117,7 +117,7 @@ pl_status tracks_copy(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs, i
   double ms = 0;
   if (pl_status s = tracks_of_lane(c, L, contigs, B, T, &ms)) return s;
   if (values)
-    if (pl_status cs = copy_to_host(values, T.vals, 4 * (size_t)PL_NTRACKS * (size_t)G, L->stream)) return cs;
+    if (pl_status cs = copy_to_host(L, values, T.vals, 4 * (size_t)PL_NTRACKS * (size_t)G)) return cs;
   if (device_ms) *device_ms = ms;
   return PL_OK;
 }
@@ -183,7 +183,7 @@ pl_status wig_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   char* out = (char*)malloc(head.size() + (size_t)total + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
   std::memcpy(out, head.data(), head.size());
-  if (pl_status cs = copy_to_host(out + head.size(), dtext, (size_t)total, st)) {
+  if (pl_status cs = copy_to_host(L, out + head.size(), dtext, (size_t)total)) {
     free(out);
     return cs;
   }

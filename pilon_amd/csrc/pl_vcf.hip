@@ -20,53 +20,65 @@ struct VcfBufs {
 
 void add_line(std::string& s, const std::string& l) { s += l; s += '\n'; }
 
-// A device-to-host copy into pageable memory through two pinned staging
-// buffers: the DMA of one chunk overlaps the host copy of the one before.
-// Small copies go directly.  Returns with the data in dst.
-pl_status copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  constexpr size_t kChunk = (size_t)32 << 20;
-  if (bytes <= 2 * kChunk) {
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return PL_OK;
-  }
-  struct Stage {
-    void* pin[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    ~Stage() {
-      for (int k = 0; k < 2; k++) {
-        if (ev[k]) (void)hipEventDestroy(ev[k]);
-        if (pin[k]) (void)hipHostFree(pin[k]);
+constexpr size_t kStageChunk = (size_t)32 << 20;
+
+// The lane's two pinned staging buffers and their events, created on first use.
+pl_status stage_ready(Lane* L) {
+  for (int k = 0; k < 2; k++) {
+    if (!L->stage[k]) {
+      if (hipHostMalloc(&L->stage[k], kStageChunk, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        L->stage[k] = nullptr;
+        return fail(PL_E_OOM, "pinned host allocation of %zu bytes failed", kStageChunk);
       }
     }
-  } S;
-  for (int k = 0; k < 2; k++) {
-    if (hipHostMalloc(&S.pin[k], kChunk, hipHostMallocDefault) != hipSuccess) {
-      (void)hipGetLastError();
-      HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));   // no pinned memory: direct
-      HIPCHK(hipStreamSynchronize(st));
-      return PL_OK;
-    }
-    HIPCHK(hipEventCreate(&S.ev[k]));
+    if (!L->stage_ev[k]) HIPCHK(hipEventCreate(&L->stage_ev[k]));
   }
+  return PL_OK;
+}
+
+// Device bytes [src, src + bytes) handed to sink(offset, data, n) in order, a
+// chunk at a time, through the lane's two pinned staging buffers: the DMA of
+// one chunk overlaps the sink of the one before.  sink returns false to
+// stop (PL_E_INVAL).  Allocates nothing after the lane's first hand-off.
+template <class Sink>
+pl_status drain_to_host(Lane* L, const void* src, size_t bytes, Sink&& sink) {
+  if (pl_status s = stage_ready(L)) return s;
+  hipStream_t st = L->stream;
   size_t pend_off[2] = {0, 0}, pend_len[2] = {0, 0};
   size_t off = 0;
+  bool stop = false;
   for (int k = 0; off < bytes || pend_len[0] || pend_len[1]; k ^= 1) {
     if (pend_len[k]) {                                        // drain this buffer's previous chunk
-      HIPCHK(hipEventSynchronize(S.ev[k]));
-      std::memcpy((char*)dst + pend_off[k], S.pin[k], pend_len[k]);
+      HIPCHK(hipEventSynchronize(L->stage_ev[k]));
+      if (!stop && !sink(pend_off[k], (const char*)L->stage[k], pend_len[k])) stop = true;
       pend_len[k] = 0;
     }
-    if (off < bytes) {
-      const size_t n = std::min(kChunk, bytes - off);
-      HIPCHK(hipMemcpyAsync(S.pin[k], (const char*)src + off, n, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipEventRecord(S.ev[k], st));
+    if (off < bytes && !stop) {
+      const size_t n = std::min(kStageChunk, bytes - off);
+      HIPCHK(hipMemcpyAsync(L->stage[k], (const char*)src + off, n, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(L->stage_ev[k], st));
       pend_off[k] = off;
       pend_len[k] = n;
       off += n;
     }
   }
+  if (stop) return fail(PL_E_INVAL, "the caller's sink stopped the hand-off");
   return PL_OK;
+}
+
+// A device-to-host copy into pageable memory: small copies directly, large
+// ones through the lane's staging buffers.
+pl_status copy_to_host(Lane* L, void* dst, const void* src, size_t bytes) {
+  if (bytes <= 2 * kStageChunk) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return PL_OK;
+  }
+  return drain_to_host(L, src, bytes, [dst](size_t off, const char* p, size_t n) {
+    std::memcpy((char*)dst + off, p, n);
+    return true;
+  });
 }
 
 // two HIP events bracketing a pass, destroyed on every exit
@@ -211,7 +223,7 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   HIPCHK(hipEventRecord(ev.e1, st));
   char* out = (char*)malloc((size_t)total + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
-  if (pl_status cs = copy_to_host(out, dtext, (size_t)total, st)) {
+  if (pl_status cs = copy_to_host(L, out, dtext, (size_t)total)) {
     free(out);
     return cs;
   }

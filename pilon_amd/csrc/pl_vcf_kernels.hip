@@ -175,7 +175,10 @@ __device__ __forceinline__ void vcf_record(VOut<WRITE>& o, const VcfArgs& V, con
   // the called allele (cBase) and whether it prints as "."
   char cb1 = base;
   if (kind == 3 && rBase == base) cb1 = altB;
-  const bool dotAlt = (kind >= 2) && (cb1 == 'N' || cb1 == rBase);
+  // Vcf.scala:131: cBase is the anchor base for a deletion (ALT "." when it
+  // is N), the anchor plus the insertion for an insertion (never "."), the
+  // called base otherwise
+  const bool dotAlt = kind >= 2 ? (cb1 == 'N' || cb1 == rBase) : (kind == 0 && rBase == 'N');
   const int ac = callType[0] == '0' ? (callType[2] == '0' ? 0 : 1) : 2;
   const int32_t dsum = j_add(refDP, altDP);
   const float af = (dsum > 0 && !dotAlt) ? (float)altDP / (float)dsum : 0.0f;
@@ -188,7 +191,7 @@ __device__ __forceinline__ void vcf_record(VOut<WRITE>& o, const VcfArgs& V, con
   if (kind == 0) for (int32_t q = 0; q < slen; q++) o.c(indel_char(R, *e, q));
   o.c('\t');
   // ALT
-  if (kind == 0) o.c(rBase);
+  if (kind == 0) o.c(dotAlt ? '.' : rBase);
   else if (kind == 1) { o.c(rBase); for (int32_t q = 0; q < slen; q++) o.c(indel_char(R, *e, q)); }
   else if (dotAlt) o.c('.');
   else o.c(cb1);

@@ -17,8 +17,8 @@ import os
 import numpy as np
 
 from .abi import (PlConfig, PlReadSoa, PlCall, PlRegionStats, PlRegionDesc, PlBatchOpts, PlBatchInfo, PlServeOpts,
-                  PlServeResult, NFIELDS, F, STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK,
-                  PL_CF_DISP_SHIFT)
+                  PlServeResult, PlTarget, NFIELDS, F, STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK,
+                  PL_CF_DISP_SHIFT, SERVE_KEEP_PILEUP, SERVE_KEEP_FIX, CHUNK_SIZE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libpilonhip.so")
@@ -68,6 +68,57 @@ def _bed(fn, h, track_line):
         return C.string_at(t, n.value).decode(), ms.value
     finally:
         lib().pl_free(t)
+
+
+def _take_text(p, n, binary=False):
+    try:
+        b = C.string_at(p, n) if n else b""
+        return b if binary else b.decode()
+    finally:
+        if p:
+            lib().pl_free(p)
+
+
+def fasta_element(header, seq):
+    """GenomeFile.writeFastaElement (GenomeFile.scala:79-82): '>header' and
+    the sequence in lines of 80."""
+    seq = seq.encode() if isinstance(seq, str) else bytes(seq)
+    t, n = C.c_void_p(), C.c_int64()
+    _check(lib().pl_fasta_element(header.encode(), seq, len(seq), C.byref(t), C.byref(n)))
+    return _take_text(t.value, n.value)
+
+
+def pilon_name(name):
+    """The polished contig name (GenomeFile.scala:137-140)."""
+    t = C.c_void_p()
+    _check(lib().pl_pilon_name(name.encode(), C.byref(t)))
+    return _take_text(t.value, len(C.string_at(t.value)))
+
+
+def contig_regions(length, chunk_size=CHUNK_SIZE):
+    """GenomeFile.contigRegions (GenomeFile.scala:67-74): [(start, stop)]."""
+    n = C.c_int32()
+    _check(lib().pl_contig_regions(int(length), int(chunk_size), None, None, 0, C.byref(n)))
+    st = np.zeros(max(1, n.value), np.int32)
+    sp = np.zeros(max(1, n.value), np.int32)
+    _check(lib().pl_contig_regions(int(length), int(chunk_size), st.ctypes.data, sp.ctypes.data, n.value,
+                                   C.byref(n)))
+    return [(int(st[k]), int(sp[k])) for k in range(n.value)]
+
+
+def parse_targets(targets, contigs):
+    """GenomeFile.parseTargets (GenomeFile.scala:215-250) over [(name,
+    length)] contigs: [(contig index, start, stop)]."""
+    m = len(contigs)
+    names = (C.c_char_p * max(1, m))(*[c[0].encode() for c in contigs])
+    lens = (C.c_int64 * max(1, m))(*[int(c[1]) for c in contigs])
+    p, n = C.c_void_p(), C.c_int32()
+    _check(lib().pl_parse_targets(targets.encode(), names, lens, m, C.byref(p), C.byref(n)))
+    try:
+        arr = C.cast(p, C.POINTER(PlTarget))
+        return [(arr[k].contig, arr[k].start, arr[k].stop) for k in range(n.value)]
+    finally:
+        lib().pl_free(p)
 
 
 class PilonError(RuntimeError):
@@ -122,6 +173,15 @@ def lib():
             "pl_region_bed": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_void_p), P(C.c_int64), P(C.c_double)]),
             "pl_vcf_header": (C.c_int, [P(C.c_char_p), P(C.c_int64), C.c_int32, C.c_uint32, C.c_char_p, C.c_char_p,
                                         C.c_char_p, C.c_char_p, P(C.c_void_p), P(C.c_int64)]),
+            "pl_batch_fix": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, P(C.c_void_p), P(C.c_int64),
+                                       P(C.c_void_p), P(C.c_int64)]),
+            "pl_fasta_element": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, P(C.c_void_p), P(C.c_int64)]),
+            "pl_pilon_name": (C.c_int, [C.c_char_p, P(C.c_void_p)]),
+            "pl_contig_regions": (C.c_int, [C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, P(C.c_int32)]),
+            "pl_parse_targets": (C.c_int, [C.c_char_p, P(C.c_char_p), P(C.c_int64), C.c_int32, P(C.c_void_p),
+                                           P(C.c_int32)]),
+            "pl_serve_fetch_fix": (C.c_int, [C.c_void_p, C.c_int64, C.c_char_p, C.c_int64, P(C.c_void_p),
+                                             P(C.c_int64), P(C.c_void_p), P(C.c_int64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -211,6 +271,21 @@ class Context:
             lib().pl_free(t)
         return text, ms.value
 
+    def batch_fix(self, element=None, want_changes=True, want_fasta=True):
+        """GenomeFile.processRegions' output loop over the last batch: (the
+        .changes text, the FASTA text).  element: one id per region (None: the
+        contig); consecutive regions sharing an id form one FASTA element."""
+        el = None
+        if element is not None:
+            el = np.ascontiguousarray(element, dtype=np.int32)
+        t, tl, f, fl = C.c_void_p(), C.c_int64(), C.c_void_p(), C.c_int64()
+        _check(lib().pl_batch_fix(self.h, el.ctypes.data if el is not None else None, len(el) if el is not None else 0,
+                                  C.byref(t) if want_changes else None, C.byref(tl),
+                                  C.byref(f) if want_fasta else None, C.byref(fl)))
+        changes = _take_text(t.value, tl.value) if want_changes else None
+        fasta = _take_text(f.value, fl.value) if want_fasta else None
+        return changes, fasta
+
     def batch_tracks(self, copy=True):
         """Tracks.standardTracks values of the last batch: ([NTRACKS, G] int32
         in batch order, or None when copy is False; device_ms)."""
@@ -270,11 +345,13 @@ class Server:
     by estimated device memory and run with continuous batching on `lanes`
     concurrent batch pipelines over the context's resident data."""
 
-    def __init__(self, ctx, lanes=2, max_batch=256, mem_budget=0, max_queue=0, window=0, block=0, copy_out=False):
+    def __init__(self, ctx, lanes=2, max_batch=256, mem_budget=0, max_queue=0, window=0, block=0, copy_out=False,
+                 keep_fix=False):
         self.ctx = ctx
         self.copy_out = bool(copy_out)
+        keep = (SERVE_KEEP_PILEUP if copy_out else 0) | (SERVE_KEEP_FIX if keep_fix else 0)
         opts = PlServeOpts(lanes=lanes, max_batch=max_batch, mem_budget=int(mem_budget), max_queue=max_queue,
-                           window=window, block=block, copy_out=int(bool(copy_out)))
+                           window=window, block=block, copy_out=keep)
         h = C.c_void_p()
         _check(lib().pl_serve_open(ctx.h, C.byref(opts), C.byref(h)))
         self.h = h
@@ -295,8 +372,15 @@ class Server:
         _check(lib().pl_serve_poll(self.h, buf, max_n, int(wait_ms), C.byref(n)))
         return [buf[i].as_dict() for i in range(n.value)]
 
+    def fetch_fix(self, ticket, new_name=None, offset=0):
+        """(.changes text, fixed bases) of a polled request of a keep_fix server."""
+        t, tl, b, bl = C.c_void_p(), C.c_int64(), C.c_void_p(), C.c_int64()
+        _check(lib().pl_serve_fetch_fix(self.h, int(ticket), new_name.encode() if new_name else None, int(offset),
+                                        C.byref(t), C.byref(tl), C.byref(b), C.byref(bl)))
+        return _take_text(t.value, tl.value), _take_text(b.value, bl.value, binary=True)
+
     def fetch(self, ticket):
-        G = self.sizes.pop(ticket)
+        G = self.sizes[ticket]
         pile = np.zeros((NFIELDS, G), np.int32)
         calls = np.zeros(G, call_dtype())
         _check(lib().pl_serve_fetch(self.h, int(ticket), pile.ctypes.data, calls.ctypes.data))

@@ -125,6 +125,22 @@ def case_deletion_call():
                 deleted=[16], disp={15: 4, 16: 0})
 
 
+def case_deletion_n_anchor():
+    """Vcf.scala:84-91,131,138-141: case_deletion_call with an N in the draft
+    at 14, the base before the deletion.  Pass 1 still calls the DEL at 15
+    (its own reference base is A, GenomeRegion.scala:255); 14 itself gets no
+    disposition (r == 'N').  The VCF prints the deletion at 14 as REF "NAG"
+    with cBase "N", so ALT is "." and AF 0.00."""
+    ref = REF_C[:13] + "N" + REF_C[14:]
+    seq = REF_C[0:14] + REF_C[16:32]
+    reads = [read(1, "14M2D16M", seq, [30] * 30) for _ in range(6)]
+    exp = {14: {"count_C": 6}, 15: {"deletions": 6, "delQual": 366, "mqSum": 366, "qSum": 180, "count_A": 0},
+           16: {"deletions": 6, "delQual": 0, "mqSum": 0}}
+    return dict(ref=ref, start=1, stop=40, reads=reads, expect=exp, changes="ctg:15-16 ctg_pilon:15 AG .\n",
+                bases=ref[0:14] + ref[16:], stats={"base_count": 60, "dels": 1, "del_bases": 2},
+                deleted=[16], disp={14: 0, 15: 4, 16: 0})
+
+
 def case_physcov_pairs():
     """PileUpRegion.scala:62-100.  Region 101-200 of a 300 bp contig.  A proper
     pair starting at 50 with TLEN 120 covers [50,170): it enters through
@@ -207,9 +223,37 @@ def case_het_deletion_oldindel():
     return case_het_deletion(old_indel=True)
 
 
+def case_min_depth_300(min_depth=300.0):
+    """GenomeRegion.scala:221-224,255-256: --mindepth 300 (>= 1: taken as an
+    Int).  310 reads 30M at 1, q30 MAPQ 60: loci 11..20 hold 310 bases
+    bases each, homozygous on the reference at q = 30 -> confirmed when
+    310 >= minDepth: 10 confirmed at 300, none at 311."""
+    reads = [read(1, "30M", REF_A[0:30], [30] * 30) for _ in range(310)]
+    n = 10 if min_depth <= 310 else 0
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect={11: {"count_C": 310}},
+                stats={"min_depth": int(min_depth), "n_confirmed": n, "base_count": 3100},
+                disp={11: 1 if n else 0, 21: 0}, cfg={"min_depth": float(min_depth)})
+
+
+def case_min_depth_311():
+    return case_min_depth_300(311.0)
+
+
+def case_deep_region():
+    """GenomeRegion.scala:221-224,255-256 at the default minDepth 0.1: 6000
+    reads 36M at 1 (q30, MAPQ 60) pile 16 trusted bases each at loci 11..26;
+    baseCount 96000 over 36 loci -> coverage roundDiv = 2667, minDepth =
+    round(266.7) = 267 (above a 256-bucket depth histogram).  Loci 11..26
+    have depth 6000, homozygous reference, q 30 -> 16 confirmed."""
+    reads = [read(1, "36M", REF_A, [30] * 36) for _ in range(6000)]
+    return dict(ref=REF_A, start=1, stop=36, reads=reads, expect={11: {"count_C": 6000, "qual_C": 6000 * 30 * 61}},
+                stats={"min_depth": 267, "coverage": 2667, "n_confirmed": 16, "base_count": 96000},
+                disp={11: 1, 26: 1, 27: 0})
+
+
 ALL = [case_trusted_flank, case_soft_clip, case_invalid_pair, case_deletion_slide, case_insertion_rotation,
-       case_deletion_call, case_physcov_pairs, case_n_base, case_no_qual, case_ambiguous, case_ambiguous_iupac,
-       case_het_deletion, case_het_deletion_oldindel]
+       case_deletion_call, case_deletion_n_anchor, case_physcov_pairs, case_n_base, case_no_qual, case_ambiguous, case_ambiguous_iupac,
+       case_het_deletion, case_het_deletion_oldindel, case_min_depth_300, case_min_depth_311, case_deep_region]
 
 
 def batch_of(case):

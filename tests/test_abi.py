@@ -82,3 +82,19 @@ def test_struct_layouts_match_c(tmp_path):
             sname, field = key.split(".")
             cls = sizes[sname]
             assert getattr(cls, field).offset == int(val), key
+
+
+@pytest.mark.parametrize("field,value", [("min_min_depth", -1), ("min_depth", float("nan")),
+                                         ("min_depth", float("inf")), ("flank", -1), ("default_qual", 128)])
+def test_ctx_create_rejects_bad_config(field, value):
+    """pl_ctx_create validates the knobs before it touches a device: a
+    negative minMinDepth, a non-finite minDepth, a negative flank or a
+    defaultQual outside a Byte are PL_E_INVAL with a message, GPU or not."""
+    lib = C.CDLL(LIB)
+    lib.pl_ctx_create.restype = C.c_int
+    lib.pl_last_error.restype = C.c_char_p
+    c = abi.default_config(**{field: value})
+    out = C.c_void_p()
+    assert lib.pl_ctx_create(C.byref(c), 0, C.byref(out)) == 1
+    assert lib.pl_last_error()
+    assert not out.value

@@ -112,17 +112,27 @@ def test_c3_full_batch(engine):
     def check(k):
         cid, s, e, _ = regions[k]
         ci = cid - 2000
-        ref = O.run_region(engine.config, d.draft(ci), s, e, [_SoaOf(d, ci)], name=d.names[ci])
+        ref = O.run_region(engine.config, d.draft(ci), s, e, [_SoaOf(d, ci)], name=d.names[ci],
+                           new_name=O.pilon_name(d.names[ci]))
         assert ref["status"] == 0, k
         a, b = int(offs[k]), int(offs[k + 1])
         assert_same_region(pile[:, a:b], calls[a:b], ref, f"c3 region {k}")
         assert_same_stats(stats[k], ref["stats"], f"c3 region {k}")
-        return ref["stats"]["base_count"]
+        return ref["stats"]["base_count"], ref["changes"], O.fasta_element(O.pilon_name(d.names[ci]),
+                                                                           ref["bases"].decode())
 
     with ThreadPoolExecutor(16) as ex:
         total = 0
-        for k, bc in enumerate(ex.map(check, range(len(regions)))):
+        want_changes, want_fasta = [], []
+        for k, (bc, ch, fa) in enumerate(ex.map(check, range(len(regions)))):
             total += bc
+            want_changes.append(ch)
+            want_fasta.append(fa)
             if k % 1000 == 999:
                 print(f"case: c3 full, {k + 1} regions match", flush=True)
     assert total > 3_000_000_000
+    # the whole batch's .changes file and polished FASTA (GenomeFile.processRegions' output loop)
+    changes, fasta = engine.batch_fix()
+    assert changes == "".join(want_changes)
+    assert fasta == "".join(want_fasta)
+    assert changes.count("\n") > 5000

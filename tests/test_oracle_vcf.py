@@ -62,6 +62,17 @@ def test_deletion_records():
                     "DC=0;XC=0;AC=0;AF=0.00\tGT\t0/0"
 
 
+def test_deletion_after_n_anchor():
+    """case_deletion_n_anchor: the deletion record's anchor is the draft's N,
+    so REF is "NAG", cBase "N" prints as "." and AF is 0.00 (Vcf.scala:131,
+    138-141); QUAL stays "." for a deletion.  Locus 14 itself: homozygous C
+    over rBase N prints 1/1 with ALT C."""
+    L = vcf_lines(K.case_deletion_n_anchor())
+    assert L[14] == "ctg\t14\t.\tNAG\t.\t.\tPASS\tDP=6;TD=6;BQ=0;MQ=60;QD=0;BC=0,0,0,0;QP=0,0,0,0;PC=6;IC=0;DC=6;" \
+                    "XC=0;AC=2;AF=0.00\tGT\t1/1"
+    assert L[13].startswith("ctg\t14\t.\tN\tC\t180\tPASS\tDP=6;") and L[13].endswith("AC=2;AF=1.00\tGT\t1/1")
+
+
 def test_ambiguous_record_and_flags():
     """case_ambiguous, locus 20: C and G 5 reads each at q30 MAPQ 60.
     qualSum C = G = 9150, base C (stable order), alt G, rBase T: the het
