@@ -76,14 +76,13 @@ def dev_sync():
 
 def regions_for(name, data, rids):
     """The regions Pilon would process: each contig split in chunks of at most
-    10 Mb (GenomeFile.contigRegions, GenomeFile.scala:67-74)."""
+    10 Mb by the library's pl_contig_regions (GenomeFile.contigRegions,
+    GenomeFile.scala:67-74)."""
+    from pilon_amd.engine import contig_regions
     out = []
     for ci in range(data.n):
-        L = len(data.draft(ci))
-        n_chunks = (L + 10_000_000 - 1) // 10_000_000
-        chunk = (L + n_chunks - 1) // n_chunks
-        for s in range(1, L + 1, chunk):
-            out.append((ci, s, min(L, s + chunk - 1), rids[ci]))
+        for s, e in contig_regions(len(data.draft(ci))):
+            out.append((ci, s, e, rids[ci]))
     return out
 
 
@@ -114,7 +113,10 @@ def cpu_baseline(cfg, data, regions, seconds):
     then are finished and counted."""
     from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
     from oracle import oracle as O
-    nth = max(1, min(16, len(os.sched_getaffinity(0))))
+    # a one-GPU box gives its process 16 CPUs, whatever the affinity mask
+    # shows; the pool takes all of them (host_cpus reports the mask)
+    host_cpus = len(os.sched_getaffinity(0))
+    nth = max(1, min(16, host_cpus))
     O.lib()
     drafts = {}
 
@@ -145,7 +147,7 @@ def cpu_baseline(cfg, data, regions, seconds):
                     drafts[ci] = data.draft(ci)
                 live.add(pool.submit(one, nxt)); nxt += 1
         wall = time.perf_counter() - t0
-    out = {"value": done / wall, "unit": "bases/s", "cores": nth, "kind": "port",
+    out = {"value": done / wall, "unit": "bases/s", "cores": nth, "host_cpus": host_cpus, "kind": "port",
            "sample": f"oracle (C restatement of Pilon's path) on the first {nreg} of {len(regions)} regions, "
                      f"one region per task on {nth} threads: {done} aligned bases in {wall:.1f}s wall "
                      f"({done / busy / 1e6:.1f} M bases/s per thread)"}
@@ -434,7 +436,10 @@ def bam_producer(data, ctx, args, W, B, aligned):
     import shutil
     import tempfile
     from pilon_amd.bam import BamFile
-    nth = max(1, min(16, len(os.sched_getaffinity(0))))
+    # a one-GPU box gives its process 16 CPUs, whatever the affinity mask
+    # shows; the pool takes all of them (host_cpus reports the mask)
+    host_cpus = len(os.sched_getaffinity(0))
+    nth = max(1, min(16, host_cpus))
     d = tempfile.mkdtemp(prefix="pilon_bam_")
     path = os.path.join(d, f"{args.config}.bam")
     try:

@@ -60,6 +60,25 @@ typedef struct pl_bam_stats {
  * host's cores (at most 64). */
 pl_status pl_bam_open(const char* path, int32_t n_threads, pl_bam** out);
 
+/* Opens a coordinate-sorted BAM through its BAI index (SAMv1 section 5.2):
+ * only the header and the index are read here; records are read per
+ * region by pl_bam_query.  index NULL looks for <path>.bai, then for path
+ * with ".bam" replaced by ".bai" (htsjdk's names).  Like Pilon
+ * (BamFile.scala:55-60, "must be indexed BAM") it refuses a BAM without an
+ * index: PL_E_INVAL. */
+pl_status pl_bam_open_indexed(const char* path, const char* index, int32_t n_threads, pl_bam** out);
+
+/* htsjdk's queryOverlapping(name, start, stop) as BamFile.process calls it
+ * (BamFile.scala:118-119): the placed records of reference `ref` whose
+ * span [alignment start, alignment end] -- one base for an unmapped read
+ * or an alignment without reference bases -- overlaps [start, stop]
+ * (1-based, closed; start <= 0 reads from the first base), in file order,
+ * as one read batch.  Only the BGZF blocks of the index chunks that can
+ * hold such records (reg2bins bins, the linear index's lower bound) are
+ * read and inflated.  The arrays belong to the pl_bam and stay valid until
+ * the next query or pl_bam_close.  Needs pl_bam_open_indexed. */
+pl_status pl_bam_query(pl_bam* bam, int32_t ref, int64_t start, int64_t stop, pl_read_soa* out);
+
 /* The sequence dictionary (@SQ order = refID). */
 int32_t     pl_bam_n_refs(const pl_bam* bam);
 const char* pl_bam_ref_name(const pl_bam* bam, int32_t ref);
@@ -70,7 +89,7 @@ int64_t     pl_bam_ref_len(const pl_bam* bam, int32_t ref);
  * pl_bam and stay valid until pl_bam_close.  The flag word is the SAM flag
  * (bits 0x001-0x800) with PL_RF_NO_QUAL, PL_RF_MATE_SAME_REF and
  * PL_RF_EXOTIC added; non-ACGT reads keep their nibbles at an even offset. */
-pl_status pl_bam_ref_reads(const pl_bam* bam, int32_t ref, pl_read_soa* out);
+pl_status pl_bam_ref_reads(const pl_bam* bam, int32_t ref, pl_read_soa* out);   /* pl_bam_open only */
 
 void pl_bam_get_stats(const pl_bam* bam, pl_bam_stats* out);
 void pl_bam_close(pl_bam* bam);

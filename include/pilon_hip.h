@@ -38,7 +38,7 @@ typedef enum pl_status {
   PL_E_OOM = 2,       /* device or host allocation failed */
   PL_E_DEVICE = 3,    /* HIP runtime error or no usable GPU */
   PL_E_EDGE = 4,      /* input on which Pilon 1.24 itself throws (see below) */
-  PL_E_OVERFLOW = 5,  /* a 32-bit device counter could leave the range Pilon's Long keeps */
+  PL_E_OVERFLOW = 5,  /* a column of 2^23 or more bases: a packed device counter could carry */
   PL_E_BUSY = 6,      /* serving queue full */
   PL_E_REJECTED = 7   /* serving request larger than the device-memory budget */
 } pl_status;
@@ -133,6 +133,8 @@ typedef struct pl_call {
 #define PL_CF_DISP_SHIFT  8        /* 3-bit disposition           */
 #define PL_CF_DISP_MASK   0x0700u
 #define PL_CF_DELETED     0x0800u  /* GenomeRegion.deleted(i)     */
+#define PL_CF_WIDE        0x1000u  /* a qualSum (a Long, BaseSum.scala:24) leaves int32: the
+                                      pileup holds its low 32 bits, pl_last_wide_columns the sums */
 
 enum pl_disposition { PL_D_NONE = 0, PL_D_CONFIRMED = 1, PL_D_SNP = 2,
                       PL_D_INS = 3, PL_D_DEL = 4, PL_D_AMB = 5 };
@@ -225,14 +227,27 @@ pl_status pl_last_timing(pl_ctx* ctx, double* pipeline_ms, double* window_ms);
  * the context's last batch (pl_batch_run, or the region that pl_region_finish
  * ran), in batch order: Vcf.writeRecord (Vcf.scala:74-163) per locus, a
  * deleted locus as an embedded "Del" record, an indel call followed by its
- * second record.  Formatted on the device from the pileup, calls and indel
- * evidence the batch left there; *text is malloc'ed (pl_free), *len bytes
- * plus a terminating NUL.  Reassembly (writeFixRecord) and duplication
- * (writeDup) records are not produced: those fixes are outside this path.
- * device_ms (may be NULL) = device time of the formatting passes. */
+ * second record, and Vcf.writeDup (Vcf.scala:194-202) before the first
+ * locus of each duplicationEvents region (copy number above 1 over more
+ * than 10,000 loci, GenomeRegion.scala:731-737).  Formatted on the device
+ * from the pileup, calls and indel evidence the batch left there; *text is
+ * malloc'ed (pl_free), *len bytes plus a terminating NUL.  Reassembly
+ * records (writeFixRecord) are not produced: those fixes are outside this
+ * path.  device_ms (may be NULL) = device time of the formatting passes
+ * and of the copy-number passes behind the duplication records. */
 #define PL_VCF_DIPLOID 0x2u   /* --diploid: no "Amb" filter              */
 #define PL_VCF_QE      0x4u   /* --vcfqe: QE= (qualSum) instead of QP=  */
 pl_status pl_batch_vcf(pl_ctx* ctx, uint32_t flags, char** text, int64_t* len, double* device_ms);
+
+/* The same text handed to the caller in order, a piece at a time, instead
+ * of one allocation: sink(user, data, n) receives each piece (data is valid
+ * during the call only) and returns 0 to go on, anything else to stop
+ * (PL_E_INVAL).  Pieces come from pinned buffers the context keeps, so a
+ * sink that writes them to a file or socket overlaps the next piece's DMA.
+ * *len (may be NULL) = the bytes handed over. */
+typedef int32_t (*pl_text_sink)(void* user, const char* data, int64_t n);
+pl_status pl_batch_vcf_stream(pl_ctx* ctx, uint32_t flags, pl_text_sink sink, void* user, int64_t* len,
+                              double* device_ms);
 
 /* The same for one finished region (pl_region_finish).  A region fed by
  * pl_region_add_reads keeps its reads on the device until pl_region_close
@@ -309,6 +324,19 @@ pl_status pl_region_bed(pl_region* reg, uint32_t flags, char** text, int64_t* le
  * The three arrays are malloc'ed (pl_free). */
 pl_status pl_kmer_graph(pl_ctx* ctx, const pl_read_soa* reads, int32_t K, int32_t min_depth, char** kmers,
                         uint8_t** next, uint8_t** alt, int64_t* n, double* device_ms);
+
+/* The columns of the context's last batch (pl_batch_run, pl_region_finish)
+ * whose qualSum of some base -- a Long in BaseSum (BaseSum.scala:24) --
+ * leaves int32, in position order: their calls carry PL_CF_WIDE and the
+ * pileup planes hold the low 32 bits of those sums.  pos is the position in
+ * the batch (the region index for pl_region_finish).  *n = their count; out
+ * (cap entries; NULL to count only) receives them.  A column needs about
+ * 66,000 bases of quality 40 at MAPQ 60 for this. */
+typedef struct pl_wide_col {
+  int64_t pos;
+  int64_t qual_sum[4];   /* BaseSum qualSum.sums for A, C, G, T */
+} pl_wide_col;
+pl_status pl_last_wide_columns(pl_ctx* ctx, pl_wide_col* out, int64_t cap, int64_t* n);
 
 /* ------------------------------------------------- fixes, FASTA, regions */
 /* GenomeFile.processRegions' output loop (GenomeFile.scala:135-162) over the

@@ -1037,10 +1037,11 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
     }
   }
 
-  /* Pass 2 (GenomeRegion.scala:273-283) and the tracks (Tracks.scala:33-190) */
-  if (vcf_flags & (8 | 16)) {
-    int32_t* t = res->tracks = (int32_t*)calloc((size_t)ORC_NTRACKS * (size_t)size, 4);
-    int16_t* copyNumber = (int16_t*)calloc((size_t)size, 2);
+  /* Pass 2 (GenomeRegion.scala:273-283): copy number, for the tracks
+     (Tracks.scala:33-190) and the VCF's duplication records */
+  int16_t* copyNumber = NULL;
+  if (vcf_flags & (1 | 8 | 16)) {
+    copyNumber = (int16_t*)calloc((size_t)size, 2);
     if (R.readCount != 0) {                                     /* Pass 2 runs after the early return */
       double sum = 0.0;
       for (int32_t i = 0; i < size; i++) sum += (double)fragCov[i];   /* NormalDistribution.mean */
@@ -1051,6 +1052,9 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
         copyNumber[i] = baseCov > 0 ? (int16_t)orc_java_round((double)sm[i] / baseCov) : 0;
       free(sm);
     }
+  }
+  if (vcf_flags & (8 | 16)) {
+    int32_t* t = res->tracks = (int32_t*)calloc((size_t)ORC_NTRACKS * (size_t)size, 4);
     int8_t* gc = (int8_t*)calloc((size_t)size, 1);
     compute_gc(contig, contig_len, start, stop, gc);
     for (int32_t i = 0; i < size; i++) {
@@ -1078,7 +1082,7 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
       bed_lines(cfg, name, start, stop, contig, res->coverage, res->badCoverage, res->clips, confirmed, kindAt,
                 deleted, copyNumber, fragCov, fsum / (double)size, &res->bed_text, &res->bed_len);
     }
-    free(gc); free(copyNumber);
+    free(gc);
   }
 
   /* final PileUp state and BaseCall per position */
@@ -1125,6 +1129,9 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
     else if (kindAt[i] == K_AMB) disp = PL_D_AMB;
     f |= (uint16_t)(disp << PL_CF_DISP_SHIFT);
     if (deleted[i]) f |= PL_CF_DELETED;
+    /* the output format's mark of a Long qualSum beyond int32 (the pileup
+       planes keep its low 32 bits; bcq holds the exact value) */
+    for (int b = 0; b < 4; b++) if (p->qs[b] != (int64_t)(int32_t)p->qs[b]) f |= PL_CF_WIDE;
     c->flags = f;
     nConfirmed += confirmed[i];
   }
@@ -1226,15 +1233,37 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
   res->stats.aligned_bases = R.alignedBases;
 
   /* GenomeRegion.writeVcf (GenomeRegion.scala:623-643) without the
-   * reassembly and duplication records: every locus, embedded when deleted */
+   * reassembly records: every locus, embedded when deleted, each
+   * duplicationEvents region (copy number > 1 with slop 2000, longer than
+   * 10,000, :731-737) announced by Vcf.writeDup (Vcf.scala:194-202) before
+   * the records of its first locus */
   if (vcf_flags & 1) {
     VcfOut o; memset(&o, 0, sizeof o);
     o.cfg = cfg; o.contig = contig; o.name = name; o.start = start; o.minDepth = minDepth;
     o.diploid = (vcf_flags & 2) != 0; o.vcfqe = (vcf_flags & 4) != 0;
     sb_append(&o.buf, &o.len, &o.cap, "", 0);
-    for (int32_t i = 0; i < size; i++) vcf_record(&o, &R.pu[i], i, deleted[i], 1);
+    uint8_t* t = (uint8_t*)calloc((size_t)size, 1);
+    for (int32_t i = 0; i < size; i++) t[i] = copyNumber[i] > 1;
+    RegionList dup = summary_regions(t, size, start, stop, 2000);
+    free(t);
+    int32_t d = 0;
+    for (int32_t i = 0; i < size; i++) {
+      while (d < dup.n && dup.v[d].stop + 1 - dup.v[d].start <= 10000) d++;
+      if (d < dup.n && dup.v[d].start == start + i) {
+        const int32_t loc = dup.v[d].start - 1;                 /* writeDup: refBase(dup.start - 1) */
+        vcf_puts(&o, name); vcf_puts(&o, "\t"); vcf_putl(&o, loc); vcf_puts(&o, "\t.\t");
+        const uint8_t rb = upper(contig[loc - 1]);
+        vcf_putn(&o, &rb, 1);
+        vcf_puts(&o, "\t<DUP>\t.\tPASS\tSVTYPE=DUP;SVLEN="); vcf_putl(&o, dup.v[d].stop + 1 - dup.v[d].start);
+        vcf_puts(&o, ";END="); vcf_putl(&o, dup.v[d].stop); vcf_puts(&o, ";IMPRECISE\tGT\t./.\n");
+        d++;
+      }
+      vcf_record(&o, &R.pu[i], i, deleted[i], 1);
+    }
+    free(dup.v);
     res->vcf_text = o.buf; res->vcf_len = o.len;
   }
+  free(copyNumber);
 
   free(tmp); free(kept); free(snpL.v); free(smallL.v); free(oneChar);
   free(confirmed); free(deleted); free(kindAt); free(fragCov);

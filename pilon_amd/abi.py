@@ -31,6 +31,7 @@ PL_CF_SCOREPOS = 0x0080
 PL_CF_DISP_SHIFT = 8
 PL_CF_DISP_MASK = 0x0700
 PL_CF_DELETED = 0x0800
+PL_CF_WIDE = 0x1000
 D_NONE, D_CONFIRMED, D_SNP, D_INS, D_DEL, D_AMB = range(6)
 
 
@@ -125,3 +126,8 @@ class PlTarget(C.Structure):
 
 SERVE_KEEP_PILEUP, SERVE_KEEP_FIX = 0x1, 0x2
 CHUNK_SIZE = 10000000   # Pilon.chunkSize (Pilon.scala:52)
+
+
+class PlWideCol(C.Structure):
+    """pl_wide_col: a column whose Long qualSum leaves int32."""
+    _fields_ = [("pos", C.c_int64), ("qual_sum", C.c_int64 * 4)]

@@ -44,6 +44,8 @@ def lib():
         P = C.POINTER
         sig = {
             "pl_bam_open": (C.c_int, [C.c_char_p, C.c_int32, P(C.c_void_p)]),
+            "pl_bam_open_indexed": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, P(C.c_void_p)]),
+            "pl_bam_query": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int64, P(PlReadSoa)]),
             "pl_bam_n_refs": (C.c_int32, [C.c_void_p]),
             "pl_bam_ref_name": (C.c_char_p, [C.c_void_p, C.c_int32]),
             "pl_bam_ref_len": (C.c_int64, [C.c_void_p, C.c_int32]),
@@ -116,7 +118,36 @@ class BamFile:
 
     def batch(self, ref):
         """A numpy-owned copy of reference `ref`'s reads."""
-        s = self.soa(ref)
+        return _batch_of(self.soa(ref))
+
+
+class IndexedBamFile(BamFile):
+    """A coordinate-sorted BAM read through its BAI index: header and index
+    at open, records per region (htsjdk's queryOverlapping, BamFile.scala:
+    55-60,118-119)."""
+
+    def __init__(self, path, index=None, threads=0):
+        h = C.c_void_p()
+        st = lib().pl_bam_open_indexed(os.fsencode(path), os.fsencode(index) if index else None, threads, C.byref(h))
+        if st:
+            raise BamError(st, lib().pl_bam_last_error().decode())
+        self.h = h
+        n = lib().pl_bam_n_refs(h)
+        self.names = [lib().pl_bam_ref_name(h, r).decode() for r in range(n)]
+        self.lens = [lib().pl_bam_ref_len(h, r) for r in range(n)]
+
+    def query(self, ref, start, stop):
+        """A numpy-owned copy of the reads of `ref` overlapping [start, stop] (1-based)."""
+        s = PlReadSoa()
+        st = lib().pl_bam_query(self.h, ref, int(start), int(stop), C.byref(s))
+        if st:
+            raise BamError(st, lib().pl_bam_last_error().decode())
+        return _batch_of(s)
+
+
+def _batch_of(s):
+    """A numpy-owned copy of a library-owned read batch."""
+    if True:
         n = s.n_reads
 
         def arr(ptr, count, ctype, dtype):

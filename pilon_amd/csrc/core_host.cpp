@@ -6,6 +6,8 @@
 // deletion pass) can be compared with the oracle without a GPU.  It is not
 // part of libpilonhip.so and the product never loads it.
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -128,6 +130,11 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
   pl_call* calls = calls_out;
   std::vector<IndelRec> recs;
   std::vector<uint32_t> defer;
+  std::map<uint32_t, std::array<int64_t, 4>> wide;   // exact qualSums of wide columns
+  auto patch = [&](Col& c, uint32_t g) {
+    auto it = wide.find(g);
+    if (it != wide.end()) for (int b = 0; b < 4; b++) c.qs[b] = it->second[b];
+  };
   std::vector<Cand> cands;
   std::vector<uint64_t> bs(4 * (size_t)W), mqq(W), insA(W), delA(W);
   std::vector<uint32_t> badA(W), clpA(W);
@@ -215,6 +222,10 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
         PosOut o;
         finalize_position(b4, mqq[p], insA[p], delA[p], badA[p], clpA[p], apc, ais, ref[L0 + p - 1], cf, o);
         if (o.overflow) S.overflow = 1;
+        if (o.wide) {
+          o.call.flags |= PL_CF_WIDE;
+          wide[(uint32_t)g] = {o.qs[0], o.qs[1], o.qs[2], o.qs[3]};
+        }
         for (int f = 0; f < PL_NFIELDS; f++) pile[(int64_t)f * G + g] = o.f[f];
         calls[g] = o.call;
         if (o.defer) defer.push_back((uint32_t)g);
@@ -249,10 +260,12 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
     const int32_t j = job_of(g);
     const JobDev& J = jobs[j];
     Col c = load_col(pile, G, g);
+    patch(c, g);
     int32_t ilen;
     int64_t wrec;
     pl_call cl = full_call(&R, J, c, ref[J.start + (g - J.out) - 1], recs.data(), rkey.data(), rval.data(),
                            (int64_t)recs.size(), g, cf, &ilen, &wrec);
+    cl.flags |= calls[g].flags & PL_CF_WIDE;
     calls[g] = cl;
     const int d = call_disp(cl);
     const int64_t dep = c.depth();
@@ -275,10 +288,12 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
     for (uint32_t k = m0; k < mut_n; k++) {
       const uint32_t g = mutated[k];
       Col c = load_col(pile, G, g);
+      patch(c, g);
       int32_t ilen;
       int64_t wrec;
       pl_call cl = full_call(&R, J, c, ref[J.start + (g - J.out) - 1], recs.data(), rkey.data(), rval.data(),
                              (int64_t)recs.size(), g, cf, &ilen, &wrec);
+      cl.flags |= calls[g].flags & PL_CF_WIDE;
       calls[g].base = cl.base; calls[g].alt = cl.alt; calls[g].q = cl.q;
       calls[g].flags = (uint16_t)((cl.flags & ~PL_CF_DISP_MASK) | (calls[g].flags & PL_CF_DISP_MASK) | PL_CF_DELETED);
     }

@@ -186,11 +186,25 @@ bool find_cg(const uint8_t* base, size_t p, size_t e, uint32_t* count, size_t* w
 
 }  // namespace
 
+// One reference of a BAI index (SAMv1 section 5.2): chunks [beg, end) of
+// virtual file offsets (compressed block offset << 16 | offset in the
+// inflated block) per bin, and the linear index of 16 kb windows.
+struct BaiRef {
+  std::vector<std::pair<uint32_t, std::vector<std::pair<uint64_t, uint64_t>>>> bins;
+  std::vector<uint64_t> lin;
+};
+
 struct pl_bam {
   std::vector<std::string> names;
   std::vector<int64_t> lens;
   std::vector<RefReads> refs;
   pl_bam_stats st;
+  // opened through its index: records are read per query
+  bool indexed = false;
+  std::string path;
+  int nth = 1;
+  std::vector<BaiRef> bai;
+  RefReads query;            // the last pl_bam_query's reads
 };
 
 namespace {
@@ -514,9 +528,306 @@ pl_status open_impl(const char* path, int nth, pl_bam* B) {
   return PL_OK;
 }
 
+// The bins that can hold records overlapping [beg, end), 0-based (SAMv1
+// section 5.3, reg2bins).
+std::vector<uint32_t> reg2bins(int64_t beg, int64_t end) {
+  std::vector<uint32_t> v{0};
+  if (end <= beg) return v;
+  end -= 1;
+  for (int64_t k = 1 + (beg >> 26); k <= 1 + (end >> 26); k++) v.push_back((uint32_t)k);
+  for (int64_t k = 9 + (beg >> 23); k <= 9 + (end >> 23); k++) v.push_back((uint32_t)k);
+  for (int64_t k = 73 + (beg >> 20); k <= 73 + (end >> 20); k++) v.push_back((uint32_t)k);
+  for (int64_t k = 585 + (beg >> 17); k <= 585 + (end >> 17); k++) v.push_back((uint32_t)k);
+  for (int64_t k = 4681 + (beg >> 14); k <= 4681 + (end >> 14); k++) v.push_back((uint32_t)k);
+  return v;
+}
+
+constexpr uint32_t kBaiMetaBin = 37450;   // pseudo-bin of per-reference counts, not a bin of records
+
+// The BAI next to a BAM: path + ".bai", or path with ".bam" replaced by ".bai"
+// (htsjdk's two names), or the caller's.
+pl_status read_bai(const std::string& path, size_t n_ref, std::vector<BaiRef>& out) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return fail(PL_E_INVAL, "cannot open index %s", path.c_str());
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(f, std::fclose);
+  std::vector<uint8_t> d;
+  uint8_t buf[1 << 16];
+  for (size_t k; (k = std::fread(buf, 1, sizeof buf, f)) > 0;) d.insert(d.end(), buf, buf + k);
+  size_t p = 0;
+  auto need = [&](size_t n) { return p + n <= d.size(); };
+  if (!need(8) || std::memcmp(d.data(), "BAI\1", 4) != 0) return fail(PL_E_INVAL, "%s is not a BAI index", path.c_str());
+  const int32_t nr = rdi32(d.data() + 4);
+  p = 8;
+  if (nr < 0 || (size_t)nr != n_ref)
+    return fail(PL_E_INVAL, "index %s has %d references, the BAM %zu", path.c_str(), nr, n_ref);
+  out.assign((size_t)nr, BaiRef());
+  for (int32_t r = 0; r < nr; r++) {
+    if (!need(4)) return fail(PL_E_INVAL, "truncated index %s", path.c_str());
+    const int32_t nbin = rdi32(d.data() + p);
+    p += 4;
+    if (nbin < 0) return fail(PL_E_INVAL, "corrupt index %s", path.c_str());
+    for (int32_t b = 0; b < nbin; b++) {
+      if (!need(8)) return fail(PL_E_INVAL, "truncated index %s", path.c_str());
+      const uint32_t bin = rd32(d.data() + p);
+      const int32_t nch = rdi32(d.data() + p + 4);
+      p += 8;
+      if (nch < 0 || !need(16 * (size_t)nch)) return fail(PL_E_INVAL, "truncated index %s", path.c_str());
+      std::vector<std::pair<uint64_t, uint64_t>> ch((size_t)nch);
+      for (int32_t c = 0; c < nch; c++) {
+        uint64_t a, e;
+        std::memcpy(&a, d.data() + p, 8);
+        std::memcpy(&e, d.data() + p + 8, 8);
+        ch[(size_t)c] = {a, e};
+        p += 16;
+      }
+      if (bin != kBaiMetaBin) out[(size_t)r].bins.emplace_back(bin, std::move(ch));
+    }
+    if (!need(4)) return fail(PL_E_INVAL, "truncated index %s", path.c_str());
+    const int32_t ni = rdi32(d.data() + p);
+    p += 4;
+    if (ni < 0 || !need(8 * (size_t)ni)) return fail(PL_E_INVAL, "truncated index %s", path.c_str());
+    out[(size_t)r].lin.resize((size_t)ni);
+    if (ni) std::memcpy(out[(size_t)r].lin.data(), d.data() + p, 8 * (size_t)ni);
+    p += 8 * (size_t)ni;
+    std::sort(out[(size_t)r].bins.begin(), out[(size_t)r].bins.end(),
+              [](const std::pair<uint32_t, std::vector<std::pair<uint64_t, uint64_t>>>& a,
+                 const std::pair<uint32_t, std::vector<std::pair<uint64_t, uint64_t>>>& b) { return a.first < b.first; });
+  }
+  return PL_OK;
+}
+
+// Whole BGZF blocks of the file from compressed offset `at`: reads until the
+// block starting at `last` (inclusive) is in, or the file ends.  Returns the
+// compressed bytes and the blocks within them.
+pl_status read_blocks(FILE* f, uint64_t at, uint64_t last, std::vector<uint8_t>& cb, std::vector<Block>& blocks,
+                      std::vector<uint64_t>& coff) {
+  cb.clear();
+  blocks.clear();
+  coff.clear();
+  if (fseeko(f, (off_t)at, SEEK_SET) != 0) return fail(PL_E_INVAL, "cannot seek to %llu", (unsigned long long)at);
+  size_t utot = 0;
+  for (uint64_t c = at; c <= last;) {
+    uint8_t h[18];
+    if (std::fread(h, 1, 18, f) != 18) break;                     // end of file
+    if (h[0] != 0x1f || h[1] != 0x8b || h[2] != 8 || !(h[3] & 4))
+      return fail(PL_E_INVAL, "not a BGZF block at byte %llu", (unsigned long long)c);
+    const size_t xlen = rd16(h + 10);
+    std::vector<uint8_t> blk(h, h + 18);
+    blk.resize(12 + xlen);
+    if (xlen > 6 && std::fread(blk.data() + 18, 1, xlen - 6, f) != xlen - 6)
+      return fail(PL_E_INVAL, "truncated BGZF block at byte %llu", (unsigned long long)c);
+    size_t bsize = 0;
+    for (size_t x = 12; x + 4 <= 12 + xlen;) {
+      const size_t slen = rd16(blk.data() + x + 2);
+      if (blk[x] == 66 && blk[x + 1] == 67 && slen == 2) bsize = (size_t)rd16(blk.data() + x + 4) + 1;
+      x += 4 + slen;
+    }
+    if (!bsize || bsize < 20 + xlen) return fail(PL_E_INVAL, "BGZF block without a valid BC (BSIZE) subfield");
+    blk.resize(bsize);
+    if (std::fread(blk.data() + 12 + xlen, 1, bsize - 12 - xlen, f) != bsize - 12 - xlen)
+      return fail(PL_E_INVAL, "truncated BGZF block at byte %llu", (unsigned long long)c);
+    Block b;
+    b.cdata = cb.size() + 12 + xlen;
+    b.clen = bsize - xlen - 20;
+    b.crc = rd32(blk.data() + bsize - 8);
+    b.isize = rd32(blk.data() + bsize - 4);
+    b.uoff = utot;
+    if (b.isize > 65536) return fail(PL_E_INVAL, "BGZF block ISIZE %u above 64 KiB", b.isize);
+    utot += b.isize;
+    cb.insert(cb.end(), blk.begin(), blk.end());
+    blocks.push_back(b);
+    coff.push_back(c);
+    c += bsize;
+  }
+  return PL_OK;
+}
+
+pl_status open_indexed_impl(const char* path, const char* index, int nth, pl_bam* B) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(PL_E_INVAL, "cannot open %s", path);
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(f, std::fclose);
+  B->path = path;
+  B->nth = nth;
+  B->indexed = true;
+  // the header: blocks from the start until it parses
+  std::vector<uint8_t> cb, ub;
+  std::vector<Block> blocks;
+  std::vector<uint64_t> coff;
+  uint64_t at = 0;
+  for (;;) {
+    if (pl_status s = read_blocks(f, at, at, cb, blocks, coff)) return s;
+    if (blocks.empty()) return fail(PL_E_INVAL, "not a BAM file (no complete header)");
+    const size_t carry = ub.size();
+    ub.resize(carry + blocks[0].isize);
+    if (pl_status s = inflate_blocks(cb, blocks, ub, carry, 1)) return s;
+    at = coff[0] + cb.size();
+    const int64_t h = parse_header(ub.data(), ub.size(), B);
+    if (h < 0) return fail(PL_E_INVAL, "not a BAM file (bad magic or header)");
+    if (h > 0) break;
+  }
+  // the index (BamFile.scala:55-60: Pilon refuses a BAM without one)
+  std::vector<std::string> cand;
+  if (index && *index) cand.push_back(index);
+  else {
+    cand.push_back(std::string(path) + ".bai");
+    const std::string p(path);
+    if (p.size() > 4 && p.compare(p.size() - 4, 4, ".bam") == 0) cand.push_back(p.substr(0, p.size() - 4) + ".bai");
+  }
+  for (const std::string& c : cand) {
+    FILE* t = std::fopen(c.c_str(), "rb");
+    if (!t) continue;
+    std::fclose(t);
+    return read_bai(c, B->names.size(), B->bai);
+  }
+  return fail(PL_E_INVAL, "%s must be indexed BAM (no %s)", path, cand[0].c_str());
+}
+
+// Records of reference `ref` overlapping [start, stop] (1-based, closed;
+// start <= 0 from the first base), through the index: the chunks of the
+// bins reg2bins gives, after the linear index's lower bound, merged and read
+// in file order; decoded like the whole-file path; kept if their span
+// [pos, pos + reference length - 1] (one base for an unmapped or
+// zero-length alignment) overlaps the interval.
+pl_status query_impl(pl_bam* B, int32_t ref, int64_t start, int64_t stop) {
+  B->query = RefReads();
+  const BaiRef& I = B->bai[(size_t)ref];
+  const int64_t beg = std::max<int64_t>(start - 1, 0), end = std::max<int64_t>(stop, beg);
+  uint64_t min_off = 0;
+  if (!I.lin.empty() && (size_t)(beg >> 14) < I.lin.size()) min_off = I.lin[(size_t)(beg >> 14)];
+  std::vector<std::pair<uint64_t, uint64_t>> ch;
+  for (uint32_t bin : reg2bins(beg, end)) {
+    auto it = std::lower_bound(I.bins.begin(), I.bins.end(), bin,
+                               [](const std::pair<uint32_t, std::vector<std::pair<uint64_t, uint64_t>>>& a,
+                                  uint32_t b) { return a.first < b; });
+    if (it == I.bins.end() || it->first != bin) continue;
+    for (const auto& c : it->second)
+      if (c.second > min_off) ch.push_back(c);
+  }
+  std::sort(ch.begin(), ch.end());
+  std::vector<std::pair<uint64_t, uint64_t>> merged;
+  for (const auto& c : ch) {
+    if (!merged.empty() && c.first <= merged.back().second) merged.back().second = std::max(merged.back().second, c.second);
+    else merged.push_back(c);
+  }
+  // decode the chunks into a scratch reader with the same dictionary
+  pl_bam Q;
+  Q.names = B->names;
+  Q.lens = B->lens;
+  Q.refs.resize(B->names.size());
+  std::memset(&Q.st, 0, sizeof Q.st);
+  FILE* f = std::fopen(B->path.c_str(), "rb");
+  if (!f) return fail(PL_E_INVAL, "cannot open %s", B->path.c_str());
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(f, std::fclose);
+  std::vector<uint8_t> cb, ub;
+  std::vector<Block> blocks;
+  std::vector<uint64_t> coff;
+  for (const auto& c : merged) {
+    const uint64_t c0 = c.first >> 16, c1 = c.second >> 16;
+    if (pl_status s = read_blocks(f, c0, c1, cb, blocks, coff)) return s;
+    if (blocks.empty() || coff[0] != c0) return fail(PL_E_INVAL, "index chunk outside the BAM");
+    size_t utot = 0;
+    for (const Block& b : blocks) utot += b.isize;
+    ub.assign(utot, 0);
+    if (pl_status s = inflate_blocks(cb, blocks, ub, 0, B->nth)) return s;
+    const size_t u0 = (size_t)(c.first & 0xffff);
+    size_t u1 = utot;
+    for (size_t k = 0; k < blocks.size(); k++)
+      if (coff[k] == c1) { u1 = blocks[k].uoff + (size_t)(c.second & 0xffff); break; }
+    if (u0 > u1 || u1 > utot) return fail(PL_E_INVAL, "index chunk outside the BAM");
+    size_t used = u0;
+    if (pl_status s = decode_records(&Q, ub, u0, u1, B->nth, &used)) return s;
+    if (used != u1) return fail(PL_E_INVAL, "index chunk ends inside a record");
+  }
+  // keep the reads of ref overlapping the interval
+  const RefReads& R = Q.refs[(size_t)ref];
+  RefReads& O = B->query;
+  for (int64_t i = 0; i < R.n; i++) {
+    int64_t rl = 0;
+    for (int64_t k = R.cigar_off[i]; k < R.cigar_off[i + 1]; k++) {
+      const uint32_t op = R.cigar[k] & 15u;
+      if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rl += R.cigar[k] >> 4;
+    }
+    const int64_t ps = R.pos[i];
+    const int64_t pe = ((R.flag[i] & 0x4) || rl == 0) ? ps : ps + rl - 1;
+    if (ps > std::max<int64_t>(stop, 1) || pe < std::max<int64_t>(start, 1)) continue;
+    const int64_t s0 = R.seq_off[i], L = R.seq_off[i + 1] - s0, oi = O.n;
+    O.pos.push_back(R.pos[i]); O.tlen.push_back(R.tlen[i]); O.flag.push_back(R.flag[i]); O.mapq.push_back(R.mapq[i]);
+    O.cigar.insert(O.cigar.end(), R.cigar.begin() + R.cigar_off[i], R.cigar.begin() + R.cigar_off[i + 1]);
+    O.cigar_off.push_back((int64_t)O.cigar.size());
+    const int64_t d0 = O.seq_off.back();
+    O.seq_off.push_back(d0 + L);
+    O.quals.insert(O.quals.end(), R.quals.begin() + s0, R.quals.begin() + s0 + L);
+    O.bases2.resize((size_t)(d0 + L + 3) / 4, 0);
+    if (R.flag[i] & PL_RF_EXOTIC) {
+      const auto it = std::lower_bound(R.exo_read.begin(), R.exo_read.end(), i);
+      const int64_t o = R.exo_off[(size_t)(it - R.exo_read.begin())];
+      O.exo_read.push_back(oi);
+      O.exo_off.push_back((int64_t)O.nib.size() * 2);
+      O.nib.insert(O.nib.end(), R.nib.begin() + o / 2, R.nib.begin() + o / 2 + (L + 1) / 2);
+    } else {
+      for (int64_t k = 0; k < L; k++) {
+        const int64_t a = s0 + k, b = d0 + k;
+        O.bases2[(size_t)(b >> 2)] |= (uint8_t)(((R.bases2[(size_t)(a >> 2)] >> (2 * (a & 3))) & 3) << (2 * (b & 3)));
+      }
+    }
+    O.n++;
+  }
+  if (O.nib.empty()) O.nib.push_back(0);
+  B->st.records += Q.st.records;
+  B->st.placed += O.n;
+  B->st.inflate_s += Q.st.inflate_s;
+  return PL_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+pl_status pl_bam_open_indexed(const char* path, const char* index, int32_t n_threads, pl_bam** out) {
+  if (!path || !out) return fail(PL_E_INVAL, "null argument");
+  *out = nullptr;
+  int nth = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
+  nth = std::max(1, std::min(nth, 64));
+  std::unique_ptr<pl_bam> B(new (std::nothrow) pl_bam());
+  if (!B) return fail(PL_E_OOM, "host allocation failed");
+  std::memset(&B->st, 0, sizeof B->st);
+  B->st.threads = nth;
+  try {
+    if (pl_status s = open_indexed_impl(path, index, nth, B.get())) return s;
+  } catch (const std::bad_alloc&) {
+    return fail(PL_E_OOM, "host allocation failed while opening %s", path);
+  }
+  *out = B.release();
+  return PL_OK;
+}
+
+pl_status pl_bam_query(pl_bam* bam, int32_t ref, int64_t start, int64_t stop, pl_read_soa* out) {
+  if (!bam || !out) return fail(PL_E_INVAL, "null argument");
+  if (!bam->indexed) return fail(PL_E_INVAL, "the BAM was not opened with its index (pl_bam_open_indexed)");
+  if (ref < 0 || ref >= (int32_t)bam->names.size()) return fail(PL_E_INVAL, "refID %d outside the dictionary", ref);
+  try {
+    if (pl_status s = query_impl(bam, ref, start, stop)) return s;
+  } catch (const std::bad_alloc&) {
+    return fail(PL_E_OOM, "host allocation failed in a query");
+  }
+  const RefReads& R = bam->query;
+  std::memset(out, 0, sizeof *out);
+  out->n_reads = R.n;
+  out->pos = R.pos.data();
+  out->tlen = R.tlen.data();
+  out->flag = R.flag.data();
+  out->mapq = R.mapq.data();
+  out->cigar_off = R.cigar_off.data();
+  out->cigar = R.cigar.data();
+  out->seq_off = R.seq_off.data();
+  out->bases2 = R.bases2.data();
+  out->quals = R.quals.data();
+  out->n_exotic = (int64_t)R.exo_read.size();
+  out->exo_read = R.exo_read.data();
+  out->exo_off = R.exo_off.data();
+  out->nib = R.nib.data();
+  return PL_OK;
+}
 
 pl_status pl_bam_open(const char* path, int32_t n_threads, pl_bam** out) {
   if (!path || !out) return fail(PL_E_INVAL, "null argument");
@@ -552,6 +863,7 @@ int64_t pl_bam_ref_len(const pl_bam* bam, int32_t ref) {
 pl_status pl_bam_ref_reads(const pl_bam* bam, int32_t ref, pl_read_soa* out) {
   if (!bam || !out) return fail(PL_E_INVAL, "null argument");
   if (ref < 0 || ref >= (int32_t)bam->refs.size()) return fail(PL_E_INVAL, "refID %d outside the dictionary", ref);
+  if (bam->indexed) return fail(PL_E_INVAL, "an indexed reader reads records per region (pl_bam_query)");
   const RefReads& R = bam->refs[(size_t)ref];
   std::memset(out, 0, sizeof *out);
   out->n_reads = R.n;

@@ -160,4 +160,6 @@ struct EngineArgs {
   uint32_t* rkey; uint32_t* rval;   // sorted record keys/indices
   uint32_t* mutated; uint32_t* mutated_n; uint32_t mutated_cap;
   int32_t* overflow;                // capacity overflow flag
+  struct WideCol* wide; uint32_t* wide_n; uint32_t wide_cap;   // exact sums of wide columns
+  int32_t* win_wide;                // [2 * n_wins]: each window's first wide entry and count
 };

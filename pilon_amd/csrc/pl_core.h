@@ -872,31 +872,68 @@ PL_HD int64_t lower_bound_u32(const uint32_t* k, int64_t n, uint32_t x) {
 }
 
 // ------------------------------------------------------------ window flush
-// Window accumulators use packed 64-bit adds: for a pair (lo, hi) of 32-bit
-// counters the word holds lo + hi * 2^32 (mod 2^64).  lo must stay inside
-// int32 (checked through the depth limit below); hi then wraps exactly like a
-// Java Int.
+// Window accumulators use packed 64-bit adds.  A per-base word holds
+// count * 2^40 + sum(qual * mq1) (mod 2^64): the sum is BaseSum's Long
+// qualSum (BaseSum.scala:24), kept exact inside +-2^39, and the count stays
+// below 2^24; one 64-bit atomic updates both.  The other words hold a pair
+// (lo, hi) of Java Ints as lo + hi * 2^32: lo (mqSum, insQual, delQual: at
+// most 256 per base) must stay inside int32, hi then wraps like a Java Int.
 PL_HD void unpack_pair(uint64_t x, int32_t* lo, int32_t* hi) {
   int32_t l = (int32_t)(uint32_t)x;
   *lo = l;
   *hi = (int32_t)(uint32_t)((x - (uint64_t)(int64_t)l) >> 32);
 }
+#define PL_BASE_SHIFT 40
+PL_HD void unpack_base(uint64_t x, int64_t* qual_sum, uint32_t* count) {
+  const int64_t l = (int64_t)(x << (64 - PL_BASE_SHIFT)) >> (64 - PL_BASE_SHIFT);   // low 40 bits, signed
+  *qual_sum = l;
+  *count = (uint32_t)((x - (uint64_t)l) >> PL_BASE_SHIFT);
+}
 PL_HD uint64_t pack_base(int32_t qual, int32_t mq1) {        // count + qual*mq1
-  return (1ull << 32) + (uint64_t)(int64_t)j_mul(qual, mq1);
+  return (1ull << PL_BASE_SHIFT) + (uint64_t)(int64_t)j_mul(qual, mq1);
 }
 PL_HD uint64_t pack_mq_q(int32_t mq1, int32_t qual) {        // mqSum += mq1, qSum += qual
   return (uint64_t)(int64_t)mq1 + ((uint64_t)(int64_t)qual << 32);
 }
 
-// Above this many bases at one position a 32-bit qualSum could differ from
-// Pilon's Long BaseSum (127 * 256 * 65536 > 2^31): refused with PL_E_OVERFLOW.
-#define PL_DEPTH_LIMIT 65536
+// Bases (plus deletions) one column may hold: beyond it the packed mqSum
+// (256 per base at MAPQ 255) could carry into qSum, and Pilon's own Int
+// mqSum wraps; refused with PL_E_OVERFLOW.
+#define PL_DEPTH_LIMIT (1 << 23)
+
+// A column whose Long qualSum of some base leaves int32 ("wide"): the
+// pileup planes hold the low 32 bits (Long.toInt), PL_CF_WIDE marks its
+// call, and the four exact sums are kept beside the batch in one list per
+// window.  Every pass that reads qualSum again patches them in.
+struct WideCol {
+  uint32_t gpos;
+  int32_t pad;
+  int64_t qs[4];
+};
+
+// The exact qualSum of a wide column (PL_CF_WIDE) into c: the entry of
+// batch position g in its window's list (win_wide: first entry and count
+// per window; windows of W positions from each region's first).
+PL_HD void wide_patch(Col& c, const WideCol* wide, uint32_t wide_cap, const int32_t* win_wide, const JobDev& J,
+                      int32_t W, int64_t g) {
+  const int32_t w = J.win0 + (int32_t)((g - J.out) / W);
+  const int64_t off = win_wide[2 * w];
+  int64_t n = win_wide[2 * w + 1];
+  if (off + n > (int64_t)wide_cap) n = (int64_t)wide_cap - off;   // an overfull list: the batch runs again
+  for (int64_t k = 0; k < n; k++)
+    if (wide[off + k].gpos == (uint32_t)g) {
+      for (int b = 0; b < 4; b++) c.qs[b] = wide[off + k].qs[b];
+      return;
+    }
+}
 
 struct PosOut {
   int32_t f[PL_NFIELDS];
   pl_call call;
   bool defer;        // needs the indel strings (indel pass)
   bool overflow;
+  bool wide;         // some qualSum leaves int32 (exact sums in qs)
+  int64_t qs[4];
 };
 
 // One column at the end of a window: counters, physCov (PileUpRegion.scala:
@@ -907,19 +944,24 @@ PL_HD void finalize_position(const uint64_t bs[4], uint64_t mqq, uint64_t ins, u
                              const KConf& cf, PosOut& o) {
   Col c;
   o.overflow = false;
+  o.wide = false;
+  uint32_t depth = 0;
   for (int b = 0; b < 4; b++) {
-    int32_t qs, n;
-    unpack_pair(bs[b], &qs, &n);
-    c.bc[b] = (int64_t)(uint32_t)n;
+    int64_t qs;
+    uint32_t n;
+    unpack_base(bs[b], &qs, &n);
+    c.bc[b] = (int64_t)n;
     c.qs[b] = qs;
-    o.f[PL_F_COUNT_A + b] = n;
-    o.f[PL_F_QUAL_A + b] = qs;
-    if ((uint32_t)n >= PL_DEPTH_LIMIT) o.overflow = true;
+    o.qs[b] = qs;
+    o.f[PL_F_COUNT_A + b] = (int32_t)n;
+    o.f[PL_F_QUAL_A + b] = (int32_t)qs;   // Long.toInt; exact value in the wide list
+    if (qs != (int64_t)(int32_t)qs) o.wide = true;
+    depth += n;
   }
   unpack_pair(mqq, &c.mqSum, &c.qSum);
   unpack_pair(ins, &c.insQual, &c.insertions);
   unpack_pair(del, &c.delQual, &c.deletions);
-  if ((uint32_t)c.insertions >= PL_DEPTH_LIMIT || (uint32_t)c.deletions >= PL_DEPTH_LIMIT) o.overflow = true;
+  if (depth + (uint32_t)c.deletions >= PL_DEPTH_LIMIT || (uint32_t)c.insertions >= PL_DEPTH_LIMIT) o.overflow = true;
   o.f[PL_F_MQSUM] = c.mqSum;
   o.f[PL_F_QSUM] = c.qSum;
   o.f[PL_F_PHYSCOV] = pc;

@@ -115,8 +115,9 @@ struct Lane {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {};
   DBuf rsarr, kparr, jobs, wins, worder, costk, costv, costtk, costtv, pile, calls, st, rec, rkey, rval, rtk, rtv,
-      cand, cands, ckey, cval, ctk, ctv, defer, mutated, counters, hist, tiles, strpool, stridx, stroff, gath;
-  uint32_t rec_cap = 0, cand_cap = 0, defer_cap = 0, mut_cap = 0;
+      cand, cands, ckey, cval, ctk, ctv, defer, mutated, counters, hist, tiles, strpool, stridx, stroff, gath,
+      wide, winwide;
+  uint32_t rec_cap = 0, cand_cap = 0, defer_cap = 0, mut_cap = 0, wide_cap = 0;
   std::vector<JobDev> hjobs;
   std::vector<WinDev> hwins;
   std::vector<JobStat> hst;
@@ -154,19 +155,22 @@ struct Lane {
   size_t device_bytes() const {
     const DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st,
                        &rec, &rkey, &rval, &rtk, &rtv, &cand, &cands, &ckey, &cval, &ctk, &ctv, &defer, &mutated,
-                       &counters, &hist, &tiles, &strpool, &stridx, &stroff, &gath};
+                       &counters, &hist, &tiles, &strpool, &stridx, &stroff, &gath, &wide, &winwide};
     size_t t = 0;
     for (const DBuf* x : b) t += x->cap;
     return t;
   }
-  void free_buffers() {
+  void for_each_buffer(void (*f)(DBuf&)) {
     DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st, &rec,
                  &rkey, &rval, &rtk, &rtv, &cand, &cands, &ckey, &cval, &ctk, &ctv, &defer, &mutated, &counters,
-                 &hist, &tiles, &strpool, &stridx, &stroff, &gath};
-    for (DBuf* x : b) x->release();
+                 &hist, &tiles, &strpool, &stridx, &stroff, &gath, &wide, &winwide};
+    for (DBuf* x : b) f(*x);
+  }
+  void free_buffers() {
+    for_each_buffer([](DBuf& d) { d.release(); });
     tables_valid = false;
     rs_seen = 0;
-    rec_cap = cand_cap = defer_cap = mut_cap = 0;
+    rec_cap = cand_cap = defer_cap = mut_cap = wide_cap = 0;
   }
 };
 
@@ -524,6 +528,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
   }
   // ---- capacities (grown and re-run on overflow)
   if (!L->rec_cap) { L->rec_cap = 1u << 18; L->cand_cap = 1u << 16; L->defer_cap = 1u << 16; L->mut_cap = 1u << 16; }
+  if (!L->wide_cap) L->wide_cap = 1u << 12;
   L->rec_cap = std::max<uint32_t>(L->rec_cap, (uint32_t)std::min<int64_t>(G / 16 + 65536, 1ll << 30));
   L->cand_cap = std::max<uint32_t>(L->cand_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
   L->defer_cap = std::max<uint32_t>(L->defer_cap, (uint32_t)std::min<int64_t>(G / 256 + 16384, 1ll << 30));
@@ -531,6 +536,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
   HIPCHK(L->jobs.ensure(sizeof(JobDev) * n_jobs));
   HIPCHK(L->wins.ensure(sizeof(WinDev) * n_wins));
   HIPCHK(L->worder.ensure(4 * (size_t)n_wins));
+  HIPCHK(L->winwide.ensure(8 * (size_t)n_wins));
   HIPCHK(L->costk.ensure(4 * (size_t)n_wins));
   HIPCHK(L->costv.ensure(4 * (size_t)n_wins));
   HIPCHK(L->costtk.ensure(4 * (size_t)n_wins));
@@ -568,6 +574,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
     HIPCHK(L->ctk.ensure(4 * (size_t)L->cand_cap)); HIPCHK(L->ctv.ensure(4 * (size_t)L->cand_cap));
     HIPCHK(L->defer.ensure(4 * (size_t)L->defer_cap));
     HIPCHK(L->mutated.ensure(4 * (size_t)L->mut_cap));
+    HIPCHK(L->wide.ensure(sizeof(WideCol) * (size_t)L->wide_cap));
     // sort scratch for the largest sort, allocated before any capture
     {
       const uint32_t mx = std::max<uint32_t>(std::max<uint32_t>(L->rec_cap, L->cand_cap), (uint32_t)n_wins);
@@ -596,6 +603,8 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
     A.defer = L->defer.as<uint32_t>(); A.defer_n = cnt + 2; A.defer_cap = L->defer_cap;
     A.mutated = L->mutated.as<uint32_t>(); A.mutated_n = cnt + 3; A.mutated_cap = L->mut_cap;
     A.overflow = (int32_t*)(cnt + 4);
+    A.wide = L->wide.as<WideCol>(); A.wide_n = cnt + 6; A.wide_cap = L->wide_cap;
+    A.win_wide = L->winwide.as<int32_t>();
 
     if (use_graph) {
       GraphKey key;
@@ -646,6 +655,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
     if (hc[1] > L->cand_cap) { L->cand_cap = (uint32_t)std::min<uint64_t>(2ull * hc[1], 1u << 30); grow = true; }
     if (hc[2] > L->defer_cap) { L->defer_cap = (uint32_t)std::min<uint64_t>(2ull * hc[2], 1u << 30); grow = true; }
     if (hc[3] > L->mut_cap) { L->mut_cap = (uint32_t)std::min<uint64_t>(2ull * hc[3], 1u << 30); grow = true; }
+    if (hc[6] > L->wide_cap) { L->wide_cap = (uint32_t)std::min<uint64_t>(2ull * hc[6], 1u << 30); grow = true; }
     if (!grow) {
       if (hc[4]) return fail(PL_E_DEVICE, "capacity overflow flag set without an overfull counter");
       return PL_OK;
@@ -757,7 +767,7 @@ pl_status job_status(Lane* L, size_t j) {
   if (S.err_read_edge != INT64_MAX)
     return fail(PL_E_EDGE, "read %lld: Pilon 1.24 throws here (indel normalised before region start, or clip without SEQ)",
                 (long long)S.err_read_edge);
-  if (S.overflow) return fail(PL_E_OVERFLOW, "a column exceeds %d bases", PL_DEPTH_LIMIT);
+  if (S.overflow) return fail(PL_E_OVERFLOW, "a column holds %d or more bases and deletions", PL_DEPTH_LIMIT);
   return PL_OK;
 }
 
@@ -1056,29 +1066,25 @@ pl_status pl_region_fix(pl_region* r, const char* new_name, int64_t offset, char
 void pl_region_close(pl_region* r) { delete r; }
 void pl_free(void* p) { free(p); }
 
-pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, const pl_batch_opts* opts, int32_t* pile,
-                       pl_call* calls, pl_region_stats* stats) {
-  if (!c || !regions || n <= 0) return fail(PL_E_INVAL, "empty batch");
-  Lane* L = &c->main;
-  HIPCHK(hipSetDevice(c->device));
-  std::vector<JobReq> q((size_t)n);
+}  // extern "C"
+
+namespace {
+
+pl_status batch_requests(pl_ctx* c, const pl_region_desc* regions, int32_t n, std::vector<JobReq>* q) {
+  q->resize((size_t)n);
   for (int32_t i = 0; i < n; i++) {
     const pl_region_desc& d = regions[i];
     if (d.readset_id < 0 || d.readset_id >= (int32_t)c->rsets.size() || !c->rsets[d.readset_id] ||
         c->rsets[d.readset_id]->contig != d.contig_id)
       return fail(PL_E_INVAL, "region %d: read set %d is not a set of contig %d", i, d.readset_id, d.contig_id);
-    q[i] = {d.readset_id, d.start, d.stop};
+    (*q)[i] = {d.readset_id, d.start, d.stop};
   }
-  int32_t W = opts ? opts->window : 0;
-  int32_t block = opts ? opts->block : 0;
-  const int persistent = opts ? opts->persistent : 0;
-  const bool use_graph = opts && opts->use_graph;
-  pl_status s;
-  if (W == 0 || block == 0) {
-    if ((s = autotune(c, L, q, persistent, &W, &block))) return s;
-  }
-  if ((s = run_jobs(c, L, q, W, persistent, block, use_graph))) return s;
-  s = fetch_batch_meta(L);
+  return PL_OK;
+}
+
+// stats and host copies of the lane's last batch (pl_batch_run's outputs)
+pl_status batch_outputs(pl_ctx* c, Lane* L, int32_t n, int32_t* pile, pl_call* calls, pl_region_stats* stats) {
+  pl_status s = fetch_batch_meta(L);
   if (s) return s;
   for (int32_t i = 0; i < n; i++) if ((s = job_status(L, i))) return s;
   if (stats)
@@ -1099,6 +1105,29 @@ pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, cons
   return PL_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, const pl_batch_opts* opts, int32_t* pile,
+                       pl_call* calls, pl_region_stats* stats) {
+  if (!c || !regions || n <= 0) return fail(PL_E_INVAL, "empty batch");
+  Lane* L = &c->main;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<JobReq> q;
+  pl_status s = batch_requests(c, regions, n, &q);
+  if (s) return s;
+  int32_t W = opts ? opts->window : 0;
+  int32_t block = opts ? opts->block : 0;
+  const int persistent = opts ? opts->persistent : 0;
+  const bool use_graph = opts && opts->use_graph;
+  if (W == 0 || block == 0) {
+    if ((s = autotune(c, L, q, persistent, &W, &block))) return s;
+  }
+  if ((s = run_jobs(c, L, q, W, persistent, block, use_graph))) return s;
+  return batch_outputs(c, L, n, pile, calls, stats);
+}
+
 pl_status pl_last_batch_info(pl_ctx* c, pl_batch_info* o) {
   Lane* L = c ? &c->main : nullptr;
   if (!c || !o) return fail(PL_E_INVAL, "null argument");
@@ -1115,6 +1144,25 @@ pl_status pl_last_batch_info(pl_ctx* c, pl_batch_info* o) {
   o->use_graph = L->last_graph ? 1 : 0;
   o->pipeline_ms = L->last_ms;
   o->window_ms = L->last_win_ms;
+  return PL_OK;
+}
+
+pl_status pl_last_wide_columns(pl_ctx* c, pl_wide_col* out, int64_t cap, int64_t* n) {
+  Lane* L = c ? &c->main : nullptr;
+  if (!c || !n) return fail(PL_E_INVAL, "null argument");
+  if (!L->tables_valid) return fail(PL_E_INVAL, "no batch on the device");
+  const uint32_t nw = std::min(L->last_counts[6], L->wide_cap);
+  *n = nw;
+  if (!out || !nw) return PL_OK;
+  if (cap < (int64_t)nw) return fail(PL_E_INVAL, "%u wide columns, room for %lld", nw, (long long)cap);
+  std::vector<WideCol> h(nw);
+  HIPCHK(hipMemcpyAsync(h.data(), L->wide.p, sizeof(WideCol) * nw, hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  std::sort(h.begin(), h.end(), [](const WideCol& a, const WideCol& b) { return a.gpos < b.gpos; });
+  for (uint32_t k = 0; k < nw; k++) {
+    out[k].pos = h[k].gpos;
+    for (int b = 0; b < 4; b++) out[k].qual_sum[b] = h[k].qs[b];
+  }
   return PL_OK;
 }
 

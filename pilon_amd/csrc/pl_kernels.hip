@@ -608,8 +608,8 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
           unsigned long long* mqp = mqq + p;
           // The base's counter array is b*W entries on: a 24-bit multiply-add
           // (b < 4, 8*W < 2^24).  With minQual >= 0 every qual that passes is
-          // non-negative, so qual*mq1 >= 0 and the count word is the constant
-          // 1 (pack_base without its sign extension).
+          // non-negative, so qual*mq1 >= 0 and the count part is the constant
+          // 1 << PL_BASE_SHIFT (pack_base without its sign extension).
           auto pile = [&](auto nonneg) {
 #pragma unroll
             for (int t = 0; t < PL_CHUNK; t++) {
@@ -618,7 +618,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
               if (t < cnt && qual >= mq) {
                 unsigned long long* cell = (unsigned long long*)((char*)bsp + __umul24(b, W8)) + t;
                 const unsigned long long v = decltype(nonneg)::value
-                                                 ? ((1ull << 32) | (uint32_t)(qual * mq1))
+                                                 ? ((1ull << PL_BASE_SHIFT) | (uint32_t)(qual * mq1))
                                                  : (unsigned long long)pack_base(qual, mq1);
                 atomicAdd(cell, v);
                 atomicAdd(mqp + t, (unsigned long long)pack_mq_q(mq1, qual));
@@ -704,13 +704,15 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
       }
     }
     const int64_t G = A.G;
-    for (int p = tid; p < nW; p += BLOCK) {
+    uint32_t wmask = 0;   // bit i: position tid + i*BLOCK is wide
+    for (int p = tid, it = 0; p < nW; p += BLOCK, it++) {
       const uint64_t b4[4] = {bs[p], bs[W + p], bs[2 * W + p], bs[3 * W + p]};
       PosOut o;
       finalize_position(b4, mqq[p], insA[p], delA[p], badA[p], clpA[p], pcA[p], isA[p], R.ref[L0 + p - 1], cf, o);
       const int64_t g = J.out + win.p0 + p;
 #pragma unroll
       for (int f = 0; f < PL_NFIELDS; f++) A.pile[(int64_t)f * G + g] = o.f[f];
+      if (o.wide) { o.call.flags |= PL_CF_WIDE; wmask |= 1u << it; }
       A.calls[g] = o.call;
       if (o.overflow) s_ovf = 1;
       const int d = call_disp(o.call);
@@ -729,6 +731,38 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
         if (kc < A.cand_cap) { Cand c; c.gpos = (uint32_t)g; c.depth = (int32_t)dep; c.kind = d; c.len = 0; c.rec = -1; A.cand[kc] = c; }
         else *A.overflow = 1;
       }
+    }
+    // wide columns (a qualSum beyond int32): the window's entries, in one
+    // segment of the batch's list, with their exact sums from the LDS words
+    if (__syncthreads_or(wmask != 0)) {
+      const uint32_t mine = (uint32_t)__popc(wmask);
+      const uint32_t inc = wave_incl_scan(mine);
+      if (lane == 63) ws_a[wave] = (int32_t)inc;
+      __syncthreads();
+      uint32_t pre = inc - mine, tot = 0;
+      for (int w2 = 0; w2 < NW; w2++) { if (w2 < wave) pre += (uint32_t)ws_a[w2]; tot += (uint32_t)ws_a[w2]; }
+      if (tid == 0) {
+        s_recbase = atomicAdd(A.wide_n, tot);
+        A.win_wide[2 * wi] = (int32_t)s_recbase;
+        A.win_wide[2 * wi + 1] = (int32_t)tot;
+      }
+      __syncthreads();
+      for (uint32_t m = wmask; m; m &= m - 1) {
+        const int p = tid + (__ffs(m) - 1) * BLOCK;
+        const uint32_t k = s_recbase + pre++;
+        if (k < A.wide_cap) {
+          WideCol wc;
+          wc.gpos = (uint32_t)(J.out + win.p0 + p);
+          wc.pad = 0;
+          for (int b = 0; b < 4; b++) { uint32_t n; unpack_base(bs[b * W + p], &wc.qs[b], &n); }
+          A.wide[k] = wc;
+        } else {
+          *A.overflow = 1;
+        }
+      }
+    } else if (tid == 0) {
+      A.win_wide[2 * wi] = 0;
+      A.win_wide[2 * wi + 1] = 0;
     }
     // region baseCount and the aligned-base unit
     unsigned long long bc = wave_sum((unsigned long long)sk.bcount);
@@ -946,7 +980,9 @@ __global__ void __launch_bounds__(256) k_indel_pass(EngineArgs A, const uint32_t
     const int32_t j = wave_job_of(A.jobs, A.n_jobs, g, lane);
     const JobDev& J = A.jobs[j];
     const RSDev& R = A.rs[J.rs];
-    const Col c = load_col(A.pile, A.G, g);
+    Col c = load_col(A.pile, A.G, g);
+    const uint16_t wflag = A.calls[g].flags & PL_CF_WIDE;
+    if (wflag) wide_patch(c, A.wide, A.wide_cap, A.win_wide, J, A.W, g);
     const uint8_t refb = R.ref[J.start + (int64_t)(g - J.out) - 1];
     const BaseFacts bf = base_facts(c);
     int32_t pct = 0;
@@ -973,7 +1009,8 @@ __global__ void __launch_bounds__(256) k_indel_pass(EngineArgs A, const uint32_t
       }
     }
     if (lane != 0) continue;
-    const pl_call cl = make_call(bf, indel, homo, refb, A.cf);
+    pl_call cl = make_call(bf, indel, homo, refb, A.cf);
+    cl.flags |= wflag;
     if (mode == 0) {
       A.calls[g] = cl;
       const int d = call_disp(cl);

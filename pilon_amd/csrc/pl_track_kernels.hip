@@ -345,6 +345,11 @@ __global__ void __launch_bounds__(256) k_trk_values(TrackArgs T) {
     const int32_t mqSum = A.pile[(int64_t)PL_F_MQSUM * G + g];
     const int32_t qSum = A.pile[(int64_t)PL_F_QSUM * G + g];
     const uint16_t f = A.calls[g].flags;
+    if (f & PL_CF_WIDE) {          // the exact Long qualSums of a wide column
+      Col c;
+      wide_patch(c, A.wide, A.wide_cap, A.win_wide, J, A.W, g);
+      qsum = c.qs[0] + c.qs[1] + c.qs[2] + c.qs[3];
+    }
     const int64_t dep = cnt + A.pile[(int64_t)PL_F_DELETIONS * G + g];
     // the final disposition (as k_final_flags)
     int disp = (f & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT;

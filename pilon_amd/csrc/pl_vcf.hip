@@ -48,7 +48,7 @@ pl_status drain_to_host(Lane* L, const void* src, size_t bytes, Sink&& sink) {
   size_t pend_off[2] = {0, 0}, pend_len[2] = {0, 0};
   size_t off = 0;
   bool stop = false;
-  for (int k = 0; off < bytes || pend_len[0] || pend_len[1]; k ^= 1) {
+  for (int k = 0; (off < bytes && !stop) || pend_len[0] || pend_len[1]; k ^= 1) {
     if (pend_len[k]) {                                        // drain this buffer's previous chunk
       HIPCHK(hipEventSynchronize(L->stage_ev[k]));
       if (!stop && !sink(pend_off[k], (const char*)L->stage[k], pend_len[k])) stop = true;
@@ -150,11 +150,69 @@ pl_status pl_vcf_header(const char* const* names, const int64_t* lengths, int32_
 
 namespace {
 
+pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs, VcfBufs& B, TrackArgs& T,
+                         double* ms, bool issues);
+
+// GenomeRegion.duplicationEvents (GenomeRegion.scala:731-737) of one region
+// from its issue bits: summaryRegions of copyNumber > 1 with slop 2000
+// (:740-763, runs within 100 loci of an end dropped by nearEdge), longer
+// than 10,000 loci.  1-based (start, stop) pairs in order.
+std::vector<std::pair<int32_t, int32_t>> dup_runs(const uint8_t* m, int32_t start, int32_t stop) {
+  std::vector<std::pair<int32_t, int32_t>> out;
+  const int32_t n = stop + 1 - start;
+  int32_t first = -1, last = -1;
+  auto close = [&]() {
+    const int32_t a = start + first, b = start + last;
+    if (!(a - start < 100 || stop - b < 100) && b + 1 - a > 10000) out.emplace_back(a, b);
+  };
+  for (int32_t i = 0; i < n; i++) {
+    if (!(m[i] & PL_ISSUE_HIGHCN)) continue;
+    if (last >= 0 && i > last + 2000 + 1) { close(); first = -1; }
+    if (first < 0) first = i;
+    last = i;
+  }
+  if (last >= 0) close();
+  return out;
+}
+
+// A batch's VCF text: the records formatted on the device, and the lines
+// the host puts between them (Vcf.writeDup before a duplication event's
+// first locus) at their byte offsets in the device text.
+struct VcfText {
+  VcfBufs B;
+  const char* dtext = nullptr;
+  uint64_t total = 0;
+  std::vector<std::pair<uint64_t, std::string>> inserts;   // ascending offsets
+  double device_ms = 0;
+  uint64_t bytes() const {
+    uint64_t n = total;
+    for (const auto& x : inserts) n += x.second.size();
+    return n;
+  }
+};
+
+// The text in order to sink(data, n): device pieces through the lane's
+// staging buffers, the inserted lines between them.
+template <class Sink>
+pl_status vcf_emit(Lane* L, const VcfText& V, Sink&& sink) {
+  uint64_t at = 0;
+  auto piece = [&](uint64_t end) -> pl_status {
+    if (end <= at) return PL_OK;
+    pl_status s = drain_to_host(L, V.dtext + at, (size_t)(end - at),
+                                [&](size_t, const char* p, size_t n) { return sink(p, n); });
+    at = end;
+    return s;
+  };
+  for (const auto& x : V.inserts) {
+    if (pl_status s = piece(x.first)) return s;
+    if (!sink(x.second.data(), x.second.size())) return fail(PL_E_INVAL, "the caller's sink stopped the hand-off");
+  }
+  return piece(V.total);
+}
+
 // The VCF records of the lane's last batch; names[j] is region j's name.
-pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, uint32_t flags, char** text,
-                      int64_t* len, double* device_ms) {
-  *text = nullptr;
-  *len = 0;
+pl_status vcf_format(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, const std::vector<int32_t>& contigs,
+                     uint32_t flags, VcfText& out) {
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = L->stream;
   const int64_t G = L->G;
@@ -165,7 +223,7 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
     names += rnames[(size_t)j];
     noff.push_back((int64_t)names.size());
   }
-  VcfBufs B;
+  VcfBufs& B = out.B;
   VcfArgs V;
   std::memset(&V, 0, sizeof V);
   V.A = L->args;
@@ -221,18 +279,63 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   k_vcf_write<<<dim3((unsigned)L->n_wins), dim3(256), 0, st>>>(V);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev.e1, st));
-  char* out = (char*)malloc((size_t)total + 1);
-  if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)total);
-  if (pl_status cs = copy_to_host(L, out, dtext, (size_t)total)) {
-    free(out);
-    return cs;
-  }
-  out[total] = 0;
+  HIPCHK(hipEventSynchronize(ev.e1));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, ev.e0, ev.e1));
-  if (device_ms) *device_ms = ms;
+  out.dtext = dtext;
+  out.total = total;
+  out.device_ms = ms;
+  // duplication events from Pass 2's copy number (the tracks' issue bits):
+  // a writeDup line (Vcf.scala:194-202) before the records of the first
+  // locus of each, at that locus's offset in the device text
+  {
+    VcfBufs B2;
+    TrackArgs T;
+    double tms = 0;
+    if ((s = tracks_of_lane(c, L, contigs, B2, T, &tms, true))) return s;
+    std::vector<uint8_t> mask((size_t)G);
+    HIPCHK(hipMemcpyAsync(mask.data(), T.mask, (size_t)G, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    out.device_ms += tms;
+    for (int32_t j = 0; j < nj; j++) {
+      const JobDev& J = L->hjobs[(size_t)j];
+      for (const auto& d : dup_runs(mask.data() + J.out, J.start, J.stop)) {
+        uint64_t at = 0;
+        HIPCHK(hipMemcpyAsync(&at, off + J.out + (d.first - J.start), 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const Contig& ct = c->contigs.at(contigs[(size_t)j]);
+        const int32_t loc = d.first - 1;
+        std::string line = rnames[(size_t)j] + "\t" + std::to_string(loc) + "\t.\t";
+        line += (char)to_upper(ct.host[(size_t)loc - 1]);
+        line += "\t<DUP>\t.\tPASS\tSVTYPE=DUP;SVLEN=" + std::to_string(d.second + 1 - d.first) + ";END=" +
+                std::to_string(d.second) + ";IMPRECISE\tGT\t./.\n";
+        out.inserts.emplace_back(at, std::move(line));
+      }
+    }
+  }
+  return PL_OK;
+}
+
+pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, const std::vector<int32_t>& contigs,
+                      uint32_t flags, char** text, int64_t* len, double* device_ms) {
+  *text = nullptr;
+  *len = 0;
+  VcfText V;
+  if (pl_status s = vcf_format(c, L, rnames, contigs, flags, V)) return s;
+  const uint64_t n = V.bytes();
+  char* out = (char*)malloc((size_t)n + 1);
+  if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)n);
+  uint64_t w = 0;
+  pl_status s = vcf_emit(L, V, [&](const char* p, size_t k) {
+    std::memcpy(out + w, p, k);
+    w += k;
+    return true;
+  });
+  if (s) { free(out); return s; }
+  out[n] = 0;
+  if (device_ms) *device_ms = V.device_ms;
   *text = out;
-  *len = (int64_t)total;
+  *len = (int64_t)n;
   return PL_OK;
 }
 
@@ -280,15 +383,31 @@ extern "C" {
 pl_status pl_batch_vcf(pl_ctx* c, uint32_t flags, char** text, int64_t* len, double* device_ms) {
   if (!c || !text || !len) return fail(PL_E_INVAL, "null argument");
   std::vector<std::string> names;
-  if (pl_status s = batch_names(c, &c->main, &names)) return s;
-  return vcf_of_lane(c, &c->main, names, flags, text, len, device_ms);
+  std::vector<int32_t> contigs;
+  if (pl_status s = batch_names(c, &c->main, &names, &contigs)) return s;
+  return vcf_of_lane(c, &c->main, names, contigs, flags, text, len, device_ms);
+}
+
+pl_status pl_batch_vcf_stream(pl_ctx* c, uint32_t flags, pl_text_sink sink, void* user, int64_t* len,
+                              double* device_ms) {
+  if (!c || !sink) return fail(PL_E_INVAL, "null argument");
+  std::vector<std::string> names;
+  std::vector<int32_t> contigs;
+  if (pl_status s = batch_names(c, &c->main, &names, &contigs)) return s;
+  VcfText V;
+  if (pl_status s = vcf_format(c, &c->main, names, contigs, flags, V)) return s;
+  if (pl_status s = vcf_emit(&c->main, V, [&](const char* p, size_t n) { return sink(user, p, (int64_t)n) == 0; }))
+    return s;
+  if (len) *len = (int64_t)V.bytes();
+  if (device_ms) *device_ms = V.device_ms;
+  return PL_OK;
 }
 
 pl_status pl_region_vcf(pl_region* r, uint32_t flags, char** text, int64_t* len, double* device_ms) {
   if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
   return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
-                                const std::vector<int32_t>&) {
-    return vcf_of_lane(c, L, names, flags, text, len, device_ms);
+                                const std::vector<int32_t>& contigs) {
+    return vcf_of_lane(c, L, names, contigs, flags, text, len, device_ms);
   });
 }
 

@@ -235,8 +235,9 @@ __device__ __forceinline__ uint32_t vcf_locus(char* out, const VcfArgs& V, const
   VOut<WRITE> o;
   o.p = out;
   o.n = 0;
-  const VcfCol v = load_vcf_col(V.A.pile, V.A.G, g);
+  VcfCol v = load_vcf_col(V.A.pile, V.A.G, g);
   const uint16_t f = V.A.calls[g].flags;
+  if (f & PL_CF_WIDE) wide_patch(v.c, V.A.wide, V.A.wide_cap, V.A.win_wide, J, V.A.W, g);
   const bool deleted = (f & PL_CF_DELETED) != 0;
   int64_t rec = -1;
   if (i > 0 && (f & (PL_CF_INSERTION | PL_CF_DELETION))) rec = V.irec[V.imap[g]];

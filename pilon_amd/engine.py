@@ -17,7 +17,7 @@ import os
 import numpy as np
 
 from .abi import (PlConfig, PlReadSoa, PlCall, PlRegionStats, PlRegionDesc, PlBatchOpts, PlBatchInfo, PlServeOpts,
-                  PlServeResult, PlTarget, NFIELDS, F, STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK,
+                  PlServeResult, PlTarget, PlWideCol, NFIELDS, F, STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK,
                   PL_CF_DISP_SHIFT, SERVE_KEEP_PILEUP, SERVE_KEEP_FIX, CHUNK_SIZE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -121,6 +121,10 @@ def parse_targets(targets, contigs):
         lib().pl_free(p)
 
 
+# pl_text_sink: int32_t (*)(void* user, const char* data, int64_t n)
+TEXT_SINK = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.POINTER(C.c_char), C.c_int64)
+
+
 class PilonError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -182,6 +186,9 @@ def lib():
                                            P(C.c_int32)]),
             "pl_serve_fetch_fix": (C.c_int, [C.c_void_p, C.c_int64, C.c_char_p, C.c_int64, P(C.c_void_p),
                                              P(C.c_int64), P(C.c_void_p), P(C.c_int64)]),
+            "pl_last_wide_columns": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_int64)]),
+            "pl_batch_vcf_stream": (C.c_int, [C.c_void_p, C.c_uint32, TEXT_SINK, C.c_void_p, P(C.c_int64),
+                                              P(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -286,6 +293,19 @@ class Context:
         fasta = _take_text(f.value, fl.value) if want_fasta else None
         return changes, fasta
 
+    def batch_vcf_stream(self, write, diploid=False, vcfqe=False):
+        """batch_vcf's text handed to write(bytes) a piece at a time (a
+        false return stops it); returns (bytes handed over, device_ms)."""
+        flags = (VCF_DIPLOID if diploid else 0) | (VCF_QE if vcfqe else 0)
+
+        def sink(_user, data, n):
+            return 0 if write(C.string_at(data, n)) is not False else 1
+
+        cb = TEXT_SINK(sink)
+        n, ms = C.c_int64(), C.c_double()
+        _check(lib().pl_batch_vcf_stream(self.h, flags, cb, None, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
     def batch_tracks(self, copy=True):
         """Tracks.standardTracks values of the last batch: ([NTRACKS, G] int32
         in batch order, or None when copy is False; device_ms)."""
@@ -328,6 +348,17 @@ class Context:
             for p in (k, nx, al):
                 if p.value:
                     lib().pl_free(p)
+
+    def wide_columns(self):
+        """Columns of the last batch whose Long qualSum leaves int32:
+        {batch position: (qualSum A, C, G, T)} (BaseSum.scala:24)."""
+        n = C.c_int64()
+        _check(lib().pl_last_wide_columns(self.h, None, 0, C.byref(n)))
+        if not n.value:
+            return {}
+        buf = (PlWideCol * n.value)()
+        _check(lib().pl_last_wide_columns(self.h, buf, n.value, C.byref(n)))
+        return {int(buf[k].pos): tuple(int(x) for x in buf[k].qual_sum) for k in range(n.value)}
 
     def batch_info(self):
         info = PlBatchInfo()

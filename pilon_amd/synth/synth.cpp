@@ -17,6 +17,9 @@
 #include <numeric>
 #include <vector>
 #include <omp.h>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <zlib.h>
 #include <cstdio>
 #include <string>
@@ -190,12 +193,18 @@ bool make_read(const ContigModel& M, Rng& r, int64_t t0, int32_t L, bool reverse
   const synth_params& P = *M.P;
   const int64_t T = (int64_t)M.truth.size();
   // events: read base + draft index (-1 inserted, -2 soft clip)
-  static thread_local std::vector<uint8_t> eb;
-  static thread_local std::vector<int64_t> ed;
+  static thread_local std::vector<uint8_t> tl_eb;
+  static thread_local std::vector<int64_t> tl_ed;
+  std::vector<uint8_t>& eb = tl_eb;    // one thread-local lookup per read, not per base
+  std::vector<int64_t>& ed = tl_ed;
   eb.clear(); ed.clear();
   for (int i = 0; i < clip_lead; i++) { eb.push_back(rand_base(r, 0.5)); ed.push_back(-2); }
   int64_t t = t0;
   int32_t body = L - clip_lead - clip_trail;
+  // heterozygous sites at or after the read's draft positions, found once:
+  // draft indices only grow along the read
+  size_t hc = 0;
+  bool hc_set = false;
   while ((int32_t)eb.size() < clip_lead + body && t < T) {
     if (P.err_indel_rate > 0 && r.uni() < P.err_indel_rate) {
       int k = (int)r.range(1, P.err_indel_max);
@@ -209,9 +218,13 @@ bool make_read(const ContigModel& M, Rng& r, int64_t t0, int32_t L, bool reverse
     uint8_t b = M.truth[t];
     int64_t d = M.t2d[t];
     if (d >= 0 && !M.het_pos.empty()) {
-      auto it = std::lower_bound(M.het_pos.begin(), M.het_pos.end(), d);
-      if (it != M.het_pos.end() && *it == d) {
-        const Het& h = M.hets[it - M.het_pos.begin()];
+      if (!hc_set) {
+        hc = (size_t)(std::lower_bound(M.het_pos.begin(), M.het_pos.end(), d) - M.het_pos.begin());
+        hc_set = true;
+      }
+      while (hc < M.het_pos.size() && M.het_pos[hc] < d) hc++;
+      if (hc < M.het_pos.size() && M.het_pos[hc] == d) {
+        const Het& h = M.hets[hc];
         if (r.uni() < h.af) b = h.alt;
       }
     }
@@ -370,6 +383,10 @@ void gen_pairs(const ContigModel& M, int ci, int64_t chunk_id, int64_t npairs, i
   Rng r(mix(mix(P.seed, 0x2000 + (uint64_t)ci), (uint64_t)chunk_id));
   const int64_t T = (int64_t)M.truth.size();
   const int32_t L = P.read_len;
+  ch.reads.reserve((size_t)(2 * npairs));
+  ch.bases.reserve((size_t)(2 * npairs * (L + 4)));
+  ch.quals.reserve((size_t)(2 * npairs * (L + 4)));
+  ch.cigar.reserve((size_t)(2 * npairs * 3));
   for (int64_t k = 0; k < npairs; k++) {
     int64_t F = (int64_t)std::llround(P.insert_mean + P.insert_sd * r.normal());
     if (F < L) F = L;
@@ -427,8 +444,16 @@ void synth_default_params(synth_params* p) {
   p->threads = 0;
 }
 
+// PL_SYNTH_PROGRESS=1 in the environment: a line on stderr per phase and
+// per tenth of the pair jobs (large workloads take minutes)
+static bool synth_progress() {
+  const char* e = getenv("PL_SYNTH_PROGRESS");
+  return e && *e && *e != '0';
+}
+
 void* synth_generate(const synth_params* Pin) {
   synth_params P = *Pin;
+  const bool prog = synth_progress();
   SynthResult* res = new SynthResult();
   res->contigs.resize((size_t)P.n_contigs);
   int nthreads = P.threads > 0 ? P.threads : omp_get_max_threads();
@@ -459,61 +484,103 @@ void* synth_generate(const synth_params* Pin) {
     }
   }
   std::vector<Chunk> chunks(jobs.size());
+  if (prog) fprintf(stderr, "synth: %d contigs built, %zu pair jobs\n", P.n_contigs, jobs.size());
+  std::atomic<size_t> jobs_done{0};
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-  for (size_t j = 0; j < jobs.size(); j++)
+  for (size_t j = 0; j < jobs.size(); j++) {
     gen_pairs(models[jobs[j].ci], jobs[j].ci, jobs[j].chunk, jobs[j].npairs, jobs[j].lo, jobs[j].hi,
               P.coverage, chunks[j]);
-  // per contig: sort by pos (stable in generation order) and pack
+    const size_t k = ++jobs_done;
+    if (prog && (k * 10 / jobs.size() != (k - 1) * 10 / jobs.size()))
+      fprintf(stderr, "synth: %zu of %zu pair jobs\n", k, jobs.size());
+  }
+  // per contig: sort by pos (stable in generation order) and pack; a batch
+  // of many contigs packs one contig per thread, a few large contigs pack
+  // each one with all threads
+  std::vector<std::vector<size_t>> jobs_of((size_t)P.n_contigs);
+  for (size_t j = 0; j < jobs.size(); j++) jobs_of[(size_t)jobs[j].ci].push_back(j);
+  struct Key { int32_t pos; uint32_t chunk; uint32_t read; };
+  std::vector<std::vector<Key>> order((size_t)P.n_contigs);
+  // reads of each contig in generation order (chunk, then read), sorted
+  // stably by position: one contig per thread
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
   for (int ci = 0; ci < P.n_contigs; ci++) {
+    std::vector<Key>& idx = order[(size_t)ci];
+    size_t total = 0;
+    for (size_t j : jobs_of[(size_t)ci]) total += chunks[j].reads.size();
+    idx.reserve(total);
+    for (size_t j : jobs_of[(size_t)ci])
+      for (size_t k = 0; k < chunks[j].reads.size(); k++) idx.push_back({chunks[j].reads[k].pos, (uint32_t)j, (uint32_t)k});
+    std::stable_sort(idx.begin(), idx.end(), [](const Key& a, const Key& b) { return a.pos < b.pos; });
+  }
+  auto pack = [&](int ci, int inner) {
     ContigOut& out = res->contigs[ci];
     out.draft = models[ci].draft;
     out.truth = models[ci].truth;
-    std::vector<std::pair<size_t, size_t>> idx;  // (chunk, read)
-    for (size_t j = 0; j < jobs.size(); j++)
-      if (jobs[j].ci == ci)
-        for (size_t k = 0; k < chunks[j].reads.size(); k++) idx.push_back({j, k});
-    std::stable_sort(idx.begin(), idx.end(), [&](const std::pair<size_t, size_t>& a, const std::pair<size_t, size_t>& b) {
-      return chunks[a.first].reads[a.second].pos < chunks[b.first].reads[b.second].pos;
-    });
+    std::vector<Key> idx;
+    idx.swap(order[(size_t)ci]);
     size_t n = idx.size();
     out.pos.resize(n); out.tlen.resize(n); out.flag.resize(n); out.mapq.resize(n);
     out.cigar_off.resize(n + 1); out.seq_off.resize(n + 1);
     out.cigar_off[0] = 0; out.seq_off[0] = 0;
     for (size_t i = 0; i < n; i++) {
-      const RawRead& rr = chunks[idx[i].first].reads[idx[i].second];
+      const RawRead& rr = chunks[idx[i].chunk].reads[idx[i].read];
       out.cigar_off[i + 1] = out.cigar_off[i] + rr.ncig;
       out.seq_off[i + 1] = out.seq_off[i] + (int64_t)rr.nseq;
     }
     out.cigar.resize((size_t)out.cigar_off[n]);
     out.quals.resize((size_t)out.seq_off[n] + 8, 0);
     out.bases2.assign((size_t)(out.seq_off[n] + 3) / 4 + 8, 0);
+    uint8_t* b2 = out.bases2.data();
+#pragma omp parallel for schedule(static) num_threads(inner) if (inner > 1)
     for (size_t i = 0; i < n; i++) {
-      const Chunk& ch = chunks[idx[i].first];
-      const RawRead& rr = ch.reads[idx[i].second];
+      const Chunk& ch = chunks[idx[i].chunk];
+      const RawRead& rr = ch.reads[idx[i].read];
       out.pos[i] = rr.pos; out.tlen[i] = rr.tlen; out.mapq[i] = rr.mapq;
       out.flag[i] = rr.flag | (rr.exotic ? PL_RF_EXOTIC : 0);
       std::memcpy(&out.cigar[(size_t)out.cigar_off[i]], &ch.cigar[rr.cig0], sizeof(uint32_t) * rr.ncig);
-      int64_t so = out.seq_off[i];
+      const int64_t so = out.seq_off[i], se = so + (int64_t)rr.nseq;
       std::memcpy(&out.quals[(size_t)so], &ch.quals[rr.seq0], rr.nseq);
-      if (rr.exotic) {
-        out.exo_read.push_back((int64_t)i);
-        out.exo_off.push_back((int64_t)out.nib.size() * 2);
-        // nibbles start on a byte boundary for each exotic read
-        std::vector<uint8_t> codes(rr.nseq);
-        for (uint64_t k = 0; k < rr.nseq; k++) codes[k] = (uint8_t)nib_code(ch.bases[rr.seq0 + k]);
-        for (uint64_t k = 0; k < rr.nseq; k += 2) {
-          uint8_t hi = codes[k], lo = k + 1 < rr.nseq ? codes[k + 1] : 0;
-          out.nib.push_back((uint8_t)((hi << 4) | lo));
-        }
-      } else {
-        for (uint64_t k = 0; k < rr.nseq; k++) {
-          int64_t q = so + (int64_t)k;
-          out.bases2[(size_t)(q >> 2)] |= (uint8_t)(code2(ch.bases[rr.seq0 + k]) << (2 * (q & 3)));
+      if (rr.exotic) continue;
+      // 2-bit codes, four to a byte; bytes shared with a neighbouring read are ORed atomically
+      uint8_t acc = 0;
+      for (int64_t q = so; q < se; q++) {
+        acc |= (uint8_t)(code2(ch.bases[rr.seq0 + (uint64_t)(q - so)]) << (2 * (q & 3)));
+        if ((q & 3) == 3 || q == se - 1) {
+          const int64_t byte = q >> 2;
+          if (byte * 4 >= so && byte * 4 + 3 < se) b2[byte] = acc;
+          else __atomic_fetch_or(&b2[byte], acc, __ATOMIC_RELAXED);
+          acc = 0;
         }
       }
     }
+    // reads with non-ACGT codes: nibbles, each read from a byte boundary
+    for (size_t i = 0; i < n; i++) {
+      const Chunk& ch = chunks[idx[i].chunk];
+      const RawRead& rr = ch.reads[idx[i].read];
+      if (!rr.exotic) continue;
+      out.exo_read.push_back((int64_t)i);
+      out.exo_off.push_back((int64_t)out.nib.size() * 2);
+      std::vector<uint8_t> codes(rr.nseq);
+      for (uint64_t k = 0; k < rr.nseq; k++) codes[k] = (uint8_t)nib_code(ch.bases[rr.seq0 + k]);
+      for (uint64_t k = 0; k < rr.nseq; k += 2) {
+        uint8_t hi = codes[k], lo = k + 1 < rr.nseq ? codes[k + 1] : 0;
+        out.nib.push_back((uint8_t)((hi << 4) | lo));
+      }
+    }
     if (out.nib.empty()) out.nib.push_back(0);
+    for (size_t j : jobs_of[(size_t)ci]) { Chunk().reads.swap(chunks[j].reads); std::vector<uint32_t>().swap(chunks[j].cigar);
+      std::vector<uint8_t>().swap(chunks[j].bases); std::vector<int8_t>().swap(chunks[j].quals); }
+  };
+  if (prog) fprintf(stderr, "synth: reads sorted, packing\n");
+  if (P.n_contigs >= 4 * nthreads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int ci = 0; ci < P.n_contigs; ci++) pack(ci, 1);
+  } else {
+    for (int ci = 0; ci < P.n_contigs; ci++) {
+      pack(ci, nthreads);
+      if (prog) fprintf(stderr, "synth: contig %d of %d packed\n", ci + 1, P.n_contigs);
+    }
   }
   return res;
 }

@@ -128,3 +128,75 @@ def same_batch(a, b, what=""):
         na = (a.nib[(oa + k) >> 1] >> (4 * (1 - ((oa + k) & 1)))) & 15
         nb_ = (b.nib[(ob + k) >> 1] >> (4 * (1 - ((ob + k) & 1)))) & 15
         assert np.array_equal(na, nb_), (what, "nib", int(r))
+
+
+def bgzf_offsets(data_len, block, level_blocks):
+    """Compressed offset of each BGZF block of a stream cut every `block`
+    bytes (the end-of-file block last), from the blocks' sizes."""
+    coff, at = [], 0
+    for b in level_blocks:
+        coff.append(at)
+        at += len(b)
+    return coff
+
+
+def write_bam_indexed(path, refs, batches, block=0xFF00, level=1, index_path=None, meta_bin=True):
+    """write_bam and, beside it, the BAI index built from SAMv1 section 5.2:
+    per reference the chunks [virtual start, virtual end) of consecutive
+    records of each bin (virtual offset = compressed block offset << 16 |
+    offset in the inflated block), the linear index of 16 kb windows (the
+    smallest virtual offset of a record overlapping the window, 0 for a
+    window without one), and the 37450 pseudo-bin of counts."""
+    data = bytearray(header(refs))
+    recs = []                                   # (ref, pos0, end0, u_start, u_end, mapped)
+    for ref in sorted(batches):
+        for rec in batch_records(batches[ref], ref):
+            pos0 = struct.unpack_from("<i", rec, 8)[0]
+            ncig = struct.unpack_from("<H", rec, 16)[0]
+            flag = struct.unpack_from("<H", rec, 18)[0]
+            l_name = rec[12]
+            cig = struct.unpack_from(f"<{ncig}I", rec, 36 + l_name)
+            rlen = sum(w >> 4 for w in cig if (w & 15) in (0, 2, 3, 7, 8))
+            recs.append((ref, pos0, pos0 + max(rlen, 1), len(data), len(data) + len(rec), not flag & 4))
+            data += rec
+    blocks = [bgzf_block(bytes(data[i:i + block]), level) for i in range(0, len(data), block)]
+    blocks.append(bgzf_block(b""))
+    coff = bgzf_offsets(len(data), block, blocks)
+
+    def voff(u):
+        return (coff[u // block] << 16) | (u % block)
+
+    with open(path, "wb") as f:
+        for b in blocks:
+            f.write(b)
+    out = bytearray(b"BAI\x01") + struct.pack("<i", len(refs))
+    for ref in range(len(refs)):
+        mine = [r for r in recs if r[0] == ref]
+        bins = {}
+        lin = {}
+        for _, p0, e0, u0, u1, _m in mine:
+            b = reg2bin(p0, e0)
+            ch = bins.setdefault(b, [])
+            if ch and ch[-1][1] == voff(u0):
+                ch[-1][1] = voff(u1)
+            else:
+                ch.append([voff(u0), voff(u1)])
+            for w in range(p0 >> 14, ((e0 - 1) >> 14) + 1):
+                lin.setdefault(w, voff(u0))
+        items = sorted(bins.items())
+        if meta_bin and mine:
+            nm = sum(1 for r in mine if r[5])
+            items.append((37450, [[voff(mine[0][3]), voff(mine[-1][4])], [nm, len(mine) - nm]]))
+        out += struct.pack("<i", len(items))
+        for b, ch in items:
+            out += struct.pack("<Ii", b, len(ch))
+            for a, e in ch:
+                out += struct.pack("<QQ", a, e)
+        n_intv = max(lin) + 1 if lin else 0
+        out += struct.pack("<i", n_intv)
+        for w in range(n_intv):
+            out += struct.pack("<Q", lin.get(w, 0))
+    out += struct.pack("<Q", 0)                  # n_no_coor
+    with open(index_path or path + ".bai", "wb") as f:
+        f.write(bytes(out))
+    return len(data)

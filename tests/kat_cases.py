@@ -251,10 +251,51 @@ def case_deep_region():
                 disp={11: 1, 26: 1, 27: 0})
 
 
+def case_deep_70k():
+    """BaseSum.scala:24-30, PileUp.scala:75-84: 70,000 reads 36M at 1 (q40,
+    MAPQ 60) pile 70,000 bases at each of loci 11..26 -- past 65,536, with
+    qualSum 70,000 * 40 * 61 = 170,800,000 (still an Int).  baseCount
+    1,120,000 / 36 -> coverage 31,111, minDepth round(3111.1) = 3111; the 16
+    loci are confirmed."""
+    reads = [read(1, "36M", REF_A, [40] * 36) for _ in range(70000)]
+    return dict(ref=REF_A, start=1, stop=36, reads=reads,
+                expect={11: {"count_C": 70000, "qual_C": 170800000, "mqSum": 4270000, "qSum": 2800000}},
+                stats={"min_depth": 3111, "coverage": 31111, "n_confirmed": 16, "base_count": 1120000},
+                disp={11: 1, 26: 1})
+
+
+WIDE_QUAL = 2_304_000_000   # 90,000 * 100 * 256
+
+
+def case_wide_qualsum():
+    """BaseSum.scala:24 (Long sums), PileUp.scala:132-167: 90,000 reads 36M
+    at 1 with q100 and MAPQ 255 (mq1 256) give qualSum 90,000 * 100 * 256 =
+    2,304,000,000 at loci 11..26, beyond an Int: the pileup plane holds its
+    low 32 bits (-1,990,967,296) and the call is made from the Long:
+    homoScore 2.304e9, heteroScore 0, score = 2.304e9 * 90,000 / mqSum
+    (23,040,000) = 9,000,000, q = 100, confirmed.  mqSum 23,040,000 and
+    qSum 9,000,000 stay Ints."""
+    reads = [read(1, "36M", REF_A, [100] * 36, mapq=255) for _ in range(90000)]
+    low = WIDE_QUAL - 2 ** 32
+    return dict(ref=REF_A, start=1, stop=36, reads=reads,
+                expect={11: {"count_C": 90000, "qual_C": low, "mqSum": 23040000, "qSum": 9000000}},
+                stats={"n_confirmed": 16, "base_count": 1440000}, disp={11: 1, 26: 1}, wide=list(range(11, 27)))
+
+
 ALL = [case_trusted_flank, case_soft_clip, case_invalid_pair, case_deletion_slide, case_insertion_rotation,
        case_deletion_call, case_deletion_n_anchor, case_physcov_pairs, case_n_base, case_no_qual, case_ambiguous, case_ambiguous_iupac,
-       case_het_deletion, case_het_deletion_oldindel, case_min_depth_300, case_min_depth_311, case_deep_region]
+       case_het_deletion, case_het_deletion_oldindel, case_min_depth_300, case_min_depth_311, case_deep_region,
+       case_deep_70k, case_wide_qualsum]
+
+
+_batches = {}
 
 
 def batch_of(case):
-    return pack_reads(case["reads"])
+    """The case's reads packed (cached per case: the deep cases hold tens of
+    thousands of identical reads)."""
+    key = (case["ref"], case["start"], case["stop"], len(case["reads"]), repr(case["reads"][:2]),
+           repr(case["reads"][-1:]))
+    if key not in _batches:
+        _batches[key] = pack_reads(case["reads"])
+    return _batches[key]

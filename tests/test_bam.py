@@ -165,3 +165,153 @@ def test_native_writer_matches_the_spec_writer(tmp_path):
                      level=1)
     assert n1 == n2
     assert a.read_bytes() == b.read_bytes()
+
+
+# ------------------------------------------------------------ BAI and queries
+def overlap_filter(batch, start, stop):
+    """Reads of a batch whose span [pos, pos + reference length - 1] (one base
+    for unmapped or zero-length alignments) overlaps [start, stop] (1-based;
+    start <= 0 from the first base): htsjdk's queryOverlapping."""
+    keep = []
+    for i in range(batch.n):
+        cig = batch.cigar[int(batch.cigar_off[i]):int(batch.cigar_off[i + 1])]
+        rl = sum(int(x) >> 4 for x in cig if int(x) & 15 in (0, 2, 3, 7, 8))
+        ps = int(batch.pos[i])
+        pe = ps if (int(batch.flag[i]) & 4 or rl == 0) else ps + rl - 1
+        if ps <= max(stop, 1) and pe >= max(start, 1):
+            keep.append(i)
+    return keep
+
+
+def same_reads(q, whole, idx, what):
+    """q holds exactly reads idx of whole, in order."""
+    from pilon_amd.reads import ReadBatch
+    assert q.n == len(idx), (what, q.n, len(idx))
+    if not idx:
+        return
+    sub = whole.slice_rows(idx) if hasattr(whole, "slice_rows") else None
+    for i_q, i_w in enumerate(idx):
+        for k in ("pos", "tlen", "flag", "mapq"):
+            assert getattr(q, k)[i_q] == getattr(whole, k)[i_w], (what, k, i_q)
+        cq = q.cigar[int(q.cigar_off[i_q]):int(q.cigar_off[i_q + 1])]
+        cw = whole.cigar[int(whole.cigar_off[i_w]):int(whole.cigar_off[i_w + 1])]
+        assert np.array_equal(cq, cw), (what, "cigar", i_q)
+        sq0, sq1 = int(q.seq_off[i_q]), int(q.seq_off[i_q + 1])
+        sw0, sw1 = int(whole.seq_off[i_w]), int(whole.seq_off[i_w + 1])
+        assert sq1 - sq0 == sw1 - sw0
+        if not int(whole.flag[i_w]) & PL_RF_NO_QUAL:
+            assert np.array_equal(q.quals[sq0:sq1], whole.quals[sw0:sw1]), (what, "quals", i_q)
+        if not int(whole.flag[i_w]) & PL_RF_EXOTIC:
+            assert np.array_equal(unpack_2bit(q.bases2, sq0, sq1 - sq0), unpack_2bit(whole.bases2, sw0, sw1 - sw0))
+    assert len(q.exo_read) == sum(1 for i in idx if int(whole.flag[i]) & PL_RF_EXOTIC), what
+    del sub
+
+
+def spanning_batches():
+    """Two references; reads spread over 200 kb with a 20 kb deletion read
+    (bin far above its neighbours), unmapped-placed and zero-length reads,
+    non-ACGT codes, absent QUAL."""
+    from pilon_amd.reads import pack_reads
+    rng = np.random.default_rng(21)
+    reads = []
+    for k in range(1500):
+        p = int(rng.integers(1, 200_000))
+        seq = "".join(rng.choice(list("ACGT"), 60))
+        reads.append(dict(pos=p, cigar="60M", seq=seq, qual=[int(x) for x in rng.integers(2, 41, 60)], flag=0x1 | 0x2,
+                          mapq=60, tlen=300))
+    reads.append(dict(pos=1000, cigar="10M20000D10M", seq="A" * 20, qual=[30] * 20, flag=0, mapq=50, tlen=0))
+    reads.append(dict(pos=150_000, cigar="*", seq="ACGT", qual=None, flag=0x4, mapq=0, tlen=0))
+    reads.append(dict(pos=90_000, cigar="5S", seq="ACGTN", qual=[20] * 5, flag=0, mapq=0, tlen=0))
+    reads.append(dict(pos=120_000, cigar="30M", seq="ACGTNACGTRACGTACGTACGTACGTACGT", qual=None, flag=0, mapq=9,
+                      tlen=0))
+    b0 = pack_reads(reads)
+    b1 = pack_reads([dict(pos=int(p), cigar="40M", seq="C" * 40, qual=[25] * 40, flag=0, mapq=60, tlen=0)
+                     for p in sorted(rng.integers(1, 50_000, 300))])
+    return [("chrA", 250_000), ("chrB", 60_000)], {0: b0, 1: b1}
+
+
+@pytest.mark.parametrize("block", [0xFF00, 3000])
+def test_index_queries_match_whole_file(tmp_path, block):
+    """Every query through the spec-built BAI returns exactly the records of
+    the whole-file read that overlap the interval, in file order: intervals
+    inside one 16 kb window, across windows and bins, before all reads,
+    after all reads, start <= 0, one base, inside the 20 kb deletion only."""
+    from pilon_amd.bam import IndexedBamFile
+    refs, batches = spanning_batches()
+    p = tmp_path / "q.bam"
+    S.write_bam_indexed(str(p), refs, batches, block=block)
+    with BamFile(p) as whole:
+        wb = [whole.batch(0), whole.batch(1)]
+    rng = np.random.default_rng(3)
+    cases = [(0, 1, 100), (0, -5, 10), (0, 15_000, 16_000), (0, 1, 250_000), (0, 16_384, 16_384), (0, 245_000, 250_000),
+             (0, 150_000, 150_000), (0, 90_000, 90_000), (1, 1, 60_000), (1, 55_000, 60_000), (0, 1000, 1000)]
+    cases += [(int(r), int(a), int(a + rng.integers(0, 30_000))) for r, a in
+              zip(rng.integers(0, 2, 40), rng.integers(1, 240_000, 40))]
+    with IndexedBamFile(p) as ib:
+        assert ib.names == ["chrA", "chrB"] and ib.lens == [250_000, 60_000]
+        for ref, a, b in cases:
+            q = ib.query(ref, a, b)
+            same_reads(q, wb[ref], overlap_filter(wb[ref], a, b), (ref, a, b))
+    # the deletion read is found inside its deletion (its bin spans 16 kb windows)
+    with IndexedBamFile(p) as ib:
+        assert 1000 in ib.query(0, 15_000, 15_001).pos.tolist()
+
+
+def test_index_names_and_missing_index(tmp_path):
+    """htsjdk's index names (<bam>.bai, <name>.bai) and Pilon's refusal of a
+    BAM without an index (BamFile.scala:55-60)."""
+    from pilon_amd.bam import IndexedBamFile
+    refs, batches = spanning_batches()
+    p = tmp_path / "x.bam"
+    S.write_bam_indexed(str(p), refs, batches, index_path=str(tmp_path / "x.bai"))
+    with IndexedBamFile(p) as ib:
+        assert ib.query(1, 1, 60_000).n == batches[1].n
+    (tmp_path / "x.bai").rename(tmp_path / "elsewhere.bai")
+    with pytest.raises(BamError) as e:
+        IndexedBamFile(p)
+    assert e.value.status == 1 and "must be indexed" in str(e.value)
+    with IndexedBamFile(p, index=tmp_path / "elsewhere.bai") as ib:
+        assert ib.query(0, 1, 10).n >= 0
+    bad = tmp_path / "bad.bai"
+    bad.write_bytes(b"BAI\x01" + struct.pack("<i", 5))
+    with pytest.raises(BamError):
+        IndexedBamFile(p, index=bad)
+    with BamFile(p) as whole, pytest.raises(BamError):
+        lib_whole = whole
+        from pilon_amd.bam import lib
+        from pilon_amd.abi import PlReadSoa
+        import ctypes as C
+        s = PlReadSoa()
+        st = lib().pl_bam_query(lib_whole.h, 0, 1, 10, C.byref(s))
+        raise BamError(st, "query on a whole-file reader")
+
+
+def test_index_query_synthetic_c2(tmp_path):
+    """c2 x0.02 (clips, improper pairs, MAPQ-0 repeats, indels) through the
+    index: random region queries equal the whole-file read filtered by
+    overlap; a region's ±10 kb query gives the oracle the same region."""
+    from oracle import oracle as O
+    from pilon_amd.bam import IndexedBamFile
+    from pilon_amd.synth import SynthData, preset
+    d = SynthData(preset("c2", scale=0.02))
+    b = d.batch(0)
+    L = len(d.draft(0))
+    p = tmp_path / "c2.bam"
+    S.write_bam_indexed(str(p), [(d.names[0], L)], {0: b})
+    with BamFile(p) as whole:
+        wb = whole.batch(0)
+    rng = np.random.default_rng(8)
+    with IndexedBamFile(p) as ib:
+        for a in rng.integers(1, L, 12):
+            a = int(a)
+            e = min(L, a + int(rng.integers(100, 30_000)))
+            q = ib.query(0, a, e)
+            same_reads(q, wb, overlap_filter(wb, a, e), (a, e))
+        s, e = L // 3, L // 2
+        q = ib.query(0, max(s - 10000, 0), min(e + 10000, L))
+    cfg = default_config()
+    r1 = O.run_region(cfg, d.draft(0), s, e, [q], name=d.names[0])
+    r2 = O.run_region(cfg, d.draft(0), s, e, [b], name=d.names[0])
+    assert r1["status"] == r2["status"] == 0
+    assert np.array_equal(r1["pile"], r2["pile"]) and r1["changes"] == r2["changes"]
+    assert r1["stats"] == r2["stats"]

@@ -1,6 +1,7 @@
 """BAM -> engine: the read batches the BAM reader decodes go through the GPU
 path (pl_readset_upload and pl_region_add_reads) and every region matches the
 oracle run on the generator's own batches."""
+import numpy as np
 import pytest
 
 from pilon_amd.abi import default_config
@@ -67,3 +68,40 @@ def test_bam_region_add_reads(engine, tmp_path):
     ref = O.run_region(engine.config, d.draft(0), 1, L, [d.batch(0)], name=d.names[0])
     assert changes == ref["changes"] and bases == ref["bases"]
     assert changes.count("\n") > 50
+
+
+def test_bam_region_through_index(engine, tmp_path):
+    """BamFile.process's queryOverlapping through the BAI (BamFile.scala:
+    118-119): each region piled from its index query of [start - 10 kb,
+    stop + 10 kb] equals the same region piled from the whole file, field
+    for field, and the oracle."""
+    from oracle import oracle as O
+    from pilon_amd.bam import BamFile, IndexedBamFile
+    from pilon_amd.engine import PileUpRegion
+    from pilon_amd.synth import SynthData, preset
+    from tests import bamspec as S
+    from tests.helpers import assert_same_region, assert_same_stats
+    print("case: c2 x0.05 regions through the BAI index", flush=True)
+    d = SynthData(preset("c2", scale=0.05))
+    draft, b = d.draft(0), d.batch(0)
+    L = len(draft)
+    path = tmp_path / "c2.bam"
+    S.write_bam_indexed(str(path), [(d.names[0], L)], {0: b})
+    engine.ref_upload(310, d.names[0], draft)
+    with BamFile(path) as whole, IndexedBamFile(path) as ib:
+        for s, e in [(1, 40_000), (100_001, 160_000), (L - 30_000, L)]:
+            q = ib.query(0, max(s - 10000, 0), min(e + 10000, L))
+            out = []
+            for reads in (q, whole.reads(0)):
+                pr = PileUpRegion(engine, 310, s, e)
+                pr.addReads(reads)
+                pr.postProcess()
+                out.append((pr.pile, pr.calls, pr.stats, pr.fix()))
+                pr.close()
+            (p1, c1, s1, f1), (p2, c2, s2, f2) = out
+            assert np.array_equal(p1, p2) and np.array_equal(c1.view(np.uint64), c2.view(np.uint64)), (s, e)
+            assert s1 == s2 and f1 == f2, (s, e)
+            ref = O.run_region(engine.config, draft, s, e, [b], name=d.names[0])
+            assert_same_region(p1, c1, ref, f"indexed {s}-{e}")
+            assert_same_stats(s1, ref["stats"])
+            assert q.n < b.n

@@ -118,10 +118,16 @@ def test_served_fix_outputs(engine):
         tickets = {srv.submit(320, s, e, rid, tag=k): k for k, (s, e) in enumerate(reqs)}
         srv.drain()
         res = srv.poll(max_n=100)
-        assert len(res) == len(reqs) and all(r["status"] == 0 for r in res)
+        assert len(res) == len(reqs)
+        n_ok = 0
         for r in res:
             k = tickets[r["ticket"]]
             s, e = reqs[k]
+            if r["status"] != 0:      # Pilon throws on this region: the oracle must too
+                assert r["status"] == 4, (k, r["status"])
+                assert O.run_region(engine.config, draft, s, e, [b], name=d.names[0])["status"] == 4, k
+                continue
+            n_ok += 1
             nn, off = (None, 0) if k % 2 == 0 else ("renamed", 1000 + k)
             ch, bases = srv.fetch_fix(r["ticket"], new_name=nn, offset=off)
             ref = O.run_region(engine.config, draft, s, e, [b], name=d.names[0],
@@ -130,5 +136,6 @@ def test_served_fix_outputs(engine):
             assert bases == ref["bases"], k
             pile, _ = srv.fetch(r["ticket"])
             assert np.array_equal(pile, ref["pile"]), k
+        assert n_ok >= 20
     finally:
         srv.close()

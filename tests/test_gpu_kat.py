@@ -34,6 +34,7 @@ def test_kat_gpu(engine, make):
         pr.addReads(K.batch_of(case))
         pr.postProcess()
         changes, bases = pr.fix()
+        wide = engine.wide_columns()
         pr.close()
     finally:
         if own is not None:
@@ -43,6 +44,11 @@ def test_kat_gpu(engine, make):
     ref = O.run_region(cfg, case["ref"], case["start"], case["stop"], [K.batch_of(case)], name="ctg")
     assert_same_region(pr.pile, pr.calls, ref, make.__name__)
     assert_same_stats(pr.stats, ref["stats"])
+    # exact Long qualSums of the columns beyond int32 (the planes hold their low bits)
+    want = {i: tuple(int(x) for x in ref["bcq"][4:8, i]) for i in range(ref["size"])
+            if any(not -2 ** 31 <= int(x) < 2 ** 31 for x in ref["bcq"][4:8, i])}
+    assert wide == want, make.__name__
+    assert (case.get("wide") is not None) == bool(want)
 
 
 def test_edge_gpu(engine):

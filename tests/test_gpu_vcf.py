@@ -129,3 +129,47 @@ def test_vcf_long_names_and_empty_regions(engine):
     ])
     assert got == ref, first_diff(got, ref)
     assert max(len(l) for l in got.splitlines()) > 400
+
+
+def test_vcf_duplication_record(engine):
+    """The two-BAM case whose second BAM doubles 15 kb (copy number 2 over
+    more than 10,000 loci): the device's VCF holds the oracle's writeDup
+    line before the records of the event's first locus, byte for byte."""
+    from oracle import oracle as O
+    from pilon_amd.engine import PileUpRegion
+    from tests.test_oracle_bed import dup_case
+    print("case: vcf duplication record, two BAMs", flush=True)
+    d, draft, sets = dup_case()
+    engine.ref_upload(902, d.names[0], draft)
+    pr = PileUpRegion(engine, 902, 1, len(draft))
+    for b in sets:
+        pr.addReads(b)
+    pr.postProcess()
+    got, _ = pr.vcf()
+    pr.close()
+    ref = O.run_region(engine.config, draft, 1, len(draft), sets, name=d.names[0], vcf=O.VCF_ON)["vcf"]
+    assert got == ref, first_diff(got, ref)
+    assert got.count("<DUP>") == 1
+
+
+def test_vcf_stream_equals_text(engine):
+    """pl_batch_vcf_stream hands over exactly pl_batch_vcf's text, in pieces
+    of at most the staging size, for a batch larger than one piece; a sink
+    that stops ends the call with PL_E_INVAL."""
+    from pilon_amd.engine import PilonError
+    from pilon_amd.synth import SynthData, preset
+    print("case: vcf stream vs text (c3 x0.05)", flush=True)
+    d = SynthData(preset("c3", scale=0.05))
+    regions = []
+    for ci in range(d.n):
+        engine.ref_upload(600 + ci, d.names[ci], d.draft(ci))
+        regions.append((600 + ci, 1, len(d.draft(ci)), engine.readset_upload(600 + ci, d.soa(ci))))
+    engine.batch_run(regions, window=0, block=0, persistent=1, copy_out=False, want_stats=False)
+    text, _ = engine.batch_vcf()
+    pieces = []
+    n, ms = engine.batch_vcf_stream(pieces.append)
+    assert n == len(text) and b"".join(pieces).decode() == text
+    assert len(pieces) > 2 and max(len(p) for p in pieces) <= 32 << 20 and ms > 0
+    with pytest.raises(PilonError) as e:
+        engine.batch_vcf_stream(lambda b: False)
+    assert e.value.status == 1

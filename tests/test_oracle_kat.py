@@ -4,7 +4,7 @@ and cannot run here), so these are what fixes the oracle's semantics."""
 import numpy as np
 import pytest
 
-from pilon_amd.abi import default_config, F, D_INS, D_DEL, PL_CF_DELETED, PL_CF_DISP_MASK, PL_CF_DISP_SHIFT
+from pilon_amd.abi import default_config, F, D_INS, D_DEL, PL_CF_DELETED, PL_CF_DISP_MASK, PL_CF_DISP_SHIFT, PL_CF_WIDE
 from pilon_amd.abi import PL_CF_HOMO, PL_CF_INDEL, PL_CF_HOMOINDEL, PL_CF_DELETION, PL_CF_INSERTION
 from tests import kat_cases as K
 
@@ -31,6 +31,9 @@ def check_case(res, case):
         assert res["calls"][locus - start]["flags"] & PL_CF_DELETED
     for locus, d in case.get("disp", {}).items():
         assert (int(res["calls"][locus - start]["flags"]) & PL_CF_DISP_MASK) >> PL_CF_DISP_SHIFT == d
+    wide = set(case.get("wide", []))
+    for i in range(len(res["calls"])):
+        assert bool(int(res["calls"][i]["flags"]) & PL_CF_WIDE) == (start + i in wide), start + i
 
 
 @pytest.mark.parametrize("make", K.ALL, ids=[f.__name__ for f in K.ALL])
@@ -196,3 +199,14 @@ def test_planted_variants_called():
     dels = [(int(p) + 2, int(n)) for p, k, n in zip(pos, kind, ln) if k == 2]
     assert res["stats"]["ins"] == len(ins)
     assert res["stats"]["dels"] >= len(dels) - 2
+
+
+def test_wide_qualsum_long_arithmetic():
+    """case_wide_qualsum: the exact Long sums (bcq) and the call made from
+    them -- q = 100 from score 9,000,000 over 90,000 bases; a qualSum
+    truncated to an Int would give a negative total and no call."""
+    case = K.case_wide_qualsum()
+    res = oracle().run_region(default_config(), case["ref"], 1, 36, [K.batch_of(case)])
+    assert int(res["bcq"][4 + 1, 10]) == K.WIDE_QUAL          # qualSum C at locus 11
+    assert int(res["calls"][10]["q"]) == 100 and int(res["calls"][10]["base"]) == 1
+    assert int(res["bcq"][4 + 1, 9]) == 0 and not (int(res["calls"][9]["flags"]) & PL_CF_WIDE)

@@ -106,3 +106,32 @@ def test_one_record_per_locus_plus_indels():
         L = vcf_lines(c)
         assert len(L) >= c["stop"] - c["start"] + 1
         assert all(l.count("\t") == 9 for l in L)
+
+
+def test_duplication_record():
+    """GenomeRegion.writeVcf (GenomeRegion.scala:623-631), Vcf.writeDup
+    (Vcf.scala:194-202): the two-BAM case whose second BAM doubles 15 kb has
+    one duplicationEvents region -- the bed's Duplication line, whose region
+    derivation is pinned in test_oracle_bed -- announced by one line before
+    the records of its first locus: refBase(start - 1), SVLEN = size, END =
+    stop, GT ./. .  Every other line is a locus record."""
+    from tests.test_oracle_bed import dup_case
+    O = oracle()
+    d, draft, sets = dup_case()
+    r = O.run_region(default_config(), draft, 1, len(draft), sets, name=d.names[0], vcf=O.VCF_ON | O.BED)
+    dups = [l.split("\t") for l in r["bed"].splitlines() if l.split("\t")[3] == "Duplication"]
+    assert len(dups) == 1
+    a, b = int(dups[0][1]) + 1, int(dups[0][2])        # 1-based start, stop of the event
+    assert b - a + 1 > 10000
+    L = r["vcf"].splitlines()
+    k = [i for i, l in enumerate(L) if "<DUP>" in l]
+    assert len(k) == 1
+    want = f"{d.names[0]}\t{a - 1}\t.\t{chr(draft[a - 2]).upper()}\t<DUP>\t.\tPASS\t" \
+           f"SVTYPE=DUP;SVLEN={b - a + 1};END={b};IMPRECISE\tGT\t./."
+    assert L[k[0]] == want
+    assert L[k[0] + 1].split("\t")[1] == str(a)        # then the records of locus a
+    assert L[k[0] - 1].split("\t")[1] == str(a - 1)
+    assert len(L) >= len(draft) + 1
+    # without the doubled reads there is none
+    r2 = O.run_region(default_config(), draft, 1, len(draft), sets[:1], name=d.names[0], vcf=O.VCF_ON)
+    assert "<DUP>" not in r2["vcf"]
