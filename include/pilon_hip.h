@@ -177,6 +177,15 @@ pl_status pl_region_open(pl_ctx* ctx, int32_t contig_id, int32_t start,
  * [max(start-10000,0), min(stop+10000,len)] and pass validateRead, and adds
  * each with addRead.  Callable repeatedly (one call per BAM or batch). */
 pl_status pl_region_add_reads(pl_region* reg, const pl_read_soa* reads);
+/* The same for one BAM of a given type (Pilon's --frags, --jumps,
+ * --unpaired): reads of every type are piled alike; the depth that "jumps"
+ * BAMs add is left out of fragment coverage, and so of copy number, the
+ * delta/dip tracks and their Pilon.bed and VCF regions (processBam,
+ * GenomeRegion.scala:287-300).  pl_region_add_reads is PL_BAM_FRAGS. */
+#define PL_BAM_FRAGS    0
+#define PL_BAM_JUMPS    1
+#define PL_BAM_UNPAIRED 2
+pl_status pl_region_add_reads_typed(pl_region* reg, const pl_read_soa* reads, int32_t bam_type);
 /* Same, for a resident read set. */
 pl_status pl_region_use_readset(pl_region* reg, int32_t readset_id);
 /* PileUpRegion.postProcess + GenomeRegion.postProcess Pass 1.  pileup

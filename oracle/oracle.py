@@ -51,6 +51,10 @@ def lib():
         L.orc_region_run.argtypes = [C.POINTER(PlConfig), C.c_char_p, C.c_int64, C.c_char_p,
                                      C.c_int32, C.c_int32, C.POINTER(C.POINTER(PlReadSoa)),
                                      C.c_int32, C.c_char_p, C.c_int64, C.c_int32]
+        L.orc_region_run_typed.restype = C.POINTER(OrcResult)
+        L.orc_region_run_typed.argtypes = [C.POINTER(PlConfig), C.c_char_p, C.c_int64, C.c_char_p,
+                                           C.c_int32, C.c_int32, C.POINTER(C.POINTER(PlReadSoa)),
+                                           C.c_void_p, C.c_int32, C.c_char_p, C.c_int64, C.c_int32]
         L.orc_result_free.argtypes = [C.POINTER(OrcResult)]
         L.orc_fmt_af.argtypes = [C.c_float, C.c_char_p]
         L.orc_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
@@ -89,11 +93,12 @@ TRACK_NAMES = ["Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverag
                "Dip Coverage", "Physical Coverage", "Clipped Alignments", "Weighted Qual", "Weighted MQ", "GC"]
 
 
-def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0, vcf=0):
+def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0, vcf=0, set_types=None):
     """One GenomeRegion through the oracle; returns a dict of numpy arrays.
     vcf: VCF_ON (| VCF_DIPLOID | VCF_QE) adds the region's VCF record lines
     (GenomeRegion.writeVcf) as out["vcf"]; TRACKS the 13 per-locus tracks as
-    out["tracks"]; BED the region's Pilon.bed lines as out["bed"]."""
+    out["tracks"]; BED the region's Pilon.bed lines as out["bed"].  set_types:
+    one per read set, 1 for a "jumps" BAM (left out of fragment coverage)."""
     L = lib()
     if isinstance(contig, str):
         contig = contig.encode()
@@ -102,8 +107,12 @@ def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, of
         new_name = name + "_pilon"
     soas = [rs.soa() for rs in readsets]
     arr = (C.POINTER(PlReadSoa) * max(1, len(soas)))(*[C.pointer(s) for s in soas])
-    rp = L.orc_region_run(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
-                          arr, len(soas), new_name.encode(), offset, vcf)
+    types = None
+    if set_types is not None:
+        types = (C.c_int32 * max(1, len(set_types)))(*set_types)
+    rp = L.orc_region_run_typed(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
+                                arr, C.cast(types, C.c_void_p) if types is not None else None, len(soas),
+                                new_name.encode(), offset, vcf)
     r = rp.contents
     out = {"status": r.status, "edge_read": r.edge_read, "edge_why": r.edge_why.decode()}
     if r.status == 0:

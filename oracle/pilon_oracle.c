@@ -927,22 +927,31 @@ static void bed_lines(const pl_config* cfg, const char* name, int32_t start, int
   free(t);
 }
 
-orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
-                           const char* name, int32_t start, int32_t stop,
-                           const pl_read_soa* const* sets, int32_t n_sets,
-                           const char* new_name, int64_t offset, int32_t vcf_flags) {
+/* set_types[si] (NULL: all fragment BAMs): 1 marks a "jumps" BAM, whose
+ * depth increments processBam leaves out of fragCoverage
+ * (GenomeRegion.scala:287-300). */
+orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
+                                 const char* name, int32_t start, int32_t stop,
+                                 const pl_read_soa* const* sets, const int32_t* set_types, int32_t n_sets,
+                                 const char* new_name, int64_t offset, int32_t vcf_flags) {
   orc_result* res = (orc_result*)calloc(1, sizeof(orc_result));
   if (start < 1 || stop < start || stop > contig_len) { res->status = PL_E_INVAL; return res; }  /* GenomeRegion.scala:34 */
   PRegion R; memset(&R, 0, sizeof R);
   R.start = start; R.stop = stop; R.size = stop + 1 - start;
   R.pu = (PileUp*)calloc((size_t)R.size, sizeof(PileUp));
   const int32_t size = R.size;
+  /* processBam's fragCoverage (GenomeRegion.scala:287-300): each non-jumps
+   * BAM's depth increment, before Pass 1 mutates any deletion count */
+  int32_t* fragCov = (int32_t*)calloc((size_t)size, 4);
+  int32_t* covBefore = (int32_t*)calloc((size_t)size, 4);
 
   /* BamFile.process (BamFile.scala:108-148): query ±10 kb, validateRead, addRead */
   int64_t qs = (int64_t)start - 10000 > 0 ? (int64_t)start - 10000 : 0;
   int64_t qe = (int64_t)stop + 10000 < contig_len ? (int64_t)stop + 10000 : contig_len;
   for (int32_t si = 0; si < n_sets && !R.edge; si++) {
     const pl_read_soa* s = sets[si];
+    const int jumps = set_types && set_types[si] == 1;
+    if (!jumps) for (int32_t i = 0; i < size; i++) covBefore[i] = (int32_t)pu_depth(&R.pu[i]);
     Decoder d = {s, NULL, NULL, 0};
     for (int64_t i = 0; i < s->n_reads; i++) {
       int32_t aEnd = alignment_end(s, i);
@@ -969,24 +978,23 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
       if (R.edge) break;
     }
     free(d.bases); free(d.quals);
+    if (!jumps)
+      for (int32_t i = 0; i < size; i++) fragCov[i] = jadd(fragCov[i], jsub((int32_t)pu_depth(&R.pu[i]), covBefore[i]));
     if (res->status) break;
   }
+  free(covBefore);
   if (R.edge) {
     res->status = PL_E_EDGE; res->edge_read = R.edgeRead;
     snprintf(res->edge_why, sizeof res->edge_why, "%s", R.edgeWhy);
   }
   if (res->status) {
     for (int32_t i = 0; i < size; i++) { free(R.pu[i].ins.v); free(R.pu[i].del.v); }
-    free(R.pu); arena_free(&R.ar);
+    free(R.pu); arena_free(&R.ar); free(fragCov);
     return res;
   }
 
-  /* processBam's fragCoverage (GenomeRegion.scala:287-300): each BAM's depth
-   * increment, before Pass 1 mutates any deletion count */
-  int32_t* fragCov = (int32_t*)calloc((size_t)size, 4);
   /* GenomeRegion.postProcess (GenomeRegion.scala:214-272) */
   pr_computePhysCov(&R);
-  for (int32_t i = 0; i < size; i++) fragCov[i] = (int32_t)pu_depth(&R.pu[i]);
   int64_t meanCoverage = roundDivL(R.baseCount, (int64_t)size);          /* PileUpRegion.coverage */
   int32_t minDepth;
   if (cfg->min_depth >= 1) minDepth = cfg->min_depth >= 2147483647.0 ? INT32_MAX : (int32_t)cfg->min_depth;  /* toInt saturates */
@@ -1272,6 +1280,14 @@ orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t 
   /* indel strings point into the arena: copied above, so it can go */
   arena_free(&R.ar);
   return res;
+}
+
+orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
+                           const char* name, int32_t start, int32_t stop,
+                           const pl_read_soa* const* sets, int32_t n_sets,
+                           const char* new_name, int64_t offset, int32_t vcf_flags) {
+  return orc_region_run_typed(cfg, contig, contig_len, name, start, stop, sets, NULL, n_sets, new_name, offset,
+                              vcf_flags);
 }
 
 /* Exposed for known-answer tests of the small helpers. */

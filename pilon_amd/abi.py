@@ -32,6 +32,9 @@ PL_CF_DISP_SHIFT = 8
 PL_CF_DISP_MASK = 0x0700
 PL_CF_DELETED = 0x0800
 PL_CF_WIDE = 0x1000
+# Pilon's BAM kinds (--frags, --jumps, --unpaired): pl_region_add_reads_typed
+BAM_TYPES = {"frags": 0, "jumps": 1, "unpaired": 2}
+
 D_NONE, D_CONFIRMED, D_SNP, D_INS, D_DEL, D_AMB = range(6)
 
 

@@ -98,6 +98,17 @@ struct WinDev {
   int64_t pad_;
 };
 
+// What the window kernel needs about one window and its region, in the
+// order the work queue hands windows out: one 80-byte load per window
+// instead of a chain of dependent loads (order -> window -> region).
+struct __attribute__((aligned(16))) WinHead {
+  int32_t wi, job, p0, p1;   // window index, region, region-relative [p0, p1)
+  int64_t lo, hi, plo;       // read ranges (WinDev)
+  int64_t out;               // the region's first batch position
+  int32_t rs, start, stop, qs, qe;   // read set and JobDev bounds
+  int32_t pad[3];
+};
+
 struct JobStat {             // device-written per job
   int64_t read_count;
   int64_t base_count;
@@ -150,6 +161,7 @@ struct EngineArgs {
   int32_t W;                 // window size (positions)
   int32_t* queue;            // persistent work queue head
   const int32_t* worder;     // window processing order
+  const WinHead* heads;      // [n_wins] window heads in processing order
   int32_t* pile;             // [PL_NFIELDS][G]
   pl_call* calls;            // [G]
   int64_t G;

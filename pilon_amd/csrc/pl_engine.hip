@@ -116,7 +116,7 @@ struct Lane {
   hipEvent_t ev[4] = {};
   DBuf rsarr, kparr, jobs, wins, worder, costk, costv, costtk, costtv, pile, calls, st, rec, rkey, rval, rtk, rtv,
       cand, cands, ckey, cval, ctk, ctv, defer, mutated, counters, hist, tiles, strpool, stridx, stroff, gath,
-      wide, winwide;
+      wide, winwide, heads;
   uint32_t rec_cap = 0, cand_cap = 0, defer_cap = 0, mut_cap = 0, wide_cap = 0;
   std::vector<JobDev> hjobs;
   std::vector<WinDev> hwins;
@@ -151,11 +151,15 @@ struct Lane {
   // destroyed, so a hand-off allocates no pinned memory
   void* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  // depth of the batch's "jumps" reads before Pass 1, per position, which
+  // fragment coverage leaves out (valid for the last batch when set)
+  DBuf jdepth;
+  bool has_jdepth = false;
 
   size_t device_bytes() const {
     const DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st,
                        &rec, &rkey, &rval, &rtk, &rtv, &cand, &cands, &ckey, &cval, &ctk, &ctv, &defer, &mutated,
-                       &counters, &hist, &tiles, &strpool, &stridx, &stroff, &gath, &wide, &winwide};
+                       &counters, &hist, &tiles, &strpool, &stridx, &stroff, &gath, &wide, &winwide, &heads};
     size_t t = 0;
     for (const DBuf* x : b) t += x->cap;
     return t;
@@ -163,7 +167,7 @@ struct Lane {
   void for_each_buffer(void (*f)(DBuf&)) {
     DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st, &rec,
                  &rkey, &rval, &rtk, &rtv, &cand, &cands, &ckey, &cval, &ctk, &ctv, &defer, &mutated, &counters,
-                 &hist, &tiles, &strpool, &stridx, &stroff, &gath, &wide, &winwide};
+                 &hist, &tiles, &strpool, &stridx, &stroff, &gath, &wide, &winwide, &heads, &jdepth};
     for (DBuf* x : b) f(*x);
   }
   void free_buffers() {
@@ -183,6 +187,7 @@ struct pl_ctx {
   std::vector<std::unique_ptr<ReadSet>> rsets;
   uint64_t rs_version = 1;         // bumped whenever rsets changes
   Lane main;                       // plain and batched forms
+  Lane aux;                        // a region's "jumps" reads piled alone
   std::map<uint64_t, std::pair<int32_t, int32_t>> tuned;   // autotuned (window, block) per batch shape
   int serving = 0;                 // servers open on this context
 };
@@ -193,6 +198,7 @@ struct pl_region {
   int32_t start, stop;
   int32_t readset = -1;          // resident set in use
   std::vector<HostReads> batches; // plain add_reads copies
+  std::vector<int32_t> types;      // PL_BAM_* of each batch
   bool finished = false;
   pl_region_stats stats;
   std::vector<plh::Change> changes;
@@ -200,7 +206,14 @@ struct pl_region {
   // table once finished, kept for pl_region_vcf until the region is closed
   std::unique_ptr<ReadSet> temp;
   int32_t temp_id = -1;
+  // the "jumps" reads alone (fragment coverage leaves them out), kept the same way
+  std::unique_ptr<ReadSet> jtemp;
+  int32_t jtemp_id = -1;
 };
+
+namespace {
+pl_status region_jump_depth(pl_region* r);   // pl_track.hip
+}
 
 namespace {
 
@@ -425,6 +438,8 @@ pl_status enqueue_batch(Lane* L, EngineArgs& A, int persistent, int block, int g
     if (s) return s;
     A.worder = (const int32_t*)sv;
   }
+  k_win_heads<<<dim3((n_wins + 255) / 256), dim3(256), 0, st>>>(A.jobs, A.wins, persistent ? A.worder : nullptr, n_wins,
+                                                                 (WinHead*)A.heads);
   k_jobprep<<<dim3(n_jobs), dim3(64), 0, st>>>(A.rs, (const uint32_t* const*)L->kparr.p, A.jobs, A.st);
   HIPCHK(hipGetLastError());
   if (timed) HIPCHK(hipEventRecord(L->ev[2], st));
@@ -475,6 +490,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
   if (window_lds_total(W, block) > 160 * 1024)
     return fail(PL_E_INVAL, "window %d with block %d needs more than the 160 KB of LDS of a CU", W, block);
   L->exported_flags = false;
+  L->has_jdepth = false;
   hipStream_t st = L->stream;
   // ---- jobs and windows (kept on the device while the batch repeats)
   const bool same = L->tables_valid && L->W == W && reqs == L->last_reqs;
@@ -537,6 +553,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
   HIPCHK(L->wins.ensure(sizeof(WinDev) * n_wins));
   HIPCHK(L->worder.ensure(4 * (size_t)n_wins));
   HIPCHK(L->winwide.ensure(8 * (size_t)n_wins));
+  HIPCHK(L->heads.ensure(sizeof(WinHead) * (size_t)n_wins));
   HIPCHK(L->costk.ensure(4 * (size_t)n_wins));
   HIPCHK(L->costv.ensure(4 * (size_t)n_wins));
   HIPCHK(L->costtk.ensure(4 * (size_t)n_wins));
@@ -594,6 +611,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
     A.W = W;
     A.queue = (int32_t*)(cnt + 5);
     A.worder = L->worder.as<int32_t>();
+    A.heads = L->heads.as<WinHead>();
     A.pile = L->pile.as<int32_t>();
     A.calls = L->calls.as<pl_call>();
     A.G = G;
@@ -935,7 +953,8 @@ pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out) {
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
   pl_status st = lane_init(&c->main);
-  if (st) { lane_destroy(&c->main); return st; }
+  if (!st) st = lane_init(&c->aux);
+  if (st) { lane_destroy(&c->main); lane_destroy(&c->aux); return st; }
   *out = c.release();
   return PL_OK;
 }
@@ -944,6 +963,7 @@ void pl_ctx_destroy(pl_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   lane_destroy(&c->main);
+  lane_destroy(&c->aux);
   c->rsets.clear();
   for (auto& kv : c->contigs) if (kv.second.dev) (void)hipFree(kv.second.dev);
   c->contigs.clear();
@@ -990,16 +1010,23 @@ pl_status pl_region_open(pl_ctx* c, int32_t contig_id, int32_t start, int32_t st
   return PL_OK;
 }
 
-pl_status pl_region_add_reads(pl_region* r, const pl_read_soa* reads) {
+pl_status pl_region_add_reads_typed(pl_region* r, const pl_read_soa* reads, int32_t bam_type) {
   if (!r || !reads) return fail(PL_E_INVAL, "null argument");
+  if (bam_type != PL_BAM_FRAGS && bam_type != PL_BAM_JUMPS && bam_type != PL_BAM_UNPAIRED)
+    return fail(PL_E_INVAL, "bam type %d is not PL_BAM_FRAGS, PL_BAM_JUMPS or PL_BAM_UNPAIRED", bam_type);
   if (r->finished) return fail(PL_E_INVAL, "region already finished");
   if (r->readset >= 0) return fail(PL_E_INVAL, "region already uses a resident read set");
   if (reads->n_reads == 0) return PL_OK;
   for (int64_t i = 1; i < reads->n_reads; i++)
     if (reads->pos[i] < reads->pos[i - 1]) return fail(PL_E_INVAL, "reads must be coordinate-sorted within a call");
   r->batches.emplace_back();   // one batch per BAM; merged by position at finish
+  r->types.push_back(bam_type);
   append_reads(r->batches.back(), reads);
   return PL_OK;
+}
+
+pl_status pl_region_add_reads(pl_region* r, const pl_read_soa* reads) {
+  return pl_region_add_reads_typed(r, reads, PL_BAM_FRAGS);
 }
 
 pl_status pl_region_use_readset(pl_region* r, int32_t id) {
@@ -1018,9 +1045,22 @@ pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_regio
   HIPCHK(hipSetDevice(c->device));
   int32_t id = r->readset;
   bool temp = false;
+  bool jumps = false;
   if (id < 0) {
     if (c->serving) return fail(PL_E_BUSY, "a server is open on this context: use a resident read set");
-    if (r->batches.empty()) r->batches.emplace_back();
+    if (r->batches.empty()) { r->batches.emplace_back(); r->types.push_back(PL_BAM_FRAGS); }
+    for (int32_t t : r->types) jumps = jumps || t == PL_BAM_JUMPS;
+    if (jumps) {
+      // the jumps BAMs alone, piled beside the region for fragment coverage
+      std::vector<HostReads> jb;
+      for (size_t k = 0; k < r->batches.size(); k++) if (r->types[k] == PL_BAM_JUMPS) jb.push_back(r->batches[k]);
+      int32_t jid = -1;
+      pl_status s = upload_host_batches(c, r->contig, jb, &jid);
+      if (s) return s;
+      r->jtemp = std::move(c->rsets[jid]);
+      r->jtemp_id = jid;
+      c->rs_version++;
+    }
     pl_status s = upload_host_batches(c, r->contig, r->batches, &id);
     if (s) return s;
     temp = true;
@@ -1037,6 +1077,7 @@ pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_regio
     r->temp_id = id;
     c->rs_version++;
   }
+  if (!s && jumps) s = region_jump_depth(r);
   if (s) return s;
   if (stats) *stats = r->stats;
   r->finished = true;

@@ -211,6 +211,22 @@ __global__ void k_plan(const RSDev* rsets, const JobDev* jobs, WinDev* wins, int
   cost[w] = 0xffffffffu - (uint32_t)(c > 0xfffffffll ? 0xfffffffll : c);
 }
 
+// the window heads in processing order (worder null: window order)
+__global__ void k_win_heads(const JobDev* jobs, const WinDev* wins, const int32_t* worder, int32_t n_wins,
+                            WinHead* heads) {
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_wins) return;
+  const int32_t wi = worder ? worder[q] : q;
+  const WinDev w = wins[wi];
+  const JobDev& J = jobs[w.job];
+  WinHead h;
+  h.wi = wi; h.job = w.job; h.p0 = w.p0; h.p1 = w.p1;
+  h.lo = w.lo; h.hi = w.hi; h.plo = w.plo; h.out = J.out;
+  h.rs = J.rs; h.start = J.start; h.stop = J.stop; h.qs = J.qs; h.qe = J.qe;
+  h.pad[0] = h.pad[1] = h.pad[2] = 0;
+  heads[q] = h;
+}
+
 // ------------------------------------------------------------------ jobprep
 // readCount of BamFile.process (BamFile.scala:118-139): kept reads whose span
 // overlaps [qs, qe], from the keep-prefix plus the reads straddling qs.
@@ -384,29 +400,41 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
   __shared__ int32_t s_ovf;
   __shared__ IndelRec s_rec[PL_WIN_RECS];
   __shared__ uint32_t s_nrec, s_recbase;
+  // the current window's head and the next one's, fetched while the
+  // current window is piled (persistent form), with their queue slots
+  __shared__ WinHead s_head[2];
+  __shared__ int32_t s_slot[2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lds_words = (PL_WIN_LDS_PER_POS * W) / 8 + 256 / 2;
   const KConf& cf = A.cf;
+  (void)s_w;
 
+  if (tid == 0) {
+    const int32_t q = persistent ? atomicAdd(A.queue, 1) : (int32_t)blockIdx.x;
+    s_slot[0] = q;
+    if (q < A.n_wins) s_head[0] = A.heads[q];
+  }
+  __syncthreads();
   for (int iter = 0;; iter++) {
-    int32_t wi;
-    if (persistent) {
-      if (tid == 0) s_w = atomicAdd(A.queue, 1);
-      __syncthreads();
-      const int32_t w = s_w;
-      __syncthreads();
-      if (w >= A.n_wins) break;
-      // the index comes through LDS: make it wave-uniform so every
-      // descriptor field below is a scalar load
-      wi = __builtin_amdgcn_readfirstlane(A.worder[__builtin_amdgcn_readfirstlane(w)]);
-    } else {
-      if (iter > 0) break;
-      wi = blockIdx.x;
-      if (wi >= A.n_wins) break;
+    const int cur = iter & 1;
+    if (__builtin_amdgcn_readfirstlane(s_slot[cur]) >= A.n_wins) break;
+    if (!persistent && iter > 0) break;
+    // the next window's slot: taken now, its head loaded after the read loop
+    int32_t next_slot = 0;
+    if (persistent && tid == 0) next_slot = atomicAdd(A.queue, 1);
+    // the head through LDS, made wave-uniform so every field is a scalar
+    WinHead win;
+    {
+      const int32_t* hw = (const int32_t*)&s_head[cur];
+      int32_t* dw = (int32_t*)&win;
+#pragma unroll
+      for (int k = 0; k < (int)(sizeof(WinHead) / 4); k++) dw[k] = __builtin_amdgcn_readfirstlane(hw[k]);
     }
-    const WinDev& win = A.wins[wi];
-    const JobDev& J = A.jobs[win.job];
-    const RSDev& R = A.rs[J.rs];
+    const int32_t wi = win.wi;
+    JobDev J;
+    J.rs = win.rs; J.start = win.start; J.stop = win.stop; J.qs = win.qs; J.qe = win.qe;
+    J.size = win.stop - win.start + 1; J.win0 = 0; J.nwin = 0; J.out = win.out; J.rlo = 0; J.rhi = 0;
+    const RSDev& R = A.rs[win.rs];
     const int32_t L0 = J.start + win.p0, L1 = J.start + win.p1, nW = win.p1 - win.p0;
     // hot streams of this read set, held in scalar registers for the window
     const int8_t* __restrict__ gq = R.quals;
@@ -656,6 +684,10 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     for (int64_t k = wave; k < R.n_outlier; k += NW) {
       const int64_t r = uniform64(R.outliers[k]);
       walk_read(R, J, cf, r, L0, L1, lane, 64, sk, true);
+    }
+    if (tid == 0) {
+      s_slot[cur ^ 1] = persistent ? next_slot : A.n_wins;
+      if (persistent && next_slot < A.n_wins) s_head[cur ^ 1] = A.heads[next_slot];
     }
     __syncthreads();
 

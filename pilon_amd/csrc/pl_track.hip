@@ -90,6 +90,7 @@ pl_status tracks_of_lane(pl_ctx* c, Lane* L, const std::vector<int32_t>& contigs
     HIPCHK(hipMemsetAsync(T.jgood, 0, 8 * (size_t)nj, st));
   }
   T.gcv = dgp;
+  T.jdepth = L->has_jdepth ? L->jdepth.as<int32_t>() : nullptr;
   HIPCHK(hipMemcpyAsync(dgp, gp.data(), sizeof(void*) * (size_t)nj, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(T.jsum, 0, 8 * (size_t)nj, st));
   EventPair ev;
@@ -311,6 +312,44 @@ pl_status bed_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   *text = out;
   *len = (int64_t)total;
   return PL_OK;
+}
+
+// The pre-Pass-1 depth of region r's "jumps" reads, per position, into the
+// main lane (processBam leaves those BAMs out of fragCoverage,
+// GenomeRegion.scala:287-300): the jumps alone are piled on the context's
+// auxiliary lane and k_trk_frag recovers their depth before Pass 1.
+pl_status region_jump_depth(pl_region* r) {
+  if (!r->jtemp && r->jtemp_id < 0) return PL_OK;
+  pl_ctx* c = r->ctx;
+  Lane* L = &c->main;
+  Lane* X = &c->aux;
+  const int32_t jid = r->jtemp_id;
+  c->rsets[jid] = std::move(r->jtemp);
+  c->rs_version++;
+  const std::vector<JobReq> q{{jid, r->start, r->stop}};
+  pl_status s = run_jobs(c, X, q, PL_DEFAULT_WINDOW, 0, PL_DEFAULT_BLOCK, false);
+  if (!s) s = fetch_batch_meta(X);
+  if (!s) {
+    VcfBufs B;
+    TrackArgs T;
+    std::memset(&T, 0, sizeof T);
+    T.A = X->args;
+    if (L->G != X->G) s = fail(PL_E_DEVICE, "jumps batch of %lld positions for a region of %lld", (long long)X->G,
+                               (long long)L->G);
+    else if (L->jdepth.ensure(4 * (size_t)L->G) != hipSuccess) s = fail(PL_E_OOM, "device allocation failed");
+    else if (!(s = B.get(&T.jsum, 8 * X->hjobs.size()))) {
+      T.frag = L->jdepth.as<int32_t>();
+      if (hipMemsetAsync(T.jsum, 0, 8 * X->hjobs.size(), X->stream) != hipSuccess ||
+          (k_trk_frag<<<dim3((unsigned)X->n_wins), dim3(256), 0, X->stream>>>(T), hipGetLastError()) != hipSuccess ||
+          hipStreamSynchronize(X->stream) != hipSuccess)
+        s = fail(PL_E_DEVICE, "jumps depth pass failed");
+      else
+        L->has_jdepth = true;
+    }
+  }
+  r->jtemp = std::move(c->rsets[jid]);
+  c->rs_version++;
+  return s;
 }
 
 }  // namespace

@@ -34,6 +34,7 @@
 struct TrackArgs {
   EngineArgs A;
   int32_t* frag;               // [G] fragment coverage
+  const int32_t* jdepth;       // [G] depth of "jumps" reads left out of it (null: none)
   int64_t* jsum;               // [n_jobs] sum of frag per region
   const uint8_t* const* gcv;   // [n_jobs] the region's contig buffer values
   int32_t* vals;               // [PL_NTRACKS][G]
@@ -231,6 +232,7 @@ __global__ void __launch_bounds__(256) k_trk_frag(TrackArgs T) {
       while (k > J.out && (A.calls[k].flags & PL_CF_DELETED)) k--;
       fc = j_sub(fc, A.pile[(int64_t)PL_F_DELETIONS * G + k]);
     }
+    if (T.jdepth) fc = j_sub(fc, T.jdepth[g]);          // processBam skips jumps BAMs (GenomeRegion.scala:291-298)
     T.frag[g] = fc;
     s += fc;
     if (T.mask && reads) {                                   // pctBadOverall's Long sums (GenomeRegion.scala:99-113)

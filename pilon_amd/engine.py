@@ -18,7 +18,7 @@ import numpy as np
 
 from .abi import (PlConfig, PlReadSoa, PlCall, PlRegionStats, PlRegionDesc, PlBatchOpts, PlBatchInfo, PlServeOpts,
                   PlServeResult, PlTarget, PlWideCol, NFIELDS, F, STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK,
-                  PL_CF_DISP_SHIFT, SERVE_KEEP_PILEUP, SERVE_KEEP_FIX, CHUNK_SIZE)
+                  PL_CF_DISP_SHIFT, SERVE_KEEP_PILEUP, SERVE_KEEP_FIX, CHUNK_SIZE, BAM_TYPES)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libpilonhip.so")
@@ -147,6 +147,7 @@ def lib():
             "pl_readset_upload": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa), P(C.c_int32)]),
             "pl_region_open": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_void_p)]),
             "pl_region_add_reads": (C.c_int, [C.c_void_p, P(PlReadSoa)]),
+            "pl_region_add_reads_typed": (C.c_int, [C.c_void_p, P(PlReadSoa), C.c_int32]),
             "pl_region_use_readset": (C.c_int, [C.c_void_p, C.c_int32]),
             "pl_region_finish": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, P(PlRegionStats)]),
             "pl_region_fix": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, P(C.c_void_p), P(C.c_int64),
@@ -495,10 +496,12 @@ class PileUpRegion:
     def __del__(self):
         self.close()
 
-    def addReads(self, reads):
-        """BamFile.process for one BAM's worth of reads (query, validateRead, addRead)."""
+    def addReads(self, reads, bam_type="frags"):
+        """BamFile.process for one BAM's worth of reads (query, validateRead,
+        addRead).  bam_type is the BAM's --frags/--jumps/--unpaired kind:
+        "jumps" reads are left out of fragment coverage (GenomeRegion.scala:287-300)."""
         soa = reads.soa() if hasattr(reads, "soa") else reads
-        _check(lib().pl_region_add_reads(self.h, C.byref(soa)))
+        _check(lib().pl_region_add_reads_typed(self.h, C.byref(soa), BAM_TYPES[bam_type]))
 
     def useReadSet(self, readset_id):
         _check(lib().pl_region_use_readset(self.h, readset_id))

@@ -114,6 +114,35 @@ class ReadBatch:
                          pack_2bit(codes), self.quals[s0:s1], exo_read, exo_off,
                          pack_nib(nib_codes))
 
+    def take(self, idx):
+        """The reads at the increasing indices idx as a new batch (a subset
+        keeps the coordinate order)."""
+        idx = np.asarray(idx, dtype=np.int64)
+        if len(idx) > 1 and np.any(np.diff(idx) <= 0):
+            raise ValueError("indices must be increasing")
+        co, so = self.cigar_off, self.seq_off
+
+        def gather(off, idx):
+            ln = off[idx + 1] - off[idx]
+            noff = np.concatenate([[0], np.cumsum(ln)]).astype(np.int64)
+            src = np.repeat(off[idx] - noff[:-1], ln) + np.arange(int(noff[-1]), dtype=np.int64)
+            return noff, src
+
+        nco, csrc = gather(co, idx)
+        nso, ssrc = gather(so, idx)
+        codes = unpack_2bit(self.bases2, 0, int(so[-1]))[ssrc] if len(ssrc) else np.zeros(0, np.uint8)
+        where = {int(r): k for k, r in enumerate(self.exo_read)}
+        exo_read, exo_off, nib_codes = [], [], []
+        for j, r in enumerate(idx):
+            k = where.get(int(r))
+            if k is not None:
+                L = int(so[r + 1] - so[r])
+                exo_read.append(j)
+                exo_off.append(len(nib_codes))
+                nib_codes.extend(unpack_nib(self.nib, int(self.exo_off[k]), L))
+        return ReadBatch(self.pos[idx], self.tlen[idx], self.flag[idx], self.mapq[idx], nco, self.cigar[csrc], nso,
+                         pack_2bit(codes), self.quals[ssrc], exo_read, exo_off, pack_nib(nib_codes))
+
 
 def pack_2bit(codes):
     codes = np.asarray(codes, dtype=np.uint8)

@@ -137,3 +137,48 @@ def test_fragment_tracks_from_the_final_pileup():
     x = O.smooth(frag.astype(np.int32)) / mean
     cn = (np.floor(x) + ((x - np.floor(x)) >= 0.5)).astype(np.int64)
     assert np.array_equal(t[2], cn - 1)
+
+
+def jumps_split(d, ci=0):
+    """A frags set F and a jumps set J out of one synthetic batch: J takes
+    every other read starting in the middle third of the contig, so F's
+    depth dips there while the pileup of F+J is the batch's."""
+    b = d.batch(ci)
+    L = len(d.draft(ci))
+    idx = np.arange(b.n)
+    jsel = (b.pos > L // 3) & (b.pos < 2 * L // 3) & (idx % 2 == 1)
+    return b, b.take(idx[~jsel]), b.take(idx[jsel])
+
+
+def test_jumps_left_out_of_fragment_coverage():
+    """processBam (GenomeRegion.scala:287-300) adds a BAM's depth to fragment
+    coverage unless it is a jumps BAM.  Pileups add across BAMs, so a region
+    given a frags set F and a jumps set J has the pileup, calls and coverage
+    of F+J as one set, and the delta, dip and copy-number tracks (functions
+    of fragment coverage alone) of F alone, in either order; an unpaired
+    BAM counts as frags."""
+    O = oracle()
+    from pilon_amd.synth import SynthData, preset
+    d = SynthData(preset("c1"))
+    draft, L = d.draft(0), len(d.draft(0))
+    b, Fs, Js = jumps_split(d)
+    assert Js.n > 1000 and Fs.n + Js.n == b.n
+    cfg = default_config()
+    full = O.run_region(cfg, draft, 1, L, [b], name="c", vcf=O.TRACKS)
+    fonly = O.run_region(cfg, draft, 1, L, [Fs], name="c", vcf=O.TRACKS)
+    frag_tracks = [2, 6, 7]
+    for sets, types in (([Fs, Js], [0, 1]), ([Js, Fs], [1, 0])):
+        r = O.run_region(cfg, draft, 1, L, sets, name="c", vcf=O.TRACKS, set_types=types)
+        assert np.array_equal(r["pile"], full["pile"])
+        assert np.array_equal(r["calls"], full["calls"])
+        other = [t for t in range(len(O.TRACK_NAMES)) if t not in frag_tracks]
+        assert np.array_equal(r["tracks"][other], full["tracks"][other])
+        assert np.array_equal(r["tracks"][frag_tracks], fonly["tracks"][frag_tracks])
+        assert not np.array_equal(r["tracks"][6], full["tracks"][6])
+    for types in ([0, 0], [0, 2], [2, 2]):
+        r = O.run_region(cfg, draft, 1, L, [Fs, Js], name="c", vcf=O.TRACKS, set_types=types)
+        assert np.array_equal(r["tracks"], full["tracks"])
+    # jumps only: no fragment coverage, so copy number 0 (track value -1)
+    # and flat delta and dip
+    r = O.run_region(cfg, draft, 1, L, [b], name="c", vcf=O.TRACKS, set_types=[1])
+    assert (r["tracks"][2] == -1).all() and not r["tracks"][6].any() and not r["tracks"][7].any()
