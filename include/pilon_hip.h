@@ -226,6 +226,23 @@ pl_status pl_batch_run(pl_ctx* ctx, const pl_region_desc* regions, int32_t n,
                        const pl_batch_opts* opts, int32_t* pileup,
                        pl_call* calls, pl_region_stats* stats);
 
+/* pl_batch_run without allocating: no hipMalloc, hipFree or other
+ * allocating routine runs in this call.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Every device buffer of the batch pipeline is carved from a scratch array
+ * of 192 MiB that the library holds on each GPU it runs on until the
+ * process ends, so a batch may hold at most 2^20 positions (PL_E_INVAL
+ * beyond; PL_E_OOM if its buffers still do not fit); window/block 0 mean
+ * 512/256 (no autotuning) and use_graph must be 0.  Inputs are the
+ * context's resident read sets, outputs the caller's host buffers, as for
+ * pl_batch_run.  All calls on one GPU share its array, which keeps the
+ * last call's contents: calls that overlap on one GPU (from any thread or
+ * context) race.  pl_last_batch_info and the VCF/track/fix entry points
+ * keep describing the last pl_batch_run, not this call. */
+pl_status pl_batch_run_noalloc(pl_ctx* ctx, const pl_region_desc* regions, int32_t n, const pl_batch_opts* opts,
+                               int32_t* pileup, pl_call* calls, pl_region_stats* stats);
+
 /* Device time of the last pl_batch_run's pipeline in milliseconds, measured
  * with HIP events on the library's stream, plus the share of the window
  * kernel alone. */

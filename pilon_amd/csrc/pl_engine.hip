@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -46,12 +47,29 @@ pl_status fail(pl_status s, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail(PL_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
   } while (0)
 
+// Device memory handed out by bumping an offset in a block the library
+// holds for the process (the no-allocation entry point's scratch array).
+struct Arena {
+  uint8_t* base = nullptr;
+  size_t size = 0, used = 0;
+  bool exhausted = false;
+};
+
 struct DBuf {
   void* p = nullptr;
   size_t cap = 0;
+  Arena* arena = nullptr;   // set: buffers are carved from it, never allocated
   hipError_t ensure(size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (bytes <= cap) return hipSuccess;
+    if (arena) {
+      const size_t at = (arena->used + 255) & ~(size_t)255;
+      if (at > arena->size || bytes > arena->size - at) { arena->exhausted = true; return hipErrorOutOfMemory; }
+      p = arena->base + at;
+      cap = bytes;
+      arena->used = at + bytes;
+      return hipSuccess;
+    }
     if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; }
     p = nullptr;
     cap = 0;
@@ -59,7 +77,7 @@ struct DBuf {
     if (e == hipSuccess) cap = bytes;
     return e;
   }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  void release() { if (p && !arena) (void)hipFree(p); p = nullptr; cap = 0; }
   template <class T> T* as() const { return (T*)p; }
 };
 
@@ -164,19 +182,49 @@ struct Lane {
     for (const DBuf* x : b) t += x->cap;
     return t;
   }
-  void for_each_buffer(void (*f)(DBuf&)) {
+  void for_each_buffer(void (*f)(DBuf&, Arena*), Arena* a) {
     DBuf* b[] = {&rsarr, &kparr, &jobs, &wins, &worder, &costk, &costv, &costtk, &costtv, &pile, &calls, &st, &rec,
                  &rkey, &rval, &rtk, &rtv, &cand, &cands, &ckey, &cval, &ctk, &ctv, &defer, &mutated, &counters,
                  &hist, &tiles, &strpool, &stridx, &stroff, &gath, &wide, &winwide, &heads, &jdepth};
-    for (DBuf* x : b) f(*x);
+    for (DBuf* x : b) f(*x, a);
   }
+  // every device buffer of the lane carved from a (no allocation, no free)
+  void use_arena(Arena* a) { for_each_buffer([](DBuf& d, Arena* x) { d.release(); d.arena = x; }, a); }
   void free_buffers() {
-    for_each_buffer([](DBuf& d) { d.release(); });
+    for_each_buffer([](DBuf& d, Arena*) { d.release(); }, nullptr);
     tables_valid = false;
     rs_seen = 0;
     rec_cap = cand_cap = defer_cap = mut_cap = wide_cap = 0;
   }
 };
+
+// Device memory of pl_batch_run_noalloc: one array per GPU the library runs
+// on, held until the process ends.  Host code never uses its name as an
+// address: the device address comes from hipGetSymbolAddress, taken once per
+// device when a context is created with that device current.
+#define PL_NOALLOC_SCRATCH_BYTES ((size_t)192 << 20)
+#define PL_NOALLOC_MAX_POSITIONS (1 << 20)
+static __device__ __align__(256) uint8_t scratch[PL_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, uint8_t*> g_scratch_ptr;   // device -> address of that device's scratch
+
+// the device address of the current device's scratch array
+pl_status scratch_address(int device, uint8_t** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it == g_scratch_ptr.end()) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(PL_E_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    it = g_scratch_ptr.emplace(device, (uint8_t*)ptr).first;
+  }
+  *out = it->second;
+  return PL_OK;
+}
+}  // namespace
 
 struct pl_ctx {
   pl_config pc;
@@ -188,6 +236,8 @@ struct pl_ctx {
   uint64_t rs_version = 1;         // bumped whenever rsets changes
   Lane main;                       // plain and batched forms
   Lane aux;                        // a region's "jumps" reads piled alone
+  Lane na;                         // pl_batch_run_noalloc: buffers carved from the device's scratch array
+  Arena na_arena;
   std::map<uint64_t, std::pair<int32_t, int32_t>> tuned;   // autotuned (window, block) per batch shape
   int serving = 0;                 // servers open on this context
 };
@@ -954,7 +1004,11 @@ pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out) {
   c->n_cu = prop.multiProcessorCount;
   pl_status st = lane_init(&c->main);
   if (!st) st = lane_init(&c->aux);
-  if (st) { lane_destroy(&c->main); lane_destroy(&c->aux); return st; }
+  if (!st) st = lane_init(&c->na);
+  if (!st) st = scratch_address(device, &c->na_arena.base);
+  if (st) { lane_destroy(&c->main); lane_destroy(&c->aux); lane_destroy(&c->na); return st; }
+  c->na_arena.size = PL_NOALLOC_SCRATCH_BYTES;
+  c->na.use_arena(&c->na_arena);
   *out = c.release();
   return PL_OK;
 }
@@ -964,6 +1018,7 @@ void pl_ctx_destroy(pl_ctx* c) {
   (void)hipSetDevice(c->device);
   lane_destroy(&c->main);
   lane_destroy(&c->aux);
+  lane_destroy(&c->na);
   c->rsets.clear();
   for (auto& kv : c->contigs) if (kv.second.dev) (void)hipFree(kv.second.dev);
   c->contigs.clear();
@@ -1167,6 +1222,35 @@ pl_status pl_batch_run(pl_ctx* c, const pl_region_desc* regions, int32_t n, cons
   }
   if ((s = run_jobs(c, L, q, W, persistent, block, use_graph))) return s;
   return batch_outputs(c, L, n, pile, calls, stats);
+}
+
+pl_status pl_batch_run_noalloc(pl_ctx* c, const pl_region_desc* regions, int32_t n, const pl_batch_opts* opts,
+                               int32_t* pile, pl_call* calls, pl_region_stats* stats) {
+  if (!c || !regions || n <= 0) return fail(PL_E_INVAL, "empty batch");
+  Lane* L = &c->na;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<JobReq> q;
+  pl_status s = batch_requests(c, regions, n, &q);
+  if (s) return s;
+  int64_t G = 0;
+  for (int32_t i = 0; i < n; i++) G += std::max<int64_t>(0, (int64_t)regions[i].stop - regions[i].start + 1);
+  if (G > PL_NOALLOC_MAX_POSITIONS)
+    return fail(PL_E_INVAL, "batch of %lld positions exceeds the no-allocation limit of %d", (long long)G,
+                PL_NOALLOC_MAX_POSITIONS);
+  if (opts && opts->use_graph) return fail(PL_E_INVAL, "the no-allocation form does not capture graphs");
+  const int32_t W = opts && opts->window ? opts->window : PL_DEFAULT_WINDOW;
+  const int32_t block = opts && opts->block ? opts->block : PL_DEFAULT_BLOCK;
+  const int persistent = opts ? opts->persistent : 0;
+  // every buffer of this call is carved afresh from the start of the array;
+  // what must start at zero is cleared by the pipeline's own kernels
+  L->free_buffers();
+  c->na_arena.used = 0;
+  c->na_arena.exhausted = false;
+  s = run_jobs(c, L, q, W, persistent, block, false);
+  if (!s) s = batch_outputs(c, L, n, pile, calls, stats);
+  if (s && c->na_arena.exhausted)
+    return fail(PL_E_OOM, "the batch's device buffers exceed the %zu-byte scratch array", PL_NOALLOC_SCRATCH_BYTES);
+  return s;
 }
 
 pl_status pl_last_batch_info(pl_ctx* c, pl_batch_info* o) {

@@ -188,6 +188,8 @@ def lib():
             "pl_serve_fetch_fix": (C.c_int, [C.c_void_p, C.c_int64, C.c_char_p, C.c_int64, P(C.c_void_p),
                                              P(C.c_int64), P(C.c_void_p), P(C.c_int64)]),
             "pl_last_wide_columns": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_int64)]),
+            "pl_batch_run_noalloc": (C.c_int, [C.c_void_p, P(PlRegionDesc), C.c_int32, P(PlBatchOpts), C.c_void_p,
+                                               C.c_void_p, C.c_void_p]),
             "pl_batch_vcf_stream": (C.c_int, [C.c_void_p, C.c_uint32, TEXT_SINK, C.c_void_p, P(C.c_int64),
                                               P(C.c_double)]),
         }
