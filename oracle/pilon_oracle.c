@@ -336,8 +336,31 @@ static int consumesRef(int op) { return op == 0 || op == 2 || op == 3 || op == 7
 
 #define EDGE(R, r, why) do { if (!(R)->edge) { (R)->edge = 1; (R)->edgeRead = (r)->index; (R)->edgeWhy = (why); } return; } while (0)
 
-/* PileUpRegion.addRead with longRead = 0 (PileUpRegion.scala:102-220). */
-static void pr_addRead(PRegion* R, const ReadView* r, const uint8_t* refBases, const pl_config* cfg) {
+/* addRead's homoRun (PileUpRegion.scala:120-126): the run of equal bytes of
+ * the contig from 0-based index i0 to the contig's end.  refBases(i0) out of
+ * range throws (*oob). */
+static int32_t pr_homoRun(const uint8_t* refBases, int64_t contig_len, int64_t i0, int* oob) {
+  if (i0 < 0 || i0 >= contig_len) { *oob = 1; return 0; }
+  for (int64_t i = i0 + 1; i < contig_len; i++)
+    if (refBases[i] != refBases[i0]) return (int32_t)(i - i0);
+  return (int32_t)(contig_len - i0);
+}
+
+/* addRead's nanoporeExclude (PileUpRegion.scala:128-134): i0 is a region
+ * index (index(locus)) but indexes the whole contig's bytes; the region test
+ * is on locus(i0 -+ 2). */
+static int pr_nanoporeExclude(const PRegion* R, const uint8_t* refBases, int64_t i0) {
+  return inRegion(R, (int64_t)R->start + i0 - 2) && inRegion(R, (int64_t)R->start + i0 + 2) &&
+         refBases[i0 - 2] == 'C' && refBases[i0 - 1] == 'C' && refBases[i0 + 1] == 'G' && refBases[i0 + 2] == 'G';
+}
+
+#define LR_NANOPORE 1   /* BamFile.nanoporeLongRead (BamFile.scala:34) */
+#define LR_PACBIO 2     /* BamFile.pacbioLongRead (:35) */
+
+/* PileUpRegion.addRead (PileUpRegion.scala:102-220); longRead is the BAM's
+ * longReadType (BamFile.scala:43-47): 0, LR_NANOPORE or LR_PACBIO. */
+static void pr_addRead(PRegion* R, const ReadView* r, const uint8_t* refBases, int64_t contig_len, int32_t longRead,
+                       const pl_config* cfg) {
   const int32_t length = r->length;
   const uint8_t* bases = r->bases;
   const int32_t mq = r->mq;
@@ -353,7 +376,8 @@ static void pr_addRead(PRegion* R, const ReadView* r, const uint8_t* refBases, c
   for (int64_t k = 0; k < r->ncigar; k++)
     if ((r->cigar[k] & 0xf) == 4) clipped = jadd(clipped, (int32_t)(r->cigar[k] >> 4));
   int32_t adjMq = roundDivI(jmul(mq, jsub(length, clipped)), length);      /* :139-141 */
-  int32_t indelMq = adjMq;                                                  /* :142 (longRead = 0) */
+  int32_t indelMq = longRead > 0 ? (adjMq < 8 ? adjMq : 8) : adjMq;       /* :142 */
+  int oob = 0;
 
   int32_t readOffset = 0, refOffset = 0;
   for (int64_t k = 0; k < r->ncigar; k++) {
@@ -371,6 +395,12 @@ static void pr_addRead(PRegion* R, const ReadView* r, const uint8_t* refBases, c
           memmove(ins + 1, ins, (size_t)len - 1);
           ins[0] = last;
         }
+        /* :160: a long read's insertion at the start of a homopolymer of 4 is dropped */
+        if (longRead > 0) {
+          int32_t run = pr_homoRun(refBases, contig_len, iloc, &oob);
+          if (oob) EDGE(R, r, "homoRun past the contig end");
+          if (run >= 4) goto next_op;
+        }
         if (iloc < R->start) EDGE(R, r, "insertion normalised before region start");
         pu_addInsertion(&R->pu[iloc - R->start], ins, len, quals[readOffset], indelMq);
       }
@@ -387,13 +417,26 @@ static void pr_addRead(PRegion* R, const ReadView* r, const uint8_t* refBases, c
             if (inRegion(R, (int64_t)dloc + len)) pr_add(R, (int64_t)dloc + len, base, qual, adjMq, valid, cfg);
           }
         }
+        /* :180-182: homoRun and nanoporeExclude of index(dloc), a region
+           index read as a contig index; a negative one throws */
+        if (longRead > 0) {
+          int32_t run = pr_homoRun(refBases, contig_len, (int64_t)dloc - R->start, &oob);
+          if (oob) EDGE(R, r, "deletion normalised before region start");
+          if (run >= 4) goto next_op;
+          if (longRead == LR_NANOPORE && pr_nanoporeExclude(R, refBases, (int64_t)dloc - R->start)) goto next_op;
+        }
         if (dloc < R->start) EDGE(R, r, "deletion normalised before region start");
         pu_addDeletion(&R->pu[dloc - R->start], refBases + (dloc - 1), len, quals[readOffset], indelMq);
       }
     } else if (op == 0 || op == 7 || op == 8) {                             /* M/=/X :184-193 */
       for (int32_t i = 0; i < len; i++) {
         int32_t rOff = readOffset + i;
-        if (TRUSTED(rOff)) pr_add(R, (int64_t)locus + i, bases[rOff], quals[rOff], adjMq, valid, cfg);
+        if (TRUSTED(rOff)) {
+          int64_t locusPlus = (int64_t)locus + i;
+          /* :190: a nanopore base inside CC_GG piles with quality 0 */
+          int32_t qual = (longRead == LR_NANOPORE && pr_nanoporeExclude(R, refBases, locusPlus - R->start)) ? 0 : quals[rOff];
+          pr_add(R, locusPlus, bases[rOff], qual, adjMq, valid, cfg);
+        }
       }
     } else if (op == 4) {                                                   /* S :194-206 */
       int32_t clipStart = readOffset == 0 ? jsub(locus, len) : locus;
@@ -410,6 +453,7 @@ static void pr_addRead(PRegion* R, const ReadView* r, const uint8_t* refBases, c
     } else {
       R->unknownOps++;                                                      /* :211-212 */
     }
+  next_op:
     if (consumesRead(op)) readOffset = jadd(readOffset, len);
     if (consumesRef(op)) refOffset = jadd(refOffset, len);
   }
@@ -927,9 +971,10 @@ static void bed_lines(const pl_config* cfg, const char* name, int32_t start, int
   free(t);
 }
 
-/* set_types[si] (NULL: all fragment BAMs): 1 marks a "jumps" BAM, whose
- * depth increments processBam leaves out of fragCoverage
- * (GenomeRegion.scala:287-300). */
+/* set_types[si] (NULL: all fragment BAMs) is the BAM's type: 0 frags, 1
+ * jumps (whose depth increments processBam leaves out of fragCoverage,
+ * GenomeRegion.scala:287-300), 2 unpaired, 3 nanopore and 4 pacbio
+ * (unpaired BAMs of longReadType 1 and 2, BamFile.scala:43-47). */
 orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
                                  const char* name, int32_t start, int32_t stop,
                                  const pl_read_soa* const* sets, const int32_t* set_types, int32_t n_sets,
@@ -951,6 +996,7 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
   for (int32_t si = 0; si < n_sets && !R.edge; si++) {
     const pl_read_soa* s = sets[si];
     const int jumps = set_types && set_types[si] == 1;
+    const int32_t longRead = !set_types ? 0 : set_types[si] == 3 ? LR_NANOPORE : set_types[si] == 4 ? LR_PACBIO : 0;
     if (!jumps) for (int32_t i = 0; i < size; i++) covBefore[i] = (int32_t)pu_depth(&R.pu[i]);
     Decoder d = {s, NULL, NULL, 0};
     for (int64_t i = 0; i < s->n_reads; i++) {
@@ -974,7 +1020,7 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
         if (consumesRead(op)) ro += len;
         if (consumesRef(op)) fo += len;
       }
-      pr_addRead(&R, &r, contig, cfg);
+      pr_addRead(&R, &r, contig, contig_len, longRead, cfg);
       if (R.edge) break;
     }
     free(d.bases); free(d.quals);
@@ -1144,6 +1190,17 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
     nConfirmed += confirmed[i];
   }
 
+  /* excludeMotifs (GenomeRegion.scala:219,432-439), when every BAM is a
+   * long-read or other non-frags BAM of a long-read run (longReadOnly,
+   * :147): loci starting a homopolymer of 4 within the region, or, with a
+   * nanopore BAM, the middle of CC_GG (upper-cased region bases), take no fix */
+  int longReadOnly = cfg->longread != 0, nano = 0;
+  for (int32_t si = 0; si < n_sets; si++) {
+    const int32_t t = set_types ? set_types[si] : 0;
+    if (t == 0) longReadOnly = 0;
+    if (t == 3) nano = 1;
+  }
+
   /* identifyAndFixIssues, snps/indels path (GenomeRegion.scala:307-380,413) */
   FixList snpL = {0}, smallL = {0};
   int snps = 0, ins = 0, dels = 0, insBases = 0, delBases = 0, amb = 0;
@@ -1162,7 +1219,16 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
     uint8_t cBase = bc.base == 4 ? 'N' : (uint8_t)"ACGT"[bc.base];
     oneChar[2 * i] = rBase; oneChar[2 * i + 1] = cBase;
     Fix f; memset(&f, 0, sizeof f); f.loc = loc;
-    switch (kindAt[i]) {
+    int excluded = 0;
+    if (longReadOnly) {
+      int32_t run = 1;                                                   /* homoRun(loc) (:416-422) */
+      while (loc + run <= stop && upper(contig[(int64_t)loc + run - 1]) == rBase) run++;
+      excluded = run >= 4 ||
+                 (nano && loc - 2 >= start && loc + 2 <= stop &&           /* nanoporeExclude (:424-430) */
+                  upper(contig[loc - 3]) == 'C' && upper(contig[loc - 2]) == 'C' &&
+                  upper(contig[loc]) == 'G' && upper(contig[loc + 1]) == 'G');
+    }
+    if (!excluded) switch (kindAt[i]) {
       case K_SNP:
         if (cfg->fix_snps) { f.was = &oneChar[2 * i]; f.wasLen = 1; f.patch = &oneChar[2 * i + 1]; f.patchLen = 1; fix_prepend(&snpL, f); }
         snps++;

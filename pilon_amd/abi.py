@@ -32,8 +32,8 @@ PL_CF_DISP_SHIFT = 8
 PL_CF_DISP_MASK = 0x0700
 PL_CF_DELETED = 0x0800
 PL_CF_WIDE = 0x1000
-# Pilon's BAM kinds (--frags, --jumps, --unpaired): pl_region_add_reads_typed
-BAM_TYPES = {"frags": 0, "jumps": 1, "unpaired": 2}
+# Pilon's BAM kinds (--frags, --jumps, --unpaired, --nanopore, --pacbio): pl_region_add_reads_typed
+BAM_TYPES = {"frags": 0, "jumps": 1, "unpaired": 2, "nanopore": 3, "pacbio": 4}
 
 D_NONE, D_CONFIRMED, D_SNP, D_INS, D_DEL, D_AMB = range(6)
 

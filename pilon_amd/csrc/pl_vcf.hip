@@ -358,9 +358,9 @@ pl_status on_region_lane(pl_region* r, F&& f) {
   if (r->temp) { c->rsets[id] = std::move(r->temp); c->rs_version++; }
   pl_status s = run_jobs(c, L, q, PL_DEFAULT_WINDOW, 0, PL_DEFAULT_BLOCK, false);
   if (!s) s = fetch_batch_meta(L);
-  if (r->temp_id >= 0 && !r->temp) { r->temp = std::move(c->rsets[id]); c->rs_version++; }
   if (!s) s = region_jump_depth(r);
   if (!s) s = f(c, L, names, contigs);
+  if (r->temp_id >= 0 && !r->temp) { r->temp = std::move(c->rsets[id]); c->rs_version++; }
   return s;
 }
 

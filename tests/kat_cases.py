@@ -287,6 +287,191 @@ ALL = [case_trusted_flank, case_soft_clip, case_invalid_pair, case_deletion_slid
        case_het_deletion, case_het_deletion_oldindel, case_min_depth_300, case_min_depth_311, case_deep_region,
        case_deep_70k, case_wide_qualsum]
 
+# ---- long-read BAMs (--nanopore / --pacbio: PileUpRegion.addRead with
+# longRead > 0, GenomeRegion.excludeMotifs).  A case's "sets" lists
+# (reads, BAM type) pairs; the other cases are one frags BAM.
+LR = {"longread": 1}
+# 40 bp: AAAAA at loci 13-17 / AAAA at 13-16, G at 12, T after the run
+REF_H5 = "GCTAGCTAGCTG" + "AAAAA" + "TCGATCGTACGTTGCATGCATGC"
+REF_H4 = "GCTAGCTAGCTG" + "AAAA" + "TCGATCGTACGTTGCATGCATGCA"
+# 43 bp: CCAGG at loci 20-24 and no other CC_GG
+REF_N = "GATCGATTGCAGTCAGTTA" + "CCAGG" + "TATGCATCGATTCGATCAG"
+# 42 bp: TTTT at loci 20-23 (G before, C after)
+REF_T = "GATCGATTGCAGTCAGCTG" + "TTTT" + "CAGTAGCATGCATCGAGCG"
+
+
+def case_lr_insertion_pacbio():
+    """PileUpRegion.scala:142,160: case_insertion_rotation from a pacbio BAM.
+    The insertion stops at 13, where homoRun(13) (refBases from index 13:
+    C) is 1, so it is kept with indelMq = adjMq min 8 = 8: insQual 6*9.
+    insPct is the larger of pct(insQual, mqSum) and pct(insertions, count)
+    (PileUp.scala:122), 100, so the call and its fix stand; excludeMotifs
+    (GenomeRegion.scala:219, every BAM long-read) leaves 13 alone: homoRun(13)
+    over the region is 1 (A then C)."""
+    c = case_insertion_rotation()
+    c["expect"] = {13: {"insertions": 6, "insQual": 54, "qSum": 6 * (32 + 36), "mqSum": 6 * 61}}
+    c["sets"] = [(c["reads"], "pacbio")]
+    c["cfg"] = LR
+    return c
+
+
+def _h_reads(ref, run):
+    # one extra A after the run: nM 1I (30 - n)M at 1, quals 30
+    seq = ref[0:12 + run] + "A" + ref[12 + run:29]
+    return [read(1, "%dM1I%dM" % (12 + run, 29 - 12 - run), seq, [30] * 30) for _ in range(6)]
+
+
+def case_lr_insertion_homopolymer5():
+    """PileUpRegion.scala:150-162: an extra A after AAAAA (13-17) slides to 13
+    (locus 12 is G).  homoRun(iloc) reads refBases from 0-based index 13,
+    i.e. loci 14-17, AAAA: 4, so a pacbio read's insertion is dropped (no
+    insertion, no insQual, no insertion qual in qSum at 13)."""
+    reads = _h_reads(REF_H5, 5)
+    return dict(ref=REF_H5, start=1, stop=40, reads=reads, sets=[(reads, "pacbio")], cfg=LR,
+                expect={13: {"insertions": 0, "insQual": 0, "qSum": 180, "count_A": 6}},
+                stats={"ins": 0})
+
+
+def case_lr_insertion_homopolymer5_frags():
+    """The same reads from a frags BAM: the insertion is added at 13 (6 x
+    insQual 61, qSum 30 + 30 per read) and fixed."""
+    reads = _h_reads(REF_H5, 5)
+    return dict(ref=REF_H5, start=1, stop=40, reads=reads,
+                expect={13: {"insertions": 6, "insQual": 366, "qSum": 360}}, stats={"ins": 1, "ins_bases": 1})
+
+
+def case_lr_insertion_homopolymer4():
+    """An extra A after AAAA (13-16) also stops at 13, but homoRun(13) sees
+    loci 14-16 then T: 3, so the pacbio insertion is kept (insQual 6*9).
+    The INS call at 13 is not fixed: excludeMotifs' homoRun(13) over the
+    region counts AAAA, 4 (GenomeRegion.scala:416-422,438)."""
+    reads = _h_reads(REF_H4, 4)
+    return dict(ref=REF_H4, start=1, stop=40, reads=reads, sets=[(reads, "pacbio")], cfg=LR,
+                expect={13: {"insertions": 6, "insQual": 54, "qSum": 360}},
+                stats={"ins": 0}, changes="", bases=REF_H4)
+
+
+def _del_reads():
+    # 15M1D15M at 1 deletes locus 16, the last A of AAAA (13-16)
+    seq = REF_H4[0:15] + REF_H4[16:31]
+    return [read(1, "15M1D15M", seq, [30] * 30) for _ in range(6)]
+
+
+def case_lr_deletion_homopolymer():
+    """PileUpRegion.scala:163-183: the deletion slides from 16 to 13, adding
+    the slid-over A of each step one locus later (14, 15, 16 gain a count).
+    homoRun(index(13)) = homoRun(12) reads loci 13-16, AAAA: 4, so the pacbio
+    deletion is dropped: no deletions, delQual or deletion mqSum at 13."""
+    reads = _del_reads()
+    return dict(ref=REF_H4, start=1, stop=40, reads=reads, sets=[(reads, "pacbio")], cfg=LR,
+                expect={13: {"deletions": 0, "delQual": 0, "mqSum": 6 * 61, "count_A": 6},
+                        14: {"count_A": 12}, 15: {"count_A": 12}, 16: {"count_A": 6}},
+                stats={"dels": 0})
+
+
+def case_lr_deletion_homopolymer_region5():
+    """The same reads in region 5-40: index(13) is 8, and homoRun(8) reads
+    refBases(8) = G, then C: 1, so the deletion is kept at 13 with indelMq
+    8 (delQual 6*9, mqSum 6*61 + 6*9).  Its DEL call is not fixed:
+    excludeMotifs' homoRun(13) over the region is 4."""
+    reads = _del_reads()
+    return dict(ref=REF_H4, start=5, stop=40, reads=reads, sets=[(reads, "pacbio")], cfg=LR,
+                expect={13: {"deletions": 6, "delQual": 54, "mqSum": 6 * 70, "qSum": 6 * 60}},
+                stats={"dels": 0})
+
+
+def _n_reads():
+    return [read(2, "40M", REF_N[1:41], [30] * 40) for _ in range(6)]
+
+
+def case_lr_nanopore_qual():
+    """PileUpRegion.scala:128-134,190: a nanopore base at a locus whose
+    index(locus) passes nanoporeExclude piles with quality 0.  Region 1-43:
+    index(22) = 21, refBases 19,20 = CC and 22,23 = GG, so the A at 22
+    counts with no quality (qual_A 0, qSum 0, mqSum 6*61)."""
+    reads = _n_reads()
+    return dict(ref=REF_N, start=1, stop=43, reads=reads, sets=[(reads, "nanopore")], cfg=LR,
+                expect={22: {"count_A": 6, "qual_A": 0, "qSum": 0, "mqSum": 366},
+                        21: {"count_C": 6, "qual_C": 6 * 30 * 61, "qSum": 180},
+                        24: {"count_G": 6, "qual_G": 6 * 30 * 61}})
+
+
+def case_lr_nanopore_qual_region3():
+    """Region 3-43: the region index is read as a contig index, so the motif
+    test passes for index 21 = locus 24 (refBases 19..23 = CCAGG): the G at
+    24 piles with quality 0 and the A at 22 keeps its own."""
+    reads = _n_reads()
+    return dict(ref=REF_N, start=3, stop=43, reads=reads, sets=[(reads, "nanopore")], cfg=LR,
+                expect={22: {"count_A": 6, "qual_A": 6 * 30 * 61, "qSum": 180},
+                        24: {"count_G": 6, "qual_G": 0, "qSum": 0, "mqSum": 366}})
+
+
+def _n_del_reads():
+    # 21M1D19M at 1 deletes the A of CCAGG (locus 22); C at 21 != A: no slide
+    seq = REF_N[0:21] + REF_N[22:41]
+    return [read(1, "21M1D19M", seq, [30] * 40) for _ in range(6)]
+
+
+def case_lr_nanopore_deletion():
+    """PileUpRegion.scala:180-182: homoRun(index(22)) is 1 but
+    nanoporeExclude(21) holds, so a nanopore deletion at 22 is dropped."""
+    reads = _n_del_reads()
+    return dict(ref=REF_N, start=1, stop=43, reads=reads, sets=[(reads, "nanopore")], cfg=LR,
+                expect={22: {"deletions": 0, "delQual": 0, "mqSum": 0, "qSum": 0}})
+
+
+def case_lr_pacbio_deletion():
+    """The same deletion from a pacbio BAM is kept with indelMq 8: delQual
+    and mqSum 6*9, qSum 6*30."""
+    reads = _n_del_reads()
+    return dict(ref=REF_N, start=1, stop=43, reads=reads, sets=[(reads, "pacbio")], cfg=LR,
+                expect={22: {"deletions": 6, "delQual": 54, "mqSum": 54, "qSum": 180}})
+
+
+def _snp_reads(ref, locus, base):
+    seq = ref[0:locus - 1] + base + ref[locus:40]
+    return [read(1, "40M", seq, [30] * 40) for _ in range(6)]
+
+
+def case_lr_snp_homopolymer_excluded():
+    """GenomeRegion.scala:219,326,432-439: with only long-read BAMs, a SNP at
+    20, the first T of TTTT, is called but takes no fix and no count."""
+    reads = _snp_reads(REF_T, 20, "C")
+    return dict(ref=REF_T, start=1, stop=42, reads=reads, sets=[(reads, "pacbio")], cfg=LR,
+                expect={20: {"count_C": 6}}, stats={"snps": 0}, changes="", bases=REF_T, disp={20: 2})
+
+
+def case_lr_snp_homopolymer_with_frags():
+    """The same with a frags BAM beside the pacbio one (fragBams non-empty,
+    GenomeRegion.scala:147): the SNP is fixed."""
+    reads = _snp_reads(REF_T, 20, "C")
+    return dict(ref=REF_T, start=1, stop=42, reads=reads, sets=[(reads, "pacbio"), ([], "frags")], cfg=LR,
+                expect={20: {"count_C": 6}}, stats={"snps": 1}, changes="ctg:20 ctg_pilon:20 T C\n",
+                bases=REF_T[:19] + "C" + REF_T[20:])
+
+
+def case_lr_snp_nanopore_motif():
+    """A SNP at 22, the middle of CCAGG, from a pacbio BAM beside an empty
+    nanopore BAM: excludeMotifs' nanoporeExclude(22) holds, so no fix."""
+    reads = _snp_reads(REF_N, 22, "T")
+    return dict(ref=REF_N, start=1, stop=43, reads=reads, sets=[(reads, "pacbio"), ([], "nanopore")], cfg=LR,
+                expect={22: {"count_T": 6}}, stats={"snps": 0}, changes="", bases=REF_N)
+
+
+def case_lr_snp_pacbio_motif():
+    """The same pacbio reads alone: nothing excludes 22, the SNP is fixed."""
+    reads = _snp_reads(REF_N, 22, "T")
+    return dict(ref=REF_N, start=1, stop=43, reads=reads, sets=[(reads, "pacbio")], cfg=LR,
+                expect={22: {"count_T": 6}}, stats={"snps": 1}, changes="ctg:22 ctg_pilon:22 A T\n",
+                bases=REF_N[:21] + "T" + REF_N[22:])
+
+
+LONG = [case_lr_insertion_pacbio, case_lr_insertion_homopolymer5, case_lr_insertion_homopolymer5_frags,
+        case_lr_insertion_homopolymer4, case_lr_deletion_homopolymer, case_lr_deletion_homopolymer_region5,
+        case_lr_nanopore_qual, case_lr_nanopore_qual_region3, case_lr_nanopore_deletion, case_lr_pacbio_deletion,
+        case_lr_snp_homopolymer_excluded, case_lr_snp_homopolymer_with_frags, case_lr_snp_nanopore_motif,
+        case_lr_snp_pacbio_motif]
+
 
 _batches = {}
 
@@ -299,3 +484,10 @@ def batch_of(case):
     if key not in _batches:
         _batches[key] = pack_reads(case["reads"])
     return _batches[key]
+
+
+def sets_of(case):
+    """[(batch, BAM type name)] of the case: its "sets", else one frags BAM."""
+    if "sets" not in case:
+        return [(batch_of(case), "frags")]
+    return [(pack_reads(reads), t) for reads, t in case["sets"]]

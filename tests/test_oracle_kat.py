@@ -36,11 +36,13 @@ def check_case(res, case):
         assert bool(int(res["calls"][i]["flags"]) & PL_CF_WIDE) == (start + i in wide), start + i
 
 
-@pytest.mark.parametrize("make", K.ALL, ids=[f.__name__ for f in K.ALL])
+@pytest.mark.parametrize("make", K.ALL + K.LONG, ids=[f.__name__ for f in K.ALL + K.LONG])
 def test_kat_oracle(make):
+    from pilon_amd.abi import BAM_TYPES
     case = make()
+    sets = K.sets_of(case)
     res = oracle().run_region(default_config(**case.get("cfg", {})), case["ref"], case["start"], case["stop"],
-                              [K.batch_of(case)])
+                              [b for b, _ in sets], set_types=[BAM_TYPES[t] for _, t in sets])
     check_case(res, case)
 
 
