@@ -168,6 +168,14 @@ pl_status pl_ref_upload(pl_ctx* ctx, int32_t contig_id, const char* name,
  * an id usable by pl_region_use_readset and by the batch/serving forms. */
 pl_status pl_readset_upload(pl_ctx* ctx, int32_t contig_id,
                             const pl_read_soa* reads, int32_t* readset_id);
+/* The same for a set that is one BAM of a given PL_BAM_* type (below):
+ * frags, unpaired, nanopore or pacbio (long-read rules as for
+ * pl_region_add_reads_typed; a region of a nanopore or pacbio set has no
+ * frags BAM, so excludeMotifs applies).  A jumps BAM is refused here
+ * (PL_E_INVAL): it only matters beside other BAMs of a region, through
+ * pl_region_add_reads_typed.  pl_readset_upload is PL_BAM_FRAGS. */
+pl_status pl_readset_upload_typed(pl_ctx* ctx, int32_t contig_id, const pl_read_soa* reads,
+                                  int32_t bam_type, int32_t* readset_id);
 
 /* --------------------------------------------------- plain region form */
 /* new PileUpRegion(name, start, stop): 1-based inclusive, stop <= len.   */
@@ -178,13 +186,23 @@ pl_status pl_region_open(pl_ctx* ctx, int32_t contig_id, int32_t start,
  * each with addRead.  Callable repeatedly (one call per BAM or batch). */
 pl_status pl_region_add_reads(pl_region* reg, const pl_read_soa* reads);
 /* The same for one BAM of a given type (Pilon's --frags, --jumps,
- * --unpaired): reads of every type are piled alike; the depth that "jumps"
- * BAMs add is left out of fragment coverage, and so of copy number, the
- * delta/dip tracks and their Pilon.bed and VCF regions (processBam,
- * GenomeRegion.scala:287-300).  pl_region_add_reads is PL_BAM_FRAGS. */
+ * --unpaired, --nanopore, --pacbio; pl_region_add_reads is PL_BAM_FRAGS).
+ * The depth that "jumps" BAMs add is left out of fragment coverage, and so
+ * of copy number, the delta/dip tracks and their Pilon.bed and VCF regions
+ * (processBam, GenomeRegion.scala:287-300).  Nanopore and pacbio BAMs are
+ * unpaired BAMs of longReadType 1 and 2 (BamFile.scala:43-47): their reads
+ * pile with addRead's long-read rules (indelMq capped at 8, insertions and
+ * deletions dropped at homopolymers, nanopore CC_GG bases at quality 0;
+ * PileUpRegion.scala:120-190) and need config.longread.  When a region has
+ * no frags BAM and config.longread is set, excludeMotifs keeps changes at
+ * homopolymer starts (and, with a nanopore BAM, CC_GG middles) out of the
+ * fixes and counts (GenomeRegion.scala:219,326,432-439).  Every call adds
+ * its BAM's type to the region, an empty one included. */
 #define PL_BAM_FRAGS    0
 #define PL_BAM_JUMPS    1
 #define PL_BAM_UNPAIRED 2
+#define PL_BAM_NANOPORE 3
+#define PL_BAM_PACBIO   4
 pl_status pl_region_add_reads_typed(pl_region* reg, const pl_read_soa* reads, int32_t bam_type);
 /* Same, for a resident read set. */
 pl_status pl_region_use_readset(pl_region* reg, int32_t readset_id);

@@ -316,13 +316,14 @@ def parse_targets(s, contigs):
     return parse_target_string(s, contigs)
 
 
-def process_regions_output(cfg, contigs, regions, readsets_of, element=None):
+def process_regions_output(cfg, contigs, regions, readsets_of, element=None, set_types=None):
     """processRegions' output loop (GenomeFile.scala:135-162) over regions
     [(contig index, start, stop)] with contigs [(name, bases)]: each region
     through run_region with its element's newName and the offset of the
     element's earlier regions (:149-153); returns (.changes text, FASTA
     text).  readsets_of(i) gives region i's read batches; element (one id
-    per region, None = the contig) groups consecutive regions."""
+    per region, None = the contig) groups consecutive regions; set_types
+    are the BAM types of every region's read sets (None: frags)."""
     changes, fasta = [], []
     i = 0
     while i < len(regions):
@@ -337,7 +338,8 @@ def process_regions_output(cfg, contigs, regions, readsets_of, element=None):
         seq = []
         for k in range(i, j):
             _, s, e = regions[k]
-            r = run_region(cfg, bases, s, e, readsets_of(k), name=name, new_name=new_name, offset=offset)
+            r = run_region(cfg, bases, s, e, readsets_of(k), name=name, new_name=new_name, offset=offset,
+                           set_types=set_types)
             if r["status"] != 0:
                 raise ValueError(f"region {k}: status {r['status']}")
             changes.append(r["changes"])

@@ -63,7 +63,9 @@ extern "C" {
 void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const char* name,
              const pl_read_soa* s, int32_t n_jobs, const int32_t* starts, const int32_t* stops,
              int32_t W, int32_t* pile_out, pl_call* calls_out, pl_region_stats* stats_out,
-             int32_t* errs) {
+             int32_t* errs, const uint8_t* lrt, int32_t set_kinds) {
+  // lrt: longReadType per read (null: none); set_kinds bit 0: a frags BAM
+  // is among the set's, bit 1: a nanopore BAM is
   KConf cf;
   cf.flank = pc->flank; cf.min_mq = pc->min_mq; cf.min_qual = pc->min_qual;
   cf.default_qual = pc->default_qual; cf.min_min_depth = pc->min_min_depth;
@@ -76,7 +78,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
   R.n = n; R.pos = s->pos; R.tlen = s->tlen; R.flag = s->flag; R.mapq = s->mapq;
   R.cigar_off = s->cigar_off; R.cigar = s->cigar; R.seq_off = s->seq_off;
   R.bases2 = s->bases2; R.quals = s->quals; R.n_exotic = s->n_exotic;
-  R.exo_read = s->exo_read; R.exo_off = s->exo_off; R.nib = s->nib;
+  R.exo_read = s->exo_read; R.exo_off = s->exo_off; R.nib = s->nib; R.lrt = lrt;
   R.meta = meta.data(); R.ref = ref; R.ref_len = ref_len;
   // ---- read-set prep (k_readset_prep)
   int32_t max_back = 0, max_lead = 0, max_back_fp = 0;
@@ -86,7 +88,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
   std::vector<Seg> segs;
   for (int64_t r = 0; r < n; r++) {
     RMeta& m = meta[r];
-    if (!((m.info & RI_KEEP) && !(m.info & (RI_SIMPLE | RI_EXOTIC | RI_BAD)))) continue;
+    if (!((m.info & RI_KEEP) && !(m.info & (RI_SIMPLE | RI_EXOTIC | RI_BAD | RI_LR_MASK)))) continue;
     const int c = build_segs(R, r, m.info, cf, nullptr);
     const size_t o = segs.size();
     segs.resize(o + (size_t)c);
@@ -325,6 +327,7 @@ void* ch_run(const pl_config* pc, const uint8_t* ref, int64_t ref_len, const cha
       }
       ch.push_back(x);
     }
+    if (pc->longread && !(set_kinds & 1)) plh::exclude_motifs(ch, ref, J.start, J.stop, (set_kinds & 2) != 0);
     std::string nm(name);
     std::string rb((const char*)ref + J.start - 1, (size_t)J.size);
     plh::FixResult fr = plh::fix_region(*pc, nm, J.start, rb, ch, nm + "_pilon", 0);

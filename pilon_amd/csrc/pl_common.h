@@ -20,9 +20,13 @@
 #define RI_UNK_SHIFT    16           // number of 'P' ops, saturated at 255
 #define RI_UNK_MASK     0x00ff0000u
 #define RI_SIMPLE       0x01000000u  // one M/=/X op covering all of SEQ, 2-bit bases
-#define RI_SEG_SHIFT    25           // complex reads: number of segments (RMeta.op0 = first)
-#define RI_SEG_MASK     0xfe000000u
-#define RI_SEG_MAX      127          // more segments: the read is walked instead
+#define RI_LR_SHIFT     25           // longReadType of the read's BAM (BamFile.scala:43-47): 0, 1 nanopore, 2 pacbio
+#define RI_LR_MASK      0x06000000u
+#define RI_SEG_SHIFT    27           // complex reads: number of segments (RMeta.op0 = first)
+#define RI_SEG_MASK     0xf8000000u
+#define RI_SEG_MAX      31           // more segments, or a long read: the read is walked instead
+#define PL_LR_NANOPORE  1
+#define PL_LR_PACBIO    2
 
 // Margins beyond which a read is an outlier and is handled from the per-set
 // outlier list instead of the sorted scan.
@@ -68,6 +72,7 @@ struct RSDev {               // one resident read set (one contig, sorted by pos
   const int64_t* exo_read;
   const int64_t* exo_off;
   const uint8_t* nib;
+  const uint8_t* lrt;        // [n] longReadType per read (null: all 0)
   RMeta* meta;               // [n] derived per-read facts
   const struct Seg* segs;    // segment table of the complex reads
   const uint8_t* ref;        // contig bytes

@@ -188,8 +188,12 @@ PL_HD PrepOut prep_read(const RSDev& R, int64_t r, const KConf& cf) {
   if (bad) info |= RI_BAD;
   if (L == 0 && hasS) info |= RI_EDGE_CLIP;
   info |= (uint32_t)(nP > 255 ? 255 : nP) << RI_UNK_SHIFT;
+  // a long read is always walked (its bases and indels depend on the
+  // reference around them): never simple, and too many segments
+  const uint32_t lr = R.lrt ? (uint32_t)R.lrt[r] : 0u;
+  if (lr) info |= (lr << RI_LR_SHIFT) | ((uint32_t)RI_SEG_MAX << RI_SEG_SHIFT);
   const uint32_t op0 = c1 > c0 ? R.cigar[c0] : 0u;
-  if (c1 - c0 == 1 && L > 0 && !bad && !(flag & PL_RF_EXOTIC)) {
+  if (c1 - c0 == 1 && L > 0 && !bad && !(flag & PL_RF_EXOTIC) && !lr) {
     const int op = (int)(op0 & 15);
     if (op == 0 || op == 7 || op == 8) info |= RI_SIMPLE;
   }
@@ -243,13 +247,35 @@ PL_HD PrepOut prep_read(const RSDev& R, int64_t r, const KConf& cf) {
 }
 
 // ------------------------------------------------------------ the read walk
+// addRead's homoRun(i0) >= 4 (PileUpRegion.scala:120-126): the contig byte
+// at 0-based index i0 (0 <= i0 < ref_len) and the three after it are equal.
+PL_HD bool homo_run4(const RSDev& R, int64_t i0) {
+  if (i0 + 3 >= R.ref_len) return false;
+  const uint8_t b = R.ref[i0];
+  return R.ref[i0 + 1] == b && R.ref[i0 + 2] == b && R.ref[i0 + 3] == b;
+}
+
+// addRead's nanoporeExclude(i0) (PileUpRegion.scala:128-134): i0 is a region
+// index (locus - start) but indexes the contig's bytes; the region test is
+// on locus(i0 -+ 2).
+PL_HD bool nanopore_exclude(const RSDev& R, const JobDev& J, int64_t i0) {
+  if (i0 < 2 || (int64_t)J.start + i0 + 2 > J.stop) return false;
+  return R.ref[i0 - 2] == 'C' && R.ref[i0 - 1] == 'C' && R.ref[i0 + 1] == 'G' && R.ref[i0 + 2] == 'G';
+}
+
 // Insertion event of one I op (PileUpRegion.scala:150-162): gate, left
 // normalisation, then addInsertion at the final locus if it lies in the window.
+// A long read's insertion (lr > 0) is dropped where homoRun(iloc) >= 4, and
+// homoRun reading past the contig's end throws (:160).
 template <class Sink>
 PL_HD void ins_event(const RSDev& R, const JobDev& J, const ReadAcc& a, int64_t r, int32_t locus, int32_t readOffset,
-                     int32_t len, int32_t mq1, int32_t L0, int32_t L1, Sink& s) {
+                     int32_t len, int32_t mq1, int32_t L0, int32_t L1, Sink& s, uint32_t lr = 0) {
   int32_t rot;
   const int32_t il = slide_insertion(R.ref, a, readOffset, len, locus, &rot);
+  if (lr) {
+    if ((int64_t)il >= R.ref_len) { s.edge(r); return; }
+    if (homo_run4(R, il)) return;
+  }
   if (il < J.start) { s.edge(r); return; }
   if (il >= L0 && il < L1) {
     uint32_t hasN;
@@ -265,10 +291,15 @@ PL_HD void ins_event(const RSDev& R, const JobDev& J, const ReadAcc& a, int64_t 
 
 // Deletion event of one D op (PileUpRegion.scala:163-183): left normalisation
 // with the double-counting add of each slid-over base (remove() is a no-op for
-// valid reads), then addDeletion at the final locus.
+// valid reads), then addDeletion at the final locus.  mq1 is adjMq + 1 for
+// the slid-over bases, imq1 indelMq + 1 for the deletion; a long read's
+// deletion is dropped where homoRun(index(dloc)) >= 4 or, for nanopore,
+// nanoporeExclude(index(dloc)) holds (:180-182).
 template <class Sink>
 PL_HD void del_event(const RSDev& R, const JobDev& J, const KConf& cf, const ReadAcc& a, int64_t r, int32_t locus,
-                     int32_t readOffset, int32_t len, int32_t L, int32_t mq1, int32_t L0, int32_t L1, Sink& s) {
+                     int32_t readOffset, int32_t len, int32_t L, int32_t mq1, int32_t L0, int32_t L1, Sink& s,
+                     uint32_t lr = 0, int32_t imq1 = 0) {
+  if (!lr) imq1 = mq1;
   int32_t dl = locus, rl = readOffset;
   const int32_t flank = cf.flank;
   while (dl > 1 && rl > 0 && R.ref[dl - 2] == R.ref[dl + len - 2]) {
@@ -285,15 +316,19 @@ PL_HD void del_event(const RSDev& R, const JobDev& J, const KConf& cf, const Rea
     }
   }
   if (dl < J.start) { s.edge(r); return; }
+  if (lr) {
+    const int64_t i0 = (int64_t)dl - J.start;
+    if (homo_run4(R, i0) || (lr == PL_LR_NANOPORE && nanopore_exclude(R, J, i0))) return;
+  }
   if (dl >= L0 && dl < L1) {
     uint32_t hasN;
     const uint32_t h = del_hash(R.ref, (int64_t)dl - 1, len, &hasN);
     IndelRec e;
     e.key = (uint32_t)(((uint64_t)(J.out + (dl - J.start)) << 1) | 1u);
-    e.len = len; e.qual = (int16_t)a.qual(readOffset); e.mq1 = (uint16_t)mq1;
+    e.len = len; e.qual = (int16_t)a.qual(readOffset); e.mq1 = (uint16_t)imq1;
     e.hash = h; e.rot = 0; e.flags = hasN ? 2u : 0u;
     e.src = (int64_t)dl - 1;
-    s.del(dl - L0, a.qual(readOffset), mq1, e);
+    s.del(dl - L0, a.qual(readOffset), imq1, e);
   }
 }
 
@@ -376,6 +411,8 @@ PL_HD void walk_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r
   const bool valid = (info & RI_VALID) != 0;
   const int32_t adjMq = (int32_t)(info & RI_ADJMQ_MASK);
   const int32_t mq1 = adjMq + 1;
+  const uint32_t lr = (info & RI_LR_MASK) >> RI_LR_SHIFT;
+  const int32_t imq1 = (lr && adjMq > 8 ? 8 : adjMq) + 1;      // indelMq (PileUpRegion.scala:142)
   const int32_t L = (int32_t)(R.seq_off[r + 1] - R.seq_off[r]);
   const int32_t flank = cf.flank;
   const ReadAcc a = read_acc(R, r, info, cf.default_qual);
@@ -400,7 +437,8 @@ PL_HD void walk_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r
         const int32_t p = locus + i - L0;
         if (valid) {
           const int b = a.bi(o);
-          const int32_t q = a.qual(o);
+          // a nanopore base inside CC_GG piles with quality 0 (:190)
+          const int32_t q = lr == PL_LR_NANOPORE && nanopore_exclude(R, J, (int64_t)locus + i - J.start) ? 0 : a.qual(o);
           if (b >= 0 && q >= cf.min_qual) s.base(p, b, q, mq1);   // PileUp.add (PileUp.scala:75-84)
           s.count_base();                                           // region baseCount (:42)
         } else {
@@ -421,12 +459,12 @@ PL_HD void walk_read(const RSDev& R, const JobDev& J, const KConf& cf, int64_t r
     } else if (op == 1) {                                      // I (:150-162)
       const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
       if (lane == 0 && valid && tr && locus >= J.start && locus <= J.stop)
-        ins_event(R, J, a, r, locus, readOffset, len, mq1, L0, L1, s);
+        ins_event(R, J, a, r, locus, readOffset, len, imq1, L0, L1, s, lr);
     } else if (op == 2) {                                      // D (:163-183)
       const bool tr = readOffset >= flank && j_sub(L, flank) > readOffset;
       if (lane == 0 && valid && tr && locus >= J.start && locus <= J.stop &&
           locus + len - 1 >= J.start && locus + len - 1 <= J.stop)
-        del_event(R, J, cf, a, r, locus, readOffset, len, L, mq1, L0, L1, s);
+        del_event(R, J, cf, a, r, locus, readOffset, len, L, mq1, L0, L1, s, lr, imq1);
     }
     if (consumes_read(op)) readOffset += len;
     if (consumes_ref(op)) refOffset += len;

@@ -95,6 +95,9 @@ struct Contig {
 struct ReadSet {
   int32_t contig = -1;
   int64_t n = 0;
+  // the BAM kinds the set holds: a frags BAM (GenomeRegion.fragBams non-empty,
+  // which turns excludeMotifs off) and a nanopore BAM (GenomeRegion.scala:147,434)
+  bool has_frags = true, has_nano = false;
   RSDev d;
   std::vector<void*> mem;
   uint32_t* kp = nullptr;
@@ -329,7 +332,11 @@ int64_t nib_bytes(const pl_read_soa* s) {
   return (mx + 1) / 2 + 1;
 }
 
-pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int32_t* out_id) {
+// lrt: each read's longReadType (null: none); kinds: RS_FRAGS | RS_NANO
+#define RS_FRAGS 1
+#define RS_NANO 2
+pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int32_t* out_id,
+                         const uint8_t* lrt = nullptr, int kinds = RS_FRAGS) {
   Lane* L = c ? &c->main : nullptr;
   auto it = c->contigs.find(contig_id);
   if (it == c->contigs.end()) return fail(PL_E_INVAL, "contig %d not uploaded", contig_id);
@@ -342,6 +349,8 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   std::unique_ptr<ReadSet> rs(new ReadSet());
   rs->contig = contig_id;
   rs->n = n;
+  rs->has_frags = (kinds & RS_FRAGS) != 0;
+  rs->has_nano = (kinds & RS_NANO) != 0;
   RSDev& d = rs->d;
   std::memset(&d, 0, sizeof d);
   d.n = n;
@@ -365,6 +374,7 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   UP(d.exo_read, s->exo_read, 8 * s->n_exotic, 0);
   UP(d.exo_off, s->exo_off, 8 * s->n_exotic, 0);
   UP(d.nib, s->nib, s->n_exotic ? nib_bytes(s) : 0, 64);
+  if (lrt) { UP(d.lrt, lrt, n, 0); }
 #undef UP
   d.n_exotic = s->n_exotic;
   d.ref = it->second.dev;
@@ -858,6 +868,8 @@ pl_status job_changes(pl_ctx* c, Lane* L, size_t j, pl_region_stats* out, std::v
     if (cd.kind == PL_D_INS) x.indel = L->hins_pool.substr((size_t)L->hins_off[k], (size_t)cd.len);
     ch.push_back(x);
   }
+  const ReadSet& rs = *c->rsets[J.rs];
+  if (c->pc.longread && !rs.has_frags) plh::exclude_motifs(ch, ct.host.data(), J.start, J.stop, rs.has_nano);
   if (out) {
     std::memset(out, 0, sizeof *out);
     out->read_count = S.read_count;
@@ -926,8 +938,9 @@ void append_reads(HostReads& h, const pl_read_soa* s) {
   }
 }
 
-// merge the plain-form batches into one coordinate-sorted set and upload it
-pl_status upload_host_batches(pl_ctx* c, int32_t contig, std::vector<HostReads>& bs, int32_t* id) {
+// merge the plain-form batches (of PL_BAM_* types) into one coordinate-sorted set and upload it
+pl_status upload_host_batches(pl_ctx* c, int32_t contig, std::vector<HostReads>& bs, const std::vector<int32_t>& types,
+                              int32_t* id) {
   struct Ref { int32_t b; int64_t r; };
   std::vector<Ref> order;
   for (int32_t b = 0; b < (int32_t)bs.size(); b++)
@@ -940,8 +953,14 @@ pl_status upload_host_batches(pl_ctx* c, int32_t contig, std::vector<HostReads>&
   std::vector<int64_t> cigar_off(n + 1, 0), seq_off(n + 1, 0), exo_read, exo_off;
   std::vector<uint32_t> cigar;
   std::vector<int8_t> quals;
-  std::vector<uint8_t> codes, nibc;
+  std::vector<uint8_t> codes, nibc, lrt(n);
+  bool any_lr = false;
+  int kinds = 0;
+  for (int32_t t : types) kinds |= (t == PL_BAM_FRAGS ? RS_FRAGS : 0) | (t == PL_BAM_NANOPORE ? RS_NANO : 0);
   for (int64_t i = 0; i < n; i++) {
+    const int32_t t = types[order[i].b];
+    lrt[i] = t == PL_BAM_NANOPORE ? PL_LR_NANOPORE : t == PL_BAM_PACBIO ? PL_LR_PACBIO : 0;
+    any_lr = any_lr || lrt[i];
     const HostReads& h = bs[order[i].b];
     const int64_t r = order[i].r;
     pos[i] = h.pos[r]; tlen[i] = h.tlen[r]; flag[i] = h.flag[r]; mapq[i] = h.mapq[r];
@@ -968,7 +987,7 @@ pl_status upload_host_batches(pl_ctx* c, int32_t contig, std::vector<HostReads>&
   s.cigar_off = cigar_off.data(); s.cigar = cigar.data(); s.seq_off = seq_off.data();
   s.bases2 = bases2.data(); s.quals = quals.data();
   s.n_exotic = (int64_t)exo_read.size(); s.exo_read = exo_read.data(); s.exo_off = exo_off.data(); s.nib = nib.data();
-  return readset_upload(c, contig, &s, id);
+  return readset_upload(c, contig, &s, id, any_lr ? lrt.data() : nullptr, kinds);
 }
 
 }  // namespace
@@ -986,7 +1005,6 @@ const char* pl_last_error(void) { return g_err.c_str(); }
 
 pl_status pl_ctx_create(const pl_config* cfg, int device, pl_ctx** out) {
   if (!cfg || !out) return fail(PL_E_INVAL, "null argument");
-  if (cfg->longread) return fail(PL_E_INVAL, "long-read BAMs (addRead longRead > 0) are not supported");
   if (cfg->flank < 0) return fail(PL_E_INVAL, "flank must be >= 0");
   if (cfg->default_qual < -128 || cfg->default_qual > 127) return fail(PL_E_INVAL, "defaultQual is a Byte");
   if (cfg->min_min_depth < 0) return fail(PL_E_INVAL, "minMinDepth must be >= 0");
@@ -1045,11 +1063,25 @@ pl_status pl_ref_upload(pl_ctx* c, int32_t contig_id, const char* name, const ui
   return PL_OK;
 }
 
-pl_status pl_readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* reads, int32_t* id) {
+pl_status pl_readset_upload_typed(pl_ctx* c, int32_t contig_id, const pl_read_soa* reads, int32_t bam_type,
+                                  int32_t* id) {
   if (!c || !reads || !id) return fail(PL_E_INVAL, "null argument");
+  if (bam_type != PL_BAM_FRAGS && bam_type != PL_BAM_UNPAIRED && bam_type != PL_BAM_NANOPORE &&
+      bam_type != PL_BAM_PACBIO)
+    return fail(PL_E_INVAL, "bam type %d of a resident read set is not PL_BAM_FRAGS, UNPAIRED, NANOPORE or PACBIO "
+                "(a jumps BAM goes through pl_region_add_reads_typed)", bam_type);
+  const bool lr = bam_type == PL_BAM_NANOPORE || bam_type == PL_BAM_PACBIO;
+  if (lr && !c->pc.longread) return fail(PL_E_INVAL, "a long-read BAM needs config.longread (Pilon.longread)");
   if (c->serving) return fail(PL_E_BUSY, "a server is open on this context: its read sets are fixed");
   HIPCHK(hipSetDevice(c->device));
-  return readset_upload(c, contig_id, reads, id);
+  std::vector<uint8_t> lrt;
+  if (lr) lrt.assign((size_t)std::max<int64_t>(reads->n_reads, 1), bam_type == PL_BAM_NANOPORE ? PL_LR_NANOPORE : PL_LR_PACBIO);
+  return readset_upload(c, contig_id, reads, id, lr ? lrt.data() : nullptr,
+                        (bam_type == PL_BAM_FRAGS ? RS_FRAGS : 0) | (bam_type == PL_BAM_NANOPORE ? RS_NANO : 0));
+}
+
+pl_status pl_readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* reads, int32_t* id) {
+  return pl_readset_upload_typed(c, contig_id, reads, PL_BAM_FRAGS, id);
 }
 
 pl_status pl_region_open(pl_ctx* c, int32_t contig_id, int32_t start, int32_t stop, pl_region** out) {
@@ -1067,16 +1099,17 @@ pl_status pl_region_open(pl_ctx* c, int32_t contig_id, int32_t start, int32_t st
 
 pl_status pl_region_add_reads_typed(pl_region* r, const pl_read_soa* reads, int32_t bam_type) {
   if (!r || !reads) return fail(PL_E_INVAL, "null argument");
-  if (bam_type != PL_BAM_FRAGS && bam_type != PL_BAM_JUMPS && bam_type != PL_BAM_UNPAIRED)
-    return fail(PL_E_INVAL, "bam type %d is not PL_BAM_FRAGS, PL_BAM_JUMPS or PL_BAM_UNPAIRED", bam_type);
+  if (bam_type < PL_BAM_FRAGS || bam_type > PL_BAM_PACBIO)
+    return fail(PL_E_INVAL, "bam type %d is not a PL_BAM_* type", bam_type);
+  if ((bam_type == PL_BAM_NANOPORE || bam_type == PL_BAM_PACBIO) && !r->ctx->pc.longread)
+    return fail(PL_E_INVAL, "a long-read BAM needs config.longread (Pilon.longread)");
   if (r->finished) return fail(PL_E_INVAL, "region already finished");
   if (r->readset >= 0) return fail(PL_E_INVAL, "region already uses a resident read set");
-  if (reads->n_reads == 0) return PL_OK;
   for (int64_t i = 1; i < reads->n_reads; i++)
     if (reads->pos[i] < reads->pos[i - 1]) return fail(PL_E_INVAL, "reads must be coordinate-sorted within a call");
   r->batches.emplace_back();   // one batch per BAM; merged by position at finish
   r->types.push_back(bam_type);
-  append_reads(r->batches.back(), reads);
+  if (reads->n_reads > 0) append_reads(r->batches.back(), reads);
   return PL_OK;
 }
 
@@ -1108,15 +1141,17 @@ pl_status pl_region_finish(pl_region* r, int32_t* pile, pl_call* calls, pl_regio
     if (jumps) {
       // the jumps BAMs alone, piled beside the region for fragment coverage
       std::vector<HostReads> jb;
-      for (size_t k = 0; k < r->batches.size(); k++) if (r->types[k] == PL_BAM_JUMPS) jb.push_back(r->batches[k]);
+      std::vector<int32_t> jt;
+      for (size_t k = 0; k < r->batches.size(); k++)
+        if (r->types[k] == PL_BAM_JUMPS) { jb.push_back(r->batches[k]); jt.push_back(PL_BAM_JUMPS); }
       int32_t jid = -1;
-      pl_status s = upload_host_batches(c, r->contig, jb, &jid);
+      pl_status s = upload_host_batches(c, r->contig, jb, jt, &jid);
       if (s) return s;
       r->jtemp = std::move(c->rsets[jid]);
       r->jtemp_id = jid;
       c->rs_version++;
     }
-    pl_status s = upload_host_batches(c, r->contig, r->batches, &id);
+    pl_status s = upload_host_batches(c, r->contig, r->batches, r->types, &id);
     if (s) return s;
     temp = true;
     r->batches.clear();

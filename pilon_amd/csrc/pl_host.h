@@ -34,6 +34,26 @@ struct Change {
 
 struct Fix { int32_t loc; std::string was, patch; };
 
+// GenomeRegion.excludeMotifs (GenomeRegion.scala:219,416-439): when every BAM
+// of a long-read run is a non-frags one (longReadOnly, :147), identifyAndFixIssues
+// neither fixes nor counts a change (:326) at a locus that starts a run of 4
+// equal upper-cased bases within the region, or, with a nanopore BAM, that
+// sits in the middle of CC_GG.  contig holds the whole contig (0-based).
+inline void exclude_motifs(std::vector<Change>& ch, const uint8_t* contig, int32_t start, int32_t stop, bool nano) {
+  auto at = [&](int64_t loc) { const uint8_t c = contig[loc - 1]; return (c >= 'a' && c <= 'z') ? (uint8_t)(c - 32) : c; };
+  std::vector<Change> keep;
+  keep.reserve(ch.size());
+  for (Change& x : ch) {
+    const int64_t loc = x.loc;
+    int64_t run = 1;                                         // homoRun(loc), bounded by the region's stop
+    while (loc + run <= stop && run < 4 && at(loc + run) == at(loc)) run++;
+    const bool motif = nano && loc - 2 >= start && loc + 2 <= stop && at(loc - 2) == 'C' && at(loc - 1) == 'C' &&
+                       at(loc + 1) == 'G' && at(loc + 2) == 'G';
+    if (run < 4 && !motif) keep.push_back(std::move(x));
+  }
+  ch.swap(keep);
+}
+
 inline char iupac(int b1, int b2) {               // Bases.toIUPAC (Bases.scala:85-88)
   static const char T[16] = {0, 'A', 'C', 'M', 'G', 'R', 'S', 'V', 'T', 'W', 'Y', 'H', 'K', 'D', 'B', 'N'};
   return T[(1 << b1) | (1 << b2)];

@@ -69,7 +69,7 @@ __device__ __forceinline__ T wave_sum(T v) {
 // ------------------------------------------------------------------ k_prep
 // segments are built for multi-op reads (not RI_SIMPLE) that addRead processes
 PL_HD bool wants_segs(uint32_t info) {
-  return (info & RI_KEEP) && !(info & (RI_SIMPLE | RI_EXOTIC | RI_BAD));
+  return (info & RI_KEEP) && !(info & (RI_SIMPLE | RI_EXOTIC | RI_BAD | RI_LR_MASK));
 }
 
 __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount, int32_t* scal, int64_t* outliers,

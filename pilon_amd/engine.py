@@ -145,6 +145,7 @@ def lib():
             "pl_ctx_destroy": (None, [C.c_void_p]),
             "pl_ref_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_char_p, C.c_int64]),
             "pl_readset_upload": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa), P(C.c_int32)]),
+            "pl_readset_upload_typed": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa), C.c_int32, P(C.c_int32)]),
             "pl_region_open": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_void_p)]),
             "pl_region_add_reads": (C.c_int, [C.c_void_p, P(PlReadSoa)]),
             "pl_region_add_reads_typed": (C.c_int, [C.c_void_p, P(PlReadSoa), C.c_int32]),
@@ -229,11 +230,12 @@ class Context:
         _check(lib().pl_ref_upload(self.h, contig_id, name.encode(), bases, len(bases)))
         self.contigs[contig_id] = (name, bases)
 
-    def readset_upload(self, contig_id, reads):
-        """reads: a ReadBatch or a PlReadSoa; returns the resident read-set id."""
+    def readset_upload(self, contig_id, reads, bam_type="frags"):
+        """reads: a ReadBatch or a PlReadSoa of one BAM of bam_type ("frags",
+        "unpaired", "nanopore" or "pacbio"); returns the resident read-set id."""
         soa = reads.soa() if hasattr(reads, "soa") else reads
         rid = C.c_int32()
-        _check(lib().pl_readset_upload(self.h, contig_id, C.byref(soa), C.byref(rid)))
+        _check(lib().pl_readset_upload_typed(self.h, contig_id, C.byref(soa), BAM_TYPES[bam_type], C.byref(rid)))
         return rid.value
 
     def batch_descs(self, regions):
@@ -500,8 +502,9 @@ class PileUpRegion:
 
     def addReads(self, reads, bam_type="frags"):
         """BamFile.process for one BAM's worth of reads (query, validateRead,
-        addRead).  bam_type is the BAM's --frags/--jumps/--unpaired kind:
-        "jumps" reads are left out of fragment coverage (GenomeRegion.scala:287-300)."""
+        addRead).  bam_type is the BAM's kind: "frags", "jumps" (left out of
+        fragment coverage, GenomeRegion.scala:287-300), "unpaired", or the
+        long-read "nanopore" / "pacbio" (PileUpRegion.scala:120-190)."""
         soa = reads.soa() if hasattr(reads, "soa") else reads
         _check(lib().pl_region_add_reads_typed(self.h, C.byref(soa), BAM_TYPES[bam_type]))
 

@@ -18,7 +18,7 @@ def lib():
         L.ch_run.restype = C.c_void_p
         L.ch_run.argtypes = [C.POINTER(PlConfig), C.c_char_p, C.c_int64, C.c_char_p, C.POINTER(PlReadSoa),
                              C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
-                             C.c_void_p, C.c_void_p]
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.ch_text.restype = C.c_int64
         L.ch_text.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_char_p)]
         L.ch_free.argtypes = [C.c_void_p]
@@ -26,8 +26,10 @@ def lib():
     return _lib
 
 
-def run(cfg, contig, batch, regions, W=1024, name="ctg"):
-    """regions: list of (start, stop). Returns list of per-region dicts."""
+def run(cfg, contig, batch, regions, W=1024, name="ctg", lrt=None, set_kinds=1):
+    """regions: list of (start, stop). Returns list of per-region dicts.
+    lrt: longReadType per read (None: none); set_kinds: bit 0 a frags BAM
+    among the set's, bit 1 a nanopore BAM."""
     L = lib()
     starts = np.array([r[0] for r in regions], np.int32)
     stops = np.array([r[1] for r in regions], np.int32)
@@ -37,9 +39,11 @@ def run(cfg, contig, batch, regions, W=1024, name="ctg"):
     stats = (PlRegionStats * len(regions))()
     errs = np.zeros(len(regions), np.int32)
     soa = batch.soa()
+    lr = None if lrt is None else np.ascontiguousarray(lrt, np.uint8)
     h = L.ch_run(C.byref(cfg), bytes(contig), len(contig), name.encode(), C.byref(soa), len(regions),
                  starts.ctypes.data, stops.ctypes.data, W, pile.ctypes.data, calls.ctypes.data,
-                 C.addressof(stats), errs.ctypes.data)
+                 C.addressof(stats), errs.ctypes.data,
+                 None if lr is None else lr.ctypes.data, set_kinds)
     out = []
     off = 0
     for j, (s, e) in enumerate(regions):

@@ -491,3 +491,17 @@ def sets_of(case):
     if "sets" not in case:
         return [(batch_of(case), "frags")]
     return [(pack_reads(reads), t) for reads, t in case["sets"]]
+
+
+LR_TYPE = {"nanopore": 1, "pacbio": 2}
+
+
+def merged_of(case):
+    """The case's BAMs as one coordinate-sorted batch (stable, in set order),
+    with each read's longReadType and the set's kinds (bit 0: a frags BAM,
+    bit 1: a nanopore BAM), as the engine merges a region's BAMs."""
+    sets = case.get("sets", [(case["reads"], "frags")])
+    tagged = [(r, LR_TYPE.get(t, 0)) for reads, t in sets for r in reads]
+    tagged.sort(key=lambda x: x[0]["pos"])
+    kinds = (1 if any(t == "frags" for _, t in sets) else 0) | (2 if any(t == "nanopore" for _, t in sets) else 0)
+    return pack_reads([r for r, _ in tagged], sort=False), [t for _, t in tagged], kinds

@@ -61,3 +61,33 @@ def test_edge_gpu(engine):
     with pytest.raises(PilonError) as e:
         pr.postProcess()
     assert e.value.status == 4
+
+
+@pytest.mark.parametrize("make", K.LONG, ids=[f.__name__ for f in K.LONG])
+def test_kat_long_read_gpu(make):
+    """The long-read cases through pl_region_add_reads_typed, one call per BAM."""
+    from pilon_amd.abi import BAM_TYPES
+    from pilon_amd.engine import Context, PileUpRegion
+    from oracle import oracle as O
+    case = make()
+    print(f"case: {make.__name__}", flush=True)
+    cfg = default_config(**case.get("cfg", {}))
+    sets = K.sets_of(case)
+    ctx = Context(cfg, device=0)
+    try:
+        ctx.ref_upload(100, "ctg", case["ref"].encode())
+        pr = PileUpRegion(ctx, 100, case["start"], case["stop"])
+        for b, t in sets:
+            pr.addReads(b, bam_type=t)
+        pr.postProcess()
+        changes, bases = pr.fix()
+        pr.close()
+    finally:
+        ctx.close()
+    res = {"status": 0, "pile": pr.pile, "calls": pr.calls, "stats": pr.stats, "changes": changes, "bases": bases}
+    check_case(res, case)
+    ref = O.run_region(cfg, case["ref"], case["start"], case["stop"], [b for b, _ in sets], name="ctg",
+                       set_types=[BAM_TYPES[t] for _, t in sets])
+    assert_same_region(pr.pile, pr.calls, ref, make.__name__)
+    assert_same_stats(pr.stats, ref["stats"])
+    assert changes == ref["changes"] and bases == ref["bases"]
