@@ -1,8 +1,9 @@
 // pl_kmer.hip — the host side of the assembler's K-mer graph (included at
 // the end of pl_engine.hip): the reads go to the device, the occurrences
-// are emitted, sorted by K-mer (six stable 32-bit radix sorts, last word
-// first), summed and called per K-mer by the k_kmer_* kernels; the kept
-// K-mers come back in byte order.
+// are emitted and grouped by K-mer (an LSD radix sort by a 64-bit hash of
+// the K-mer; should two distinct K-mers share a hash, six stable 32-bit
+// sorts by the K-mer's own words, last word first), summed and called per
+// K-mer by the k_kmer_* kernels; the kept K-mers come back in byte order.
 
 namespace {
 
@@ -71,6 +72,7 @@ pl_status kmer_graph(pl_ctx* c, const pl_read_soa* R, int32_t K, int32_t min_dep
   const uint32_t nocc = (uint32_t)N;
   uint32_t *perm, *keys, *tk, *tv, *ncount, *head, *seg;
   if ((s = B.get(&A.kw, 4 * (size_t)A.words * (size_t)N)) || (s = B.get(&A.val, 4 * (size_t)N)) ||
+      (s = B.get(&A.hw, 8 * (size_t)N)) || (s = B.get(&A.collide, 4)) ||
       (s = B.get(&perm, 4 * (size_t)N)) || (s = B.get(&keys, 4 * (size_t)N)) || (s = B.get(&tk, 4 * (size_t)N)) ||
       (s = B.get(&tv, 4 * (size_t)N)) || (s = B.get(&ncount, 4)) || (s = B.get(&head, 4 * (size_t)N)) ||
       (s = B.get(&seg, 4 * ((size_t)N + 1))))                // the scan also writes the total at [N]
@@ -79,19 +81,38 @@ pl_status kmer_graph(pl_ctx* c, const pl_read_soa* R, int32_t K, int32_t min_dep
   HIPCHK(ev.create());
   HIPCHK(hipEventRecord(ev.e0, st));
   const unsigned gb = (unsigned)((N + 255) / 256);
-  k_kmer_emit<<<dim3(gb), dim3(256), 0, st>>>(A);
-  k_iota<<<dim3(std::min<unsigned>(gb, 4096)), dim3(256), 0, st>>>(perm, (int32_t)N, ncount);
+  k_kmer_emit<<<dim3((unsigned)n), dim3(256), 0, st>>>(A);
   HIPCHK(hipGetLastError());
-  // least significant word first: each stable sort keeps the order of the later words
-  for (int w = A.words - 1; w >= 0; w--) {
-    k_kmer_gather<<<dim3(gb), dim3(256), 0, st>>>(A.kw + (size_t)w * N, perm, nocc, keys);
-    uint32_t *rk, *rv;
-    if ((s = radix_sort(L, keys, perm, tk, tv, ncount, nocc, 32, &rk, &rv))) return s;
-    if (rv != perm) HIPCHK(hipMemcpyAsync(perm, rv, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
-  }
+  // sort the occurrences (perm) by the 32-bit keys src[0..nw) of each, the
+  // least significant first: each stable sort keeps the order of the later keys
+  auto sort_by = [&](const uint32_t* const* src, int nw) -> pl_status {
+    k_iota<<<dim3(std::min<unsigned>(gb, 4096)), dim3(256), 0, st>>>(perm, (int32_t)N, ncount);
+    for (int w = 0; w < nw; w++) {
+      k_kmer_gather<<<dim3(gb), dim3(256), 0, st>>>(src[w], perm, nocc, keys);
+      uint32_t *rk, *rv;
+      if (pl_status e = radix_sort(L, keys, perm, tk, tv, ncount, nocc, 32, &rk, &rv)) return e;
+      if (rv != perm) HIPCHK(hipMemcpyAsync(perm, rv, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
+    }
+    return PL_OK;
+  };
   A.perm = perm;
   A.head = head;
+  const uint32_t* by_hash[2] = {A.hw, A.hw + (size_t)N};
+  if ((s = sort_by(by_hash, 2))) return s;
+  A.by_hash = 1;
+  HIPCHK(hipMemsetAsync(A.collide, 0, 4, st));
   k_kmer_heads<<<dim3(gb), dim3(256), 0, st>>>(A);
+  uint32_t collide = 0;
+  HIPCHK(hipMemcpyAsync(&collide, A.collide, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (collide) {
+    // two K-mers share a hash: order by the words themselves instead
+    const uint32_t* by_words[PL_KMER_WORDS];
+    for (int w = 0; w < A.words; w++) by_words[w] = A.kw + (size_t)(A.words - 1 - w) * N;
+    if ((s = sort_by(by_words, A.words))) return s;
+    A.by_hash = 0;
+    k_kmer_heads<<<dim3(gb), dim3(256), 0, st>>>(A);
+  }
   if ((s = scan_u32(L, head, seg, N))) return s;
   uint32_t ns = 0;                                          // distinct K-mers
   HIPCHK(hipMemcpyAsync(&ns, seg + N, 4, hipMemcpyDeviceToHost, st));

@@ -6,10 +6,13 @@
 //                 the K-mer as 4-bit BAM codes packed in six 32-bit words
 //                 (first base most significant), and the base after it
 //                 with its quality and the read's MAPQ;
-//   (a stable LSD radix sort of the occurrences by the six words)
+//   (an LSD radix sort of the occurrences by a 64-bit hash of the six
+//   words, two 32-bit keys; the six words themselves only if two distinct
+//   K-mers share a hash)
 //   k_kmer_heads  the first occurrence of each distinct K-mer (then an
 //                 exclusive scan numbers the K-mers, k_kmer_starts lists
-//                 where each begins);
+//                 where each begins), and a flag when equal hashes hold
+//                 different K-mers;
 //   k_kmer_call   one thread per K-mer: PileUp.add of its occurrences and,
 //                 at depth >= minDepth, BaseCall: kGraph's base and, when
 //                 not homozygous, altGraph's.
@@ -30,7 +33,10 @@ struct KmerArgs {
   int32_t K, words;
   int32_t default_qual, min_qual, min_depth;
   uint32_t* kw;                  // [PL_KMER_WORDS][n_occ]
+  uint32_t* hw;                  // [2][n_occ] the K-mer's hash, low word then high word
   uint32_t* val;                 // [n_occ] code | qual << 8 | mapq << 16
+  uint32_t* collide;             // set when sorted neighbours share a hash but not their words
+  int32_t by_hash;               // the order is by hash (heads compare hashes too)
   const uint32_t* perm;          // [n_occ] occurrences in K-mer order
   uint32_t* head;                // [n_occ]
   const uint32_t* seg;           // [n_occ] heads before each sorted occurrence (exclusive scan)
@@ -55,6 +61,18 @@ __device__ __forceinline__ uint32_t kmer_code(const KmerArgs& A, int64_t r, int6
   return 1u << ((A.bases2[k >> 2] >> (2 * (k & 3))) & 3u);
 }
 
+// a 64-bit mix of the K-mer's words (a grouping key: equal words give
+// equal hashes; k_kmer_heads checks that equal hashes hold equal words)
+__device__ __forceinline__ uint64_t kmer_hash(const uint32_t* w, int words) {
+  uint64_t h = 0x9e3779b97f4a7c15ull * (uint64_t)(words + 1);
+  for (int k = 0; k < words; k++) {
+    h = (h ^ w[k]) * 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+  }
+  h *= 0xc4ceb9fe1a85ec53ull;
+  return h ^ (h >> 29);
+}
+
 // Bases.complement (Bases.scala:47-53): A<->T, C<->G, any other code kept
 __device__ __forceinline__ uint32_t kmer_comp(uint32_t c) {
   return c == 1u ? 8u : c == 8u ? 1u : c == 2u ? 4u : c == 4u ? 2u : c;
@@ -66,32 +84,89 @@ __device__ __forceinline__ int32_t kmer_qual(const KmerArgs& A, int64_t r, int64
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_kmer_emit(KmerArgs A) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)A.n_occ) return;
-  int64_t lo = 0, hi = A.n_reads;                          // the read: last occ_off <= t
-  while (hi - lo > 1) {
-    const int64_t m = (lo + hi) >> 1;
-    if (A.occ_off[m] <= t) lo = m; else hi = m;
-  }
-  const int64_t r = lo;
-  const int64_t local = t - A.occ_off[r];
-  const bool rc = local & 1;
-  const int64_t o = local >> 1;
-  const int64_t L = A.seq_off[r + 1] - A.seq_off[r];
-  uint32_t w[PL_KMER_WORDS] = {0, 0, 0, 0, 0, 0};
-  for (int j = 0; j < A.K; j++) {
-    const int64_t p = o + j;
-    const uint32_t c = rc ? kmer_comp(kmer_code(A, r, L - 1 - p)) : kmer_code(A, r, p);
-    w[j >> 3] |= c << (28 - 4 * (j & 7));
-  }
-  const int64_t pn = o + A.K;
-  const uint32_t next = rc ? kmer_comp(kmer_code(A, r, L - 1 - pn)) : kmer_code(A, r, pn);
-  const int32_t qual = rc ? kmer_qual(A, r, L - 1 - pn) : kmer_qual(A, r, pn);
+// the occurrence's outputs: its K-mer words, their hash, the next base
+__device__ __forceinline__ void kmer_store(const KmerArgs& A, int64_t t, const uint32_t* w, uint32_t next,
+                                           int32_t qual, uint32_t mapq) {
 #pragma unroll
   for (int k = 0; k < PL_KMER_WORDS; k++)
     if (k < A.words) A.kw[(int64_t)k * A.n_occ + t] = w[k];
-  A.val[t] = next | ((uint32_t)(uint8_t)(int8_t)qual << 8) | ((uint32_t)A.mapq[r] << 16);
+  const uint64_t h = kmer_hash(w, A.words);
+  A.hw[t] = (uint32_t)h;
+  A.hw[(int64_t)A.n_occ + t] = (uint32_t)(h >> 32);
+  A.val[t] = next | ((uint32_t)(uint8_t)(int8_t)qual << 8) | (mapq << 16);
+}
+
+#define PL_KMER_LDS_BASES 8192   // reads up to this long are staged in LDS
+
+// One workgroup per read: the read's codes, forward and reverse-complemented,
+// packed eight to a 32-bit word (first code most significant) in LDS; each
+// occurrence's K-mer is then six funnel shifts of that stream.  Occurrence
+// 2*o + rc of the read is the K-mer at offset o of the forward (rc = 0) or
+// reverse-complement (rc = 1) sequence (Assembler.addRead, Assembler.scala:65-85).
+__global__ void __launch_bounds__(256) k_kmer_emit(KmerArgs A) {
+  __shared__ uint32_t fw[PL_KMER_LDS_BASES / 8 + 8], rw[PL_KMER_LDS_BASES / 8 + 8];
+  const int64_t r = blockIdx.x;
+  const int64_t t0 = A.occ_off[r], t1 = A.occ_off[r + 1];
+  if (t1 <= t0) return;
+  const int64_t L = A.seq_off[r + 1] - A.seq_off[r];
+  const uint32_t mapq = A.mapq[r];
+  if (L > PL_KMER_LDS_BASES) {
+    // a long read: each occurrence reads its K + 1 codes itself
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+      const int64_t local = t - t0;
+      const bool rc = local & 1;
+      const int64_t o = local >> 1;
+      uint32_t w[PL_KMER_WORDS] = {0, 0, 0, 0, 0, 0};
+      for (int j = 0; j < A.K; j++) {
+        const int64_t p = o + j;
+        const uint32_t c = rc ? kmer_comp(kmer_code(A, r, L - 1 - p)) : kmer_code(A, r, p);
+        w[j >> 3] |= c << (28 - 4 * (j & 7));
+      }
+      const int64_t pn = o + A.K;
+      const uint32_t next = rc ? kmer_comp(kmer_code(A, r, L - 1 - pn)) : kmer_code(A, r, pn);
+      const int32_t qual = rc ? kmer_qual(A, r, L - 1 - pn) : kmer_qual(A, r, pn);
+      kmer_store(A, t, w, next, qual, mapq);
+    }
+    return;
+  }
+  const int nw = (int)((L + 7) / 8);
+  for (int k = threadIdx.x; k < nw + 8; k += blockDim.x) {
+    uint32_t f = 0, b = 0;
+    for (int j = 0; j < 8; j++) {
+      const int64_t p = 8 * (int64_t)k + j;
+      if (p < L) {
+        f |= kmer_code(A, r, p) << (28 - 4 * j);
+        b |= kmer_comp(kmer_code(A, r, L - 1 - p)) << (28 - 4 * j);
+      }
+    }
+    fw[k] = f;
+    rw[k] = b;
+  }
+  __syncthreads();
+  const int K = A.K;
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const int64_t local = t - t0;
+    const bool rc = local & 1;
+    const int o = (int)(local >> 1);
+    const uint32_t* sw = rc ? rw : fw;
+    const int a = o >> 3, sh = 4 * (o & 7);
+    uint32_t w[PL_KMER_WORDS];
+#pragma unroll
+    for (int k = 0; k < PL_KMER_WORDS; k++) {
+      const uint32_t hi = sw[a + k], lo = sw[a + k + 1];
+      w[k] = sh ? (hi << sh) | (lo >> (32 - sh)) : hi;
+    }
+    // codes past the K-th are zero
+#pragma unroll
+    for (int k = 0; k < PL_KMER_WORDS; k++) {
+      const int keep = K - 8 * k;
+      w[k] = keep >= 8 ? w[k] : keep <= 0 ? 0u : (w[k] & ~(0xffffffffu >> (4 * keep)));
+    }
+    const int pn = o + K;
+    const uint32_t next = (sw[pn >> 3] >> (28 - 4 * (pn & 7))) & 15u;
+    const int32_t qual = rc ? kmer_qual(A, r, L - 1 - pn) : kmer_qual(A, r, pn);
+    kmer_store(A, t, w, next, qual, mapq);
+  }
 }
 
 // keys[i] = word k of the i-th occurrence in the current order
@@ -108,6 +183,9 @@ __global__ void __launch_bounds__(256) k_kmer_heads(KmerArgs A) {
     const uint32_t a = A.perm[i], b = A.perm[i - 1];
     h = 0;
     for (int k = 0; k < A.words; k++) h |= A.kw[(int64_t)k * A.n_occ + a] != A.kw[(int64_t)k * A.n_occ + b];
+    // sorted by hash: equal hashes with different words may interleave
+    if (A.by_hash && h && A.hw[a] == A.hw[b] && A.hw[(int64_t)A.n_occ + a] == A.hw[(int64_t)A.n_occ + b])
+      atomicOr(A.collide, 1u);
   }
   A.head[i] = h;
 }

@@ -13,6 +13,7 @@ the HIP library is missing or no GPU is usable, construction raises.
 """
 import ctypes as C
 import os
+import time
 
 import numpy as np
 
@@ -273,7 +274,9 @@ class Context:
         is False."""
         flags = (VCF_DIPLOID if diploid else 0) | (VCF_QE if vcfqe else 0)
         t, n, ms = C.c_void_p(), C.c_int64(), C.c_double()
+        t0 = time.perf_counter()
         _check(lib().pl_batch_vcf(self.h, flags, C.byref(t), C.byref(n), C.byref(ms)))
+        self.last_vcf_wall_ms = (time.perf_counter() - t0) * 1000.0
         try:
             if not decode:
                 buf = np.ctypeslib.as_array(C.cast(t, C.POINTER(C.c_uint8)), shape=(n.value,))
@@ -298,13 +301,16 @@ class Context:
         fasta = _take_text(f.value, fl.value) if want_fasta else None
         return changes, fasta
 
-    def batch_vcf_stream(self, write, diploid=False, vcfqe=False):
+    def batch_vcf_stream(self, write, diploid=False, vcfqe=False, raw=False):
         """batch_vcf's text handed to write(bytes) a piece at a time (a
-        false return stops it); returns (bytes handed over, device_ms)."""
+        false return stops it); returns (bytes handed over, device_ms).  With
+        raw, write(address, length) gets the library's staging memory itself,
+        valid during the call."""
         flags = (VCF_DIPLOID if diploid else 0) | (VCF_QE if vcfqe else 0)
 
         def sink(_user, data, n):
-            return 0 if write(C.string_at(data, n)) is not False else 1
+            r = write(data, n) if raw else write(C.string_at(data, n))
+            return 0 if r is not False else 1
 
         cb = TEXT_SINK(sink)
         n, ms = C.c_int64(), C.c_double()

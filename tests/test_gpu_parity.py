@@ -143,3 +143,59 @@ def test_window_too_big_for_lds(engine):
     with pytest.raises(PilonError) as e:
         engine.batch_run([(6, 1, len(draft), rid)], window=2048, block=1024)
     assert e.value.status == 1
+
+
+@pytest.mark.parametrize("min_qual", [20, -5])
+def test_min_qual_batched(min_qual):
+    """PileUp.add's minQual test (PileUp.scala:77): with minQual 20 about a
+    fifth of the synthetic bases fail it (no count, qualSum, mqSum or qSum);
+    with -5 every base passes and the packed words take the signed path."""
+    from oracle import oracle as O
+    from pilon_amd.engine import Context
+    from pilon_amd.synth import SynthData, preset
+    print(f"case: minQual {min_qual}, c2 x0.04 batched", flush=True)
+    p = preset("c2", scale=0.04)
+    p.update({"err_indel_rate": 0.002})
+    d = SynthData(p)
+    cfg = default_config(min_qual=min_qual)
+    draft, b = d.draft(0), d.batch(0)
+    L = len(draft)
+    ctx = Context(cfg, device=0)
+    try:
+        ctx.ref_upload(0, d.names[0], draft)
+        rid = ctx.readset_upload(0, b)
+        regions = [(1, L // 2), (L // 2 + 1, L)]
+        pile, calls, stats = ctx.batch_run([(0, s, e, rid) for s, e in regions], window=512, block=256, persistent=1)
+    finally:
+        ctx.close()
+    off = 0
+    for k, (s, e) in enumerate(regions):
+        ref = O.run_region(cfg, draft, s, e, [b], name=d.names[0])
+        n = e - s + 1
+        assert_same_region(pile[:, off:off + n], calls[off:off + n], ref, f"minQual {min_qual} {s}-{e}")
+        assert_same_stats(stats[k], ref["stats"])
+        off += n
+
+
+def test_negative_quals(engine):
+    """Quality bytes of 128 and more are negative Java bytes: with the
+    default minQual 0 those bases fail PileUp.add's test while their
+    neighbours pile (the read set takes the tested base loop)."""
+    from oracle import oracle as O
+    from pilon_amd.synth import SynthData, preset
+    print("case: negative quals", flush=True)
+    d = SynthData(preset("c1"))
+    draft, b = d.draft(0), d.batch(0)
+    L = int(b.seq_off[-1])
+    rng = np.random.default_rng(7)
+    q = b.quals.copy()
+    hit = rng.random(L) < 0.02
+    q[:L][hit] = -rng.integers(1, 100, int(hit.sum()))
+    b.quals = q
+    engine.ref_upload(30, d.names[0], draft)
+    rid = engine.readset_upload(30, b)
+    n = len(draft)
+    pile, calls, stats = engine.batch_run([(30, 1, n, rid)], window=512, block=256, persistent=1)
+    ref = O.run_region(engine.config, draft, 1, n, [b], name=d.names[0])
+    assert_same_region(pile, calls, ref, "negative quals")
+    assert_same_stats(stats[0], ref["stats"])
