@@ -64,6 +64,55 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Wave-wide integer scans through DPP lane moves (no LDS round trips): a
+// Hillis-Steele scan inside each row of 16 lanes (row_shr 1, 2, 4, 8, lanes
+// shifted in from outside the row read 0), then the last lane of row 0 added
+// into row 1 and of row 2 into row 3 (row_bcast:15), and the last lane of
+// row 1 into rows 2 and 3 (row_bcast:31).  Integer adds only, so the result
+// equals the shuffle scan's; all 64 lanes must be active.
+template <int CTRL, int ROW = 0xf>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW, 0xf, true);
+}
+template <class T>
+__device__ __forceinline__ T wave_scan_dpp(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit integers");
+  if constexpr (sizeof(T) == 4) {
+    uint32_t x = (uint32_t)v;
+    x += dpp_move<0x111>(x);
+    x += dpp_move<0x112>(x);
+    x += dpp_move<0x114>(x);
+    x += dpp_move<0x118>(x);
+    x += dpp_move<0x142, 0xa>(x);
+    x += dpp_move<0x143, 0xc>(x);
+    return (T)x;
+  } else {
+    uint64_t x = (uint64_t)v;
+    auto mv = [](auto ctrl, auto row, uint64_t y) {
+      return (uint64_t)dpp_move<decltype(ctrl)::value, decltype(row)::value>((uint32_t)y) |
+             ((uint64_t)dpp_move<decltype(ctrl)::value, decltype(row)::value>((uint32_t)(y >> 32)) << 32);
+    };
+    using std::integral_constant;
+    x += mv(integral_constant<int, 0x111>{}, integral_constant<int, 0xf>{}, x);
+    x += mv(integral_constant<int, 0x112>{}, integral_constant<int, 0xf>{}, x);
+    x += mv(integral_constant<int, 0x114>{}, integral_constant<int, 0xf>{}, x);
+    x += mv(integral_constant<int, 0x118>{}, integral_constant<int, 0xf>{}, x);
+    x += mv(integral_constant<int, 0x142>{}, integral_constant<int, 0xa>{}, x);
+    x += mv(integral_constant<int, 0x143>{}, integral_constant<int, 0xc>{}, x);
+    return (T)x;
+  }
+}
+// lane 63's value in every lane (wave-uniform)
+template <class T>
+__device__ __forceinline__ T lane63(T v) {
+  if constexpr (sizeof(T) == 4) return (T)__builtin_amdgcn_readlane((int)v, 63);
+  else
+    return (T)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), 63) << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, 63));
+}
+template <class T>
+__device__ __forceinline__ T wave_total_dpp(T v) { return lane63(wave_scan_dpp(v)); }
+
 }  // namespace
 
 // ------------------------------------------------------------------ k_prep
@@ -514,7 +563,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
       for (uint64_t pend = __ballot(segr); pend;) {
         const bool inp = (pend >> lane) & 1ull;
         const int nsg = inp ? (int)((rinfo & RI_SEG_MASK) >> RI_SEG_SHIFT) : 0;
-        const int sinc = wave_incl_scan(nsg);
+        const int sinc = wave_scan_dpp(nsg);
         const uint64_t take = __ballot(inp && sinc <= 64);
         if (!take) {                         // over 64 segments: walked with the other complex reads
           if (lane == __ffsll((unsigned long long)pend) - 1) cplx = true;
@@ -558,8 +607,8 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
       // are cut in chunks of PL_CHUNK bases; .off is the run's first chunk in
       // the chunk stream, .n its length in bases.
       const uint64_t v = nrun > 0 ? (1ull << 32) + (uint64_t)((ru0.n + PL_CHUNK - 1) / PL_CHUNK) : 0ull;
-      const uint64_t inc = wave_incl_scan(v);
-      const uint64_t tot = __shfl(inc, 63, 64);
+      const uint64_t inc = wave_scan_dpp(v);
+      const uint64_t tot = lane63(inc);
       if (nrun > 0) {
         const uint64_t ex = inc - v;
         wrun[(int32_t)(ex >> 32)] =
@@ -574,8 +623,8 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
         RunL cr = RunL{0, 0u, 0, 0};
         if (lane < ncr) cr = wrun[64 + lane];
         const int32_t cn = lane < ncr ? (cr.off + PL_CHUNK - 1) / PL_CHUNK : 0;
-        const int32_t ci = wave_incl_scan(cn);
-        Cc = __shfl(ci, 63, 64);
+        const int32_t ci = wave_scan_dpp(cn);
+        Cc = lane63(ci);
         if (lane < ncr) wrun[Es + lane] = RunL{Cs + ci - cn, cr.pi, cr.q0, cr.off};
       }
       const int32_t E = Es + ncr;
@@ -655,7 +704,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
           };
           if (mq >= 0) pile(std::integral_constant<bool, true>{});
           else pile(std::integral_constant<bool, false>{});
-          cur = __shfl(rr, 63, 64);
+          cur = lane63(rr);
         }
       }
       // ---- everything else about the chunk's complex reads, one at a time
@@ -677,8 +726,8 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
         const FragTail t = ft[r];
         if (frag_tail_event(t, L0, L1, sk)) { c0++; s0 += (uint32_t)t.size; }
       }
-      c0 = wave_sum(c0);
-      s0 = wave_sum(s0);
+      c0 = wave_total_dpp(c0);
+      s0 = wave_total_dpp(s0);
       if (lane == 0 && c0) sk.frag(0, (int32_t)c0, (int32_t)s0);
     }
     for (int64_t k = wave; k < R.n_outlier; k += NW) {
@@ -700,12 +749,8 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
         if (b0 + i < W) {
           spc = j_add(spc, pcA[b0 + i]); sis = j_add(sis, isA[b0 + i]); sbd = j_add(sbd, bdA[b0 + i]);
         }
-      int32_t ipc = spc, iis = sis, ibd = sbd;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        int32_t o1 = __shfl_up(ipc, d, 64), o2 = __shfl_up(iis, d, 64), o3 = __shfl_up(ibd, d, 64);
-        if (lane >= d) { ipc = j_add(ipc, o1); iis = j_add(iis, o2); ibd = j_add(ibd, o3); }
-      }
+      // (Int adds wrap like Java's: the scan's 32-bit adds are the same)
+      const int32_t ipc = wave_scan_dpp(spc), iis = wave_scan_dpp(sis), ibd = wave_scan_dpp(sbd);
       if (lane == 63) { ws_a[wave] = ipc; ws_b[wave] = iis; ws_c[wave] = ibd; }
       __syncthreads();
       int32_t cpc = j_sub(ipc, spc), cis = j_sub(iis, sis), cbd = j_sub(ibd, sbd);
@@ -768,7 +813,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     // segment of the batch's list, with their exact sums from the LDS words
     if (__syncthreads_or(wmask != 0)) {
       const uint32_t mine = (uint32_t)__popc(wmask);
-      const uint32_t inc = wave_incl_scan(mine);
+      const uint32_t inc = wave_scan_dpp(mine);
       if (lane == 63) ws_a[wave] = (int32_t)inc;
       __syncthreads();
       uint32_t pre = inc - mine, tot = 0;
@@ -797,8 +842,8 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
       A.win_wide[2 * wi + 1] = 0;
     }
     // region baseCount and the aligned-base unit
-    unsigned long long bc = wave_sum((unsigned long long)sk.bcount);
-    unsigned long long al = wave_sum((unsigned long long)sk.aligned_n);
+    unsigned long long bc = wave_total_dpp((unsigned long long)sk.bcount);
+    unsigned long long al = wave_total_dpp((unsigned long long)sk.aligned_n);
     if (lane == 0) { red[0][wave] = bc; red[1][wave] = al; }
     __syncthreads();
     JobStat& S = A.st[win.job];
