@@ -2,10 +2,11 @@
 // (Assembler.addReads / addToPileups / buildGraph, Assembler.scala:46-85,
 // 131-153) on the device.  Included at the end of pl_kernels.hip.
 //
-//   k_kmer_emit   one thread per K-mer occurrence (read, strand, offset):
-//                 the K-mer as 4-bit BAM codes packed in six 32-bit words
-//                 (first base most significant), and the base after it
-//                 with its quality and the read's MAPQ;
+//   k_kmer_emit   one workgroup per read, one thread per K-mer occurrence
+//                 (strand, offset): the K-mer as 4-bit BAM codes packed in
+//                 six 32-bit words (first base most significant), their
+//                 hash, and the base after it with its quality and the
+//                 read's MAPQ;
 //   (an LSD radix sort of the occurrences by a 64-bit hash of the six
 //   words, two 32-bit keys; the six words themselves only if two distinct
 //   K-mers share a hash)
@@ -208,17 +209,33 @@ __global__ void __launch_bounds__(256) k_kmer_call(KmerArgs A) {
     c.mqSum = c.qSum = c.deletions = c.insertions = c.insQual = c.delQual = 0;
     i0 = A.starts[s];
     const uint32_t i1 = A.starts[s + 1];
-    for (uint32_t i = i0; i < i1; i++) {
-      const uint32_t v = A.val[A.perm[i]];
-      const uint32_t code = v & 15u;
-      const int bi = code == 1u ? 0 : code == 2u ? 1 : code == 4u ? 2 : code == 8u ? 3 : -1;
-      const int32_t qual = (int32_t)(int8_t)(uint8_t)(v >> 8);
-      const int32_t mq1 = (int32_t)(v >> 16) + 1;
-      if (bi >= 0 && qual >= A.min_qual) {
-        c.bc[bi] += 1;
-        c.qs[bi] += (int64_t)j_mul(qual, mq1);
-        c.mqSum = j_add(c.mqSum, mq1);
-        c.qSum = j_add(c.qSum, qual);
+    // the occurrences' values come through the permutation: eight
+    // permutation loads, then eight value loads, in flight together
+    for (uint32_t i = i0; i < i1; i += 8) {
+      uint32_t pb[8], vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) pb[u] = i + u < i1 ? A.perm[i + u] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; u++) vb[u] = i + u < i1 ? A.val[pb[u]] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (i + u >= i1) break;
+        const uint32_t v = vb[u];
+        const uint32_t code = v & 15u;
+        const int bi = code == 1u ? 0 : code == 2u ? 1 : code == 4u ? 2 : code == 8u ? 3 : -1;
+        const int32_t qual = (int32_t)(int8_t)(uint8_t)(v >> 8);
+        const int32_t mq1 = (int32_t)(v >> 16) + 1;
+        if (bi >= 0 && qual >= A.min_qual) {
+          // each sum by a constant index, so the column stays in registers
+          const int64_t qm = (int64_t)j_mul(qual, mq1);
+#pragma unroll
+          for (int b = 0; b < 4; b++) {
+            c.bc[b] += bi == b ? 1 : 0;
+            c.qs[b] += bi == b ? qm : 0;
+          }
+          c.mqSum = j_add(c.mqSum, mq1);
+          c.qSum = j_add(c.qSum, qual);
+        }
       }
     }
     keep = c.depth() >= A.min_depth;
