@@ -261,6 +261,48 @@ pl_status pl_batch_run(pl_ctx* ctx, const pl_region_desc* regions, int32_t n,
 pl_status pl_batch_run_noalloc(pl_ctx* ctx, const pl_region_desc* regions, int32_t n, const pl_batch_opts* opts,
                                int32_t* pileup, pl_call* calls, pl_region_stats* stats);
 
+/* --------------------------------------------- local reassembly (f4) */
+/* GapFiller.fillGap / fixBreak (GapFiller.scala:39-262) given the reads the
+ * caller recruited for the break (BamFile.recruitFlankReads and
+ * recruitBadMates stay with the caller: they need read names and the
+ * whole-BAM insert statistics).  The K-mer graph of the reads is built on
+ * the device (Assembler.addReads + buildGraph); Assembler's path search
+ * (Assembler.scala:163-280) and the joins run on the host. */
+typedef struct pl_asm_opts {
+  int32_t k;              /* Assembler.K = 47                                   */
+  int32_t min_depth;      /* Assembler.minDepth = 5                              */
+  int32_t break_radius;   /* GapFiller.breakRadius: max(3K, mean insert size of
+                             the region's frags BAMs, rounded)                  */
+  int32_t tr_safe;        /* Pilon.trSafe = 1                                    */
+  int32_t multi_closure;  /* Pilon.multiClosure = 0                              */
+  int32_t fix_breaks;     /* Pilon.fixBreaks = 0 (--fix breaks)                  */
+  int32_t gap_margin;     /* Pilon.gapMargin = 100000                            */
+  int32_t min_gap;        /* Pilon.minGap = 10                                   */
+} pl_asm_opts;
+void pl_asm_opts_default(pl_asm_opts* opts);
+
+/* The fix GapFiller returns: (start, ref, patch), (0, "", "") for none, and
+ * the tandem repeat its path search noted.  The three strings are
+ * malloc'ed, NUL-terminated; free each with pl_free. */
+typedef struct pl_asm_fix {
+  int32_t start;
+  char* ref;
+  int64_t ref_len;
+  char* patch;
+  int64_t patch_len;
+  char* tandem_repeat;
+} pl_asm_fix;
+
+/* One gap (is_gap = 1, fillGap) or break (fixBreak) [break_start,
+ * break_stop] of region [region_start, region_stop] of a contig.
+ * region_bases: the region's current bases (GenomeRegion.bases, SNP fixes
+ * applied), region_stop - region_start + 1 bytes; contig: the whole contig
+ * as read (contigBases), contig_len bytes.  Returns PL_E_EDGE where Pilon
+ * 1.24 throws (a substring that reaches outside the region). */
+pl_status pl_assemble_break(pl_ctx* ctx, const pl_read_soa* reads, const char* region_bases, const char* contig,
+                            int64_t contig_len, int32_t region_start, int32_t region_stop, int32_t break_start,
+                            int32_t break_stop, int32_t is_gap, const pl_asm_opts* opts, pl_asm_fix* out);
+
 /* Device time of the last pl_batch_run's pipeline in milliseconds, measured
  * with HIP events on the library's stream, plus the share of the window
  * kernel alone. */

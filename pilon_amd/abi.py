@@ -134,3 +134,18 @@ CHUNK_SIZE = 10000000   # Pilon.chunkSize (Pilon.scala:52)
 class PlWideCol(C.Structure):
     """pl_wide_col: a column whose Long qualSum leaves int32."""
     _fields_ = [("pos", C.c_int64), ("qual_sum", C.c_int64 * 4)]
+
+
+class PlAsmOpts(C.Structure):
+    """pl_asm_opts: the knobs of GapFiller.fillGap / fixBreak (Assembler.K,
+    minDepth, breakRadius, and Pilon's trSafe, multiClosure, fixBreaks,
+    gapMargin, minGap)."""
+    _fields_ = [("k", C.c_int32), ("min_depth", C.c_int32), ("break_radius", C.c_int32), ("tr_safe", C.c_int32),
+                ("multi_closure", C.c_int32), ("fix_breaks", C.c_int32), ("gap_margin", C.c_int32),
+                ("min_gap", C.c_int32)]
+
+
+class PlAsmFix(C.Structure):
+    """pl_asm_fix: GapFiller's (start, ref, patch) and the tandem repeat noted."""
+    _fields_ = [("start", C.c_int32), ("ref", C.c_void_p), ("ref_len", C.c_int64), ("patch", C.c_void_p),
+                ("patch_len", C.c_int64), ("tandem_repeat", C.c_void_p)]

@@ -20,8 +20,11 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "pl_kernels.hip"
@@ -1341,4 +1344,5 @@ pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
 #include "pl_track.hip"
 #include "pl_kmer_kernels.hip"
 #include "pl_kmer.hip"
+#include "pl_asm.hip"
 #include "pl_fix.hip"

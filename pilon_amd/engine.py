@@ -19,7 +19,7 @@ import numpy as np
 
 from .abi import (PlConfig, PlReadSoa, PlCall, PlRegionStats, PlRegionDesc, PlBatchOpts, PlBatchInfo, PlServeOpts,
                   PlServeResult, PlTarget, PlWideCol, NFIELDS, F, STATUS_NAMES, call_dtype, default_config, PL_CF_DISP_MASK,
-                  PL_CF_DISP_SHIFT, SERVE_KEEP_PILEUP, SERVE_KEEP_FIX, CHUNK_SIZE, BAM_TYPES)
+                  PL_CF_DISP_SHIFT, SERVE_KEEP_PILEUP, SERVE_KEEP_FIX, CHUNK_SIZE, BAM_TYPES, PlAsmOpts, PlAsmFix)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libpilonhip.so")
@@ -194,6 +194,9 @@ def lib():
                                                C.c_void_p, C.c_void_p]),
             "pl_batch_vcf_stream": (C.c_int, [C.c_void_p, C.c_uint32, TEXT_SINK, C.c_void_p, P(C.c_int64),
                                               P(C.c_double)]),
+            "pl_asm_opts_default": (None, [P(PlAsmOpts)]),
+            "pl_assemble_break": (C.c_int, [C.c_void_p, P(PlReadSoa), C.c_char_p, C.c_char_p, C.c_int64, C.c_int32,
+                                            C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(PlAsmOpts), P(PlAsmFix)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -359,6 +362,40 @@ class Context:
             for p in (k, nx, al):
                 if p.value:
                     lib().pl_free(p)
+
+    def assemble_break(self, reads, bases, contig, start, stop, break_start, break_stop, is_gap, **opts):
+        """GapFiller.fillGap (is_gap) or fixBreak (GapFiller.scala:39-98) of
+        [break_start, break_stop] in region [start, stop], given the reads
+        recruited for it: bases are the region's current bases, contig the
+        contig as read.  Keyword options are pl_asm_opts fields (k,
+        min_depth, break_radius, tr_safe, multi_closure, fix_breaks,
+        gap_margin, min_gap).  Returns ((fix start, ref, patch), tandem
+        repeat); (0, "", "") when there is no fix.  Raises PilonError (EDGE)
+        where Pilon 1.24 throws."""
+        soa = reads.soa() if hasattr(reads, "soa") else reads
+        o = PlAsmOpts()
+        lib().pl_asm_opts_default(C.byref(o))
+        for key, v in opts.items():
+            if key not in dict(PlAsmOpts._fields_):
+                raise TypeError("unknown assembly option %r" % key)
+            if v is not None:
+                setattr(o, key, int(v))
+        b = bases.encode() if isinstance(bases, str) else bytes(bases)
+        g = contig.encode() if isinstance(contig, str) else bytes(contig)
+        if len(b) != stop + 1 - start:
+            raise ValueError("bases hold %d bytes for region %d-%d" % (len(b), start, stop))
+        out = PlAsmFix()
+        _check(lib().pl_assemble_break(self.h, C.byref(soa), b, g, len(g), start, stop, break_start, break_stop,
+                                       int(bool(is_gap)), C.byref(o), C.byref(out)))
+        try:
+            ref = C.string_at(out.ref, out.ref_len).decode()
+            patch = C.string_at(out.patch, out.patch_len).decode()
+            loop = C.string_at(out.tandem_repeat).decode()
+        finally:
+            for p in (out.ref, out.patch, out.tandem_repeat):
+                if p:
+                    lib().pl_free(p)
+        return (out.start, ref, patch), loop
 
     def wide_columns(self):
         """Columns of the last batch whose Long qualSum leaves int32:

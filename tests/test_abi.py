@@ -74,7 +74,7 @@ def test_struct_layouts_match_c(tmp_path):
              "pl_region_stats": abi.PlRegionStats, "pl_region_desc": abi.PlRegionDesc,
              "pl_batch_opts": abi.PlBatchOpts, "pl_batch_info": abi.PlBatchInfo,
              "pl_serve_opts": abi.PlServeOpts, "pl_serve_result": abi.PlServeResult,
-             "pl_bam_stats": PlBamStats}
+             "pl_bam_stats": PlBamStats, "pl_asm_opts": abi.PlAsmOpts, "pl_asm_fix": abi.PlAsmFix}
     for name, cls in sizes.items():
         assert int(out[name]) == C.sizeof(cls), name
     for key, val in out.items():
@@ -82,6 +82,15 @@ def test_struct_layouts_match_c(tmp_path):
             sname, field = key.split(".")
             cls = sizes[sname]
             assert getattr(cls, field).offset == int(val), key
+
+
+def test_asm_opts_default_matches_pilon():
+    """Assembler.K 47, minDepth 5, breakRadius floor 3K, trSafe on,
+    multiClosure and fixBreaks off, gapMargin 100000, minGap 10."""
+    lib = C.CDLL(LIB)
+    o = abi.PlAsmOpts()
+    lib.pl_asm_opts_default(C.byref(o))
+    assert [getattr(o, f) for f, _ in abi.PlAsmOpts._fields_] == [47, 5, 141, 1, 0, 0, 100000, 10]
 
 
 @pytest.mark.parametrize("field,value", [("min_min_depth", -1), ("min_depth", float("nan")),
