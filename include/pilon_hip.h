@@ -278,6 +278,8 @@ typedef struct pl_asm_opts {
   int32_t fix_breaks;     /* Pilon.fixBreaks = 0 (--fix breaks)                  */
   int32_t gap_margin;     /* Pilon.gapMargin = 100000                            */
   int32_t min_gap;        /* Pilon.minGap = 10                                   */
+  int32_t fix_gaps;       /* "gaps" in --fix (default on): fill the gaps         */
+  int32_t fix_local;      /* "local" in --fix (default on): fix possible breaks  */
 } pl_asm_opts;
 void pl_asm_opts_default(pl_asm_opts* opts);
 
@@ -302,6 +304,31 @@ typedef struct pl_asm_fix {
 pl_status pl_assemble_break(pl_ctx* ctx, const pl_read_soa* reads, const char* region_bases, const char* contig,
                             int64_t contig_len, int32_t region_start, int32_t region_stop, int32_t break_start,
                             int32_t break_stop, int32_t is_gap, const pl_asm_opts* opts, pl_asm_fix* out);
+
+/* identifyAndFixIssues' local reassembly for one finished region
+ * (GenomeRegion.scala:382-410) as Pilon runs it with --nostrays: for each
+ * gap (a run of 10 or more N, not within 100 of the region's ends) fillGap,
+ * then for each possible break (GenomeRegion.possibleBreaks) fixBreak, each
+ * over the reads recruited from the read sets given (BamFile.readsInRegion
+ * over flankRegion: the valid reads overlapping the break widened by the
+ * set's maxInsertSize, frags sets first, then the unpaired ones unless
+ * config.longread) and the region's bases after its SNP fixes.  A fix kept
+ * (a gap's with start > 0, a break's when its ref or patch exceeds 10
+ * bases) joins bigFixList: pl_region_fix then applies it after the
+ * snps/indels fixes and lists it in the changes, and pl_region_bed adds each
+ * logged attempt's outcome line (Tracks.scala:211-213).  sets are the read
+ * sets the region was built from, with their PL_BAM_* types and
+ * maxInsertSize (BamFile.maxInsertSize); a jumps set is PL_E_INVAL
+ * (recruitBadMates pairs reads by name).  opts->break_radius is
+ * GapFiller.breakRadius for this region.  *n_fixes = the fixes kept.
+ * Calling it again replaces the region's reassembly. */
+pl_status pl_region_reassemble(pl_region* reg, const pl_read_soa* const* sets, const int32_t* set_types,
+                               const int32_t* max_insert, int32_t n_sets, const pl_asm_opts* opts, int32_t* n_fixes);
+
+/* The lines GenomeRegion.logFix prints for the last pl_region_reassemble
+ * ("fix gap: ctg:801-900 801 -0 +100 ClosedGap", "# fix break: ... 0 -0 +0
+ * NoSolution"), newline-terminated; free with pl_free. */
+pl_status pl_region_reassembly_log(pl_region* reg, char** text, int64_t* len);
 
 /* Device time of the last pl_batch_run's pipeline in milliseconds, measured
  * with HIP events on the library's stream, plus the share of the window

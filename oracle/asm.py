@@ -293,3 +293,195 @@ def assemble_break(cfg, reads, bases, contig, start, stop, bstart, bstop, is_gap
     except Edge:
         return 4, NO_SOLUTION, ""
     return 0, sol, g.tandem_repeat
+
+
+# ------------------------------------------------------------------ region
+# identifyAndFixIssues' local reassembly (GenomeRegion.scala:307-413) over
+# the C oracle's region run: its changes (the snps/indels fix lists), its
+# Pilon.bed gap and break regions, and recruitment from the read sets.
+
+CIGAR_OPS = "MIDNSHP=X"
+
+
+def valid_read(cfg, f):                               # BamFile.validateRead (BamFile.scala:100-104)
+    return (cfg.nonpf or not f & 0x200) and (cfg.duplicates or not f & 0x400) and not f & 0x100
+
+
+def recruit_flank(cfg, batch, start, stop, flank):
+    """BamFile.readsInRegion over flankRegion (BamFile.scala:339-364): the
+    indices of the valid reads whose span [pos, pos + reference length - 1]
+    (one base when unmapped or of no reference length) overlaps
+    [max(start - flank, 1), stop + flank]."""
+    fs, fe = max(start - flank, 1), stop + flank
+    idx = []
+    for i in range(batch.n):
+        f = int(batch.flag[i])
+        if not valid_read(cfg, f):
+            continue
+        ps = int(batch.pos[i])
+        words = batch.cigar[int(batch.cigar_off[i]):int(batch.cigar_off[i + 1])]
+        rl = sum(int(w) >> 4 for w in words if int(w) & 15 in (0, 2, 3, 7, 8))
+        pe = ps if (f & 0x4 or rl == 0) else ps + rl - 1
+        if ps <= fe and pe >= fs:
+            idx.append(i)
+    return idx
+
+
+def read_dicts(batch, idx):
+    """The reads at idx as the dicts pack_reads takes."""
+    from pilon_amd.abi import PL_RF_MATE_SAME_REF, PL_RF_NO_QUAL
+    from pilon_amd.reads import unpack_2bit, unpack_nib
+    nibs = "=ACMGRSVTWYHKDBN"
+    exo = {int(r): int(o) for r, o in zip(batch.exo_read, batch.exo_off)}
+    out = []
+    for i in idx:
+        s0, s1 = int(batch.seq_off[i]), int(batch.seq_off[i + 1])
+        if i in exo:
+            seq = "".join(nibs[c] for c in unpack_nib(batch.nib, exo[i], s1 - s0))
+        else:
+            seq = "".join("ACGT"[c] for c in unpack_2bit(batch.bases2, s0, s1 - s0))
+        f = int(batch.flag[i])
+        words = batch.cigar[int(batch.cigar_off[i]):int(batch.cigar_off[i + 1])]
+        cig = "".join("%d%s" % (int(w) >> 4, CIGAR_OPS[int(w) & 15]) for w in words) or "*"
+        out.append(dict(pos=int(batch.pos[i]), tlen=int(batch.tlen[i]), flag=f & 0x0FFF, mapq=int(batch.mapq[i]),
+                        cigar=cig, seq=seq or "*",
+                        qual=None if f & PL_RF_NO_QUAL else [int(q) for q in batch.quals[s0:s1]],
+                        mate_same_ref=bool(f & PL_RF_MATE_SAME_REF)))
+    return out
+
+
+def fix_fix_list(fixes):                              # GenomeRegion.fixFixList (GenomeRegion.scala:557-593)
+    fixes = sorted(fixes, key=lambda f: f[0])         # sortWith is stable
+    out = []
+    while fixes:
+        if len(fixes) == 1:
+            out.append(fixes[0])
+            break
+        f1, f2 = fixes[0], fixes[1]
+        e1 = f1[0] + max(len(f1[1]) - 1, 0)
+        e2 = f2[0] + max(len(f2[1]) - 1, 0)
+        if f2[0] <= e1 and e2 >= f1[0]:               # Region.overlaps
+            keep = f1 if len(f1[1]) + len(f1[2]) >= len(f2[1]) + len(f2[2]) else f2
+            fixes = [keep] + fixes[2:]
+        else:
+            out.append(f1)
+            fixes = fixes[1:]
+    return out
+
+
+def fix_issues(bases, start, fixes):                  # GenomeRegion.fixIssues (GenomeRegion.scala:595-621)
+    for loc, was, patch in reversed(fix_fix_list(fixes)):
+        i = loc - start
+        bases = bases[:i] + patch + bases[i + len(was):]
+    return bases
+
+
+def region_string(name, s, e):                        # Region.regionString (Region.scala:42)
+    return "%s:%d" % (name, s) if e + 1 - s < 2 else "%s:%d-%d" % (name, s, e)
+
+
+def write_changes(fixes, name, new_name, offset):     # GenomeRegion.writeChanges (GenomeRegion.scala:645-657)
+    out, delta = [], 0
+    for loc, was, patch in fix_fix_list(fixes):
+        new_loc = loc + delta
+        out.append("%s %s %s %s\n" % (region_string(name, loc, loc + len(was) - 1),
+                                      region_string(new_name, new_loc + offset, new_loc + offset + len(patch) - 1),
+                                      was or ".", patch or "."))
+        delta += len(patch) - len(was)
+    return "".join(out)
+
+
+def log_fix(name, reg, sol, gap_size, tandem_repeat):  # GenomeRegion.logFix (GenomeRegion.scala:472-502)
+    loc, ref, patch = sol
+    n_ref, n_patch = ref.count("N"), patch.count("N")
+    if loc == 0:
+        msg = "NoSolution"
+    elif gap_size == 0 and not ref and not patch:
+        msg = "NoChange"
+    elif gap_size == 0 and n_patch == 0:
+        msg = "BreakFix"
+    elif n_patch > 0 and n_ref == 0:
+        msg = "OpenedGap"
+    elif gap_size > 0 and n_ref == gap_size and n_patch == 0:
+        msg = "ClosedGap"
+    elif gap_size > 0:
+        msg = "PartialFill"
+    else:
+        msg = "Unknown!"
+    outcome = "TandemRepeat" if tandem_repeat else msg
+    if tandem_repeat:
+        msg += " TandemRepeat %d" % len(tandem_repeat)
+    line = "fix %s: %s %d -%d +%d %s\n" % ("gap" if gap_size > 0 else "break", region_string(name, *reg), loc,
+                                           len(ref) - n_ref, len(patch) - n_patch, msg)
+    return outcome, line
+
+
+def reassemble_region(cfg, contig, start, stop, sets, name="ctg", new_name=None, offset=0, break_radius=None,
+                      K=47, min_depth=5, tr_safe=True, multi_closure=False, fix_breaks=False, gap_margin=100000,
+                      min_gap=10, fix_gaps=True, fix_local=True):
+    """One region through identifyAndFixIssues with its gap filling and
+    break fixing (GenomeRegion.scala:307-413, --nostrays), then writeChanges
+    and Pilon.bed.  sets: [(ReadBatch, BAM type name, maxInsertSize)].
+    Returns a dict: status (4 where Pilon throws), changes, bases, bed (the
+    region's lines with the reassembly outcomes), log (logFix lines)."""
+    from oracle import oracle as O
+    from pilon_amd.abi import BAM_TYPES
+    from pilon_amd.reads import pack_reads
+    if new_name is None:
+        new_name = name + "_pilon"
+    res = O.run_region(cfg, contig, start, stop, [b for b, _, _ in sets], name=name, new_name=new_name,
+                       offset=offset, vcf=O.BED, set_types=[BAM_TYPES[t] for _, t, _ in sets])
+    if res["status"]:
+        return {"status": res["status"]}
+    contig_s = contig.decode() if isinstance(contig, bytes) else contig
+    # the snps/indels fix lists from the changes (Pass 1 changes never overlap)
+    snp, small = [], []
+    for line in res["changes"].splitlines():
+        old, _, was, patch = line.split(" ")
+        loc = int(old.rsplit(":", 1)[1].split("-")[0])
+        was, patch = ("" if was == "." else was), ("" if patch == "." else patch)
+        (snp if len(was) == 1 and len(patch) == 1 and patch in "ACGT" else small).append((loc, was, patch))
+    region_bases = contig_s[start - 1:stop]
+    bases = fix_issues(region_bases, start, snp)      # fixIssues(snpFixList) (:380)
+    gaps, breaks = [], []
+    for line in res["bed"].splitlines():
+        f = line.split("\t")
+        if len(f) > 3 and f[3] in ("Gap", "Break"):
+            (gaps if f[3] == "Gap" else breaks).append((int(f[1]) + 1, int(f[2])))
+    long_run = bool(cfg.longread)
+    big, bed_extra, log = [], [], []
+    attempts = ([(g, True) for g in gaps] if fix_gaps else []) + ([(b, False) for b in breaks] if fix_local else [])
+    for reg, is_gap in attempts:
+        recruited = []
+        for b, t, m in sets:                           # recruitFrags
+            if t == "frags":
+                recruited += read_dicts(b, recruit_flank(cfg, b, reg[0], reg[1], m))
+        if not long_run:                               # recruitUnpaired
+            for b, t, m in sets:
+                if t != "frags":
+                    recruited += read_dicts(b, recruit_flank(cfg, b, reg[0], reg[1], m))
+        st, sol, loop = assemble_break(cfg, pack_reads(recruited), bases, contig_s, start, stop, reg[0], reg[1],
+                                       is_gap, K=K, min_depth=min_depth, break_radius=break_radius, tr_safe=tr_safe,
+                                       multi_closure=multi_closure, fix_breaks=fix_breaks, gap_margin=gap_margin,
+                                       min_gap=min_gap)
+        if st:
+            return {"status": st}
+        logged = None
+        if is_gap:
+            if sol[0] > 0:
+                big.append(sol)
+                logged = log_fix(name, reg, sol, reg[1] + 1 - reg[0], loop)
+        elif sol[0] > 0 and max(len(sol[1]), len(sol[2])) > 10:
+            big.append(sol)
+            logged = log_fix(name, reg, sol, 0, loop)
+        elif sol[0] == 0:
+            outcome, line = log_fix(name, reg, sol, 0, loop)
+            logged = (outcome, "# " + line)
+        if logged:
+            bed_extra.append("%s\t%d\t%d\t%s\t0\t+\t%d\t%d\t0,128,0\n" % (name, reg[0] - 1, reg[1], logged[0],
+                                                                        reg[0] - 1, reg[1]))
+            log.append(logged[1])
+    big_list = list(reversed(big))                     # bigFixList ::= fix
+    final = fix_issues(bases, start, small + big_list)  # fixIssues(smallFixList ++ bigFixList)
+    return {"status": 0, "changes": write_changes(snp + small + big_list, name, new_name, offset),
+            "bases": final, "bed": res["bed"] + "".join(bed_extra), "log": "".join(log), "run": res}

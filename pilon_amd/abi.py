@@ -139,10 +139,10 @@ class PlWideCol(C.Structure):
 class PlAsmOpts(C.Structure):
     """pl_asm_opts: the knobs of GapFiller.fillGap / fixBreak (Assembler.K,
     minDepth, breakRadius, and Pilon's trSafe, multiClosure, fixBreaks,
-    gapMargin, minGap)."""
+    gapMargin, minGap) and the "gaps" and "local" fixes of --fix."""
     _fields_ = [("k", C.c_int32), ("min_depth", C.c_int32), ("break_radius", C.c_int32), ("tr_safe", C.c_int32),
                 ("multi_closure", C.c_int32), ("fix_breaks", C.c_int32), ("gap_margin", C.c_int32),
-                ("min_gap", C.c_int32)]
+                ("min_gap", C.c_int32), ("fix_gaps", C.c_int32), ("fix_local", C.c_int32)]
 
 
 class PlAsmFix(C.Structure):

@@ -283,33 +283,11 @@ struct GapFill {
   }
 };
 
-}  // namespace
-
-extern "C" {
-
-void pl_asm_opts_default(pl_asm_opts* o) {
-  if (!o) return;
-  o->k = 47;
-  o->min_depth = 5;
-  o->break_radius = 3 * 47;
-  o->tr_safe = 1;
-  o->multi_closure = 0;
-  o->fix_breaks = 0;
-  o->gap_margin = 100000;
-  o->min_gap = 10;
-}
-
-pl_status pl_assemble_break(pl_ctx* c, const pl_read_soa* reads, const char* region_bases, const char* contig,
-                            int64_t contig_len, int32_t region_start, int32_t region_stop, int32_t break_start,
-                            int32_t break_stop, int32_t is_gap, const pl_asm_opts* opts, pl_asm_fix* out) {
-  if (!c || !reads || !region_bases || !contig || !out) return fail(PL_E_INVAL, "null argument");
-  std::memset(out, 0, sizeof *out);
-  pl_asm_opts o;
-  if (opts) o = *opts; else pl_asm_opts_default(&o);
-  if (o.k < 1 || o.k > 8 * PL_KMER_WORDS) return fail(PL_E_INVAL, "K = %d outside [1, %d]", o.k, 8 * PL_KMER_WORDS);
-  if (region_start < 1 || region_stop < region_start || region_stop > contig_len)
-    return fail(PL_E_INVAL, "region %d-%d outside a contig of %lld", region_start, region_stop, (long long)contig_len);
-  if (break_start > break_stop) return fail(PL_E_INVAL, "break %d-%d", break_start, break_stop);
+// One fillGap / fixBreak given the recruited reads: the graph on the device,
+// the walk and joins on the host.  PL_E_EDGE where Pilon throws.
+pl_status assemble_core(pl_ctx* c, const pl_read_soa* reads, const char* region_bases, const char* contig,
+                        int32_t region_start, int32_t region_stop, int32_t break_start, int32_t break_stop, bool is_gap,
+                        const pl_asm_opts& o, AsmFix* fix, std::string* loop) {
   char* km = nullptr;
   uint8_t *nx = nullptr, *al = nullptr;
   int64_t nk = 0;
@@ -331,9 +309,181 @@ pl_status pl_assemble_break(pl_ctx* c, const pl_read_soa* reads, const char* reg
   R.start = region_start; R.stop = region_stop; R.bases = region_bases; R.contig = contig;
   GapFill g;
   g.R = &R; g.A = &A; g.o = o;
-  std::string loop;
-  AsmFix f = g.across(break_start, break_stop, is_gap != 0, &loop);
+  *fix = g.across(break_start, break_stop, is_gap, loop);
   if (R.edge) return fail(PL_E_EDGE, "Pilon 1.24 throws here: a substring outside region %d-%d", region_start, region_stop);
+  return PL_OK;
+}
+
+pl_status check_opts(const pl_asm_opts& o) {
+  if (o.k < 1 || o.k > 8 * PL_KMER_WORDS) return fail(PL_E_INVAL, "K = %d outside [1, %d]", o.k, 8 * PL_KMER_WORDS);
+  if (o.break_radius < 0 || o.gap_margin < 0 || o.min_gap < 0) return fail(PL_E_INVAL, "negative assembly option");
+  return PL_OK;
+}
+
+// Reads of several batches gathered into one SoA: the reads recruited for a
+// break.
+struct SoaGather {
+  std::vector<int32_t> pos, tlen;
+  std::vector<uint16_t> flag;
+  std::vector<uint8_t> mapq;
+  std::vector<int64_t> cigar_off{0}, seq_off{0}, exo_read, exo_off;
+  std::vector<uint32_t> cigar;
+  std::vector<uint8_t> bases2, nib;
+  std::vector<int8_t> quals;
+  int64_t n_nib = 0;
+
+  void add(const pl_read_soa& S, int64_t i) {
+    const int64_t s0 = S.seq_off[i], L = S.seq_off[i + 1] - s0, d0 = seq_off.back();
+    const int64_t r = (int64_t)pos.size();
+    pos.push_back(S.pos[i]);
+    tlen.push_back(S.tlen ? S.tlen[i] : 0);
+    flag.push_back(S.flag[i]);
+    mapq.push_back(S.mapq[i]);
+    if (S.cigar_off) {
+      for (int64_t k = S.cigar_off[i]; k < S.cigar_off[i + 1]; k++) cigar.push_back(S.cigar[k]);
+    }
+    cigar_off.push_back((int64_t)cigar.size());
+    seq_off.push_back(d0 + L);
+    bases2.resize((size_t)((d0 + L + 3) / 4), 0);
+    for (int64_t j = 0; j < L; j++) {
+      const int64_t a = s0 + j, b = d0 + j;
+      const uint8_t v = (S.bases2[a >> 2] >> (2 * (a & 3))) & 3u;
+      bases2[(size_t)(b >> 2)] |= (uint8_t)(v << (2 * (b & 3)));
+    }
+    quals.insert(quals.end(), S.quals + s0, S.quals + s0 + L);
+    // a read with non-ACGT codes: its nibbles too (exo_read is ascending)
+    if (S.n_exotic > 0) {
+      const int64_t* e = std::lower_bound(S.exo_read, S.exo_read + S.n_exotic, i);
+      if (e != S.exo_read + S.n_exotic && *e == i) {
+        const int64_t o0 = S.exo_off[e - S.exo_read];
+        exo_read.push_back(r);
+        exo_off.push_back(n_nib);
+        nib.resize((size_t)((n_nib + L + 1) / 2), 0);
+        for (int64_t j = 0; j < L; j++) {
+          const int64_t a = o0 + j, b = n_nib + j;
+          const uint8_t v = (S.nib[a >> 1] >> ((a & 1) ? 0 : 4)) & 15u;
+          nib[(size_t)(b >> 1)] |= (uint8_t)(v << ((b & 1) ? 0 : 4));
+        }
+        n_nib += L;
+      }
+    }
+  }
+  pl_read_soa soa() const {
+    pl_read_soa o;
+    std::memset(&o, 0, sizeof o);
+    o.n_reads = (int64_t)pos.size();
+    o.pos = pos.data(); o.tlen = tlen.data(); o.flag = flag.data(); o.mapq = mapq.data();
+    o.cigar_off = cigar_off.data(); o.cigar = cigar.data(); o.seq_off = seq_off.data();
+    o.bases2 = bases2.data(); o.quals = quals.data();
+    o.n_exotic = (int64_t)exo_read.size();
+    o.exo_read = exo_read.data(); o.exo_off = exo_off.data(); o.nib = nib.data();
+    return o;
+  }
+};
+
+// BamFile.validateRead (BamFile.scala:100-104)
+bool valid_read(const pl_config& cf, uint16_t f) {
+  return (cf.nonpf || !(f & 0x200)) && (cf.duplicates || !(f & 0x400)) && !(f & 0x100);
+}
+
+// BamFile.readsInRegion over flankRegion (BamFile.scala:339-364): the valid
+// reads whose span [pos, pos + reference length - 1] (one base for an
+// unmapped or zero-length alignment) overlaps [max(start - flank, 1),
+// stop + flank]
+void recruit_flank(const pl_config& cf, const pl_read_soa& S, int32_t start, int32_t stop, int32_t flank, SoaGather& out) {
+  const int64_t fs = std::max<int64_t>((int64_t)start - flank, 1), fe = (int64_t)stop + flank;
+  for (int64_t i = 0; i < S.n_reads; i++) {
+    if (!valid_read(cf, S.flag[i])) continue;
+    const int64_t ps = S.pos[i];
+    if (ps > fe) continue;
+    int64_t rl = 0;
+    if (S.cigar_off)
+      for (int64_t k = S.cigar_off[i]; k < S.cigar_off[i + 1]; k++) {
+        const uint32_t op = S.cigar[k] & 15u;
+        if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rl += S.cigar[k] >> 4;
+      }
+    const int64_t pe = ((S.flag[i] & 0x4) || rl == 0) ? ps : ps + rl - 1;
+    if (pe < fs) continue;
+    out.add(S, i);
+  }
+}
+
+// GenomeRegion.logFix (GenomeRegion.scala:472-502): the log line and the
+// reassemblyFixes outcome of one attempt
+plh::Outcome log_fix(const std::string& name, int32_t rs, int32_t re, const AsmFix& f, int32_t gap_size,
+                     const std::string& loop, bool hash_prefix) {
+  const int64_t n_ref = std::count(f.ref.begin(), f.ref.end(), 'N');
+  const int64_t n_patch = std::count(f.patch.begin(), f.patch.end(), 'N');
+  std::string msg;
+  if (f.start == 0) msg = "NoSolution";
+  else if (gap_size == 0 && f.ref.empty() && f.patch.empty()) msg = "NoChange";
+  else if (gap_size == 0 && n_patch == 0) msg = "BreakFix";
+  else if (n_patch > 0 && n_ref == 0) msg = "OpenedGap";
+  else if (gap_size > 0 && n_ref == gap_size && n_patch == 0) msg = "ClosedGap";
+  else if (gap_size > 0) msg = "PartialFill";
+  else msg = "Unknown!";
+  plh::Outcome o;
+  o.start = rs;
+  o.stop = re;
+  o.outcome = loop.empty() ? msg : std::string("TandemRepeat");
+  if (!loop.empty()) msg += " TandemRepeat " + std::to_string(loop.size());
+  o.line = std::string(hash_prefix ? "# " : "") + "fix " + (gap_size > 0 ? "gap" : "break") + ": " +
+           plh::region_string(name, rs, re) + " " + std::to_string(f.start) + " -" +
+           std::to_string((int64_t)f.ref.size() - n_ref) + " +" + std::to_string((int64_t)f.patch.size() - n_patch) +
+           " " + msg + "\n";
+  return o;
+}
+
+// The issue regions of a finished region (gaps, possible breaks).
+pl_status region_issue_lists(pl_region* r, std::vector<std::pair<int32_t, int32_t>> lists[7]) {
+  return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>&, const std::vector<int32_t>& contigs) {
+    VcfBufs B;
+    TrackArgs T;
+    double ms = 0;
+    if (pl_status s = tracks_of_lane(c, L, contigs, B, T, &ms, true)) return s;
+    std::vector<uint8_t> mask((size_t)L->G);
+    HIPCHK(hipMemcpyAsync(mask.data(), T.mask, (size_t)L->G, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const JobDev& J = L->hjobs[0];
+    issue_lists(mask.data() + J.out, J.start, J.stop, lists);
+    return PL_OK;
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+void pl_asm_opts_default(pl_asm_opts* o) {
+  if (!o) return;
+  o->k = 47;
+  o->min_depth = 5;
+  o->break_radius = 3 * 47;
+  o->tr_safe = 1;
+  o->multi_closure = 0;
+  o->fix_breaks = 0;
+  o->gap_margin = 100000;
+  o->min_gap = 10;
+  o->fix_gaps = 1;
+  o->fix_local = 1;
+}
+
+pl_status pl_assemble_break(pl_ctx* c, const pl_read_soa* reads, const char* region_bases, const char* contig,
+                            int64_t contig_len, int32_t region_start, int32_t region_stop, int32_t break_start,
+                            int32_t break_stop, int32_t is_gap, const pl_asm_opts* opts, pl_asm_fix* out) {
+  if (!c || !reads || !region_bases || !contig || !out) return fail(PL_E_INVAL, "null argument");
+  std::memset(out, 0, sizeof *out);
+  pl_asm_opts o;
+  if (opts) o = *opts; else pl_asm_opts_default(&o);
+  if (pl_status s = check_opts(o)) return s;
+  if (region_start < 1 || region_stop < region_start || region_stop > contig_len)
+    return fail(PL_E_INVAL, "region %d-%d outside a contig of %lld", region_start, region_stop, (long long)contig_len);
+  if (break_start > break_stop) return fail(PL_E_INVAL, "break %d-%d", break_start, break_stop);
+  AsmFix f;
+  std::string loop;
+  if (pl_status s = assemble_core(c, reads, region_bases, contig, region_start, region_stop, break_start, break_stop,
+                                  is_gap != 0, o, &f, &loop))
+    return s;
   auto dup = [](const std::string& x) { char* p = (char*)malloc(x.size() + 1); if (p) { std::memcpy(p, x.data(), x.size()); p[x.size()] = 0; } return p; };
   out->start = f.start;
   out->ref = dup(f.ref);
@@ -346,6 +496,85 @@ pl_status pl_assemble_break(pl_ctx* c, const pl_read_soa* reads, const char* reg
     std::memset(out, 0, sizeof *out);
     return fail(PL_E_OOM, "host allocation failed");
   }
+  return PL_OK;
+}
+
+pl_status pl_region_reassemble(pl_region* r, const pl_read_soa* const* sets, const int32_t* set_types,
+                               const int32_t* max_insert, int32_t n_sets, const pl_asm_opts* opts, int32_t* n_fixes) {
+  if (!r || (n_sets > 0 && (!sets || !set_types || !max_insert)) || n_sets < 0) return fail(PL_E_INVAL, "null argument");
+  if (!r->finished) return fail(PL_E_INVAL, "region not finished (pl_region_finish first)");
+  pl_asm_opts o;
+  if (opts) o = *opts; else pl_asm_opts_default(&o);
+  if (pl_status s = check_opts(o)) return s;
+  pl_ctx* c = r->ctx;
+  const pl_config& cf = c->pc;
+  for (int32_t k = 0; k < n_sets; k++) {
+    if (!sets[k]) return fail(PL_E_INVAL, "read set %d is null", k);
+    if (set_types[k] < PL_BAM_FRAGS || set_types[k] > PL_BAM_PACBIO) return fail(PL_E_INVAL, "read set %d: BAM type %d", k, set_types[k]);
+    if (set_types[k] == PL_BAM_JUMPS)
+      return fail(PL_E_INVAL, "read set %d is a jumps BAM: recruitBadMates pairs reads by name, which a read set does not carry", k);
+    if (max_insert[k] < 0) return fail(PL_E_INVAL, "read set %d: negative maxInsertSize", k);
+  }
+  r->big.clear();
+  r->outcomes.clear();
+  std::vector<std::pair<int32_t, int32_t>> lists[7];
+  if (pl_status s = region_issue_lists(r, lists)) return s;
+  const Contig& ct = c->contigs[r->contig];
+  const std::string rb((const char*)ct.host.data() + r->start - 1, (size_t)(r->stop - r->start + 1));
+  const std::string bases = plh::snp_fixed_bases(cf, r->start, rb, r->changes);   // fixIssues(snpFixList)
+  const char* contig = (const char*)ct.host.data();
+  // recruitReads (GapFiller.scala:364-403, --nostrays): each frags BAM's
+  // flank reads, then each unpaired BAM's unless a long-read run
+  auto recruit = [&](int32_t bs, int32_t be, SoaGather& g) {
+    for (int32_t k = 0; k < n_sets; k++)
+      if (set_types[k] == PL_BAM_FRAGS) recruit_flank(cf, *sets[k], bs, be, max_insert[k], g);
+    if (!cf.longread)
+      for (int32_t k = 0; k < n_sets; k++)
+        if (set_types[k] != PL_BAM_FRAGS) recruit_flank(cf, *sets[k], bs, be, max_insert[k], g);
+  };
+  // fillGap over the gaps, then fixBreak over the possible breaks
+  // (GenomeRegion.scala:382-410)
+  for (int pass = 0; pass < 2; pass++) {
+    const bool is_gap = pass == 0;
+    if (is_gap ? !o.fix_gaps : !o.fix_local) continue;
+    for (const auto& br : lists[is_gap ? ISSUE_GAPS : ISSUE_BREAKS]) {
+      SoaGather g;
+      recruit(br.first, br.second, g);
+      const pl_read_soa soa = g.soa();
+      AsmFix f;
+      std::string loop;
+      if (pl_status s = assemble_core(c, &soa, bases.data(), contig, r->start, r->stop, br.first, br.second, is_gap, o,
+                                      &f, &loop)) {
+        r->big.clear();
+        r->outcomes.clear();
+        return s;
+      }
+      const int32_t gap_size = is_gap ? br.second + 1 - br.first : 0;
+      if (is_gap) {
+        if (f.start > 0) {
+          r->big.push_back({f.start, f.ref, f.patch});
+          r->outcomes.push_back(log_fix(ct.name, br.first, br.second, f, gap_size, loop, false));
+        }
+      } else if (f.start > 0 && std::max(f.ref.size(), f.patch.size()) > 10) {
+        r->big.push_back({f.start, f.ref, f.patch});
+        r->outcomes.push_back(log_fix(ct.name, br.first, br.second, f, 0, loop, false));
+      } else if (f.start == 0) {
+        r->outcomes.push_back(log_fix(ct.name, br.first, br.second, f, 0, loop, true));
+      }
+    }
+  }
+  if (n_fixes) *n_fixes = (int32_t)r->big.size();
+  return PL_OK;
+}
+
+pl_status pl_region_reassembly_log(pl_region* r, char** text, int64_t* len) {
+  if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
+  std::string t;
+  for (const plh::Outcome& o : r->outcomes) t += o.line;
+  *text = (char*)malloc(t.size() + 1);
+  if (!*text) return fail(PL_E_OOM, "host allocation failed");
+  std::memcpy(*text, t.c_str(), t.size() + 1);
+  *len = (int64_t)t.size();
   return PL_OK;
 }
 

@@ -265,6 +265,10 @@ struct pl_region {
   // the "jumps" reads alone (fragment coverage leaves them out), kept the same way
   std::unique_ptr<ReadSet> jtemp;
   int32_t jtemp_id = -1;
+  // pl_region_reassemble: bigFixList in the order found, and per attempt
+  // logged the region, its outcome (reassemblyFixes) and the log line
+  std::vector<plh::Fix> big;
+  std::vector<plh::Outcome> outcomes;
 };
 
 namespace {
@@ -1183,7 +1187,9 @@ pl_status pl_region_fix(pl_region* r, const char* new_name, int64_t offset, char
   const Contig& ct = r->ctx->contigs[r->contig];
   std::string rb((const char*)ct.host.data() + r->start - 1, (size_t)(r->stop - r->start + 1));
   std::string nn = new_name ? new_name : plh::pilon_name(ct.name);
-  plh::FixResult fr = plh::fix_region(r->ctx->pc, ct.name, r->start, rb, r->changes, nn, offset);
+  // bigFixList holds the reassembly's fixes last found first
+  const std::vector<plh::Fix> big(r->big.rbegin(), r->big.rend());
+  plh::FixResult fr = plh::fix_region(r->ctx->pc, ct.name, r->start, rb, r->changes, nn, offset, big);
   if (changes_text) {
     *changes_text = (char*)malloc(fr.changes.size() + 1);
     std::memcpy(*changes_text, fr.changes.c_str(), fr.changes.size() + 1);

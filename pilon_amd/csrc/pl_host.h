@@ -33,6 +33,9 @@ struct Change {
 };
 
 struct Fix { int32_t loc; std::string was, patch; };
+// one logged reassembly attempt (GenomeRegion.logFix): its region, the
+// reassemblyFixes outcome and the log line
+struct Outcome { int32_t start, stop; std::string outcome, line; };
 
 // GenomeRegion.excludeMotifs (GenomeRegion.scala:219,416-439): when every BAM
 // of a long-read run is a non-frags one (longReadOnly, :147), identifyAndFixIssues
@@ -118,14 +121,11 @@ inline std::string region_string(const std::string& name, int64_t s, int64_t e) 
 
 struct FixResult { std::string changes, bases; int snps = 0, amb = 0, ins = 0, ins_bases = 0, dels = 0, del_bases = 0; };
 
-// identifyAndFixIssues for the snps/indels fixes (GenomeRegion.scala:307-380,
-// 413) and writeChanges (:645-657) for one region.  changes must be sorted by
-// locus; region_bases is contigBases[start-1 .. stop-1].
-inline FixResult fix_region(const pl_config& cf, const std::string& name, int32_t start,
-                            const std::string& region_bases, const std::vector<Change>& changes,
-                            const std::string& new_name, int64_t offset) {
-  FixResult R;
-  std::vector<Fix> snp, small;   // in the order the Scala code prepends them
+// identifyAndFixIssues' snpFixList and smallFixList (GenomeRegion.scala:
+// 307-355), each in List order (the last one prepended first), with the
+// counts it logs.  changes must be sorted by locus.
+inline void fix_lists(const pl_config& cf, const std::vector<Change>& changes, std::vector<Fix>& snp,
+                      std::vector<Fix>& small, FixResult& R) {
   for (const Change& c : changes) {
     const std::string r(1, (char)c.ref);
     const std::string b(1, c.base == 4 ? 'N' : "ACGT"[c.base]);
@@ -153,14 +153,39 @@ inline FixResult fix_region(const pl_config& cf, const std::string& name, int32_
         break;
     }
   }
-  // List order: the last prepended first
   std::reverse(snp.begin(), snp.end());
   std::reverse(small.begin(), small.end());
+}
+
+// The region's bases as local reassembly sees them: after
+// fixIssues(snpFixList) (GenomeRegion.scala:380).
+inline std::string snp_fixed_bases(const pl_config& cf, int32_t start, const std::string& region_bases,
+                                   const std::vector<Change>& changes) {
+  std::vector<Fix> snp, small;
+  FixResult R;
+  fix_lists(cf, changes, snp, small, R);
+  std::string b = region_bases;
+  apply_fixes(b, start, snp);
+  return b;
+}
+
+// identifyAndFixIssues for one region (GenomeRegion.scala:307-413): the
+// snps/indels fixes, with big (bigFixList, in List order: the reassembly
+// fixes, the last one found first) applied after them, and writeChanges
+// (:645-657).  region_bases is contigBases[start-1 .. stop-1].
+inline FixResult fix_region(const pl_config& cf, const std::string& name, int32_t start,
+                            const std::string& region_bases, const std::vector<Change>& changes,
+                            const std::string& new_name, int64_t offset, const std::vector<Fix>& big = {}) {
+  FixResult R;
+  std::vector<Fix> snp, small;
+  fix_lists(cf, changes, snp, small, R);
   R.bases = region_bases;
   apply_fixes(R.bases, start, snp);
-  apply_fixes(R.bases, start, small);
+  std::vector<Fix> later = small;                  // fixIssues(smallFixList ++ bigFixList)
+  later.insert(later.end(), big.begin(), big.end());
+  apply_fixes(R.bases, start, later);
   std::vector<Fix> all = snp;
-  all.insert(all.end(), small.begin(), small.end());
+  all.insert(all.end(), later.begin(), later.end());
   int64_t delta = 0;
   for (const Fix& f : fix_fix_list(all)) {
     const int64_t newLoc = f.loc + delta;

@@ -213,10 +213,11 @@ void summary_close(Summary& S, int32_t start, int32_t stop) {
   if (!(a - start < 100 || stop - b < 100)) S.out.emplace_back(a, b);
 }
 
-// the region's lines of Tracks.makeBedTrack (Tracks.scala:194-216), from
-// its issue bits: unconfirmed, change, low-coverage, collapsed-repeat,
-// duplication, gap and break regions, in that order
-void bed_of_region(const uint8_t* m, int32_t start, int32_t stop, const std::string& name, std::string& out) {
+// The issue regions of one region from its issue bits: unconfirmed, change,
+// low-coverage, collapsed-repeat, duplication, gap and break regions, in
+// that order (Tracks.scala:204-210)
+enum { ISSUE_GAPS = 5, ISSUE_BREAKS = 6 };
+void issue_lists(const uint8_t* m, int32_t start, int32_t stop, std::vector<std::pair<int32_t, int32_t>> lists[7]) {
   const int32_t n = stop + 1 - start;
   Summary S[7] = {{PL_ISSUE_UNCONFIRMED, 100}, {PL_ISSUE_CHANGED, 1}, {PL_ISSUE_LOWCOV, 100}, {PL_ISSUE_HIGHCN, 100},
                   {PL_ISSUE_HIGHCN, 2000}, {PL_ISSUE_N, 100}, {PL_ISSUE_BREAK, 200}};
@@ -239,7 +240,6 @@ void bed_of_region(const uint8_t* m, int32_t start, int32_t stop, const std::str
   }
   for (Summary& q : S)
     if (q.last >= 0) summary_close(q, start, stop);
-  std::vector<std::pair<int32_t, int32_t>> lists[7];
   lists[0] = std::move(S[0].out);
   lists[1] = std::move(S[1].out);
   lists[2] = std::move(S[2].out);
@@ -256,6 +256,12 @@ void bed_of_region(const uint8_t* m, int32_t start, int32_t stop, const std::str
           std::abs(r.first - g.second) <= 300) { near = true; break; }
     if (!near) lists[6].push_back(r);
   }
+}
+
+// the region's lines of Tracks.makeBedTrack (Tracks.scala:194-216)
+void bed_of_region(const uint8_t* m, int32_t start, int32_t stop, const std::string& name, std::string& out) {
+  std::vector<std::pair<int32_t, int32_t>> lists[7];
+  issue_lists(m, start, stop, lists);
   static const char* const labels[7] = {"?", "X", "LowCov", "Copy#", "Duplication", "Gap", "Break"};
   static const char* const rgbs[7] = {"0,128,128", "255,0,0", "128,0,128", "128,128,0", "255,128,0", "0,0,0", "255,0,255"};
   for (int k = 0; k < 7; k++)
@@ -398,10 +404,24 @@ pl_status pl_batch_bed(pl_ctx* c, uint32_t flags, char** text, int64_t* len, dou
 
 pl_status pl_region_bed(pl_region* r, uint32_t flags, char** text, int64_t* len, double* device_ms) {
   if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
-  return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
-                               const std::vector<int32_t>& contigs) {
+  pl_status s = on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
+                                      const std::vector<int32_t>& contigs) {
     return bed_of_lane(c, L, names, contigs, flags, text, len, device_ms);
   });
+  if (s || r->outcomes.empty()) return s;
+  // the reassembly outcomes after the region's issue lines (Tracks.scala:211-213)
+  std::string extra;
+  const std::string& name = r->ctx->contigs[r->contig].name;
+  for (const plh::Outcome& o : r->outcomes) {
+    const std::string a = std::to_string(o.start - 1), b = std::to_string(o.stop);
+    extra += name + '\t' + a + '\t' + b + '\t' + o.outcome + "\t0\t+\t" + a + '\t' + b + "\t0,128,0\n";
+  }
+  char* t = (char*)realloc(*text, (size_t)*len + extra.size() + 1);
+  if (!t) return fail(PL_E_OOM, "host allocation failed");
+  std::memcpy(t + *len, extra.data(), extra.size() + 1);
+  *text = t;
+  *len += (int64_t)extra.size();
+  return PL_OK;
 }
 
 }  // extern "C"

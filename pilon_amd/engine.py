@@ -195,6 +195,9 @@ def lib():
             "pl_batch_vcf_stream": (C.c_int, [C.c_void_p, C.c_uint32, TEXT_SINK, C.c_void_p, P(C.c_int64),
                                               P(C.c_double)]),
             "pl_asm_opts_default": (None, [P(PlAsmOpts)]),
+            "pl_region_reassemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                               P(PlAsmOpts), P(C.c_int32)]),
+            "pl_region_reassembly_log": (C.c_int, [C.c_void_p, P(C.c_void_p), P(C.c_int64)]),
             "pl_assemble_break": (C.c_int, [C.c_void_p, P(PlReadSoa), C.c_char_p, C.c_char_p, C.c_int64, C.c_int32,
                                             C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(PlAsmOpts), P(PlAsmFix)]),
         }
@@ -603,6 +606,36 @@ class PileUpRegion:
     def bed(self, track_line=True):
         """Tracks.makeBedTrack (Pilon.bed) lines of this region: (text, device_ms)."""
         return _bed(lib().pl_region_bed, self.h, track_line)
+
+    def reassemble(self, sets, break_radius=None, **opts):
+        """identifyAndFixIssues' gap filling and break fixing for this finished
+        region (GenomeRegion.scala:382-410, as with --nostrays).  sets: the
+        region's read sets as (reads, bam_type, maxInsertSize) tuples, from
+        which each break's reads are recruited; break_radius is
+        GapFiller.breakRadius (3K by default); opts are other pl_asm_opts
+        fields.  Returns (number of fixes kept, the logFix lines); fix() and
+        bed() then include them."""
+        o = PlAsmOpts()
+        lib().pl_asm_opts_default(C.byref(o))
+        if break_radius is not None:
+            o.break_radius = int(break_radius)
+        for key, v in opts.items():
+            if key not in dict(PlAsmOpts._fields_):
+                raise TypeError("unknown assembly option %r" % key)
+            setattr(o, key, int(v))
+        soas = [r.soa() if hasattr(r, "soa") else r for r, _, _ in sets]
+        n = len(soas)
+        arr = (C.POINTER(PlReadSoa) * max(1, n))(*[C.pointer(x) for x in soas])
+        types = (C.c_int32 * max(1, n))(*[BAM_TYPES[t] for _, t, _ in sets])
+        ins = (C.c_int32 * max(1, n))(*[int(m) for _, _, m in sets])
+        nf = C.c_int32()
+        _check(lib().pl_region_reassemble(self.h, arr, types, ins, n, C.byref(o), C.byref(nf)))
+        t, tl = C.c_void_p(), C.c_int64()
+        _check(lib().pl_region_reassembly_log(self.h, C.byref(t), C.byref(tl)))
+        try:
+            return nf.value, C.string_at(t, tl.value).decode()
+        finally:
+            lib().pl_free(t)
 
     def fix(self, new_name=None, offset=0):
         """identifyAndFixIssues + writeChanges: returns (changes text, fixed bases)."""

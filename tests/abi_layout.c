@@ -16,6 +16,6 @@ int main(void) {
   printf("pl_bam_stats %zu\n", sizeof(pl_bam_stats));
   P(pl_bam_stats, inflate_s); P(pl_bam_stats, threads); P(pl_bam_stats, eof_marker);
   printf("pl_asm_opts %zu\npl_asm_fix %zu\n", sizeof(pl_asm_opts), sizeof(pl_asm_fix));
-  P(pl_asm_opts, min_gap); P(pl_asm_fix, ref_len); P(pl_asm_fix, patch); P(pl_asm_fix, tandem_repeat);
+  P(pl_asm_opts, min_gap); P(pl_asm_opts, fix_local); P(pl_asm_fix, ref_len); P(pl_asm_fix, patch); P(pl_asm_fix, tandem_repeat);
   return 0;
 }

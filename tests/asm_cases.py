@@ -107,3 +107,48 @@ def parity_cases():
     c["name"] = "no_change"
     out.append(c)
     return out
+
+
+def planted_region(seed=31, n=12000, gap=(3000, 100), deletion=(8000, 40), snps=(1500, 5500, 10500), step=5,
+                   length=150, unmapped_in_gap=False, truth=None):
+    """A draft made from a random truth with SNPs (the next base, ACGT
+    order), a run of N standing for truth[gap[0]:gap[0]+gap[1]] (0-based,
+    same length) and truth[deletion[0]:deletion[0]+deletion[1]] left out,
+    with error-free reads of the truth every `step` bases, alternately
+    reverse, aligned to the draft as an aligner would: M across the N run,
+    and across the deletion the longer side as M with the rest soft-clipped.
+    Returns dict(truth, draft, reads, loci of the draft's features); a
+    truth given is used as it is."""
+    rng = np.random.default_rng(seed)
+    d0, dl = deletion
+    while truth is None:
+        truth = random_bases(rng, n)
+        if truth[d0] == truth[d0 + dl]:
+            truth = None
+    n = len(truth)
+    g0, gl = gap
+    t = list(truth)
+    for p in snps:
+        t[p] = "ACGT"[("ACGT".index(truth[p]) + 1) % 4]
+    t[g0:g0 + gl] = "N" * gl
+    draft = "".join(t[:d0] + t[d0 + dl:])
+    reads = []
+    for k, s in enumerate(range(0, n - length + 1, step)):
+        seq = truth[s:s + length]
+        flag = 0x10 if k % 2 else 0
+        e = s + length
+        if e <= d0:
+            pos, cig = s + 1, "%dM" % length
+        elif s >= d0 + dl:
+            pos, cig = s - dl + 1, "%dM" % length
+        else:
+            left, right = max(d0 - s, 0), max(e - (d0 + dl), 0)
+            if left >= right:
+                pos, cig = s + 1, "%dM%dS" % (left, length - left)
+            else:
+                pos, cig = d0 + 1, "%dS%dM" % (length - right, right)
+        if unmapped_in_gap and s >= g0 and e <= g0 + gl:
+            continue
+        reads.append(dict(pos=pos, cigar=cig, seq=seq, qual=[30] * length, flag=flag, mapq=60))
+    return dict(truth=truth, draft=draft, reads=pack_reads(reads), gap=(g0 + 1, g0 + gl),
+                junction=d0 + 1, snps=[p + 1 for p in snps])

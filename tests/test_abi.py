@@ -86,11 +86,12 @@ def test_struct_layouts_match_c(tmp_path):
 
 def test_asm_opts_default_matches_pilon():
     """Assembler.K 47, minDepth 5, breakRadius floor 3K, trSafe on,
-    multiClosure and fixBreaks off, gapMargin 100000, minGap 10."""
+    multiClosure and fixBreaks off, gapMargin 100000, minGap 10, and the
+    gaps and local fixes on (--fix all)."""
     lib = C.CDLL(LIB)
     o = abi.PlAsmOpts()
     lib.pl_asm_opts_default(C.byref(o))
-    assert [getattr(o, f) for f, _ in abi.PlAsmOpts._fields_] == [47, 5, 141, 1, 0, 0, 100000, 10]
+    assert [getattr(o, f) for f, _ in abi.PlAsmOpts._fields_] == [47, 5, 141, 1, 0, 0, 100000, 10, 1, 1]
 
 
 @pytest.mark.parametrize("field,value", [("min_min_depth", -1), ("min_depth", float("nan")),
