@@ -316,7 +316,10 @@ pl_status pl_assemble_break(pl_ctx* ctx, const pl_read_soa* reads, const char* r
  * (a gap's with start > 0, a break's when its ref or patch exceeds 10
  * bases) joins bigFixList: pl_region_fix then applies it after the
  * snps/indels fixes and lists it in the changes, and pl_region_bed adds each
- * logged attempt's outcome line (Tracks.scala:211-213).  sets are the read
+ * logged attempt's outcome line (Tracks.scala:211-213); pl_region_vcf
+ * writes a Vcf.writeFixRecord record at each kept fix that is not a small
+ * one, the loci it replaces embedded, and a pl_region_bed after it counts
+ * those loci as changed (Pilon writes Pilon.bed after the VCF).  sets are the read
  * sets the region was built from, with their PL_BAM_* types and
  * maxInsertSize (BamFile.maxInsertSize); a jumps set is PL_E_INVAL
  * (recruitBadMates pairs reads by name).  opts->break_radius is

@@ -418,12 +418,15 @@ def log_fix(name, reg, sol, gap_size, tandem_repeat):  # GenomeRegion.logFix (Ge
 
 def reassemble_region(cfg, contig, start, stop, sets, name="ctg", new_name=None, offset=0, break_radius=None,
                       K=47, min_depth=5, tr_safe=True, multi_closure=False, fix_breaks=False, gap_margin=100000,
-                      min_gap=10, fix_gaps=True, fix_local=True):
+                      min_gap=10, fix_gaps=True, fix_local=True, vcf=False):
     """One region through identifyAndFixIssues with its gap filling and
-    break fixing (GenomeRegion.scala:307-413, --nostrays), then writeChanges
-    and Pilon.bed.  sets: [(ReadBatch, BAM type name, maxInsertSize)].
-    Returns a dict: status (4 where Pilon throws), changes, bases, bed (the
-    region's lines with the reassembly outcomes), log (logFix lines)."""
+    break fixing (GenomeRegion.scala:307-413, --nostrays), then writeChanges,
+    writeVcf when vcf (before Pilon.bed, whose change regions then include
+    the loci the reassembly records replace) and Pilon.bed.  sets:
+    [(ReadBatch, BAM type name, maxInsertSize)].  Returns a dict: status (4
+    where Pilon throws), changes, bases, vcf, bed (the region's lines with
+    the reassembly outcomes), log (logFix lines); changes_py and bases_py
+    are writeChanges and fixIssues restated here over the same lists."""
     from oracle import oracle as O
     from pilon_amd.abi import BAM_TYPES
     from pilon_amd.reads import pack_reads
@@ -483,5 +486,11 @@ def reassemble_region(cfg, contig, start, stop, sets, name="ctg", new_name=None,
             log.append(logged[1])
     big_list = list(reversed(big))                     # bigFixList ::= fix
     final = fix_issues(bases, start, small + big_list)  # fixIssues(smallFixList ++ bigFixList)
-    return {"status": 0, "changes": write_changes(snp + small + big_list, name, new_name, offset),
-            "bases": final, "bed": res["bed"] + "".join(bed_extra), "log": "".join(log), "run": res}
+    res2 = O.run_region(cfg, contig, start, stop, [b for b, _, _ in sets], name=name, new_name=new_name,
+                        offset=offset, vcf=O.BED | (O.VCF_ON if vcf else 0),
+                        set_types=[BAM_TYPES[t] for _, t, _ in sets], big=big_list)
+    if res2["status"]:
+        return {"status": res2["status"]}
+    return {"status": 0, "changes": res2["changes"], "bases": res2["bases"].decode(), "vcf": res2.get("vcf"),
+            "bed": res2["bed"] + "".join(bed_extra), "log": "".join(log), "run": res,
+            "changes_py": write_changes(snp + small + big_list, name, new_name, offset), "bases_py": final}

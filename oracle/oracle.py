@@ -55,6 +55,11 @@ def lib():
         L.orc_region_run_typed.argtypes = [C.POINTER(PlConfig), C.c_char_p, C.c_int64, C.c_char_p,
                                            C.c_int32, C.c_int32, C.POINTER(C.POINTER(PlReadSoa)),
                                            C.c_void_p, C.c_int32, C.c_char_p, C.c_int64, C.c_int32]
+        L.orc_region_run_big.restype = C.POINTER(OrcResult)
+        L.orc_region_run_big.argtypes = [C.POINTER(PlConfig), C.c_char_p, C.c_int64, C.c_char_p,
+                                         C.c_int32, C.c_int32, C.POINTER(C.POINTER(PlReadSoa)),
+                                         C.c_void_p, C.c_int32, C.c_char_p, C.c_int64, C.c_int32,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.orc_result_free.argtypes = [C.POINTER(OrcResult)]
         L.orc_fmt_af.argtypes = [C.c_float, C.c_char_p]
         L.orc_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
@@ -93,12 +98,15 @@ TRACK_NAMES = ["Changes", "Unconfirmed", "Copy Number", "Coverage", "Bad Coverag
                "Dip Coverage", "Physical Coverage", "Clipped Alignments", "Weighted Qual", "Weighted MQ", "GC"]
 
 
-def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0, vcf=0, set_types=None):
+def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, offset=0, vcf=0, set_types=None,
+               big=None):
     """One GenomeRegion through the oracle; returns a dict of numpy arrays.
     vcf: VCF_ON (| VCF_DIPLOID | VCF_QE) adds the region's VCF record lines
     (GenomeRegion.writeVcf) as out["vcf"]; TRACKS the 13 per-locus tracks as
     out["tracks"]; BED the region's Pilon.bed lines as out["bed"].  set_types:
-    one per read set, 1 for a "jumps" BAM (left out of fragment coverage)."""
+    one per read set, 1 for a "jumps" BAM (left out of fragment coverage).
+    big: bigFixList as (locus, was, patch) tuples in List order (local
+    reassembly's fixes, the last found first)."""
     L = lib()
     if isinstance(contig, str):
         contig = contig.encode()
@@ -110,9 +118,15 @@ def run_region(cfg, contig, start, stop, readsets, name="ctg", new_name=None, of
     types = None
     if set_types is not None:
         types = (C.c_int32 * max(1, len(set_types)))(*set_types)
-    rp = L.orc_region_run_typed(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
-                                arr, C.cast(types, C.c_void_p) if types is not None else None, len(soas),
-                                new_name.encode(), offset, vcf)
+    big = list(big or [])
+    nb = len(big)
+    b_loc = (C.c_int32 * max(1, nb))(*[f[0] for f in big])
+    b_was = (C.c_char_p * max(1, nb))(*[f[1].encode() for f in big])
+    b_patch = (C.c_char_p * max(1, nb))(*[f[2].encode() for f in big])
+    rp = L.orc_region_run_big(C.byref(cfg), contig, len(contig), name.encode(), start, stop,
+                              arr, C.cast(types, C.c_void_p) if types is not None else None, len(soas),
+                              new_name.encode(), offset, vcf, C.cast(b_loc, C.c_void_p), C.cast(b_was, C.c_void_p),
+                              C.cast(b_patch, C.c_void_p), nb)
     r = rp.contents
     out = {"status": r.status, "edge_read": r.edge_read, "edge_why": r.edge_why.decode()}
     if r.status == 0:

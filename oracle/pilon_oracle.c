@@ -975,10 +975,21 @@ static void bed_lines(const pl_config* cfg, const char* name, int32_t start, int
  * jumps (whose depth increments processBam leaves out of fragCoverage,
  * GenomeRegion.scala:287-300), 2 unpaired, 3 nanopore and 4 pacbio
  * (unpaired BAMs of longReadType 1 and 2, BamFile.scala:43-47). */
-orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
-                                 const char* name, int32_t start, int32_t stop,
-                                 const pl_read_soa* const* sets, const int32_t* set_types, int32_t n_sets,
-                                 const char* new_name, int64_t offset, int32_t vcf_flags) {
+static int fix_eq(const Fix* a, const Fix* b) {            /* tuple equality of (loc, was, patch) */
+  return a->loc == b->loc && a->wasLen == b->wasLen && a->patchLen == b->patchLen &&
+         (a->wasLen == 0 || !memcmp(a->was, b->was, (size_t)a->wasLen)) &&
+         (a->patchLen == 0 || !memcmp(a->patch, b->patch, (size_t)a->patchLen));
+}
+
+/* big_*: bigFixList (the local reassembly's fixes, GenomeRegion.scala:
+ * 382-410) in List order, applied after the small fixes, listed in the
+ * changes and written as Vcf.writeFixRecord records (n_big 0: none). */
+orc_result* orc_region_run_big(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
+                               const char* name, int32_t start, int32_t stop,
+                               const pl_read_soa* const* sets, const int32_t* set_types, int32_t n_sets,
+                               const char* new_name, int64_t offset, int32_t vcf_flags,
+                               const int32_t* big_loc, const char* const* big_was, const char* const* big_patch,
+                               int32_t n_big) {
   orc_result* res = (orc_result*)calloc(1, sizeof(orc_result));
   if (start < 1 || stop < start || stop > contig_len) { res->status = PL_E_INVAL; return res; }  /* GenomeRegion.scala:34 */
   PRegion R; memset(&R, 0, sizeof R);
@@ -1130,12 +1141,6 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
       t[11 * (size_t)size + i] = res->weightedMq[i];
       t[12 * (size_t)size + i] = gc[i];
     }
-    if (vcf_flags & 16) {
-      double fsum = 0.0;
-      for (int32_t i = 0; i < size; i++) fsum += (double)fragCov[i];
-      bed_lines(cfg, name, start, stop, contig, res->coverage, res->badCoverage, res->clips, confirmed, kindAt,
-                deleted, copyNumber, fragCov, fsum / (double)size, &res->bed_text, &res->bed_len);
-    }
     free(gc);
   }
 
@@ -1266,17 +1271,26 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
   int64_t blen = size;
   uint8_t* bases = (uint8_t*)malloc((size_t)size);
   memcpy(bases, contig + start - 1, (size_t)size);                       /* GenomeRegion.refBases */
-  Fix* tmp = (Fix*)malloc(sizeof(Fix) * (size_t)(snpL.n + smallL.n + 1));
+  Fix* tmp = (Fix*)malloc(sizeof(Fix) * (size_t)(snpL.n + smallL.n + n_big + 1));
+  Fix* bigV = (Fix*)calloc((size_t)n_big + 1, sizeof(Fix));
+  for (int32_t k = 0; k < n_big; k++) {
+    bigV[k].loc = big_loc[k];
+    bigV[k].was = (const uint8_t*)big_was[k]; bigV[k].wasLen = (int32_t)strlen(big_was[k]);
+    bigV[k].patch = (const uint8_t*)big_patch[k]; bigV[k].patchLen = (int32_t)strlen(big_patch[k]);
+  }
   fix_as_list(&snpL, tmp);
   fixIssues(&bases, &blen, &R, tmp, snpL.n);
   fix_as_list(&smallL, tmp);
-  fixIssues(&bases, &blen, &R, tmp, smallL.n);
+  memcpy(tmp + smallL.n, bigV, sizeof(Fix) * (size_t)n_big);
+  fixIssues(&bases, &blen, &R, tmp, smallL.n + n_big);
 
-  /* writeChanges (GenomeRegion.scala:645-657) */
+  /* writeChanges (GenomeRegion.scala:645-657) over snpFixList ++
+   * smallFixList ++ bigFixList */
   fix_as_list(&snpL, tmp);
   fix_as_list(&smallL, tmp + snpL.n);
-  Fix* kept = (Fix*)malloc(sizeof(Fix) * (size_t)(snpL.n + smallL.n + 1));
-  int32_t nk = fixFixList(tmp, snpL.n + smallL.n, kept);
+  memcpy(tmp + snpL.n + smallL.n, bigV, sizeof(Fix) * (size_t)n_big);
+  Fix* kept = (Fix*)malloc(sizeof(Fix) * (size_t)(snpL.n + smallL.n + n_big + 1));
+  int32_t nk = fixFixList(tmp, snpL.n + smallL.n + n_big, kept);
   char* txt = NULL; int64_t tl = 0, tc = 0;
   sb_append(&txt, &tl, &tc, "", 0);
   int64_t delta = 0;
@@ -1306,11 +1320,13 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
   res->stats.n_unknown_ops = (int32_t)R.unknownOps;
   res->stats.aligned_bases = R.alignedBases;
 
-  /* GenomeRegion.writeVcf (GenomeRegion.scala:623-643) without the
-   * reassembly records: every locus, embedded when deleted, each
-   * duplicationEvents region (copy number > 1 with slop 2000, longer than
-   * 10,000, :731-737) announced by Vcf.writeDup (Vcf.scala:194-202) before
-   * the records of its first locus */
+  /* GenomeRegion.writeVcf (GenomeRegion.scala:623-643): every locus,
+   * embedded when deleted, each duplicationEvents region (copy number > 1
+   * with slop 2000, longer than 10,000, :731-737) announced by Vcf.writeDup
+   * (Vcf.scala:194-202) before the records of its first locus, then at the
+   * locus of each kept fix from bigFixList and not from smallFixList its
+   * Vcf.writeFixRecord record (Vcf.scala:178-192), the loci it replaces
+   * marked deleted from there on */
   if (vcf_flags & 1) {
     VcfOut o; memset(&o, 0, sizeof o);
     o.cfg = cfg; o.contig = contig; o.name = name; o.start = start; o.minDepth = minDepth;
@@ -1320,8 +1336,8 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
     for (int32_t i = 0; i < size; i++) t[i] = copyNumber[i] > 1;
     RegionList dup = summary_regions(t, size, start, stop, 2000);
     free(t);
-    int32_t d = 0;
-    for (int32_t i = 0; i < size; i++) {
+    int32_t d = 0, fk = 0;
+    for (int32_t i = 0; i < size && !res->status; i++) {
       while (d < dup.n && dup.v[d].stop + 1 - dup.v[d].start <= 10000) d++;
       if (d < dup.n && dup.v[d].start == start + i) {
         const int32_t loc = dup.v[d].start - 1;                 /* writeDup: refBase(dup.start - 1) */
@@ -1332,12 +1348,42 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
         vcf_puts(&o, ";END="); vcf_putl(&o, dup.v[d].stop); vcf_puts(&o, ";IMPRECISE\tGT\t./.\n");
         d++;
       }
+      if (fk < nk && kept[fk].loc == start + i) {
+        const Fix* f = &kept[fk];
+        int inBig = 0, inSmall = 0;
+        for (int32_t k = 0; k < n_big && !inBig; k++) inBig = fix_eq(f, &bigV[k]);
+        for (int32_t k = 0; k < smallL.n && !inSmall; k++) inSmall = fix_eq(f, &smallL.v[k]);
+        if (inBig && !inSmall) {
+          const int32_t loc = f->loc - 1;
+          if (loc < start) { res->status = PL_E_EDGE; break; }  /* refBase's require(inRegion(loc)) */
+          const uint8_t rb = upper(contig[loc - 1]);
+          const int64_t refLen = 1 + f->wasLen, altLen = 1 + f->patchLen, svlen = altLen - refLen;
+          vcf_puts(&o, name); vcf_puts(&o, "\t"); vcf_putl(&o, loc); vcf_puts(&o, "\t.\t");
+          vcf_putn(&o, &rb, 1); vcf_putn(&o, f->was, f->wasLen); vcf_puts(&o, "\t");
+          vcf_putn(&o, &rb, 1); vcf_putn(&o, f->patch, f->patchLen);
+          vcf_puts(&o, "\t.\tPASS\tSVTYPE="); vcf_puts(&o, svlen < 0 ? "DEL" : "INS");
+          vcf_puts(&o, ";SVLEN="); vcf_putl(&o, svlen); vcf_puts(&o, ";END="); vcf_putl(&o, loc + refLen - 1);
+          if (f->patchLen && memchr(f->patch, 'N', (size_t)f->patchLen)) vcf_puts(&o, ";IMPRECISE");
+          vcf_puts(&o, "\tGT\t1/1\n");
+          for (int32_t j = 0; j < f->wasLen; j++) deleted[i + j] = 1;
+        }
+        fk++;
+      }
       vcf_record(&o, &R.pu[i], i, deleted[i], 1);
     }
     free(dup.v);
     res->vcf_text = o.buf; res->vcf_len = o.len;
   }
+  /* Pilon.bed comes after the VCF (Pilon.scala:111-116), so its change
+   * regions see the loci writeVcf marked deleted */
+  if ((vcf_flags & 16) && !res->status) {
+    double fsum = 0.0;
+    for (int32_t i = 0; i < size; i++) fsum += (double)fragCov[i];
+    bed_lines(cfg, name, start, stop, contig, res->coverage, res->badCoverage, res->clips, confirmed, kindAt,
+              deleted, copyNumber, fragCov, fsum / (double)size, &res->bed_text, &res->bed_len);
+  }
   free(copyNumber);
+  free(bigV);
 
   free(tmp); free(kept); free(snpL.v); free(smallL.v); free(oneChar);
   free(confirmed); free(deleted); free(kindAt); free(fragCov);
@@ -1346,6 +1392,14 @@ orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, in
   /* indel strings point into the arena: copied above, so it can go */
   arena_free(&R.ar);
   return res;
+}
+
+orc_result* orc_region_run_typed(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,
+                                 const char* name, int32_t start, int32_t stop,
+                                 const pl_read_soa* const* sets, const int32_t* set_types, int32_t n_sets,
+                                 const char* new_name, int64_t offset, int32_t vcf_flags) {
+  return orc_region_run_big(cfg, contig, contig_len, name, start, stop, sets, set_types, n_sets, new_name, offset,
+                            vcf_flags, NULL, NULL, NULL, 0);
 }
 
 orc_result* orc_region_run(const pl_config* cfg, const uint8_t* contig, int64_t contig_len,

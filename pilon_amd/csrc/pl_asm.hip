@@ -517,6 +517,7 @@ pl_status pl_region_reassemble(pl_region* r, const pl_read_soa* const* sets, con
   }
   r->big.clear();
   r->outcomes.clear();
+  r->vcf_embed.clear();
   std::vector<std::pair<int32_t, int32_t>> lists[7];
   if (pl_status s = region_issue_lists(r, lists)) return s;
   const Contig& ct = c->contigs[r->contig];

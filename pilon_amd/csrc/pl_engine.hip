@@ -269,6 +269,9 @@ struct pl_region {
   // logged the region, its outcome (reassemblyFixes) and the log line
   std::vector<plh::Fix> big;
   std::vector<plh::Outcome> outcomes;
+  // the loci pl_region_vcf marked deleted for the reassembly's records:
+  // Pilon.bed, written after the VCF, counts them as changed
+  std::vector<uint8_t> vcf_embed;
 };
 
 namespace {

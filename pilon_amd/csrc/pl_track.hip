@@ -272,8 +272,12 @@ void bed_of_region(const uint8_t* m, int32_t start, int32_t stop, const std::str
     }
 }
 
+// changed (or null): loci of the batch's only region to count as changed
+// besides the device's (those writeVcf marked deleted after a reassembly
+// record)
 pl_status bed_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, const std::vector<int32_t>& contigs,
-                      uint32_t flags, char** text, int64_t* len, double* device_ms) {
+                      uint32_t flags, char** text, int64_t* len, double* device_ms,
+                      const std::vector<uint8_t>* changed = nullptr) {
   *text = nullptr;
   *len = 0;
   const int64_t G = L->G;
@@ -285,6 +289,9 @@ pl_status bed_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   std::vector<uint8_t> mask((size_t)G);
   HIPCHK(hipMemcpyAsync(mask.data(), T.mask, (size_t)G, hipMemcpyDeviceToHost, L->stream));
   HIPCHK(hipStreamSynchronize(L->stream));
+  if (changed && nj == 1 && (int64_t)changed->size() == L->hjobs[0].stop + 1 - L->hjobs[0].start)
+    for (size_t i = 0; i < changed->size(); i++)
+      if ((*changed)[i]) mask[(size_t)L->hjobs[0].out + i] |= PL_ISSUE_CHANGED;
   // regions in parallel on the host's cores, joined in batch order
   std::vector<std::string> parts((size_t)nj);
   const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -406,7 +413,7 @@ pl_status pl_region_bed(pl_region* r, uint32_t flags, char** text, int64_t* len,
   if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
   pl_status s = on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
                                       const std::vector<int32_t>& contigs) {
-    return bed_of_lane(c, L, names, contigs, flags, text, len, device_ms);
+    return bed_of_lane(c, L, names, contigs, flags, text, len, device_ms, r->vcf_embed.empty() ? nullptr : &r->vcf_embed);
   });
   if (s || r->outcomes.empty()) return s;
   // the reassembly outcomes after the region's issue lines (Tracks.scala:211-213)

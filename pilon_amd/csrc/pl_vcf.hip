@@ -210,9 +210,19 @@ pl_status vcf_emit(Lane* L, const VcfText& V, Sink&& sink) {
   return piece(V.total);
 }
 
-// The VCF records of the lane's last batch; names[j] is region j's name.
+// A region's local reassembly in its VCF (GenomeRegion.writeVcf,
+// GenomeRegion.scala:623-643): the Vcf.writeFixRecord line of each record
+// at its locus (after a writeDup line there), and the loci the fixes
+// replace, embedded from then on.
+struct FixRecords {
+  std::vector<uint8_t> embed;                              // [region size]
+  std::vector<std::pair<int32_t, std::string>> lines;      // (locus, line), ascending
+};
+
+// The VCF records of the lane's last batch; names[j] is region j's name;
+// fr (or null) the reassembly records of the batch's only region.
 pl_status vcf_format(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, const std::vector<int32_t>& contigs,
-                     uint32_t flags, VcfText& out) {
+                     uint32_t flags, VcfText& out, const FixRecords* fr = nullptr) {
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = L->stream;
   const int64_t G = L->G;
@@ -236,6 +246,15 @@ pl_status vcf_format(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames,
   HIPCHK(hipMemcpyAsync(dnoff, noff.data(), 8 * noff.size(), hipMemcpyHostToDevice, st));
   V.names = dnames;
   V.name_off = dnoff;
+  if (fr) {
+    if (nj != 1 || (int64_t)fr->embed.size() != L->hjobs[0].stop + 1 - L->hjobs[0].start)
+      return fail(PL_E_INVAL, "reassembly records for a batch that is not their region");
+    uint8_t* emb;
+    if ((s = B.get(&emb, (size_t)G))) return s;
+    HIPCHK(hipMemsetAsync(emb, 0, (size_t)G, st));
+    HIPCHK(hipMemcpyAsync(emb + L->hjobs[0].out, fr->embed.data(), fr->embed.size(), hipMemcpyHostToDevice, st));
+    V.embed = emb;
+  }
   if ((s = B.get(&V.ilist_n, 4)) || (s = B.get(&V.imap, 4 * (size_t)G)) || (s = B.get(&V.len, 4 * (size_t)G)))
     return s;
   uint64_t* off;
@@ -313,15 +332,29 @@ pl_status vcf_format(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames,
       }
     }
   }
+  if (fr && !fr->lines.empty()) {
+    const JobDev& J = L->hjobs[0];
+    for (const auto& x : fr->lines) {
+      uint64_t at = 0;
+      HIPCHK(hipMemcpyAsync(&at, off + J.out + (x.first - J.start), 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      out.inserts.emplace_back(at, x.second);
+    }
+    // a locus's writeDup line comes before its reassembly record
+    std::stable_sort(out.inserts.begin(), out.inserts.end(),
+                     [](const std::pair<uint64_t, std::string>& a, const std::pair<uint64_t, std::string>& b) {
+                       return a.first < b.first;
+                     });
+  }
   return PL_OK;
 }
 
 pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames, const std::vector<int32_t>& contigs,
-                      uint32_t flags, char** text, int64_t* len, double* device_ms) {
+                      uint32_t flags, char** text, int64_t* len, double* device_ms, const FixRecords* fr = nullptr) {
   *text = nullptr;
   *len = 0;
   VcfText V;
-  if (pl_status s = vcf_format(c, L, rnames, contigs, flags, V)) return s;
+  if (pl_status s = vcf_format(c, L, rnames, contigs, flags, V, fr)) return s;
   const uint64_t n = V.bytes();
   char* out = (char*)malloc((size_t)n + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)n);
@@ -362,6 +395,46 @@ pl_status on_region_lane(pl_region* r, F&& f) {
   if (!s) s = f(c, L, names, contigs);
   if (r->temp_id >= 0 && !r->temp) { r->temp = std::move(c->rsets[id]); c->rs_version++; }
   return s;
+}
+
+// The reassembly records of a region's VCF: at each kept fix (fixFixList
+// over snpFixList ++ smallFixList ++ bigFixList) that is one of the big
+// fixes and not a small one, Vcf.writeFixRecord (Vcf.scala:178-192) and the
+// loci it replaces marked (GenomeRegion.scala:632-636).  PL_E_EDGE where
+// refBase(locus - 1) leaves the region, as Pilon throws there.
+pl_status region_fix_records(pl_region* r, FixRecords* out) {
+  out->embed.assign((size_t)(r->stop + 1 - r->start), 0);
+  out->lines.clear();
+  if (r->big.empty()) return PL_OK;
+  const Contig& ct = r->ctx->contigs[r->contig];
+  std::vector<plh::Fix> snp, small;
+  plh::FixResult counts;
+  plh::fix_lists(r->ctx->pc, r->changes, snp, small, counts);
+  const std::vector<plh::Fix> big(r->big.rbegin(), r->big.rend());
+  std::vector<plh::Fix> all = snp;
+  all.insert(all.end(), small.begin(), small.end());
+  all.insert(all.end(), big.begin(), big.end());
+  auto same = [](const plh::Fix& a, const plh::Fix& b) { return a.loc == b.loc && a.was == b.was && a.patch == b.patch; };
+  auto in = [&](const std::vector<plh::Fix>& l, const plh::Fix& f) {
+    for (const plh::Fix& x : l) if (same(x, f)) return true;
+    return false;
+  };
+  for (const plh::Fix& f : plh::fix_fix_list(all)) {
+    if (!in(big, f) || in(small, f)) continue;
+    const int32_t loc = f.loc - 1;
+    if (loc < r->start) return fail(PL_E_EDGE, "Pilon 1.24 throws here: a reassembly record at the region's first locus");
+    const char rb = (char)to_upper(ct.host[(size_t)loc - 1]);
+    const std::string ref = rb + f.was, alt = rb + f.patch;
+    const int64_t svlen = (int64_t)alt.size() - (int64_t)ref.size();
+    std::string line = ct.name + "\t" + std::to_string(loc) + "\t.\t" + ref + "\t" + alt + "\t.\tPASS\tSVTYPE=" +
+                       (svlen < 0 ? "DEL" : "INS") + ";SVLEN=" + std::to_string(svlen) + ";END=" +
+                       std::to_string((int64_t)loc + (int64_t)ref.size() - 1);
+    if (f.patch.find('N') != std::string::npos) line += ";IMPRECISE";
+    line += "\tGT\t1/1\n";
+    out->lines.emplace_back(f.loc, std::move(line));
+    for (size_t j = 0; j < f.was.size(); j++) out->embed[(size_t)(f.loc - r->start) + j] = 1;
+  }
+  return PL_OK;
 }
 
 // the region names and contigs of the lane's last batch (PL_E_INVAL when a
@@ -406,10 +479,14 @@ pl_status pl_batch_vcf_stream(pl_ctx* c, uint32_t flags, pl_text_sink sink, void
 
 pl_status pl_region_vcf(pl_region* r, uint32_t flags, char** text, int64_t* len, double* device_ms) {
   if (!r || !text || !len) return fail(PL_E_INVAL, "null argument");
-  return on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
-                                const std::vector<int32_t>& contigs) {
-    return vcf_of_lane(c, L, names, contigs, flags, text, len, device_ms);
+  FixRecords fr;
+  if (pl_status s = region_fix_records(r, &fr)) return s;
+  pl_status s = on_region_lane(r, [&](pl_ctx* c, Lane* L, const std::vector<std::string>& names,
+                                      const std::vector<int32_t>& contigs) {
+    return vcf_of_lane(c, L, names, contigs, flags, text, len, device_ms, r->big.empty() ? nullptr : &fr);
   });
+  if (!s && !r->big.empty()) r->vcf_embed = std::move(fr.embed);
+  return s;
 }
 
 }  // extern "C"

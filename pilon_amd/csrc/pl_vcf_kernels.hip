@@ -27,6 +27,7 @@ struct VcfArgs {
   uint32_t* len;              // [G] bytes of the column's lines
   const uint64_t* off;        // [G + 1] their offsets
   char* text;
+  const uint8_t* embed;       // [G] or null: loci writeVcf marks deleted after a reassembly record
 };
 
 namespace {
@@ -238,7 +239,7 @@ __device__ __forceinline__ uint32_t vcf_locus(char* out, const VcfArgs& V, const
   VcfCol v = load_vcf_col(V.A.pile, V.A.G, g);
   const uint16_t f = V.A.calls[g].flags;
   if (f & PL_CF_WIDE) wide_patch(v.c, V.A.wide, V.A.wide_cap, V.A.win_wide, J, V.A.W, g);
-  const bool deleted = (f & PL_CF_DELETED) != 0;
+  const bool deleted = (f & PL_CF_DELETED) != 0 || (V.embed && V.embed[g]);
   int64_t rec = -1;
   if (i > 0 && (f & (PL_CF_INSERTION | PL_CF_DELETION))) rec = V.irec[V.imap[g]];
   vcf_record<WRITE>(o, V, J, R, name, name_len, minDepth, v, f, rec, i, deleted, true);

@@ -3,7 +3,8 @@ gaps and possible breaks from the device's issue regions, recruitment from
 the read sets, the K-mer graph on the GPU, the walk and joins on the host,
 the fixes in pl_region_fix and pl_region_bed) against the oracle's
 restatement of identifyAndFixIssues, exactly: .changes, polished bases,
-Pilon.bed lines and logFix lines."""
+the VCF with its writeFixRecord records, Pilon.bed lines before and after
+the VCF, and logFix lines."""
 import pytest
 
 from pilon_amd.abi import default_config
@@ -23,7 +24,8 @@ def both(c, conf, sets_of=None, **kw):
     d = c["draft"]
     sets = sets_of(c) if sets_of else [(c["reads"], "frags", 500)]
     ref = A.reassemble_region(conf, d, 1, len(d), sets, name="ctg", **kw)
-    assert ref["status"] == 0
+    ref_v = A.reassemble_region(conf, d, 1, len(d), sets, name="ctg", vcf=True, **kw)
+    assert ref["status"] == 0 and ref_v["status"] == 0
     ctx = Context(conf, device=0)
     try:
         ctx.ref_upload(0, "ctg", d)
@@ -35,6 +37,8 @@ def both(c, conf, sets_of=None, **kw):
         n, log = reg.reassemble(sets, **opts)
         changes, bases = reg.fix()
         bed, _ = reg.bed(track_line=False)
+        vcf, _ = reg.vcf()
+        bed_after, _ = reg.bed(track_line=False)
         reg.close()
     finally:
         ctx.close()
@@ -42,7 +46,9 @@ def both(c, conf, sets_of=None, **kw):
     assert changes == ref["changes"]
     assert bases.decode() == ref["bases"]
     assert bed == ref["bed"]
-    return ref, n
+    assert vcf == ref_v["vcf"]
+    assert bed_after == ref_v["bed"]
+    return ref_v, n
 
 
 def test_planted_region():

@@ -61,6 +61,44 @@ def test_each_fix_kind_can_be_left_out(planted):
     assert r["log"] == "" and r["changes"] == r["run"]["changes"]
 
 
+def test_python_fix_lists_agree_with_the_c_oracle(planted):
+    """The C oracle's fixIssues(smallFixList ++ bigFixList) and writeChanges
+    over the big fixes given to it equal the Python restatement's."""
+    r = run(planted)
+    assert r["changes"] == r["changes_py"] and r["bases"] == r["bases_py"]
+    r = run(AC.planted_region(seed=33, n=20000, gap=(4000, 120), deletion=(15000, 25), snps=(1000, 18000)))
+    assert r["changes"] == r["changes_py"] and r["bases"] == r["bases_py"]
+
+
+def test_fix_records_in_the_vcf(planted):
+    """writeVcf with the reassembly's fixes: a writeFixRecord line at each
+    big fix's locus (SVTYPE INS for the closed gap, whose was and patch are
+    both 100 long, and for the restored deletion), the gap's loci embedded
+    ("Del" filter, DP from the base count); the change regions of Pilon.bed
+    then cover the replaced N run."""
+    c = planted
+    r = run(c, vcf=True)
+    g0, g1 = c["gap"]
+    j = c["junction"]
+    t, d = c["truth"], c["draft"]
+    lines = r["vcf"].splitlines()
+    fixes = [l for l in lines if "SVTYPE=" in l and "<DUP>" not in l]
+    assert fixes == [
+        "ctg\t%d\t.\t%s\t%s\t.\tPASS\tSVTYPE=INS;SVLEN=0;END=%d\tGT\t1/1" % (g0 - 1, d[g0 - 2] + "N" * 100,
+                                                                          d[g0 - 2] + t[g0 - 1:g1], g1),
+        "ctg\t%d\t.\t%s\t%s\t.\tPASS\tSVTYPE=INS;SVLEN=40;END=%d\tGT\t1/1" % (j - 1, d[j - 2],
+                                                                           d[j - 2] + t[j - 1:j + 39], j - 1)]
+    at = lines.index(fixes[0])
+    assert lines[at + 1].split("\t")[1] == str(g0)        # before the records of the fix's locus
+    for k in range(100):
+        assert lines[at + 1 + k].split("\t")[6] == "Del"
+    assert lines[at + 101].split("\t")[6] != "Del"
+    xs = [l.split("\t")[1:3] for l in r["bed"].splitlines() if l.split("\t")[3] == "X"]
+    assert [str(g0 - 1), str(g1)] in xs
+    assert [str(g0 - 1), str(g1)] not in [l.split("\t")[1:3] for l in run(c)["bed"].splitlines()
+                                          if l.split("\t")[3] == "X"]
+
+
 def test_write_changes_agrees_with_the_c_oracle():
     """With no reassembly the Python writeChanges over the recovered fix
     lists is the C oracle's own .changes text, offset and names included."""
