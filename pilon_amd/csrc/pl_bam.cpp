@@ -15,13 +15,17 @@
 //     and in full in the CG:B,I tag.
 //
 // Each slice of the file (about 64 MiB of inflated BAM) goes through three
-// steps: the blocks are inflated in parallel, the records are delimited and
-// assigned their output offsets in one sequential pass, and the records are
-// decoded into the per-reference batches in parallel.  Non-ACGT reads get
-// their nibbles copied afterwards, in file order.
+// steps, each in parallel: the blocks are inflated; the records are
+// delimited (the block_size chain, walked per segment from a guessed start
+// and checked against the previous segment's end) and given their output
+// offsets (runs of one reference, their bases assigned in file order);
+// and the records are decoded into the per-reference batches.  Non-ACGT
+// reads get their nibbles copied afterwards, in file order.
 #include "../../include/pilon_bam.h"
 
 #include <libdeflate.h>
+
+#include <sys/mman.h>
 
 #include <algorithm>
 #include <atomic>
@@ -30,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <new>
 #include <string>
@@ -95,17 +100,70 @@ struct PairTable {
 };
 const PairTable kPair;
 
-// Vectors whose growth leaves the new elements uninitialised: the parallel
-// decode writes every one of them (and touches its pages first).
+// An array of a trivial type that only grows: its pages are anonymous
+// memory, remapped (not copied) when it grows, and new elements are zero
+// (fresh pages), first touched by the threads that write them.
 template <class T>
-struct NoInit : std::allocator<T> {
-  template <class U> struct rebind { using other = NoInit<U>; };
-  NoInit() = default;
-  template <class U> NoInit(const NoInit<U>&) {}
-  template <class U> void construct(U* p) { ::new ((void*)p) U; }
-  template <class U, class... A> void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+struct Buf {
+  T* p = nullptr;
+  size_t n = 0, cap = 0, bytes = 0;
+  Buf() = default;
+  Buf(std::initializer_list<T> il) { for (const T& v : il) push_back(v); }
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  Buf(Buf&& o) noexcept : p(o.p), n(o.n), cap(o.cap), bytes(o.bytes) { o.p = nullptr; o.n = o.cap = o.bytes = 0; }
+  Buf& operator=(Buf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; cap = o.cap; bytes = o.bytes;
+      o.p = nullptr; o.n = o.cap = o.bytes = 0;
+    }
+    return *this;
+  }
+  ~Buf() { release(); }
+  void release() {
+    if (p) munmap(p, bytes);
+    p = nullptr;
+    n = cap = bytes = 0;
+  }
+  void reserve(size_t c) {
+    if (c <= cap) return;
+    const size_t want = std::max(c, cap + cap / 2);
+    const size_t nb = (want * sizeof(T) + 4095) / 4096 * 4096;
+    void* q = p ? mremap(p, bytes, nb, MREMAP_MAYMOVE)
+                : mmap(nullptr, nb, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (q == MAP_FAILED) throw std::bad_alloc();
+    p = (T*)q;
+    bytes = nb;
+    cap = nb / sizeof(T);
+  }
+  void resize(size_t m) { reserve(m); n = m; }
+  void resize(size_t m, const T& v) {
+    reserve(m);
+    if (!(v == T())) for (size_t i = n; i < m; i++) p[i] = v;
+    n = std::max(n, m);
+  }
+  void push_back(const T& v) { reserve(n + 1); p[n++] = v; }
+  template <class It> void insert(T* at, It a, It b) {       // appends only
+    (void)at;
+    const size_t k = (size_t)(b - a);
+    reserve(n + k);
+    std::copy(a, b, p + n);
+    n += k;
+  }
+  T* data() { return p; }
+  const T* data() const { return p; }
+  size_t size() const { return n; }
+  bool empty() const { return n == 0; }
+  T* begin() { return p; }
+  T* end() { return p + n; }
+  const T* begin() const { return p; }
+  const T* end() const { return p + n; }
+  T& back() { return p[n - 1]; }
+  const T& back() const { return p[n - 1]; }
+  T& operator[](size_t i) { return p[i]; }
+  const T& operator[](size_t i) const { return p[i]; }
 };
-template <class T> using Buf = std::vector<T, NoInit<T>>;
 
 struct RefReads {
   Buf<int32_t> pos, tlen;
@@ -263,23 +321,103 @@ pl_status inflate_blocks(const std::vector<uint8_t>& cb, std::vector<Block>& blo
   return PL_OK;
 }
 
-// Delimit the records of ub[u0, n) and decode every placed one into its
-// reference's batch; returns the offset of the first incomplete record.
-pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, size_t n, int nth, size_t* used) {
-  const uint8_t* u = ub.data();
-  const size_t nref = B->refs.size();
-  // record starts: the block_size chain
-  std::vector<size_t> offs;
-  size_t p = u0;
-  while (p + 4 <= n) {
+// A record start that fits the SAMv1 record layout (block_size, refIDs,
+// a NUL-terminated name, the variable fields within block_size): the test
+// a speculative record search uses.
+bool plausible_record(const uint8_t* u, size_t q, size_t n, int32_t nref) {
+  if (q + 36 > n) return false;
+  const uint8_t* r = u + q;
+  const uint32_t bs = rd32(r);
+  if (bs < 32 || bs > (1u << 28)) return false;
+  const int32_t ref = rdi32(r + 4), pos = rdi32(r + 8), next_ref = rdi32(r + 24);
+  if (ref < -1 || ref >= nref || next_ref < -1 || next_ref >= nref || pos < -1) return false;
+  const uint32_t l_name = r[12], ncig = rd16(r + 16), lseq = rd32(r + 20);
+  if (l_name < 1 || lseq > bs) return false;
+  if (36 + (size_t)l_name + 4 * (size_t)ncig + ((size_t)lseq + 1) / 2 + lseq > 4 + (size_t)bs) return false;
+  return q + 36 + l_name > n || r[36 + l_name - 1] == 0;
+}
+
+// The block_size chain from record start p: the starts below `end` go to
+// offs; *exit is the first start at or past `end`, or where the data stops
+// holding a whole record; *bad is the start of a record whose block_size is
+// below the fixed 32 bytes (the chain stops there).
+void walk_chain(const uint8_t* u, size_t p, size_t end, size_t n, std::vector<size_t>& offs, size_t* exit,
+                size_t* bad) {
+  *bad = SIZE_MAX;
+  while (p < end && p + 4 <= n) {
     const uint32_t bs = rd32(u + p);
-    if (bs < 32) return fail(PL_E_INVAL, "BAM record %lld: block_size %u below the fixed 32 bytes",
-                             (long long)(B->st.records + (int64_t)offs.size()), bs);
+    if (bs < 32) { *bad = p; break; }
     if (p + 4 + (size_t)bs > n) break;
     offs.push_back(p);
     p += 4 + (size_t)bs;
   }
-  *used = p;
+  *exit = p;
+}
+
+// Record starts of ub[u0, n), as the sequential block_size chain gives
+// them, found in parallel: the range is cut into one segment per thread;
+// each segment after the first guesses its first record start (the first
+// offset from which three records in a row fit the layout) and walks the
+// chain from there; a pass over the segments in order then keeps a walk
+// only where its guess is the previous segment's exit, and walks again
+// from that exit where it is not.  The result is the sequential chain
+// whatever the guesses.  *used = the start of the first incomplete record.
+pl_status delimit_records(pl_bam* B, const uint8_t* u, size_t u0, size_t n, int nth, std::vector<size_t>& offs,
+                          size_t* used) {
+  const int32_t nref = (int32_t)B->refs.size();
+  const int T = (n - u0 > ((size_t)4 << 20)) ? std::max(1, nth) : 1;
+  struct Seg { size_t lo, hi, start = SIZE_MAX, exit = 0, bad = SIZE_MAX; std::vector<size_t> offs; };
+  std::vector<Seg> seg((size_t)T);
+  for (int t = 0; t < T; t++) {
+    seg[(size_t)t].lo = u0 + (n - u0) * (size_t)t / (size_t)T;
+    seg[(size_t)t].hi = t + 1 == T ? n : u0 + (n - u0) * (size_t)(t + 1) / (size_t)T;
+  }
+  parallel_for(T, T, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; t++) {
+      Seg& S = seg[(size_t)t];
+      if (t == 0) {
+        S.start = u0;
+      } else {
+        for (size_t q = S.lo; q < S.hi && q < S.lo + ((size_t)1 << 20); q++) {
+          bool ok = true;
+          size_t r = q;
+          for (int k = 0; k < 3 && ok && r + 36 <= n; k++) {
+            ok = plausible_record(u, r, n, nref);
+            r += 4 + (size_t)rd32(u + r);
+          }
+          if (ok) { S.start = q; break; }
+        }
+      }
+      if (S.start != SIZE_MAX) {
+        S.offs.reserve((S.hi - S.lo) / 200 + 16);
+        walk_chain(u, S.start, S.hi, n, S.offs, &S.exit, &S.bad);
+      }
+    }
+  });
+  size_t expected = u0;
+  for (int t = 0; t < T; t++) {
+    Seg& S = seg[(size_t)t];
+    if (S.start != expected) {                 // a wrong guess: the chain from the true start
+      S.offs.clear();
+      walk_chain(u, expected, t + 1 == T ? n : std::max(S.hi, expected), n, S.offs, &S.exit, &S.bad);
+    }
+    if (S.bad != SIZE_MAX)
+      return fail(PL_E_INVAL, "BAM record %lld: block_size %u below the fixed 32 bytes",
+                  (long long)(B->st.records + (int64_t)offs.size() + (int64_t)S.offs.size()), rd32(u + S.bad));
+    offs.insert(offs.end(), S.offs.begin(), S.offs.end());
+    expected = S.exit;
+  }
+  *used = expected;
+  return PL_OK;
+}
+
+// Delimit the records of ub[u0, n) and decode every placed one into its
+// reference's batch; returns the offset of the first incomplete record.
+pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, size_t n, int nth, size_t* used) {
+  const uint8_t* u = ub.data();
+  const int64_t nref = (int64_t)B->refs.size();
+  std::vector<size_t> offs;
+  if (pl_status s = delimit_records(B, u, u0, n, nth, offs, used)) return s;
   // per-record facts, in parallel: lengths checked, the CG tag resolved
   const int64_t N = (int64_t)offs.size();
   std::vector<Rec> all((size_t)N);
@@ -317,37 +455,99 @@ pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, s
   if (bad.load() != INT64_MAX)
     return fail(PL_E_INVAL, "BAM record %lld: name, CIGAR, SEQ and QUAL exceed block_size",
                 (long long)(B->st.records + bad.load()));
-  // output offsets per reference, in file order; coordinate order checked
-  std::vector<Rec> recs;
-  recs.reserve((size_t)N);
-  std::vector<int64_t> add_n(nref, 0), add_c(nref, 0), add_s(nref, 0);
+  // output offsets per reference, in file order: runs of one reference per
+  // chunk (in parallel), their bases assigned in file order (serially, one
+  // step per run), then every record's offsets (in parallel)
+  struct Run {
+    int32_t ref;
+    int64_t k0, k1;                    // records [k0, k1)
+    int64_t nrec = 0, ncig = 0, nseq = 0;
+    int32_t first_pos, last_pos;
+    int64_t unsorted = -1;             // first record placed before its predecessor
+    int64_t i0 = 0, c0 = 0, s0 = 0;
+  };
+  const int TC = (int)std::max<int64_t>(1, std::min<int64_t>(nth, N / 4096 + 1));
+  std::vector<std::vector<Run>> runs((size_t)TC);
+  std::vector<int64_t> badref((size_t)TC, -1);
+  parallel_for(TC, TC, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; t++) {
+      const int64_t a = N * t / TC, b = N * (t + 1) / TC;
+      std::vector<Run>& V = runs[(size_t)t];
+      for (int64_t k = a; k < b; k++) {
+        const Rec& rc = all[(size_t)k];
+        if (rc.ref >= nref || rc.ref < -1) { badref[(size_t)t] = k; break; }
+        if (rc.ref < 0) continue;
+        const int32_t pos1 = (int32_t)rc.i + 1;
+        if (V.empty() || V.back().ref != rc.ref || V.back().k1 != k) {
+          Run r;
+          r.ref = rc.ref; r.k0 = k; r.k1 = k; r.first_pos = pos1; r.last_pos = pos1;
+          V.push_back(r);
+        }
+        Run& r = V.back();
+        if (pos1 < r.last_pos && r.unsorted < 0) r.unsorted = k;
+        r.last_pos = std::max(r.last_pos, pos1);
+        r.k1 = k + 1;
+        r.nrec++;
+        r.ncig += rc.ncig;
+        r.nseq += rc.lseq;
+      }
+    }
+  });
+  std::vector<int64_t> add_n((size_t)nref, 0), add_c((size_t)nref, 0), add_s((size_t)nref, 0);
   std::vector<int32_t> touched;
-  for (int64_t k = 0; k < N; k++) {
-    Rec rc = all[(size_t)k];
-    const int32_t ref = rc.ref;
-    if (ref >= (int32_t)nref || ref < -1)
-      return fail(PL_E_INVAL, "BAM record %lld: refID %d outside the dictionary", (long long)(B->st.records + k), ref);
-    if (ref < 0) continue;
-    RefReads& R = B->refs[(size_t)ref];
-    const int32_t pos1 = (int32_t)rc.i + 1;
-    if (pos1 < R.last_pos)
-      return fail(PL_E_INVAL, "BAM is not coordinate-sorted: %s pos %d after %d", B->names[(size_t)ref].c_str(),
-                  pos1, R.last_pos);
-    R.last_pos = pos1;
-    if (add_n[(size_t)ref] == 0) touched.push_back(ref);
-    rc.i = R.n + add_n[(size_t)ref]++;
-    rc.coff = R.cigar_off.back() + add_c[(size_t)ref];
-    rc.soff = R.seq_off.back() + add_s[(size_t)ref];
-    add_c[(size_t)ref] += rc.ncig;
-    add_s[(size_t)ref] += rc.lseq;
-    recs.push_back(rc);
+  int64_t placed = 0;
+  for (int t = 0; t < TC; t++) {
+    for (Run& r : runs[(size_t)t]) {
+      RefReads& R = B->refs[(size_t)r.ref];
+      if (r.first_pos < R.last_pos || r.unsorted >= 0) {
+        // the first record in file order placed before its predecessor
+        int32_t prev = R.last_pos;
+        for (int64_t k = r.k0; k < r.k1; k++) {
+          const Rec& rc = all[(size_t)k];
+          if (rc.ref != r.ref) continue;
+          const int32_t pos1 = (int32_t)rc.i + 1;
+          if (pos1 < prev)
+            return fail(PL_E_INVAL, "BAM is not coordinate-sorted: %s pos %d after %d", B->names[(size_t)r.ref].c_str(),
+                        pos1, prev);
+          prev = pos1;
+        }
+      }
+      R.last_pos = r.last_pos;
+      if (add_n[(size_t)r.ref] == 0) touched.push_back(r.ref);
+      r.i0 = R.n + add_n[(size_t)r.ref];
+      r.c0 = R.cigar_off.back() + add_c[(size_t)r.ref];
+      r.s0 = R.seq_off.back() + add_s[(size_t)r.ref];
+      add_n[(size_t)r.ref] += r.nrec;
+      add_c[(size_t)r.ref] += r.ncig;
+      add_s[(size_t)r.ref] += r.nseq;
+      placed += r.nrec;
+    }
+    if (badref[(size_t)t] >= 0) {
+      const int64_t k = badref[(size_t)t];
+      return fail(PL_E_INVAL, "BAM record %lld: refID %d outside the dictionary", (long long)(B->st.records + k),
+                  all[(size_t)k].ref);
+    }
   }
+  parallel_for(TC, TC, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; t++)
+      for (const Run& r : runs[(size_t)t]) {
+        int64_t i = r.i0, c = r.c0, so = r.s0;
+        for (int64_t k = r.k0; k < r.k1; k++) {
+          Rec& rc = all[(size_t)k];
+          if (rc.ref != r.ref) continue;
+          rc.i = i++;
+          rc.coff = c;
+          rc.soff = so;
+          c += rc.ncig;
+          so += rc.lseq;
+        }
+      }
+  });
   B->st.records += N;
-  // grow the batches of the references this slice touches
-  std::vector<size_t> old_b2(nref, 0);
+  // grow the batches of the references this slice touches (new bytes are
+  // zero: bytes of bases2 that two reads share are OR-ed by both)
   for (int32_t ref : touched) {
     RefReads& R = B->refs[(size_t)ref];
-    old_b2[(size_t)ref] = R.bases2.size();
     const size_t nn = (size_t)(R.n + add_n[(size_t)ref]);
     const int64_t cn = R.cigar_off.back() + add_c[(size_t)ref];
     const int64_t sn = R.seq_off.back() + add_s[(size_t)ref];
@@ -357,22 +557,12 @@ pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, s
     R.quals.resize((size_t)sn);
     R.bases2.resize((size_t)(sn + 3) / 4);
   }
-  // bytes of bases2 that two reads share are OR-ed by both: clear the first
-  // and last byte of every read's range that this slice added (the bytes
-  // wholly inside a read are stored whole; the byte a read of an earlier
-  // slice ended in is already set up)
-  for (const Rec& rc : recs) {
-    if (!rc.lseq) continue;
-    RefReads& R = B->refs[(size_t)rc.ref];
-    const size_t b0 = (size_t)(rc.soff >> 2), b1 = (size_t)((rc.soff + rc.lseq - 1) >> 2);
-    if (b0 >= old_b2[(size_t)rc.ref]) R.bases2[b0] = 0;
-    if (b1 >= old_b2[(size_t)rc.ref]) R.bases2[b1] = 0;
-  }
   // decode in parallel; bytes of bases2 shared by two reads are OR-ed atomically
-  std::vector<uint8_t> exotic(recs.size(), 0);
-  parallel_for(nth, (int64_t)recs.size(), [&](int, int64_t lo, int64_t hi) {
+  std::vector<uint8_t> exotic((size_t)N, 0);
+  parallel_for(nth, N, [&](int, int64_t lo, int64_t hi) {
     for (int64_t k = lo; k < hi; k++) {
-      const Rec& rc = recs[(size_t)k];
+      const Rec& rc = all[(size_t)k];
+      if (rc.ref < 0) continue;
       RefReads& R = B->refs[(size_t)rc.ref];
       const uint8_t* r = u + rc.u;
       const uint32_t l_name = r[12];
@@ -423,9 +613,9 @@ pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, s
     }
   });
   // nibbles of the non-ACGT reads, in file order, each from an even nibble
-  for (size_t k = 0; k < recs.size(); k++) {
+  for (size_t k = 0; k < (size_t)N; k++) {
     if (!exotic[k]) continue;
-    const Rec& rc = recs[k];
+    const Rec& rc = all[k];
     RefReads& R = B->refs[(size_t)rc.ref];
     const uint8_t* r = u + rc.u;
     const uint8_t* seq = r + 36 + r[12] + 4 * (size_t)rd16(r + 16);
@@ -436,7 +626,7 @@ pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, s
     B->st.exotic++;
   }
   for (int32_t ref : touched) B->refs[(size_t)ref].n += add_n[(size_t)ref];
-  B->st.placed += (int64_t)recs.size();
+  B->st.placed += placed;
   return PL_OK;
 }
 

@@ -152,6 +152,39 @@ def test_synthetic_roundtrip_many_contigs(tmp_path, monkeypatch):
     assert st["bam_bytes"] == nbytes
 
 
+def test_synthetic_roundtrip_segmented_delimiting(tmp_path, monkeypatch):
+    """Slices of 8 MiB on five threads: each slice's records are delimited
+    per segment from a guessed start, checked against the previous
+    segment's end."""
+    st, nbytes = _synth_roundtrip(tmp_path, "c2", 0.02, {"n_rate": 0.001}, 5, monkeypatch, 8192)
+    assert st["bam_bytes"] == nbytes and nbytes > 2 * (8192 << 10)
+
+
+def test_lookalike_record_chains_in_quals(tmp_path, monkeypatch):
+    """QUAL bytes that hold three well-formed records in a row: a segment
+    whose start falls before them guesses them as its first record, the
+    check against the previous segment's end rejects the guess, and the
+    chain from the true start gives the same batch as one thread does."""
+    fake = b"".join(S.record(0, 7 + k, [w(1, "M")], "", [], name="x") for k in range(3))
+    fake_q = list(fake)
+    n_rec, L = 60000, 4 * len(fake_q)
+    seq = "ACGT" * (L // 4)
+    q = (fake_q * 4)[:L]
+    body = S.header([("c", 10 ** 7)]) + b"".join(S.record(0, 10 * k, [w(L, "M")], seq, q, name="r%d" % k)
+                                                 for k in range(n_rec))
+    p = tmp_path / "lookalike.bam"
+    p.write_bytes(S.bgzf(body, level=1))
+    assert len(body) > (16 << 20)
+    monkeypatch.setenv("PILON_BAM_SLICE_KIB", "16384")
+    with BamFile(p, threads=7) as b:
+        many = b.batch(0)
+        assert b.stats["placed"] == n_rec
+    with BamFile(p, threads=1) as b:
+        one = b.batch(0)
+    S.same_batch(one, many, "lookalike chains")
+    assert list(many.pos[:3]) == [1, 11, 21] and bytes(many.quals[:len(fake)].astype("uint8")) == fake
+
+
 def test_native_writer_matches_the_spec_writer(tmp_path):
     """The generator's BAM writer (used for workload-size files) produces the
     same bytes as the spec-literal writer at the same level and block size."""
