@@ -442,8 +442,8 @@ def main():
 def bam_producer(data, ctx, args, W, B, aligned):
     """The BAM -> read batch producer on this workload: its reads written as a
     coordinate-sorted BAM (BGZF, deflate level 1, one reference per contig),
-    decoded on the host's threads (libpilonbam), uploaded as read sets and
-    piled in one batched pass.  End to end = decode + upload + pile, from a
+    decoded on the host's threads (libpilonbam), uploaded as read sets in
+    one pass (pl_readset_upload_many) and piled in one batched pass.  End to end = decode + upload + pile, from a
     file in the page cache."""
     import shutil
     import tempfile
@@ -464,7 +464,7 @@ def bam_producer(data, ctx, args, W, B, aligned):
         t_dec = time.perf_counter() - t0
         st = bam.stats
         t0 = time.perf_counter()
-        rids = [ctx.readset_upload(ci, bam.reads(ci)) for ci in range(data.n)]
+        rids = ctx.readset_upload_many([(ci, bam.reads(ci), "frags") for ci in range(data.n)])
         dev_sync()
         t_up = time.perf_counter() - t0
         regions = regions_for(args.config, data, rids)

@@ -177,6 +177,17 @@ pl_status pl_readset_upload(pl_ctx* ctx, int32_t contig_id,
 pl_status pl_readset_upload_typed(pl_ctx* ctx, int32_t contig_id, const pl_read_soa* reads,
                                   int32_t bam_type, int32_t* readset_id);
 
+/* Many read sets at once, as pl_readset_upload_typed would make each (one
+ * per contig, e.g. every reference of a BAM): reads[i] of contig
+ * contig_ids[i] with BAM type bam_types[i] (null: all PL_BAM_FRAGS); ids[i]
+ * gets its id.  The host arrays go to the device through pinned staging
+ * buffers filled by host threads, and the sets share device allocations, so
+ * the cost is that of the bytes rather than of the sets.  No counterpart in
+ * the reference: the resident form's loader (BamFile.process reads a BAM's
+ * records per region, BamFile.scala:108-148). */
+pl_status pl_readset_upload_many(pl_ctx* ctx, int32_t n_sets, const int32_t* contig_ids,
+                                 const pl_read_soa* const* reads, const int32_t* bam_types, int32_t* ids);
+
 /* --------------------------------------------------- plain region form */
 /* new PileUpRegion(name, start, stop): 1-based inclusive, stop <= len.   */
 pl_status pl_region_open(pl_ctx* ctx, int32_t contig_id, int32_t start,

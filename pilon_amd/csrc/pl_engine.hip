@@ -107,6 +107,8 @@ struct ReadSet {
   int32_t scal[8] = {0};
   int64_t bytes = 0;
   uint32_t nseg = 0;
+  // device arenas shared with the other sets of one pl_readset_upload_many
+  std::shared_ptr<void> shared[3];
   ~ReadSet() { for (void* p : mem) (void)hipFree(p); }
 };
 
@@ -445,6 +447,255 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   c->rsets.push_back(std::move(rs));
   c->rs_version++;
   *out_id = (int32_t)c->rsets.size() - 1;
+  return PL_OK;
+}
+
+// Many read sets in one pass, each as readset_upload would make it: the
+// host arrays of every set packed into one device arena through pinned
+// staging buffers (host threads fill one while the other is copied), the
+// per-set passes (k_prep, the two scans, k_segfill) queued back to back on
+// the lane's stream with one synchronisation between the passes that need
+// a count from the device, and the derived tables in two more arenas.
+pl_status readsets_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_ids, const pl_read_soa* const* sets,
+                               const int32_t* bam_types, int32_t* out_ids) {
+  Lane* L = &c->main;
+  hipStream_t st = L->stream;
+  struct Field { const void* src; size_t bytes, pad; size_t at; };
+  struct Plan {
+    int64_t n, ncig, nb, ocap;
+    std::vector<Field> f;              // pos tlen flag mapq cigar_off cigar seq_off bases2 quals exo_read exo_off nib [lrt]
+    size_t meta, ftail, kp, outl, keep, segc, sego;
+    std::vector<uint8_t> lrt;
+  };
+  std::vector<Plan> P((size_t)n_sets);
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t up = 0, dev = 0, tmp = 0;
+  int64_t max_n = 1;
+  for (int32_t i = 0; i < n_sets; i++) {
+    const pl_read_soa* s = sets[i];
+    if (!s) return fail(PL_E_INVAL, "read set %d is null", i);
+    if (c->contigs.find(contig_ids[i]) == c->contigs.end()) return fail(PL_E_INVAL, "contig %d not uploaded", contig_ids[i]);
+    const int32_t bt = bam_types ? bam_types[i] : PL_BAM_FRAGS;
+    if (bt != PL_BAM_FRAGS && bt != PL_BAM_UNPAIRED && bt != PL_BAM_NANOPORE && bt != PL_BAM_PACBIO)
+      return fail(PL_E_INVAL, "read set %d: bam type %d of a resident read set is not PL_BAM_FRAGS, UNPAIRED, NANOPORE or "
+                  "PACBIO", i, bt);
+    const bool lr = bt == PL_BAM_NANOPORE || bt == PL_BAM_PACBIO;
+    if (lr && !c->pc.longread) return fail(PL_E_INVAL, "a long-read BAM needs config.longread (Pilon.longread)");
+    const int64_t n = s->n_reads;
+    if (n < 0) return fail(PL_E_INVAL, "read set %d: negative read count", i);
+    for (int64_t k = 1; k < n; k++)
+      if (s->pos[k] < s->pos[k - 1])
+        return fail(PL_E_INVAL, "read set %d: reads are not coordinate-sorted at read %lld", i, (long long)k);
+    for (int64_t k = 1; k < s->n_exotic; k++)
+      if (s->exo_read[k] <= s->exo_read[k - 1]) return fail(PL_E_INVAL, "read set %d: exo_read must be ascending", i);
+    Plan& q = P[(size_t)i];
+    q.n = n;
+    q.ncig = s->cigar_off[n];
+    q.nb = s->seq_off[n];
+    q.ocap = 4096 + n / 256;
+    if (lr) q.lrt.assign((size_t)std::max<int64_t>(n, 1), bt == PL_BAM_NANOPORE ? PL_LR_NANOPORE : PL_LR_PACBIO);
+    q.f = {{s->pos, (size_t)(4 * n), 0, 0}, {s->tlen, (size_t)(4 * n), 0, 0}, {s->flag, (size_t)(2 * n), 0, 0},
+           {s->mapq, (size_t)n, 0, 0}, {s->cigar_off, (size_t)(8 * (n + 1)), 0, 0}, {s->cigar, (size_t)(4 * q.ncig), 0, 0},
+           {s->seq_off, (size_t)(8 * (n + 1)), 0, 0}, {s->bases2, (size_t)((q.nb + 3) / 4), 64, 0},
+           {s->quals, (size_t)q.nb, 64, 0}, {s->exo_read, (size_t)(8 * s->n_exotic), 0, 0},
+           {s->exo_off, (size_t)(8 * s->n_exotic), 0, 0}, {s->nib, (size_t)(s->n_exotic ? nib_bytes(s) : 0), 64, 0}};
+    if (lr) q.f.push_back({q.lrt.data(), (size_t)n, 0, 0});
+    for (Field& f : q.f) { f.at = up; up = align(up + f.bytes + f.pad + 16); }
+    q.meta = dev; dev = align(dev + sizeof(RMeta) * (size_t)(n + 1));
+    q.ftail = dev; dev = align(dev + sizeof(FragTail) * (size_t)(n + 1));
+    q.kp = dev; dev = align(dev + 4 * (size_t)(n + 1));
+    q.outl = dev; dev = align(dev + 8 * (size_t)q.ocap);
+    q.keep = tmp; tmp = align(tmp + 4 * (size_t)(n + 1));
+    q.segc = tmp; tmp = align(tmp + 4 * (size_t)(n + 1));
+    q.sego = tmp; tmp = align(tmp + 4 * (size_t)(n + 1));
+    max_n = std::max(max_n, n);
+  }
+  const size_t scal_at = dev;
+  dev = align(dev + 64 * (size_t)std::max(n_sets, 1));
+  HIPCHK(hipSetDevice(c->device));
+  auto dfree = [](void* p) { (void)hipFree(p); };
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(up, 256)));
+  std::shared_ptr<void> up_arena(p, dfree);
+  HIPCHK(hipMalloc(&p, dev));
+  std::shared_ptr<void> dev_arena(p, dfree);
+  HIPCHK(hipMalloc(&p, std::max<size_t>(tmp, 256)));
+  std::unique_ptr<void, decltype(dfree)> tmp_arena(p, dfree);
+  uint8_t* U = (uint8_t*)up_arena.get();
+  uint8_t* D = (uint8_t*)dev_arena.get();
+  uint8_t* X = (uint8_t*)tmp_arena.get();
+  // the host arrays through two pinned staging buffers, packed by host
+  // threads into the arena's layout (padding zeroed)
+  {
+    const size_t chunk = std::min<size_t>(align(up), (size_t)128 << 20);
+    void* sb[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    pl_status e = PL_OK;
+    for (int b = 0; b < 2 && !e; b++) {
+      if (hipHostMalloc(&sb[b], chunk, hipHostMallocDefault) != hipSuccess) e = fail(PL_E_OOM, "pinned staging of %zu bytes", chunk);
+      else if (hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess) e = fail(PL_E_DEVICE, "event");
+    }
+    std::vector<const Field*> all;                          // in arena order
+    for (const Plan& q : P) for (const Field& f : q.f) all.push_back(&f);
+    auto field_end = [&](size_t k) { return k + 1 < all.size() ? all[k + 1]->at : up; };
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    int b = 0;
+    for (size_t a0 = 0; a0 < up && !e; a0 += chunk, b ^= 1) {
+      const size_t a1 = std::min(up, a0 + chunk);
+      if (hipEventSynchronize(ev[b]) != hipSuccess) { e = fail(PL_E_DEVICE, "staging event"); break; }
+      uint8_t* S = (uint8_t*)sb[b];
+      // nt pieces of [a0, a1), each over the fields (with their padding and
+      // alignment gap, zeroed) that meet it, taken by whichever thread is free
+      const int pieces = nt;
+      std::atomic<int> next(0);
+      auto work = [&]() {
+        for (int w; (w = next.fetch_add(1)) < pieces;) {
+          const size_t lo = a0 + (a1 - a0) * (size_t)w / (size_t)pieces;
+          const size_t hi = a0 + (a1 - a0) * (size_t)(w + 1) / (size_t)pieces;
+          size_t k = (size_t)(std::upper_bound(all.begin(), all.end(), lo,
+                                               [](size_t x, const Field* f) { return x < f->at; }) - all.begin());
+          k = k > 0 ? k - 1 : 0;
+          for (; k < all.size() && all[k]->at < hi; k++) {
+            const Field& f = *all[k];
+            const size_t x0 = std::max(lo, f.at), x1 = std::min(hi, field_end(k));
+            if (x0 >= x1) continue;
+            const size_t d1 = std::min(x1, f.at + f.bytes);
+            if (d1 > x0) std::memcpy(S + (x0 - a0), (const uint8_t*)f.src + (x0 - f.at), d1 - x0);
+            const size_t z0 = std::max(x0, f.at + f.bytes);
+            if (x1 > z0) std::memset(S + (z0 - a0), 0, x1 - z0);
+          }
+        }
+      };
+      std::vector<std::thread> th;
+      try {
+        for (int t = 1; t < nt; t++) th.emplace_back(work);
+      } catch (const std::exception&) {
+        // fewer threads: the pieces left are taken by the ones running
+      }
+      work();
+      for (auto& t : th) t.join();
+      if (hipMemcpyAsync(U + a0, S, a1 - a0, hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipEventRecord(ev[b], st) != hipSuccess)
+        e = fail(PL_E_DEVICE, "staged copy");
+    }
+    (void)hipStreamSynchronize(st);
+    for (int k = 0; k < 2; k++) {
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+      if (sb[k]) (void)hipHostFree(sb[k]);
+    }
+    if (e) return e;
+  }
+  // the sets' device views, then k_prep over every set
+  std::vector<std::unique_ptr<ReadSet>> rs((size_t)n_sets);
+  HIPCHK(hipMemsetAsync(D + scal_at, 0, 64 * (size_t)std::max(n_sets, 1), st));
+  for (int32_t i = 0; i < n_sets; i++) {
+    Plan& q = P[(size_t)i];
+    const int32_t bt = bam_types ? bam_types[i] : PL_BAM_FRAGS;
+    rs[(size_t)i].reset(new ReadSet());
+    ReadSet& R = *rs[(size_t)i];
+    R.contig = contig_ids[i];
+    R.n = q.n;
+    R.has_frags = bt == PL_BAM_FRAGS;
+    R.has_nano = bt == PL_BAM_NANOPORE;
+    R.shared[0] = up_arena;
+    R.shared[1] = dev_arena;
+    RSDev& d = R.d;
+    std::memset(&d, 0, sizeof d);
+    d.n = q.n;
+    d.pos = (const int32_t*)(U + q.f[0].at); d.tlen = (const int32_t*)(U + q.f[1].at);
+    d.flag = (const uint16_t*)(U + q.f[2].at); d.mapq = (const uint8_t*)(U + q.f[3].at);
+    d.cigar_off = (const int64_t*)(U + q.f[4].at); d.cigar = (const uint32_t*)(U + q.f[5].at);
+    d.seq_off = (const int64_t*)(U + q.f[6].at); d.bases2 = (const uint8_t*)(U + q.f[7].at);
+    d.quals = (const int8_t*)(U + q.f[8].at); d.exo_read = (const int64_t*)(U + q.f[9].at);
+    d.exo_off = (const int64_t*)(U + q.f[10].at); d.nib = (const uint8_t*)(U + q.f[11].at);
+    if (q.f.size() > 12) d.lrt = (const uint8_t*)(U + q.f[12].at);
+    d.n_exotic = sets[i]->n_exotic;
+    const Contig& ct = c->contigs.at(contig_ids[i]);
+    d.ref = ct.dev;
+    d.ref_len = ct.len;
+    d.meta = (RMeta*)(D + q.meta);
+    d.scal = (int32_t*)(D + scal_at + 64 * (size_t)i);
+    d.ftail = (FragTail*)(D + q.ftail);
+    R.kp = (uint32_t*)(D + q.kp);
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (q.n + 255) / 256), 8192);
+    k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(d, c->cf, (uint32_t*)(X + q.keep), (uint32_t*)(X + q.segc),
+                                                         d.scal, (int64_t*)(D + q.outl), q.ocap);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<int32_t> scal(16 * (size_t)std::max(n_sets, 1));
+  HIPCHK(hipMemcpyAsync(scal.data(), D + scal_at, 64 * (size_t)std::max(n_sets, 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // a set whose outliers overflow their list: a list of the size counted, k_prep again
+  std::vector<int64_t*> outl((size_t)n_sets);
+  for (int32_t i = 0; i < n_sets; i++) {
+    Plan& q = P[(size_t)i];
+    ReadSet& R = *rs[(size_t)i];
+    outl[(size_t)i] = (int64_t*)(D + q.outl);
+    const int32_t cnt = scal[16 * (size_t)i + 3];
+    if (cnt > q.ocap) {
+      HIPCHK(hipMalloc(&p, 8 * (size_t)cnt));
+      R.mem.push_back(p);
+      outl[(size_t)i] = (int64_t*)p;
+      HIPCHK(hipMemsetAsync(R.d.scal, 0, 64, st));
+      const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (q.n + 255) / 256), 8192);
+      k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(R.d, c->cf, (uint32_t*)(X + q.keep), (uint32_t*)(X + q.segc),
+                                                           R.d.scal, (int64_t*)p, cnt);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&scal[16 * (size_t)i], R.d.scal, 64, hipMemcpyDeviceToHost, st));
+    }
+  }
+  // the kept-read and segment scans of every set, and each set's segment count
+  HIPCHK(L->tiles.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, (max_n + SCAN_TILE - 1) / SCAN_TILE)));
+  std::vector<uint32_t> nseg((size_t)std::max(n_sets, 1), 0);
+  for (int32_t i = 0; i < n_sets; i++) {
+    Plan& q = P[(size_t)i];
+    ReadSet& R = *rs[(size_t)i];
+    if (pl_status e = scan_u32(L, (uint32_t*)(X + q.keep), R.kp, q.n)) return e;
+    if (pl_status e = scan_u32(L, (uint32_t*)(X + q.segc), (uint32_t*)(X + q.sego), q.n)) return e;
+    HIPCHK(hipMemcpyAsync(&nseg[(size_t)i], (uint32_t*)(X + q.sego) + q.n, 4, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  // the segment tables
+  size_t segsz = 0;
+  std::vector<size_t> segat((size_t)std::max(n_sets, 1));
+  for (int32_t i = 0; i < n_sets; i++) { segat[(size_t)i] = segsz; segsz = align(segsz + sizeof(Seg) * ((size_t)nseg[(size_t)i] + 1)); }
+  HIPCHK(hipMalloc(&p, std::max<size_t>(segsz, 256)));
+  std::shared_ptr<void> seg_arena(p, dfree);
+  for (int32_t i = 0; i < n_sets; i++) {
+    Plan& q = P[(size_t)i];
+    ReadSet& R = *rs[(size_t)i];
+    R.shared[2] = seg_arena;
+    Seg* sg = (Seg*)((uint8_t*)seg_arena.get() + segat[(size_t)i]);
+    R.d.segs = sg;
+    R.nseg = nseg[(size_t)i];
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (q.n + 255) / 256), 8192);
+    k_segfill<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(R.d, c->cf, (uint32_t*)(X + q.sego), sg);
+    HIPCHK(hipGetLastError());
+  }
+  // the outlier lists in order, and the sets into the context's table
+  for (int32_t i = 0; i < n_sets; i++) {
+    ReadSet& R = *rs[(size_t)i];
+    for (int k = 0; k < 8; k++) R.scal[k] = scal[16 * (size_t)i + k];
+    const int64_t no = R.scal[3];
+    if (no > 0) {
+      std::vector<int64_t> h((size_t)no);
+      HIPCHK(hipMemcpyAsync(h.data(), outl[(size_t)i], 8 * (size_t)no, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      std::sort(h.begin(), h.end());
+      HIPCHK(hipMemcpyAsync(outl[(size_t)i], h.data(), 8 * (size_t)no, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    R.d.outliers = outl[(size_t)i];
+    R.d.n_outlier = no;
+    const int64_t n = R.n, ncig = P[(size_t)i].ncig, nb = P[(size_t)i].nb;
+    R.bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 32 * n + 8 * n + 8 * n;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  for (int32_t i = 0; i < n_sets; i++) {
+    c->rsets.push_back(std::move(rs[(size_t)i]));
+    out_ids[i] = (int32_t)c->rsets.size() - 1;
+  }
+  c->rs_version++;
   return PL_OK;
 }
 
@@ -1092,6 +1343,18 @@ pl_status pl_readset_upload_typed(pl_ctx* c, int32_t contig_id, const pl_read_so
 
 pl_status pl_readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* reads, int32_t* id) {
   return pl_readset_upload_typed(c, contig_id, reads, PL_BAM_FRAGS, id);
+}
+
+pl_status pl_readset_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_ids, const pl_read_soa* const* reads,
+                                 const int32_t* bam_types, int32_t* ids) {
+  if (!c || n_sets < 0 || (n_sets > 0 && (!contig_ids || !reads || !ids))) return fail(PL_E_INVAL, "null argument");
+  if (c->serving) return fail(PL_E_BUSY, "a server is open on this context: its read sets are fixed");
+  if (n_sets == 0) return PL_OK;
+  try {
+    return readsets_upload_many(c, n_sets, contig_ids, reads, bam_types, ids);
+  } catch (const std::bad_alloc&) {
+    return fail(PL_E_OOM, "host allocation failed");
+  }
 }
 
 pl_status pl_region_open(pl_ctx* c, int32_t contig_id, int32_t start, int32_t stop, pl_region** out) {

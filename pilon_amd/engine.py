@@ -147,6 +147,8 @@ def lib():
             "pl_ref_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_char_p, C.c_int64]),
             "pl_readset_upload": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa), P(C.c_int32)]),
             "pl_readset_upload_typed": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa), C.c_int32, P(C.c_int32)]),
+            "pl_readset_upload_many": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]),
             "pl_region_open": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_void_p)]),
             "pl_region_add_reads": (C.c_int, [C.c_void_p, P(PlReadSoa)]),
             "pl_region_add_reads_typed": (C.c_int, [C.c_void_p, P(PlReadSoa), C.c_int32]),
@@ -244,6 +246,21 @@ class Context:
         rid = C.c_int32()
         _check(lib().pl_readset_upload_typed(self.h, contig_id, C.byref(soa), BAM_TYPES[bam_type], C.byref(rid)))
         return rid.value
+
+    def readset_upload_many(self, items):
+        """items: (contig_id, reads, bam_type) tuples; every set uploaded in
+        one pass (pl_readset_upload_many).  Returns their read-set ids."""
+        items = list(items)
+        n = len(items)
+        if n == 0:
+            return []
+        soas = [r.soa() if hasattr(r, "soa") else r for _, r, _ in items]
+        arr = (C.POINTER(PlReadSoa) * n)(*[C.pointer(x) for x in soas])
+        cids = (C.c_int32 * n)(*[int(c) for c, _, _ in items])
+        types = (C.c_int32 * n)(*[BAM_TYPES[t] for _, _, t in items])
+        ids = (C.c_int32 * n)()
+        _check(lib().pl_readset_upload_many(self.h, n, cids, arr, types, ids))
+        return list(ids)
 
     def batch_descs(self, regions):
         n = len(regions)
