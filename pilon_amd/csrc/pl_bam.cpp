@@ -25,7 +25,10 @@
 
 #include <libdeflate.h>
 
+#include <fcntl.h>
 #include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -70,7 +73,6 @@ const uint8_t kEofBlock[28] = {0x1f, 0x8b, 0x08, 0x04, 0x00, 0x00, 0x00, 0x00, 0
                                0x02, 0x00, 0x1b, 0x00, 0x03, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00};
 
 constexpr size_t kSliceBytes = (size_t)64 << 20;   // inflated bytes per slice (PILON_BAM_SLICE_KIB overrides)
-constexpr size_t kReadBytes = (size_t)16 << 20;    // compressed bytes per file read
 
 size_t slice_bytes() {
   const char* e = std::getenv("PILON_BAM_SLICE_KIB");
@@ -296,9 +298,8 @@ int64_t parse_header(const uint8_t* u, size_t n, pl_bam* B) {
   return (int64_t)p;
 }
 
-// Inflate the slice's blocks into ub (after `carry` bytes already there).
-pl_status inflate_blocks(const std::vector<uint8_t>& cb, std::vector<Block>& blocks, std::vector<uint8_t>& ub,
-                         size_t carry, int nth) {
+// Inflate blocks of the compressed bytes cb into out (block b at out + b.uoff).
+pl_status inflate_blocks(const uint8_t* cb, std::vector<Block>& blocks, uint8_t* out, int nth) {
   std::atomic<int64_t> bad{-1};
   parallel_for(nth, (int64_t)blocks.size(), [&](int, int64_t lo, int64_t hi) {
     libdeflate_decompressor* dz = libdeflate_alloc_decompressor();
@@ -306,9 +307,9 @@ pl_status inflate_blocks(const std::vector<uint8_t>& cb, std::vector<Block>& blo
     for (int64_t k = lo; k < hi; k++) {
       const Block& b = blocks[(size_t)k];
       uint8_t none = 0;
-      uint8_t* dst = b.isize ? ub.data() + carry + b.uoff : &none;   // an empty block (the EOF marker) still inflates
+      uint8_t* dst = b.isize ? out + b.uoff : &none;   // an empty block (the EOF marker) still inflates
       size_t got = 0;
-      const libdeflate_result rc = libdeflate_deflate_decompress(dz, cb.data() + b.cdata, b.clen, dst,
+      const libdeflate_result rc = libdeflate_deflate_decompress(dz, cb + b.cdata, b.clen, dst,
                                                                  b.isize ? b.isize : 1, &got);
       if (rc != LIBDEFLATE_SUCCESS || got != b.isize || libdeflate_crc32(0, dst, b.isize) != b.crc) {
         int64_t e = -1;
@@ -413,8 +414,7 @@ pl_status delimit_records(pl_bam* B, const uint8_t* u, size_t u0, size_t n, int 
 
 // Delimit the records of ub[u0, n) and decode every placed one into its
 // reference's batch; returns the offset of the first incomplete record.
-pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, size_t n, int nth, size_t* used) {
-  const uint8_t* u = ub.data();
+pl_status decode_records(pl_bam* B, const uint8_t* u, size_t u0, size_t n, int nth, size_t* used) {
   const int64_t nref = (int64_t)B->refs.size();
   std::vector<size_t> offs;
   if (pl_status s = delimit_records(B, u, u0, n, nth, offs, used)) return s;
@@ -630,35 +630,44 @@ pl_status decode_records(pl_bam* B, const std::vector<uint8_t>& ub, size_t u0, s
   return PL_OK;
 }
 
+// The file mapped read-only (no copy through a read buffer).
+struct Mapped {
+  const uint8_t* p = nullptr;
+  size_t n = 0;
+  ~Mapped() { if (p) munmap((void*)p, n); }
+};
+
 pl_status open_impl(const char* path, int nth, pl_bam* B) {
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return fail(PL_E_INVAL, "cannot open %s", path);
-  std::unique_ptr<FILE, int (*)(FILE*)> fh(f, std::fclose);
-  std::vector<uint8_t> cb;       // compressed bytes not yet inflated
-  std::vector<uint8_t> ub;       // inflated bytes not yet decoded
-  std::vector<uint8_t> last_block;
-  size_t carry = 0;
-  const size_t slice = slice_bytes();
-  bool header = false, file_eof = false;
-  for (;;) {
-    // top up the compressed buffer
-    if (!file_eof && cb.size() < kReadBytes) {
-      const size_t old = cb.size();
-      cb.resize(old + kReadBytes);
-      const size_t got = std::fread(cb.data() + old, 1, kReadBytes, f);
-      cb.resize(old + got);
-      B->st.compressed_bytes += (int64_t)got;
-      if (got == 0) file_eof = true;
+  const int fd = ::open(path, O_RDONLY);
+  if (fd < 0) return fail(PL_E_INVAL, "cannot open %s", path);
+  struct stat sb;
+  Mapped m;
+  if (fstat(fd, &sb) == 0 && sb.st_size > 0) {
+    void* q = mmap(nullptr, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+    if (q != MAP_FAILED) {
+      m.p = (const uint8_t*)q;
+      m.n = (size_t)sb.st_size;
+      (void)madvise(q, m.n, MADV_SEQUENTIAL);
     }
-    // delimit whole blocks, up to one slice of inflated bytes
+  }
+  ::close(fd);
+  if (sb.st_size > 0 && !m.p) return fail(PL_E_INVAL, "cannot map %s", path);
+  const uint8_t* cb = m.p;       // the compressed file
+  B->st.compressed_bytes = (int64_t)m.n;
+  Buf<uint8_t> ub;               // inflated bytes not yet decoded
+  size_t carry = 0, p = 0, last_at = SIZE_MAX, last_end = 0;
+  const size_t slice = slice_bytes();
+  bool header = false;
+  while (p < m.n) {
+    // whole blocks from p, up to one slice of inflated bytes
     std::vector<Block> blocks;
-    size_t p = 0, utot = 0, last_at = 0;
-    while (p + 18 <= cb.size() && utot < slice) {
-      const uint8_t* h = cb.data() + p;
+    size_t q = p, utot = 0;
+    while (q + 18 <= m.n && utot < slice) {
+      const uint8_t* h = cb + q;
       if (h[0] != 0x1f || h[1] != 0x8b || h[2] != 8 || !(h[3] & 4))
-        return fail(PL_E_INVAL, "not a BGZF block at byte %lld", (long long)(B->st.compressed_bytes - cb.size() + p));
+        return fail(PL_E_INVAL, "not a BGZF block at byte %lld", (long long)q);
       const size_t xlen = rd16(h + 10);
-      if (p + 12 + xlen > cb.size()) break;
+      if (q + 12 + xlen > m.n) break;
       size_t bsize = 0;
       for (size_t x = 12; x + 4 <= 12 + xlen;) {
         const size_t slen = rd16(h + x + 2);
@@ -666,35 +675,29 @@ pl_status open_impl(const char* path, int nth, pl_bam* B) {
         x += 4 + slen;
       }
       if (!bsize || bsize < 20 + xlen) return fail(PL_E_INVAL, "BGZF block without a valid BC (BSIZE) subfield");
-      if (p + bsize > cb.size()) break;
-      Block b;
-      b.cdata = p + 12 + xlen;
-      b.clen = bsize - xlen - 20;
-      b.crc = rd32(h + bsize - 8);
-      b.isize = rd32(h + bsize - 4);
-      b.uoff = utot;
-      if (b.isize > 65536) return fail(PL_E_INVAL, "BGZF block ISIZE %u above 64 KiB", b.isize);
-      utot += b.isize;
-      blocks.push_back(b);
-      last_at = p;
-      p += bsize;
+      if (q + bsize > m.n) break;
+      Block bk;
+      bk.cdata = q + 12 + xlen;
+      bk.clen = bsize - xlen - 20;
+      bk.crc = rd32(h + bsize - 8);
+      bk.isize = rd32(h + bsize - 4);
+      bk.uoff = utot;
+      if (bk.isize > 65536) return fail(PL_E_INVAL, "BGZF block ISIZE %u above 64 KiB", bk.isize);
+      utot += bk.isize;
+      blocks.push_back(bk);
+      last_at = q;
+      last_end = q + bsize;
+      q += bsize;
     }
-    if (!blocks.empty()) last_block.assign(cb.data() + last_at, cb.data() + p);
-    if (blocks.empty()) {
-      if (file_eof) {
-        if (!cb.empty()) return fail(PL_E_INVAL, "truncated BGZF block at the end of the file");
-        break;
-      }
-      continue;
-    }
+    if (blocks.empty()) return fail(PL_E_INVAL, "truncated BGZF block at the end of the file");
     B->st.blocks += (int64_t)blocks.size();
     ub.resize(carry + utot);
     double t0 = now_s();
-    pl_status s = inflate_blocks(cb, blocks, ub, carry, nth);
+    pl_status s = inflate_blocks(cb, blocks, ub.data() + carry, nth);
     B->st.inflate_s += now_s() - t0;
     if (s) return s;
     B->st.bam_bytes += (int64_t)utot;
-    cb.erase(cb.begin(), cb.begin() + (std::ptrdiff_t)p);
+    p = q;
     size_t un = carry + utot, u0 = 0;
     if (!header) {
       const int64_t h = parse_header(ub.data(), un, B);
@@ -705,7 +708,7 @@ pl_status open_impl(const char* path, int nth, pl_bam* B) {
     }
     t0 = now_s();
     size_t used = u0;
-    s = decode_records(B, ub, u0, un, nth, &used);
+    s = decode_records(B, ub.data(), u0, un, nth, &used);
     B->st.decode_s += now_s() - t0;
     if (s) return s;
     carry = un - used;
@@ -713,8 +716,8 @@ pl_status open_impl(const char* path, int nth, pl_bam* B) {
   }
   if (!header) return fail(PL_E_INVAL, "not a BAM file (no complete header)");
   if (carry) return fail(PL_E_INVAL, "truncated BAM record at the end of the file");
-  B->st.eof_marker = last_block.size() == sizeof kEofBlock &&
-                     std::memcmp(last_block.data(), kEofBlock, sizeof kEofBlock) == 0;
+  B->st.eof_marker = last_at != SIZE_MAX && last_end - last_at == sizeof kEofBlock &&
+                     std::memcmp(cb + last_at, kEofBlock, sizeof kEofBlock) == 0;
   return PL_OK;
 }
 
@@ -849,7 +852,7 @@ pl_status open_indexed_impl(const char* path, const char* index, int nth, pl_bam
     if (blocks.empty()) return fail(PL_E_INVAL, "not a BAM file (no complete header)");
     const size_t carry = ub.size();
     ub.resize(carry + blocks[0].isize);
-    if (pl_status s = inflate_blocks(cb, blocks, ub, carry, 1)) return s;
+    if (pl_status s = inflate_blocks(cb.data(), blocks, ub.data() + carry, 1)) return s;
     at = coff[0] + cb.size();
     const int64_t h = parse_header(ub.data(), ub.size(), B);
     if (h < 0) return fail(PL_E_INVAL, "not a BAM file (bad magic or header)");
@@ -918,14 +921,14 @@ pl_status query_impl(pl_bam* B, int32_t ref, int64_t start, int64_t stop) {
     size_t utot = 0;
     for (const Block& b : blocks) utot += b.isize;
     ub.assign(utot, 0);
-    if (pl_status s = inflate_blocks(cb, blocks, ub, 0, B->nth)) return s;
+    if (pl_status s = inflate_blocks(cb.data(), blocks, ub.data(), B->nth)) return s;
     const size_t u0 = (size_t)(c.first & 0xffff);
     size_t u1 = utot;
     for (size_t k = 0; k < blocks.size(); k++)
       if (coff[k] == c1) { u1 = blocks[k].uoff + (size_t)(c.second & 0xffff); break; }
     if (u0 > u1 || u1 > utot) return fail(PL_E_INVAL, "index chunk outside the BAM");
     size_t used = u0;
-    if (pl_status s = decode_records(&Q, ub, u0, u1, B->nth, &used)) return s;
+    if (pl_status s = decode_records(&Q, ub.data(), u0, u1, B->nth, &used)) return s;
     if (used != u1) return fail(PL_E_INVAL, "index chunk ends inside a record");
   }
   // keep the reads of ref overlapping the interval
