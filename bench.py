@@ -68,10 +68,36 @@ def measured_traffic(path, workload, window, block):
     return int(tb.group(1)), os.path.relpath(path, ROOT)
 
 
-def dev_sync():
+def measured_copy_bandwidth():
+    """The device's copy bandwidth, measured in a child process that loads
+    only torch (a process that has loaded the engine's HIP runtime does not
+    also get torch's): GB/s read plus written, or None."""
+    import subprocess
+    try:
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--measure-copy"], capture_output=True,
+                             text=True, timeout=300)
+        return float(out.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError):
+        return None
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """A device-to-device copy of nbytes, timed with events over reps copies
+    after one untimed; bytes read plus bytes written per second, in GB/s."""
     import torch
-    if torch.cuda.is_available():
-        torch.cuda.synchronize()
+    a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms / 1000.0) / 1e9
 
 
 def regions_for(name, data, rids):
@@ -220,8 +246,12 @@ def main():
                     help="also run the BAM producer: the workload written as a BAM, decoded, uploaded and piled")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB on the device three times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--measure-copy", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "k_window_c3_traffic.md"))
     args = ap.parse_args()
+    if args.measure_copy:
+        print(copy_bandwidth(), flush=True)
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
@@ -279,13 +309,13 @@ def main():
         step(args.graph)
     pipe = []
     # each step returns after the library's stream has drained; the device
-    # syncs around the loop make the bracket explicit
-    dev_sync()
+    # synchronisations around the loop make the bracket explicit
+    ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(args.graph)
         pipe.append(ctx.batch_info()["pipeline_ms"])
-    dev_sync()
+    ctx.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = aligned / (ms_per_step / 1000.0)
@@ -403,9 +433,15 @@ def main():
     producer = None
     if args.bam:
         producer = bam_producer(data, ctx, args, W, B, aligned)
-        log(f"bench: BAM producer: {producer['records']} records decoded in {producer['decode_s']:.2f}s on "
-            f"{producer['threads']} threads ({producer['decode_bases_per_s'] / 1e9:.2f} G aligned bases/s), "
-            f"file to pileup {producer['end_to_end_bases_per_s'] / 1e9:.2f} G bases/s")
+        log(f"bench: BAM producer: {producer['records']} records; device loader {producer['device_loader_s']:.2f}s "
+            f"(file to pileup {producer['end_to_end_bases_per_s'] / 1e9:.2f} G bases/s); host decode "
+            f"{producer['host_decode_s']:.2f}s + upload {producer['host_upload_s']:.2f}s "
+            f"({producer['host_end_to_end_bases_per_s'] / 1e9:.2f} G bases/s)")
+
+    copy_gbs = measured_copy_bandwidth()
+    if copy_gbs:
+        log(f"bench: device copy of 2 GiB: {copy_gbs:.0f} GB/s read + written "
+            f"(k_window at {achieved / copy_gbs:.1%} of it)")
 
     serving = None
     if args.serve:
@@ -423,7 +459,9 @@ def main():
                    "persistent": args.persistent, "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_window", "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms},
+                     "kernel": "k_window", "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms,
+                     "measured_copy_GB_per_s": copy_gbs,
+                     "frac_of_measured_copy": achieved / copy_gbs if copy_gbs else None},
         "device_pipeline_ms": pipe_ms,
         "cpu_baseline": cpu,
         "p99_latency_ms": serving["p99_ms"] if serving else None,
@@ -440,11 +478,13 @@ def main():
 
 
 def bam_producer(data, ctx, args, W, B, aligned):
-    """The BAM -> read batch producer on this workload: its reads written as a
+    """The BAM -> read set producer on this workload: its reads written as a
     coordinate-sorted BAM (BGZF, deflate level 1, one reference per contig),
-    decoded on the host's threads (libpilonbam), uploaded as read sets in
-    one pass (pl_readset_upload_many) and piled in one batched pass.  End to end = decode + upload + pile, from a
-    file in the page cache."""
+    then read two ways and piled in one batched pass each: the device
+    loader (inflate on the host's threads into pinned staging, records
+    decoded on the device: pl_loader_*), and the host decoder (libpilonbam)
+    with one upload pass (pl_readset_upload_many).  End to end = load +
+    pile, from a file in the page cache."""
     import shutil
     import tempfile
     from pilon_amd.bam import BamFile
@@ -464,8 +504,7 @@ def bam_producer(data, ctx, args, W, B, aligned):
         t_dec = time.perf_counter() - t0
         st = bam.stats
         t0 = time.perf_counter()
-        rids = ctx.readset_upload_many([(ci, bam.reads(ci), "frags") for ci in range(data.n)])
-        dev_sync()
+        rids = ctx.readset_upload_many([(ci, bam.reads(ci), "frags") for ci in range(data.n)])   # returns synchronised
         t_up = time.perf_counter() - t0
         regions = regions_for(args.config, data, rids)
         t0 = time.perf_counter()
@@ -474,17 +513,36 @@ def bam_producer(data, ctx, args, W, B, aligned):
         got = sum(x["aligned_bases"] for x in stats)
         assert got == aligned, ("aligned bases through the BAM differ", got, aligned)
         bam.close()
+        del bam
+        # the same file through the device loader: host inflate into pinned
+        # staging, record decode on the device
+        t0 = time.perf_counter()
+        dids = ctx.readsets_from_bam(path, list(range(data.n)), threads=nth)                   # returns synchronised
+        t_load = time.perf_counter() - t0
+        lst = ctx.last_bam_stats
+        dregions = regions_for(args.config, data, dids)
+        t0 = time.perf_counter()
+        _, _, dstats = ctx.batch_run(dregions, window=W, block=B, persistent=1, copy_out=False, want_stats=True)
+        t_dpile = time.perf_counter() - t0
+        got = sum(x["aligned_bases"] for x in dstats)
+        assert got == aligned, ("aligned bases through the device loader differ", got, aligned)
+        assert dstats == stats, "the device loader's regions differ from the host decoder's"
     finally:
         shutil.rmtree(d, ignore_errors=True)
-    e2e = t_dec + t_up + t_pile
+    e2e_host = t_dec + t_up + t_pile
+    e2e = t_load + t_dpile
     return {"file_bytes": fsize, "bam_bytes": stream, "records": st["records"], "threads": nth,
-            "write_s": t_write, "decode_s": t_dec, "inflate_s": st["inflate_s"], "parse_s": st["decode_s"],
-            "upload_s": t_up, "pile_s": t_pile,
-            "decode_records_per_s": st["records"] / t_dec, "decode_bases_per_s": aligned / t_dec,
-            "decode_compressed_MB_per_s": fsize / t_dec / 1e6,
-            "end_to_end_bases_per_s": aligned / e2e,
-            "note": "aligned bases per second of the whole path from a BAM in the page cache: host decode, "
-                    "upload, one batched pileup (not a graph replay)"}
+            "write_s": t_write,
+            "device_loader_s": t_load, "loader_inflate_s": lst["inflate_s"], "loader_delimit_s": lst["decode_s"],
+            "loader_phases_s": ctx.last_load_s,
+            "pile_s": t_dpile, "end_to_end_bases_per_s": aligned / e2e,
+            "host_decode_s": t_dec, "host_inflate_s": st["inflate_s"], "host_parse_s": st["decode_s"],
+            "host_upload_s": t_up, "host_pile_s": t_pile, "host_end_to_end_bases_per_s": aligned / e2e_host,
+            "host_decode_records_per_s": st["records"] / t_dec,
+            "note": "aligned bases per second of the whole path from a BAM in the page cache: the device loader "
+                    "(host inflate into pinned staging, records decoded into read sets on the device) and one "
+                    "batched pileup (not a graph replay); host_* is the same file decoded on the host "
+                    "(libpilonbam) and uploaded in one pass"}
 
 
 def serve_workload(cfg, args):

@@ -91,6 +91,24 @@ int64_t     pl_bam_ref_len(const pl_bam* bam, int32_t ref);
  * PL_RF_EXOTIC added; non-ACGT reads keep their nibbles at an even offset. */
 pl_status pl_bam_ref_reads(const pl_bam* bam, int32_t ref, pl_read_soa* out);   /* pl_bam_open only */
 
+/* A BAM read as a stream of record bytes, for a consumer that decodes the
+ * records itself (the device loader, pl_loader_* in pilon_hip.h): the
+ * header is parsed at open (pl_bam_n_refs, pl_bam_ref_name, pl_bam_ref_len
+ * answer as after pl_bam_open); each pl_bam_stream_next inflates the next
+ * BGZF blocks into dst (cap bytes, at least 64 KiB more than the longest
+ * record), after the bytes of the record the previous call cut, and hands
+ * over the whole records there: *len bytes holding *n_rec records whose
+ * starts (their block_size fields) go to rec_off (rec_cap entries; cap / 36
+ * + 1 always suffice).  *len == 0 at the end of the file.  Records are
+ * delimited as pl_bam_open does; their fields are not checked here. */
+pl_status pl_bam_stream_open(const char* path, int32_t n_threads, pl_bam** out);
+pl_status pl_bam_stream_next(pl_bam* bam, uint8_t* dst, int64_t cap, int64_t* len, int64_t* rec_off,
+                             int64_t rec_cap, int64_t* n_rec);
+/* A stream's inflated size (header included): the ISIZE fields of all its
+ * BGZF blocks, which bounds the record bytes it hands over; -1 if not a
+ * stream. */
+int64_t pl_bam_stream_bytes(const pl_bam* bam);
+
 void pl_bam_get_stats(const pl_bam* bam, pl_bam_stats* out);
 void pl_bam_close(pl_bam* bam);
 

@@ -185,8 +185,37 @@ pl_status pl_readset_upload_typed(pl_ctx* ctx, int32_t contig_id, const pl_read_
  * the cost is that of the bytes rather than of the sets.  No counterpart in
  * the reference: the resident form's loader (BamFile.process reads a BAM's
  * records per region, BamFile.scala:108-148). */
+/* Waits until the context's device has finished all work queued on it
+ * (hipDeviceSynchronize on its device).  Every pl_* call that returns
+ * results returns synchronised; this is for callers that time or bracket
+ * work.  No counterpart in the reference. */
+pl_status pl_ctx_synchronize(pl_ctx* ctx);
+
 pl_status pl_readset_upload_many(pl_ctx* ctx, int32_t n_sets, const int32_t* contig_ids,
                                  const pl_read_soa* const* reads, const int32_t* bam_types, int32_t* ids);
+
+/* Resident read sets built on the device from a BAM's record bytes: the
+ * host inflates (libpilonbam's pl_bam_stream_next) into the loader's pinned
+ * staging buffers, the slices go to the device as they come, and
+ * pl_loader_finish decodes every record there into the per-reference
+ * arrays pl_bam_open would produce on the host, then prepares the sets as
+ * pl_readset_upload_many does.  record_bytes bounds the record bytes of all
+ * slices (the stream's inflated size); slice_bytes is the staging size.
+ * The BAM must be sorted as samtools sorts (references in header order,
+ * records of no reference last: PL_E_INVAL otherwise).  contig_ids[r] is
+ * the context contig of BAM reference r (-1: skipped); ids[r] its set id
+ * (-1 when skipped).  No counterpart in the reference: the resident form's
+ * loader (BamFile.process reads a BAM's records per region,
+ * BamFile.scala:108-148).  Use:
+ *   pl_loader_open; loop { pl_loader_staging; pl_bam_stream_next into it;
+ *   pl_loader_push } until the stream ends; pl_loader_finish;
+ *   pl_loader_close. */
+typedef struct pl_loader pl_loader;
+pl_status pl_loader_open(pl_ctx* ctx, int64_t record_bytes, int64_t slice_bytes, pl_loader** out);
+pl_status pl_loader_staging(pl_loader* ld, uint8_t** bytes, int64_t* cap, int64_t** rec_off, int64_t* rec_cap);
+pl_status pl_loader_push(pl_loader* ld, int64_t len, int64_t n_rec);
+pl_status pl_loader_finish(pl_loader* ld, int32_t n_refs, const int32_t* contig_ids, int32_t bam_type, int32_t* ids);
+void pl_loader_close(pl_loader* ld);
 
 /* --------------------------------------------------- plain region form */
 /* new PileUpRegion(name, start, stop): 1-based inclusive, stop <= len.   */

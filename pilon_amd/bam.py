@@ -51,6 +51,10 @@ def lib():
             "pl_bam_ref_len": (C.c_int64, [C.c_void_p, C.c_int32]),
             "pl_bam_ref_reads": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa)]),
             "pl_bam_get_stats": (None, [C.c_void_p, P(PlBamStats)]),
+            "pl_bam_stream_open": (C.c_int, [C.c_char_p, C.c_int32, P(C.c_void_p)]),
+            "pl_bam_stream_bytes": (C.c_int64, [C.c_void_p]),
+            "pl_bam_stream_next": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, P(C.c_int64), C.c_void_p, C.c_int64,
+                                             P(C.c_int64)]),
             "pl_bam_close": (None, [C.c_void_p]),
             "pl_bam_last_error": (C.c_char_p, []),
         }
@@ -170,3 +174,46 @@ def _batch_of(s):
                          arr(s.bases2, (nb + 3) // 4, C.c_uint8, np.uint8),
                          arr(s.quals, nb, C.c_int8, np.int8), ex_r, ex_o,
                          arr(s.nib, nib_len, C.c_uint8, np.uint8))
+
+
+class BamStream(BamFile):
+    """A BAM read as slices of whole records (pl_bam_stream_open / _next):
+    the header at open, then the record bytes, for a consumer that decodes
+    them itself."""
+
+    def __init__(self, path, threads=0):
+        h = C.c_void_p()
+        st = lib().pl_bam_stream_open(os.fsencode(path), threads, C.byref(h))
+        if st:
+            raise BamError(st, lib().pl_bam_last_error().decode())
+        self.h = h
+        n = lib().pl_bam_n_refs(h)
+        self.names = [lib().pl_bam_ref_name(h, r).decode() for r in range(n)]
+        self.lens = [lib().pl_bam_ref_len(h, r) for r in range(n)]
+
+    @property
+    def total_bytes(self):
+        """The stream's inflated size, which bounds the record bytes it hands over."""
+        return lib().pl_bam_stream_bytes(self.h)
+
+    def next_into(self, dst, cap, rec_off, rec_cap):
+        """Inflate the next slice into the buffer at address dst (cap bytes)
+        and its record starts into rec_off (rec_cap int64s): (bytes, records),
+        (0, 0) at the end."""
+        ln, nr = C.c_int64(), C.c_int64()
+        st = lib().pl_bam_stream_next(self.h, dst, cap, C.byref(ln), rec_off, rec_cap, C.byref(nr))
+        if st:
+            raise BamError(st, lib().pl_bam_last_error().decode())
+        return ln.value, nr.value
+
+    def slices(self, cap=64 << 20):
+        """The slices as (numpy bytes, numpy record starts) pairs, into host
+        buffers of this object's own."""
+        import numpy as np
+        buf = np.empty(cap, np.uint8)
+        offs = np.empty(cap // 36 + 1, np.int64)
+        while True:
+            n, r = self.next_into(buf.ctypes.data, cap, offs.ctypes.data, len(offs))
+            if n == 0:
+                return
+            yield buf[:n], offs[:r]

@@ -254,6 +254,18 @@ struct BaiRef {
   std::vector<uint64_t> lin;
 };
 
+namespace {
+// A file mapped read-only (no copy through a read buffer).
+struct Mapped {
+  const uint8_t* p = nullptr;
+  size_t n = 0;
+  Mapped() = default;
+  Mapped(const Mapped&) = delete;
+  Mapped& operator=(const Mapped&) = delete;
+  ~Mapped() { if (p) munmap((void*)p, n); }
+};
+}  // namespace
+
 struct pl_bam {
   std::vector<std::string> names;
   std::vector<int64_t> lens;
@@ -265,6 +277,13 @@ struct pl_bam {
   int nth = 1;
   std::vector<BaiRef> bai;
   RefReads query;            // the last pl_bam_query's reads
+  // opened as a stream (pl_bam_stream_open): the mapped file, the next
+  // block, and the inflated bytes of the record the last slice cut
+  bool streamed = false;
+  Mapped map;
+  size_t next_block = 0, last_at = SIZE_MAX, last_end = 0;
+  Buf<uint8_t> carry;
+  int64_t stream_bytes = 0;  // inflated bytes of every block (the ISIZE fields)
 };
 
 namespace {
@@ -630,18 +649,11 @@ pl_status decode_records(pl_bam* B, const uint8_t* u, size_t u0, size_t n, int n
   return PL_OK;
 }
 
-// The file mapped read-only (no copy through a read buffer).
-struct Mapped {
-  const uint8_t* p = nullptr;
-  size_t n = 0;
-  ~Mapped() { if (p) munmap((void*)p, n); }
-};
-
-pl_status open_impl(const char* path, int nth, pl_bam* B) {
+pl_status map_file(const char* path, Mapped& m) {
   const int fd = ::open(path, O_RDONLY);
   if (fd < 0) return fail(PL_E_INVAL, "cannot open %s", path);
   struct stat sb;
-  Mapped m;
+  std::memset(&sb, 0, sizeof sb);
   if (fstat(fd, &sb) == 0 && sb.st_size > 0) {
     void* q = mmap(nullptr, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
     if (q != MAP_FAILED) {
@@ -652,6 +664,51 @@ pl_status open_impl(const char* path, int nth, pl_bam* B) {
   }
   ::close(fd);
   if (sb.st_size > 0 && !m.p) return fail(PL_E_INVAL, "cannot map %s", path);
+  return PL_OK;
+}
+
+// The whole BGZF blocks of the mapped file from byte q whose inflated bytes
+// fit `room` (at least one block when any is there); *end = the byte after
+// the last one taken.
+pl_status next_blocks(const Mapped& m, size_t q, size_t room, std::vector<Block>& blocks, size_t* end, size_t* last_at) {
+  const uint8_t* cb = m.p;
+  size_t utot = 0;
+  blocks.clear();
+  while (q + 18 <= m.n) {
+    const uint8_t* h = cb + q;
+    if (h[0] != 0x1f || h[1] != 0x8b || h[2] != 8 || !(h[3] & 4))
+      return fail(PL_E_INVAL, "not a BGZF block at byte %lld", (long long)q);
+    const size_t xlen = rd16(h + 10);
+    if (q + 12 + xlen > m.n) break;
+    size_t bsize = 0;
+    for (size_t x = 12; x + 4 <= 12 + xlen;) {
+      const size_t slen = rd16(h + x + 2);
+      if (h[x] == 66 && h[x + 1] == 67 && slen == 2) bsize = (size_t)rd16(h + x + 4) + 1;
+      x += 4 + slen;
+    }
+    if (!bsize || bsize < 20 + xlen) return fail(PL_E_INVAL, "BGZF block without a valid BC (BSIZE) subfield");
+    if (q + bsize > m.n) break;
+    Block bk;
+    bk.cdata = q + 12 + xlen;
+    bk.clen = bsize - xlen - 20;
+    bk.crc = rd32(h + bsize - 8);
+    bk.isize = rd32(h + bsize - 4);
+    if (bk.isize > 65536) return fail(PL_E_INVAL, "BGZF block ISIZE %u above 64 KiB", bk.isize);
+    if (!blocks.empty() && utot + bk.isize > room) break;
+    bk.uoff = utot;
+    utot += bk.isize;
+    blocks.push_back(bk);
+    *last_at = q;
+    q += bsize;
+  }
+  if (blocks.empty() && q < m.n) return fail(PL_E_INVAL, "truncated BGZF block at the end of the file");
+  *end = q;
+  return PL_OK;
+}
+
+pl_status open_impl(const char* path, int nth, pl_bam* B) {
+  Mapped m;
+  if (pl_status s = map_file(path, m)) return s;
   const uint8_t* cb = m.p;       // the compressed file
   B->st.compressed_bytes = (int64_t)m.n;
   Buf<uint8_t> ub;               // inflated bytes not yet decoded
@@ -718,6 +775,52 @@ pl_status open_impl(const char* path, int nth, pl_bam* B) {
   if (carry) return fail(PL_E_INVAL, "truncated BAM record at the end of the file");
   B->st.eof_marker = last_at != SIZE_MAX && last_end - last_at == sizeof kEofBlock &&
                      std::memcmp(cb + last_at, kEofBlock, sizeof kEofBlock) == 0;
+  return PL_OK;
+}
+
+// pl_bam_stream_next's work: the carried bytes, then whole blocks that fit
+// the caller's buffer, inflated there; the records delimited; the cut
+// record carried to the next call.
+pl_status stream_next_impl(pl_bam* B, uint8_t* dst, int64_t cap, int64_t* len, int64_t* rec_off, int64_t rec_cap,
+                           int64_t* n_rec) {
+  *len = 0;
+  *n_rec = 0;
+  const size_t cc = B->carry.size();
+  if ((size_t)cap < cc + 65536) return fail(PL_E_INVAL, "slice buffer of %lld bytes below one record and one block", (long long)cap);
+  if (cc) std::memcpy(dst, B->carry.data(), cc);
+  std::vector<Block> blocks;
+  size_t end = B->next_block;
+  if (pl_status s = next_blocks(B->map, B->next_block, (size_t)cap - cc, blocks, &end, &B->last_at)) return s;
+  size_t utot = 0;
+  for (const Block& b : blocks) utot += b.isize;
+  if (blocks.empty()) {
+    if (cc) return fail(PL_E_INVAL, "truncated BAM record at the end of the file");
+    B->st.eof_marker = B->last_at != SIZE_MAX && B->last_end - B->last_at == sizeof kEofBlock &&
+                       std::memcmp(B->map.p + B->last_at, kEofBlock, sizeof kEofBlock) == 0;
+    return PL_OK;
+  }
+  B->last_end = end;
+  B->st.blocks += (int64_t)blocks.size();
+  double t0 = now_s();
+  if (pl_status s = inflate_blocks(B->map.p, blocks, dst + cc, B->nth)) return s;
+  B->st.inflate_s += now_s() - t0;
+  B->st.bam_bytes += (int64_t)utot;
+  B->next_block = end;
+  const size_t un = cc + utot;
+  t0 = now_s();
+  std::vector<size_t> offs;
+  size_t used = 0;
+  if (pl_status s = delimit_records(B, dst, 0, un, B->nth, offs, &used)) return s;
+  B->st.decode_s += now_s() - t0;
+  if ((int64_t)offs.size() > rec_cap)
+    return fail(PL_E_INVAL, "%zu records in a slice above the %lld offsets given", offs.size(), (long long)rec_cap);
+  for (size_t k = 0; k < offs.size(); k++) rec_off[k] = (int64_t)offs[k];
+  B->st.records += (int64_t)offs.size();
+  B->carry.resize(0);
+  B->carry.resize(un - used);
+  if (un > used) std::memcpy(B->carry.data(), dst + used, un - used);
+  *len = (int64_t)used;
+  *n_rec = (int64_t)offs.size();
   return PL_OK;
 }
 
@@ -1057,6 +1160,7 @@ pl_status pl_bam_ref_reads(const pl_bam* bam, int32_t ref, pl_read_soa* out) {
   if (!bam || !out) return fail(PL_E_INVAL, "null argument");
   if (ref < 0 || ref >= (int32_t)bam->refs.size()) return fail(PL_E_INVAL, "refID %d outside the dictionary", ref);
   if (bam->indexed) return fail(PL_E_INVAL, "an indexed reader reads records per region (pl_bam_query)");
+  if (bam->streamed) return fail(PL_E_INVAL, "a stream hands over record bytes (pl_bam_stream_next)");
   const RefReads& R = bam->refs[(size_t)ref];
   std::memset(out, 0, sizeof *out);
   out->n_reads = R.n;
@@ -1075,6 +1179,78 @@ pl_status pl_bam_ref_reads(const pl_bam* bam, int32_t ref, pl_read_soa* out) {
   out->nib = R.nib.data();
   return PL_OK;
 }
+
+pl_status pl_bam_stream_open(const char* path, int32_t n_threads, pl_bam** out) {
+  if (!path || !out) return fail(PL_E_INVAL, "null argument");
+  *out = nullptr;
+  int nth = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
+  nth = std::max(1, std::min(nth, 64));
+  std::unique_ptr<pl_bam> B(new (std::nothrow) pl_bam());
+  if (!B) return fail(PL_E_OOM, "host allocation failed");
+  std::memset(&B->st, 0, sizeof B->st);
+  B->st.threads = nth;
+  B->nth = nth;
+  B->streamed = true;
+  try {
+    if (pl_status s = map_file(path, B->map)) return s;
+    B->st.compressed_bytes = (int64_t)B->map.n;
+    // the inflated size: every block's ISIZE (the BSIZE chain)
+    for (size_t q = 0; q + 18 <= B->map.n;) {
+      const uint8_t* h = B->map.p + q;
+      const size_t xlen = rd16(h + 10);
+      size_t bsize = 0;
+      for (size_t x = 12; x + 4 <= 12 + xlen && q + x + 6 <= B->map.n;) {
+        const size_t slen = rd16(h + x + 2);
+        if (h[x] == 66 && h[x + 1] == 67 && slen == 2) bsize = (size_t)rd16(h + x + 4) + 1;
+        x += 4 + slen;
+      }
+      if (!bsize || q + bsize > B->map.n) break;      // next_blocks reports a bad block when it reaches it
+      B->stream_bytes += rd32(h + bsize - 4);
+      q += bsize;
+    }
+    // blocks until the header parses; what follows it is carried
+    Buf<uint8_t> ub;
+    for (;;) {
+      std::vector<Block> blocks;
+      size_t end = B->next_block;
+      if (pl_status s = next_blocks(B->map, B->next_block, 1, blocks, &end, &B->last_at)) return s;   // one block
+      if (blocks.empty()) return fail(PL_E_INVAL, "not a BAM file (no complete header)");
+      B->last_end = end;
+      size_t utot = 0;
+      for (const Block& b : blocks) utot += b.isize;
+      const size_t c0 = ub.size();
+      ub.resize(c0 + utot);
+      if (pl_status s = inflate_blocks(B->map.p, blocks, ub.data() + c0, nth)) return s;
+      B->st.blocks += (int64_t)blocks.size();
+      B->st.bam_bytes += (int64_t)utot;
+      B->next_block = end;
+      const int64_t h = parse_header(ub.data(), ub.size(), B.get());
+      if (h < 0) return fail(PL_E_INVAL, "not a BAM file (bad magic or header)");
+      if (h > 0) {
+        B->carry.resize(ub.size() - (size_t)h);
+        std::memcpy(B->carry.data(), ub.data() + h, ub.size() - (size_t)h);
+        break;
+      }
+    }
+  } catch (const std::bad_alloc&) {
+    return fail(PL_E_OOM, "host allocation failed while opening %s", path);
+  }
+  *out = B.release();
+  return PL_OK;
+}
+
+pl_status pl_bam_stream_next(pl_bam* bam, uint8_t* dst, int64_t cap, int64_t* len, int64_t* rec_off, int64_t rec_cap,
+                             int64_t* n_rec) {
+  if (!bam || !dst || !len || !rec_off || !n_rec) return fail(PL_E_INVAL, "null argument");
+  if (!bam->streamed) return fail(PL_E_INVAL, "not a stream (pl_bam_stream_open)");
+  try {
+    return stream_next_impl(bam, dst, cap, len, rec_off, rec_cap, n_rec);
+  } catch (const std::bad_alloc&) {
+    return fail(PL_E_OOM, "host allocation failed");
+  }
+}
+
+int64_t pl_bam_stream_bytes(const pl_bam* bam) { return bam && bam->streamed ? bam->stream_bytes : -1; }
 
 void pl_bam_get_stats(const pl_bam* bam, pl_bam_stats* out) {
   if (bam && out) *out = bam->st;

@@ -450,6 +450,232 @@ pl_status readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* s, int
   return PL_OK;
 }
 
+// Each set's kept-read prefix and segment offsets from the two scans of
+// all sets' flags laid end to end: set i's flags start at B[i], its n + 1
+// outputs at OB[i] = B[i] + i; nseg[i] = its segment count.
+__global__ void __launch_bounds__(256) k_sets_fix(const uint32_t* ks, const uint32_t* ss, const int64_t* B,
+                                                  const int64_t* OB, int32_t n_sets, uint32_t* kp, uint32_t* sego,
+                                                  uint32_t* nseg) {
+  const int64_t total = OB[n_sets];
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+    int32_t lo = 0, hi = n_sets;                 // the last set whose OB <= g
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) / 2;
+      if (OB[mid] <= g) lo = mid; else hi = mid;
+    }
+    const int64_t b = B[lo], j = g - OB[lo];
+    kp[g] = ks[b + j] - ks[b];
+    sego[g] = ss[b + j] - ss[b];
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_sets; i += (int64_t)gridDim.x * blockDim.x)
+    nseg[i] = ss[B[i + 1]] - ss[B[i]];
+}
+
+// The derived tables of read sets whose reads are already on the device
+// (each set's d holds its read arrays): per set what readset_upload makes
+// after its copies (k_prep's read facts and outlier list, the kept-read
+// prefix, the segment table); k_prep and k_segfill per set queued back to
+// back on the lane's stream, the scans of all sets at once, a
+// synchronisation only where a count from the device is needed, the tables
+// in shared arenas.  The sets go into the context's table in order; ids[i]
+// = set i's id.
+pl_status prep_sets(pl_ctx* c, std::vector<std::unique_ptr<ReadSet>>& rs, const std::vector<int64_t>& ncig,
+                    const std::vector<int64_t>& nb, int32_t* ids) {
+  Lane* L = &c->main;
+  hipStream_t st = L->stream;
+  const int32_t n_sets = (int32_t)rs.size();
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  struct Tab { int64_t ocap; size_t meta, ftail, outl; };
+  std::vector<Tab> T((size_t)n_sets);
+  std::vector<int64_t> Bv((size_t)n_sets + 1, 0), OBv((size_t)n_sets + 1, 0);
+  size_t dev = 0;
+  for (int32_t i = 0; i < n_sets; i++) {
+    const int64_t n = rs[(size_t)i]->n;
+    Tab& t = T[(size_t)i];
+    t.ocap = 4096 + n / 256;
+    t.meta = dev; dev = align(dev + sizeof(RMeta) * (size_t)(n + 1));
+    t.ftail = dev; dev = align(dev + sizeof(FragTail) * (size_t)(n + 1));
+    t.outl = dev; dev = align(dev + 8 * (size_t)t.ocap);
+    Bv[(size_t)i + 1] = Bv[(size_t)i] + n;
+    OBv[(size_t)i] = Bv[(size_t)i] + i;
+  }
+  OBv[(size_t)n_sets] = Bv[(size_t)n_sets] + n_sets;
+  const int64_t nall = Bv[(size_t)n_sets], nout = OBv[(size_t)n_sets];
+  const size_t kp_at = dev;
+  dev = align(dev + 4 * (size_t)nout);
+  const size_t scal_at = dev;
+  dev = align(dev + 64 * (size_t)std::max(n_sets, 1));
+  size_t tmp = 0;
+  auto take = [&](size_t bytes) { const size_t a = tmp; tmp = align(tmp + bytes); return a; };
+  const size_t keep_at = take(4 * (size_t)(nall + 1)), segc_at = take(4 * (size_t)(nall + 1)),
+               ks_at = take(4 * (size_t)(nall + 1)), ss_at = take(4 * (size_t)(nall + 1)),
+               sego_at = take(4 * (size_t)nout), b_at = take(8 * Bv.size()), ob_at = take(8 * OBv.size()),
+               nseg_at = take(4 * (size_t)std::max(n_sets, 1));
+  const size_t v_at = take(sizeof(RSDev) * (size_t)std::max(n_sets, 1)), blk_at = take(8 * ((size_t)n_sets + 1)),
+               oc_at = take(8 * (size_t)std::max(n_sets, 1)), ol_at = take(sizeof(int64_t*) * (size_t)std::max(n_sets, 1)),
+               sg_at = take(sizeof(Seg*) * (size_t)std::max(n_sets, 1));
+  auto dfree = [](void* p) { (void)hipFree(p); };
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, dev));
+  std::shared_ptr<void> dev_arena(p, dfree);
+  HIPCHK(hipMalloc(&p, std::max<size_t>(tmp, 256)));
+  std::unique_ptr<void, decltype(dfree)> tmp_arena(p, dfree);
+  uint8_t* D = (uint8_t*)dev_arena.get();
+  uint8_t* X = (uint8_t*)tmp_arena.get();
+  uint32_t* KP = (uint32_t*)(D + kp_at);
+  uint32_t* KEEP = (uint32_t*)(X + keep_at);
+  uint32_t* SEGC = (uint32_t*)(X + segc_at);
+  uint32_t* SEGO = (uint32_t*)(X + sego_at);
+  HIPCHK(hipMemcpyAsync(X + b_at, Bv.data(), 8 * Bv.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(X + ob_at, OBv.data(), 8 * OBv.size(), hipMemcpyHostToDevice, st));
+  // the sets' views, then k_prep over all of them in one launch (set i on
+  // blocks [blk[i], blk[i + 1]), its flags in its part of the flag arrays)
+  std::vector<RSDev> views((size_t)std::max(n_sets, 1));
+  std::vector<int64_t> blk((size_t)n_sets + 1, 0), ocaps((size_t)std::max(n_sets, 1));
+  std::vector<int64_t*> outls((size_t)std::max(n_sets, 1));
+  for (int32_t i = 0; i < n_sets; i++) {
+    Tab& t = T[(size_t)i];
+    ReadSet& R = *rs[(size_t)i];
+    R.shared[1] = dev_arena;
+    RSDev& d = R.d;
+    const Contig& ct = c->contigs.at(R.contig);
+    d.ref = ct.dev;
+    d.ref_len = ct.len;
+    d.meta = (RMeta*)(D + t.meta);
+    d.scal = (int32_t*)(D + scal_at + 64 * (size_t)i);
+    d.ftail = (FragTail*)(D + t.ftail);
+    R.kp = KP + OBv[(size_t)i];
+    views[(size_t)i] = d;
+    blk[(size_t)i + 1] = blk[(size_t)i] + std::min<int64_t>(std::max<int64_t>(1, (R.n + 255) / 256), 8192);
+    ocaps[(size_t)i] = t.ocap;
+    outls[(size_t)i] = (int64_t*)(D + t.outl);
+  }
+  HIPCHK(hipMemsetAsync(D + scal_at, 0, 64 * (size_t)std::max(n_sets, 1), st));
+  if (n_sets > 0) {
+    HIPCHK(hipMemcpyAsync(X + v_at, views.data(), sizeof(RSDev) * views.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(X + blk_at, blk.data(), 8 * blk.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(X + oc_at, ocaps.data(), 8 * ocaps.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(X + ol_at, outls.data(), sizeof(int64_t*) * outls.size(), hipMemcpyHostToDevice, st));
+    k_prep_sets<<<dim3((unsigned)blk[(size_t)n_sets]), dim3(256), 0, st>>>(
+        (const RSDev*)(X + v_at), (const int64_t*)(X + blk_at), n_sets, c->cf, KEEP, SEGC, (const int64_t*)(X + b_at),
+        (int64_t* const*)(X + ol_at), (const int64_t*)(X + oc_at));
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<int32_t> scal(16 * (size_t)std::max(n_sets, 1));
+  HIPCHK(hipMemcpyAsync(scal.data(), D + scal_at, 64 * (size_t)std::max(n_sets, 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // a set whose outliers overflow their list: a list of the size counted, k_prep again
+  std::vector<int64_t*> outl((size_t)n_sets);
+  for (int32_t i = 0; i < n_sets; i++) {
+    Tab& t = T[(size_t)i];
+    ReadSet& R = *rs[(size_t)i];
+    outl[(size_t)i] = (int64_t*)(D + t.outl);
+    const int32_t cnt = scal[16 * (size_t)i + 3];
+    if (cnt > t.ocap) {
+      HIPCHK(hipMalloc(&p, 8 * (size_t)cnt));
+      R.mem.push_back(p);
+      outl[(size_t)i] = (int64_t*)p;
+      HIPCHK(hipMemsetAsync(R.d.scal, 0, 64, st));
+      const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (R.n + 255) / 256), 8192);
+      k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(R.d, c->cf, KEEP + Bv[(size_t)i], SEGC + Bv[(size_t)i],
+                                                           R.d.scal, (int64_t*)p, cnt);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&scal[16 * (size_t)i], R.d.scal, 64, hipMemcpyDeviceToHost, st));
+    }
+  }
+  // the kept-read and segment scans of all sets at once, split per set
+  HIPCHK(L->tiles.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, (nall + SCAN_TILE - 1) / SCAN_TILE)));
+  if (pl_status e = scan_u32(L, KEEP, (uint32_t*)(X + ks_at), nall)) return e;
+  if (pl_status e = scan_u32(L, SEGC, (uint32_t*)(X + ss_at), nall)) return e;
+  {
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((nout + 255) / 256, 1 << 16));
+    k_sets_fix<<<dim3((unsigned)blocks), dim3(256), 0, st>>>((const uint32_t*)(X + ks_at), (const uint32_t*)(X + ss_at),
+                                                             (const int64_t*)(X + b_at), (const int64_t*)(X + ob_at),
+                                                             n_sets, KP, SEGO, (uint32_t*)(X + nseg_at));
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<uint32_t> nseg((size_t)std::max(n_sets, 1), 0);
+  HIPCHK(hipMemcpyAsync(nseg.data(), X + nseg_at, 4 * (size_t)n_sets, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // the segment tables
+  size_t segsz = 0;
+  std::vector<size_t> segat((size_t)std::max(n_sets, 1));
+  for (int32_t i = 0; i < n_sets; i++) { segat[(size_t)i] = segsz; segsz = align(segsz + sizeof(Seg) * ((size_t)nseg[(size_t)i] + 1)); }
+  HIPCHK(hipMalloc(&p, std::max<size_t>(segsz, 256)));
+  std::shared_ptr<void> seg_arena(p, dfree);
+  std::vector<Seg*> segp((size_t)std::max(n_sets, 1));
+  for (int32_t i = 0; i < n_sets; i++) {
+    ReadSet& R = *rs[(size_t)i];
+    R.shared[2] = seg_arena;
+    Seg* sg = (Seg*)((uint8_t*)seg_arena.get() + segat[(size_t)i]);
+    R.d.segs = sg;
+    R.nseg = nseg[(size_t)i];
+    segp[(size_t)i] = sg;
+  }
+  if (n_sets > 0) {                                        // k_segfill over all sets in one launch
+    HIPCHK(hipMemcpyAsync(X + sg_at, segp.data(), sizeof(Seg*) * segp.size(), hipMemcpyHostToDevice, st));
+    k_segfill_sets<<<dim3((unsigned)blk[(size_t)n_sets]), dim3(256), 0, st>>>(
+        (const RSDev*)(X + v_at), (const int64_t*)(X + blk_at), n_sets, c->cf, SEGO, (const int64_t*)(X + ob_at),
+        (Seg* const*)(X + sg_at));
+    HIPCHK(hipGetLastError());
+  }
+  // the outlier lists in order: every list to one pinned buffer, sorted
+  // there by host threads, and back
+  std::vector<int64_t> oat((size_t)n_sets + 1, 0);
+  for (int32_t i = 0; i < n_sets; i++) {
+    ReadSet& R = *rs[(size_t)i];
+    for (int k = 0; k < 8; k++) R.scal[k] = scal[16 * (size_t)i + k];
+    oat[(size_t)i + 1] = oat[(size_t)i] + std::max<int64_t>(R.scal[3], 0);
+  }
+  const int64_t n_out = oat[(size_t)n_sets];
+  if (n_out > 0) {
+    int64_t* ho = nullptr;
+    HIPCHK(hipHostMalloc((void**)&ho, 8 * (size_t)n_out, hipHostMallocDefault));
+    std::unique_ptr<int64_t, hipError_t (*)(void*)> hold(ho, hipHostFree);
+    for (int32_t i = 0; i < n_sets; i++) {
+      const int64_t no = oat[(size_t)i + 1] - oat[(size_t)i];
+      if (no > 0)
+        HIPCHK(hipMemcpyAsync(ho + oat[(size_t)i], outl[(size_t)i], 8 * (size_t)no, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    std::atomic<int32_t> next(0);
+    auto work = [&]() {
+      for (int32_t i; (i = next.fetch_add(1)) < n_sets;)
+        std::sort(ho + oat[(size_t)i], ho + oat[(size_t)i + 1]);
+    };
+    std::vector<std::thread> th;
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    try {
+      for (int t = 1; t < nt && t < n_sets; t++) th.emplace_back(work);
+    } catch (const std::exception&) {
+      // fewer threads: the lists left are sorted by the ones running
+    }
+    work();
+    for (auto& t : th) t.join();
+    for (int32_t i = 0; i < n_sets; i++) {
+      const int64_t no = oat[(size_t)i + 1] - oat[(size_t)i];
+      if (no > 0)
+        HIPCHK(hipMemcpyAsync(outl[(size_t)i], ho + oat[(size_t)i], 8 * (size_t)no, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  // the sets into the context's table
+  for (int32_t i = 0; i < n_sets; i++) {
+    ReadSet& R = *rs[(size_t)i];
+    R.d.outliers = outl[(size_t)i];
+    R.d.n_outlier = R.scal[3];
+    const int64_t n = R.n, nc = ncig[(size_t)i], nbs = nb[(size_t)i];
+    R.bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * nc + 8 * (n + 1) + (nbs + 3) / 4 + nbs + 32 * n + 8 * n + 8 * n;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  for (int32_t i = 0; i < n_sets; i++) {
+    c->rsets.push_back(std::move(rs[(size_t)i]));
+    ids[i] = (int32_t)c->rsets.size() - 1;
+  }
+  c->rs_version++;
+  return PL_OK;
+}
+
 // Many read sets in one pass, each as readset_upload would make it: the
 // host arrays of every set packed into one device arena through pinned
 // staging buffers (host threads fill one while the other is copied), the
@@ -462,15 +688,13 @@ pl_status readsets_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_
   hipStream_t st = L->stream;
   struct Field { const void* src; size_t bytes, pad; size_t at; };
   struct Plan {
-    int64_t n, ncig, nb, ocap;
+    int64_t n, ncig, nb;
     std::vector<Field> f;              // pos tlen flag mapq cigar_off cigar seq_off bases2 quals exo_read exo_off nib [lrt]
-    size_t meta, ftail, kp, outl, keep, segc, sego;
     std::vector<uint8_t> lrt;
   };
   std::vector<Plan> P((size_t)n_sets);
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  size_t up = 0, dev = 0, tmp = 0;
-  int64_t max_n = 1;
+  size_t up = 0;
   for (int32_t i = 0; i < n_sets; i++) {
     const pl_read_soa* s = sets[i];
     if (!s) return fail(PL_E_INVAL, "read set %d is null", i);
@@ -492,7 +716,6 @@ pl_status readsets_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_
     q.n = n;
     q.ncig = s->cigar_off[n];
     q.nb = s->seq_off[n];
-    q.ocap = 4096 + n / 256;
     if (lr) q.lrt.assign((size_t)std::max<int64_t>(n, 1), bt == PL_BAM_NANOPORE ? PL_LR_NANOPORE : PL_LR_PACBIO);
     q.f = {{s->pos, (size_t)(4 * n), 0, 0}, {s->tlen, (size_t)(4 * n), 0, 0}, {s->flag, (size_t)(2 * n), 0, 0},
            {s->mapq, (size_t)n, 0, 0}, {s->cigar_off, (size_t)(8 * (n + 1)), 0, 0}, {s->cigar, (size_t)(4 * q.ncig), 0, 0},
@@ -501,29 +724,13 @@ pl_status readsets_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_
            {s->exo_off, (size_t)(8 * s->n_exotic), 0, 0}, {s->nib, (size_t)(s->n_exotic ? nib_bytes(s) : 0), 64, 0}};
     if (lr) q.f.push_back({q.lrt.data(), (size_t)n, 0, 0});
     for (Field& f : q.f) { f.at = up; up = align(up + f.bytes + f.pad + 16); }
-    q.meta = dev; dev = align(dev + sizeof(RMeta) * (size_t)(n + 1));
-    q.ftail = dev; dev = align(dev + sizeof(FragTail) * (size_t)(n + 1));
-    q.kp = dev; dev = align(dev + 4 * (size_t)(n + 1));
-    q.outl = dev; dev = align(dev + 8 * (size_t)q.ocap);
-    q.keep = tmp; tmp = align(tmp + 4 * (size_t)(n + 1));
-    q.segc = tmp; tmp = align(tmp + 4 * (size_t)(n + 1));
-    q.sego = tmp; tmp = align(tmp + 4 * (size_t)(n + 1));
-    max_n = std::max(max_n, n);
   }
-  const size_t scal_at = dev;
-  dev = align(dev + 64 * (size_t)std::max(n_sets, 1));
   HIPCHK(hipSetDevice(c->device));
   auto dfree = [](void* p) { (void)hipFree(p); };
   void* p = nullptr;
   HIPCHK(hipMalloc(&p, std::max<size_t>(up, 256)));
   std::shared_ptr<void> up_arena(p, dfree);
-  HIPCHK(hipMalloc(&p, dev));
-  std::shared_ptr<void> dev_arena(p, dfree);
-  HIPCHK(hipMalloc(&p, std::max<size_t>(tmp, 256)));
-  std::unique_ptr<void, decltype(dfree)> tmp_arena(p, dfree);
   uint8_t* U = (uint8_t*)up_arena.get();
-  uint8_t* D = (uint8_t*)dev_arena.get();
-  uint8_t* X = (uint8_t*)tmp_arena.get();
   // the host arrays through two pinned staging buffers, packed by host
   // threads into the arena's layout (padding zeroed)
   {
@@ -585,9 +792,9 @@ pl_status readsets_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_
     }
     if (e) return e;
   }
-  // the sets' device views, then k_prep over every set
+  // the sets' device views
   std::vector<std::unique_ptr<ReadSet>> rs((size_t)n_sets);
-  HIPCHK(hipMemsetAsync(D + scal_at, 0, 64 * (size_t)std::max(n_sets, 1), st));
+  std::vector<int64_t> ncig((size_t)n_sets), nb((size_t)n_sets);
   for (int32_t i = 0; i < n_sets; i++) {
     Plan& q = P[(size_t)i];
     const int32_t bt = bam_types ? bam_types[i] : PL_BAM_FRAGS;
@@ -598,7 +805,6 @@ pl_status readsets_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_
     R.has_frags = bt == PL_BAM_FRAGS;
     R.has_nano = bt == PL_BAM_NANOPORE;
     R.shared[0] = up_arena;
-    R.shared[1] = dev_arena;
     RSDev& d = R.d;
     std::memset(&d, 0, sizeof d);
     d.n = q.n;
@@ -610,93 +816,10 @@ pl_status readsets_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_
     d.exo_off = (const int64_t*)(U + q.f[10].at); d.nib = (const uint8_t*)(U + q.f[11].at);
     if (q.f.size() > 12) d.lrt = (const uint8_t*)(U + q.f[12].at);
     d.n_exotic = sets[i]->n_exotic;
-    const Contig& ct = c->contigs.at(contig_ids[i]);
-    d.ref = ct.dev;
-    d.ref_len = ct.len;
-    d.meta = (RMeta*)(D + q.meta);
-    d.scal = (int32_t*)(D + scal_at + 64 * (size_t)i);
-    d.ftail = (FragTail*)(D + q.ftail);
-    R.kp = (uint32_t*)(D + q.kp);
-    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (q.n + 255) / 256), 8192);
-    k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(d, c->cf, (uint32_t*)(X + q.keep), (uint32_t*)(X + q.segc),
-                                                         d.scal, (int64_t*)(D + q.outl), q.ocap);
-    HIPCHK(hipGetLastError());
+    ncig[(size_t)i] = q.ncig;
+    nb[(size_t)i] = q.nb;
   }
-  std::vector<int32_t> scal(16 * (size_t)std::max(n_sets, 1));
-  HIPCHK(hipMemcpyAsync(scal.data(), D + scal_at, 64 * (size_t)std::max(n_sets, 1), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  // a set whose outliers overflow their list: a list of the size counted, k_prep again
-  std::vector<int64_t*> outl((size_t)n_sets);
-  for (int32_t i = 0; i < n_sets; i++) {
-    Plan& q = P[(size_t)i];
-    ReadSet& R = *rs[(size_t)i];
-    outl[(size_t)i] = (int64_t*)(D + q.outl);
-    const int32_t cnt = scal[16 * (size_t)i + 3];
-    if (cnt > q.ocap) {
-      HIPCHK(hipMalloc(&p, 8 * (size_t)cnt));
-      R.mem.push_back(p);
-      outl[(size_t)i] = (int64_t*)p;
-      HIPCHK(hipMemsetAsync(R.d.scal, 0, 64, st));
-      const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (q.n + 255) / 256), 8192);
-      k_prep<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(R.d, c->cf, (uint32_t*)(X + q.keep), (uint32_t*)(X + q.segc),
-                                                           R.d.scal, (int64_t*)p, cnt);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(&scal[16 * (size_t)i], R.d.scal, 64, hipMemcpyDeviceToHost, st));
-    }
-  }
-  // the kept-read and segment scans of every set, and each set's segment count
-  HIPCHK(L->tiles.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, (max_n + SCAN_TILE - 1) / SCAN_TILE)));
-  std::vector<uint32_t> nseg((size_t)std::max(n_sets, 1), 0);
-  for (int32_t i = 0; i < n_sets; i++) {
-    Plan& q = P[(size_t)i];
-    ReadSet& R = *rs[(size_t)i];
-    if (pl_status e = scan_u32(L, (uint32_t*)(X + q.keep), R.kp, q.n)) return e;
-    if (pl_status e = scan_u32(L, (uint32_t*)(X + q.segc), (uint32_t*)(X + q.sego), q.n)) return e;
-    HIPCHK(hipMemcpyAsync(&nseg[(size_t)i], (uint32_t*)(X + q.sego) + q.n, 4, hipMemcpyDeviceToHost, st));
-  }
-  HIPCHK(hipStreamSynchronize(st));
-  // the segment tables
-  size_t segsz = 0;
-  std::vector<size_t> segat((size_t)std::max(n_sets, 1));
-  for (int32_t i = 0; i < n_sets; i++) { segat[(size_t)i] = segsz; segsz = align(segsz + sizeof(Seg) * ((size_t)nseg[(size_t)i] + 1)); }
-  HIPCHK(hipMalloc(&p, std::max<size_t>(segsz, 256)));
-  std::shared_ptr<void> seg_arena(p, dfree);
-  for (int32_t i = 0; i < n_sets; i++) {
-    Plan& q = P[(size_t)i];
-    ReadSet& R = *rs[(size_t)i];
-    R.shared[2] = seg_arena;
-    Seg* sg = (Seg*)((uint8_t*)seg_arena.get() + segat[(size_t)i]);
-    R.d.segs = sg;
-    R.nseg = nseg[(size_t)i];
-    const int64_t blocks = std::min<int64_t>(std::max<int64_t>(1, (q.n + 255) / 256), 8192);
-    k_segfill<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(R.d, c->cf, (uint32_t*)(X + q.sego), sg);
-    HIPCHK(hipGetLastError());
-  }
-  // the outlier lists in order, and the sets into the context's table
-  for (int32_t i = 0; i < n_sets; i++) {
-    ReadSet& R = *rs[(size_t)i];
-    for (int k = 0; k < 8; k++) R.scal[k] = scal[16 * (size_t)i + k];
-    const int64_t no = R.scal[3];
-    if (no > 0) {
-      std::vector<int64_t> h((size_t)no);
-      HIPCHK(hipMemcpyAsync(h.data(), outl[(size_t)i], 8 * (size_t)no, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      std::sort(h.begin(), h.end());
-      HIPCHK(hipMemcpyAsync(outl[(size_t)i], h.data(), 8 * (size_t)no, hipMemcpyHostToDevice, st));
-      HIPCHK(hipStreamSynchronize(st));
-    }
-    R.d.outliers = outl[(size_t)i];
-    R.d.n_outlier = no;
-    const int64_t n = R.n, ncig = P[(size_t)i].ncig, nb = P[(size_t)i].nb;
-    R.bytes = 4 * n + 4 * n + 2 * n + n + 8 * (n + 1) + 4 * ncig + 8 * (n + 1) + (nb + 3) / 4 + nb + 32 * n + 8 * n + 8 * n;
-  }
-  HIPCHK(hipStreamSynchronize(st));
-  for (int32_t i = 0; i < n_sets; i++) {
-    c->rsets.push_back(std::move(rs[(size_t)i]));
-    out_ids[i] = (int32_t)c->rsets.size() - 1;
-  }
-  c->rs_version++;
-  return PL_OK;
+  return prep_sets(c, rs, ncig, nb, out_ids);
 }
 
 // Everything a captured graph bakes in: the kernel arguments (buffer
@@ -1345,6 +1468,13 @@ pl_status pl_readset_upload(pl_ctx* c, int32_t contig_id, const pl_read_soa* rea
   return pl_readset_upload_typed(c, contig_id, reads, PL_BAM_FRAGS, id);
 }
 
+pl_status pl_ctx_synchronize(pl_ctx* c) {
+  if (!c) return fail(PL_E_INVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
+  return PL_OK;
+}
+
 pl_status pl_readset_upload_many(pl_ctx* c, int32_t n_sets, const int32_t* contig_ids, const pl_read_soa* const* reads,
                                  const int32_t* bam_types, int32_t* ids) {
   if (!c || n_sets < 0 || (n_sets > 0 && (!contig_ids || !reads || !ids))) return fail(PL_E_INVAL, "null argument");
@@ -1617,4 +1747,5 @@ pl_status pl_last_timing(pl_ctx* c, double* pipeline_ms, double* window_ms) {
 #include "pl_kmer_kernels.hip"
 #include "pl_kmer.hip"
 #include "pl_asm.hip"
+#include "pl_load.hip"
 #include "pl_fix.hip"

@@ -121,64 +121,106 @@ PL_HD bool wants_segs(uint32_t info) {
   return (info & RI_KEEP) && !(info & (RI_SIMPLE | RI_EXOTIC | RI_BAD | RI_LR_MASK));
 }
 
+// k_prep's work on read r of set R
+__device__ __forceinline__ void prep_one(const RSDev& R, int64_t r, const KConf& cf, uint32_t* keepflag,
+                                         uint32_t* segcount, int32_t* scal, int64_t* outliers, int64_t out_cap) {
+  PrepOut o = prep_read(R, r, cf);
+  RMeta m = o.m;
+  // a read without QUAL piles defaultQual at every base (PileUpRegion.scala:114-115):
+  // written into its quality bytes once, so the window kernel reads quals unconditionally
+  if (m.info & RI_NOQUAL) {
+    int8_t* qd = (int8_t*)R.quals + R.seq_off[r];
+    const int64_t L = R.seq_off[r + 1] - R.seq_off[r];
+    for (int64_t k = 0; k < L; k++) qd[k] = (int8_t)cf.default_qual;
+  }
+  segcount[r] = wants_segs(m.info) ? (uint32_t)build_segs(R, r, m.info, cf, nullptr) : 0u;
+  const bool keep = (m.info & RI_KEEP) != 0;
+  if (keep) {
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    if (m.fp_lo <= m.fp_hi) { lo = m.fp_lo; hi = m.fp_hi; }
+    if (m.fr_lo < m.fr_hi) {
+      lo = lo < m.fr_lo ? lo : m.fr_lo;
+      hi = hi > (int64_t)m.fr_hi - 1 ? hi : (int64_t)m.fr_hi - 1;
+    }
+    if (lo <= hi) {
+      const int64_t lead = m.pos - lo, back = hi - m.pos;
+      if (lead > PL_MAX_LEAD || back > PL_MAX_BACK || R.seq_off[r + 1] - R.seq_off[r] >= PL_MAX_SEQ) {
+        m.info |= RI_OUTLIER;
+        int k = atomicAdd(&scal[3], 1);
+        if (k < out_cap) outliers[k] = r;
+      } else {
+        atomicMax(&scal[0], (int32_t)(back > 0 ? back : 0));
+        atomicMax(&scal[1], (int32_t)(lead > 0 ? lead : 0));
+        if (m.fp_lo <= m.fp_hi && m.fp_hi > m.pos) atomicMax(&scal[6], m.fp_hi - m.pos);
+      }
+    }
+    const int32_t spanEnd = m.aend != 0 ? m.aend : m.pos;
+    atomicMax(&scal[2], spanEnd - m.pos > 0 ? spanEnd - m.pos : 0);
+    if (m.info & RI_UNK_MASK) atomicOr(&scal[4], 1);
+    if (m.info & (RI_BAD | RI_EDGE_CLIP)) atomicOr(&scal[5], 1);
+  }
+  R.meta[r] = m;
+  R.ftail[r] = frag_tail(m);
+  keepflag[r] = keep ? 1u : 0u;
+}
+
 __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount, int32_t* scal, int64_t* outliers,
                        int64_t out_cap) {
   const int64_t n = R.n;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    PrepOut o = prep_read(R, r, cf);
-    RMeta m = o.m;
-    // a read without QUAL piles defaultQual at every base (PileUpRegion.scala:114-115):
-    // written into its quality bytes once, so the window kernel reads quals unconditionally
-    if (m.info & RI_NOQUAL) {
-      int8_t* qd = (int8_t*)R.quals + R.seq_off[r];
-      const int64_t L = R.seq_off[r + 1] - R.seq_off[r];
-      for (int64_t k = 0; k < L; k++) qd[k] = (int8_t)cf.default_qual;
-    }
-    segcount[r] = wants_segs(m.info) ? (uint32_t)build_segs(R, r, m.info, cf, nullptr) : 0u;
-    const bool keep = (m.info & RI_KEEP) != 0;
-    if (keep) {
-      int64_t lo = INT64_MAX, hi = INT64_MIN;
-      if (m.fp_lo <= m.fp_hi) { lo = m.fp_lo; hi = m.fp_hi; }
-      if (m.fr_lo < m.fr_hi) {
-        lo = lo < m.fr_lo ? lo : m.fr_lo;
-        hi = hi > (int64_t)m.fr_hi - 1 ? hi : (int64_t)m.fr_hi - 1;
-      }
-      if (lo <= hi) {
-        const int64_t lead = m.pos - lo, back = hi - m.pos;
-        if (lead > PL_MAX_LEAD || back > PL_MAX_BACK || R.seq_off[r + 1] - R.seq_off[r] >= PL_MAX_SEQ) {
-          m.info |= RI_OUTLIER;
-          int k = atomicAdd(&scal[3], 1);
-          if (k < out_cap) outliers[k] = r;
-        } else {
-          atomicMax(&scal[0], (int32_t)(back > 0 ? back : 0));
-          atomicMax(&scal[1], (int32_t)(lead > 0 ? lead : 0));
-          if (m.fp_lo <= m.fp_hi && m.fp_hi > m.pos) atomicMax(&scal[6], m.fp_hi - m.pos);
-        }
-      }
-      const int32_t spanEnd = m.aend != 0 ? m.aend : m.pos;
-      atomicMax(&scal[2], spanEnd - m.pos > 0 ? spanEnd - m.pos : 0);
-      if (m.info & RI_UNK_MASK) atomicOr(&scal[4], 1);
-      if (m.info & (RI_BAD | RI_EDGE_CLIP)) atomicOr(&scal[5], 1);
-    }
-    R.meta[r] = m;
-    R.ftail[r] = frag_tail(m);
-    keepflag[r] = keep ? 1u : 0u;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    prep_one(R, r, cf, keepflag, segcount, scal, outliers, out_cap);
+}
+
+// the set of block b when set s has blocks [blk[s], blk[s + 1])
+__device__ __forceinline__ int32_t set_of_block(const int64_t* blk, int32_t n_sets, int64_t b) {
+  int32_t lo = 0, hi = n_sets;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) / 2;
+    if (blk[mid] <= b) lo = mid; else hi = mid;
   }
+  return lo;
+}
+
+// k_prep over many sets in one launch: set s's reads on blocks [blk[s],
+// blk[s + 1]), its flags at keep + at[s], its outlier list outl[s] of
+// ocap[s] entries
+__global__ void k_prep_sets(const RSDev* sets, const int64_t* blk, int32_t n_sets, KConf cf, uint32_t* keep,
+                            uint32_t* segc, const int64_t* at, int64_t* const* outl, const int64_t* ocap) {
+  const int32_t s = set_of_block(blk, n_sets, blockIdx.x);
+  const RSDev& R = sets[s];
+  const int64_t nb = blk[s + 1] - blk[s];
+  for (int64_t r = (blockIdx.x - blk[s]) * blockDim.x + threadIdx.x; r < R.n; r += nb * blockDim.x)
+    prep_one(R, r, cf, keep + at[s], segc + at[s], R.scal, outl[s], ocap[s]);
 }
 
 // fills the segment table; RMeta.op0 of a multi-op read becomes its first
 // segment and the info word carries the count (RI_SEG_MAX = too many: walked)
+__device__ __forceinline__ void segfill_one(const RSDev& R, int64_t r, const KConf& cf, const uint32_t* segoff,
+                                            Seg* segs) {
+  RMeta m = R.meta[r];
+  if (!wants_segs(m.info)) return;
+  const uint32_t o = segoff[r], c = segoff[r + 1] - o;
+  if (c) build_segs(R, r, m.info, cf, segs + o);
+  m.op0 = o;
+  m.info = (m.info & ~RI_SEG_MASK) | ((c < RI_SEG_MAX ? c : (uint32_t)RI_SEG_MAX) << RI_SEG_SHIFT);
+  R.meta[r] = m;
+}
+
 __global__ void k_segfill(RSDev R, KConf cf, const uint32_t* segoff, Seg* segs) {
   const int64_t n = R.n;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    RMeta m = R.meta[r];
-    if (!wants_segs(m.info)) continue;
-    const uint32_t o = segoff[r], c = segoff[r + 1] - o;
-    if (c) build_segs(R, r, m.info, cf, segs + o);
-    m.op0 = o;
-    m.info = (m.info & ~RI_SEG_MASK) | ((c < RI_SEG_MAX ? c : (uint32_t)RI_SEG_MAX) << RI_SEG_SHIFT);
-    R.meta[r] = m;
-  }
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    segfill_one(R, r, cf, segoff, segs);
+}
+
+// k_segfill over many sets in one launch: set s's segment offsets at
+// segoff + at[s], its table segs[s]
+__global__ void k_segfill_sets(const RSDev* sets, const int64_t* blk, int32_t n_sets, KConf cf, const uint32_t* segoff,
+                               const int64_t* at, Seg* const* segs) {
+  const int32_t s = set_of_block(blk, n_sets, blockIdx.x);
+  const RSDev& R = sets[s];
+  const int64_t nb = blk[s + 1] - blk[s];
+  for (int64_t r = (blockIdx.x - blk[s]) * blockDim.x + threadIdx.x; r < R.n; r += nb * blockDim.x)
+    segfill_one(R, r, cf, segoff + at[s], segs[s]);
 }
 
 // ------------------------------------------------------------------ scans

@@ -149,6 +149,12 @@ def lib():
             "pl_readset_upload_typed": (C.c_int, [C.c_void_p, C.c_int32, P(PlReadSoa), C.c_int32, P(C.c_int32)]),
             "pl_readset_upload_many": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p]),
+            "pl_ctx_synchronize": (C.c_int, [C.c_void_p]),
+            "pl_loader_open": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, P(C.c_void_p)]),
+            "pl_loader_staging": (C.c_int, [C.c_void_p, P(C.c_void_p), P(C.c_int64), P(C.c_void_p), P(C.c_int64)]),
+            "pl_loader_push": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
+            "pl_loader_finish": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+            "pl_loader_close": (None, [C.c_void_p]),
             "pl_region_open": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_void_p)]),
             "pl_region_add_reads": (C.c_int, [C.c_void_p, P(PlReadSoa)]),
             "pl_region_add_reads_typed": (C.c_int, [C.c_void_p, P(PlReadSoa), C.c_int32]),
@@ -247,6 +253,10 @@ class Context:
         _check(lib().pl_readset_upload_typed(self.h, contig_id, C.byref(soa), BAM_TYPES[bam_type], C.byref(rid)))
         return rid.value
 
+    def synchronize(self):
+        """Wait until the device has finished all work queued on it."""
+        _check(lib().pl_ctx_synchronize(self.h))
+
     def readset_upload_many(self, items):
         """items: (contig_id, reads, bam_type) tuples; every set uploaded in
         one pass (pl_readset_upload_many).  Returns their read-set ids."""
@@ -261,6 +271,43 @@ class Context:
         ids = (C.c_int32 * n)()
         _check(lib().pl_readset_upload_many(self.h, n, cids, arr, types, ids))
         return list(ids)
+
+    def readsets_from_bam(self, path, contig_ids, bam_type="frags", threads=0, slice_bytes=64 << 20):
+        """Every reference of a BAM as a resident read set, its records decoded
+        on the device (pl_loader_*): the host inflates the BGZF blocks
+        (libpilonbam's stream) into the loader's pinned staging buffers.
+        contig_ids: the context contig of each BAM reference (-1: skip).
+        Returns the read-set ids (-1 for a skipped reference)."""
+        from .bam import BamStream
+        contig_ids = list(contig_ids)
+        ids = (C.c_int32 * max(1, len(contig_ids)))()
+        t = [time.perf_counter()]
+        with BamStream(path, threads=threads) as s:
+            if len(contig_ids) != len(s.names):
+                raise ValueError("%d contig ids for %d BAM references" % (len(contig_ids), len(s.names)))
+            ld = C.c_void_p()
+            t.append(time.perf_counter())
+            _check(lib().pl_loader_open(self.h, max(s.total_bytes, 0), slice_bytes, C.byref(ld)))
+            t.append(time.perf_counter())
+            try:
+                buf, offs, cap, rcap = C.c_void_p(), C.c_void_p(), C.c_int64(), C.c_int64()
+                while True:
+                    _check(lib().pl_loader_staging(ld, C.byref(buf), C.byref(cap), C.byref(offs), C.byref(rcap)))
+                    n, r = s.next_into(buf, cap.value, offs, rcap.value)
+                    if n == 0:
+                        break
+                    _check(lib().pl_loader_push(ld, n, r))
+                t.append(time.perf_counter())
+                cids = (C.c_int32 * max(1, len(contig_ids)))(*contig_ids)
+                _check(lib().pl_loader_finish(ld, len(contig_ids), cids, BAM_TYPES[bam_type], ids))
+                t.append(time.perf_counter())
+            finally:
+                lib().pl_loader_close(ld)
+            self.last_bam_stats = s.stats
+        t.append(time.perf_counter())
+        self.last_load_s = dict(zip(["open", "loader_open", "slices", "finish", "close"],
+                                    [b - a for a, b in zip(t, t[1:])]))
+        return list(ids)[:len(contig_ids)]
 
     def batch_descs(self, regions):
         n = len(regions)

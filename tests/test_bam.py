@@ -185,6 +185,39 @@ def test_lookalike_record_chains_in_quals(tmp_path, monkeypatch):
     assert list(many.pos[:3]) == [1, 11, 21] and bytes(many.quals[:len(fake)].astype("uint8")) == fake
 
 
+@pytest.mark.parametrize("cap_kib", [128, 8192])
+def test_stream_hands_over_every_record(tmp_path, monkeypatch, cap_kib):
+    """pl_bam_stream_*: the header as pl_bam_open parses it, then every record
+    in file order, whole, in slices of the caller's size (a record cut at a
+    slice's end comes first in the next one): their refIDs and positions
+    are the decoded batches'."""
+    import struct
+    from pilon_amd.bam import BamStream
+    from pilon_amd.synth import SynthData, preset
+    d = SynthData(preset("c2", scale=0.01))
+    refs = [(d.names[i], len(d.draft(i))) for i in range(d.n)]
+    batches = {i: d.batch(i) for i in range(d.n)}
+    path = tmp_path / "s.bam"
+    S.write_bam(path, refs, batches)
+    with BamStream(path, threads=3) as s:
+        assert s.names == [r[0] for r in refs] and s.lens == [r[1] for r in refs]
+        got = []
+        for buf, offs in s.slices(cap=cap_kib << 10):
+            raw = buf.tobytes()
+            for o in offs:
+                bs, ref, pos = struct.unpack_from("<Iii", raw, int(o))
+                assert int(o) + 4 + bs <= len(raw)
+                got.append((ref, pos + 1))
+        st = s.stats
+        total = s.total_bytes
+    want = [(i, int(p)) for i in range(d.n) for p in batches[i].pos]
+    assert got == want and st["records"] == len(want) and st["eof_marker"] == 1
+    assert st["bam_bytes"] == total
+    with pytest.raises(BamError):
+        with BamStream(path) as s:
+            s.soa(0)
+
+
 def test_native_writer_matches_the_spec_writer(tmp_path):
     """The generator's BAM writer (used for workload-size files) produces the
     same bytes as the spec-literal writer at the same level and block size."""
