@@ -280,12 +280,15 @@ def main():
     alg_read_bytes = 0
     n_reads = 0
     t0 = time.time()
+    soas = []
     for ci in range(data.n):
         ctx.ref_upload(ci, data.names[ci], data.draft(ci))
         soa = data.soa(ci)
-        rids.append(ctx.readset_upload(ci, soa))
+        soas.append(soa)
         alg_read_bytes += read_bytes(soa)
         n_reads += soa.n_reads
+    rids = ctx.readset_upload_many([(ci, soas[ci], "frags") for ci in range(data.n)])
+    del soas
     log(f"bench: uploaded {n_reads} reads in {time.time() - t0:.1f}s")
     regions = regions_for(args.config, data, rids)
     descs = ctx.batch_descs(regions)
@@ -557,10 +560,9 @@ def serve_workload(cfg, args):
     t0 = time.time()
     data = SynthData(preset("c4", scale=args.serve_scale))
     ctx = Context(cfg, device=0)
-    rids = []
     for ci in range(data.n):
         ctx.ref_upload(ci, data.names[ci], data.draft(ci))
-        rids.append(ctx.readset_upload(ci, data.soa(ci)))
+    rids = ctx.readset_upload_many([(ci, data.soa(ci), "frags") for ci in range(data.n)])
     G_res = sum(len(data.draft(ci)) for ci in range(data.n))
     log(f"serve: c4 x{args.serve_scale}: {data.n} contigs, {G_res} bp resident, ready in {time.time() - t0:.1f}s")
     reqs = serve_requests(data, args.serve_requests, 2_000, 1_000_000, seed=1004)
