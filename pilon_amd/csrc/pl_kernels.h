@@ -7,7 +7,7 @@
 #define PL_WIN_BLOCK 256          // threads per window workgroup
 #endif
 #ifndef PL_CHUNK
-#define PL_CHUNK 17               // consecutive bases of one run per lane in the base loop (odd: LDS banks)
+#define PL_CHUNK 13               // consecutive bases of one run per lane in the base loop (odd: LDS banks)
 #endif
 #ifndef PL_WIN_WAVES_EU
 #define PL_WIN_WAVES_EU 3         // minimum waves per SIMD the window kernel is compiled for

@@ -472,7 +472,7 @@ static_assert(PL_CHUNK % 2 == 1 && PL_CHUNK <= 31, "an odd chunk of at most 31 b
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A, int persistent) {
   constexpr int NW = BLOCK / 64;
-  extern __shared__ unsigned long long lds[];
+  extern __shared__ __align__(16) unsigned long long lds[];
   const int W = A.W;
   unsigned long long* bs = lds;
   unsigned long long* mqq = bs + 4 * W;
@@ -532,7 +532,11 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
     const uint8_t* __restrict__ gb = R.bases2;
     const int64_t* __restrict__ gso = R.seq_off;
     const RMeta* __restrict__ gm = R.meta;
-    for (int k = tid; k < lds_words; k += BLOCK) lds[k] = 0ull;
+    {
+      uint4* l4 = (uint4*)lds;   // 16-byte stores: lds is 16-byte aligned
+      for (int k = tid; k < lds_words / 2; k += BLOCK) l4[k] = make_uint4(0u, 0u, 0u, 0u);
+      if ((lds_words & 1) && tid == 0) lds[lds_words - 1] = 0ull;
+    }
     if (tid == 0) { s_ovf = 0; s_nrec = 0; }
     __syncthreads();
 
@@ -676,8 +680,10 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
       // above is what the loop below reads
       __builtin_amdgcn_wave_barrier();
       // ---- the bases of all runs: one chunk of PL_CHUNK consecutive bases of
-      // one run per lane.  A chunk's quals come in as 6 dwords and its 2-bit
-      // bases as 3, realigned with funnel shifts; each base then costs a
+      // one run per lane.  A chunk's quals come in as PL_CHUNK/4 + 2 dwords
+      // and its 2-bit bases as PL_CHUNK/16 + 2, realigned with funnel shifts
+      // (the loads of the next batch of chunks are in flight while the
+      // current batch does its atomics); each base then costs a
       // byte and a 2-bit extract and two LDS atomics whose step offset is the
       // instruction's immediate.  Lanes of one run are PL_CHUNK (odd)
       // positions apart, so their 64-bit atomics spread over all LDS banks.
@@ -688,11 +694,10 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
         const gu32* tq32 = (const gu32*)(gq + qb4);
         const gu32* tb32 = (const gu32*)(gb + (qb16 >> 2));
         const uint32_t qd = (uint32_t)(qb4 - qb16);
-        int cur = 0;
-        for (int32_t cb = 0; cb < CT; cb += 64) {
+        // run of chunk cb + lane = the last run whose first chunk is <= it
+        auto run_of = [&](int32_t cb, int cur) -> int {
           const int32_t C = cb + lane;
           const int32_t lim = CT < cb + 64 ? CT : cb + 64;
-          // run of chunk C = the last run whose first chunk is <= C
           int rr = cur;
           for (int c = cur;;) {
             const int ib = c + 1 + lane;
@@ -702,6 +707,27 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
             if (nb < 64) break;
             c += 64;
           }
+          return rr;
+        };
+        // sequence offset of chunk C of run ru (0 for a lane past the end)
+        auto chunk_q = [&](const RunL& ru, int32_t C) -> uint32_t {
+          return C < CT ? (uint32_t)(ru.q0 + PL_CHUNK * (C - ru.off)) : 0u;
+        };
+        // The loads of a batch are issued before the previous batch's
+        // atomics, so they are in flight while those run.
+        uint32_t qw[PL_CHUNK / 4 + 2], bw[PL_CHUNK / 16 + 2];
+        auto load = [&](uint32_t q) {
+#pragma unroll
+          for (int k = 0; k < PL_CHUNK / 4 + 2; k++) qw[k] = tq32[(q >> 2) + k];
+          const uint32_t qq = q + qd;
+#pragma unroll
+          for (int k = 0; k < PL_CHUNK / 16 + 2; k++) bw[k] = tb32[(qq >> 4) + k];
+        };
+        int cur = 0;
+        int rr = run_of(0, 0);
+        load(chunk_q(wrun[lane < CT ? rr : 0], lane));
+        for (int32_t cb = 0; cb < CT; cb += 64) {
+          const int32_t C = cb + lane;
           const bool act = C < CT;
           const RunL ru = wrun[act ? rr : cur];
           const int32_t d = PL_CHUNK * (C - ru.off);
@@ -710,19 +736,22 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
           const int32_t p = (int32_t)(ru.pi & 0xffffu) + (act ? d : 0);
           const uint32_t q = act ? (uint32_t)(ru.q0 + d) : 0u;
           const int32_t mq1 = (int32_t)((ru.pi >> 16) & 0x1ffu);
-          uint32_t qw[PL_CHUNK / 4 + 2];
-#pragma unroll
-          for (int k = 0; k < PL_CHUNK / 4 + 2; k++) qw[k] = tq32[(q >> 2) + k];
           const uint32_t qq = q + qd;
-          uint32_t bw[PL_CHUNK / 16 + 2];
-#pragma unroll
-          for (int k = 0; k < PL_CHUNK / 16 + 2; k++) bw[k] = tb32[(qq >> 4) + k];
           uint32_t w[PL_CHUNK / 4 + 1];
 #pragma unroll
           for (int k = 0; k < PL_CHUNK / 4 + 1; k++) w[k] = __builtin_amdgcn_alignbyte(qw[k + 1], qw[k], q & 3u);
           uint32_t x[PL_CHUNK / 16 + 1];
 #pragma unroll
           for (int k = 0; k < PL_CHUNK / 16 + 1; k++) x[k] = __builtin_amdgcn_alignbit(bw[k + 1], bw[k], 2u * (qq & 15u));
+          // the next batch: its runs, and its loads
+          int rrn = rr;
+          uint32_t qn = 0u;
+          if (cb + 64 < CT) {
+            const int curn = lane63(rr);
+            rrn = run_of(cb + 64, curn);
+            qn = chunk_q(wrun[C + 64 < CT ? rrn : curn], C + 64);
+          }
+          load(qn);
           unsigned long long* bsp = bs + p;
           unsigned long long* mqp = mqq + p;
           // The base's counter array is b*W entries on: a 24-bit multiply-add
@@ -747,6 +776,7 @@ __global__ void __launch_bounds__(BLOCK, PL_WIN_WAVES_EU) k_window(EngineArgs A,
           if (mq >= 0) pile(std::integral_constant<bool, true>{});
           else pile(std::integral_constant<bool, false>{});
           cur = lane63(rr);
+          rr = rrn;
         }
       }
       // ---- everything else about the chunk's complex reads, one at a time
