@@ -127,8 +127,11 @@ void serve_worker(pl_server* S, Lane* L) {
       d.res.status = s;
       if (s) d.keep_fix = d.keep_pile = false;
     }
-    // keep a lane's buffers near what its batches need
-    if ((int64_t)L->device_bytes() > 2 * est + ((int64_t)64 << 20)) {
+    // a lane keeps its buffers while they fit its share of the budget (no
+    // free and allocate again between batches of varying size); above that
+    // share they are kept only near what its batches need
+    const int64_t held = (int64_t)L->device_bytes();
+    if (held > S->budget / (int64_t)S->lanes.size() && held > 2 * est + ((int64_t)64 << 20)) {
       (void)hipStreamSynchronize(L->stream);
       drop_graph(L);
       L->free_buffers();
