@@ -121,9 +121,35 @@ PL_HD bool wants_segs(uint32_t info) {
   return (info & RI_KEEP) && !(info & (RI_SIMPLE | RI_EXOTIC | RI_BAD | RI_LR_MASK));
 }
 
+// the read set's scalars (scal[0], [1], [2], [6]: maxima, -1 = none;
+// flags: scal[4], scal[5]) gathered over a thread's reads, then over its
+// wave, so one wave gives one atomic per scalar
+struct PrepAcc {
+  int32_t back = -1, lead = -1, span = -1, fp = -1;
+  uint32_t flags = 0;
+};
+__device__ __forceinline__ void prep_flush(PrepAcc a, int32_t* scal) {
+  for (int d = 32; d >= 1; d >>= 1) {
+    a.back = max(a.back, __shfl_xor(a.back, d, 64));
+    a.lead = max(a.lead, __shfl_xor(a.lead, d, 64));
+    a.span = max(a.span, __shfl_xor(a.span, d, 64));
+    a.fp = max(a.fp, __shfl_xor(a.fp, d, 64));
+    a.flags |= (uint32_t)__shfl_xor((int)a.flags, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (a.back >= 0) atomicMax(&scal[0], a.back);
+    if (a.lead >= 0) atomicMax(&scal[1], a.lead);
+    if (a.span >= 0) atomicMax(&scal[2], a.span);
+    if (a.fp >= 0) atomicMax(&scal[6], a.fp);
+    if (a.flags & 1u) atomicOr(&scal[4], 1);
+    if (a.flags & 2u) atomicOr(&scal[5], 1);
+  }
+}
+
 // k_prep's work on read r of set R
 __device__ __forceinline__ void prep_one(const RSDev& R, int64_t r, const KConf& cf, uint32_t* keepflag,
-                                         uint32_t* segcount, int32_t* scal, int64_t* outliers, int64_t out_cap) {
+                                         uint32_t* segcount, int32_t* scal, int64_t* outliers, int64_t out_cap,
+                                         PrepAcc& acc) {
   PrepOut o = prep_read(R, r, cf);
   RMeta m = o.m;
   // a read without QUAL piles defaultQual at every base (PileUpRegion.scala:114-115):
@@ -149,15 +175,15 @@ __device__ __forceinline__ void prep_one(const RSDev& R, int64_t r, const KConf&
         int k = atomicAdd(&scal[3], 1);
         if (k < out_cap) outliers[k] = r;
       } else {
-        atomicMax(&scal[0], (int32_t)(back > 0 ? back : 0));
-        atomicMax(&scal[1], (int32_t)(lead > 0 ? lead : 0));
-        if (m.fp_lo <= m.fp_hi && m.fp_hi > m.pos) atomicMax(&scal[6], m.fp_hi - m.pos);
+        acc.back = max(acc.back, (int32_t)(back > 0 ? back : 0));
+        acc.lead = max(acc.lead, (int32_t)(lead > 0 ? lead : 0));
+        if (m.fp_lo <= m.fp_hi && m.fp_hi > m.pos) acc.fp = max(acc.fp, m.fp_hi - m.pos);
       }
     }
     const int32_t spanEnd = m.aend != 0 ? m.aend : m.pos;
-    atomicMax(&scal[2], spanEnd - m.pos > 0 ? spanEnd - m.pos : 0);
-    if (m.info & RI_UNK_MASK) atomicOr(&scal[4], 1);
-    if (m.info & (RI_BAD | RI_EDGE_CLIP)) atomicOr(&scal[5], 1);
+    acc.span = max(acc.span, spanEnd - m.pos > 0 ? spanEnd - m.pos : 0);
+    if (m.info & RI_UNK_MASK) acc.flags |= 1u;
+    if (m.info & (RI_BAD | RI_EDGE_CLIP)) acc.flags |= 2u;
   }
   R.meta[r] = m;
   R.ftail[r] = frag_tail(m);
@@ -167,8 +193,10 @@ __device__ __forceinline__ void prep_one(const RSDev& R, int64_t r, const KConf&
 __global__ void k_prep(RSDev R, KConf cf, uint32_t* keepflag, uint32_t* segcount, int32_t* scal, int64_t* outliers,
                        int64_t out_cap) {
   const int64_t n = R.n;
+  PrepAcc acc;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
-    prep_one(R, r, cf, keepflag, segcount, scal, outliers, out_cap);
+    prep_one(R, r, cf, keepflag, segcount, scal, outliers, out_cap, acc);
+  prep_flush(acc, scal);
 }
 
 // the set of block b when set s has blocks [blk[s], blk[s + 1])
@@ -189,8 +217,10 @@ __global__ void k_prep_sets(const RSDev* sets, const int64_t* blk, int32_t n_set
   const int32_t s = set_of_block(blk, n_sets, blockIdx.x);
   const RSDev& R = sets[s];
   const int64_t nb = blk[s + 1] - blk[s];
+  PrepAcc acc;
   for (int64_t r = (blockIdx.x - blk[s]) * blockDim.x + threadIdx.x; r < R.n; r += nb * blockDim.x)
-    prep_one(R, r, cf, keep + at[s], segc + at[s], R.scal, outl[s], ocap[s]);
+    prep_one(R, r, cf, keep + at[s], segc + at[s], R.scal, outl[s], ocap[s], acc);
+  prep_flush(acc, R.scal);   // every wave of a block is in set s
 }
 
 // fills the segment table; RMeta.op0 of a multi-op read becomes its first

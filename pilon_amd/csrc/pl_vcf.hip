@@ -22,6 +22,71 @@ void add_line(std::string& s, const std::string& l) { s += l; s += '\n'; }
 
 constexpr size_t kStageChunk = (size_t)32 << 20;
 
+// memcpy of large pieces split over host threads that live for one
+// hand-off, so the copy out of the staging buffers (and the first-touch page
+// faults of a new destination) keep up with the DMA of the next piece
+class CopyPool {
+ public:
+  explicit CopyPool(int n) : nt_(n < 1 ? 1 : n) {
+    for (int i = 1; i < nt_; i++) th_.emplace_back([this, i] { run(i); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      quit_ = true;
+    }
+    cv_.notify_all();
+    for (std::thread& t : th_) t.join();
+  }
+  void copy(char* dst, const char* src, size_t n) {
+    if (nt_ == 1 || n < ((size_t)1 << 20)) { std::memcpy(dst, src, n); return; }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      dst_ = dst; src_ = src; n_ = n; left_ = nt_ - 1; gen_++;
+    }
+    cv_.notify_all();
+    slice(0);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return left_ == 0; });
+  }
+
+ private:
+  void slice(int i) {
+    const size_t per = ((n_ + (size_t)nt_ - 1) / (size_t)nt_ + 4095) & ~(size_t)4095;
+    const size_t a = std::min(n_, (size_t)i * per), b = std::min(n_, a + per);
+    if (b > a) std::memcpy(dst_ + a, src_ + a, b - a);
+  }
+  void run(int i) {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return quit_ || gen_ != seen; });
+        if (quit_) return;
+        seen = gen_;
+      }
+      slice(i);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--left_ == 0) done_.notify_one();
+    }
+  }
+  int nt_;
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  char* dst_ = nullptr;
+  const char* src_ = nullptr;
+  size_t n_ = 0;
+  int left_ = 0;
+  uint64_t gen_ = 0;
+  bool quit_ = false;
+};
+
+// host threads for a copy of n bytes out of the staging buffers
+int copy_threads(size_t n) {
+  return n < ((size_t)64 << 20) ? 1 : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
 // The lane's two pinned staging buffers and their events, created on first use.
 pl_status stage_ready(Lane* L) {
   for (int k = 0; k < 2; k++) {
@@ -75,8 +140,9 @@ pl_status copy_to_host(Lane* L, void* dst, const void* src, size_t bytes) {
     HIPCHK(hipStreamSynchronize(L->stream));
     return PL_OK;
   }
-  return drain_to_host(L, src, bytes, [dst](size_t off, const char* p, size_t n) {
-    std::memcpy((char*)dst + off, p, n);
+  CopyPool pool(copy_threads(bytes));
+  return drain_to_host(L, src, bytes, [dst, &pool](size_t off, const char* p, size_t n) {
+    pool.copy((char*)dst + off, p, n);
     return true;
   });
 }
@@ -359,8 +425,9 @@ pl_status vcf_of_lane(pl_ctx* c, Lane* L, const std::vector<std::string>& rnames
   char* out = (char*)malloc((size_t)n + 1);
   if (!out) return fail(PL_E_OOM, "host allocation of %llu bytes failed", (unsigned long long)n);
   uint64_t w = 0;
+  CopyPool pool(copy_threads((size_t)n));
   pl_status s = vcf_emit(L, V, [&](const char* p, size_t k) {
-    std::memcpy(out + w, p, k);
+    pool.copy(out + w, p, k);
     w += k;
     return true;
   });
