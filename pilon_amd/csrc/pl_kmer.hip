@@ -71,7 +71,7 @@ pl_status kmer_graph(pl_ctx* c, const pl_read_soa* R, int32_t K, int32_t min_dep
   A.occ_off = docc;
   const uint32_t nocc = (uint32_t)N;
   uint32_t *perm, *keys, *tk, *tv, *ncount, *head, *seg;
-  if ((s = B.get(&A.kw, 4 * (size_t)A.words * (size_t)N)) || (s = B.get(&A.val, 4 * (size_t)N)) ||
+  if ((s = B.get(&A.kw, 4 * (size_t)PL_KMER_STRIDE * (size_t)N)) || (s = B.get(&A.val, 4 * (size_t)N)) ||
       (s = B.get(&A.hw, 8 * (size_t)N)) || (s = B.get(&A.collide, 4)) ||
       (s = B.get(&perm, 4 * (size_t)N)) || (s = B.get(&keys, 4 * (size_t)N)) || (s = B.get(&tk, 4 * (size_t)N)) ||
       (s = B.get(&tv, 4 * (size_t)N)) || (s = B.get(&ncount, 4)) || (s = B.get(&head, 4 * (size_t)N)) ||
@@ -85,10 +85,10 @@ pl_status kmer_graph(pl_ctx* c, const pl_read_soa* R, int32_t K, int32_t min_dep
   HIPCHK(hipGetLastError());
   // sort the occurrences (perm) by the 32-bit keys src[0..nw) of each, the
   // least significant first: each stable sort keeps the order of the later keys
-  auto sort_by = [&](const uint32_t* const* src, int nw) -> pl_status {
+  auto sort_by = [&](const uint32_t* const* src, uint32_t stride, int nw) -> pl_status {
     k_iota<<<dim3(std::min<unsigned>(gb, 4096)), dim3(256), 0, st>>>(perm, (int32_t)N, ncount);
     for (int w = 0; w < nw; w++) {
-      k_kmer_gather<<<dim3(gb), dim3(256), 0, st>>>(src[w], perm, nocc, keys);
+      k_kmer_gather<<<dim3(gb), dim3(256), 0, st>>>(src[w], stride, perm, nocc, keys);
       uint32_t *rk, *rv;
       if (pl_status e = radix_sort(L, keys, perm, tk, tv, ncount, nocc, 32, &rk, &rv)) return e;
       if (rv != perm) HIPCHK(hipMemcpyAsync(perm, rv, 4 * (size_t)N, hipMemcpyDeviceToDevice, st));
@@ -98,18 +98,20 @@ pl_status kmer_graph(pl_ctx* c, const pl_read_soa* R, int32_t K, int32_t min_dep
   A.perm = perm;
   A.head = head;
   const uint32_t* by_hash[2] = {A.hw, A.hw + (size_t)N};
-  if ((s = sort_by(by_hash, 2))) return s;
+  if ((s = sort_by(by_hash, 1, 2))) return s;
   A.by_hash = 1;
   HIPCHK(hipMemsetAsync(A.collide, 0, 4, st));
   k_kmer_heads<<<dim3(gb), dim3(256), 0, st>>>(A);
   uint32_t collide = 0;
   HIPCHK(hipMemcpyAsync(&collide, A.collide, 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (collide) {
+  // PILON_HIP_KMER_WORD_SORT in the environment takes this path without a
+  // collision (tests compare it with the hash order's graph)
+  if (collide || std::getenv("PILON_HIP_KMER_WORD_SORT")) {
     // two K-mers share a hash: order by the words themselves instead
     const uint32_t* by_words[PL_KMER_WORDS];
-    for (int w = 0; w < A.words; w++) by_words[w] = A.kw + (size_t)(A.words - 1 - w) * N;
-    if ((s = sort_by(by_words, A.words))) return s;
+    for (int w = 0; w < A.words; w++) by_words[w] = A.kw + (size_t)(A.words - 1 - w);
+    if ((s = sort_by(by_words, PL_KMER_STRIDE, A.words))) return s;
     A.by_hash = 0;
     k_kmer_heads<<<dim3(gb), dim3(256), 0, st>>>(A);
   }

@@ -19,6 +19,7 @@
 //                 not homozygous, altGraph's.
 
 #define PL_KMER_WORDS 6          // K <= 48: eight 4-bit codes per word
+#define PL_KMER_STRIDE 8         // words per occurrence in kw: one 32-byte record (unused words zero)
 
 struct KmerArgs {
   int64_t n_reads;
@@ -33,7 +34,7 @@ struct KmerArgs {
   uint32_t n_occ;
   int32_t K, words;
   int32_t default_qual, min_qual, min_depth;
-  uint32_t* kw;                  // [PL_KMER_WORDS][n_occ]
+  uint32_t* kw;                  // [n_occ][PL_KMER_STRIDE] each occurrence's words together
   uint32_t* hw;                  // [2][n_occ] the K-mer's hash, low word then high word
   uint32_t* val;                 // [n_occ] code | qual << 8 | mapq << 16
   uint32_t* collide;             // set when sorted neighbours share a hash but not their words
@@ -88,9 +89,13 @@ __device__ __forceinline__ int32_t kmer_qual(const KmerArgs& A, int64_t r, int64
 // the occurrence's outputs: its K-mer words, their hash, the next base
 __device__ __forceinline__ void kmer_store(const KmerArgs& A, int64_t t, const uint32_t* w, uint32_t next,
                                            int32_t qual, uint32_t mapq) {
+  uint4* rec = (uint4*)(A.kw + (int64_t)t * PL_KMER_STRIDE);
+  static_assert(PL_KMER_WORDS <= PL_KMER_STRIDE && PL_KMER_STRIDE == 8, "two 16-byte halves per occurrence");
+  uint32_t v[PL_KMER_STRIDE];
 #pragma unroll
-  for (int k = 0; k < PL_KMER_WORDS; k++)
-    if (k < A.words) A.kw[(int64_t)k * A.n_occ + t] = w[k];
+  for (int k = 0; k < PL_KMER_STRIDE; k++) v[k] = (k < PL_KMER_WORDS && k < A.words) ? w[k] : 0u;
+  rec[0] = make_uint4(v[0], v[1], v[2], v[3]);
+  rec[1] = make_uint4(v[4], v[5], v[6], v[7]);
   const uint64_t h = kmer_hash(w, A.words);
   A.hw[t] = (uint32_t)h;
   A.hw[(int64_t)A.n_occ + t] = (uint32_t)(h >> 32);
@@ -170,23 +175,44 @@ __global__ void __launch_bounds__(256) k_kmer_emit(KmerArgs A) {
   }
 }
 
-// keys[i] = word k of the i-th occurrence in the current order
-__global__ void __launch_bounds__(256) k_kmer_gather(const uint32_t* kw, const uint32_t* perm, uint32_t n, uint32_t* keys) {
+// keys[i] = the key word of the i-th occurrence in the current order: src
+// holds occurrence o's word at src[o * stride]
+__global__ void __launch_bounds__(256) k_kmer_gather(const uint32_t* src, uint32_t stride, const uint32_t* perm,
+                                                     uint32_t n, uint32_t* keys) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) keys[i] = kw[perm[i]];
+  if (i < n) keys[i] = src[(int64_t)perm[i] * stride];
 }
 
 __global__ void __launch_bounds__(256) k_kmer_heads(KmerArgs A) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n_occ) return;
-  uint32_t h = 1;
-  if (i > 0) {
-    const uint32_t a = A.perm[i], b = A.perm[i - 1];
-    h = 0;
-    for (int k = 0; k < A.words; k++) h |= A.kw[(int64_t)k * A.n_occ + a] != A.kw[(int64_t)k * A.n_occ + b];
+  const int lane = threadIdx.x & 63;
+  const bool in = i < A.n_occ;
+  // each occurrence's words are gathered once: the previous sorted
+  // occurrence's come from the lane before (lane 0 gathers them itself)
+  const bool first = lane == 0 && in && i > 0;
+  const uint32_t a = in ? A.perm[i] : 0u;
+  const uint32_t b = first ? A.perm[i - 1] : 0u;
+  const uint4* ra = (const uint4*)(A.kw + (int64_t)a * PL_KMER_STRIDE);
+  const uint4* rb = (const uint4*)(A.kw + (int64_t)b * PL_KMER_STRIDE);
+  uint4 x0 = make_uint4(0u, 0u, 0u, 0u), x1 = x0, y0 = x0, y1 = x0;
+  if (in) { x0 = ra[0]; x1 = ra[1]; }
+  if (first) { y0 = rb[0]; y1 = rb[1]; }
+  const uint32_t xs[PL_KMER_STRIDE] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+  const uint32_t ys[PL_KMER_STRIDE] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+  uint32_t h = 0;
+#pragma unroll
+  for (int k = 0; k < PL_KMER_WORDS; k++) {
+    uint32_t y = (uint32_t)__shfl_up((int)xs[k], 1, 64);
+    if (first) y = ys[k];
+    h |= (k < A.words) && xs[k] != y;
+  }
+  if (!in) return;
+  if (i == 0) {
+    h = 1;
+  } else if (A.by_hash && h) {
     // sorted by hash: equal hashes with different words may interleave
-    if (A.by_hash && h && A.hw[a] == A.hw[b] && A.hw[(int64_t)A.n_occ + a] == A.hw[(int64_t)A.n_occ + b])
-      atomicOr(A.collide, 1u);
+    const uint32_t bp = A.perm[i - 1];
+    if (A.hw[a] == A.hw[bp] && A.hw[(int64_t)A.n_occ + a] == A.hw[(int64_t)A.n_occ + bp]) atomicOr(A.collide, 1u);
   }
   A.head[i] = h;
 }
@@ -244,7 +270,7 @@ __global__ void __launch_bounds__(256) k_kmer_call(KmerArgs A) {
   if (!keep) return;
   const BaseFacts f = base_facts(c);
   const uint32_t p = A.perm[i0];
-  for (int w = 0; w < A.words; w++) A.out_kw[(int64_t)w * A.n_seg + k] = A.kw[(int64_t)w * A.n_occ + p];
+  for (int w = 0; w < A.words; w++) A.out_kw[(int64_t)w * A.n_seg + k] = A.kw[(int64_t)p * PL_KMER_STRIDE + w];
   A.out_next[k] = (uint8_t)"ACGTN"[f.base];
   A.out_alt[k] = f.homo ? (uint8_t)0 : (uint8_t)"ACGT"[f.alt];
 }

@@ -69,6 +69,26 @@ def test_kmer_synthetic_reads(engine, n_reads, min_depth):
     assert ms > 0
 
 
+@pytest.mark.parametrize("K", [31, 47])
+def test_kmer_word_order_path(engine, K, monkeypatch):
+    """The order by the K-mer words themselves (taken when two K-mers share
+    a hash; PILON_HIP_KMER_WORD_SORT takes it without one) gives the same
+    graph as the order by hash, and the oracle's."""
+    from oracle import oracle as O
+    from pilon_amd.synth import SynthData, preset
+    print(f"case: kmer word order K={K}", flush=True)
+    p = preset("c2", scale=0.04)
+    p.update({"n_rate": 0.002})
+    b = SynthData(p).batch(0).slice(0, 20000)
+    by_hash, _ = engine.kmer_graph(b, K=K, min_depth=3)
+    monkeypatch.setenv("PILON_HIP_KMER_WORD_SORT", "1")
+    by_words, _ = engine.kmer_graph(b, K=K, min_depth=3)
+    monkeypatch.delenv("PILON_HIP_KMER_WORD_SORT")
+    st, ref = O.kmer_graph(engine.config, b, K=K, min_depth=3)
+    assert st == 0 and len(ref) > 1000
+    assert by_words == by_hash == ref
+
+
 def test_kmer_prune_point_refused(engine):
     from pilon_amd.engine import PilonError
     from pilon_amd.synth import SynthData, preset
