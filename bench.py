@@ -231,6 +231,9 @@ def main():
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup; 0 = autotuned")
     ap.add_argument("--graph", type=int, default=1, help="time the steps as hipGraph replays")
+    ap.add_argument("--repeat", type=int, default=1,
+                    help="with --graph, queue the timed replays back to back (pl_batch_repeat) instead of one "
+                         "library call per step")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-region comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU oracle sample")
     ap.add_argument("--serve", type=int, default=1, help="also run the serving workload (c4 request stream)")
@@ -308,16 +311,23 @@ def main():
         ctx.batch_run(regions, window=W, persistent=args.persistent, copy_out=False, want_stats=False,
                       descs=descs, block=B, use_graph=bool(graph))
 
-    for _ in range(args.warmup):
+    for _ in range(max(args.warmup, 1 if args.graph else 0)):
         step(args.graph)
     pipe = []
-    # each step returns after the library's stream has drained; the device
-    # synchronisations around the loop make the bracket explicit
+    # with graphs the K steps are K replays of the captured pipeline queued
+    # back to back (pl_batch_repeat: every pass's counters are checked after
+    # they drain); otherwise each step returns after the library's stream has
+    # drained.  The device synchronisations around the loop make the bracket
+    # explicit.
     ctx.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(args.graph)
+    if args.graph and args.repeat:
+        ctx.batch_repeat(args.steps)
         pipe.append(ctx.batch_info()["pipeline_ms"])
+    else:
+        for _ in range(args.steps):
+            step(args.graph)
+            pipe.append(ctx.batch_info()["pipeline_ms"])
     ctx.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
@@ -333,7 +343,8 @@ def main():
     b_alg = alg_read_bytes + G * 80.25 + 32 * info["n_records"]
     achieved = b_alg / (win_ms / 1000.0) / 1e9
     traffic, traffic_src = measured_traffic(args.traffic_profile, args.config, W, B) if args.scale == 1.0 else (None, None)
-    log(f"bench: {ms_per_step:.3f} ms/step wall ({'graph' if args.graph else 'eager'}), device pipeline {pipe_ms:.3f} ms, "
+    mode = ("graph replays queued back to back" if args.repeat else "graph") if args.graph else "eager"
+    log(f"bench: {ms_per_step:.3f} ms/step wall ({mode}), device pipeline {pipe_ms:.3f} ms, "
         f"window kernel {win_ms:.3f} ms, {value / 1e9:.2f} G bases/s, window kernel {achieved:.0f} GB/s algorithmic")
 
     extra = {}
@@ -459,7 +470,7 @@ def main():
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "scale": args.scale,
                    "regions": len(regions), "positions": G, "reads": n_reads, "aligned_bases": aligned,
                    "window": W, "block": B, "autotuned": not (args.window and args.block),
-                   "persistent": args.persistent, "graph": bool(args.graph)},
+                   "persistent": args.persistent, "graph": bool(args.graph), "repeat": bool(args.graph and args.repeat)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_window", "alg_bytes_per_launch": b_alg, "kernel_ms": win_ms,

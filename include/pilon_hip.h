@@ -284,6 +284,17 @@ pl_status pl_batch_run(pl_ctx* ctx, const pl_region_desc* regions, int32_t n,
                        const pl_batch_opts* opts, int32_t* pileup,
                        pl_call* calls, pl_region_stats* stats);
 
+/* n_passes more passes of the context's last batch, which pl_batch_run ran
+ * with use_graph: its captured pipeline replayed back to back with no host
+ * round trip between passes, each pass's evidence counters copied to pinned
+ * memory and checked once the passes (at most 64 at a time) have drained
+ * (PL_E_OOM if one overflowed: run the batch with pl_batch_run again).  The
+ * results of the last pass stay resident, as after pl_batch_run;
+ * pl_last_timing's pipeline time is the mean per pass.  No counterpart in
+ * the reference: for callers that run the same resident batch repeatedly
+ * (benchmarks, re-polishing loops) without a host wait per pass. */
+pl_status pl_batch_repeat(pl_ctx* ctx, int32_t n_passes);
+
 /* pl_batch_run without allocating: no hipMalloc, hipFree or other
  * allocating routine runs in this call.  It has no counterpart in the
  * reference; it is for callers that cannot allocate device memory (code

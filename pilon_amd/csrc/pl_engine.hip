@@ -178,6 +178,8 @@ struct Lane {
   // destroyed, so a hand-off allocates no pinned memory
   void* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  // pinned slots for the counters of repeated passes (pl_batch_repeat)
+  uint32_t* rep_hc = nullptr;
   // depth of the batch's "jumps" reads before Pass 1, per position, which
   // fragment coverage leaves out (valid for the last batch when set)
   DBuf jdepth;
@@ -856,6 +858,8 @@ void lane_destroy(Lane* L) {
     L->stage_ev[k] = nullptr;
     L->stage[k] = nullptr;
   }
+  if (L->rep_hc) (void)hipHostFree(L->rep_hc);
+  L->rep_hc = nullptr;
   for (int i = 0; i < 4; i++) if (L->ev[i]) (void)hipEventDestroy(L->ev[i]);
   if (L->stream) (void)hipStreamDestroy(L->stream);
   L->stream = nullptr;
@@ -1692,6 +1696,49 @@ pl_status pl_batch_run_noalloc(pl_ctx* c, const pl_region_desc* regions, int32_t
   if (s && c->na_arena.exhausted)
     return fail(PL_E_OOM, "the batch's device buffers exceed the %zu-byte scratch array", PL_NOALLOC_SCRATCH_BYTES);
   return s;
+}
+
+pl_status pl_batch_repeat(pl_ctx* c, int32_t n_passes) {
+  if (!c || n_passes < 0) return fail(PL_E_INVAL, "null argument or negative pass count");
+  if (c->serving) return fail(PL_E_BUSY, "a server is open on this context");
+  Lane* L = &c->main;
+  if (!L->tables_valid || !L->last_graph || !L->gexec)
+    return fail(PL_E_INVAL, "the last batch was not run with use_graph (nothing to repeat)");
+  HIPCHK(hipSetDevice(c->device));
+  constexpr int kSlots = 64;   // passes in flight between two checks
+  if (!L->rep_hc) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sizeof(uint32_t) * 8 * kSlots, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PL_E_OOM, "pinned host allocation failed");
+    }
+    L->rep_hc = (uint32_t*)p;
+  }
+  hipStream_t st = L->stream;
+  const uint32_t* cnt = L->counters.as<uint32_t>();
+  for (int32_t done = 0; done < n_passes;) {
+    const int32_t m = std::min<int32_t>(kSlots, n_passes - done);
+    HIPCHK(hipEventRecord(L->ev[0], st));
+    for (int32_t i = 0; i < m; i++) {
+      HIPCHK(hipGraphLaunch(L->gexec, st));
+      HIPCHK(hipMemcpyAsync(L->rep_hc + 8 * i, cnt, 32, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipEventRecord(L->ev[1], st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int32_t i = 0; i < m; i++) {
+      const uint32_t* hc = L->rep_hc + 8 * i;
+      if (hc[0] > L->rec_cap || hc[1] > L->cand_cap || hc[2] > L->defer_cap || hc[3] > L->mut_cap ||
+          hc[6] > L->wide_cap || hc[4])
+        return fail(PL_E_OOM, "pass %d of the repeated batch overflowed its buffers: run it with pl_batch_run",
+                    done + i);
+    }
+    std::memcpy(L->last_counts, L->rep_hc + 8 * (m - 1), sizeof L->last_counts);
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, L->ev[0], L->ev[1]));
+    L->last_ms = ms / (float)m;
+    done += m;
+  }
+  return PL_OK;
 }
 
 pl_status pl_last_batch_info(pl_ctx* c, pl_batch_info* o) {

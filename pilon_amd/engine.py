@@ -166,6 +166,7 @@ def lib():
             "pl_free": (None, [C.c_void_p]),
             "pl_batch_run": (C.c_int, [C.c_void_p, P(PlRegionDesc), C.c_int32, P(PlBatchOpts), C.c_void_p,
                                        C.c_void_p, C.c_void_p]),
+            "pl_batch_repeat": (C.c_int, [C.c_void_p, C.c_int32]),
             "pl_last_timing": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double)]),
             "pl_last_batch_info": (C.c_int, [C.c_void_p, P(PlBatchInfo)]),
             "pl_serve_open": (C.c_int, [C.c_void_p, P(PlServeOpts), P(C.c_void_p)]),
@@ -256,6 +257,12 @@ class Context:
     def synchronize(self):
         """Wait until the device has finished all work queued on it."""
         _check(lib().pl_ctx_synchronize(self.h))
+
+    def batch_repeat(self, n_passes):
+        """n_passes more passes of the last batch (run with use_graph=True),
+        replayed back to back with one host wait at the end
+        (pl_batch_repeat)."""
+        _check(lib().pl_batch_repeat(self.h, int(n_passes)))
 
     def readset_upload_many(self, items):
         """items: (contig_id, reads, bam_type) tuples; every set uploaded in

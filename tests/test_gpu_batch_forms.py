@@ -66,6 +66,36 @@ def test_graph_replays_match_eager(many):
         _same(got2, other_eager, f"second graph replay {i}")
 
 
+def test_repeated_passes_match(many):
+    """pl_batch_repeat: 70 back-to-back replays of the captured batch (two
+    rounds of queued passes) leave the same resident results as the eager
+    run (its VCF, tracks and record counts); after an eager batch there is
+    nothing to repeat."""
+    from pilon_amd.engine import PilonError
+    d, ctx, regions = many
+    print("case: repeated graph passes", flush=True)
+    eager = ctx.batch_run(regions, window=512, block=256, persistent=1)
+    _same(ctx.batch_run(regions, window=512, block=256, persistent=1, use_graph=True), eager, "graph run")
+    info0 = ctx.batch_info()
+    vcf0, _ = ctx.batch_vcf()
+    trk0, _ = ctx.batch_tracks()
+    ctx.batch_repeat(70)
+    info1 = ctx.batch_info()
+    assert info1["pipeline_ms"] > 0
+    assert (info1["n_records"], info1["n_candidates"], info1["n_deferred"]) == \
+        (info0["n_records"], info0["n_candidates"], info0["n_deferred"])
+    vcf1, _ = ctx.batch_vcf()
+    trk1, _ = ctx.batch_tracks()
+    assert vcf1 == vcf0
+    assert np.array_equal(trk1, trk0)
+    ctx.batch_repeat(0)
+    other = regions[: max(1, len(regions) // 3)]
+    ctx.batch_run(other, window=256, block=256, persistent=0)
+    with pytest.raises(PilonError) as e:
+        ctx.batch_repeat(1)
+    assert e.value.status == 1
+
+
 def test_autotuned_matches_fixed(many):
     d, ctx, regions = many
     print("case: autotuned window/block", flush=True)
