@@ -179,4 +179,5 @@ struct EngineArgs {
   int32_t* overflow;                // capacity overflow flag
   struct WideCol* wide; uint32_t* wide_n; uint32_t wide_cap;   // exact sums of wide columns
   int32_t* win_wide;                // [2 * n_wins]: each window's first wide entry and count
+  uint32_t* deep;                   // set when some region's minDepth is above the confirmed histogram
 };

@@ -1059,6 +1059,7 @@ pl_status run_jobs(pl_ctx* c, Lane* L, const std::vector<JobReq>& reqs, int32_t 
     A.n_wins = n_wins;
     A.W = W;
     A.queue = (int32_t*)(cnt + 5);
+    A.deep = cnt + 7;
     A.worder = L->worder.as<int32_t>();
     A.heads = L->heads.as<WinHead>();
     A.pile = L->pile.as<int32_t>();

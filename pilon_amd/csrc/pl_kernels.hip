@@ -1311,6 +1311,7 @@ __global__ void __launch_bounds__(64) k_finalize(EngineArgs A, const Cand* sorte
     st.min_depth = md;
     int64_t nconf = 0;   // minDepth above the histogram: counted by k_confirm_exact
     if (md <= PL_CONF_HIST_MAX) for (int d = md; d < 256; d++) nconf += st.hist[d];
+    else if (st.read_count != 0) atomicOr(A.deep, 1u);
     if (nconf > 0) nconf = nconf - dropped > 0 ? nconf - dropped : 0;   // the serial scan stops decrementing at 0
     st.n_confirmed = (int32_t)nconf;
   }
@@ -1323,6 +1324,7 @@ __global__ void __launch_bounds__(64) k_finalize(EngineArgs A, const Cand* sorte
 // launch costs a few microseconds when no region needs it.
 __global__ void __launch_bounds__(256) k_confirm_exact(EngineArgs A) {
   __shared__ uint32_t s_n;
+  if (*(volatile uint32_t*)A.deep == 0) return;   // no region needs the exact count
   for (int32_t w = blockIdx.x; w < A.n_wins; w += gridDim.x) {
     const WinDev win = A.wins[w];
     JobStat& S = A.st[win.job];
