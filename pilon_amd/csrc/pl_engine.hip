@@ -883,7 +883,7 @@ pl_status enqueue_batch(Lane* L, EngineArgs& A, int persistent, int block, int g
     k_iota<<<dim3((unsigned)std::min<int64_t>(1024, (n_wins + 255) / 256)), dim3(256), 0, st>>>(L->costv.as<uint32_t>(), n_wins, cnt + 8);
     uint32_t *sk, *sv;
     pl_status s = radix_sort(L, L->costk.as<uint32_t>(), L->costv.as<uint32_t>(), L->costtk.as<uint32_t>(),
-                             L->costtv.as<uint32_t>(), cnt + 8, (uint32_t)n_wins, 32, &sk, &sv);
+                             L->costtv.as<uint32_t>(), cnt + 8, (uint32_t)n_wins, 16, &sk, &sv);
     if (s) return s;
     A.worder = (const int32_t*)sv;
   }

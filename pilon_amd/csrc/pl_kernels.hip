@@ -327,9 +327,11 @@ __global__ void k_plan(const RSDev* rsets, const JobDev* jobs, WinDev* wins, int
   win.lo = lower_pos_dev(R.pos, 0, R.n, L0 - back);
   win.hi = lower_pos_dev(R.pos, win.lo, R.n, L1 + lead);
   win.plo = fragment_only_end(R.pos, win.lo, win.hi, J, L0, back, R.scal[6]);
-  // descending-cost order key for the work queue (largest windows first)
+  // descending-cost order key for the work queue (largest windows first):
+  // 16 bits, so the sort takes two passes; windows beyond 65,535 cost units
+  // share the front of the queue
   const int64_t c = (win.hi - win.plo) + (win.plo - win.lo) / 8 + (int64_t)(win.p1 - win.p0) / 16;
-  cost[w] = 0xffffffffu - (uint32_t)(c > 0xfffffffll ? 0xfffffffll : c);
+  cost[w] = 0xffffu - (uint32_t)(c > 0xffffll ? 0xffffll : c);
 }
 
 // the window heads in processing order (worder null: window order)
